@@ -1,0 +1,259 @@
+/*
+ * patchtable_amd.h -- C ABI of the MI355X (gfx950) PatchTable library.
+ *
+ * This is the drop-in boundary for the reference's construct-then-lookup API:
+ *
+ *   reference (patchtable/patchtable.h, patchtable_params.h)        this ABI
+ *   -------------------------------------------------------------   ---------------------------
+ *   PatchTableParams::PatchTableParams()   patchtable.cpp:375-377     pt_params_default
+ *   PatchTableParams::set_speed(int)       patchtable.cpp:138-150     pt_params_set_speed
+ *   PatchTableParams::set_from_switches    patchtable.cpp:152-373     pt_params_set_switches
+ *   PatchTable<>::PatchTable(p, b, allowed) patchtable.h:4397-4424    pt_table_create
+ *   PatchTable<>::lookup(a, ann, prev, sv) patchtable.h:4448-4468     pt_table_lookup
+ *   PatchTable<>::save(filename)           patchtable.h:4470-4490     pt_table_save
+ *   PatchTableFixedN(p, filename) (-load)  patchtable.h:1593-1629     pt_table_load
+ *   PatchTable<>::~PatchTable()            patchtable.h:4426-4446     pt_table_destroy
+ *   ASSERT2 / exit(1) error path           solver/util.h:33           int status + pt_last_error
+ *
+ * Conventions
+ *   - Every entry point returns PT_OK (0) or a negative PT_E* status; the message of the last
+ *     failure on the calling thread is returned by pt_last_error().
+ *   - Images are float32, row-major height x width x channels (HWC), exactly the layout of the
+ *     reference's Array<float> (solver/array.h:251-294).
+ *   - The nearest-neighbour field `ann` is (h-7) x (w-7) x 3 doubles per query image of h x w:
+ *     [NNF_X, NNF_Y, NNF_DIST] = (b patch left x, b patch top y, exact RGB SSD), the layout of
+ *     the reference's Array<double> ann (patchtable.h:175-177, 3500).
+ *   - A table owns its device memory and one HIP stream; calls on one table are serialised by
+ *     the caller; distinct tables may be used from distinct threads.
+ *   - Entry points with the _dev suffix take device pointers (inputs already resident in HBM) and
+ *     run on the table's stream; the others take host pointers and copy.
+ */
+#ifndef PATCHTABLE_AMD_H
+#define PATCHTABLE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_OK              0
+#define PT_E_INVALID      -1   /* invalid argument or parameter value                      */
+#define PT_E_UNSUPPORTED  -2   /* parameter combination outside this build's scope         */
+#define PT_E_HIP          -3   /* a HIP runtime call failed                                */
+#define PT_E_IO           -4   /* file could not be read or written                        */
+#define PT_E_NOMEM        -5   /* allocation failed                                        */
+
+/* Enumerations, same values as the reference (patchtable.h:198-236). */
+#define PT_DT_ALGO_RASTER      0
+#define PT_DT_ALGO_PROP        1
+#define PT_DT_ALGO_BRUTE       2
+#define PT_DT_ALGO_KDTREE      3
+#define PT_DT_ALGO_DOWNSAMPLE  4
+#define PT_DT_ALGO_HYBRID      5
+#define PT_DT_ALGO_EUCLIDEAN   6
+#define PT_COLORSPACE_YUV      0
+#define PT_COLORSPACE_LAB      1
+#define PT_LOOKUP_ALGO_TABLE   0
+#define PT_LOOKUP_ALGO_BRUTE   1
+#define PT_LOOKUP_ALGO_KDTREE  2
+#define PT_LOOKUP_ALGO_PM      3
+#define PT_LOOKUP_ALGO_TREECANN 4
+#define PT_KCOH_ALGO_FLANN     0
+#define PT_KCOH_ALGO_PM        1
+#define PT_KCOH_ALGO_ANN       2
+#define PT_KCOH_ALGO_TREECANN  3
+#define PT_DIM_ALGO_WH         0
+#define PT_DIM_ALGO_PCA        1
+
+#define PT_MAX_DIMS 64
+
+/* Plain-old-data mirror of PatchTableParams (patchtable_params.h:23-161). Field names and
+ * meanings follow the reference; switch names accepted by pt_params_set_switches are the
+ * reference's ("-speed", "-limit", "-ndims", ...). Fields that only select code paths outside
+ * this build's scope are kept so that switch strings parse the same way; pt_table_create rejects
+ * values it does not implement with PT_E_UNSUPPORTED. */
+typedef struct pt_params {
+    int    randomize_dt;
+    int    parallel_dt;
+    double limit;              /* max table cells (the "-limit n" switch is in millions)  */
+    int    threads;
+    int    dt_threads;
+    int    convert_colorspace;
+    int    colorspace;
+    int    regular_grid;
+    int    verbose;
+    int    table_knn;
+    int    populate_nearest;
+    int    populate_random;
+    int    partition_step;
+    int    check_existing;
+    int    do_table_lookup;
+    int    init_random;
+    double treecann_eps;
+    int    allowed_index;
+
+    int    ndims;
+    int    nchroma;
+    int    ntables;
+    int    nslices_count;      /* 0: derive slices automatically                          */
+    int    nslices[PT_MAX_DIMS];
+    int    populate_exptime;
+    int    dt_mode;
+    int    dt_algo;
+    int    dt_iters;
+    int    run_dt;
+    int    query_step;
+    int    grid_ndims;         /* -1: same as ndims                                       */
+
+    int    prop_dist;
+    int    calc_exact_dist;
+    int    do_prop;
+    int    prop_y_step;
+    int    do_rs;
+    int    spatial;
+    double prob_rs;
+    int    do_calc_dist;
+    int    lookup_algo;
+
+    int    is_descriptor;
+    int    descriptor_padded;
+    int    sanitize_input;
+
+    char   load_filename[1024];
+    char   save_filename[1024];
+
+    int    patch_w;
+    double coherence_spatial;
+    double coherence_temporal;
+    int    recalc_dist_temporal;
+    double table_ratio;
+    int    dt_knn;
+    int    treecann_agrid;
+    int    treecann_bgrid;
+    int    flann_build_step;
+    int    flann_trees;
+    int    flann_checks;
+    double flann_eps;
+    int    flann_reset_step;
+    int    query_iters;
+    int    prop_iters;
+    double enrich_limit;
+
+    int    kcoherence;
+    int    kcoherence_min_dist;
+    int    kcoherence_algo;
+    int    kcoherence_iter;
+    int    kcoherence_step;
+    int    kcoherence_enrich;
+    int    save_kcoherence;
+    int    incoherence;
+    int    incoherence_min_dist;
+    double triangle_factor;
+
+    int    pm_iters;
+    int    pm_knn;
+    int    pm_min_dist;
+    int    pm_enrich;
+
+    int    pca_samples;
+    int    pca_mean;
+    int    pca_subtract_mean;
+    int    dim_algo;
+    int    filter_dims_count;
+    int    filter_dims[PT_MAX_DIMS];
+    int    auto_filter;
+    int    kdtree_add_unique;
+    int    ann;
+    int    ann_agrid;
+    int    ann_bgrid;
+    double ann_eps;
+
+    /* ---- Parameters of this build's GPU algorithms (no reference counterpart). ---------------
+     * kcoh_window: half-width of the sorted-key candidate window of the k-coherence kNN.
+     * jf_rounds:   number of parallel propagation rounds that replace the raster-order passes.
+     * Set by pt_params_default; accepted as switches "-kcoh_window" and "-jf_rounds". */
+    int    kcoh_window;
+    int    jf_rounds;
+} pt_params;
+
+typedef struct pt_table pt_table;
+
+/* ---- Parameters (patchtable.cpp:26-387) ------------------------------------------------ */
+int pt_params_default(pt_params *p);                          /* PatchTableParams()          */
+int pt_params_set_speed(pt_params *p, int speed);             /* set_speed(i), i in 0..9     */
+int pt_params_set_switches(pt_params *p, const char *s);      /* set_from_switches(string)   */
+
+/* ---- Table construction (PatchTableFixedN constructor, patchtable.h:2584-3373) ---------
+ * b: host image h x w x c (c must be 3 unless p->is_descriptor). allowed: NULL or an
+ * (h-7) x (w-7) int32 mask compared against p->allowed_index. p is read, and the fields the
+ * reference writes back (grid_ndims, patchtable.h:2603) are written back. */
+int pt_table_create(pt_params *p, const float *b, int h, int w, int c,
+                    const int32_t *allowed, pt_table **out);
+/* Same with b already in device memory (HBM). */
+int pt_table_create_dev(pt_params *p, const float *b_dev, int h, int w, int c,
+                        const int32_t *allowed, pt_table **out);
+
+/* ---- Query (lookup_templated, patchtable.h:3468-4243) ----------------------------------
+ * a: host image h x w x c; ann_out: host (h-7) x (w-7) x 3 doubles. ann_prev: NULL or a
+ * previous field of the same shape; sv: NULL or (h-7) x (w-7) floats of per-patch temporal
+ * scale. seconds (may be NULL) receives the wall time of the lookup, measured like the
+ * reference (query-side colour conversion and GCK included, patchtable.h:3477, 4211). */
+int pt_table_lookup(pt_table *t, const float *a, int h, int w, int c, double *ann_out,
+                    const double *ann_prev, const float *sv, double *seconds);
+/* Same with a, ann_out (and ann_prev, sv if given) in device memory. */
+int pt_table_lookup_dev(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_out_dev,
+                        const double *ann_prev_dev, const float *sv_dev, double *seconds);
+
+/* ---- Persistence (patchtable.h:1574-1622; patchtable.cpp:499-573) ---------------------- */
+int pt_table_save(pt_table *t, const char *filename);
+int pt_table_load(pt_params *p, const char *filename, pt_table **out);
+int pt_table_destroy(pt_table *t);
+
+/* ---- Errors ------------------------------------------------------------------------------ */
+const char *pt_last_error(void);
+
+/* ---- Introspection of a built table (for tests and the C++ header shim). ------------------
+ * Shapes: descriptor planes are (h+7) x (w+7) x ndims (HWC, the reference's wh0 layout,
+ * patchtable.h:1538); table is prod(nslices) int32 values XY_TO_INT(x,y) in padded coordinates
+ * (patchtable.h:163-165, 3138); kcoherence is (h-7) x (w-7) x k pairs (XY_TO_INT position,
+ * exact SSD) (patchtable.h:1512-1521). */
+int pt_table_info(const pt_table *t, int *grid_ndims, int *nslices /* [PT_MAX_DIMS] */,
+                  int64_t *ncells, int *table_add_count, int *kcoherence);
+int pt_table_get_descriptor(const pt_table *t, float *wh0_out);
+int pt_table_get_partition(const pt_table *t, int dim, float *min_val, float *max_val,
+                           float *centers /* nslices[dim] */, int32_t *to_bin /* 16384 */);
+int pt_table_get_cells(const pt_table *t, int32_t *table_out);
+int pt_table_get_populated(const pt_table *t, int32_t *table_out);
+int pt_table_get_kcoherence(const pt_table *t, int32_t *pos_out, float *dist_out);
+/* Query-side descriptor of the most recent lookup, (h+7) x (w+7) x ndims. */
+int pt_table_get_query_descriptor(const pt_table *t, float *wh_out);
+/* Per-stage device times (milliseconds) of the last create / lookup, from HIP events:
+ * [0] gck, [1] quantiser, [2] populate, [3] dt, [4] kcoherence, [5] build total,
+ * [6] query gck, [7] query candidates, [8] query propagation, [9] query total. */
+int pt_table_stage_ms(const pt_table *t, double *ms /* [10] */);
+
+/* ---- Stand-alone kernels (device pointers, stream 0 of the current device) ----------------
+ * Used by the parity tests and the benchmark to exercise one kernel at a time. */
+/* GCK projection of an RGB (or YUV when convert_colorspace is 0) image: out is
+ * (h+7) x (w+7) x (n+2*nchroma) HWC, as gck<float,float>(in,out,n,nchroma,8) (gck.h:1957). */
+int pt_gck_dev(const float *img_dev, int h, int w, int n, int nchroma, int convert_yuv,
+               float *out_dev);
+/* Exact 8x8 RGB SSD of npairs patch pairs with the reference's lane order
+ * (patch_dist_exact, patchtable.h:330-373). Patch i compares a at (ax[i],ay[i]) with b at
+ * (bx[i],by[i]); images are HWC float32 with 3 channels. */
+int pt_patch_ssd_dev(const float *a_dev, int aw, const float *b_dev, int bw,
+                     const int32_t *ax, const int32_t *ay, const int32_t *bx, const int32_t *by,
+                     int npairs, float *out_dev);
+
+/* ---- Seeded synthetic test images (the generator used by the tests and the benchmark) ----
+ * a = 1/f^2 spectral noise plus 40 random coloured rectangles/ellipses; b = a warped by a
+ * smooth random displacement plus Gaussian noise (sigma 0.02), clipped to [0,1]. Host memory,
+ * h x w x 3 float32 each; deterministic for a given (seed, h, w). */
+int pt_synth_pair(uint64_t seed, int h, int w, float *a, float *b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PATCHTABLE_AMD_H */

@@ -1,0 +1,1344 @@
+// ============================================================================================
+// PatchTable ORACLE -- TEST INFRASTRUCTURE ONLY.
+//
+// A CPU restatement of the reference's hot path (uva-graphics/patchtable), written from the
+// reference's source as the checker for the HIP implementation. Only tests/, the smoke check
+// and the CPU-baseline leg of bench.py load it; the product library never links or calls it.
+//
+// Every function cites the reference file:line it follows. Floating-point expressions keep the
+// reference's operand order and precision (FP64 where the reference uses double literals), and
+// the file is compiled with -ffp-contract=off so no fused multiply-add changes a rounding.
+//
+// Parity status per function (how each is pinned; see tests/test_oracle_*.py):
+//   glibc rand / random_shuffle ... golden vectors from the system glibc/libstdc++ (tests/golden)
+//   rgb2yuv ........................ closed form in numpy float64 -> float32
+//   gck / gck recurrences .......... WH-basis known-answer test (gck.py:70-77), recurrence table
+//                                    extracted from gck.h (tests/golden), brute-force 64-tap
+//                                    correlation in float64 on small images
+//   quantiser ...................... brute force (sort, quantiles, nearest centre), limit
+//                                    invariants of scale_slices_to_limit
+//   populate ....................... invariant: each cell holds its minimum-rank patch
+//   dt prop (BFS) .................. hand-worked known answers on tiny tables; invariants
+//   dt downsample / euclidean ...... brute-force definitions on tiny tables
+//   patch SSD ...................... float64 recompute, SSE lane order restated
+//   k-coherence .................... this build's own kNN (the reference uses FLANN, absent
+//                                    and unpinned): parity unpinned vs the reference; recall
+//                                    against exact kNN is measured instead
+//   lookup (raster) ................ invariants of stress.py:134-154; quality only
+// ============================================================================================
+#include "../include/patchtable_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace orc {
+
+typedef std::vector<float> fvec;
+typedef std::vector<int32_t> ivec;
+
+static const int TABLE_UNUSED = (1 << 30) - 1;   // patchtable.h:109-112
+static const int NMAP = 16384;                    // PATCH_PARTITION_NMAP, patchtable.h:216
+static const double DIST_INFINITY = 100000;       // patchtable.h:173
+
+static inline int XY_TO_INT(int x, int y) { return x | (y << 16); }   // patchtable.h:163
+static inline int INT_TO_X(int v) { return v & 65535; }
+static inline int INT_TO_Y(int v) { return v >> 16; }
+
+// --------------------------------------------------------------------------------------------
+// glibc random(3), TYPE_3 additive feedback generator, as rand() after the implicit srand(1).
+// Restates glibc stdlib/random_r.c (__srandom_r, __random_r): r[0]=seed, r[i]=16807*r[i-1]
+// mod (2^31-1) for i<31, r[i]=r[i-31] for i in 31..33, then r[i]=r[i-3]+r[i-31] (mod 2^32);
+// the first 310 values after the seed block are discarded and each output is r>>1.
+// --------------------------------------------------------------------------------------------
+struct GlibcRand {
+    uint32_t r[34];
+    int pos;   // index (mod 34) of the next value to produce
+    explicit GlibcRand(uint32_t seed = 1) {
+        int32_t s[344];
+        s[0] = (int32_t)(seed == 0 ? 1 : seed);
+        for (int i = 1; i < 31; i++) {
+            int64_t hi = s[i - 1] / 127773, lo = s[i - 1] % 127773;
+            int64_t word = 16807 * lo - 2836 * hi;
+            if (word < 0) word += 2147483647;
+            s[i] = (int32_t)word;
+        }
+        for (int i = 31; i < 34; i++) s[i] = s[i - 31];
+        uint32_t u[344];
+        for (int i = 0; i < 34; i++) u[i] = (uint32_t)s[i];
+        for (int i = 34; i < 344; i++) u[i] = u[i - 31] + u[i - 3];
+        for (int i = 0; i < 34; i++) r[i] = u[310 + i];
+        pos = 0;   // r holds u[310..343]; next value is u[344]
+    }
+    int32_t next() {
+        // u[k] = u[k-31] + u[k-3]; ring of the last 34 values, r[pos] = u[k-34].
+        uint32_t v = r[(pos + 3) % 34] + r[(pos + 31) % 34];
+        r[pos] = v;
+        pos = (pos + 1) % 34;
+        return (int32_t)(v >> 1);
+    }
+};
+
+// libstdc++ std::random_shuffle(first, last) with rand() (bits/stl_algo.h:4568-4584):
+// for i = 1..n-1: j = rand() % (i+1); if (i != j) swap(a[i], a[j]).
+static void random_shuffle(int32_t *a, int n, GlibcRand &rng) {
+    for (int i = 1; i < n; i++) {
+        int j = rng.next() % (i + 1);
+        if (i != j) std::swap(a[i], a[j]);
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+// rgb2yuv (solver/array.h:1116-1134): double-precision weights, float storage.
+// --------------------------------------------------------------------------------------------
+static void rgb2yuv(const float *in, int h, int w, float *out) {
+    for (long i = 0; i < (long)h * w; i++) {
+        float R = in[i * 3 + 0], G = in[i * 3 + 1], B = in[i * 3 + 2];
+        out[i * 3 + 0] = (float)(0.299 * R + 0.587 * G + 0.114 * B);
+        out[i * 3 + 1] = (float)(-0.14713 * R + -0.28886 * G + 0.436 * B);
+        out[i * 3 + 2] = (float)(0.615 * R + -0.51499 * G + -0.10001 * B);
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+// Walsh-Hadamard basis in Gray-code order and the snake ordering of 2-D kernels
+// (gck.py:37-67). Each consecutive pair of kernels in snake order is linked by one recurrence
+// along x or y with offset delta and sign (gck.py:389-400).
+// --------------------------------------------------------------------------------------------
+static void wh_basis_rec(std::vector<int> &L, int level, std::vector<int> &alpha, int w,
+                         std::vector<int> &sign_level, std::vector<std::vector<int>> &vecs,
+                         std::vector<std::vector<int>> &alphas) {
+    std::vector<int> L1 = L, L2 = L;
+    for (int x : L) L1.push_back(sign_level[level] * x);
+    for (int x : L) L2.push_back(-sign_level[level] * x);
+    std::vector<int> a1 = alpha, a2 = alpha;
+    a1.push_back(sign_level[level]);
+    a2.push_back(-sign_level[level]);
+    sign_level[level] = -sign_level[level];
+    if ((int)L1.size() == w) {
+        vecs.push_back(L1);
+        vecs.push_back(L2);
+        alphas.push_back(a1);
+        alphas.push_back(a2);
+    } else {
+        wh_basis_rec(L1, level + 1, a1, w, sign_level, vecs, alphas);
+        wh_basis_rec(L2, level + 1, a2, w, sign_level, vecs, alphas);
+    }
+}
+
+static void wh_basis(int w, std::vector<std::vector<int>> &vecs, std::vector<std::vector<int>> &alphas) {
+    int k = 0;
+    while ((1 << k) < w) k++;
+    std::vector<int> sign_level(k, 1), L(1, 1), alpha;
+    vecs.clear();
+    alphas.clear();
+    wh_basis_rec(L, 0, alpha, w, sign_level, vecs, alphas);
+}
+
+static std::vector<std::pair<int, int>> snake_order(int w) {
+    std::vector<std::pair<int, int>> L;
+    for (int r = 0; r < w; r++)
+        for (int c = 0; c < w; c++) L.push_back({r, c});
+    auto key = [w](const std::pair<int, int> &rc) {
+        int r = rc.first, c = rc.second, m = std::max(r, c);
+        return m + 0.5 + (r - c + w) * (1.0 / (4 * w)) * (m % 2 == 0 ? -1 : 1);
+    };
+    std::stable_sort(L.begin(), L.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+        return key(a) < key(b);
+    });
+    return L;
+}
+
+struct Recur { int axis; int delta; int sign; };   // axis 0: along x, 1: along y
+
+static std::vector<Recur> gck_recurrences(int w) {
+    std::vector<std::vector<int>> vecs, alphas;
+    wh_basis(w, vecs, alphas);
+    auto order = snake_order(w);
+    std::vector<Recur> out;
+    out.push_back({-1, 0, 0});   // kernel 0: box filter
+    for (size_t i = 1; i < order.size(); i++) {
+        bool horiz = order[i].second != order[i - 1].second;
+        const std::vector<int> &a1 = horiz ? alphas[order[i - 1].second] : alphas[order[i - 1].first];
+        const std::vector<int> &a2 = horiz ? alphas[order[i].second] : alphas[order[i].first];
+        int idx = 0;
+        while (a1[idx] == a2[idx]) idx++;
+        out.push_back({horiz ? 0 : 1, 1 << idx, a2[idx]});
+    }
+    return out;
+}
+
+// --------------------------------------------------------------------------------------------
+// GCK projection gck_8<float,float,n,nchroma> (gck.h:29-1950).
+// in: h x w x 3 (YUV). out: (h+7) x (w+7) x (n+2*nchroma), HWC.
+//   H pass (gck.h:54-165):  t(y,0)=in(y-3,0); t(y,x)=(t(y,x-1)+in(y-3,x))-in(y-3,x-8)
+//   V pass (gck.h:166-216): o(0,x)=t(3,x);    o(y,x)=(o(y-1,x)+t(y+3,x))-t(y-5,x)
+//   kernel k>=1 (gck.h:222-1946): k(y,x) = ((s*k(p-d)) + k-1(y,x)) + s*k-1(p-d), p-d the
+//   position delta back along the recurrence axis; out-of-range reads are get_zero()
+//   (solver/array.h:703-708), i.e. element (0,0,0) times zero.
+// --------------------------------------------------------------------------------------------
+static inline int gck_channel(int k, int nchroma) { return k < nchroma ? 3 * k : 3 * nchroma + (k - nchroma); }
+
+static void gck(const float *in, int h, int w, int n, int nchroma, float *out) {
+    const int H = h + 7, W = w + 7, C = n + 2 * nchroma;
+    std::vector<float> temp((size_t)H * W * 3);
+    const float in_zero = in[0] * 0.0f;
+    auto inz = [&](int y, int x, int c) -> float {
+        return (y >= 0 && y < h && x >= 0 && x < w) ? in[((size_t)y * w + x) * 3 + c] : in_zero;
+    };
+    for (int y = 0; y < H; y++) {
+        int ys = y - 3;
+        for (int c = 0; c < 3; c++) temp[((size_t)y * W) * 3 + c] = inz(ys, 0, c);
+        for (int x = 1; x < W; x++)
+            for (int c = 0; c < 3; c++) {
+                float prev = temp[((size_t)y * W + x - 1) * 3 + c];
+                temp[((size_t)y * W + x) * 3 + c] = (prev + inz(ys, x, c)) - inz(ys, x - 8, c);
+            }
+    }
+    const float t_zero = temp[0] * 0.0f;
+    auto tz = [&](int y, int x, int c) -> float {
+        return (y >= 0 && y < H) ? temp[((size_t)y * W + x) * 3 + c] : t_zero;
+    };
+    for (int x = 0; x < W; x++)
+        for (int c = 0; c < 3; c++) out[((size_t)x) * C + c] = temp[((size_t)3 * W + x) * 3 + c];
+    for (int y = 1; y < H; y++)
+        for (int x = 0; x < W; x++)
+            for (int c = 0; c < 3; c++) {
+                float prev = out[((size_t)(y - 1) * W + x) * C + c];
+                out[((size_t)y * W + x) * C + c] = (prev + tz(y + 3, x, c)) - tz(y - 5, x, c);
+            }
+    std::vector<Recur> rec = gck_recurrences(8);
+    const float o_zero = out[0] * 0.0f;
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++)
+            for (int k = 1; k < n; k++) {
+                int cc = k < nchroma ? 3 : 1;
+                int chk = gck_channel(k, nchroma), chp = gck_channel(k - 1, nchroma);
+                const Recur &r = rec[k];
+                int yy = r.axis == 1 ? y - r.delta : y, xx = r.axis == 0 ? x - r.delta : x;
+                bool ok = yy >= 0 && xx >= 0;
+                for (int c = 0; c < cc; c++) {
+                    float a1 = ok ? out[((size_t)yy * W + xx) * C + chk + c] : o_zero;
+                    float a2 = ok ? out[((size_t)yy * W + xx) * C + chp + c] : o_zero;
+                    float cur = out[((size_t)y * W + x) * C + chp + c];
+                    float s1 = r.sign < 0 ? -a1 : a1, s2 = r.sign < 0 ? -a2 : a2;
+                    out[((size_t)y * W + x) * C + chk + c] = (s1 + cur) + s2;
+                }
+            }
+}
+
+// --------------------------------------------------------------------------------------------
+// Exact 8x8 RGB SSD with the SSE lane order of patch_dist_exact (patchtable.h:330-373):
+// four lane accumulators; per row, six 4-wide chunks are added in order; lanes summed last.
+// --------------------------------------------------------------------------------------------
+static inline float patch_ssd(const float *a, int aw, int ax, int ay, const float *b, int bw, int bx, int by) {
+    float L[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int dy = 0; dy < 8; dy++) {
+        const float *ar = a + ((size_t)(ay + dy) * aw + ax) * 3;
+        const float *br = b + ((size_t)(by + dy) * bw + bx) * 3;
+        for (int j = 0; j < 4; j++) {
+            float acc = L[j];
+            for (int n = 0; n < 6; n++) {
+                float d = ar[4 * n + j] - br[4 * n + j];
+                acc = acc + d * d;
+            }
+            L[j] = acc;
+        }
+    }
+    return ((L[0] + L[1]) + L[2]) + L[3];
+}
+
+// --------------------------------------------------------------------------------------------
+// Quantiser.
+// --------------------------------------------------------------------------------------------
+// int(float) as the x86 cvttss2si instruction computes it: out-of-range and NaN give INT_MIN.
+static inline int x86_ftoi(float f) {
+    if (!(f >= -2147483648.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int)f;
+}
+
+struct Partition1D {                       // PatchPartition1D (patchtable.h:1131-1326)
+    fvec centers, lsizes, rsizes;
+    ivec to_bin;
+    float min_val, max_val, range;
+    int nslices;
+    int get_bin(float v) const {           // patchtable.h:1298-1303
+        int idx = x86_ftoi((v - min_val) * (float)(NMAP - 1) / range);
+        if (idx < 0) idx = 0;
+        else if (idx >= NMAP) idx = NMAP - 1;
+        return to_bin[idx];
+    }
+};
+
+// scale_slices_to_limit (patchtable.cpp:611-642) with product()/multiply() (util.cpp:186-206).
+static std::vector<int> scale_slices_to_limit(std::vector<int> ns, double limit) {
+    auto product = [](const std::vector<int> &L) {
+        double a = 1.0;
+        for (int v : L) a *= v;
+        return a;
+    };
+    if (limit > 0) {
+        std::vector<int> ns0(ns);
+        double factor = 1.0;
+        for (auto &v : ns) v = std::max(v, 1);
+        while (product(ns) > limit) {
+            factor *= 0.995;
+            for (size_t i = 0; i < ns.size(); i++) ns[i] = int(ns0[i] * factor + 0.5);
+            for (auto &v : ns) v = std::max(v, 1);
+        }
+        for (;;) {
+            ns0 = ns;
+            bool success = false;
+            for (size_t i = 0; i < ns.size(); i++) {
+                ns[i]++;
+                if (product(ns) > limit) ns = ns0;
+                else {
+                    success = true;
+                    ns0 = ns;
+                }
+            }
+            if (!success) break;
+        }
+    }
+    return ns;
+}
+
+// PatchPartition1D constructor, non-regular grid, randomize=false (patchtable.h:1172-1286).
+// `vals` are the channel's values over the listed patches; only their sorted order matters.
+static Partition1D make_partition(std::vector<float> vals, int nslices, float min_val, float max_val) {
+    Partition1D P;
+    P.nslices = nslices;
+    P.min_val = min_val;
+    P.max_val = max_val;
+    P.range = max_val - min_val;
+    std::sort(vals.begin(), vals.end());
+    long n = (long)vals.size();
+    P.centers.resize(nslices);
+    P.lsizes.resize(nslices);
+    P.rsizes.resize(nslices);
+    for (int i = 0; i < nslices; i++) {
+        long idx1 = (n - 1) * (long)i / nslices;
+        long idx2 = (n - 1) * (long)(i + 1) / nslices;
+        float cell_max = vals[idx2], cell_min = vals[idx1];
+        P.centers[i] = (cell_min + cell_max) / 2;
+    }
+    for (int i = 0; i < nslices; i++) {
+        float prev_center = (i - 1 >= 0) ? P.centers[i - 1] : min_val;
+        float next_center = (i + 1 < nslices) ? P.centers[i + 1] : max_val;
+        P.lsizes[i] = P.centers[i] - prev_center;
+        if (i - 1 >= 0) P.lsizes[i] /= 2;
+        P.rsizes[i] = next_center - P.centers[i];
+        if (i + 1 < nslices) P.rsizes[i] /= 2;
+    }
+    P.to_bin.resize(NMAP);
+    int icenter = 0;
+    for (int i = 0; i < NMAP; i++) {
+        double coord = min_val + (i + 0.5) * P.range / (NMAP);
+        while (coord > P.centers[icenter]) {
+            if (icenter + 1 < nslices) icenter++;
+            else break;
+        }
+        int prev_center = icenter - 1;
+        if (prev_center < 0) prev_center = 0;
+        int next_center = icenter;
+        if (next_center == 0) next_center++;
+        if (nslices == 1) next_center = prev_center = 0;
+        double mid = (P.centers[prev_center] + P.centers[next_center]) / 2.0;
+        P.to_bin[i] = (coord < mid) ? prev_center : next_center;
+    }
+    return P;
+}
+
+// --------------------------------------------------------------------------------------------
+// The built table (PatchTableFixedN members, patchtable.h:1484-1540).
+// --------------------------------------------------------------------------------------------
+struct Table {
+    pt_params p;
+    int h = 0, w = 0;                 // b image size
+    int ndims = 0, G = 0;             // descriptor dims, grid dims
+    fvec b0;                          // RGB copy of b (patchtable.h:2628)
+    fvec wh0;                         // (h+7) x (w+7) x ndims
+    std::vector<int> nslices;
+    std::vector<int64_t> stride;      // row-major over nslices, dim 0 slowest
+    int64_t ncells = 0;
+    std::vector<Partition1D> part;
+    ivec table_pop;                   // after populate
+    ivec table;                       // after DT
+    int table_add_count = 0;
+    int K = 0;                        // k-coherence list length
+    ivec kpos;                        // (h-7) x (w-7) x K
+    fvec kdist;
+    int bew() const { return w - 7; }
+    int beh() const { return h - 7; }
+    int cell_of(const float *desc) const {
+        int64_t idx = 0;
+        for (int i = 0; i < G; i++) idx += (int64_t)part[i].get_bin(desc[i]) * stride[i];
+        return (int)idx;
+    }
+};
+
+// Descriptor of an RGB image: colour conversion then GCK (get_padded_descriptor,
+// patchtable.h:2492-2546, and reduce_dim, :1687-1728, for dim_algo WH without filter_dims).
+static void descriptor(const pt_params &p, const float *img, int h, int w, fvec &wh) {
+    fvec yuv((size_t)h * w * 3);
+    if (p.convert_colorspace) rgb2yuv(img, h, w, yuv.data());
+    else std::copy(img, img + (size_t)h * w * 3, yuv.begin());
+    wh.assign((size_t)(h + 7) * (w + 7) * p.ndims, 0.f);
+    gck(yuv.data(), h, w, p.ndims - p.nchroma * 2, p.nchroma, wh.data());
+}
+
+// ---------------------------- distance transforms -------------------------------------------
+
+// do_dt_prop (patchtable.h:2081-2165): level-synchronous multi-source BFS over the cell
+// lattice with PropEdge labels (patchtable.h:1447-1458); first claim in list order wins.
+static void dt_prop_bfs(Table &T, int dt_iters) {
+    ivec &tab = T.table;
+    int G = T.G;
+    int64_t nel = T.ncells;
+    struct Elem { int64_t gi; int32_t val; uint8_t label; };
+    std::vector<Elem> cur, nxt;
+    for (int64_t j = 0; j < nel; j++)
+        if (tab[j] != TABLE_UNUSED) cur.push_back({j, tab[j], 0});
+    int nlabels = 1 + G * 2;
+    struct Edge { int dim, dir, label; };
+    std::vector<std::vector<Edge>> next_label;
+    auto mk = [&](int dim, int dir) { return Edge{dim, dir, 2 * dim + 1 + (dir + 1) / 2}; };
+    std::vector<Edge> e0;
+    for (int i = 0; i < G; i++) {
+        e0.push_back(mk(i, -1));
+        e0.push_back(mk(i, 1));
+    }
+    next_label.push_back(e0);
+    for (int i = 0; i < G; i++)
+        for (int dir = -1; dir <= 1; dir += 2) {
+            std::vector<Edge> e;
+            e.push_back(mk(i, dir));
+            for (int j = i + 1; j < G; j++) {
+                e.push_back(mk(j, -1));
+                e.push_back(mk(j, 1));
+            }
+            next_label.push_back(e);
+        }
+    (void)nlabels;
+    for (int iter = 0;; iter++) {
+        if (dt_iters > 0 && iter > dt_iters) break;
+        if (cur.empty()) break;
+        nxt.clear();
+        for (const Elem &el : cur) {
+            for (const Edge &e : next_label[el.label]) {
+                int64_t gp = el.gi + e.dir * T.stride[e.dim];
+                if (gp >= 0 && gp < nel && tab[gp] == TABLE_UNUSED) {
+                    int64_t dim_index = (el.gi / T.stride[e.dim]) % T.nslices[e.dim] + e.dir;
+                    if (dim_index >= 0 && dim_index < T.nslices[e.dim]) {
+                        tab[gp] = el.val;
+                        nxt.push_back({gp, el.val, (uint8_t)e.label});
+                    }
+                }
+            }
+        }
+        std::swap(cur, nxt);
+    }
+}
+
+// Separable form of the same transform (the GPU algorithm): nearest filled cell in L1 over the
+// cell lattice, ties to the smallest source cell index; optionally limited to distance
+// max_dist (-1: none). Used by the tests to show it equals dt_prop_bfs.
+static void dt_prop_separable(Table &T, int max_dist) {
+    int64_t nel = T.ncells;
+    const uint64_t INF = ~0ull;
+    std::vector<uint64_t> key(nel);
+    for (int64_t j = 0; j < nel; j++) key[j] = T.table[j] != TABLE_UNUSED ? (uint64_t)j : INF;
+    for (int d = 0; d < T.G; d++) {
+        int64_t st = T.stride[d], n = T.nslices[d];
+        for (int64_t base = 0; base < nel; base++) {
+            if ((base / st) % n != 0) continue;   // line start along dim d
+            uint64_t run = INF;
+            for (int64_t i = 0; i < n; i++) {
+                uint64_t &k = key[base + i * st];
+                if (run != INF) run += (1ull << 32);
+                if (k < run) run = k;
+                k = run;
+            }
+            run = INF;
+            for (int64_t i = n - 1; i >= 0; i--) {
+                uint64_t &k = key[base + i * st];
+                if (run != INF) run += (1ull << 32);
+                if (k < run) run = k;
+                k = run;
+            }
+        }
+    }
+    ivec out(nel);
+    for (int64_t j = 0; j < nel; j++) {
+        uint64_t k = key[j];
+        if (k == INF || (max_dist >= 0 && (int64_t)(k >> 32) > max_dist)) out[j] = TABLE_UNUSED;
+        else out[j] = T.table[(int64_t)(k & 0xffffffffu)];
+    }
+    T.table.swap(out);
+}
+
+// do_dt_downsample (patchtable.h:2339-2361) with dt_downsample / dt_upsample
+// (dt_downsample.h, generated for 3..N dims): halve every dimension ((s+1)/2) until all are 1;
+// a coarse cell takes the first filled child in nested-loop order (dim 0 outermost); then going
+// back up, each empty fine cell copies its parent.
+static void dt_downsample(Table &T) {
+    int G = T.G;
+    std::vector<std::vector<int>> sizes;
+    std::vector<ivec> pyr;
+    sizes.push_back(T.nslices);
+    pyr.push_back(T.table);
+    auto strides_of = [&](const std::vector<int> &s) {
+        std::vector<int64_t> st(G);
+        int64_t acc = 1;
+        for (int i = G - 1; i >= 0; i--) {
+            st[i] = acc;
+            acc *= s[i];
+        }
+        return st;
+    };
+    auto all_one = [&](const std::vector<int> &s) {
+        for (int v : s) if (v != 1) return false;
+        return true;
+    };
+    while (!all_one(sizes.back())) {
+        const std::vector<int> &hs = sizes.back();
+        std::vector<int> ls(G);
+        for (int i = 0; i < G; i++) ls[i] = (hs[i] + 1) / 2;
+        auto hst = strides_of(hs), lst = strides_of(ls);
+        int64_t ln = 1;
+        for (int v : ls) ln *= v;
+        ivec lo(ln);
+        const ivec &hi = pyr.back();
+        std::vector<int> lc(G), hc(G);
+        for (int64_t li = 0; li < ln; li++) {
+            for (int i = 0; i < G; i++) lc[i] = (int)((li / lst[i]) % ls[i]);
+            int nchild = 1;
+            std::vector<int> lo_c(G), cnt(G);
+            for (int i = 0; i < G; i++) {
+                lo_c[i] = 2 * lc[i];
+                cnt[i] = std::min(2 * (lc[i] + 1), hs[i]) - lo_c[i];
+                nchild *= cnt[i];
+            }
+            int32_t v_out = TABLE_UNUSED;
+            for (int c = 0; c < nchild; c++) {   // nested order, dim 0 outermost
+                int rem = c;
+                int64_t hidx = 0;
+                for (int i = G - 1; i >= 0; i--) {
+                    int ci = rem % cnt[i];
+                    rem /= cnt[i];
+                    hidx += (int64_t)(lo_c[i] + ci) * hst[i];
+                }
+                int32_t v = hi[hidx];
+                if (!(v & (1 << 30)) && v != TABLE_UNUSED) {
+                    v_out = v;
+                    break;
+                }
+            }
+            lo[li] = v_out;
+        }
+        sizes.push_back(ls);
+        pyr.push_back(lo);
+    }
+    for (int lvl = (int)pyr.size() - 1; lvl >= 1; lvl--) {
+        const std::vector<int> &hs = sizes[lvl - 1], &ls = sizes[lvl];
+        auto hst = strides_of(hs), lst = strides_of(ls);
+        int64_t hn = 1;
+        for (int v : hs) hn *= v;
+        ivec &hi = pyr[lvl - 1];
+        const ivec &lo = pyr[lvl];
+        for (int64_t hidx = 0; hidx < hn; hidx++) {
+            if (hi[hidx] != TABLE_UNUSED) continue;
+            int64_t lidx = 0;
+            for (int i = 0; i < G; i++) lidx += (int64_t)(((hidx / hst[i]) % hs[i]) / 2) * lst[i];
+            hi[hidx] = lo[lidx];
+        }
+    }
+    T.table.swap(pyr[0]);
+}
+
+// euclidean_dt + do_dt_euclidean (patchtable.h:4244-4352): Felzenszwalb-Huttenlocher lower
+// envelope along dim 0, then 1, ..., carrying the table value; float arithmetic, INF = 1e20.
+static void euclidean_line(float *f, int n, float *dist, int *v, float *z, int *idx) {
+    const float INF = 1E20;
+    int k = 0;
+    v[0] = 0;
+    z[0] = -INF;
+    z[1] = +INF;
+    for (int q = 1; q <= n - 1; q++) {
+        float s = ((f[q] + q * q) - (f[v[k]] + v[k] * v[k])) / (2 * q - 2 * v[k]);
+        while (s <= z[k]) {
+            k--;
+            s = ((f[q] + q * q) - (f[v[k]] + v[k] * v[k])) / (2 * q - 2 * v[k]);
+        }
+        k++;
+        v[k] = q;
+        z[k] = s;
+        z[k + 1] = +INF;
+    }
+    k = 0;
+    for (int q = 0; q <= n - 1; q++) {
+        while (z[k + 1] < q) k++;
+        dist[q] = (q - v[k]) * (q - v[k]) + f[v[k]];
+        idx[q] = idx[v[k]];
+    }
+}
+
+static void dt_euclidean(Table &T) {
+    const float INF = 1E20;
+    int64_t nel = T.ncells;
+    int G = T.G;
+    fvec td(nel);
+    for (int64_t j = 0; j < nel; j++) td[j] = T.table[j] == TABLE_UNUSED ? INF : 0.0f;
+    int maxdim = 0;
+    for (int v : T.nslices) maxdim = std::max(maxdim, v);
+    fvec f(maxdim), dist(maxdim), z(maxdim + 1);
+    std::vector<int> v(maxdim), idx(maxdim);
+    std::vector<int64_t> outer(G);
+    for (int inner = 0; inner < G; inner++) {
+        int64_t product = 1;
+        for (int dim = G - 1; dim >= 0; dim--)
+            if (dim != inner) {
+                outer[dim] = product;
+                product *= T.nslices[dim];
+            }
+        for (int64_t j = 0; j < product; j++) {
+            int64_t ai = 0;
+            for (int dim = 0; dim < G; dim++)
+                if (dim != inner) ai += ((j / outer[dim]) % T.nslices[dim]) * T.stride[dim];
+            int64_t st = T.stride[inner];
+            int n = T.nslices[inner];
+            for (int q = 0; q < n; q++) {
+                f[q] = td[ai + q * st];
+                idx[q] = T.table[ai + q * st];
+            }
+            euclidean_line(f.data(), n, dist.data(), v.data(), z.data(), idx.data());
+            for (int q = 0; q < n; q++) {
+                td[ai + q * st] = dist[q];
+                T.table[ai + q * st] = idx[q];
+            }
+        }
+    }
+}
+
+// ---------------------------- k-coherence ---------------------------------------------------
+// This build's k-nearest-neighbour lists (replaces FLANN, patchtable.h:2653-2881). For every
+// database patch p:
+//   1. candidates: the patches within `window` positions of p in two sort orders of all patches,
+//      by the Morton interleave of their table bins and by the Morton interleave of the bins
+//      offset by half the slice count;
+//   2. valid candidates are at pixel distance^2 >= kcoherence_min_dist^2 from p
+//      (patchtable.h:2758-2768);
+//   3. the K valid candidates nearest in descriptor L2 (all ndims, summed in index order;
+//      ties to the smaller raster index) are kept;
+//   4. they are re-scored by exact RGB SSD and sorted by (SSD, XY_TO_INT) as the reference
+//      sorts its (dist, (index, valid)) triples (patchtable.h:2815-2835).
+// Lists shorter than K (tiny images) repeat their last entry, or p itself when empty.
+static int morton_bits(const Table &T) {
+    int mx = 1;
+    for (int i = 0; i < T.G; i++) mx = std::max(mx, T.nslices[i] + (T.nslices[i] >> 1));
+    int B = 0;
+    while ((1 << B) < mx) B++;
+    return B;
+}
+
+static uint64_t morton_key(const Table &T, const int *bins, int shifted, int B) {
+    int Gk = T.G;
+    while (Gk * B > 64) Gk--;
+    uint64_t key = 0;
+    for (int b = B - 1; b >= 0; b--)
+        for (int d = 0; d < Gk; d++) {
+            uint32_t v = (uint32_t)bins[d] + (shifted ? (uint32_t)(T.nslices[d] >> 1) : 0u);
+            key = (key << 1) | ((v >> b) & 1u);
+        }
+    return key;
+}
+
+static void build_kcoherence(Table &T) {
+    const int K = T.p.kcoherence, W = T.p.kcoh_window;
+    const int bew = T.bew(), beh = T.beh(), nd = T.ndims, Wp = T.w + 7;
+    const int md2 = T.p.kcoherence_min_dist * T.p.kcoherence_min_dist;
+    const long N = (long)bew * beh;
+    T.K = K;
+    T.kpos.assign((size_t)N * K, 0);
+    T.kdist.assign((size_t)N * K, 0.f);
+    if (K <= 0 || N == 0) return;
+    const int B = morton_bits(T);
+    std::vector<uint64_t> keys[2];
+    std::vector<int32_t> perm[2], pos[2];
+    std::vector<int> bins(T.G);
+    for (int o = 0; o < 2; o++) keys[o].resize(N);
+    for (long i = 0; i < N; i++) {
+        int px = (int)(i % bew), py = (int)(i / bew);
+        const float *d = &T.wh0[((size_t)(py + 7) * Wp + px + 7) * nd];
+        for (int g = 0; g < T.G; g++) bins[g] = T.part[g].get_bin(d[g]);
+        keys[0][i] = morton_key(T, bins.data(), 0, B);
+        keys[1][i] = morton_key(T, bins.data(), 1, B);
+    }
+    for (int o = 0; o < 2; o++) {
+        perm[o].resize(N);
+        pos[o].resize(N);
+        for (long i = 0; i < N; i++) perm[o][i] = (int32_t)i;
+        const std::vector<uint64_t> &ky = keys[o];
+        std::sort(perm[o].begin(), perm[o].end(), [&](int32_t a, int32_t b) {
+            return ky[a] != ky[b] ? ky[a] < ky[b] : a < b;
+        });
+        for (long r = 0; r < N; r++) pos[o][perm[o][r]] = (int32_t)r;
+    }
+    struct Cand { float d; int32_t q; };
+    std::vector<Cand> cands;
+    for (long i = 0; i < N; i++) {
+        int px = (int)(i % bew), py = (int)(i / bew);
+        const float *dp = &T.wh0[((size_t)(py + 7) * Wp + px + 7) * nd];
+        cands.clear();
+        for (int o = 0; o < 2; o++) {
+            long r = pos[o][i];
+            for (long t = r - W; t <= r + W; t++) {
+                if (t < 0 || t >= N || t == r) continue;
+                int32_t q = perm[o][t];
+                int qx = q % bew, qy = q / bew;
+                int dx = qx - px, dy = qy - py;
+                if (dx * dx + dy * dy < md2) continue;
+                const float *dq = &T.wh0[((size_t)(qy + 7) * Wp + qx + 7) * nd];
+                float dd = 0.f;
+                for (int c = 0; c < nd; c++) {
+                    float e = dp[c] - dq[c];
+                    dd = dd + e * e;
+                }
+                cands.push_back({dd, q});
+            }
+        }
+        std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) {
+            return a.d != b.d ? a.d < b.d : a.q < b.q;
+        });
+        cands.erase(std::unique(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.q == b.q; }),
+                    cands.end());
+        int m = std::min((int)cands.size(), K);
+        struct Sel { float ssd; int32_t xy; };
+        std::vector<Sel> sel(m);
+        for (int k = 0; k < m; k++) {
+            int qx = cands[k].q % bew, qy = cands[k].q / bew;
+            sel[k].ssd = patch_ssd(T.b0.data(), T.w, px, py, T.b0.data(), T.w, qx, qy);
+            sel[k].xy = XY_TO_INT(qx, qy);
+        }
+        std::sort(sel.begin(), sel.end(), [](const Sel &a, const Sel &b) {
+            return a.ssd != b.ssd ? a.ssd < b.ssd : a.xy < b.xy;
+        });
+        for (int k = 0; k < K; k++) {
+            Sel s = m > 0 ? sel[std::min(k, m - 1)] : Sel{0.f, XY_TO_INT(px, py)};
+            T.kpos[(size_t)i * K + k] = s.xy;
+            T.kdist[(size_t)i * K + k] = s.ssd;
+        }
+    }
+}
+
+// ---------------------------- table construction --------------------------------------------
+// PatchTableFixedN constructor (patchtable.h:2584-3373), default options: no allowed mask,
+// colour conversion + GCK, k-coherence, quantiser, fast random populate, DT, fill_zeros.
+static Table *build(const pt_params &p_in, const float *b, int h, int w) {
+    Table *T = new Table();
+    T->p = p_in;
+    pt_params &p = T->p;
+    if (p.grid_ndims < 0) p.grid_ndims = p.ndims;      // patchtable.h:2603
+    T->h = h;
+    T->w = w;
+    T->ndims = p.ndims;
+    T->G = p.grid_ndims;
+    T->b0.assign(b, b + (size_t)h * w * 3);
+    descriptor(p, b, h, w, T->wh0);
+    const int Wp = w + 7, nd = T->ndims, G = T->G;
+    const int bew = T->bew(), beh = T->beh();
+    // set_min_max (patchtable.h:1758-1778)
+    std::vector<float> mn(G, (float)1e20), mx(G, (float)-1e20);
+    for (int y = 7; y < h; y++)
+        for (int x = 7; x < w; x++)
+            for (int i = 0; i < G; i++) {
+                float v = T->wh0[((size_t)y * Wp + x) * nd + i];
+                mn[i] = mn[i] < v ? mn[i] : v;
+                mx[i] = mx[i] > v ? mx[i] : v;
+            }
+    std::vector<float> range(G);
+    for (int i = 0; i < G; i++) range[i] = mx[i] - mn[i];
+    // set_default_nslices (patchtable.h:1780-1810) or explicit -nslices, then the limit.
+    std::vector<int> ns(G);
+    if (p.nslices_count == 0) {
+        float max_range = range[0];
+        for (int i = 1; i < G; i++) max_range = std::max(max_range, range[i]);
+        double scale = 200.0 / max_range;
+        for (int i = 0; i < G; i++) ns[i] = max_range > 0 ? int(range[i] * scale) : 0;
+    } else {
+        for (int i = 0; i < G; i++) ns[i] = p.nslices[i];
+    }
+    T->nslices = scale_slices_to_limit(ns, p.limit);
+    T->stride.assign(G, 0);
+    int64_t acc = 1;
+    for (int i = G - 1; i >= 0; i--) {
+        T->stride[i] = acc;
+        acc *= T->nslices[i];
+    }
+    T->ncells = acc;
+    // k-coherence is built before the partition in the reference; it reads the table bins
+    // here, so it is built after the partition (the lists do not depend on the order).
+    for (int i = 0; i < G; i++) {
+        std::vector<float> vals;
+        vals.reserve((size_t)bew * beh);
+        for (int y = 7; y < h; y += p.partition_step)
+            for (int x = 7; x < w; x += p.partition_step) vals.push_back(T->wh0[((size_t)y * Wp + x) * nd + i]);
+        T->part.push_back(make_partition(vals, T->nslices[i], mn[i], mx[i]));
+    }
+    // Populate (patchtable.h:3121-3159), TABLE_POPULATE_FAST.
+    T->table.assign(T->ncells, TABLE_UNUSED);
+    std::vector<int32_t> order;
+    for (int y = 7; y < h; y++)
+        for (int x = 7; x < w; x++) order.push_back(XY_TO_INT(x, y));
+    if (p.populate_random) {
+        GlibcRand rng(1);
+        random_shuffle(order.data(), (int)order.size(), rng);
+    }
+    T->table_add_count = 0;
+    for (int32_t pos : order) {
+        int x = INT_TO_X(pos), y = INT_TO_Y(pos);
+        int cell = T->cell_of(&T->wh0[((size_t)y * Wp + x) * nd]);
+        if (T->table[cell] == TABLE_UNUSED) {
+            T->table[cell] = pos;
+            T->table_add_count++;
+        }
+    }
+    T->table_pop = T->table;
+    // DT dispatch (patchtable.h:3254-3295).
+    bool zeros_filled = false;
+    if (p.run_dt) {
+        if (p.dt_algo == PT_DT_ALGO_PROP) {
+            dt_prop_bfs(*T, p.dt_iters);
+            zeros_filled = p.dt_iters <= 0;
+        } else if (p.dt_algo == PT_DT_ALGO_DOWNSAMPLE) {
+            dt_downsample(*T);
+            zeros_filled = true;
+        } else if (p.dt_algo == PT_DT_ALGO_EUCLIDEAN) {
+            dt_euclidean(*T);
+            zeros_filled = true;
+        }
+    }
+    if (!zeros_filled) {   // fill_zeros (patchtable.h:2063-2079)
+        int32_t vfill = XY_TO_INT(7, 7);
+        for (auto &v : T->table)
+            if (v == TABLE_UNUSED) v = vfill;
+    }
+    if (p.kcoherence > 0) build_kcoherence(*T);
+    return T;
+}
+
+// ---------------------------- lookup (reference raster order) -------------------------------
+// lookup_templated<is_descriptor=0, use_allowed_patches=0> (patchtable.h:3468-4217), no
+// ann_prev, no init_random, query_step 1, prop_dist 0, single table.
+static void lookup_raster(const Table &T, const float *a, int ah, int aw, double *ann) {
+    const pt_params &p = T.p;
+    fvec a_wh;
+    descriptor(p, a, ah, aw, a_wh);
+    const int ann_h = ah - 7, ann_w = aw - 7;
+    const int b0_ew = T.w - 7, b0_eh = T.h - 7;
+    const int a_w = aw, a_h = ah, b_w = T.w, b_h = T.h;
+    const int nd = T.ndims, AWp = aw + 7;
+    const float *b = T.b0.data();
+    for (int y = 0; y < ann_h; y++)
+        for (int x = 0; x < ann_w; x++) {
+            double *q = &ann[((size_t)y * ann_w + x) * 3];
+            q[0] = b0_ew + 1;
+            q[1] = 0;        // left uninitialised by the reference; any value behaves the same
+            q[2] = DIST_INFINITY;
+        }
+    const double prop_scale = 1.0 / (1 + p.coherence_spatial);
+    const int qs = p.query_step;
+    const int prop_w = (qs + p.prop_dist) * 2 + 1;
+    const int sat_w = 8 + prop_w + 1 - 1;
+    const int sat_w_nopad = sat_w - 1;
+    const bool do_rs_or_spatial = p.do_rs || p.spatial;
+    auto A = [&](int yy, int xx, int c) -> float { return a[((size_t)yy * aw + xx) * 3 + c]; };
+    auto Bp = [&](int yy, int xx, int c) -> float { return b[((size_t)yy * b_w + xx) * 3 + c]; };
+    auto ANN = [&](int yy, int xx) -> double * { return &ann[((size_t)yy * ann_w + xx) * 3]; };
+    for (int prop_iter = 0; prop_iter < p.prop_iters; prop_iter++) {
+        bool reverse = prop_iter % 2;
+        bool do_lookup = prop_iter == 0 && p.do_table_lookup;
+        bool do_kcoh = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == prop_iter);
+        for (int y0 = 0; y0 < ann_h; y0 += qs) {
+            int y = reverse ? (ann_h - 1 - y0) : y0;
+            std::vector<float> sat(sat_w * sat_w, 0.f);
+            for (int x0 = 0; x0 < ann_w; x0 += qs) {
+                int x = reverse ? (ann_w - 1 - x0) : x0;
+                int xbest = (int)ANN(y, x)[0];
+                int ybest = (int)ANN(y, x)[1];
+                float dbest = (float)(ANN(y, x)[2] * prop_scale);
+                auto dist = [&](int xs, int ys) { return patch_ssd(a, aw, x, y, b, b_w, xs, ys); };
+                if (do_lookup) {
+                    const float *whp = &a_wh[((size_t)(y + 7) * AWp + x + 7) * nd];
+                    int v = T.table[T.cell_of(whp)];
+                    int xsrc = INT_TO_X(v) - 7, ysrc = INT_TO_Y(v) - 7;
+                    if (xsrc != xbest || ysrc != ybest) {
+                        float d = dist(xsrc, ysrc);
+                        if (d < dbest) {
+                            dbest = d;
+                            xbest = xsrc;
+                            ybest = ysrc;
+                        }
+                    }
+                }
+                if (do_kcoh) {
+                    const int32_t *kp = &T.kpos[((size_t)ybest * b0_ew + xbest) * T.K];
+                    const float *kd = &T.kdist[((size_t)ybest * b0_ew + xbest) * T.K];
+                    for (int k = 0; k < p.kcoherence; k++) {
+                        if (kd[k] >= p.triangle_factor * dbest) break;
+                        int xsrc = INT_TO_X(kp[k]), ysrc = INT_TO_Y(kp[k]);
+                        float d = dist(xsrc, ysrc);
+                        if (d < dbest) {
+                            dbest = d;
+                            xbest = xsrc;
+                            ybest = ysrc;
+                        }
+                    }
+                }
+                if (do_rs_or_spatial && p.spatial) {
+                    int symin = ybest - p.spatial, symax = ybest + p.spatial + 1;
+                    int sxmin = xbest - p.spatial, sxmax = xbest + p.spatial + 1;
+                    int xb0 = xbest, yb0 = ybest;
+                    if (symin < 0) symin = 0;
+                    else if (symax > b0_eh) symax = b0_eh;
+                    if (sxmin < 0) sxmin = 0;
+                    else if (sxmax > b0_ew) sxmax = b0_ew;
+                    if (symax > b0_eh) symax = b0_eh;   // only differs where the reference reads out of bounds
+                    if (sxmax > b0_ew) sxmax = b0_ew;
+                    for (int ys = symin; ys < symax; ys++)
+                        for (int xs = sxmin; xs < sxmax; xs++) {
+                            if (xs != xb0 || ys != yb0) {
+                                float d = dist(xs, ys);
+                                if (d < dbest) {
+                                    dbest = d;
+                                    xbest = xs;
+                                    ybest = ys;
+                                }
+                            }
+                        }
+                }
+                if (p.do_prop) {
+                    int ax_shift = x - qs, ay_shift = y - qs;
+                    int bx_shift = xbest - qs, by_shift = ybest - qs;
+                    int dy_min = 0, dy_max = sat_w_nopad, dx_min = 0, dx_max = sat_w_nopad;
+                    if (ax_shift + dx_min < 0) dx_min = -ax_shift;
+                    else if (ax_shift + dx_max > a_w) dx_max = a_w - ax_shift;
+                    if (ay_shift + dy_min < 0) dy_min = -ay_shift;
+                    else if (ay_shift + dy_max > a_h) dy_max = a_h - ay_shift;
+                    if (bx_shift + dx_min < 0) dx_min = -bx_shift;
+                    else if (bx_shift + dx_max > b_w) dx_max = b_w - bx_shift;
+                    if (by_shift + dy_min < 0) dy_min = -by_shift;
+                    else if (by_shift + dy_max > b_h) dy_max = b_h - by_shift;
+                    if (dx_min > 0 || dy_min > 0) {
+                        if (dx_min > 0)
+                            for (int dy = dy_min > 0 ? dy_min - 1 : dy_min; dy < dy_max; dy++) sat[(dy + 1) * sat_w + dx_min] = 0;
+                        if (dy_min > 0)
+                            for (int dx = dx_min; dx < dx_max; dx++) sat[(dy_min)*sat_w + dx + 1] = 0;
+                    }
+                    for (int dy = dy_min; dy < dy_max; dy++) {
+                        float row_sum = 0;
+                        for (int dx = dx_min; dx < dx_max; dx++) {
+                            float d0 = A(ay_shift + dy, ax_shift + dx, 0) - Bp(by_shift + dy, bx_shift + dx, 0);
+                            float d1 = A(ay_shift + dy, ax_shift + dx, 1) - Bp(by_shift + dy, bx_shift + dx, 1);
+                            float d2 = A(ay_shift + dy, ax_shift + dx, 2) - Bp(by_shift + dy, bx_shift + dx, 2);
+                            row_sum += d0 * d0 + d1 * d1 + d2 * d2;
+                            sat[(dy + 1) * sat_w + 1 + dx] = row_sum + sat[dy * sat_w + 1 + dx];
+                        }
+                    }
+                    int dy_max_patch = dy_max - 8 + 1, dx_max_patch = dx_max - 8 + 1;
+                    for (int dy = dy_min; dy < dy_max_patch; dy++) {
+                        const float *s1 = &sat[dy * sat_w], *s2 = &sat[(dy + 8) * sat_w];
+                        int ysrc = dy + by_shift;
+                        for (int dx = dx_min; dx < dx_max_patch; dx++) {
+                            int xsrc = dx + bx_shift;
+                            int dx2 = dx + 8;
+                            float dcur = s1[dx] - s1[dx2] - s2[dx] + s2[dx2];
+                            double *ap = ANN(ay_shift + dy, ax_shift + dx);
+                            if (dcur < ap[2]) {
+                                ap[0] = xsrc;
+                                ap[1] = ysrc;
+                                ap[2] = dcur;
+                            }
+                        }
+                    }
+                    if (dx_max < sat_w_nopad || dy_max < sat_w_nopad) {
+                        int dy_start, dy_end, dx_start, dx_end;
+                        if (reverse) {
+                            dx_start = 0; dx_end = qs;
+                            dy_start = 0; dy_end = qs;
+                            if (ax_shift + dx_start < 0) dx_start = -ax_shift;
+                            if (ay_shift + dy_start < 0) dy_start = -ay_shift;
+                        } else {
+                            dx_start = qs; dx_end = qs * 2;
+                            dy_start = qs; dy_end = qs * 2;
+                            if (ax_shift + dx_end > ann_w) dx_end = ann_w - ax_shift;
+                            if (ay_shift + dy_end > ann_h) dy_end = ann_h - ay_shift;
+                        }
+                        for (int dy = dy_start; dy < dy_end; dy++) {
+                            int ysrc = ybest + (ay_shift + dy) - y;
+                            if (ysrc < 0) ysrc = 0;
+                            else if (ysrc >= b0_eh) ysrc = b0_eh - 1;
+                            for (int dx = dx_start; dx < dx_end; dx++) {
+                                double *ap = ANN(ay_shift + dy, ax_shift + dx);
+                                if (ap[2] == DIST_INFINITY) {
+                                    int xsrc = xbest + (ax_shift + dx) - x;
+                                    if (xsrc < 0) xsrc = 0;
+                                    else if (xsrc >= b0_ew) xsrc = b0_ew - 1;
+                                    float d = patch_ssd(a, aw, ax_shift + dx, ay_shift + dy, b, b_w, xsrc, ysrc);
+                                    ap[2] = d;
+                                    ap[0] = xsrc;
+                                    ap[1] = ysrc;
+                                }
+                            }
+                        }
+                    }
+                } else {
+                    double *ap = ANN(y, x);
+                    ap[0] = xbest;
+                    ap[1] = ybest;
+                    ap[2] = dbest;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------- lookup (this build's parallel form) ---------------------------
+// Model of the GPU lookup, evaluated patch-parallel with double-buffered fields so that every
+// patch of a round reads the previous round's field only:
+//   round 0 (candidates): table cell candidate; then the k-coherence list of the current best
+//     with the triangle test (patchtable.h:3779-3784); then the (2*spatial+1)^2 window around
+//     the best (patchtable.h:3853-3885);
+//   rounds 1..R (propagation): for the 8 neighbours n = p + J*(dx,dy) the candidate
+//     NNF(n) - J*(dx,dy) if inside b; then the spatial window around the best. Jumps J run
+//     from large to 1 (jump flooding), followed by unit-jump rounds.
+// Candidates are tried in a fixed order and replace the best only when strictly closer, and
+// every distance is the exact SSD of patch_ssd, so NNF_DIST is the exact SSD of its match.
+static void jf_jumps(int rounds, std::vector<int> &J) {
+    // e.g. rounds=6 -> 8,4,2,1,1,1
+    J.clear();
+    int big = rounds >= 4 ? 8 : (rounds == 3 ? 4 : (rounds == 2 ? 2 : 1));
+    for (int j = big; j >= 1 && (int)J.size() < rounds; j >>= 1) J.push_back(j);
+    while ((int)J.size() < rounds) J.push_back(1);
+}
+
+static void lookup_parallel(const Table &T, const float *a, int ah, int aw, double *ann) {
+    const pt_params &p = T.p;
+    fvec a_wh;
+    descriptor(p, a, ah, aw, a_wh);
+    const int ann_h = ah - 7, ann_w = aw - 7;
+    const int bew = T.w - 7, beh = T.h - 7;
+    const int nd = T.ndims, AWp = aw + 7;
+    const float *b = T.b0.data();
+    const long N = (long)ann_h * ann_w;
+    std::vector<int32_t> cx(N), cy(N), nx(N), ny(N);
+    fvec cd(N), ndist(N);
+    auto spatial = [&](int x, int y, int &xb, int &yb, float &db) {
+        if (!p.spatial) return;
+        int symin = yb - p.spatial, symax = yb + p.spatial + 1;
+        int sxmin = xb - p.spatial, sxmax = xb + p.spatial + 1;
+        int xb0 = xb, yb0 = yb;
+        if (symin < 0) symin = 0;
+        else if (symax > beh) symax = beh;
+        if (sxmin < 0) sxmin = 0;
+        else if (sxmax > bew) sxmax = bew;
+        if (symax > beh) symax = beh;
+        if (sxmax > bew) sxmax = bew;
+        for (int ys = symin; ys < symax; ys++)
+            for (int xs = sxmin; xs < sxmax; xs++)
+                if (xs != xb0 || ys != yb0) {
+                    float d = patch_ssd(a, aw, x, y, b, T.w, xs, ys);
+                    if (d < db) {
+                        db = d;
+                        xb = xs;
+                        yb = ys;
+                    }
+                }
+    };
+    for (int y = 0; y < ann_h; y++)
+        for (int x = 0; x < ann_w; x++) {
+            const float *whp = &a_wh[((size_t)(y + 7) * AWp + x + 7) * nd];
+            int v = T.table[T.cell_of(whp)];
+            int xb = INT_TO_X(v) - 7, yb = INT_TO_Y(v) - 7;
+            float db = patch_ssd(a, aw, x, y, b, T.w, xb, yb);
+            if (p.kcoherence > 0) {
+                const int32_t *kp = &T.kpos[((size_t)yb * bew + xb) * T.K];
+                const float *kd = &T.kdist[((size_t)yb * bew + xb) * T.K];
+                for (int k = 0; k < p.kcoherence; k++) {
+                    if (kd[k] >= p.triangle_factor * db) break;
+                    int xs = INT_TO_X(kp[k]), ys = INT_TO_Y(kp[k]);
+                    float d = patch_ssd(a, aw, x, y, b, T.w, xs, ys);
+                    if (d < db) {
+                        db = d;
+                        xb = xs;
+                        yb = ys;
+                    }
+                }
+            }
+            spatial(x, y, xb, yb, db);
+            long i = (long)y * ann_w + x;
+            cx[i] = xb;
+            cy[i] = yb;
+            cd[i] = db;
+        }
+    std::vector<int> J;
+    jf_jumps(p.jf_rounds, J);
+    static const int DX[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+    static const int DY[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+    for (size_t r = 0; r < J.size(); r++) {
+        int j = J[r];
+        for (int y = 0; y < ann_h; y++)
+            for (int x = 0; x < ann_w; x++) {
+                long i = (long)y * ann_w + x;
+                int xb = cx[i], yb = cy[i];
+                float db = cd[i];
+                for (int t = 0; t < 8; t++) {
+                    int qx = x + j * DX[t], qy = y + j * DY[t];
+                    if (qx < 0 || qy < 0 || qx >= ann_w || qy >= ann_h) continue;
+                    long qi = (long)qy * ann_w + qx;
+                    int xs = cx[qi] - j * DX[t], ys = cy[qi] - j * DY[t];
+                    if (xs < 0 || ys < 0 || xs >= bew || ys >= beh) continue;
+                    if (xs == xb && ys == yb) continue;
+                    float d = patch_ssd(a, aw, x, y, b, T.w, xs, ys);
+                    if (d < db) {
+                        db = d;
+                        xb = xs;
+                        yb = ys;
+                    }
+                }
+                spatial(x, y, xb, yb, db);
+                nx[i] = xb;
+                ny[i] = yb;
+                ndist[i] = db;
+            }
+        cx.swap(nx);
+        cy.swap(ny);
+        cd.swap(ndist);
+    }
+    for (long i = 0; i < N; i++) {
+        ann[i * 3 + 0] = cx[i];
+        ann[i * 3 + 1] = cy[i];
+        ann[i * 3 + 2] = cd[i];
+    }
+}
+
+}  // namespace orc
+
+// ============================================================================================
+// C ABI of the oracle (loaded by tests with ctypes).
+// ============================================================================================
+using namespace orc;
+
+extern "C" {
+
+int orc_glibc_rand(uint32_t seed, int n, int32_t *out) {
+    GlibcRand r(seed);
+    for (int i = 0; i < n; i++) out[i] = r.next();
+    return 0;
+}
+
+int orc_random_shuffle(int n, int32_t *a) {
+    GlibcRand r(1);
+    random_shuffle(a, n, r);
+    return 0;
+}
+
+int orc_rgb2yuv(const float *in, int h, int w, float *out) {
+    rgb2yuv(in, h, w, out);
+    return 0;
+}
+
+int orc_wh_basis(int w, int *vecs_out, int *alphas_out) {
+    std::vector<std::vector<int>> vecs, alphas;
+    wh_basis(w, vecs, alphas);
+    int k = (int)alphas[0].size();
+    for (int i = 0; i < w; i++)
+        for (int j = 0; j < w; j++) vecs_out[i * w + j] = vecs[i][j];
+    for (int i = 0; i < w; i++)
+        for (int j = 0; j < k; j++) alphas_out[i * k + j] = alphas[i][j];
+    return 0;
+}
+
+int orc_snake_order(int w, int *rows, int *cols) {
+    auto L = snake_order(w);
+    for (size_t i = 0; i < L.size(); i++) {
+        rows[i] = L[i].first;
+        cols[i] = L[i].second;
+    }
+    return 0;
+}
+
+int orc_gck_recurrences(int w, int *axis, int *delta, int *sign) {
+    auto R = gck_recurrences(w);
+    for (size_t i = 0; i < R.size(); i++) {
+        axis[i] = R[i].axis;
+        delta[i] = R[i].delta;
+        sign[i] = R[i].sign;
+    }
+    return (int)R.size();
+}
+
+int orc_gck(const float *in, int h, int w, int n, int nchroma, float *out) {
+    gck(in, h, w, n, nchroma, out);
+    return 0;
+}
+
+int orc_descriptor(const pt_params *p, const float *img, int h, int w, float *out) {
+    fvec wh;
+    descriptor(*p, img, h, w, wh);
+    std::copy(wh.begin(), wh.end(), out);
+    return 0;
+}
+
+float orc_patch_ssd(const float *a, int aw, int ax, int ay, const float *b, int bw, int bx, int by) {
+    return patch_ssd(a, aw, ax, ay, b, bw, bx, by);
+}
+
+int orc_scale_slices_to_limit(int n, int *ns, double limit) {
+    std::vector<int> v(ns, ns + n);
+    v = scale_slices_to_limit(v, limit);
+    for (int i = 0; i < n; i++) ns[i] = v[i];
+    return 0;
+}
+
+int orc_partition(const float *vals, int n, int nslices, float min_val, float max_val, float *centers, int32_t *to_bin) {
+    Partition1D P = make_partition(std::vector<float>(vals, vals + n), nslices, min_val, max_val);
+    std::copy(P.centers.begin(), P.centers.end(), centers);
+    std::copy(P.to_bin.begin(), P.to_bin.end(), to_bin);
+    return 0;
+}
+
+void *orc_build(const pt_params *p, const float *b, int h, int w) { return build(*p, b, h, w); }
+
+void orc_free(void *t) { delete (Table *)t; }
+
+// The reference's table keeps a pointer to the caller's parameters (patchtable.h:2602), so
+// parameters changed after construction apply to later lookups; this mirrors that.
+int orc_set_params(void *t, const pt_params *p) {
+    Table *T = (Table *)t;
+    int g = T->p.grid_ndims;
+    T->p = *p;
+    T->p.grid_ndims = g;
+    return 0;
+}
+
+int orc_info(void *t, int *G, int *nslices, int64_t *ncells, int *add_count, int *K) {
+    Table *T = (Table *)t;
+    *G = T->G;
+    for (int i = 0; i < T->G; i++) nslices[i] = T->nslices[i];
+    *ncells = T->ncells;
+    *add_count = T->table_add_count;
+    *K = T->K;
+    return 0;
+}
+
+int orc_get_descriptor(void *t, float *out) {
+    Table *T = (Table *)t;
+    std::copy(T->wh0.begin(), T->wh0.end(), out);
+    return 0;
+}
+
+int orc_get_partition(void *t, int dim, float *mn, float *mx, float *centers, int32_t *to_bin) {
+    Table *T = (Table *)t;
+    const Partition1D &P = T->part[dim];
+    *mn = P.min_val;
+    *mx = P.max_val;
+    std::copy(P.centers.begin(), P.centers.end(), centers);
+    std::copy(P.to_bin.begin(), P.to_bin.end(), to_bin);
+    return 0;
+}
+
+int orc_get_table(void *t, int32_t *out) {
+    Table *T = (Table *)t;
+    std::copy(T->table.begin(), T->table.end(), out);
+    return 0;
+}
+
+int orc_get_populated(void *t, int32_t *out) {
+    Table *T = (Table *)t;
+    std::copy(T->table_pop.begin(), T->table_pop.end(), out);
+    return 0;
+}
+
+int orc_get_kcoherence(void *t, int32_t *pos, float *dist) {
+    Table *T = (Table *)t;
+    std::copy(T->kpos.begin(), T->kpos.end(), pos);
+    std::copy(T->kdist.begin(), T->kdist.end(), dist);
+    return 0;
+}
+
+// Re-run a distance transform on a given populated table (tests of the DT variants).
+// algo: 0 = prop BFS, 1 = prop separable, 2 = downsample, 3 = euclidean.
+int orc_dt(int G, const int *nslices, int32_t *table, int algo, int dt_iters) {
+    Table T;
+    T.G = G;
+    T.nslices.assign(nslices, nslices + G);
+    T.stride.assign(G, 0);
+    int64_t acc = 1;
+    for (int i = G - 1; i >= 0; i--) {
+        T.stride[i] = acc;
+        acc *= nslices[i];
+    }
+    T.ncells = acc;
+    T.table.assign(table, table + acc);
+    if (algo == 0) dt_prop_bfs(T, dt_iters);
+    else if (algo == 1) dt_prop_separable(T, dt_iters > 0 ? dt_iters + 1 : -1);
+    else if (algo == 2) dt_downsample(T);
+    else if (algo == 3) dt_euclidean(T);
+    else return -1;
+    std::copy(T.table.begin(), T.table.end(), table);
+    return 0;
+}
+
+int orc_lookup_raster(void *t, const float *a, int h, int w, double *ann) {
+    lookup_raster(*(Table *)t, a, h, w, ann);
+    return 0;
+}
+
+int orc_lookup_parallel(void *t, const float *a, int h, int w, double *ann) {
+    lookup_parallel(*(Table *)t, a, h, w, ann);
+    return 0;
+}
+
+// Mean of sqrt(max(SSD,0)) over the field (get_mean_dist, patchtable_main.cpp:90-101).
+double orc_mean_dist(const double *ann, long n) {
+    double s = 0;
+    for (long i = 0; i < n; i++) {
+        double d = ann[i * 3 + 2];
+        if (d < 0) d = 0;
+        s += sqrt(d);
+    }
+    return s / double(n);
+}
+
+// recompute_mean_dist<0> (patchtable_main.cpp:103-172): float64 SSD of every match, clamped
+// coordinates; optionally stores the recomputed SSD per patch in dist_out.
+double orc_recompute_mean_dist(const float *a, int ah, int aw, const float *b, int bh, int bw, const double *ann, double *dist_out) {
+    int ann_w = aw - 7, ann_h = ah - 7, bew = bw - 7, beh = bh - 7;
+    double sum = 0.0;
+    for (int ay = 0; ay < ann_h; ay++)
+        for (int ax = 0; ax < ann_w; ax++) {
+            const double *q = &ann[((size_t)ay * ann_w + ax) * 3];
+            int bx = (int)q[0], by = (int)q[1];
+            if (bx < 0) bx = 0;
+            else if (bx >= bew) bx = bew - 1;
+            if (by < 0) by = 0;
+            else if (by >= beh) by = beh - 1;
+            double dist = 0.0;
+            for (int dy = 0; dy < 8; dy++)
+                for (int dx = 0; dx < 8; dx++)
+                    for (int c = 0; c < 3; c++) {
+                        double delta = a[((size_t)(ay + dy) * aw + ax + dx) * 3 + c] - b[((size_t)(by + dy) * bw + bx + dx) * 3 + c];
+                        dist += delta * delta;
+                    }
+            if (dist_out) dist_out[(size_t)ay * ann_w + ax] = dist;
+            sum += sqrt(dist);
+        }
+    return sum / double(ann_w * ann_h);
+}
+
+}  // extern "C"

@@ -1,0 +1,229 @@
+"""MI355X-native PatchTable: Python host mirror of the reference's C++ API.
+
+    from patchtable_amd import PatchTableParams, PatchTable
+    p = PatchTableParams(); p.set_speed(1)
+    table = PatchTable(p, b)              # b: H x W x 3 float32 in [0,1]
+    ann = table.lookup(a)                 # (Ha-7) x (Wa-7) x 3 float64: NNF_X, NNF_Y, NNF_DIST
+
+Mirrors PatchTableParams (patchtable/patchtable_params.h:23-161, patchtable.cpp:26-387) and
+PatchTable<> (patchtable/patchtable.h:4359-4491). All work runs in the HIP library
+libpatchtable_hip.so through its C ABI (include/patchtable_amd.h); there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import PatchTableError, PtParams, check, load
+
+NNF_X = 0
+NNF_Y = 1
+NNF_DIST = 2
+
+__all__ = ["PatchTableParams", "PatchTable", "PatchTableError", "NNF_X", "NNF_Y", "NNF_DIST",
+           "synth_pair"]
+
+
+class PatchTableParams:
+    """Parameters with the reference's defaults, speed presets and switches.
+
+    PatchTableParams()            -> set_defaults() (patchtable.cpp:375-377)
+    PatchTableParams(argv)        -> defaults then set_from_switches(argc, argv); argv[0] is the
+                                     program name and is skipped as in parse_switches
+    p.set_speed(i)                -> patchtable.cpp:138-150
+    p.set_from_switches("...")    -> patchtable.cpp:152-161 (first token dropped, like the
+                                     reference)
+    Attributes read and write the pt_params fields by their reference names.
+    """
+
+    def __init__(self, argv=None):
+        object.__setattr__(self, "_p", PtParams())
+        check(load().pt_params_default(ctypes.byref(self._p)))
+        if argv is not None:
+            self.set_from_switches(" ".join(str(a) for a in argv))
+
+    def set_speed(self, i):
+        check(load().pt_params_set_speed(ctypes.byref(self._p), int(i)))
+
+    def set_from_switches(self, s):
+        check(load().pt_params_set_switches(ctypes.byref(self._p), s.encode()))
+
+    def __getattr__(self, name):
+        p = object.__getattribute__(self, "_p")
+        v = getattr(p, name)
+        if name in ("nslices", "filter_dims"):
+            n = p.nslices_count if name == "nslices" else p.filter_dims_count
+            return [v[i] for i in range(n)]
+        if isinstance(v, bytes):
+            return v.decode()
+        return v
+
+    def __setattr__(self, name, value):
+        p = self._p
+        if name in ("nslices", "filter_dims"):
+            arr = getattr(p, name)
+            for i, x in enumerate(value):
+                arr[i] = int(x)
+            setattr(p, name + "_count", len(value))
+            if name == "nslices":
+                p.ndims = len(value)
+            return
+        if name in ("load_filename", "save_filename"):
+            value = value.encode()
+        setattr(p, name, value)
+
+    @property
+    def raw(self):
+        return self._p
+
+
+class PatchTable:
+    """PatchTable<float, float, int32_t> (patchtable.h:4359-4491) backed by HIP kernels.
+
+    b may be a numpy array (host) or a torch CUDA tensor (device, HWC float32 contiguous).
+    """
+
+    def __init__(self, p, b, allowed_patches=None):
+        self.p = p
+        self._t = ctypes.c_void_p()
+        allowed = None
+        if allowed_patches is not None:
+            self._allowed = np.ascontiguousarray(allowed_patches, dtype=np.int32)
+            allowed = self._allowed.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        if _is_device_tensor(b):
+            h, w, c = (int(s) for s in b.shape)
+            check(load().pt_table_create_dev(ctypes.byref(p.raw), ctypes.c_void_p(b.data_ptr()),
+                                             h, w, c, allowed, ctypes.byref(self._t)))
+        else:
+            b = np.ascontiguousarray(b, dtype=np.float32)
+            h, w, c = b.shape
+            check(load().pt_table_create(ctypes.byref(p.raw), b.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                         h, w, c, allowed, ctypes.byref(self._t)))
+        self.b_shape = (h, w, c)
+
+    @classmethod
+    def load(cls, p, filename):
+        self = cls.__new__(cls)
+        self.p = p
+        self._t = ctypes.c_void_p()
+        check(load().pt_table_load(ctypes.byref(p.raw), filename.encode(), ctypes.byref(self._t)))
+        return self
+
+    def close(self):
+        if getattr(self, "_t", None) and self._t.value:
+            load().pt_table_destroy(self._t)
+            self._t = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def lookup(self, a, ann=None, ann_prev=None, coherence_temporal_sv=None):
+        """Returns the NNF; also stores the lookup's wall time in self.last_seconds.
+
+        Host arrays in, host array out; with a torch CUDA tensor `a`, the field is returned as a
+        torch CUDA float64 tensor (and `ann`, if given, must be one of the right shape)."""
+        secs = ctypes.c_double()
+        if _is_device_tensor(a):
+            import torch
+            h, w, c = (int(s) for s in a.shape)
+            if ann is None:
+                ann = torch.empty((h - 7, w - 7, 3), dtype=torch.float64, device=a.device)
+            prev = ctypes.c_void_p(ann_prev.data_ptr()) if ann_prev is not None else None
+            sv = ctypes.c_void_p(coherence_temporal_sv.data_ptr()) if coherence_temporal_sv is not None else None
+            check(load().pt_table_lookup_dev(self._t, ctypes.c_void_p(a.data_ptr()), h, w, c,
+                                             ctypes.c_void_p(ann.data_ptr()), prev, sv, ctypes.byref(secs)))
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            h, w, c = a.shape
+            if ann is None:
+                ann = np.empty((h - 7, w - 7, 3), np.float64)
+            prev = None
+            if ann_prev is not None:
+                self._prev = np.ascontiguousarray(ann_prev, dtype=np.float64)
+                prev = self._prev.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            sv = None
+            if coherence_temporal_sv is not None:
+                self._sv = np.ascontiguousarray(coherence_temporal_sv, dtype=np.float32)
+                sv = self._sv.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            check(load().pt_table_lookup(self._t, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), h, w, c,
+                                         ann.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), prev, sv,
+                                         ctypes.byref(secs)))
+        self.last_seconds = secs.value
+        return ann
+
+    def save(self, filename):
+        check(load().pt_table_save(self._t, filename.encode()))
+
+    # ---- introspection (tests) ----
+    def info(self):
+        G = ctypes.c_int()
+        ns = (ctypes.c_int * _ffi.PT_MAX_DIMS)()
+        nc = ctypes.c_int64()
+        add = ctypes.c_int()
+        K = ctypes.c_int()
+        check(load().pt_table_info(self._t, ctypes.byref(G), ns, ctypes.byref(nc), ctypes.byref(add), ctypes.byref(K)))
+        return dict(grid_ndims=G.value, nslices=[ns[i] for i in range(G.value)], ncells=nc.value,
+                    table_add_count=add.value, kcoherence=K.value)
+
+    def descriptor(self):
+        h, w, _ = self.b_shape
+        out = np.empty((h + 7, w + 7, self.p.ndims), np.float32)
+        check(load().pt_table_get_descriptor(self._t, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def query_descriptor(self, h, w):
+        out = np.empty((h + 7, w + 7, self.p.ndims), np.float32)
+        check(load().pt_table_get_query_descriptor(self._t, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def partition(self, dim):
+        info = self.info()
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        centers = np.empty(info["nslices"][dim], np.float32)
+        to_bin = np.empty(16384, np.int32)
+        check(load().pt_table_get_partition(self._t, dim, ctypes.byref(mn), ctypes.byref(mx),
+                                            centers.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                            to_bin.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return mn.value, mx.value, centers, to_bin
+
+    def cells(self):
+        out = np.empty(self.info()["ncells"], np.int32)
+        check(load().pt_table_get_cells(self._t, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
+    def populated(self):
+        out = np.empty(self.info()["ncells"], np.int32)
+        check(load().pt_table_get_populated(self._t, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
+    def kcoherence(self):
+        h, w, _ = self.b_shape
+        K = self.info()["kcoherence"]
+        pos = np.empty((h - 7, w - 7, K), np.int32)
+        dist = np.empty((h - 7, w - 7, K), np.float32)
+        check(load().pt_table_get_kcoherence(self._t, pos.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                             dist.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return pos, dist
+
+    def stage_ms(self):
+        out = (ctypes.c_double * 10)()
+        check(load().pt_table_stage_ms(self._t, out))
+        names = ["gck", "quantiser", "populate", "dt", "kcoherence", "build_total",
+                 "query_gck", "query_candidates", "query_propagation", "query_total"]
+        return dict(zip(names, list(out)))
+
+
+def _is_device_tensor(x):
+    return hasattr(x, "is_cuda") and bool(getattr(x, "is_cuda"))
+
+
+def synth_pair(seed, h, w):
+    """Seeded synthetic (a, b) pair, each h x w x 3 float32 (see include/patchtable_amd.h)."""
+    a = np.empty((h, w, 3), np.float32)
+    b = np.empty((h, w, 3), np.float32)
+    check(load().pt_synth_pair(int(seed), int(h), int(w), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               b.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return a, b

@@ -1,0 +1,285 @@
+// Distance-transform table fill on gfx950 (default dt_algo "prop", patchtable.h:2081-2165).
+//
+// The reference fills empty cells by a level-synchronous multi-source BFS whose edge labels
+// only allow paths that move along dim 0 first, then dim 1, ... (PropEdge/PropElement,
+// patchtable.h:1447-1468; next_label built at :2100-2120), where the first claim in list order
+// wins. The initial list is the filled cells in increasing linear index, and the list order of
+// every later level is lexicographic in (source index, edge sequence) of the claiming path.
+// Induction over the levels shows every empty cell c is claimed at level d(c) = its L1 distance
+// (in cell units) to the nearest filled cell, by the nearest filled cell of smallest linear
+// index: every prefix of that source's dimension-ordered path to c is claimed by the same
+// source through that path, and any other claimant at that level comes from a source of larger
+// index. The fill is therefore a separable L1 distance transform over the cell lattice that
+// minimises the pair (distance, source index) -- computed here as one forward and one backward
+// sweep along each dimension. dt_iters > 0 (the reference stops after dt_iters+1 levels) keeps
+// cells farther than dt_iters+1 empty, to be filled by fill_zeros (patchtable.h:2063-2079).
+//
+// Keys are 64-bit: distance in the high word, source cell index in the low word; empty = ~0.
+#include "pt_device.h"
+#include "pt_kernels.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+constexpr uint64_t KEY_INF = ~0ull;
+constexpr uint64_t KEY_STEP = 1ull << 32;
+
+__global__ void k_dt_init(const int32_t *__restrict__ table, int64_t n, uint64_t *__restrict__ key) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    key[c] = table[c] != PT_TABLE_UNUSED ? (uint64_t)c : KEY_INF;
+}
+
+// One line per lane along a dimension of size len and stride st (in cells).
+__global__ void k_dt_line(uint64_t *__restrict__ key, int64_t nlines, int len, int64_t st) {
+    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlines) return;
+    int64_t o = l / st, i = l - o * st;
+    uint64_t *p = key + o * (int64_t)len * st + i;
+    uint64_t run = KEY_INF;
+    for (int t = 0; t < len; t++) {
+        uint64_t k = p[(int64_t)t * st];
+        if (run != KEY_INF) run += KEY_STEP;
+        if (k < run) run = k;
+        p[(int64_t)t * st] = run;
+    }
+    run = KEY_INF;
+    for (int t = len - 1; t >= 0; t--) {
+        uint64_t k = p[(int64_t)t * st];
+        if (run != KEY_INF) run += KEY_STEP;
+        if (k < run) run = k;
+        p[(int64_t)t * st] = run;
+    }
+}
+
+// table[c] = value of the winning source, or `fill` for cells beyond max_dist / unreached.
+// Source cells keep their own value, so reading table[src] while writing other cells is safe.
+__global__ void k_dt_final(const uint64_t *__restrict__ key, int64_t n, int32_t *__restrict__ table, int64_t max_dist,
+                           int32_t fill) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    uint64_t k = key[c];
+    int32_t v;
+    if (k == KEY_INF || (max_dist >= 0 && (int64_t)(k >> 32) > max_dist)) v = fill;
+    else v = table[(int64_t)(k & 0xffffffffull)];
+    if (v != table[c]) table[c] = v;
+}
+
+// fill_zeros (patchtable.h:2063-2079): every empty cell takes `fill`.
+__global__ void k_fill_unused(int32_t *__restrict__ table, int64_t n, int32_t fill) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n && table[c] == PT_TABLE_UNUSED) table[c] = fill;
+}
+
+// ---- DOWNSAMPLE (do_dt_downsample, patchtable.h:2339-2361; dt_downsample.h) ----------------
+struct LevelShape {
+    int G;
+    int hs[PT_MAX_DIMS];      // fine sizes
+    int ls[PT_MAX_DIMS];      // coarse sizes ((hs+1)/2)
+    int64_t hst[PT_MAX_DIMS]; // fine strides
+    int64_t lst[PT_MAX_DIMS]; // coarse strides
+};
+
+// A coarse cell takes the first filled child (no TABLE_HI_MASK bit) in nested-loop order with
+// dim 0 outermost, else TABLE_UNUSED.
+__global__ void k_downsample(const int32_t *__restrict__ hi, int32_t *__restrict__ lo, int64_t nlo, LevelShape S) {
+    int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nlo) return;
+    int64_t base = 0;
+    int cnt[PT_MAX_DIMS];
+    int64_t nchild = 1;
+    for (int i = 0; i < S.G; i++) {
+        int lc = (int)((li / S.lst[i]) % S.ls[i]);
+        int c0 = 2 * lc;
+        int c1 = 2 * (lc + 1) < S.hs[i] ? 2 * (lc + 1) : S.hs[i];
+        cnt[i] = c1 - c0;
+        base += (int64_t)c0 * S.hst[i];
+        nchild *= cnt[i];
+    }
+    int32_t out = PT_TABLE_UNUSED;
+    for (int64_t c = 0; c < nchild; c++) {
+        int64_t rem = c, off = 0;
+        for (int i = S.G - 1; i >= 0; i--) {
+            int ci = (int)(rem % cnt[i]);
+            rem /= cnt[i];
+            off += (int64_t)ci * S.hst[i];
+        }
+        int32_t v = hi[base + off];
+        if (!(v & (1 << 30)) && v != PT_TABLE_UNUSED) {
+            out = v;
+            break;
+        }
+    }
+    lo[li] = out;
+}
+
+// An empty fine cell copies its coarse parent.
+__global__ void k_upsample(int32_t *__restrict__ hi, const int32_t *__restrict__ lo, int64_t nhi, LevelShape S) {
+    int64_t hidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (hidx >= nhi) return;
+    if (hi[hidx] != PT_TABLE_UNUSED) return;
+    int64_t lidx = 0;
+    for (int i = 0; i < S.G; i++) lidx += (int64_t)(((hidx / S.hst[i]) % S.hs[i]) / 2) * S.lst[i];
+    hi[hidx] = lo[lidx];
+}
+
+// ---- EUCLIDEAN (euclidean_dt + do_dt_euclidean, patchtable.h:4244-4352) ---------------------
+// Felzenszwalb-Huttenlocher lower envelope along one dimension per pass, dims in order
+// 0..G-1, in the reference's float arithmetic; the carried table value is updated in place
+// exactly as the reference's index_array (idx[q] = idx[v[k]]). Per-line work arrays live in
+// global memory at the positions of the line's cells.
+__global__ void k_eucl_line(int32_t *__restrict__ idx, float *__restrict__ f, float *__restrict__ dist,
+                            int32_t *__restrict__ vb, float *__restrict__ zb, float *__restrict__ zlast, int64_t nlines,
+                            int n, int64_t st) {
+    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlines) return;
+    int64_t o = l / st, i = l - o * st;
+    const int64_t base = o * (int64_t)n * st + i;
+    const float INF = 1E20;
+#define F_(q) f[base + (int64_t)(q) * st]
+#define V_(k) vb[base + (int64_t)(k) * st]
+#define Z_(k) (*((k) < n ? &zb[base + (int64_t)(k) * st] : &zlast[l]))
+    int k = 0;
+    V_(0) = 0;
+    Z_(0) = -INF;
+    Z_(1) = +INF;
+    for (int q = 1; q <= n - 1; q++) {
+        int vk = V_(k);
+        float s = ((F_(q) + q * q) - (F_(vk) + vk * vk)) / (2 * q - 2 * vk);
+        while (s <= Z_(k)) {
+            k--;
+            vk = V_(k);
+            s = ((F_(q) + q * q) - (F_(vk) + vk * vk)) / (2 * q - 2 * vk);
+        }
+        k++;
+        V_(k) = q;
+        Z_(k) = s;
+        Z_(k + 1) = +INF;
+    }
+    k = 0;
+    for (int q = 0; q <= n - 1; q++) {
+        while (Z_(k + 1) < q) k++;
+        int vk = V_(k);
+        dist[base + (int64_t)q * st] = (q - vk) * (q - vk) + F_(vk);
+        idx[base + (int64_t)q * st] = idx[base + (int64_t)vk * st];
+    }
+    for (int q = 0; q <= n - 1; q++) F_(q) = dist[base + (int64_t)q * st];
+#undef F_
+#undef V_
+#undef Z_
+}
+
+__global__ void k_eucl_init(const int32_t *__restrict__ table, int64_t n, float *__restrict__ f) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) f[c] = table[c] == PT_TABLE_UNUSED ? (float)1E20 : 0.0f;
+}
+
+}  // namespace
+
+int pt_fill_unused_launch(int32_t *table, int64_t ncells, int32_t fill, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_unused, dim3(pt_blocks(ncells, 256)), dim3(256), 0, s, table, ncells, fill);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_dt_downsample_launch(int32_t *table, int G, const int *nslices, PtScratch &sc, hipStream_t s) {
+    std::vector<std::vector<int>> sizes(1, std::vector<int>(nslices, nslices + G));
+    auto all_one = [](const std::vector<int> &v) {
+        for (int x : v)
+            if (x != 1) return false;
+        return true;
+    };
+    while (!all_one(sizes.back())) {
+        std::vector<int> ls(G);
+        for (int i = 0; i < G; i++) ls[i] = (sizes.back()[i] + 1) / 2;
+        sizes.push_back(ls);
+    }
+    const int L = (int)sizes.size();
+    std::vector<int32_t *> lvl(L, nullptr);
+    std::vector<int64_t> cnt(L);
+    lvl[0] = table;
+    for (int k = 0; k < L; k++) {
+        int64_t c = 1;
+        for (int v : sizes[k]) c *= v;
+        cnt[k] = c;
+        if (k > 0) {
+            lvl[k] = (int32_t *)sc.get(50 + k, (size_t)c * 4);
+            if (!lvl[k]) return pt_fail(PT_E_NOMEM, "downsample pyramid allocation failed");
+        }
+    }
+    auto shape = [&](int k) {   // fine level k-1 -> coarse level k
+        LevelShape S;
+        S.G = G;
+        int64_t ah = 1, al = 1;
+        for (int i = G - 1; i >= 0; i--) {
+            S.hs[i] = sizes[k - 1][i];
+            S.ls[i] = sizes[k][i];
+            S.hst[i] = ah;
+            S.lst[i] = al;
+            ah *= S.hs[i];
+            al *= S.ls[i];
+        }
+        return S;
+    };
+    for (int k = 1; k < L; k++)
+        hipLaunchKernelGGL(k_downsample, dim3(pt_blocks(cnt[k], 256)), dim3(256), 0, s, lvl[k - 1], lvl[k], cnt[k], shape(k));
+    for (int k = L - 1; k >= 1; k--)
+        hipLaunchKernelGGL(k_upsample, dim3(pt_blocks(cnt[k - 1], 256)), dim3(256), 0, s, lvl[k - 1], lvl[k], cnt[k - 1],
+                           shape(k));
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_dt_euclidean_launch(int32_t *table, int G, const int *nslices, int64_t ncells, float *f, hipStream_t s) {
+    // f: ncells floats (the reference's table_dist); work arrays below.
+    float *dist = nullptr, *zb = nullptr, *zlast = nullptr;
+    int32_t *vb = nullptr;
+    int64_t maxlines = 0;
+    for (int d = 0; d < G; d++) maxlines = std::max<int64_t>(maxlines, ncells / nslices[d]);
+    if (hipMalloc(&dist, ncells * 4) != hipSuccess || hipMalloc(&zb, ncells * 4) != hipSuccess ||
+        hipMalloc(&vb, ncells * 4) != hipSuccess || hipMalloc(&zlast, maxlines * 4) != hipSuccess) {
+        (void)hipFree(dist);
+        (void)hipFree(zb);
+        (void)hipFree(vb);
+        (void)hipFree(zlast);
+        return pt_fail(PT_E_NOMEM, "euclidean dt work arrays allocation failed");
+    }
+    hipLaunchKernelGGL(k_eucl_init, dim3(pt_blocks(ncells, 256)), dim3(256), 0, s, table, ncells, f);
+    std::vector<int64_t> stride(G);
+    int64_t acc = 1;
+    for (int d = G - 1; d >= 0; d--) {
+        stride[d] = acc;
+        acc *= nslices[d];
+    }
+    for (int d = 0; d < G; d++) {
+        int64_t nlines = ncells / nslices[d];
+        hipLaunchKernelGGL(k_eucl_line, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, table, f, dist, vb, zb, zlast,
+                           nlines, nslices[d], stride[d]);
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    (void)hipFree(dist);
+    (void)hipFree(zb);
+    (void)hipFree(vb);
+    (void)hipFree(zlast);
+    if (e != hipSuccess) return pt_failf(PT_E_HIP, "euclidean dt failed: %s", hipGetErrorString(e));
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
+                      uint64_t *key, hipStream_t s) {
+    const int TB = 256;
+    hipLaunchKernelGGL(k_dt_init, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, table, ncells, key);
+    int64_t st = 1;
+    for (int d = G - 1; d >= 0; d--) {
+        int64_t nlines = ncells / nslices[d];
+        hipLaunchKernelGGL(k_dt_line, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines, nslices[d], st);
+        st *= nslices[d];
+    }
+    int64_t max_dist = dt_iters > 0 ? (int64_t)dt_iters + 1 : -1;
+    hipLaunchKernelGGL(k_dt_final, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, key, ncells, table, max_dist, fill);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}

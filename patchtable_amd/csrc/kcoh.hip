@@ -1,0 +1,227 @@
+// k-coherence lists on gfx950 (replaces the FLANN kd-tree search of patchtable.h:2653-2881).
+//
+// For every database patch p the reference keeps its k nearest database patches in descriptor
+// space that lie at pixel distance >= kcoherence_min_dist, re-scored by exact RGB SSD and sorted
+// by it. FLANN's arithmetic is third-party and unpinned; this build finds candidates with a
+// data-parallel approximate kNN instead:
+//   1. every patch gets two 64-bit keys, the Morton interleave of its table bins and of its
+//      bins offset by half the slice count; patches are radix-sorted by each key (ties by
+//      raster index);
+//   2. the candidates of p are the patches within `window` sorted positions of p in either
+//      order; those at pixel distance^2 < min_dist^2 are dropped;
+//   3. the K candidates nearest in descriptor L2 (all dims, summed in index order; ties to the
+//      smaller raster index, duplicates removed) are kept in a register top-K list;
+//   4. they are re-scored by exact SSD and sorted by (SSD, XY_TO_INT) as the reference sorts its
+//      (dist, (index, valid)) triples (patchtable.h:2815-2835).
+// Lists shorter than K repeat their last entry (p itself with SSD 0 when empty).
+#include "pt_device.h"
+#include "pt_kernels.h"
+
+#include <hipcub/hipcub.hpp>
+
+namespace {
+
+struct KeyArgs {
+    int G;          // grid dims
+    int Gk;         // dims interleaved in the key
+    int B;          // bits per dim
+    int half[PT_MAX_DIMS];
+};
+
+__global__ void k_keys(const float *__restrict__ desc, size_t plane, int Wp, int bew, long N, DevQuant q, KeyArgs ka,
+                       uint64_t *__restrict__ k0, uint64_t *__restrict__ k1, int32_t *__restrict__ idx) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int px = (int)(i % bew), py = (int)(i / bew);
+    size_t off = (size_t)(py + 7) * Wp + px + 7;
+    uint32_t bins[PT_MAX_DIMS];
+    for (int d = 0; d < ka.Gk; d++)
+        bins[d] = (uint32_t)pt_get_bin(desc[d * plane + off], q.min_val[d], q.range[d], q.to_bin + (size_t)d * PT_NMAP);
+    uint64_t a = 0, b = 0;
+    for (int bit = ka.B - 1; bit >= 0; bit--)
+        for (int d = 0; d < ka.Gk; d++) {
+            a = (a << 1) | ((bins[d] >> bit) & 1u);
+            b = (b << 1) | (((bins[d] + (uint32_t)ka.half[d]) >> bit) & 1u);
+        }
+    k0[i] = a;
+    k1[i] = b;
+    idx[i] = (int32_t)i;
+}
+
+__global__ void k_positions(const int32_t *__restrict__ perm, long N, int32_t *__restrict__ pos) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < N) pos[perm[r]] = (int32_t)r;
+}
+
+__global__ void k_gather_desc(const float *__restrict__ desc, size_t plane, int Wp, int bew, int nd,
+                              const int32_t *__restrict__ perm, long N, float *__restrict__ ds) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    int i = perm[r];
+    int px = i % bew, py = i / bew;
+    size_t off = (size_t)(py + 7) * Wp + px + 7;
+    for (int c = 0; c < nd; c++) ds[r * nd + c] = desc[c * plane + off];
+}
+
+template <int KMAX>
+__global__ void k_select(const int32_t *__restrict__ perm0, const int32_t *__restrict__ perm1,
+                         const int32_t *__restrict__ pos1, const float *__restrict__ ds0, const float *__restrict__ ds1,
+                         long N, int bew, int nd, int K, int W, int md2, const float *__restrict__ b0, int bw,
+                         int32_t *__restrict__ kpos, float *__restrict__ kdist) {
+    long r0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r0 >= N) return;
+    const int i = perm0[r0];
+    const int px = i % bew, py = i / bew;
+    float own[PT_MAX_DIMS];
+    for (int c = 0; c < nd; c++) own[c] = ds0[r0 * nd + c];
+    float bd[KMAX];
+    int bq[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        bd[k] = 0.f;
+        bq[k] = 0x7fffffff;
+    }
+    int cnt = 0;
+    float wd = 0.f;   // K-th entry once the list is full
+    int wq = 0x7fffffff;
+    for (int o = 0; o < 2; o++) {
+        const int32_t *perm = o == 0 ? perm0 : perm1;
+        const float *ds = o == 0 ? ds0 : ds1;
+        long r = o == 0 ? r0 : (long)pos1[i];
+        long t0 = r - W < 0 ? 0 : r - W, t1 = r + W >= N ? N - 1 : r + W;
+        for (long t = t0; t <= t1; t++) {
+            if (t == r) continue;
+            int qi = perm[t];
+            int qx = qi % bew, qy = qi / bew;
+            int dx = qx - px, dy = qy - py;
+            if (dx * dx + dy * dy < md2) continue;
+            float dd = 0.f;
+            for (int c = 0; c < nd; c++) {
+                float e = own[c] - ds[t * nd + c];
+                dd = dd + e * e;
+            }
+            // reject if not better than the current K-th entry
+            if (cnt == K && !(dd < wd || (dd == wd && qi < wq))) continue;
+            bool dup = false;
+#pragma unroll
+            for (int k = 0; k < KMAX; k++) dup |= (k < cnt && bq[k] == qi);
+            if (dup) continue;
+            // insert keeping (dd, qi) ascending
+            int n = cnt < K ? cnt + 1 : K;
+            float cd = dd;
+            int cq = qi;
+#pragma unroll
+            for (int k = 0; k < KMAX; k++) {
+                if (k < n) {
+                    bool empty = k >= cnt;
+                    bool less = empty || cd < bd[k] || (cd == bd[k] && cq < bq[k]);
+                    if (less) {
+                        float td = bd[k];
+                        int tq = bq[k];
+                        bd[k] = cd;
+                        bq[k] = cq;
+                        cd = td;
+                        cq = tq;
+                    }
+                }
+            }
+            cnt = n;
+#pragma unroll
+            for (int k = 0; k < KMAX; k++)
+                if (k == K - 1) {
+                    wd = bd[k];
+                    wq = bq[k];
+                }
+        }
+    }
+    // exact SSD re-rank
+    float ss[KMAX];
+    int xy[KMAX];
+    const float *pa = b0 + ((size_t)py * bw + px) * 3;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        if (k < cnt) {
+            int qx = bq[k] % bew, qy = bq[k] / bew;
+            ss[k] = pt_patch_ssd(pa, bw * 3, b0 + ((size_t)qy * bw + qx) * 3, bw * 3);
+            xy[k] = PT_XY_TO_INT(qx, qy);
+        }
+    }
+    for (int a = 1; a < cnt; a++) {   // insertion sort by (ssd, xy)
+        float s = ss[a];
+        int x = xy[a];
+        int b = a - 1;
+        while (b >= 0 && (ss[b] > s || (ss[b] == s && xy[b] > x))) {
+            ss[b + 1] = ss[b];
+            xy[b + 1] = xy[b];
+            b--;
+        }
+        ss[b + 1] = s;
+        xy[b + 1] = x;
+    }
+    int32_t *op = kpos + (size_t)i * K;
+    float *od = kdist + (size_t)i * K;
+    for (int k = 0; k < K; k++) {
+        if (cnt == 0) {
+            op[k] = PT_XY_TO_INT(px, py);
+            od[k] = 0.f;
+        } else {
+            int kk = k < cnt ? k : cnt - 1;
+            op[k] = xy[kk];
+            od[k] = ss[kk];
+        }
+    }
+}
+
+}  // namespace
+
+int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, const float *b0, int bw, int bh,
+                         const DevQuant &q, const int *nslices_host, int K, int window, int min_dist2,
+                         int32_t *kpos, float *kdist, PtScratch &sc, hipStream_t s) {
+    const int bew = bw - 7, beh = bh - 7;
+    const long N = (long)bew * beh;
+    if (K <= 0 || N <= 0) return PT_OK;
+    if (K > 32) return pt_failf(PT_E_UNSUPPORTED, "kcoherence %d > 32 is not supported", K);
+    if (ndims > PT_MAX_DIMS) return pt_fail(PT_E_UNSUPPORTED, "too many dims");
+    KeyArgs ka;
+    ka.G = q.G;
+    int mx = 1;
+    for (int d = 0; d < q.G; d++) {
+        int v = nslices_host[d] + (nslices_host[d] >> 1);
+        mx = v > mx ? v : mx;
+    }
+    ka.B = 0;
+    while ((1 << ka.B) < mx) ka.B++;
+    if (ka.B == 0) ka.B = 1;
+    ka.Gk = q.G;
+    while (ka.Gk * ka.B > 64) ka.Gk--;
+    for (int d = 0; d < PT_MAX_DIMS; d++) ka.half[d] = d < q.G ? (nslices_host[d] >> 1) : 0;
+    const int nbits = ka.Gk * ka.B;
+
+    uint64_t *k0 = (uint64_t *)sc.get(20, N * 8), *k1 = (uint64_t *)sc.get(21, N * 8);
+    uint64_t *k0s = (uint64_t *)sc.get(22, N * 8), *k1s = (uint64_t *)sc.get(23, N * 8);
+    int32_t *idx = (int32_t *)sc.get(24, N * 4);
+    int32_t *perm0 = (int32_t *)sc.get(25, N * 4), *perm1 = (int32_t *)sc.get(26, N * 4);
+    int32_t *pos1 = (int32_t *)sc.get(27, N * 4);
+    float *ds0 = (float *)sc.get(28, (size_t)N * ndims * 4), *ds1 = (float *)sc.get(29, (size_t)N * ndims * 4);
+    if (!k0 || !k1 || !k0s || !k1s || !idx || !perm0 || !perm1 || !pos1 || !ds0 || !ds1)
+        return pt_fail(PT_E_NOMEM, "kcoherence: scratch allocation failed");
+    const int TB = 256;
+    hipLaunchKernelGGL(k_keys, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, N, q, ka, k0, k1, idx);
+    size_t tmp_bytes = 0;
+    PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k0s, idx, perm0, (int)N, 0, nbits, s));
+    void *tmp = sc.get(30, tmp_bytes);
+    if (!tmp) return pt_fail(PT_E_NOMEM, "kcoherence: sort scratch allocation failed");
+    PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k0s, idx, perm0, (int)N, 0, nbits, s));
+    PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k1, k1s, idx, perm1, (int)N, 0, nbits, s));
+    hipLaunchKernelGGL(k_positions, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, perm1, N, pos1);
+    hipLaunchKernelGGL(k_gather_desc, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, ndims, perm0, N, ds0);
+    hipLaunchKernelGGL(k_gather_desc, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, ndims, perm1, N, ds1);
+    if (K <= 16)
+        hipLaunchKernelGGL(k_select<16>, dim3(pt_blocks(N, 64)), dim3(64), 0, s, perm0, perm1, pos1, ds0, ds1, N, bew,
+                           ndims, K, window, min_dist2, b0, bw, kpos, kdist);
+    else
+        hipLaunchKernelGGL(k_select<32>, dim3(pt_blocks(N, 64)), dim3(64), 0, s, perm0, perm1, pos1, ds0, ds1, N, bew,
+                           ndims, K, window, min_dist2, b0, bw, kpos, kdist);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}

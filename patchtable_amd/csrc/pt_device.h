@@ -1,0 +1,91 @@
+// Device-side helpers shared by the HIP kernels of the PatchTable library (gfx950).
+#ifndef PT_DEVICE_H
+#define PT_DEVICE_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pt_internal.h"
+
+#define PT_HIP_TRY(expr)                                                                  \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return pt_failf(PT_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+#define PT_LAUNCH_CHECK() PT_HIP_TRY(hipGetLastError())
+
+static inline unsigned pt_blocks(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// int(float) with the x86 cvttss2si result for out-of-range and NaN inputs (INT_MIN), which is
+// what the reference's compiled get_bin computes (patchtable.h:1299).
+__device__ __forceinline__ int pt_x86_ftoi(float f) {
+    if (!(f >= -2147483648.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int)f;
+}
+
+// get_bin (patchtable.h:1298-1303): FP32 (v-min)*16383/range truncated, clamped, remapped.
+__device__ __forceinline__ int pt_get_bin(float v, float min_val, float range, const int32_t *__restrict__ to_bin) {
+    int idx = pt_x86_ftoi((v - min_val) * (float)(PT_NMAP - 1) / range);
+    if (idx < 0) idx = 0;
+    else if (idx >= PT_NMAP) idx = PT_NMAP - 1;
+    return to_bin[idx];
+}
+
+// Exact 8x8 RGB SSD with the lane order of patch_dist_exact (patchtable.h:330-373): lane j
+// accumulates elements 4n+j (n = 0..5) of each row in order, lanes are summed last.
+// a, b point at the patches' top-left pixels; pitches are row strides in floats (3*width).
+__device__ __forceinline__ float pt_patch_ssd(const float *__restrict__ a, int apitch,
+                                              const float *__restrict__ b, int bpitch) {
+    float L0 = 0.f, L1 = 0.f, L2 = 0.f, L3 = 0.f;
+#pragma unroll
+    for (int dy = 0; dy < 8; dy++) {
+        const float *ar = a + dy * apitch;
+        const float *br = b + dy * bpitch;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            float d0 = ar[4 * n + 0] - br[4 * n + 0];
+            float d1 = ar[4 * n + 1] - br[4 * n + 1];
+            float d2 = ar[4 * n + 2] - br[4 * n + 2];
+            float d3 = ar[4 * n + 3] - br[4 * n + 3];
+            L0 = L0 + d0 * d0;
+            L1 = L1 + d1 * d1;
+            L2 = L2 + d2 * d2;
+            L3 = L3 + d3 * d3;
+        }
+    }
+    return ((L0 + L1) + L2) + L3;
+}
+
+// Same sum, abandoned once a partial sum reaches `bound`. Partial lane sums never exceed the
+// final ones (adding non-negative floats is monotone), so a result >= bound means the exact SSD
+// is >= bound too; a result < bound is the exact SSD.
+__device__ __forceinline__ float pt_patch_ssd_bounded(const float *__restrict__ a, int apitch,
+                                                      const float *__restrict__ b, int bpitch, float bound) {
+    float L0 = 0.f, L1 = 0.f, L2 = 0.f, L3 = 0.f;
+#pragma unroll
+    for (int dy = 0; dy < 8; dy++) {
+        const float *ar = a + dy * apitch;
+        const float *br = b + dy * bpitch;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            float d0 = ar[4 * n + 0] - br[4 * n + 0];
+            float d1 = ar[4 * n + 1] - br[4 * n + 1];
+            float d2 = ar[4 * n + 2] - br[4 * n + 2];
+            float d3 = ar[4 * n + 3] - br[4 * n + 3];
+            L0 = L0 + d0 * d0;
+            L1 = L1 + d1 * d1;
+            L2 = L2 + d2 * d2;
+            L3 = L3 + d3 * d3;
+        }
+        if ((dy & 1) && dy < 7) {
+            float part = ((L0 + L1) + L2) + L3;
+            if (part >= bound) return part;
+        }
+    }
+    return ((L0 + L1) + L2) + L3;
+}
+
+#endif

@@ -1,0 +1,104 @@
+// Host-side launchers of the HIP kernels (one .hip file each) and small device-memory helpers.
+#ifndef PT_KERNELS_H
+#define PT_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "pt_internal.h"
+
+// Grow-only device scratch buffers addressed by slot; freed with the owner.
+struct PtScratch {
+    std::vector<void *> ptr;
+    std::vector<size_t> cap;
+    void *get(int slot, size_t bytes) {
+        if ((int)ptr.size() <= slot) {
+            ptr.resize(slot + 1, nullptr);
+            cap.resize(slot + 1, 0);
+        }
+        if (cap[slot] < bytes || !ptr[slot]) {
+            if (ptr[slot]) (void)hipFree(ptr[slot]);
+            ptr[slot] = nullptr;
+            cap[slot] = 0;
+            size_t b = bytes ? bytes : 16;
+            if (hipMalloc(&ptr[slot], b) != hipSuccess) return nullptr;
+            cap[slot] = b;
+        }
+        return ptr[slot];
+    }
+    void release() {
+        for (void *p : ptr)
+            if (p) (void)hipFree(p);
+        ptr.clear();
+        cap.clear();
+    }
+    ~PtScratch() { release(); }
+};
+
+// Quantiser state on the device: per grid dimension min, range, 16384-entry bin map, stride.
+struct DevQuant {
+    int G = 0;
+    const float *min_val = nullptr;
+    const float *range = nullptr;
+    const int32_t *to_bin = nullptr;
+    const int64_t *stride = nullptr;
+};
+
+// gck.hip
+int pt_gck_supported(int n, int nchroma);
+int pt_gck_launch(const float *img, int h, int w, int n, int nchroma, int convert_yuv, float *yuv_planes,
+                  float *temp, float *desc, hipStream_t s);
+
+// quant.hip
+int pt_minmax_launch(const float *desc, size_t plane, int Wp, int h, int w, int G, float *partials, int nblocks,
+                     hipStream_t s);
+int pt_quantile_values(const float *desc, size_t plane, int Wp, int h, int w, int step, int dim, const int64_t *ranks_dev,
+                       int nranks, float *values_dev, PtScratch &sc, hipStream_t s);
+
+// populate.hip
+int pt_shuffle_ranks(const int32_t *rnd_dev, int n, int populate_random, PtScratch &sc, int32_t *rank, hipStream_t s);
+int pt_populate_launch(const float *desc, size_t plane, int Wp, const int32_t *list_xy, const int32_t *rank, int n,
+                       const DevQuant &q, int32_t *table, int64_t ncells, int64_t *cell_scratch,
+                       uint8_t *win_scratch, int *count_dev, hipStream_t s);
+int pt_fill_launch(int32_t *p, int64_t n, int32_t v, hipStream_t s);
+
+// dt.hip
+int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
+                      uint64_t *key, hipStream_t s);
+int pt_fill_unused_launch(int32_t *table, int64_t ncells, int32_t fill, hipStream_t s);
+int pt_dt_downsample_launch(int32_t *table, int G, const int *nslices, PtScratch &sc, hipStream_t s);
+int pt_dt_euclidean_launch(int32_t *table, int G, const int *nslices, int64_t ncells, float *dist_scratch,
+                           hipStream_t s);
+
+// kcoh.hip
+int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, const float *b0, int bw, int bh,
+                         const DevQuant &q, const int *nslices_host, int K, int window, int min_dist2,
+                         int32_t *kpos, float *kdist, PtScratch &sc, hipStream_t s);
+
+// lookup.hip
+struct LookupArgs {
+    const float *a = nullptr;       // query RGB, HWC
+    int ah = 0, aw = 0;
+    const float *a_desc = nullptr;  // query descriptor planes
+    const float *b = nullptr;       // database RGB, HWC
+    int bh = 0, bw = 0;
+    const int32_t *table = nullptr;
+    DevQuant q;
+    const int32_t *kpos = nullptr;
+    const float *kdist = nullptr;
+    int K = 0;
+    int kcoh_iter0 = 1;             // k-coherence enabled on the first pass
+    double triangle_factor = 2.0;
+    int spatial = 1;
+    int do_table_lookup = 1;
+    int rounds = 4;
+};
+int pt_lookup_launch(const LookupArgs &L, int32_t *nx, int32_t *ny, float *nd, int32_t *nx2, int32_t *ny2, float *nd2,
+                     double *ann_out, hipStream_t s, hipEvent_t ev_after_candidates);
+int pt_ssd_pairs_launch(const float *a, int aw, const float *b, int bw, const int32_t *ax, const int32_t *ay,
+                        const int32_t *bx, const int32_t *by, int n, float *out, hipStream_t s);
+
+#endif

@@ -1,0 +1,647 @@
+// The PatchTable object behind the C ABI: construction (PatchTableFixedN constructor,
+// patchtable.h:2584-3373) and lookup (lookup_templated, patchtable.h:3468-4243) orchestrated on
+// one HIP stream per table; all per-patch and per-cell work runs in the HIP kernels.
+#include "pt_internal.h"
+#include "pt_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+void pt_glibc_rand_stream(uint32_t seed, int64_t n, int32_t *out);
+
+#define TRY(expr)                          \
+    do {                                   \
+        int rc_ = (expr);                  \
+        if (rc_ != PT_OK) return rc_;      \
+    } while (0)
+
+#define HIPTRY(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return pt_failf(PT_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+namespace {
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t count) {
+        if (count <= n && p) return PT_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc((void **)&p, (count ? count : 1) * sizeof(T)) != hipSuccess)
+            return pt_failf(PT_E_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+        n = count;
+        return PT_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+struct HostPartition {   // PatchPartition1D fields (patchtable.h:1133-1141)
+    std::vector<int32_t> to_bin;
+    std::vector<float> centers, lsizes, rsizes;
+    float min_val = 0, max_val = 0, range = 0;
+    int nslices = 0;
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct pt_table {
+    pt_params *p = nullptr;   // the caller's parameters, read at lookup time (patchtable.h:2602)
+    pt_params build;          // parameters the table was built with
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[12] = {};
+    int h = 0, w = 0;
+    int ndims = 0, nchroma = 0, G = 0;
+    std::vector<int> nslices;
+    std::vector<int64_t> stride;
+    int64_t ncells = 0;
+    std::vector<HostPartition> part;
+    int table_add_count = 0;
+    int K = 0;
+    double stage_ms[10] = {};
+
+    DevBuf<float> b0, desc;
+    DevBuf<float> qmin, qrange;
+    DevBuf<int32_t> qtobin;
+    DevBuf<int64_t> qstride;
+    DevBuf<int32_t> table;
+    DevBuf<int32_t> list_xy, rank, rnd;
+    DevBuf<int64_t> cells;
+    DevBuf<uint8_t> win;
+    DevBuf<int> counter;
+    DevBuf<uint64_t> dtkey;
+    DevBuf<int32_t> kpos;
+    DevBuf<float> kdist;
+    // query side
+    DevBuf<float> a, adesc, yuv, temp;
+    DevBuf<int32_t> nx, ny, nx2, ny2;
+    DevBuf<float> nd, nd2;
+    DevBuf<double> ann;
+    int qh = 0, qw = 0;
+    PtScratch sc;
+
+    DevQuant quant() const {
+        DevQuant q;
+        q.G = G;
+        q.min_val = qmin.p;
+        q.range = qrange.p;
+        q.to_bin = qtobin.p;
+        q.stride = qstride.p;
+        return q;
+    }
+    ~pt_table() {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int check_build_params(const pt_params *p, int c) {
+    if (p->is_descriptor) return pt_fail(PT_E_UNSUPPORTED, "is_descriptor mode is not supported by this build yet");
+    if (p->lookup_algo != PT_LOOKUP_ALGO_TABLE) return pt_fail(PT_E_UNSUPPORTED, "only lookup_algo table is supported");
+    if (p->dim_algo != PT_DIM_ALGO_WH) return pt_fail(PT_E_UNSUPPORTED, "only dim_algo wh is supported");
+    if (p->convert_colorspace && p->colorspace != PT_COLORSPACE_YUV)
+        return pt_fail(PT_E_UNSUPPORTED, "only the yuv and rgb colorspaces are supported");
+    if (p->patch_w != 8) return pt_fail(PT_E_UNSUPPORTED, "gck: patch_w unsupported (only 8)");
+    if (p->filter_dims_count) return pt_fail(PT_E_UNSUPPORTED, "filter_dims is not supported by this build yet");
+    if (p->ntables != 1) return pt_fail(PT_E_UNSUPPORTED, "ntables > 1 is not supported by this build yet");
+    if (p->kcoherence && p->kcoherence_algo != PT_KCOH_ALGO_FLANN)
+        return pt_fail(PT_E_UNSUPPORTED, "only the default kcoherence_algo is supported");
+    if (p->kcoherence_step != 1) return pt_fail(PT_E_UNSUPPORTED, "kcoherence_step > 1 is not supported");
+    if (p->incoherence || p->kcoherence_enrich) return pt_fail(PT_E_UNSUPPORTED, "incoherence/kcoherence_enrich unsupported");
+    if (p->randomize_dt) return pt_fail(PT_E_UNSUPPORTED, "randomize_dt is not supported");
+    if (p->run_dt && !(p->dt_algo == PT_DT_ALGO_PROP || p->dt_algo == PT_DT_ALGO_DOWNSAMPLE ||
+                       p->dt_algo == PT_DT_ALGO_EUCLIDEAN))
+        return pt_fail(PT_E_UNSUPPORTED, "dt_algo must be prop, downsample or euclidean");
+    if (p->regular_grid) return pt_fail(PT_E_UNSUPPORTED, "regular_grid is not supported");
+    if (c != 3) return pt_fail(PT_E_INVALID, "expected 3 channel image as input to PatchTable");
+    if (p->kcoherence < 0 || p->kcoherence > 32) return pt_fail(PT_E_UNSUPPORTED, "kcoherence must be in 0..32");
+    if (p->partition_step < 1) return pt_fail(PT_E_INVALID, "partition_step must be >= 1");
+    int gd = p->grid_ndims < 0 ? p->ndims : p->grid_ndims;
+    if (p->ndims < 1 || p->ndims > PT_MAX_DIMS || gd < 1 || gd > p->ndims)
+        return pt_fail(PT_E_INVALID, "invalid ndims/grid_ndims");
+    if (!pt_gck_supported(p->ndims - 2 * p->nchroma, p->nchroma))
+        return pt_failf(PT_E_UNSUPPORTED, "gck: n, nchroma unsupported: %d, %d", p->ndims - 2 * p->nchroma, p->nchroma);
+    return PT_OK;
+}
+
+// scale_slices_to_limit (patchtable.cpp:611-642) with product()/multiply() (util.cpp:186-206).
+std::vector<int> scale_slices(std::vector<int> ns, double limit) {
+    auto product = [](const std::vector<int> &L) {
+        double a = 1.0;
+        for (int v : L) a *= v;
+        return a;
+    };
+    if (limit > 0) {
+        std::vector<int> ns0(ns);
+        double factor = 1.0;
+        for (auto &v : ns) v = std::max(v, 1);
+        while (product(ns) > limit) {
+            factor *= 0.995;
+            for (size_t i = 0; i < ns.size(); i++) ns[i] = int(ns0[i] * factor + 0.5);
+            for (auto &v : ns) v = std::max(v, 1);
+        }
+        for (;;) {
+            ns0 = ns;
+            bool ok = false;
+            for (size_t i = 0; i < ns.size(); i++) {
+                ns[i]++;
+                if (product(ns) > limit) ns = ns0;
+                else {
+                    ok = true;
+                    ns0 = ns;
+                }
+            }
+            if (!ok) break;
+        }
+    }
+    return ns;
+}
+
+// Centres, half sizes and the 16384-entry value->bin map of one dimension
+// (PatchPartition1D constructor, patchtable.h:1193-1251), from the order statistics q of the
+// sorted values: q[i] is the value at rank (n-1)*i/nslices, i = 0..nslices.
+void make_partition(HostPartition &P, const std::vector<float> &q) {
+    const int ns = P.nslices;
+    P.centers.resize(ns);
+    P.lsizes.resize(ns);
+    P.rsizes.resize(ns);
+    for (int i = 0; i < ns; i++) {
+        float cell_min = q[i], cell_max = q[i + 1];
+        P.centers[i] = (cell_min + cell_max) / 2;
+    }
+    for (int i = 0; i < ns; i++) {
+        float prev_center = i - 1 >= 0 ? P.centers[i - 1] : P.min_val;
+        float next_center = i + 1 < ns ? P.centers[i + 1] : P.max_val;
+        P.lsizes[i] = P.centers[i] - prev_center;
+        if (i - 1 >= 0) P.lsizes[i] /= 2;
+        P.rsizes[i] = next_center - P.centers[i];
+        if (i + 1 < ns) P.rsizes[i] /= 2;
+    }
+    P.to_bin.resize(PT_NMAP);
+    int icenter = 0;
+    for (int i = 0; i < PT_NMAP; i++) {
+        double coord = P.min_val + (i + 0.5) * P.range / (PT_NMAP);
+        while (coord > P.centers[icenter]) {
+            if (icenter + 1 < ns) icenter++;
+            else break;
+        }
+        int prev_center = icenter - 1;
+        if (prev_center < 0) prev_center = 0;
+        int next_center = icenter;
+        if (next_center == 0) next_center++;
+        if (ns == 1) next_center = prev_center = 0;
+        double mid = (P.centers[prev_center] + P.centers[next_center]) / 2.0;
+        P.to_bin[i] = (coord < mid) ? prev_center : next_center;
+    }
+}
+
+int upload_quant(pt_table *t) {
+    const int G = t->G;
+    std::vector<float> mn(G), rg(G);
+    std::vector<int32_t> tb((size_t)G * PT_NMAP);
+    for (int i = 0; i < G; i++) {
+        mn[i] = t->part[i].min_val;
+        rg[i] = t->part[i].range;
+        std::copy(t->part[i].to_bin.begin(), t->part[i].to_bin.end(), tb.begin() + (size_t)i * PT_NMAP);
+    }
+    TRY(t->qmin.ensure(G));
+    TRY(t->qrange.ensure(G));
+    TRY(t->qtobin.ensure(tb.size()));
+    TRY(t->qstride.ensure(G));
+    HIPTRY(hipMemcpyAsync(t->qmin.p, mn.data(), G * 4, hipMemcpyHostToDevice, t->stream));
+    HIPTRY(hipMemcpyAsync(t->qrange.p, rg.data(), G * 4, hipMemcpyHostToDevice, t->stream));
+    HIPTRY(hipMemcpyAsync(t->qtobin.p, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, t->stream));
+    HIPTRY(hipMemcpyAsync(t->qstride.p, t->stride.data(), G * 8, hipMemcpyHostToDevice, t->stream));
+    HIPTRY(hipStreamSynchronize(t->stream));
+    return PT_OK;
+}
+
+__attribute__((unused)) int build_list(pt_table *t, std::vector<int32_t> &list) {
+    list.clear();
+    for (int y = 7; y < t->h; y++)
+        for (int x = 7; x < t->w; x++) list.push_back(PT_XY_TO_INT(x, y));
+    return PT_OK;
+}
+
+int build_table(pt_table *t, const float *b_dev) {
+    const pt_params &p = t->build;
+    const int h = t->h, w = t->w, Hp = h + 7, Wp = w + 7;
+    const size_t plane = (size_t)Hp * Wp;
+    hipStream_t s = t->stream;
+    HIPTRY(hipEventRecord(t->ev[0], s));
+    // 1. keep a copy of b (patchtable.h:2628) and project it (get_padded_descriptor, :2626)
+    TRY(t->b0.ensure((size_t)h * w * 3));
+    HIPTRY(hipMemcpyAsync(t->b0.p, b_dev, (size_t)h * w * 3 * 4, hipMemcpyDeviceToDevice, s));
+    TRY(t->desc.ensure(plane * t->ndims));
+    TRY(t->yuv.ensure((size_t)h * w * 3));
+    TRY(t->temp.ensure(plane * 3));
+    TRY(pt_gck_launch(t->b0.p, h, w, t->ndims - 2 * t->nchroma, t->nchroma, p.convert_colorspace, t->yuv.p, t->temp.p,
+                      t->desc.p, s));
+    HIPTRY(hipEventRecord(t->ev[1], s));
+    // 2. quantiser: min/max (patchtable.h:1758-1778), slices, order statistics, to_bin
+    const int G = t->G, NB = 256;
+    float *partials = (float *)t->sc.get(40, (size_t)NB * G * 2 * 4);
+    if (!partials) return pt_fail(PT_E_NOMEM, "min/max scratch allocation failed");
+    TRY(pt_minmax_launch(t->desc.p, plane, Wp, h, w, G, partials, NB, s));
+    std::vector<float> hp((size_t)NB * G * 2);
+    HIPTRY(hipMemcpyAsync(hp.data(), partials, hp.size() * 4, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    std::vector<float> mn(G, (float)1e20), mx(G, (float)-1e20), range(G);
+    for (int bk = 0; bk < NB; bk++)
+        for (int i = 0; i < G; i++) {
+            float a = hp[((size_t)bk * G + i) * 2], b = hp[((size_t)bk * G + i) * 2 + 1];
+            mn[i] = mn[i] < a ? mn[i] : a;
+            mx[i] = mx[i] > b ? mx[i] : b;
+        }
+    for (int i = 0; i < G; i++) range[i] = mx[i] - mn[i];
+    std::vector<int> ns(G);
+    if (p.nslices_count == 0) {   // set_default_nslices (patchtable.h:1780-1810)
+        float max_range = range[0];
+        for (int i = 1; i < G; i++) max_range = std::max(max_range, range[i]);
+        double scale = 200.0 / max_range;
+        for (int i = 0; i < G; i++) ns[i] = max_range > 0 ? int(range[i] * scale) : 0;
+    } else {
+        if (p.nslices_count != G) return pt_fail(PT_E_INVALID, "expected nslices size to match GRID_NCHANNELS");
+        for (int i = 0; i < G; i++) ns[i] = p.nslices[i];
+    }
+    t->nslices = scale_slices(ns, p.limit);
+    t->stride.assign(G, 0);
+    int64_t acc = 1;
+    for (int i = G - 1; i >= 0; i--) {
+        if (t->nslices[i] < 1) return pt_fail(PT_E_INVALID, "expected nslices >= 1");
+        t->stride[i] = acc;
+        acc *= t->nslices[i];
+    }
+    t->ncells = acc;
+    if (acc > ((int64_t)1 << 31) - 1) return pt_fail(PT_E_UNSUPPORTED, "table larger than 2^31 cells");
+    const int step = p.partition_step;
+    const int64_t npart = (int64_t)((w - 7 + step - 1) / step) * ((h - 7 + step - 1) / step);
+    t->part.assign(G, HostPartition());
+    {
+        int maxns = *std::max_element(t->nslices.begin(), t->nslices.end());
+        int64_t *ranks = (int64_t *)t->sc.get(41, (size_t)(maxns + 1) * 8);
+        float *vals = (float *)t->sc.get(42, (size_t)(maxns + 1) * 4);
+        if (!ranks || !vals) return pt_fail(PT_E_NOMEM, "quantile scratch allocation failed");
+        for (int i = 0; i < G; i++) {
+            HostPartition &P = t->part[i];
+            P.nslices = t->nslices[i];
+            P.min_val = mn[i];
+            P.max_val = mx[i];
+            P.range = mx[i] - mn[i];
+            std::vector<int64_t> hr(P.nslices + 1);
+            for (int j = 0; j <= P.nslices; j++) hr[j] = (npart - 1) * (int64_t)j / P.nslices;
+            HIPTRY(hipMemcpyAsync(ranks, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, s));
+            TRY(pt_quantile_values(t->desc.p, plane, Wp, h, w, step, i, ranks, (int)hr.size(), vals, t->sc, s));
+            std::vector<float> q(hr.size());
+            HIPTRY(hipMemcpyAsync(q.data(), vals, q.size() * 4, hipMemcpyDeviceToHost, s));
+            HIPTRY(hipStreamSynchronize(s));
+            make_partition(P, q);
+        }
+    }
+    TRY(upload_quant(t));
+    HIPTRY(hipEventRecord(t->ev[2], s));
+    // 3. populate (patchtable.h:3121-3159)
+    const int bew = w - 7, beh = h - 7;
+    const int n = bew * beh;
+    {
+        std::vector<int32_t> list;
+        build_list(t, list);
+        TRY(t->list_xy.ensure(n));
+        HIPTRY(hipMemcpyAsync(t->list_xy.p, list.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+        std::vector<int32_t> rnd(n > 1 ? n - 1 : 1);
+        if (p.populate_random && n > 1) pt_glibc_rand_stream(1, n - 1, rnd.data());
+        TRY(t->rnd.ensure(rnd.size()));
+        HIPTRY(hipMemcpyAsync(t->rnd.p, rnd.data(), rnd.size() * 4, hipMemcpyHostToDevice, s));
+        TRY(t->rank.ensure(n));
+        TRY(pt_shuffle_ranks(t->rnd.p, n, p.populate_random, t->sc, t->rank.p, s));
+        TRY(t->table.ensure((size_t)t->ncells));
+        TRY(t->cells.ensure(n));
+        TRY(t->win.ensure(n));
+        TRY(t->counter.ensure(1));
+        TRY(pt_populate_launch(t->desc.p, plane, Wp, t->list_xy.p, t->rank.p, n, t->quant(), t->table.p, t->ncells,
+                               t->cells.p, t->win.p, t->counter.p, s));
+        HIPTRY(hipMemcpyAsync(&t->table_add_count, t->counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+    }
+    HIPTRY(hipEventRecord(t->ev[3], s));
+    // 4. distance transform (patchtable.h:3254-3295) and fill_zeros (:2063-2079)
+    const int32_t vfill = PT_XY_TO_INT(7, 7);
+    if (p.run_dt && p.dt_algo == PT_DT_ALGO_PROP) {
+        TRY(t->dtkey.ensure((size_t)t->ncells));
+        TRY(pt_dt_prop_launch(t->table.p, G, t->nslices.data(), t->ncells, p.dt_iters, vfill, t->dtkey.p, s));
+        t->dtkey.release();
+    } else if (p.run_dt && p.dt_algo == PT_DT_ALGO_DOWNSAMPLE) {
+        TRY(pt_dt_downsample_launch(t->table.p, G, t->nslices.data(), t->sc, s));
+    } else if (p.run_dt && p.dt_algo == PT_DT_ALGO_EUCLIDEAN) {
+        DevBuf<float> dist;
+        TRY(dist.ensure((size_t)t->ncells));
+        TRY(pt_dt_euclidean_launch(t->table.p, G, t->nslices.data(), t->ncells, dist.p, s));
+        HIPTRY(hipStreamSynchronize(s));
+    } else {
+        // run_dt off: fill_zeros replaces every empty cell (patchtable.h:3289-3293)
+        TRY(pt_fill_unused_launch(t->table.p, t->ncells, vfill, s));
+    }
+    HIPTRY(hipEventRecord(t->ev[4], s));
+    // 5. k-coherence (patchtable.h:2653-2987)
+    t->K = p.kcoherence;
+    if (t->K > 0) {
+        TRY(t->kpos.ensure((size_t)n * t->K));
+        TRY(t->kdist.ensure((size_t)n * t->K));
+        TRY(pt_kcoherence_launch(t->desc.p, plane, Wp, t->ndims, t->b0.p, w, h, t->quant(), t->nslices.data(), t->K,
+                                 p.kcoh_window, p.kcoherence_min_dist * p.kcoherence_min_dist, t->kpos.p, t->kdist.p,
+                                 t->sc, s));
+    }
+    HIPTRY(hipEventRecord(t->ev[5], s));
+    HIPTRY(hipStreamSynchronize(s));
+    float ms = 0;
+    for (int k = 0; k < 5; k++) {
+        HIPTRY(hipEventElapsedTime(&ms, t->ev[k], t->ev[k + 1]));
+        t->stage_ms[k] = ms;
+    }
+    HIPTRY(hipEventElapsedTime(&ms, t->ev[0], t->ev[5]));
+    t->stage_ms[5] = ms;
+    t->rnd.release();
+    return PT_OK;
+}
+
+int create_common(pt_params *p, const float *b, bool b_on_device, int h, int w, int c, const int32_t *allowed,
+                  pt_table **out) {
+    if (!p || !out) return pt_fail(PT_E_INVALID, "pt_table_create: null argument");
+    *out = nullptr;
+    if (p->load_filename[0]) return pt_table_load(p, p->load_filename, out);
+    if (!b) return pt_fail(PT_E_INVALID, "pt_table_create: null image");
+    if (allowed) return pt_fail(PT_E_UNSUPPORTED, "allowed_patches is not supported by this build yet");
+    TRY(check_build_params(p, c));
+    if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "database image must be at least 8x8");
+    if (h > 65536 - 8 || w > 65536 - 8) return pt_fail(PT_E_INVALID, "image too large for XY_TO_INT packing");
+    if (p->grid_ndims < 0) p->grid_ndims = p->ndims;   // patchtable.h:2603
+    pt_table *t = new pt_table();
+    t->p = p;
+    t->build = *p;
+    t->h = h;
+    t->w = w;
+    t->ndims = p->ndims;
+    t->nchroma = p->nchroma;
+    t->G = p->grid_ndims;
+    int rc = PT_OK;
+    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = pt_fail(PT_E_HIP, "hipStreamCreate failed");
+    for (int i = 0; rc == PT_OK && i < 12; i++)
+        if (hipEventCreate(&t->ev[i]) != hipSuccess) rc = pt_fail(PT_E_HIP, "hipEventCreate failed");
+    DevBuf<float> staging;
+    const float *bd = b;
+    if (rc == PT_OK && !b_on_device) {
+        rc = staging.ensure((size_t)h * w * 3);
+        if (rc == PT_OK && hipMemcpy(staging.p, b, (size_t)h * w * 3 * 4, hipMemcpyHostToDevice) != hipSuccess)
+            rc = pt_fail(PT_E_HIP, "hipMemcpy of the database image failed");
+        bd = staging.p;
+    }
+    if (rc == PT_OK) rc = build_table(t, bd);
+    if (rc == PT_OK && p->save_filename[0]) rc = pt_table_save(t, p->save_filename);
+    if (rc != PT_OK) {
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return PT_OK;
+}
+
+int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, double *seconds) {
+    const pt_params &p = *t->p;
+    if (c != 3) return pt_fail(PT_E_INVALID, "expected 3 channel query image");
+    if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "query image must be at least 8x8");
+    if (!p.do_table_lookup) return pt_fail(PT_E_UNSUPPORTED, "do_table_lookup 0 requires ann_prev (not supported)");
+    if (p.init_random) return pt_fail(PT_E_UNSUPPORTED, "init_random is not supported");
+    if (p.query_step != 1) return pt_fail(PT_E_UNSUPPORTED, "query_step > 1 is not supported by this build yet");
+    if (p.prop_iters < 1) return pt_fail(PT_E_UNSUPPORTED, "prop_iters must be >= 1");
+    if (!p.spatial && p.do_rs) return pt_fail(PT_E_UNSUPPORTED, "random search without spatial search is not supported");
+    hipStream_t s = t->stream;
+    const int Hp = h + 7, Wp = w + 7;
+    const size_t plane = (size_t)Hp * Wp;
+    const long n = (long)(h - 7) * (w - 7);
+    TRY(t->adesc.ensure(plane * t->ndims));
+    TRY(t->yuv.ensure((size_t)h * w * 3));
+    TRY(t->temp.ensure(plane * 3));
+    TRY(t->nx.ensure(n));
+    TRY(t->ny.ensure(n));
+    TRY(t->nd.ensure(n));
+    TRY(t->nx2.ensure(n));
+    TRY(t->ny2.ensure(n));
+    TRY(t->nd2.ensure(n));
+    t->qh = h;
+    t->qw = w;
+    double T0 = now_s();
+    HIPTRY(hipEventRecord(t->ev[6], s));
+    TRY(pt_gck_launch(a_dev, h, w, t->ndims - 2 * t->nchroma, t->nchroma, t->build.convert_colorspace, t->yuv.p,
+                      t->temp.p, t->adesc.p, s));
+    HIPTRY(hipEventRecord(t->ev[7], s));
+    LookupArgs L;
+    L.a = a_dev;
+    L.ah = h;
+    L.aw = w;
+    L.a_desc = t->adesc.p;
+    L.b = t->b0.p;
+    L.bh = t->h;
+    L.bw = t->w;
+    L.table = t->table.p;
+    L.q = t->quant();
+    L.kpos = t->kpos.p;
+    L.kdist = t->kdist.p;
+    L.K = std::min(t->K, p.kcoherence);
+    L.kcoh_iter0 = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == 0);
+    L.triangle_factor = p.triangle_factor;
+    L.spatial = p.spatial;
+    L.do_table_lookup = p.do_table_lookup;
+    L.rounds = p.jf_rounds;
+    TRY(pt_lookup_launch(L, t->nx.p, t->ny.p, t->nd.p, t->nx2.p, t->ny2.p, t->nd2.p, ann_dev, s, t->ev[8]));
+    HIPTRY(hipEventRecord(t->ev[9], s));
+    HIPTRY(hipStreamSynchronize(s));
+    double T1 = now_s();
+    if (seconds) *seconds = T1 - T0;
+    float ms = 0;
+    HIPTRY(hipEventElapsedTime(&ms, t->ev[6], t->ev[7]));
+    t->stage_ms[6] = ms;
+    HIPTRY(hipEventElapsedTime(&ms, t->ev[7], t->ev[8]));
+    t->stage_ms[7] = ms;
+    HIPTRY(hipEventElapsedTime(&ms, t->ev[8], t->ev[9]));
+    t->stage_ms[8] = ms;
+    HIPTRY(hipEventElapsedTime(&ms, t->ev[6], t->ev[9]));
+    t->stage_ms[9] = ms;
+    return PT_OK;
+}
+
+}  // namespace
+
+extern "C" int pt_table_create(pt_params *p, const float *b, int h, int w, int c, const int32_t *allowed,
+                               pt_table **out) {
+    return create_common(p, b, false, h, w, c, allowed, out);
+}
+
+extern "C" int pt_table_create_dev(pt_params *p, const float *b_dev, int h, int w, int c, const int32_t *allowed,
+                                   pt_table **out) {
+    return create_common(p, b_dev, true, h, w, c, allowed, out);
+}
+
+extern "C" int pt_table_lookup(pt_table *t, const float *a, int h, int w, int c, double *ann_out,
+                               const double *ann_prev, const float *sv, double *seconds) {
+    if (!t || !a || !ann_out) return pt_fail(PT_E_INVALID, "pt_table_lookup: null argument");
+    if (ann_prev || sv) return pt_fail(PT_E_UNSUPPORTED, "ann_prev / coherence_temporal_sv are not supported yet");
+    if (c != 3 || h < 8 || w < 8) return pt_fail(PT_E_INVALID, "expected a 3 channel query image of at least 8x8");
+    TRY(t->a.ensure((size_t)h * w * 3));
+    const long n = (long)(h - 7) * (w - 7);
+    TRY(t->ann.ensure((size_t)n * 3));
+    HIPTRY(hipMemcpyAsync(t->a.p, a, (size_t)h * w * 3 * 4, hipMemcpyHostToDevice, t->stream));
+    TRY(lookup_common(t, t->a.p, h, w, c, t->ann.p, seconds));
+    HIPTRY(hipMemcpyAsync(ann_out, t->ann.p, (size_t)n * 3 * 8, hipMemcpyDeviceToHost, t->stream));
+    HIPTRY(hipStreamSynchronize(t->stream));
+    return PT_OK;
+}
+
+extern "C" int pt_table_lookup_dev(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_out_dev,
+                                   const double *ann_prev_dev, const float *sv_dev, double *seconds) {
+    if (!t || !a_dev || !ann_out_dev) return pt_fail(PT_E_INVALID, "pt_table_lookup_dev: null argument");
+    if (ann_prev_dev || sv_dev) return pt_fail(PT_E_UNSUPPORTED, "ann_prev / coherence_temporal_sv are not supported yet");
+    return lookup_common(t, a_dev, h, w, c, ann_out_dev, seconds);
+}
+
+extern "C" int pt_table_destroy(pt_table *t) {
+    delete t;
+    return PT_OK;
+}
+
+extern "C" int pt_table_save(pt_table *t, const char *filename) {
+    (void)t;
+    (void)filename;
+    return pt_fail(PT_E_UNSUPPORTED, "save is not supported by this build yet");
+}
+
+extern "C" int pt_table_load(pt_params *p, const char *filename, pt_table **out) {
+    (void)p;
+    (void)filename;
+    (void)out;
+    return pt_fail(PT_E_UNSUPPORTED, "load is not supported by this build yet");
+}
+
+// ---------------------------------------------------------------------------------------------
+// Introspection
+// ---------------------------------------------------------------------------------------------
+extern "C" int pt_table_info(const pt_table *t, int *grid_ndims, int *nslices, int64_t *ncells, int *table_add_count,
+                             int *kcoherence) {
+    if (!t) return pt_fail(PT_E_INVALID, "null table");
+    if (grid_ndims) *grid_ndims = t->G;
+    if (nslices)
+        for (int i = 0; i < t->G; i++) nslices[i] = t->nslices[i];
+    if (ncells) *ncells = t->ncells;
+    if (table_add_count) *table_add_count = t->table_add_count;
+    if (kcoherence) *kcoherence = t->K;
+    return PT_OK;
+}
+
+static int planes_to_hwc(const float *planes, int Hp, int Wp, int C, float *out) {
+    std::vector<float> host((size_t)Hp * Wp * C);
+    HIPTRY(hipMemcpy(host.data(), planes, host.size() * 4, hipMemcpyDeviceToHost));
+    const size_t plane = (size_t)Hp * Wp;
+    for (size_t i = 0; i < plane; i++)
+        for (int c = 0; c < C; c++) out[i * C + c] = host[c * plane + i];
+    return PT_OK;
+}
+
+extern "C" int pt_table_get_descriptor(const pt_table *t, float *out) {
+    if (!t || !out) return pt_fail(PT_E_INVALID, "null argument");
+    return planes_to_hwc(t->desc.p, t->h + 7, t->w + 7, t->ndims, out);
+}
+
+extern "C" int pt_table_get_query_descriptor(const pt_table *t, float *out) {
+    if (!t || !out || !t->qh) return pt_fail(PT_E_INVALID, "no query descriptor");
+    return planes_to_hwc(t->adesc.p, t->qh + 7, t->qw + 7, t->ndims, out);
+}
+
+extern "C" int pt_table_get_partition(const pt_table *t, int dim, float *min_val, float *max_val, float *centers,
+                                      int32_t *to_bin) {
+    if (!t || dim < 0 || dim >= t->G) return pt_fail(PT_E_INVALID, "bad dim");
+    const HostPartition &P = t->part[dim];
+    if (min_val) *min_val = P.min_val;
+    if (max_val) *max_val = P.max_val;
+    if (centers) std::copy(P.centers.begin(), P.centers.end(), centers);
+    if (to_bin) std::copy(P.to_bin.begin(), P.to_bin.end(), to_bin);
+    return PT_OK;
+}
+
+extern "C" int pt_table_get_cells(const pt_table *t, int32_t *out) {
+    if (!t || !out) return pt_fail(PT_E_INVALID, "null argument");
+    HIPTRY(hipMemcpy(out, t->table.p, (size_t)t->ncells * 4, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+extern "C" int pt_table_get_populated(const pt_table *t, int32_t *out) {
+    if (!t || !out) return pt_fail(PT_E_INVALID, "null argument");
+    const long n = (long)(t->w - 7) * (t->h - 7);
+    std::vector<int64_t> cells(n);
+    std::vector<uint8_t> win(n);
+    std::vector<int32_t> xy(n);
+    HIPTRY(hipMemcpy(cells.data(), t->cells.p, n * 8, hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(win.data(), t->win.p, n, hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(xy.data(), t->list_xy.p, n * 4, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < t->ncells; i++) out[i] = PT_TABLE_UNUSED;
+    for (long k = 0; k < n; k++)
+        if (win[k]) out[cells[k]] = xy[k];
+    return PT_OK;
+}
+
+extern "C" int pt_table_get_kcoherence(const pt_table *t, int32_t *pos_out, float *dist_out) {
+    if (!t) return pt_fail(PT_E_INVALID, "null table");
+    const size_t n = (size_t)(t->w - 7) * (t->h - 7) * t->K;
+    if (n == 0) return PT_OK;
+    if (pos_out) HIPTRY(hipMemcpy(pos_out, t->kpos.p, n * 4, hipMemcpyDeviceToHost));
+    if (dist_out) HIPTRY(hipMemcpy(dist_out, t->kdist.p, n * 4, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+extern "C" int pt_table_stage_ms(const pt_table *t, double *ms) {
+    if (!t || !ms) return pt_fail(PT_E_INVALID, "null argument");
+    for (int i = 0; i < 10; i++) ms[i] = t->stage_ms[i];
+    return PT_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stand-alone kernels
+// ---------------------------------------------------------------------------------------------
+extern "C" int pt_gck_dev(const float *img_dev, int h, int w, int n, int nchroma, int convert_yuv, float *out_dev) {
+    if (!img_dev || !out_dev || h < 1 || w < 1) return pt_fail(PT_E_INVALID, "pt_gck_dev: bad arguments");
+    DevBuf<float> yuv, temp;
+    TRY(yuv.ensure((size_t)h * w * 3));
+    TRY(temp.ensure((size_t)(h + 7) * (w + 7) * 3));
+    TRY(pt_gck_launch(img_dev, h, w, n, nchroma, convert_yuv, yuv.p, temp.p, out_dev, 0));
+    HIPTRY(hipStreamSynchronize(0));
+    return PT_OK;
+}
+
+extern "C" int pt_patch_ssd_dev(const float *a_dev, int aw, const float *b_dev, int bw, const int32_t *ax,
+                                const int32_t *ay, const int32_t *bx, const int32_t *by, int npairs, float *out_dev) {
+    TRY(pt_ssd_pairs_launch(a_dev, aw, b_dev, bw, ax, ay, bx, by, npairs, out_dev, 0));
+    HIPTRY(hipStreamSynchronize(0));
+    return PT_OK;
+}

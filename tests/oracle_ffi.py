@@ -1,0 +1,175 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline use it, as the checker.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_P = ctypes.POINTER
+_f32p = _P(ctypes.c_float)
+_f64p = _P(ctypes.c_double)
+_i32p = _P(ctypes.c_int32)
+_int = ctypes.c_int
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        from patchtable_amd._ffi import PtParams
+        L = ctypes.CDLL(LIB)
+        sig = {
+            "orc_glibc_rand": (_int, [ctypes.c_uint32, _int, _i32p]),
+            "orc_random_shuffle": (_int, [_int, _i32p]),
+            "orc_rgb2yuv": (_int, [_f32p, _int, _int, _f32p]),
+            "orc_wh_basis": (_int, [_int, _P(_int), _P(_int)]),
+            "orc_snake_order": (_int, [_int, _P(_int), _P(_int)]),
+            "orc_gck_recurrences": (_int, [_int, _P(_int), _P(_int), _P(_int)]),
+            "orc_gck": (_int, [_f32p, _int, _int, _int, _int, _f32p]),
+            "orc_descriptor": (_int, [_P(PtParams), _f32p, _int, _int, _f32p]),
+            "orc_patch_ssd": (ctypes.c_float, [_f32p, _int, _int, _int, _f32p, _int, _int, _int]),
+            "orc_scale_slices_to_limit": (_int, [_int, _P(_int), ctypes.c_double]),
+            "orc_partition": (_int, [_f32p, _int, _int, ctypes.c_float, ctypes.c_float, _f32p, _i32p]),
+            "orc_build": (ctypes.c_void_p, [_P(PtParams), _f32p, _int, _int]),
+            "orc_free": (None, [ctypes.c_void_p]),
+            "orc_set_params": (_int, [ctypes.c_void_p, _P(PtParams)]),
+            "orc_info": (_int, [ctypes.c_void_p, _P(_int), _P(_int), _P(ctypes.c_int64), _P(_int), _P(_int)]),
+            "orc_get_descriptor": (_int, [ctypes.c_void_p, _f32p]),
+            "orc_get_partition": (_int, [ctypes.c_void_p, _int, _f32p, _f32p, _f32p, _i32p]),
+            "orc_get_table": (_int, [ctypes.c_void_p, _i32p]),
+            "orc_get_populated": (_int, [ctypes.c_void_p, _i32p]),
+            "orc_get_kcoherence": (_int, [ctypes.c_void_p, _i32p, _f32p]),
+            "orc_dt": (_int, [_int, _P(_int), _i32p, _int, _int]),
+            "orc_lookup_raster": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f64p]),
+            "orc_lookup_parallel": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f64p]),
+            "orc_mean_dist": (ctypes.c_double, [_f64p, ctypes.c_long]),
+            "orc_recompute_mean_dist": (ctypes.c_double, [_f32p, _int, _int, _f32p, _int, _int, _f64p, _f64p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(_P(ctype))
+
+
+def f32(a):
+    return ptr(a, ctypes.c_float)
+
+
+def f64(a):
+    return ptr(a, ctypes.c_double)
+
+
+def i32(a):
+    return ptr(a, ctypes.c_int32)
+
+
+class OracleTable:
+    """CPU restatement of PatchTable construction (oracle/pt_oracle.cpp `build`)."""
+
+    def __init__(self, params, b):
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        self.h, self.w = b.shape[:2]
+        self.b = b
+        self.params = params
+        self.handle = lib().orc_build(ctypes.byref(params), f32(b), self.h, self.w)
+        G = ctypes.c_int()
+        ns = (ctypes.c_int * 64)()
+        nc = ctypes.c_int64()
+        add = ctypes.c_int()
+        K = ctypes.c_int()
+        lib().orc_info(self.handle, ctypes.byref(G), ns, ctypes.byref(nc), ctypes.byref(add), ctypes.byref(K))
+        self.G = G.value
+        self.nslices = [ns[i] for i in range(self.G)]
+        self.ncells = nc.value
+        self.table_add_count = add.value
+        self.K = K.value
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().orc_free(self.handle)
+            self.handle = None
+
+    def set_params(self, params):
+        self.params = params
+        lib().orc_set_params(self.handle, ctypes.byref(params))
+
+    def descriptor(self):
+        nd = self.params.ndims
+        out = np.empty((self.h + 7, self.w + 7, nd), np.float32)
+        lib().orc_get_descriptor(self.handle, f32(out))
+        return out
+
+    def partition(self, dim):
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        centers = np.empty(self.nslices[dim], np.float32)
+        to_bin = np.empty(16384, np.int32)
+        lib().orc_get_partition(self.handle, dim, ctypes.byref(mn), ctypes.byref(mx), f32(centers), i32(to_bin))
+        return mn.value, mx.value, centers, to_bin
+
+    def table(self):
+        out = np.empty(self.ncells, np.int32)
+        lib().orc_get_table(self.handle, i32(out))
+        return out
+
+    def populated(self):
+        out = np.empty(self.ncells, np.int32)
+        lib().orc_get_populated(self.handle, i32(out))
+        return out
+
+    def kcoherence(self):
+        n = (self.h - 7) * (self.w - 7) * self.K
+        pos = np.empty(n, np.int32)
+        dist = np.empty(n, np.float32)
+        lib().orc_get_kcoherence(self.handle, i32(pos), f32(dist))
+        return pos.reshape(self.h - 7, self.w - 7, self.K), dist.reshape(self.h - 7, self.w - 7, self.K)
+
+    def lookup_raster(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        ann = np.empty((a.shape[0] - 7, a.shape[1] - 7, 3), np.float64)
+        lib().orc_lookup_raster(self.handle, f32(a), a.shape[0], a.shape[1], f64(ann))
+        return ann
+
+    def lookup_parallel(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        ann = np.empty((a.shape[0] - 7, a.shape[1] - 7, 3), np.float64)
+        lib().orc_lookup_parallel(self.handle, f32(a), a.shape[0], a.shape[1], f64(ann))
+        return ann
+
+
+def mean_dist(ann):
+    ann = np.ascontiguousarray(ann, dtype=np.float64)
+    return lib().orc_mean_dist(f64(ann), ann.shape[0] * ann.shape[1])
+
+
+def recompute(a, b, ann):
+    """(mean sqrt SSD recomputed in float64, per-patch float64 SSD)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    ann = np.ascontiguousarray(ann, dtype=np.float64)
+    d = np.empty(ann.shape[:2], np.float64)
+    m = lib().orc_recompute_mean_dist(f32(a), a.shape[0], a.shape[1], f32(b), b.shape[0], b.shape[1], f64(ann), f64(d))
+    return m, d
+
+
+def dt(nslices, table, algo, dt_iters=-1):
+    t = np.ascontiguousarray(table, dtype=np.int32).copy()
+    ns = (ctypes.c_int * len(nslices))(*nslices)
+    rc = lib().orc_dt(len(nslices), ns, i32(t), algo, dt_iters)
+    assert rc == 0
+    return t
