@@ -228,10 +228,11 @@ int pt_table_get_populated(const pt_table *t, int32_t *table_out);
 int pt_table_get_kcoherence(const pt_table *t, int32_t *pos_out, float *dist_out);
 /* Query-side descriptor of the most recent lookup, (h+7) x (w+7) x ndims. */
 int pt_table_get_query_descriptor(const pt_table *t, float *wh_out);
-/* Per-stage device times (milliseconds) of the last create / lookup, from HIP events:
- * [0] gck, [1] quantiser, [2] populate, [3] dt, [4] kcoherence, [5] build total,
- * [6] query gck, [7] query candidates, [8] query propagation, [9] query total. */
-int pt_table_stage_ms(const pt_table *t, double *ms /* [10] */);
+/* Per-stage device times (milliseconds) of the last create / lookup, from HIP events on the
+ * table's stream: [0] gck, [1] quantiser, [2] populate, [3] dt, [4] kcoherence, [5] build
+ * total, [6] query gck, [7] query candidate kernel, [8] query propagation rounds (all
+ * launches), [9] query total, [10] number of propagation launches, [11] unused. */
+int pt_table_stage_ms(const pt_table *t, double *ms /* [12] */);
 
 /* ---- Stand-alone kernels (device pointers, stream 0 of the current device) ----------------
  * Used by the parity tests and the benchmark to exercise one kernel at a time. */

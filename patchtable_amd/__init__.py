@@ -209,10 +209,11 @@ class PatchTable:
         return pos, dist
 
     def stage_ms(self):
-        out = (ctypes.c_double * 10)()
+        out = (ctypes.c_double * 12)()
         check(load().pt_table_stage_ms(self._t, out))
         names = ["gck", "quantiser", "populate", "dt", "kcoherence", "build_total",
-                 "query_gck", "query_candidates", "query_propagation", "query_total"]
+                 "query_gck", "query_candidates", "query_propagation", "query_total",
+                 "propagation_launches"]
         return dict(zip(names, list(out)))
 
 

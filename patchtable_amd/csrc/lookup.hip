@@ -145,7 +145,7 @@ void pt_jump_schedule(int rounds, std::vector<int> &J) {
 }
 
 int pt_lookup_launch(const LookupArgs &L, int32_t *nx, int32_t *ny, float *nd, int32_t *nx2, int32_t *ny2, float *nd2,
-                     double *ann_out, hipStream_t s, hipEvent_t ev_after_candidates) {
+                     double *ann_out, hipStream_t s, hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds) {
     const long n = (long)(L.aw - 7) * (L.ah - 7);
     const int TB = 64;
     hipLaunchKernelGGL(k_candidates, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, L, nx, ny, nd);
@@ -160,6 +160,7 @@ int pt_lookup_launch(const LookupArgs &L, int32_t *nx, int32_t *ny, float *nd, i
         std::swap(cy, oy);
         std::swap(cdist, odist);
     }
+    if (ev_after_rounds) PT_HIP_TRY(hipEventRecord(ev_after_rounds, s));
     hipLaunchKernelGGL(k_write_ann, dim3(pt_blocks(n, 256)), dim3(256), 0, s, cx, cy, cdist, n, ann_out);
     PT_LAUNCH_CHECK();
     return PT_OK;

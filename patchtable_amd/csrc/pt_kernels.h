@@ -97,7 +97,7 @@ struct LookupArgs {
     int rounds = 4;
 };
 int pt_lookup_launch(const LookupArgs &L, int32_t *nx, int32_t *ny, float *nd, int32_t *nx2, int32_t *ny2, float *nd2,
-                     double *ann_out, hipStream_t s, hipEvent_t ev_after_candidates);
+                     double *ann_out, hipStream_t s, hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds);
 int pt_ssd_pairs_launch(const float *a, int aw, const float *b, int bw, const int32_t *ax, const int32_t *ay,
                         const int32_t *bx, const int32_t *by, int n, float *out, hipStream_t s);
 

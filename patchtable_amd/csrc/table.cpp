@@ -78,7 +78,7 @@ struct pt_table {
     std::vector<HostPartition> part;
     int table_add_count = 0;
     int K = 0;
-    double stage_ms[10] = {};
+    double stage_ms[12] = {};
 
     DevBuf<float> b0, desc;
     DevBuf<float> qmin, qrange;
@@ -476,7 +476,7 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     L.spatial = p.spatial;
     L.do_table_lookup = p.do_table_lookup;
     L.rounds = p.jf_rounds;
-    TRY(pt_lookup_launch(L, t->nx.p, t->ny.p, t->nd.p, t->nx2.p, t->ny2.p, t->nd2.p, ann_dev, s, t->ev[8]));
+    TRY(pt_lookup_launch(L, t->nx.p, t->ny.p, t->nd.p, t->nx2.p, t->ny2.p, t->nd2.p, ann_dev, s, t->ev[8], t->ev[10]));
     HIPTRY(hipEventRecord(t->ev[9], s));
     HIPTRY(hipStreamSynchronize(s));
     double T1 = now_s();
@@ -486,8 +486,9 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     t->stage_ms[6] = ms;
     HIPTRY(hipEventElapsedTime(&ms, t->ev[7], t->ev[8]));
     t->stage_ms[7] = ms;
-    HIPTRY(hipEventElapsedTime(&ms, t->ev[8], t->ev[9]));
+    HIPTRY(hipEventElapsedTime(&ms, t->ev[8], t->ev[10]));
     t->stage_ms[8] = ms;
+    t->stage_ms[10] = L.rounds;
     HIPTRY(hipEventElapsedTime(&ms, t->ev[6], t->ev[9]));
     t->stage_ms[9] = ms;
     return PT_OK;
@@ -622,7 +623,7 @@ extern "C" int pt_table_get_kcoherence(const pt_table *t, int32_t *pos_out, floa
 
 extern "C" int pt_table_stage_ms(const pt_table *t, double *ms) {
     if (!t || !ms) return pt_fail(PT_E_INVALID, "null argument");
-    for (int i = 0; i < 10; i++) ms[i] = t->stage_ms[i];
+    for (int i = 0; i < 12; i++) ms[i] = t->stage_ms[i];
     return PT_OK;
 }
 
