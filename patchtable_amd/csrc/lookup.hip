@@ -11,6 +11,13 @@
 //     propose their offset; then the spatial window around the best again.
 // Candidates replace the best only when strictly closer and every distance is the exact SSD
 // with the reference's lane order, so NNF_DIST is always the exact SSD of the stored match.
+//
+// Work layout: a workgroup of 256 lanes owns a 64 x 4 tile of query patches and stages the
+// tile's 71 x 11 query pixels in LDS (one float4 per pixel), so each SSD reads only the
+// database patch from global memory. The field is packed as XY_TO_INT position + distance, plus
+// a flag recording that the spatial window around the stored best has already been searched
+// without improvement; a later round whose best is unchanged skips that window (it would find
+// nothing strictly closer), which leaves every result unchanged.
 #include "pt_device.h"
 #include "pt_kernels.h"
 
@@ -18,12 +25,81 @@
 
 namespace {
 
+constexpr int TILE_X = 64, TILE_Y = 4, THREADS = TILE_X * TILE_Y;
+constexpr int AX = TILE_X + 7, AY = TILE_Y + 7;   // staged query pixels
+
 __constant__ int c_dx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 __constant__ int c_dy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 
-// Spatial search (patchtable.h:3861-3885), including the reference's "else if" clamping.
-__device__ __forceinline__ void spatial_search(const LookupArgs &L, const float *pa, int apitch, int &xb, int &yb,
-                                               float &db) {
+// Stages the query pixels of this block's tile into LDS, zero outside the image.
+__device__ __forceinline__ void stage_query(const LookupArgs &L, float4 *tile) {
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
+    for (int i = threadIdx.x; i < AX * AY; i += THREADS) {
+        int ly = i / AX, lx = i - ly * AX;
+        int gx = x0 + lx, gy = y0 + ly;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (gx < L.aw && gy < L.ah) {
+            const float *p = L.a + ((size_t)gy * L.aw + gx) * 3;
+            v = make_float4(p[0], p[1], p[2], 0.f);
+        }
+        tile[i] = v;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ float4 ldg4(const float *p) {
+    float4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
+
+#define PT_ACC(e, bv)                                  \
+    {                                                  \
+        float d_ = av[e] - (bv);                       \
+        acc[(e) & 3] = acc[(e) & 3] + d_ * d_;         \
+    }
+
+// Exact SSD of the staged query patch at `aq` (row pitch AX pixels) against the database patch
+// at `b` (row pitch `bpitch` floats), lane order of patch_dist_exact (patchtable.h:330-373):
+// element e of a 24-float row goes to lane e & 3, lanes accumulate in increasing e then row.
+// With `bounded`, the sum is abandoned once a partial sum (after rows 2, 4, 6) reaches `bound`.
+// Rows are processed one at a time (83 VGPRs, 5 waves per SIMD).
+template <bool bounded>
+__device__ __forceinline__ float ssd_staged(const float4 *aq, const float *__restrict__ b, int bpitch, float bound) {
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int dy = 0; dy < 8; dy++) {
+        const float4 *ar = aq + dy * AX;
+        const float *br = b + (size_t)dy * bpitch;
+        float4 b0 = ldg4(br), b1 = ldg4(br + 4), b2 = ldg4(br + 8), b3 = ldg4(br + 12), b4 = ldg4(br + 16),
+               b5 = ldg4(br + 20);
+        float av[24];
+#pragma unroll
+        for (int px = 0; px < 8; px++) {
+            float4 p = ar[px];
+            av[3 * px + 0] = p.x;
+            av[3 * px + 1] = p.y;
+            av[3 * px + 2] = p.z;
+        }
+        PT_ACC(0, b0.x) PT_ACC(1, b0.y) PT_ACC(2, b0.z) PT_ACC(3, b0.w)
+        PT_ACC(4, b1.x) PT_ACC(5, b1.y) PT_ACC(6, b1.z) PT_ACC(7, b1.w)
+        PT_ACC(8, b2.x) PT_ACC(9, b2.y) PT_ACC(10, b2.z) PT_ACC(11, b2.w)
+        PT_ACC(12, b3.x) PT_ACC(13, b3.y) PT_ACC(14, b3.z) PT_ACC(15, b3.w)
+        PT_ACC(16, b4.x) PT_ACC(17, b4.y) PT_ACC(18, b4.z) PT_ACC(19, b4.w)
+        PT_ACC(20, b5.x) PT_ACC(21, b5.y) PT_ACC(22, b5.z) PT_ACC(23, b5.w)
+        if (bounded && (dy & 1) && dy < 7) {
+            float part = ((acc[0] + acc[1]) + acc[2]) + acc[3];
+            if (part >= bound) return part;
+        }
+    }
+    return ((acc[0] + acc[1]) + acc[2]) + acc[3];
+}
+#undef PT_ACC
+
+// Spatial search (patchtable.h:3861-3885), including the reference's "else if" clamping; the
+// far side is also clamped where the reference would read out of bounds (a database fewer than
+// 2r+1 patches tall or wide). Returns true if the best changed.
+__device__ __forceinline__ bool spatial_search(const LookupArgs &L, const float4 *aq, int &xb, int &yb, float &db) {
     const int bew = L.bw - 7, beh = L.bh - 7, r = L.spatial;
     int symin = yb - r, symax = yb + r + 1, sxmin = xb - r, sxmax = xb + r + 1;
     const int xb0 = xb, yb0 = yb;
@@ -31,30 +107,33 @@ __device__ __forceinline__ void spatial_search(const LookupArgs &L, const float 
     else if (symax > beh) symax = beh;
     if (sxmin < 0) sxmin = 0;
     else if (sxmax > bew) sxmax = bew;
-    // The reference's "else if" leaves the far side unclamped when the window overhangs both
-    // sides (a database fewer than 2r+1 patches tall or wide), where it reads out of bounds.
     if (symax > beh) symax = beh;
     if (sxmax > bew) sxmax = bew;
     const int bpitch = L.bw * 3;
+    bool changed = false;
     for (int ys = symin; ys < symax; ys++)
         for (int xs = sxmin; xs < sxmax; xs++) {
             if (xs == xb0 && ys == yb0) continue;
-            float d = pt_patch_ssd_bounded(pa, apitch, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
+            float d = ssd_staged<true>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
             if (d < db) {
                 db = d;
                 xb = xs;
                 yb = ys;
+                changed = true;
             }
         }
+    return changed;
 }
 
-__global__ void k_candidates(LookupArgs L, int32_t *__restrict__ nx, int32_t *__restrict__ ny, float *__restrict__ nd) {
+__global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField out) {
+    __shared__ float4 tile[AX * AY];
+    stage_query(L, tile);
     const int ann_w = L.aw - 7, ann_h = L.ah - 7;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)ann_w * ann_h) return;
-    const int x = (int)(i % ann_w), y = (int)(i / ann_w);
-    const int apitch = L.aw * 3, bpitch = L.bw * 3, bew = L.bw - 7;
-    const float *pa = L.a + ((size_t)y * L.aw + x) * 3;
+    const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
+    const int x = blockIdx.x * TILE_X + lx, y = blockIdx.y * TILE_Y + ly;
+    if (x >= ann_w || y >= ann_h) return;
+    const float4 *aq = tile + ly * AX + lx;
+    const int bpitch = L.bw * 3, bew = L.bw - 7;
     // table cell of the query descriptor at (y+7, x+7)
     const size_t plane = (size_t)(L.ah + 7) * (L.aw + 7);
     const size_t off = (size_t)(y + 7) * (L.aw + 7) + x + 7;
@@ -65,14 +144,14 @@ __global__ void k_candidates(LookupArgs L, int32_t *__restrict__ nx, int32_t *__
                 L.q.stride[d];
     int v = L.table[cell];
     int xb = PT_INT_TO_X(v) - 7, yb = PT_INT_TO_Y(v) - 7;
-    float db = pt_patch_ssd(pa, apitch, L.b + ((size_t)yb * L.bw + xb) * 3, bpitch);
+    float db = ssd_staged<false>(aq, L.b + ((size_t)yb * L.bw + xb) * 3, bpitch, 0.f);
     if (L.K > 0 && L.kcoh_iter0) {
         const int32_t *kp = L.kpos + ((size_t)yb * bew + xb) * L.K;
         const float *kd = L.kdist + ((size_t)yb * bew + xb) * L.K;
         for (int k = 0; k < L.K; k++) {
             if ((double)kd[k] >= L.triangle_factor * (double)db) break;
             int xs = PT_INT_TO_X(kp[k]), ys = PT_INT_TO_Y(kp[k]);
-            float d = pt_patch_ssd_bounded(pa, apitch, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
+            float d = ssd_staged<true>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
             if (d < db) {
                 db = d;
                 xb = xs;
@@ -80,50 +159,57 @@ __global__ void k_candidates(LookupArgs L, int32_t *__restrict__ nx, int32_t *__
             }
         }
     }
-    if (L.spatial) spatial_search(L, pa, apitch, xb, yb, db);
-    nx[i] = xb;
-    ny[i] = yb;
-    nd[i] = db;
+    bool done = false;
+    if (L.spatial) done = !spatial_search(L, aq, xb, yb, db);
+    const long i = (long)y * ann_w + x;
+    out.xy[i] = PT_XY_TO_INT(xb, yb);
+    out.d[i] = db;
+    out.done[i] = done;
 }
 
-__global__ void k_propagate(LookupArgs L, int J, const int32_t *__restrict__ cx, const int32_t *__restrict__ cy,
-                            const float *__restrict__ cd, int32_t *__restrict__ nx, int32_t *__restrict__ ny,
-                            float *__restrict__ nd) {
+__global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, NnfField cur, NnfField nxt) {
+    __shared__ float4 tile[AX * AY];
+    stage_query(L, tile);
     const int ann_w = L.aw - 7, ann_h = L.ah - 7;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)ann_w * ann_h) return;
-    const int x = (int)(i % ann_w), y = (int)(i / ann_w);
-    const int apitch = L.aw * 3, bpitch = L.bw * 3, bew = L.bw - 7, beh = L.bh - 7;
-    const float *pa = L.a + ((size_t)y * L.aw + x) * 3;
-    int xb = cx[i], yb = cy[i];
-    float db = cd[i];
+    const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
+    const int x = blockIdx.x * TILE_X + lx, y = blockIdx.y * TILE_Y + ly;
+    if (x >= ann_w || y >= ann_h) return;
+    const float4 *aq = tile + ly * AX + lx;
+    const int bpitch = L.bw * 3, bew = L.bw - 7, beh = L.bh - 7;
+    const long i = (long)y * ann_w + x;
+    const int v0 = cur.xy[i];
+    int xb = PT_INT_TO_X(v0), yb = PT_INT_TO_Y(v0);
+    float db = cur.d[i];
+    bool moved = false;
     for (int t = 0; t < 8; t++) {
         int qx = x + J * c_dx[t], qy = y + J * c_dy[t];
         if (qx < 0 || qy < 0 || qx >= ann_w || qy >= ann_h) continue;
-        long qi = (long)qy * ann_w + qx;
-        int xs = cx[qi] - J * c_dx[t], ys = cy[qi] - J * c_dy[t];
+        int vq = cur.xy[(long)qy * ann_w + qx];
+        int xs = PT_INT_TO_X(vq) - J * c_dx[t], ys = PT_INT_TO_Y(vq) - J * c_dy[t];
         if (xs < 0 || ys < 0 || xs >= bew || ys >= beh) continue;
         if (xs == xb && ys == yb) continue;
-        float d = pt_patch_ssd_bounded(pa, apitch, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
+        float d = ssd_staged<true>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
         if (d < db) {
             db = d;
             xb = xs;
             yb = ys;
+            moved = true;
         }
     }
-    if (L.spatial) spatial_search(L, pa, apitch, xb, yb, db);
-    nx[i] = xb;
-    ny[i] = yb;
-    nd[i] = db;
+    bool done = cur.done[i] && !moved;
+    if (L.spatial && !done) done = !spatial_search(L, aq, xb, yb, db);
+    nxt.xy[i] = PT_XY_TO_INT(xb, yb);
+    nxt.d[i] = db;
+    nxt.done[i] = done;
 }
 
-__global__ void k_write_ann(const int32_t *__restrict__ nx, const int32_t *__restrict__ ny, const float *__restrict__ nd,
-                            long n, double *__restrict__ ann) {
+__global__ void k_write_ann(NnfField f, long n, double *__restrict__ ann) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    ann[i * 3 + 0] = (double)nx[i];
-    ann[i * 3 + 1] = (double)ny[i];
-    ann[i * 3 + 2] = (double)nd[i];
+    int v = f.xy[i];
+    ann[i * 3 + 0] = (double)PT_INT_TO_X(v);
+    ann[i * 3 + 1] = (double)PT_INT_TO_Y(v);
+    ann[i * 3 + 2] = (double)f.d[i];
 }
 
 __global__ void k_ssd_pairs(const float *__restrict__ a, int aw, const float *__restrict__ b, int bw,
@@ -144,24 +230,21 @@ void pt_jump_schedule(int rounds, std::vector<int> &J) {
     while ((int)J.size() < rounds) J.push_back(1);
 }
 
-int pt_lookup_launch(const LookupArgs &L, int32_t *nx, int32_t *ny, float *nd, int32_t *nx2, int32_t *ny2, float *nd2,
-                     double *ann_out, hipStream_t s, hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds) {
-    const long n = (long)(L.aw - 7) * (L.ah - 7);
-    const int TB = 64;
-    hipLaunchKernelGGL(k_candidates, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, L, nx, ny, nd);
+int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, double *ann_out, hipStream_t s,
+                     hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds) {
+    const int ann_w = L.aw - 7, ann_h = L.ah - 7;
+    const long n = (long)ann_w * ann_h;
+    dim3 grid((ann_w + TILE_X - 1) / TILE_X, (ann_h + TILE_Y - 1) / TILE_Y);
+    hipLaunchKernelGGL(k_candidates, grid, dim3(THREADS), 0, s, L, f0);
     if (ev_after_candidates) PT_HIP_TRY(hipEventRecord(ev_after_candidates, s));
     std::vector<int> J;
     pt_jump_schedule(L.rounds, J);
-    int32_t *cx = nx, *cy = ny, *ox = nx2, *oy = ny2;
-    float *cdist = nd, *odist = nd2;
     for (int j : J) {
-        hipLaunchKernelGGL(k_propagate, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, L, j, cx, cy, cdist, ox, oy, odist);
-        std::swap(cx, ox);
-        std::swap(cy, oy);
-        std::swap(cdist, odist);
+        hipLaunchKernelGGL(k_propagate, grid, dim3(THREADS), 0, s, L, j, f0, f1);
+        std::swap(f0, f1);
     }
     if (ev_after_rounds) PT_HIP_TRY(hipEventRecord(ev_after_rounds, s));
-    hipLaunchKernelGGL(k_write_ann, dim3(pt_blocks(n, 256)), dim3(256), 0, s, cx, cy, cdist, n, ann_out);
+    hipLaunchKernelGGL(k_write_ann, dim3(pt_blocks(n, 256)), dim3(256), 0, s, f0, n, ann_out);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }

@@ -96,8 +96,15 @@ struct LookupArgs {
     int do_table_lookup = 1;
     int rounds = 4;
 };
-int pt_lookup_launch(const LookupArgs &L, int32_t *nx, int32_t *ny, float *nd, int32_t *nx2, int32_t *ny2, float *nd2,
-                     double *ann_out, hipStream_t s, hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds);
+// One nearest-neighbour field: XY_TO_INT(b patch x, y), exact SSD, and whether the spatial
+// window around that match was already searched without improvement.
+struct NnfField {
+    int32_t *xy = nullptr;
+    float *d = nullptr;
+    uint8_t *done = nullptr;
+};
+int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, double *ann_out, hipStream_t s,
+                     hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds);
 int pt_ssd_pairs_launch(const float *a, int aw, const float *b, int bw, const int32_t *ax, const int32_t *ay,
                         const int32_t *bx, const int32_t *by, int n, float *out, hipStream_t s);
 

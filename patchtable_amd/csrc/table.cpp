@@ -94,8 +94,9 @@ struct pt_table {
     DevBuf<float> kdist;
     // query side
     DevBuf<float> a, adesc, yuv, temp;
-    DevBuf<int32_t> nx, ny, nx2, ny2;
+    DevBuf<int32_t> nxy, nxy2;
     DevBuf<float> nd, nd2;
+    DevBuf<uint8_t> ndone, ndone2;
     DevBuf<double> ann;
     int qh = 0, qw = 0;
     PtScratch sc;
@@ -445,12 +446,12 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     TRY(t->adesc.ensure(plane * t->ndims));
     TRY(t->yuv.ensure((size_t)h * w * 3));
     TRY(t->temp.ensure(plane * 3));
-    TRY(t->nx.ensure(n));
-    TRY(t->ny.ensure(n));
+    TRY(t->nxy.ensure(n));
     TRY(t->nd.ensure(n));
-    TRY(t->nx2.ensure(n));
-    TRY(t->ny2.ensure(n));
+    TRY(t->ndone.ensure(n));
+    TRY(t->nxy2.ensure(n));
     TRY(t->nd2.ensure(n));
+    TRY(t->ndone2.ensure(n));
     t->qh = h;
     t->qw = w;
     double T0 = now_s();
@@ -476,7 +477,14 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     L.spatial = p.spatial;
     L.do_table_lookup = p.do_table_lookup;
     L.rounds = p.jf_rounds;
-    TRY(pt_lookup_launch(L, t->nx.p, t->ny.p, t->nd.p, t->nx2.p, t->ny2.p, t->nd2.p, ann_dev, s, t->ev[8], t->ev[10]));
+    NnfField f0, f1;
+    f0.xy = t->nxy.p;
+    f0.d = t->nd.p;
+    f0.done = t->ndone.p;
+    f1.xy = t->nxy2.p;
+    f1.d = t->nd2.p;
+    f1.done = t->ndone2.p;
+    TRY(pt_lookup_launch(L, f0, f1, ann_dev, s, t->ev[8], t->ev[10]));
     HIPTRY(hipEventRecord(t->ev[9], s));
     HIPTRY(hipStreamSynchronize(s));
     double T1 = now_s();
