@@ -1026,7 +1026,13 @@ static void jf_jumps(int rounds, std::vector<int> &J) {
     while ((int)J.size() < rounds) J.push_back(1);
 }
 
+// Counters of SSD evaluations in the parallel model (diagnostics): [0] table + k-coherence,
+// [1] spatial window of the candidate pass, [2 + 2r] neighbours of round r, [3 + 2r] spatial
+// window of round r.
+static long g_ssd_count[64];
+
 static void lookup_parallel(const Table &T, const float *a, int ah, int aw, double *ann) {
+    for (auto &c : g_ssd_count) c = 0;
     const pt_params &p = T.p;
     fvec a_wh;
     descriptor(p, a, ah, aw, a_wh);
@@ -1037,8 +1043,11 @@ static void lookup_parallel(const Table &T, const float *a, int ah, int aw, doub
     const long N = (long)ann_h * ann_w;
     std::vector<int32_t> cx(N), cy(N), nx(N), ny(N);
     fvec cd(N), ndist(N);
-    auto spatial = [&](int x, int y, int &xb, int &yb, float &db) {
-        if (!p.spatial) return;
+    // Returns true if the window improved the best. A window whose search found nothing around
+    // an unchanged best would find nothing again, so callers may skip it (results unchanged).
+    auto spatial = [&](int x, int y, int &xb, int &yb, float &db, int slot) -> bool {
+        if (!p.spatial) return false;
+        bool changed = false;
         int symin = yb - p.spatial, symax = yb + p.spatial + 1;
         int sxmin = xb - p.spatial, sxmax = xb + p.spatial + 1;
         int xb0 = xb, yb0 = yb;
@@ -1052,19 +1061,24 @@ static void lookup_parallel(const Table &T, const float *a, int ah, int aw, doub
             for (int xs = sxmin; xs < sxmax; xs++)
                 if (xs != xb0 || ys != yb0) {
                     float d = patch_ssd(a, aw, x, y, b, T.w, xs, ys);
+                    g_ssd_count[slot]++;
                     if (d < db) {
                         db = d;
                         xb = xs;
                         yb = ys;
+                        changed = true;
                     }
                 }
+        return changed;
     };
+    std::vector<uint8_t> cdone(N), ndone(N), cchg(N, 1), nchg(N);
     for (int y = 0; y < ann_h; y++)
         for (int x = 0; x < ann_w; x++) {
             const float *whp = &a_wh[((size_t)(y + 7) * AWp + x + 7) * nd];
             int v = T.table[T.cell_of(whp)];
             int xb = INT_TO_X(v) - 7, yb = INT_TO_Y(v) - 7;
             float db = patch_ssd(a, aw, x, y, b, T.w, xb, yb);
+            g_ssd_count[0]++;
             if (p.kcoherence > 0) {
                 const int32_t *kp = &T.kpos[((size_t)yb * bew + xb) * T.K];
                 const float *kd = &T.kdist[((size_t)yb * bew + xb) * T.K];
@@ -1072,6 +1086,7 @@ static void lookup_parallel(const Table &T, const float *a, int ah, int aw, doub
                     if (kd[k] >= p.triangle_factor * db) break;
                     int xs = INT_TO_X(kp[k]), ys = INT_TO_Y(kp[k]);
                     float d = patch_ssd(a, aw, x, y, b, T.w, xs, ys);
+                    g_ssd_count[0]++;
                     if (d < db) {
                         db = d;
                         xb = xs;
@@ -1079,11 +1094,12 @@ static void lookup_parallel(const Table &T, const float *a, int ah, int aw, doub
                     }
                 }
             }
-            spatial(x, y, xb, yb, db);
+            bool changed = spatial(x, y, xb, yb, db, 1);
             long i = (long)y * ann_w + x;
             cx[i] = xb;
             cy[i] = yb;
             cd[i] = db;
+            cdone[i] = p.spatial && !changed;
         }
     std::vector<int> J;
     jf_jumps(p.jf_rounds, J);
@@ -1091,33 +1107,50 @@ static void lookup_parallel(const Table &T, const float *a, int ah, int aw, doub
     static const int DY[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
     for (size_t r = 0; r < J.size(); r++) {
         int j = J[r];
+        // A candidate from neighbour n repeats the previous round's when that round had the same
+        // jump and neither the patch nor n changed in it; it was rejected against the same best
+        // then, so it is skipped (results unchanged).
+        const bool same_jump = r > 0 && J[r - 1] == j;
         for (int y = 0; y < ann_h; y++)
             for (int x = 0; x < ann_w; x++) {
                 long i = (long)y * ann_w + x;
                 int xb = cx[i], yb = cy[i];
                 float db = cd[i];
+                bool moved = false;
                 for (int t = 0; t < 8; t++) {
                     int qx = x + j * DX[t], qy = y + j * DY[t];
                     if (qx < 0 || qy < 0 || qx >= ann_w || qy >= ann_h) continue;
                     long qi = (long)qy * ann_w + qx;
+                    if (same_jump && !cchg[i] && !cchg[qi]) continue;
                     int xs = cx[qi] - j * DX[t], ys = cy[qi] - j * DY[t];
                     if (xs < 0 || ys < 0 || xs >= bew || ys >= beh) continue;
                     if (xs == xb && ys == yb) continue;
                     float d = patch_ssd(a, aw, x, y, b, T.w, xs, ys);
+                    g_ssd_count[2 + 2 * r]++;
                     if (d < db) {
                         db = d;
                         xb = xs;
                         yb = ys;
+                        moved = true;
                     }
                 }
-                spatial(x, y, xb, yb, db);
+                bool done = cdone[i] && !moved;
+                bool schg = false;
+                if (p.spatial && !done) {
+                    schg = spatial(x, y, xb, yb, db, 3 + 2 * (int)r);
+                    done = !schg;
+                }
                 nx[i] = xb;
                 ny[i] = yb;
                 ndist[i] = db;
+                ndone[i] = done;
+                nchg[i] = moved || schg;
             }
         cx.swap(nx);
         cy.swap(ny);
         cd.swap(ndist);
+        cdone.swap(ndone);
+        cchg.swap(nchg);
     }
     for (long i = 0; i < N; i++) {
         ann[i * 3 + 0] = cx[i];
@@ -1296,6 +1329,11 @@ int orc_dt(int G, const int *nslices, int32_t *table, int algo, int dt_iters) {
 
 int orc_lookup_raster(void *t, const float *a, int h, int w, double *ann) {
     lookup_raster(*(Table *)t, a, h, w, ann);
+    return 0;
+}
+
+int orc_ssd_counts(long *out) {
+    for (int i = 0; i < 64; i++) out[i] = g_ssd_count[i];
     return 0;
 }
 

@@ -54,6 +54,35 @@ __global__ void k_dt_line(uint64_t *__restrict__ key, int64_t nlines, int len, i
     }
 }
 
+// The same sweeps for the last dimension, whose lines are contiguous: a block stages
+// `lpb` consecutive lines in LDS with coalesced loads, sweeps each line from LDS, writes back.
+__global__ void k_dt_line_contig(uint64_t *__restrict__ key, int64_t nlines, int len, int lpb) {
+    extern __shared__ uint64_t sh[];
+    const int64_t l0 = (int64_t)blockIdx.x * lpb;
+    const int nl = (int)(nlines - l0 < lpb ? nlines - l0 : lpb);
+    uint64_t *g = key + l0 * len;
+    const int total = nl * len;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) sh[i] = g[i];
+    __syncthreads();
+    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+        uint64_t *p = sh + l * len;
+        uint64_t run = KEY_INF;
+        for (int t = 0; t < len; t++) {
+            if (run != KEY_INF) run += KEY_STEP;
+            if (p[t] < run) run = p[t];
+            p[t] = run;
+        }
+        run = KEY_INF;
+        for (int t = len - 1; t >= 0; t--) {
+            if (run != KEY_INF) run += KEY_STEP;
+            if (p[t] < run) run = p[t];
+            p[t] = run;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < total; i += blockDim.x) g[i] = sh[i];
+}
+
 // table[c] = value of the winning source, or `fill` for cells beyond max_dist / unreached.
 // Source cells keep their own value, so reading table[src] while writing other cells is safe.
 __global__ void k_dt_final(const uint64_t *__restrict__ key, int64_t n, int32_t *__restrict__ table, int64_t max_dist,
@@ -275,7 +304,13 @@ int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells,
     int64_t st = 1;
     for (int d = G - 1; d >= 0; d--) {
         int64_t nlines = ncells / nslices[d];
-        hipLaunchKernelGGL(k_dt_line, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines, nslices[d], st);
+        if (st == 1 && nslices[d] <= 256) {
+            const int lpb = 8192 / nslices[d] < 256 ? 8192 / nslices[d] : 256;   // <= 64 KB of keys
+            hipLaunchKernelGGL(k_dt_line_contig, dim3(pt_blocks(nlines, lpb)), dim3(256), (size_t)lpb * nslices[d] * 8,
+                               s, key, nlines, nslices[d], lpb);
+        } else {
+            hipLaunchKernelGGL(k_dt_line, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines, nslices[d], st);
+        }
         st *= nslices[d];
     }
     int64_t max_dist = dt_iters > 0 ? (int64_t)dt_iters + 1 : -1;

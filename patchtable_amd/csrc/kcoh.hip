@@ -16,6 +16,7 @@
 // Lists shorter than K repeat their last entry (p itself with SSD 0 when empty).
 #include "pt_device.h"
 #include "pt_kernels.h"
+#include "pt_tile.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -63,16 +64,24 @@ __global__ void k_gather_desc(const float *__restrict__ desc, size_t plane, int 
     for (int c = 0; c < nd; c++) ds[r * nd + c] = desc[c * plane + off];
 }
 
-template <int KMAX>
-__global__ void k_select(const int32_t *__restrict__ perm0, const int32_t *__restrict__ perm1,
-                         const int32_t *__restrict__ pos1, const float *__restrict__ ds0, const float *__restrict__ ds1,
-                         long N, int bew, int nd, int K, int W, int md2, const float *__restrict__ b0, int bw,
-                         int32_t *__restrict__ kpos, float *__restrict__ kdist) {
-    long r0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r0 >= N) return;
-    const int i = perm0[r0];
-    const int px = i % bew, py = i / bew;
-    float own[PT_MAX_DIMS];
+// One lane per database patch p, in 64 x 4 raster tiles whose pixels are staged in LDS for the
+// exact-SSD re-rank. ND is the descriptor width when known at compile time (0: runtime `nd`).
+template <int KMAX, int ND>
+__global__ __launch_bounds__(pt_tile::THREADS) void k_select(
+    const int32_t *__restrict__ perm0, const int32_t *__restrict__ perm1, const int32_t *__restrict__ pos0,
+    const int32_t *__restrict__ pos1, const float *__restrict__ ds0, const float *__restrict__ ds1, long N, int bew,
+    int beh, int nd_rt, int K, int W, int md2, const float *__restrict__ b0, int bw, int bh, int32_t *__restrict__ kpos,
+    float *__restrict__ kdist) {
+    using namespace pt_tile;
+    __shared__ float4 tile[AX * AY];
+    stage_tile(b0, bh, bw, tile);
+    const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
+    const int px = blockIdx.x * TILE_X + lx, py = blockIdx.y * TILE_Y + ly;
+    if (px >= bew || py >= beh) return;
+    const int nd = ND > 0 ? ND : nd_rt;
+    const int i = py * bew + px;
+    float own[ND > 0 ? ND : PT_MAX_DIMS];
+    const long r0 = pos0[i];
     for (int c = 0; c < nd; c++) own[c] = ds0[r0 * nd + c];
     float bd[KMAX];
     int bq[KMAX];
@@ -87,17 +96,18 @@ __global__ void k_select(const int32_t *__restrict__ perm0, const int32_t *__res
     for (int o = 0; o < 2; o++) {
         const int32_t *perm = o == 0 ? perm0 : perm1;
         const float *ds = o == 0 ? ds0 : ds1;
-        long r = o == 0 ? r0 : (long)pos1[i];
-        long t0 = r - W < 0 ? 0 : r - W, t1 = r + W >= N ? N - 1 : r + W;
+        const long r = o == 0 ? r0 : (long)pos1[i];
+        const long t0 = r - W < 0 ? 0 : r - W, t1 = r + W >= N ? N - 1 : r + W;
         for (long t = t0; t <= t1; t++) {
             if (t == r) continue;
-            int qi = perm[t];
-            int qx = qi % bew, qy = qi / bew;
-            int dx = qx - px, dy = qy - py;
+            const int qi = perm[t];
+            const int qx = qi % bew, qy = qi / bew;
+            const int dx = qx - px, dy = qy - py;
             if (dx * dx + dy * dy < md2) continue;
             float dd = 0.f;
+            const float *dq = ds + t * nd;
             for (int c = 0; c < nd; c++) {
-                float e = own[c] - ds[t * nd + c];
+                float e = own[c] - dq[c];
                 dd = dd + e * e;
             }
             // reject if not better than the current K-th entry
@@ -107,7 +117,7 @@ __global__ void k_select(const int32_t *__restrict__ perm0, const int32_t *__res
             for (int k = 0; k < KMAX; k++) dup |= (k < cnt && bq[k] == qi);
             if (dup) continue;
             // insert keeping (dd, qi) ascending
-            int n = cnt < K ? cnt + 1 : K;
+            const int n = cnt < K ? cnt + 1 : K;
             float cd = dd;
             int cq = qi;
 #pragma unroll
@@ -134,40 +144,50 @@ __global__ void k_select(const int32_t *__restrict__ perm0, const int32_t *__res
                 }
         }
     }
-    // exact SSD re-rank
+    // exact SSD re-rank, then insertion by (SSD, XY_TO_INT)
+    const float4 *aq = tile + ly * AX + lx;
     float ss[KMAX];
     int xy[KMAX];
-    const float *pa = b0 + ((size_t)py * bw + px) * 3;
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
-        if (k < cnt) {
-            int qx = bq[k] % bew, qy = bq[k] / bew;
-            ss[k] = pt_patch_ssd(pa, bw * 3, b0 + ((size_t)qy * bw + qx) * 3, bw * 3);
-            xy[k] = PT_XY_TO_INT(qx, qy);
-        }
+        ss[k] = 0.f;
+        xy[k] = 0x7fffffff;
     }
-    for (int a = 1; a < cnt; a++) {   // insertion sort by (ssd, xy)
-        float s = ss[a];
-        int x = xy[a];
-        int b = a - 1;
-        while (b >= 0 && (ss[b] > s || (ss[b] == s && xy[b] > x))) {
-            ss[b + 1] = ss[b];
-            xy[b + 1] = xy[b];
-            b--;
+#pragma unroll
+    for (int a = 0; a < KMAX; a++) {
+        if (a < cnt) {
+            const int qx = bq[a] % bew, qy = bq[a] / bew;
+            float cs = ssd_staged<false>(aq, b0 + ((size_t)qy * bw + qx) * 3, bw * 3, 0.f);
+            int cx = PT_XY_TO_INT(qx, qy);
+#pragma unroll
+            for (int k = 0; k < KMAX; k++) {
+                if (k <= a) {
+                    bool less = k == a || cs < ss[k] || (cs == ss[k] && cx < xy[k]);
+                    if (less) {
+                        float ts = ss[k];
+                        int tx = xy[k];
+                        ss[k] = cs;
+                        xy[k] = cx;
+                        cs = ts;
+                        cx = tx;
+                    }
+                }
+            }
         }
-        ss[b + 1] = s;
-        xy[b + 1] = x;
     }
     int32_t *op = kpos + (size_t)i * K;
     float *od = kdist + (size_t)i * K;
-    for (int k = 0; k < K; k++) {
-        if (cnt == 0) {
-            op[k] = PT_XY_TO_INT(px, py);
-            od[k] = 0.f;
-        } else {
-            int kk = k < cnt ? k : cnt - 1;
-            op[k] = xy[kk];
-            od[k] = ss[kk];
+    int lastxy = PT_XY_TO_INT(px, py);
+    float lasts = 0.f;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        if (k < cnt) {
+            lastxy = xy[k];
+            lasts = ss[k];
+        }
+        if (k < K) {
+            op[k] = lastxy;
+            od[k] = lasts;
         }
     }
 }
@@ -201,9 +221,9 @@ int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, con
     uint64_t *k0s = (uint64_t *)sc.get(22, N * 8), *k1s = (uint64_t *)sc.get(23, N * 8);
     int32_t *idx = (int32_t *)sc.get(24, N * 4);
     int32_t *perm0 = (int32_t *)sc.get(25, N * 4), *perm1 = (int32_t *)sc.get(26, N * 4);
-    int32_t *pos1 = (int32_t *)sc.get(27, N * 4);
+    int32_t *pos1 = (int32_t *)sc.get(27, N * 4), *pos0 = (int32_t *)sc.get(31, N * 4);
     float *ds0 = (float *)sc.get(28, (size_t)N * ndims * 4), *ds1 = (float *)sc.get(29, (size_t)N * ndims * 4);
-    if (!k0 || !k1 || !k0s || !k1s || !idx || !perm0 || !perm1 || !pos1 || !ds0 || !ds1)
+    if (!k0 || !k1 || !k0s || !k1s || !idx || !perm0 || !perm1 || !pos0 || !pos1 || !ds0 || !ds1)
         return pt_fail(PT_E_NOMEM, "kcoherence: scratch allocation failed");
     const int TB = 256;
     hipLaunchKernelGGL(k_keys, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, N, q, ka, k0, k1, idx);
@@ -213,15 +233,23 @@ int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, con
     if (!tmp) return pt_fail(PT_E_NOMEM, "kcoherence: sort scratch allocation failed");
     PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k0s, idx, perm0, (int)N, 0, nbits, s));
     PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k1, k1s, idx, perm1, (int)N, 0, nbits, s));
+    hipLaunchKernelGGL(k_positions, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, perm0, N, pos0);
     hipLaunchKernelGGL(k_positions, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, perm1, N, pos1);
     hipLaunchKernelGGL(k_gather_desc, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, ndims, perm0, N, ds0);
     hipLaunchKernelGGL(k_gather_desc, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, ndims, perm1, N, ds1);
-    if (K <= 16)
-        hipLaunchKernelGGL(k_select<16>, dim3(pt_blocks(N, 64)), dim3(64), 0, s, perm0, perm1, pos1, ds0, ds1, N, bew,
-                           ndims, K, window, min_dist2, b0, bw, kpos, kdist);
-    else
-        hipLaunchKernelGGL(k_select<32>, dim3(pt_blocks(N, 64)), dim3(64), 0, s, perm0, perm1, pos1, ds0, ds1, N, bew,
-                           ndims, K, window, min_dist2, b0, bw, kpos, kdist);
+    dim3 grid((bew + pt_tile::TILE_X - 1) / pt_tile::TILE_X, (beh + pt_tile::TILE_Y - 1) / pt_tile::TILE_Y);
+    dim3 blk(pt_tile::THREADS);
+#define PT_SELECT(KM, ND)                                                                                      \
+    hipLaunchKernelGGL((k_select<KM, ND>), grid, blk, 0, s, perm0, perm1, pos0, pos1, ds0, ds1, N, bew, beh, ndims, \
+                       K, window, min_dist2, b0, bw, bh, kpos, kdist)
+    if (K <= 16) {
+        if (ndims == 6) PT_SELECT(16, 6);
+        else PT_SELECT(16, 0);
+    } else {
+        if (ndims == 6) PT_SELECT(32, 6);
+        else PT_SELECT(32, 0);
+    }
+#undef PT_SELECT
     PT_LAUNCH_CHECK();
     return PT_OK;
 }

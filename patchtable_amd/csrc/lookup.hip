@@ -17,84 +17,20 @@
 // database patch from global memory. The field is packed as XY_TO_INT position + distance, plus
 // a flag recording that the spatial window around the stored best has already been searched
 // without improvement; a later round whose best is unchanged skips that window (it would find
-// nothing strictly closer), which leaves every result unchanged.
+// nothing strictly closer), which leaves every result unchanged. A second flag records whether
+// the best changed in the round that produced the field (used to skip repeated candidates).
 #include "pt_device.h"
 #include "pt_kernels.h"
+#include "pt_tile.h"
 
 #include <vector>
 
 namespace {
 
-constexpr int TILE_X = 64, TILE_Y = 4, THREADS = TILE_X * TILE_Y;
-constexpr int AX = TILE_X + 7, AY = TILE_Y + 7;   // staged query pixels
+using namespace pt_tile;
 
 __constant__ int c_dx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 __constant__ int c_dy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
-
-// Stages the query pixels of this block's tile into LDS, zero outside the image.
-__device__ __forceinline__ void stage_query(const LookupArgs &L, float4 *tile) {
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
-    for (int i = threadIdx.x; i < AX * AY; i += THREADS) {
-        int ly = i / AX, lx = i - ly * AX;
-        int gx = x0 + lx, gy = y0 + ly;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (gx < L.aw && gy < L.ah) {
-            const float *p = L.a + ((size_t)gy * L.aw + gx) * 3;
-            v = make_float4(p[0], p[1], p[2], 0.f);
-        }
-        tile[i] = v;
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ float4 ldg4(const float *p) {
-    float4 v;
-    __builtin_memcpy(&v, p, 16);
-    return v;
-}
-
-#define PT_ACC(e, bv)                                  \
-    {                                                  \
-        float d_ = av[e] - (bv);                       \
-        acc[(e) & 3] = acc[(e) & 3] + d_ * d_;         \
-    }
-
-// Exact SSD of the staged query patch at `aq` (row pitch AX pixels) against the database patch
-// at `b` (row pitch `bpitch` floats), lane order of patch_dist_exact (patchtable.h:330-373):
-// element e of a 24-float row goes to lane e & 3, lanes accumulate in increasing e then row.
-// With `bounded`, the sum is abandoned once a partial sum (after rows 2, 4, 6) reaches `bound`.
-// Rows are processed one at a time (83 VGPRs, 5 waves per SIMD).
-template <bool bounded>
-__device__ __forceinline__ float ssd_staged(const float4 *aq, const float *__restrict__ b, int bpitch, float bound) {
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-    for (int dy = 0; dy < 8; dy++) {
-        const float4 *ar = aq + dy * AX;
-        const float *br = b + (size_t)dy * bpitch;
-        float4 b0 = ldg4(br), b1 = ldg4(br + 4), b2 = ldg4(br + 8), b3 = ldg4(br + 12), b4 = ldg4(br + 16),
-               b5 = ldg4(br + 20);
-        float av[24];
-#pragma unroll
-        for (int px = 0; px < 8; px++) {
-            float4 p = ar[px];
-            av[3 * px + 0] = p.x;
-            av[3 * px + 1] = p.y;
-            av[3 * px + 2] = p.z;
-        }
-        PT_ACC(0, b0.x) PT_ACC(1, b0.y) PT_ACC(2, b0.z) PT_ACC(3, b0.w)
-        PT_ACC(4, b1.x) PT_ACC(5, b1.y) PT_ACC(6, b1.z) PT_ACC(7, b1.w)
-        PT_ACC(8, b2.x) PT_ACC(9, b2.y) PT_ACC(10, b2.z) PT_ACC(11, b2.w)
-        PT_ACC(12, b3.x) PT_ACC(13, b3.y) PT_ACC(14, b3.z) PT_ACC(15, b3.w)
-        PT_ACC(16, b4.x) PT_ACC(17, b4.y) PT_ACC(18, b4.z) PT_ACC(19, b4.w)
-        PT_ACC(20, b5.x) PT_ACC(21, b5.y) PT_ACC(22, b5.z) PT_ACC(23, b5.w)
-        if (bounded && (dy & 1) && dy < 7) {
-            float part = ((acc[0] + acc[1]) + acc[2]) + acc[3];
-            if (part >= bound) return part;
-        }
-    }
-    return ((acc[0] + acc[1]) + acc[2]) + acc[3];
-}
-#undef PT_ACC
 
 // Spatial search (patchtable.h:3861-3885), including the reference's "else if" clamping; the
 // far side is also clamped where the reference would read out of bounds (a database fewer than
@@ -127,7 +63,7 @@ __device__ __forceinline__ bool spatial_search(const LookupArgs &L, const float4
 
 __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField out) {
     __shared__ float4 tile[AX * AY];
-    stage_query(L, tile);
+    stage_tile(L.a, L.ah, L.aw, tile);
     const int ann_w = L.aw - 7, ann_h = L.ah - 7;
     const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
     const int x = blockIdx.x * TILE_X + lx, y = blockIdx.y * TILE_Y + ly;
@@ -165,11 +101,15 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
     out.xy[i] = PT_XY_TO_INT(xb, yb);
     out.d[i] = db;
     out.done[i] = done;
+    out.chg[i] = 1;
 }
 
-__global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, NnfField cur, NnfField nxt) {
+// same_jump: the previous round used the same jump. A candidate from neighbour n then repeats
+// that round's when neither this patch nor n changed in it; it was rejected against the same
+// best, so it is skipped (results unchanged).
+__global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int same_jump, NnfField cur, NnfField nxt) {
     __shared__ float4 tile[AX * AY];
-    stage_query(L, tile);
+    stage_tile(L.a, L.ah, L.aw, tile);
     const int ann_w = L.aw - 7, ann_h = L.ah - 7;
     const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
     const int x = blockIdx.x * TILE_X + lx, y = blockIdx.y * TILE_Y + ly;
@@ -181,10 +121,13 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, NnfF
     int xb = PT_INT_TO_X(v0), yb = PT_INT_TO_Y(v0);
     float db = cur.d[i];
     bool moved = false;
+    const bool stable = same_jump && !cur.chg[i];
     for (int t = 0; t < 8; t++) {
         int qx = x + J * c_dx[t], qy = y + J * c_dy[t];
         if (qx < 0 || qy < 0 || qx >= ann_w || qy >= ann_h) continue;
-        int vq = cur.xy[(long)qy * ann_w + qx];
+        const long qi = (long)qy * ann_w + qx;
+        if (stable && !cur.chg[qi]) continue;
+        int vq = cur.xy[qi];
         int xs = PT_INT_TO_X(vq) - J * c_dx[t], ys = PT_INT_TO_Y(vq) - J * c_dy[t];
         if (xs < 0 || ys < 0 || xs >= bew || ys >= beh) continue;
         if (xs == xb && ys == yb) continue;
@@ -197,10 +140,15 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, NnfF
         }
     }
     bool done = cur.done[i] && !moved;
-    if (L.spatial && !done) done = !spatial_search(L, aq, xb, yb, db);
+    bool schg = false;
+    if (L.spatial && !done) {
+        schg = spatial_search(L, aq, xb, yb, db);
+        done = !schg;
+    }
     nxt.xy[i] = PT_XY_TO_INT(xb, yb);
     nxt.d[i] = db;
     nxt.done[i] = done;
+    nxt.chg[i] = moved || schg;
 }
 
 __global__ void k_write_ann(NnfField f, long n, double *__restrict__ ann) {
@@ -239,8 +187,9 @@ int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, double *ann_
     if (ev_after_candidates) PT_HIP_TRY(hipEventRecord(ev_after_candidates, s));
     std::vector<int> J;
     pt_jump_schedule(L.rounds, J);
-    for (int j : J) {
-        hipLaunchKernelGGL(k_propagate, grid, dim3(THREADS), 0, s, L, j, f0, f1);
+    for (size_t r = 0; r < J.size(); r++) {
+        int same = r > 0 && J[r - 1] == J[r];
+        hipLaunchKernelGGL(k_propagate, grid, dim3(THREADS), 0, s, L, J[r], same, f0, f1);
         std::swap(f0, f1);
     }
     if (ev_after_rounds) PT_HIP_TRY(hipEventRecord(ev_after_rounds, s));

@@ -23,11 +23,56 @@
 
 namespace {
 
-__global__ void k_shuffle_targets(const int32_t *__restrict__ rnd, int n, uint32_t *__restrict__ j, int32_t *__restrict__ m) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    j[i] = i == 0 ? 0u : (uint32_t)(rnd[i - 1] % (i + 1));
-    m[i] = i;
+constexpr int RAND_CHUNK = 256;   // rand() outputs generated per thread
+
+// j_m = rand_{m-1} % (m+1) for every shuffle step m (j_0 = 0), and m itself. Thread c jumps the
+// 31-word generator state to output c*RAND_CHUNK with the matrices A^(RAND_CHUNK*2^i) and then
+// steps the recurrence r[k] = r[k-31] + r[k-3] in blocks of 31 so every state index is static.
+__global__ void k_shuffle_targets(const uint32_t *__restrict__ jump, const uint32_t *__restrict__ state0, int n,
+                                  uint32_t *__restrict__ j, int32_t *__restrict__ m) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const long first = (long)c * RAND_CHUNK;   // output index
+    const long nout = (long)n - 1;             // outputs used by steps 1..n-1
+    if (c == 0) {
+        j[0] = 0;
+        m[0] = 0;
+    }
+    if (first >= nout) return;
+    uint32_t st[31];
+#pragma unroll
+    for (int t = 0; t < 31; t++) st[t] = state0[t];
+    for (int lv = 0; (c >> lv) != 0; lv++) {
+        if (!((c >> lv) & 1)) continue;
+        const uint32_t *P = jump + (size_t)lv * 961;
+        uint32_t nx[31];
+#pragma unroll
+        for (int r = 0; r < 31; r++) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < 31; k++) acc += P[r * 31 + k] * st[k];
+            nx[r] = acc;
+        }
+#pragma unroll
+        for (int t = 0; t < 31; t++) st[t] = nx[t];
+    }
+    const long last = first + RAND_CHUNK < nout ? first + RAND_CHUNK : nout;
+    for (long base = first; base < last; base += 31) {
+#pragma unroll
+        for (int t = 0; t < 31; t++) {
+            st[t] = st[t] + st[(t + 28) % 31];
+            long o = base + t;
+            if (o < last) {
+                uint32_t step = (uint32_t)(o + 1);
+                j[step] = (st[t] >> 1) % (step + 1);
+                m[step] = (int32_t)step;
+            }
+        }
+    }
+}
+
+__global__ void k_list_xy(int bew, int n, int32_t *__restrict__ list) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) list[k] = PT_XY_TO_INT(7 + k % bew, 7 + k / bew);
 }
 
 // From the sorted (j, m) pairs: G of every element and F of every occupied position.
@@ -121,7 +166,14 @@ __global__ void k_fill(int32_t *__restrict__ p, long n, int32_t v) {
 
 }  // namespace
 
-int pt_shuffle_ranks(const int32_t *rnd_dev, int n, int populate_random, PtScratch &sc, int32_t *rank, hipStream_t s) {
+int pt_list_launch(int bew, int n, int32_t *list, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_list_xy, dim3(pt_blocks(n, 256)), dim3(256), 0, s, bew, n, list);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_shuffle_ranks(const uint32_t *jump_dev, const uint32_t *state_dev, int n, int populate_random, PtScratch &sc,
+                     int32_t *rank, hipStream_t s) {
     const int TB = 256;
     if (n <= 0) return PT_OK;
     if (!populate_random) {
@@ -137,7 +189,9 @@ int pt_shuffle_ranks(const int32_t *rnd_dev, int n, int populate_random, PtScrat
     int32_t *F = (int32_t *)sc.get(5, (size_t)n * 4);
     int32_t *s0 = (int32_t *)sc.get(6, (size_t)n * 4);
     if (!j || !m || !js || !ms || !Gk || !F || !s0) return pt_fail(PT_E_NOMEM, "populate: scratch allocation failed");
-    hipLaunchKernelGGL(k_shuffle_targets, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, rnd_dev, n, j, m);
+    const long nchunks = ((long)n - 1 + RAND_CHUNK - 1) / RAND_CHUNK;
+    hipLaunchKernelGGL(k_shuffle_targets, dim3(pt_blocks(nchunks > 0 ? nchunks : 1, 64)), dim3(64), 0, s, jump_dev,
+                       state_dev, n, j, m);
     int bits = 1;
     while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)n) bits++;
     size_t tmp_bytes = 0;

@@ -59,7 +59,14 @@ int pt_quantile_values(const float *desc, size_t plane, int Wp, int h, int w, in
                        int nranks, float *values_dev, PtScratch &sc, hipStream_t s);
 
 // populate.hip
-int pt_shuffle_ranks(const int32_t *rnd_dev, int n, int populate_random, PtScratch &sc, int32_t *rank, hipStream_t s);
+int pt_list_launch(int bew, int n, int32_t *list, hipStream_t s);
+int pt_shuffle_ranks(const uint32_t *jump_dev, const uint32_t *state_dev, int n, int populate_random, PtScratch &sc,
+                     int32_t *rank, hipStream_t s);
+// rand.cpp: glibc generator state before its first output and jump matrices A^(L*2^i).
+void pt_glibc_initial_state(uint32_t seed, uint32_t state[31]);
+const uint32_t *pt_glibc_jump_matrices(int L, int levels);
+#define PT_RAND_CHUNK 256
+#define PT_RAND_LEVELS 24
 int pt_populate_launch(const float *desc, size_t plane, int Wp, const int32_t *list_xy, const int32_t *rank, int n,
                        const DevQuant &q, int32_t *table, int64_t ncells, int64_t *cell_scratch,
                        uint8_t *win_scratch, int *count_dev, hipStream_t s);
@@ -96,12 +103,13 @@ struct LookupArgs {
     int do_table_lookup = 1;
     int rounds = 4;
 };
-// One nearest-neighbour field: XY_TO_INT(b patch x, y), exact SSD, and whether the spatial
-// window around that match was already searched without improvement.
+// One nearest-neighbour field: XY_TO_INT(b patch x, y), exact SSD, whether the spatial window
+// around that match was already searched without improvement, and whether the match changed.
 struct NnfField {
     int32_t *xy = nullptr;
     float *d = nullptr;
     uint8_t *done = nullptr;
+    uint8_t *chg = nullptr;   // best changed in the round that produced this field
 };
 int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, double *ann_out, hipStream_t s,
                      hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds);

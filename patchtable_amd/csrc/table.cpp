@@ -13,7 +13,6 @@
 #include <cstring>
 #include <vector>
 
-void pt_glibc_rand_stream(uint32_t seed, int64_t n, int32_t *out);
 
 #define TRY(expr)                          \
     do {                                   \
@@ -85,7 +84,8 @@ struct pt_table {
     DevBuf<int32_t> qtobin;
     DevBuf<int64_t> qstride;
     DevBuf<int32_t> table;
-    DevBuf<int32_t> list_xy, rank, rnd;
+    DevBuf<int32_t> list_xy, rank;
+    DevBuf<uint32_t> rnd;   // generator state + jump matrices
     DevBuf<int64_t> cells;
     DevBuf<uint8_t> win;
     DevBuf<int> counter;
@@ -96,7 +96,7 @@ struct pt_table {
     DevBuf<float> a, adesc, yuv, temp;
     DevBuf<int32_t> nxy, nxy2;
     DevBuf<float> nd, nd2;
-    DevBuf<uint8_t> ndone, ndone2;
+    DevBuf<uint8_t> ndone, ndone2, nchg, nchg2;
     DevBuf<double> ann;
     int qh = 0, qw = 0;
     PtScratch sc;
@@ -240,13 +240,6 @@ int upload_quant(pt_table *t) {
     return PT_OK;
 }
 
-__attribute__((unused)) int build_list(pt_table *t, std::vector<int32_t> &list) {
-    list.clear();
-    for (int y = 7; y < t->h; y++)
-        for (int x = 7; x < t->w; x++) list.push_back(PT_XY_TO_INT(x, y));
-    return PT_OK;
-}
-
 int build_table(pt_table *t, const float *b_dev) {
     const pt_params &p = t->build;
     const int h = t->h, w = t->w, Hp = h + 7, Wp = w + 7;
@@ -328,16 +321,20 @@ int build_table(pt_table *t, const float *b_dev) {
     const int bew = w - 7, beh = h - 7;
     const int n = bew * beh;
     {
-        std::vector<int32_t> list;
-        build_list(t, list);
         TRY(t->list_xy.ensure(n));
-        HIPTRY(hipMemcpyAsync(t->list_xy.p, list.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
-        std::vector<int32_t> rnd(n > 1 ? n - 1 : 1);
-        if (p.populate_random && n > 1) pt_glibc_rand_stream(1, n - 1, rnd.data());
-        TRY(t->rnd.ensure(rnd.size()));
-        HIPTRY(hipMemcpyAsync(t->rnd.p, rnd.data(), rnd.size() * 4, hipMemcpyHostToDevice, s));
+        TRY(pt_list_launch(bew, n, t->list_xy.p, s));
+        if (p.populate_random) {
+            if ((int64_t)n > ((int64_t)PT_RAND_CHUNK << PT_RAND_LEVELS))
+                return pt_fail(PT_E_UNSUPPORTED, "too many database patches for the rand() jump tables");
+            uint32_t st[31];
+            pt_glibc_initial_state(1, st);
+            const uint32_t *jm = pt_glibc_jump_matrices(PT_RAND_CHUNK, PT_RAND_LEVELS);
+            TRY(t->rnd.ensure(31 + (size_t)PT_RAND_LEVELS * 961));
+            HIPTRY(hipMemcpyAsync(t->rnd.p, st, 31 * 4, hipMemcpyHostToDevice, s));
+            HIPTRY(hipMemcpyAsync(t->rnd.p + 31, jm, (size_t)PT_RAND_LEVELS * 961 * 4, hipMemcpyHostToDevice, s));
+        }
         TRY(t->rank.ensure(n));
-        TRY(pt_shuffle_ranks(t->rnd.p, n, p.populate_random, t->sc, t->rank.p, s));
+        TRY(pt_shuffle_ranks(t->rnd.p + 31, t->rnd.p, n, p.populate_random, t->sc, t->rank.p, s));
         TRY(t->table.ensure((size_t)t->ncells));
         TRY(t->cells.ensure(n));
         TRY(t->win.ensure(n));
@@ -452,6 +449,8 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     TRY(t->nxy2.ensure(n));
     TRY(t->nd2.ensure(n));
     TRY(t->ndone2.ensure(n));
+    TRY(t->nchg.ensure(n));
+    TRY(t->nchg2.ensure(n));
     t->qh = h;
     t->qw = w;
     double T0 = now_s();
@@ -481,9 +480,11 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     f0.xy = t->nxy.p;
     f0.d = t->nd.p;
     f0.done = t->ndone.p;
+    f0.chg = t->nchg.p;
     f1.xy = t->nxy2.p;
     f1.d = t->nd2.p;
     f1.done = t->ndone2.p;
+    f1.chg = t->nchg2.p;
     TRY(pt_lookup_launch(L, f0, f1, ann_dev, s, t->ev[8], t->ev[10]));
     HIPTRY(hipEventRecord(t->ev[9], s));
     HIPTRY(hipStreamSynchronize(s));
