@@ -32,24 +32,86 @@ using namespace pt_tile;
 __constant__ int c_dx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 __constant__ int c_dy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 
-// Spatial search (patchtable.h:3861-3885), including the reference's "else if" clamping; the
-// far side is also clamped where the reference would read out of bounds (a database fewer than
-// 2r+1 patches tall or wide). Returns true if the best changed.
-__device__ __forceinline__ bool spatial_search(const LookupArgs &L, const float4 *aq, int &xb, int &yb, float &db) {
+// ---- Block work queue -------------------------------------------------------------------------
+// A lane owns one query patch. Instead of each lane looping over its own candidates (a wave then
+// waits for its slowest lane), the lanes of a block push their candidate SSDs to a queue in LDS,
+// all lanes of the block evaluate the queue, and each lane reduces its own results in candidate
+// order. Every task is bounded by its patch's best distance at the start of the phase, so a
+// result is exact whenever it is below that bound; reducing in order with strict "<" then picks
+// the same first minimum as evaluating the candidates one after another against the running best.
+constexpr int SLOTS = 8;   // tasks per lane and phase
+
+struct BlockQueue {
+    unsigned short id[THREADS * SLOTS];   // queued task ids: lane * SLOTS + slot
+    int cxy[THREADS * SLOTS];             // candidate XY_TO_INT, by task id
+    float res[THREADS * SLOTS];           // SSD, by task id
+    float bound[THREADS];                 // per lane
+    int n;
+};
+
+// Queues the calling lane's task `slot` when `pred`. Called by every lane of the wave (the
+// ballot needs them all); one LDS atomic per wave.
+__device__ __forceinline__ void q_push(BlockQueue &Q, bool pred, int slot, int cand) {
+    const unsigned long long m = __ballot(pred);
+    if (m == 0) return;
+    const int wl = threadIdx.x & 63;
+    const int leader = __builtin_ctzll(m);
+    int base = 0;
+    if (wl == leader) base = atomicAdd(&Q.n, __popcll(m));
+    base = __shfl(base, leader);
+    if (pred) {
+        const int pos = base + __popcll(m & ((1ull << wl) - 1));
+        Q.id[pos] = (unsigned short)(threadIdx.x * SLOTS + slot);
+        Q.cxy[threadIdx.x * SLOTS + slot] = cand;
+    }
+}
+
+// Evaluates every queued task; on return the results are visible to all lanes and the queue is
+// empty again.
+__device__ __forceinline__ void q_run(BlockQueue &Q, const float4 *tile, const LookupArgs &L) {
+    __syncthreads();
+    const int n = Q.n;
+    const int bpitch = L.bw * 3;
+    for (int k = threadIdx.x; k < n; k += THREADS) {
+        const int id = Q.id[k];
+        const int lane = id / SLOTS;
+        const int c = Q.cxy[id];
+        const float4 *aq = tile + (lane / TILE_X) * AX + (lane % TILE_X);
+        Q.res[id] = ssd_staged<true>(aq, L.b + ((size_t)PT_INT_TO_Y(c) * L.bw + PT_INT_TO_X(c)) * 3, bpitch,
+                                     Q.bound[lane]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) Q.n = 0;
+    __syncthreads();
+}
+
+// Spatial search (patchtable.h:3861-3885) around (xb, yb), including the reference's "else if"
+// clamping; the far side is also clamped where the reference would read out of bounds (a
+// database fewer than 2r+1 patches tall or wide). Window positions are visited row by row.
+struct Window {
+    int ymin, ymax, xmin, xmax, xb0, yb0;
+};
+
+__device__ __forceinline__ Window spatial_window(const LookupArgs &L, int xb, int yb) {
     const int bew = L.bw - 7, beh = L.bh - 7, r = L.spatial;
-    int symin = yb - r, symax = yb + r + 1, sxmin = xb - r, sxmax = xb + r + 1;
-    const int xb0 = xb, yb0 = yb;
-    if (symin < 0) symin = 0;
-    else if (symax > beh) symax = beh;
-    if (sxmin < 0) sxmin = 0;
-    else if (sxmax > bew) sxmax = bew;
-    if (symax > beh) symax = beh;
-    if (sxmax > bew) sxmax = bew;
+    Window w{yb - r, yb + r + 1, xb - r, xb + r + 1, xb, yb};
+    if (w.ymin < 0) w.ymin = 0;
+    else if (w.ymax > beh) w.ymax = beh;
+    if (w.xmin < 0) w.xmin = 0;
+    else if (w.xmax > bew) w.xmax = bew;
+    if (w.ymax > beh) w.ymax = beh;
+    if (w.xmax > bew) w.xmax = bew;
+    return w;
+}
+
+// Sequential form, for spatial radius > 1 (more than SLOTS positions). Returns true if improved.
+__device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const float4 *aq, int &xb, int &yb, float &db) {
+    const Window w = spatial_window(L, xb, yb);
     const int bpitch = L.bw * 3;
     bool changed = false;
-    for (int ys = symin; ys < symax; ys++)
-        for (int xs = sxmin; xs < sxmax; xs++) {
-            if (xs == xb0 && ys == yb0) continue;
+    for (int ys = w.ymin; ys < w.ymax; ys++)
+        for (int xs = w.xmin; xs < w.xmax; xs++) {
+            if (xs == w.xb0 && ys == w.yb0) continue;
             float d = ssd_staged<true>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
             if (d < db) {
                 db = d;
@@ -61,42 +123,111 @@ __device__ __forceinline__ bool spatial_search(const LookupArgs &L, const float4
     return changed;
 }
 
+// Spatial search through the block queue (radius 1: at most 8 positions). `want` is false for
+// lanes that skip it; every lane of the block must call this. Returns true if improved.
+__device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *tile, const LookupArgs &L, bool want,
+                                                 int &xb, int &yb, float &db) {
+    Window w = spatial_window(L, xb, yb);
+    if (!want) w.ymax = w.ymin;   // empty
+    if (want) Q.bound[threadIdx.x] = db;
+    int slot = 0;
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            if (dx == 0 && dy == 0) continue;
+            // slot order = row-by-row order of the window positions other than the centre
+            const int ys = w.yb0 + dy, xs = w.xb0 + dx;
+            const bool in = ys >= w.ymin && ys < w.ymax && xs >= w.xmin && xs < w.xmax;
+            q_push(Q, in, slot, PT_XY_TO_INT(xs, ys));
+            slot++;
+        }
+    q_run(Q, tile, L);
+    bool changed = false;
+    if (want) {
+        int s = 0;
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                if (dx == 0 && dy == 0) continue;
+                const int ys = w.yb0 + dy, xs = w.xb0 + dx;
+                if (ys >= w.ymin && ys < w.ymax && xs >= w.xmin && xs < w.xmax) {
+                    const float d = Q.res[threadIdx.x * SLOTS + s];
+                    if (d < db) {
+                        db = d;
+                        xb = xs;
+                        yb = ys;
+                        changed = true;
+                    }
+                }
+                s++;
+            }
+    }
+    return changed;
+}
+
 __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField out) {
     __shared__ float4 tile[AX * AY];
+    __shared__ BlockQueue Q;
+    if (threadIdx.x == 0) Q.n = 0;
     stage_tile(L.a, L.ah, L.aw, tile);
     const int ann_w = L.aw - 7, ann_h = L.ah - 7;
     const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
     const int x = blockIdx.x * TILE_X + lx, y = blockIdx.y * TILE_Y + ly;
-    if (x >= ann_w || y >= ann_h) return;
+    const bool valid = x < ann_w && y < ann_h;
     const float4 *aq = tile + ly * AX + lx;
     const int bpitch = L.bw * 3, bew = L.bw - 7;
-    // table cell of the query descriptor at (y+7, x+7)
-    const size_t plane = (size_t)(L.ah + 7) * (L.aw + 7);
-    const size_t off = (size_t)(y + 7) * (L.aw + 7) + x + 7;
-    int64_t cell = 0;
-    for (int d = 0; d < L.q.G; d++)
-        cell += (int64_t)pt_get_bin(L.a_desc[d * plane + off], L.q.min_val[d], L.q.range[d],
-                                    L.q.to_bin + (size_t)d * PT_NMAP) *
-                L.q.stride[d];
-    int v = L.table[cell];
-    int xb = PT_INT_TO_X(v) - 7, yb = PT_INT_TO_Y(v) - 7;
-    float db = ssd_staged<false>(aq, L.b + ((size_t)yb * L.bw + xb) * 3, bpitch, 0.f);
+    int xb = 0, yb = 0;
+    float db = 0.f;
+    if (valid) {
+        // table cell of the query descriptor at (y+7, x+7)
+        const size_t plane = (size_t)(L.ah + 7) * (L.aw + 7);
+        const size_t off = (size_t)(y + 7) * (L.aw + 7) + x + 7;
+        int64_t cell = 0;
+        for (int d = 0; d < L.q.G; d++)
+            cell += (int64_t)pt_get_bin(L.a_desc[d * plane + off], L.q.min_val[d], L.q.range[d],
+                                        L.q.to_bin + (size_t)d * PT_NMAP) *
+                    L.q.stride[d];
+        const int v = L.table[cell];
+        xb = PT_INT_TO_X(v) - 7;
+        yb = PT_INT_TO_Y(v) - 7;
+        db = ssd_staged<false>(aq, L.b + ((size_t)yb * L.bw + xb) * 3, bpitch, 0.f);
+    }
     if (L.K > 0 && L.kcoh_iter0) {
-        const int32_t *kp = L.kpos + ((size_t)yb * bew + xb) * L.K;
-        const float *kd = L.kdist + ((size_t)yb * bew + xb) * L.K;
-        for (int k = 0; k < L.K; k++) {
-            if ((double)kd[k] >= L.triangle_factor * (double)db) break;
-            int xs = PT_INT_TO_X(kp[k]), ys = PT_INT_TO_Y(kp[k]);
-            float d = ssd_staged<true>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
-            if (d < db) {
-                db = d;
-                xb = xs;
-                yb = ys;
+        // The k-coherence list of the table candidate is walked in order and abandoned at the
+        // first entry with kd >= triangle_factor * dbest (patchtable.h:3779-3784); kd is sorted,
+        // so per chunk of SLOTS entries the queued superset is those with kd below the bound at
+        // the chunk's start, and the in-order reduce replays the early-out exactly.
+        const size_t lst = ((size_t)yb * bew + xb) * L.K;
+        bool broken = !valid;
+        for (int k0 = 0; k0 < L.K; k0 += SLOTS) {
+            const double tb = L.triangle_factor * (double)db;
+            if (!broken) Q.bound[threadIdx.x] = db;
+            for (int s = 0; s < SLOTS; s++) {
+                const int k = k0 + s;
+                bool pred = !broken && k < L.K && (double)L.kdist[lst + k] < tb;
+                int pos = pred ? L.kpos[lst + k] : 0;
+                q_push(Q, pred, s, pos);
+            }
+            q_run(Q, tile, L);
+            for (int s = 0; s < SLOTS && !broken; s++) {
+                const int k = k0 + s;
+                if (k >= L.K) break;
+                if ((double)L.kdist[lst + k] >= L.triangle_factor * (double)db) {
+                    broken = true;
+                    break;
+                }
+                const float d = Q.res[threadIdx.x * SLOTS + s];
+                if (d < db) {
+                    db = d;
+                    const int c = Q.cxy[threadIdx.x * SLOTS + s];
+                    xb = PT_INT_TO_X(c);
+                    yb = PT_INT_TO_Y(c);
+                }
             }
         }
     }
     bool done = false;
-    if (L.spatial) done = !spatial_search(L, aq, xb, yb, db);
+    if (L.spatial == 1) done = !spatial_search_q(Q, tile, L, valid, xb, yb, db);
+    else if (L.spatial > 1 && valid) done = !spatial_search_seq(L, aq, xb, yb, db);
+    if (!valid) return;
     const long i = (long)y * ann_w + x;
     out.xy[i] = PT_XY_TO_INT(xb, yb);
     out.d[i] = db;
@@ -109,42 +240,69 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
 // best, so it is skipped (results unchanged).
 __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int same_jump, NnfField cur, NnfField nxt) {
     __shared__ float4 tile[AX * AY];
+    __shared__ BlockQueue Q;
+    if (threadIdx.x == 0) Q.n = 0;
     stage_tile(L.a, L.ah, L.aw, tile);
     const int ann_w = L.aw - 7, ann_h = L.ah - 7;
     const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
     const int x = blockIdx.x * TILE_X + lx, y = blockIdx.y * TILE_Y + ly;
-    if (x >= ann_w || y >= ann_h) return;
+    const bool valid = x < ann_w && y < ann_h;
     const float4 *aq = tile + ly * AX + lx;
-    const int bpitch = L.bw * 3, bew = L.bw - 7, beh = L.bh - 7;
+    const int bew = L.bw - 7, beh = L.bh - 7;
     const long i = (long)y * ann_w + x;
-    const int v0 = cur.xy[i];
-    int xb = PT_INT_TO_X(v0), yb = PT_INT_TO_Y(v0);
-    float db = cur.d[i];
-    bool moved = false;
-    const bool stable = same_jump && !cur.chg[i];
+    int xb = 0, yb = 0;
+    float db = 0.f;
+    bool stable = false, was_done = false;
+    if (valid) {
+        const int v0 = cur.xy[i];
+        xb = PT_INT_TO_X(v0);
+        yb = PT_INT_TO_Y(v0);
+        db = cur.d[i];
+        stable = same_jump && !cur.chg[i];
+        was_done = cur.done[i];
+        Q.bound[threadIdx.x] = db;
+    }
+    unsigned pushed = 0;
     for (int t = 0; t < 8; t++) {
-        int qx = x + J * c_dx[t], qy = y + J * c_dy[t];
-        if (qx < 0 || qy < 0 || qx >= ann_w || qy >= ann_h) continue;
-        const long qi = (long)qy * ann_w + qx;
-        if (stable && !cur.chg[qi]) continue;
-        int vq = cur.xy[qi];
-        int xs = PT_INT_TO_X(vq) - J * c_dx[t], ys = PT_INT_TO_Y(vq) - J * c_dy[t];
-        if (xs < 0 || ys < 0 || xs >= bew || ys >= beh) continue;
-        if (xs == xb && ys == yb) continue;
-        float d = ssd_staged<true>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
+        const int qx = x + J * c_dx[t], qy = y + J * c_dy[t];
+        bool pred = valid && qx >= 0 && qy >= 0 && qx < ann_w && qy < ann_h;
+        int xs = 0, ys = 0;
+        if (pred) {
+            const long qi = (long)qy * ann_w + qx;
+            if (stable && !cur.chg[qi]) pred = false;
+            else {
+                const int vq = cur.xy[qi];
+                xs = PT_INT_TO_X(vq) - J * c_dx[t];
+                ys = PT_INT_TO_Y(vq) - J * c_dy[t];
+                pred = xs >= 0 && ys >= 0 && xs < bew && ys < beh && !(xs == xb && ys == yb);
+            }
+        }
+        if (pred) pushed |= 1u << t;
+        q_push(Q, pred, t, PT_XY_TO_INT(xs, ys));
+    }
+    q_run(Q, tile, L);
+    bool moved = false;
+    for (int t = 0; t < 8; t++) {
+        if (!((pushed >> t) & 1u)) continue;
+        const float d = Q.res[threadIdx.x * SLOTS + t];
         if (d < db) {
             db = d;
-            xb = xs;
-            yb = ys;
+            const int c = Q.cxy[threadIdx.x * SLOTS + t];
+            xb = PT_INT_TO_X(c);
+            yb = PT_INT_TO_Y(c);
             moved = true;
         }
     }
-    bool done = cur.done[i] && !moved;
+    bool done = was_done && !moved;
     bool schg = false;
-    if (L.spatial && !done) {
-        schg = spatial_search(L, aq, xb, yb, db);
+    if (L.spatial == 1) {
+        schg = spatial_search_q(Q, tile, L, valid && !done, xb, yb, db);
+        if (valid && !done) done = !schg;
+    } else if (L.spatial > 1 && valid && !done) {
+        schg = spatial_search_seq(L, aq, xb, yb, db);
         done = !schg;
     }
+    if (!valid) return;
     nxt.xy[i] = PT_XY_TO_INT(xb, yb);
     nxt.d[i] = db;
     nxt.done[i] = done;

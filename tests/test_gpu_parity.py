@@ -119,7 +119,7 @@ def test_table_build_speed1_full_size_cfg1(pair_cfg1):
     assert np.array_equal(gp, op) and np.array_equal(gd.view(np.uint32), od.view(np.uint32))
 
 
-@pytest.mark.parametrize("rounds", [0, 1, 4, 6])
+@pytest.mark.parametrize("rounds", [0, 1, 4, 8])
 def test_lookup_bit_exact_vs_parallel_model(pair_cfg1, rounds):
     a, b = pair_cfg1
     p = speed1(f"-limit 1 -jf_rounds {rounds}")
