@@ -45,6 +45,33 @@ def propagate_alg_bytes(ah, aw, bh, bw):
     return 24 * nq + 12 * ah * aw + 12 * bh * bw
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_propagate_2048x2048_traffic.md")
+
+
+def measured_traffic(H, W):
+    """HBM bytes per k_propagate launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary for this workload (already scaled by the copy-kernel calibration), or None."""
+    if (H, W) != (2048, 2048) or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
+def copy_probe(torch, mib=1024, reps=3):
+    """A plain device-to-device copy of known size, for calibrating HBM byte counters: each
+    dispatch reads and writes exactly `mib` MiB."""
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: {reps} copies of {mib} MiB")
+    del src, dst
+
+
 def cpu_baseline(h=768, w=1024, seed=2):
     """Oracle (CPU restatement of the reference, 1 thread) on a bounded pair; returns its lookup
     Mpatch/s, build ms and raster mean SSD, plus the GPU mean SSD on the same pair."""
@@ -76,6 +103,8 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--build-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (counter calibration for the traffic figure)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -141,7 +170,7 @@ def main():
     prop_bytes = propagate_alg_bytes(H, W, H, W)
     achieved = prop_bytes / (prop_launch_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "kernel": "k_propagate", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(H, W),
                 "alg_bytes_per_launch": prop_bytes, "avg_launch_ms": round(prop_launch_ms, 4),
                 "launches_per_step": launches}
     lk_bytes = lookup_alg_bytes(H, W, H, W, info["kcoherence"])
@@ -173,6 +202,9 @@ def main():
         "mean_dist": round(mean_dist, 6),
         "roofline": roofline,
     }
+
+    if args.copy_probe:
+        copy_probe(torch)
 
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle (1 thread) on a 1024x768 pair, seed 2")
