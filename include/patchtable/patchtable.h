@@ -89,6 +89,27 @@ class Array {
     }
 };
 
+/* load_color_image<real> (solver/array.h:1637-1684): .png as 8-bit RGB * (1/255.0), .pfm as
+ * stored; h x w x 3 (h x w x 1 for a one-channel .pfm). */
+template <class real>
+Array<real> load_color_image(const char *filename) {
+    int h = 0, w = 0, c = 0;
+    patchtable_amd_detail::check(pt_image_size(filename, &h, &w, &c));
+    std::vector<float> f((size_t)h * w * c);
+    patchtable_amd_detail::check(pt_image_load(filename, f.data(), h, w, c));
+    Array<real> ans(h, w, c);
+    for (size_t i = 0; i < f.size(); i++) ans.data[i] = (real)f[i];
+    return ans;
+}
+
+/* save_color_image (solver/array.h:1745-1827) for .pfm files of 1 or 3 channels (an NNF). */
+template <class real>
+void save_color_image(const Array<real> &a, const char *filename) {
+    std::vector<double> d((size_t)a.nelems);
+    for (size_t i = 0; i < d.size(); i++) d[i] = (double)a.data[i];
+    patchtable_amd_detail::check(pt_pfm_save(filename, d.data(), a.height(), a.width(), a.channels()));
+}
+
 /* Parameters: same field names as the reference; defaults, speed presets and switch parsing are
  * the library's restatement of patchtable.cpp:26-387. */
 class PatchTableParams : public pt_params {

@@ -205,10 +205,26 @@ int pt_table_lookup(pt_table *t, const float *a, int h, int w, int c, double *an
 int pt_table_lookup_dev(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_out_dev,
                         const double *ann_prev_dev, const float *sv_dev, double *seconds);
 
-/* ---- Persistence (patchtable.h:1574-1622; patchtable.cpp:499-573) ---------------------- */
+/* ---- Persistence (patchtable.h:1574-1622; patchtable.cpp:499-573) ----------------------
+ * The file is the reference's: "PatchTable", table, b0, wh0, nslices, PatchPartition of one
+ * PatchPartition1D per grid dimension, "EndPatchTable" (strings and vectors length-prefixed
+ * with int32, Arrays as their sizes vector then the data), so files move both ways.
+ * pt_table_load (PatchTableFixedN(p, filename), patchtable.h:2591-2594; also reached through
+ * pt_table_create when p->load_filename is set) takes the grid dimensions from the file and,
+ * since the format holds no k-coherence lists, rebuilds them on the GPU when p->kcoherence > 0. */
 int pt_table_save(pt_table *t, const char *filename);
 int pt_table_load(pt_params *p, const char *filename, pt_table **out);
 int pt_table_destroy(pt_table *t);
+
+/* ---- Image and NNF files (solver/array.h:1637-1684, 1745-1827; solver/pfm.cpp) -----------
+ * pt_image_size: height, width and channels of a .png (always 3) or .pfm file.
+ * pt_image_load: load_color_image<float>: PNG as 8-bit RGB * (1/255.0), PFM as stored (rows
+ *   flipped to top-first); out is h x w x c floats.
+ * pt_pfm_save: save_color_image<double> to .pfm of a 1 or 3 channel h x w x c array (an NNF:
+ *   c = 3), values converted to float, rows written bottom-first, little endian. */
+int pt_image_size(const char *path, int *h, int *w, int *c);
+int pt_image_load(const char *path, float *out, int h, int w, int c);
+int pt_pfm_save(const char *path, const double *data, int h, int w, int c);
 
 /* ---- Errors ------------------------------------------------------------------------------ */
 const char *pt_last_error(void);
@@ -220,6 +236,8 @@ const char *pt_last_error(void);
  * exact SSD) (patchtable.h:1512-1521). */
 int pt_table_info(const pt_table *t, int *grid_ndims, int *nslices /* [PT_MAX_DIMS] */,
                   int64_t *ncells, int *table_add_count, int *kcoherence);
+/* Database image height and width (b0) and descriptor dimensions of a built or loaded table. */
+int pt_table_shape(const pt_table *t, int *h, int *w, int *ndims);
 int pt_table_get_descriptor(const pt_table *t, float *wh0_out);
 int pt_table_get_partition(const pt_table *t, int dim, float *min_val, float *max_val,
                            float *centers /* nslices[dim] */, int32_t *to_bin /* 16384 */);

@@ -1304,6 +1304,44 @@ int orc_get_kcoherence(void *t, int32_t *pos, float *dist) {
     return 0;
 }
 
+// PatchTableFixedN::save (patchtable.h:1574-1582) with the serialisation of patchtable.cpp:499-558:
+// strings and vectors prefixed with their int32 length, Arrays with their sizes vector.
+int orc_save(void *t, const char *path) {
+    Table *T = (Table *)t;
+    FILE *f = fopen(path, "wb");
+    if (!f) return -1;
+    auto raw = [&](const void *d, size_t n) { fwrite(d, 1, n, f); };
+    auto i32 = [&](int32_t v) { raw(&v, 4); };
+    auto str = [&](const char *s) { i32((int32_t)strlen(s)); raw(s, strlen(s)); };
+    auto ivecw = [&](const std::vector<int> &v) { i32((int32_t)v.size()); raw(v.data(), v.size() * 4); };
+    auto fvecw = [&](const fvec &v) { i32((int32_t)v.size()); raw(v.data(), v.size() * 4); };
+    str("PatchTable");
+    ivecw(T->nslices);
+    raw(T->table.data(), T->table.size() * 4);
+    ivecw({T->h, T->w, 3});
+    raw(T->b0.data(), T->b0.size() * 4);
+    ivecw({T->h + 7, T->w + 7, T->ndims});
+    raw(T->wh0.data(), T->wh0.size() * 4);
+    ivecw(T->nslices);
+    str("PatchPartition");
+    i32(T->G);
+    for (const Partition1D &P : T->part) {
+        str("PatchPartition1D");
+        ivecw(P.to_bin);
+        fvecw(P.centers);
+        fvecw(P.lsizes);
+        fvecw(P.rsizes);
+        raw(&P.min_val, 4);
+        raw(&P.max_val, 4);
+        raw(&P.range, 4);
+        i32(P.nslices);
+        str("EndPatchPartition1D");
+    }
+    str("EndPatchPartition");
+    str("EndPatchTable");
+    return fclose(f) == 0 ? 0 : -1;
+}
+
 // Re-run a distance transform on a given populated table (tests of the DT variants).
 // algo: 0 = prop BFS, 1 = prop separable, 2 = downsample, 3 = euclidean.
 int orc_dt(int G, const int *nslices, int32_t *table, int algo, int dt_iters) {

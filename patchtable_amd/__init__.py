@@ -21,7 +21,7 @@ NNF_Y = 1
 NNF_DIST = 2
 
 __all__ = ["PatchTableParams", "PatchTable", "PatchTableError", "NNF_X", "NNF_Y", "NNF_DIST",
-           "synth_pair"]
+           "synth_pair", "load_color_image", "save_color_image"]
 
 
 class PatchTableParams:
@@ -107,6 +107,9 @@ class PatchTable:
         self.p = p
         self._t = ctypes.c_void_p()
         check(load().pt_table_load(ctypes.byref(p.raw), filename.encode(), ctypes.byref(self._t)))
+        h, w, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(load().pt_table_shape(self._t, ctypes.byref(h), ctypes.byref(w), ctypes.byref(c)))
+        self.b_shape = (h.value, w.value, 3)
         return self
 
     def close(self):
@@ -219,6 +222,26 @@ class PatchTable:
 
 def _is_device_tensor(x):
     return hasattr(x, "is_cuda") and bool(getattr(x, "is_cuda"))
+
+
+def load_color_image(filename):
+    """load_color_image<float> (solver/array.h:1637-1684): a .png as 8-bit RGB * (1/255.0), a
+    .pfm as stored; returns h x w x 3 (h x w x 1 for a one-channel .pfm) float32."""
+    h, w, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(load().pt_image_size(filename.encode(), ctypes.byref(h), ctypes.byref(w), ctypes.byref(c)))
+    out = np.empty((h.value, w.value, c.value), np.float32)
+    check(load().pt_image_load(filename.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               h.value, w.value, c.value))
+    return out
+
+
+def save_color_image(a, filename):
+    """save_color_image (solver/array.h:1745-1827) of a 1 or 3 channel array (an NNF) to .pfm."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim == 2:
+        a = a[..., None]
+    check(load().pt_pfm_save(filename.encode(), a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             a.shape[0], a.shape[1], a.shape[2]))
 
 
 def synth_pair(seed, h, w):

@@ -140,6 +140,10 @@ SIGNATURES = {
     "pt_gck_dev": (_int, [_vp, _int, _int, _int, _int, _int, _vp]),
     "pt_patch_ssd_dev": (_int, [_vp, _int, _vp, _int, _vp, _vp, _vp, _vp, _int, _vp]),
     "pt_synth_pair": (_int, [ctypes.c_uint64, _int, _int, _f32p, _f32p]),
+    "pt_table_shape": (_int, [_vp, _P(_int), _P(_int), _P(_int)]),
+    "pt_image_size": (_int, [ctypes.c_char_p, _P(_int), _P(_int), _P(_int)]),
+    "pt_image_load": (_int, [ctypes.c_char_p, _f32p, _int, _int, _int]),
+    "pt_pfm_save": (_int, [ctypes.c_char_p, _f64p, _int, _int, _int]),
 }
 
 _lib = None

@@ -12,6 +12,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpatchtable_hip.so")
+CLI = os.path.join(HERE, "patchtable")
+CLI_SRC = os.path.join(HERE, "cli", "patchtable_main.cpp")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -46,8 +49,12 @@ def build(jobs=8):
     for p in procs:
         if p.wait() != 0:
             raise RuntimeError("compilation failed")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", OUT + ".tmp", *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", OUT + ".tmp", *objs, "-lz"])
     os.replace(OUT + ".tmp", OUT)
+    # the command-line front end (host code only) links the library by its in-tree path
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, CLI_SRC, "-o", CLI + ".tmp",
+          "-L" + HERE, "-lpatchtable_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(CLI + ".tmp", CLI)
     return OUT
 
 

@@ -10,6 +10,13 @@
 int pt_fail(int code, const char *msg);
 int pt_failf(int code, const char *fmt, ...);
 
+// Returns from the calling function when `expr` yields a status other than PT_OK.
+#define TRY_PT(expr)                      \
+    do {                                  \
+        int rc_ = (expr);                 \
+        if (rc_ != PT_OK) return rc_;     \
+    } while (0)
+
 // Padded-coordinate packing of the reference (XY_TO_INT / INT_TO_X / INT_TO_Y,
 // patchtable.h:163-165).
 #define PT_XY_TO_INT(x, y) ((x) | ((y) << 16))

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 
@@ -118,6 +119,14 @@ struct pt_table {
 };
 
 namespace {
+
+int init_stream(pt_table *t) {
+    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
+        return pt_fail(PT_E_HIP, "hipStreamCreate failed");
+    for (int i = 0; i < 12; i++)
+        if (hipEventCreate(&t->ev[i]) != hipSuccess) return pt_fail(PT_E_HIP, "hipEventCreate failed");
+    return PT_OK;
+}
 
 int check_build_params(const pt_params *p, int c) {
     if (p->is_descriptor) return pt_fail(PT_E_UNSUPPORTED, "is_descriptor mode is not supported by this build yet");
@@ -404,11 +413,7 @@ int create_common(pt_params *p, const float *b, bool b_on_device, int h, int w, 
     t->ndims = p->ndims;
     t->nchroma = p->nchroma;
     t->G = p->grid_ndims;
-    int rc = PT_OK;
-    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
-        rc = pt_fail(PT_E_HIP, "hipStreamCreate failed");
-    for (int i = 0; rc == PT_OK && i < 12; i++)
-        if (hipEventCreate(&t->ev[i]) != hipSuccess) rc = pt_fail(PT_E_HIP, "hipEventCreate failed");
+    int rc = init_stream(t);
     DevBuf<float> staging;
     const float *bd = b;
     if (rc == PT_OK && !b_on_device) {
@@ -542,17 +547,259 @@ extern "C" int pt_table_destroy(pt_table *t) {
     return PT_OK;
 }
 
-extern "C" int pt_table_save(pt_table *t, const char *filename) {
-    (void)t;
-    (void)filename;
-    return pt_fail(PT_E_UNSUPPORTED, "save is not supported by this build yet");
+extern "C" int pt_table_shape(const pt_table *t, int *h, int *w, int *ndims) {
+    if (!t) return pt_fail(PT_E_INVALID, "null table");
+    if (h) *h = t->h;
+    if (w) *w = t->w;
+    if (ndims) *ndims = t->ndims;
+    return PT_OK;
 }
 
+static int planes_to_hwc(const float *planes, int Hp, int Wp, int C, float *out) {
+    std::vector<float> host((size_t)Hp * Wp * C);
+    HIPTRY(hipMemcpy(host.data(), planes, host.size() * 4, hipMemcpyDeviceToHost));
+    const size_t plane = (size_t)Hp * Wp;
+    for (size_t i = 0; i < plane; i++)
+        for (int c = 0; c < C; c++) out[i * C + c] = host[c * plane + i];
+    return PT_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Persistence: the reference's binary format (save/load of basic types, Arrays and strings,
+// patchtable.cpp:499-573; PatchTableFixedN::save/load, patchtable.h:1574-1593; PatchPartition
+// and PatchPartition1D, patchtable.h:1145-1170, 1334-1353).
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+struct Writer {
+    FILE *f;
+    bool ok = true;
+    void raw(const void *p, size_t bytes) {
+        if (ok && bytes && fwrite(p, 1, bytes, f) != bytes) ok = false;
+    }
+    void i32(int32_t v) { raw(&v, 4); }
+    void f32(float v) { raw(&v, 4); }
+    void str(const char *s) {
+        i32((int32_t)strlen(s));
+        raw(s, strlen(s));
+    }
+    template <class T>
+    void vec(const std::vector<T> &v) {
+        i32((int32_t)v.size());
+        raw(v.data(), v.size() * sizeof(T));
+    }
+    template <class T>
+    void array(const std::vector<int> &sizes, const T *data, size_t n) {
+        vec(sizes);
+        raw(data, n * sizeof(T));
+    }
+};
+
+struct Reader {
+    FILE *f;
+    const char *path;
+    int raw(void *p, size_t bytes) {
+        if (bytes && fread(p, 1, bytes, f) != bytes) return pt_failf(PT_E_IO, "%s: truncated PatchTable file", path);
+        return PT_OK;
+    }
+    int i32(int32_t &v) { return raw(&v, 4); }
+    int f32(float &v) { return raw(&v, 4); }
+    int id(const char *expect) {   // load_id (patchtable.cpp:560-567)
+        int32_t n = 0;
+        TRY(i32(n));
+        if (n < 0 || n > 256) return pt_failf(PT_E_IO, "%s: expected |%s|", path, expect);
+        std::string s((size_t)n, '\0');
+        TRY(raw(&s[0], (size_t)n));
+        if (s != expect) return pt_failf(PT_E_IO, "%s: load_id expected |%s|, received |%s|", path, expect, s.c_str());
+        return PT_OK;
+    }
+    template <class T>
+    int vec(std::vector<T> &v, int32_t max_n) {
+        int32_t n = 0;
+        TRY(i32(n));
+        if (n < 0 || n > max_n) return pt_failf(PT_E_IO, "%s: bad vector length %d", path, n);
+        v.resize((size_t)n);
+        return raw(v.data(), (size_t)n * sizeof(T));
+    }
+    template <class T>
+    int array(std::vector<int> &sizes, std::vector<T> &data) {
+        TRY(vec(sizes, 64));
+        size_t n = 1;
+        for (int d : sizes) {
+            if (d < 0) return pt_failf(PT_E_IO, "%s: bad array size", path);
+            n *= (size_t)d;
+        }
+        if (n > ((size_t)1 << 34)) return pt_failf(PT_E_IO, "%s: array too large", path);
+        data.resize(n);
+        return raw(data.data(), n * sizeof(T));
+    }
+};
+
+int hwc_to_planes(const float *hwc, size_t plane, int C, std::vector<float> &planes) {
+    planes.resize(plane * C);
+    for (size_t i = 0; i < plane; i++)
+        for (int c = 0; c < C; c++) planes[c * plane + i] = hwc[i * C + c];
+    return PT_OK;
+}
+
+}  // namespace
+
+extern "C" int pt_table_save(pt_table *t, const char *filename) {
+    if (!t || !filename) return pt_fail(PT_E_INVALID, "pt_table_save: null argument");
+    const int Hp = t->h + 7, Wp = t->w + 7;
+    std::vector<int32_t> cells((size_t)t->ncells);
+    std::vector<float> b0((size_t)t->h * t->w * 3), wh0((size_t)Hp * Wp * t->ndims);
+    HIPTRY(hipStreamSynchronize(t->stream));
+    HIPTRY(hipMemcpy(cells.data(), t->table.p, cells.size() * 4, hipMemcpyDeviceToHost));
+    HIPTRY(hipMemcpy(b0.data(), t->b0.p, b0.size() * 4, hipMemcpyDeviceToHost));
+    TRY(planes_to_hwc(t->desc.p, Hp, Wp, t->ndims, wh0.data()));
+    FILE *f = fopen(filename, "wb");
+    if (!f) return pt_failf(PT_E_IO, "PatchTable: could not save to %s", filename);
+    Writer W{f};
+    W.str("PatchTable");
+    W.array(t->nslices, cells.data(), cells.size());
+    W.array(std::vector<int>{t->h, t->w, 3}, b0.data(), b0.size());
+    W.array(std::vector<int>{Hp, Wp, t->ndims}, wh0.data(), wh0.size());
+    W.vec(t->nslices);
+    W.str("PatchPartition");
+    W.i32(t->G);
+    for (const HostPartition &P : t->part) {
+        W.str("PatchPartition1D");
+        W.vec(P.to_bin);
+        W.vec(P.centers);
+        W.vec(P.lsizes);
+        W.vec(P.rsizes);
+        W.f32(P.min_val);
+        W.f32(P.max_val);
+        W.f32(P.range);
+        W.i32(P.nslices);
+        W.str("EndPatchPartition1D");
+    }
+    W.str("EndPatchPartition");
+    W.str("EndPatchTable");
+    const bool ok = W.ok && fclose(f) == 0;
+    if (!ok) return pt_failf(PT_E_IO, "PatchTable: write to %s failed", filename);
+    return PT_OK;
+}
+
+namespace {
+
+int load_file(pt_table *t, const char *filename) {
+    FILE *f = fopen(filename, "rb");
+    if (!f) return pt_failf(PT_E_IO, "PatchTable: could not load from %s", filename);
+    struct Closer {
+        FILE *f;
+        ~Closer() { fclose(f); }
+    } closer{f};
+    Reader R{f, filename};
+    std::vector<int> tsz, bsz, wsz, ns;
+    std::vector<int32_t> cells;
+    std::vector<float> b0, wh0;
+    TRY(R.id("PatchTable"));
+    TRY(R.array(tsz, cells));
+    TRY(R.array(bsz, b0));
+    TRY(R.array(wsz, wh0));
+    TRY(R.vec(ns, PT_MAX_DIMS));
+    if (bsz.size() != 3 || bsz[2] != 3 || wsz.size() != 3 || wsz[0] != bsz[0] + 7 || wsz[1] != bsz[1] + 7)
+        return pt_failf(PT_E_IO, "%s: inconsistent image and descriptor sizes", filename);
+    if (tsz != ns || ns.empty()) return pt_failf(PT_E_IO, "%s: table sizes do not match nslices", filename);
+    t->h = bsz[0];
+    t->w = bsz[1];
+    t->ndims = wsz[2];
+    t->G = (int)ns.size();
+    if (t->h < 8 || t->w < 8 || t->ndims < 1 || t->ndims > PT_MAX_DIMS || t->G > t->ndims)
+        return pt_failf(PT_E_IO, "%s: bad table dimensions", filename);
+    t->nslices = ns;
+    t->stride.assign(t->G, 0);
+    int64_t acc = 1;
+    for (int i = t->G - 1; i >= 0; i--) {
+        if (ns[i] < 1) return pt_failf(PT_E_IO, "%s: bad nslices", filename);
+        t->stride[i] = acc;
+        acc *= ns[i];
+    }
+    t->ncells = acc;
+    TRY(R.id("PatchPartition"));
+    int32_t np = 0;
+    TRY(R.i32(np));
+    if (np != t->G) return pt_failf(PT_E_IO, "%s: partition count %d != grid dims %d", filename, np, t->G);
+    t->part.assign(t->G, HostPartition());
+    for (int i = 0; i < t->G; i++) {
+        HostPartition &P = t->part[i];
+        TRY(R.id("PatchPartition1D"));
+        TRY(R.vec(P.to_bin, PT_NMAP));
+        TRY(R.vec(P.centers, 1 << 24));
+        TRY(R.vec(P.lsizes, 1 << 24));
+        TRY(R.vec(P.rsizes, 1 << 24));
+        TRY(R.f32(P.min_val));
+        TRY(R.f32(P.max_val));
+        TRY(R.f32(P.range));
+        TRY(R.i32(P.nslices));
+        TRY(R.id("EndPatchPartition1D"));
+        if ((int)P.to_bin.size() != PT_NMAP || P.nslices != ns[i])
+            return pt_failf(PT_E_IO, "%s: bad partition of dimension %d", filename, i);
+        for (int32_t v : P.to_bin)
+            if (v < 0 || v >= P.nslices) return pt_failf(PT_E_IO, "%s: bin index out of range", filename);
+    }
+    TRY(R.id("EndPatchPartition"));
+    TRY(R.id("EndPatchTable"));
+    const int bew = t->w - 7, beh = t->h - 7;
+    for (int32_t v : cells) {
+        const int x = PT_INT_TO_X(v) - 7, y = PT_INT_TO_Y(v) - 7;
+        if (x < 0 || y < 0 || x >= bew || y >= beh) return pt_failf(PT_E_IO, "%s: table entry out of range", filename);
+    }
+    // upload
+    hipStream_t s = t->stream;
+    const size_t plane = (size_t)(t->h + 7) * (t->w + 7);
+    std::vector<float> planes;
+    hwc_to_planes(wh0.data(), plane, t->ndims, planes);
+    TRY(t->b0.ensure(b0.size()));
+    TRY(t->desc.ensure(planes.size()));
+    TRY(t->table.ensure(cells.size()));
+    HIPTRY(hipMemcpyAsync(t->b0.p, b0.data(), b0.size() * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(t->desc.p, planes.data(), planes.size() * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(t->table.p, cells.data(), cells.size() * 4, hipMemcpyHostToDevice, s));
+    TRY(upload_quant(t));
+    return PT_OK;
+}
+
+}  // namespace
+
 extern "C" int pt_table_load(pt_params *p, const char *filename, pt_table **out) {
-    (void)p;
-    (void)filename;
-    (void)out;
-    return pt_fail(PT_E_UNSUPPORTED, "load is not supported by this build yet");
+    if (!p || !filename || !out) return pt_fail(PT_E_INVALID, "pt_table_load: null argument");
+    *out = nullptr;
+    pt_table *t = new pt_table();
+    t->p = p;
+    t->build = *p;
+    int rc = init_stream(t);
+    if (rc == PT_OK) rc = load_file(t, filename);
+    if (rc == PT_OK && t->ndims != p->ndims)
+        rc = pt_failf(PT_E_INVALID, "%s holds %d-dimensional descriptors, parameters say ndims %d", filename, t->ndims,
+                      p->ndims);
+    if (rc == PT_OK) {
+        t->nchroma = p->nchroma;
+        t->build.grid_ndims = t->G;
+        // The format holds no k-coherence lists; rebuild them from the loaded b0 and wh0.
+        t->K = p->kcoherence;
+        if (t->K < 0 || t->K > 32) rc = pt_fail(PT_E_UNSUPPORTED, "kcoherence must be in 0..32");
+    }
+    if (rc == PT_OK && t->K > 0) {
+        const size_t n = (size_t)(t->w - 7) * (t->h - 7);
+        const size_t plane = (size_t)(t->h + 7) * (t->w + 7);
+        rc = t->kpos.ensure(n * t->K);
+        if (rc == PT_OK) rc = t->kdist.ensure(n * t->K);
+        if (rc == PT_OK)
+            rc = pt_kcoherence_launch(t->desc.p, plane, t->w + 7, t->ndims, t->b0.p, t->w, t->h, t->quant(),
+                                      t->nslices.data(), t->K, p->kcoh_window,
+                                      p->kcoherence_min_dist * p->kcoherence_min_dist, t->kpos.p, t->kdist.p, t->sc,
+                                      t->stream);
+    }
+    if (rc == PT_OK && hipStreamSynchronize(t->stream) != hipSuccess) rc = pt_fail(PT_E_HIP, "load: stream sync failed");
+    if (rc != PT_OK) {
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return PT_OK;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -567,15 +814,6 @@ extern "C" int pt_table_info(const pt_table *t, int *grid_ndims, int *nslices, i
     if (ncells) *ncells = t->ncells;
     if (table_add_count) *table_add_count = t->table_add_count;
     if (kcoherence) *kcoherence = t->K;
-    return PT_OK;
-}
-
-static int planes_to_hwc(const float *planes, int Hp, int Wp, int C, float *out) {
-    std::vector<float> host((size_t)Hp * Wp * C);
-    HIPTRY(hipMemcpy(host.data(), planes, host.size() * 4, hipMemcpyDeviceToHost));
-    const size_t plane = (size_t)Hp * Wp;
-    for (size_t i = 0; i < plane; i++)
-        for (int c = 0; c < C; c++) out[i * C + c] = host[c * plane + i];
     return PT_OK;
 }
 

@@ -49,6 +49,7 @@ def lib():
             "orc_get_table": (_int, [ctypes.c_void_p, _i32p]),
             "orc_get_populated": (_int, [ctypes.c_void_p, _i32p]),
             "orc_get_kcoherence": (_int, [ctypes.c_void_p, _i32p, _f32p]),
+            "orc_save": (_int, [ctypes.c_void_p, ctypes.c_char_p]),
             "orc_dt": (_int, [_int, _P(_int), _i32p, _int, _int]),
             "orc_lookup_raster": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f64p]),
             "orc_lookup_parallel": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f64p]),
@@ -108,6 +109,9 @@ class OracleTable:
     def set_params(self, params):
         self.params = params
         lib().orc_set_params(self.handle, ctypes.byref(params))
+
+    def save(self, path):
+        assert lib().orc_save(self.handle, str(path).encode()) == 0
 
     def descriptor(self):
         nd = self.params.ndims
