@@ -253,6 +253,34 @@ static inline float patch_ssd(const float *a, int aw, int ax, int ay, const floa
 }
 
 // --------------------------------------------------------------------------------------------
+// Descriptor distance of descriptor mode, patch_dist_approx (patchtable.h:376-437) with
+// TABLE_SSE: the first four dimensions as one 4-lane (a-b)^2; for C = 8, 12, 16, 20 the other
+// 4-wide chunks are added lane-wise and the lanes summed last, otherwise the lanes are summed
+// and the remaining dimensions added one by one. C >= 4 (the SSE path reads four values).
+// --------------------------------------------------------------------------------------------
+static inline float desc_dist(const float *a, const float *b, int C) {
+    float L[4];
+    for (int j = 0; j < 4; j++) {
+        float e = a[j] - b[j];
+        L[j] = e * e;
+    }
+    if (C == 8 || C == 12 || C == 16 || C == 20) {
+        for (int n = 1; n < C / 4; n++)
+            for (int j = 0; j < 4; j++) {
+                float e = a[4 * n + j] - b[4 * n + j];
+                L[j] = L[j] + e * e;
+            }
+        return ((L[0] + L[1]) + L[2]) + L[3];
+    }
+    float d = ((L[0] + L[1]) + L[2]) + L[3];
+    for (int i = 4; i < C; i++) {
+        float e = a[i] - b[i];
+        d += e * e;
+    }
+    return d;
+}
+
+// --------------------------------------------------------------------------------------------
 // Quantiser.
 // --------------------------------------------------------------------------------------------
 // int(float) as the x86 cvttss2si instruction computes it: out-of-range and NaN give INT_MIN.
@@ -358,10 +386,13 @@ static Partition1D make_partition(std::vector<float> vals, int nslices, float mi
 // --------------------------------------------------------------------------------------------
 struct Table {
     pt_params p;
-    int h = 0, w = 0;                 // b image size
+    int h = 0, w = 0;                 // "image" size: b in image mode; descriptor size + 7 otherwise
     int ndims = 0, G = 0;             // descriptor dims, grid dims
-    fvec b0;                          // RGB copy of b (patchtable.h:2628)
+    bool desc = false;                // descriptor mode (is_descriptor, patchtable.h:4398-4401)
+    fvec b0;                          // RGB copy of b (patchtable.h:2628); zeros in descriptor mode
     fvec wh0;                         // (h+7) x (w+7) x ndims
+    ivec allowed;                     // (h-7) x (w-7) allowed_patches, empty without a mask
+    int allowed_index = 1;
     std::vector<int> nslices;
     std::vector<int64_t> stride;      // row-major over nslices, dim 0 slowest
     int64_t ncells = 0;
@@ -369,17 +400,45 @@ struct Table {
     ivec table_pop;                   // after populate
     ivec table;                       // after DT
     int table_add_count = 0;
+    long rand_used = 0;               // rand() outputs consumed by the populate shuffles
     int K = 0;                        // k-coherence list length
     ivec kpos;                        // (h-7) x (w-7) x K
     fvec kdist;
+    std::vector<Table *> extra;       // the ntables-1 further tables (patchtable.h:3359-3372)
+    Table() {}
+    Table(const Table &) = delete;
+    ~Table() {
+        for (Table *t : extra) delete t;
+    }
     int bew() const { return w - 7; }
     int beh() const { return h - 7; }
+    const float *bdesc(int xs, int ys) const { return &wh0[((size_t)(ys + 7) * (w + 7) + xs + 7) * ndims]; }
+    bool is_allowed(int xs, int ys, int index) const {
+        return allowed.empty() || allowed[(size_t)ys * bew() + xs] == index;
+    }
     int cell_of(const float *desc) const {
         int64_t idx = 0;
         for (int i = 0; i < G; i++) idx += (int64_t)part[i].get_bin(desc[i]) * stride[i];
         return (int)idx;
     }
 };
+
+// Descriptor-mode input placed as the padded descriptor (get_padded_descriptor,
+// patchtable.h:2498-2507): unpadded h_in x w_in x C goes to rows/columns 7.. of an
+// (h_in+14) x (w_in+14) array (zeros elsewhere); a padded input is taken as is.
+static void pad_descriptor(const pt_params &p, const float *in, int h_in, int w_in, int C, fvec &wh, int &h, int &w) {
+    if (p.descriptor_padded) {
+        h = h_in - 7;
+        w = w_in - 7;
+        wh.assign(in, in + (size_t)h_in * w_in * C);
+        return;
+    }
+    h = h_in + 7;
+    w = w_in + 7;
+    wh.assign((size_t)(h + 7) * (w + 7) * C, 0.f);
+    for (int y = 0; y < h_in; y++)
+        std::copy(in + (size_t)y * w_in * C, in + (size_t)(y + 1) * w_in * C, &wh[((size_t)(y + 7) * (w + 7) + 7) * C]);
+}
 
 // Descriptor of an RGB image: colour conversion then GCK (get_padded_descriptor,
 // patchtable.h:2492-2546, and reduce_dim, :1687-1728, for dim_algo WH without filter_dims).
@@ -647,7 +706,7 @@ static int morton_bits(const Table &T) {
 
 static uint64_t morton_key(const Table &T, const int *bins, int shifted, int B) {
     int Gk = T.G;
-    while (Gk * B > 64) Gk--;
+    while (Gk * B > 63) Gk--;   // one bit of the 64 is left for the allowed flag of the sort
     uint64_t key = 0;
     for (int b = B - 1; b >= 0; b--)
         for (int d = 0; d < Gk; d++) {
@@ -657,6 +716,13 @@ static uint64_t morton_key(const Table &T, const int *bins, int shifted, int B) 
     return key;
 }
 
+// This build's k-coherence lists (the reference searches a FLANN kd-tree, patchtable.h:2653-2881):
+// over the allowed database patches (TABLE_FOR_ALLOWED_PATCHES_STEP, patchtable.h:2711), sorted
+// by two Morton keys of their table bins, the candidates of p are the patches within W sorted
+// positions in either order at pixel distance >= kcoherence_min_dist; the K nearest in
+// descriptor L2 are re-scored with the lookup distance (exact RGB SSD, or patch_dist_approx in
+// descriptor mode) and sorted by (distance, XY_TO_INT). Short lists repeat their last entry;
+// patches outside the mask get K entries (p, +inf), which end any walk at once.
 static void build_kcoherence(Table &T) {
     const int K = T.p.kcoherence, W = T.p.kcoh_window;
     const int bew = T.bew(), beh = T.beh(), nd = T.ndims, Wp = T.w + 7;
@@ -670,34 +736,46 @@ static void build_kcoherence(Table &T) {
     std::vector<uint64_t> keys[2];
     std::vector<int32_t> perm[2], pos[2];
     std::vector<int> bins(T.G);
+    std::vector<char> ok(N);
     for (int o = 0; o < 2; o++) keys[o].resize(N);
     for (long i = 0; i < N; i++) {
         int px = (int)(i % bew), py = (int)(i / bew);
+        ok[i] = T.is_allowed(px, py, T.allowed_index);
         const float *d = &T.wh0[((size_t)(py + 7) * Wp + px + 7) * nd];
         for (int g = 0; g < T.G; g++) bins[g] = T.part[g].get_bin(d[g]);
         keys[0][i] = morton_key(T, bins.data(), 0, B);
         keys[1][i] = morton_key(T, bins.data(), 1, B);
     }
+    long Na = 0;
     for (int o = 0; o < 2; o++) {
-        perm[o].resize(N);
-        pos[o].resize(N);
-        for (long i = 0; i < N; i++) perm[o][i] = (int32_t)i;
+        perm[o].clear();
+        pos[o].assign(N, -1);
+        for (long i = 0; i < N; i++)
+            if (ok[i]) perm[o].push_back((int32_t)i);
         const std::vector<uint64_t> &ky = keys[o];
         std::sort(perm[o].begin(), perm[o].end(), [&](int32_t a, int32_t b) {
             return ky[a] != ky[b] ? ky[a] < ky[b] : a < b;
         });
-        for (long r = 0; r < N; r++) pos[o][perm[o][r]] = (int32_t)r;
+        Na = (long)perm[o].size();
+        for (long r = 0; r < Na; r++) pos[o][perm[o][r]] = (int32_t)r;
     }
     struct Cand { float d; int32_t q; };
     std::vector<Cand> cands;
     for (long i = 0; i < N; i++) {
         int px = (int)(i % bew), py = (int)(i / bew);
+        if (!ok[i]) {
+            for (int k = 0; k < K; k++) {
+                T.kpos[(size_t)i * K + k] = XY_TO_INT(px, py);
+                T.kdist[(size_t)i * K + k] = INFINITY;
+            }
+            continue;
+        }
         const float *dp = &T.wh0[((size_t)(py + 7) * Wp + px + 7) * nd];
         cands.clear();
         for (int o = 0; o < 2; o++) {
             long r = pos[o][i];
             for (long t = r - W; t <= r + W; t++) {
-                if (t < 0 || t >= N || t == r) continue;
+                if (t < 0 || t >= Na || t == r) continue;
                 int32_t q = perm[o][t];
                 int qx = q % bew, qy = q / bew;
                 int dx = qx - px, dy = qy - py;
@@ -721,7 +799,8 @@ static void build_kcoherence(Table &T) {
         std::vector<Sel> sel(m);
         for (int k = 0; k < m; k++) {
             int qx = cands[k].q % bew, qy = cands[k].q / bew;
-            sel[k].ssd = patch_ssd(T.b0.data(), T.w, px, py, T.b0.data(), T.w, qx, qy);
+            sel[k].ssd = T.desc ? desc_dist(T.bdesc(px, py), T.bdesc(qx, qy), nd)
+                                : patch_ssd(T.b0.data(), T.w, px, py, T.b0.data(), T.w, qx, qy);
             sel[k].xy = XY_TO_INT(qx, qy);
         }
         std::sort(sel.begin(), sel.end(), [](const Sel &a, const Sel &b) {
@@ -736,30 +815,53 @@ static void build_kcoherence(Table &T) {
 }
 
 // ---------------------------- table construction --------------------------------------------
-// PatchTableFixedN constructor (patchtable.h:2584-3373), default options: no allowed mask,
-// colour conversion + GCK, k-coherence, quantiser, fast random populate, DT, fill_zeros.
-static Table *build(const pt_params &p_in, const float *b, int h, int w) {
+// PatchTableFixedN constructor (patchtable.h:2584-3373): descriptor (colour conversion + GCK, or
+// the padded input in descriptor mode), quantiser, fast populate (random order), DT or
+// fill_zeros, k-coherence, then the ntables-1 further tables. Only allowed patches take part
+// when a mask is given (TABLE_FOR_ALLOWED_PATCHES, patchtable.h:120-145). `rand_skip` is the
+// number of rand() outputs consumed before this table's shuffle (the further tables continue
+// the process-wide rand() stream after the earlier tables' shuffles).
+static Table *build(const pt_params &p_in, const float *b, int h_in, int w_in, int c_in, const int32_t *allowed,
+                    long rand_skip = 0) {
     Table *T = new Table();
     T->p = p_in;
     pt_params &p = T->p;
+    T->desc = p.is_descriptor != 0;
+    if (T->desc) {   // patchtable.h:4398-4401
+        p.ndims = c_in;
+        p.nchroma = 0;
+    }
     if (p.grid_ndims < 0) p.grid_ndims = p.ndims;      // patchtable.h:2603
-    T->h = h;
-    T->w = w;
     T->ndims = p.ndims;
     T->G = p.grid_ndims;
-    T->b0.assign(b, b + (size_t)h * w * 3);
-    descriptor(p, b, h, w, T->wh0);
+    if (T->desc) {
+        pad_descriptor(p, b, h_in, w_in, c_in, T->wh0, T->h, T->w);
+        T->b0.assign((size_t)T->h * T->w * 3, 0.f);
+    } else {
+        T->h = h_in;
+        T->w = w_in;
+        T->b0.assign(b, b + (size_t)h_in * w_in * 3);
+        descriptor(p, b, h_in, w_in, T->wh0);
+    }
+    const int h = T->h, w = T->w;
     const int Wp = w + 7, nd = T->ndims, G = T->G;
     const int bew = T->bew(), beh = T->beh();
+    if (allowed) {
+        T->allowed.assign(allowed, allowed + (size_t)bew * beh);
+        T->allowed_index = p.allowed_index;
+    }
+    auto ok = [&](int x, int y) { return T->is_allowed(x - 7, y - 7, T->allowed_index); };   // padded coords
     // set_min_max (patchtable.h:1758-1778)
     std::vector<float> mn(G, (float)1e20), mx(G, (float)-1e20);
     for (int y = 7; y < h; y++)
-        for (int x = 7; x < w; x++)
+        for (int x = 7; x < w; x++) {
+            if (!ok(x, y)) continue;
             for (int i = 0; i < G; i++) {
                 float v = T->wh0[((size_t)y * Wp + x) * nd + i];
                 mn[i] = mn[i] < v ? mn[i] : v;
                 mx[i] = mx[i] > v ? mx[i] : v;
             }
+        }
     std::vector<float> range(G);
     for (int i = 0; i < G; i++) range[i] = mx[i] - mn[i];
     // set_default_nslices (patchtable.h:1780-1810) or explicit -nslices, then the limit.
@@ -780,23 +882,27 @@ static Table *build(const pt_params &p_in, const float *b, int h, int w) {
         acc *= T->nslices[i];
     }
     T->ncells = acc;
-    // k-coherence is built before the partition in the reference; it reads the table bins
-    // here, so it is built after the partition (the lists do not depend on the order).
+    // PatchPartition over the allowed patches of the partition grid (patchtable.h:1355-1385).
     for (int i = 0; i < G; i++) {
         std::vector<float> vals;
         vals.reserve((size_t)bew * beh);
         for (int y = 7; y < h; y += p.partition_step)
-            for (int x = 7; x < w; x += p.partition_step) vals.push_back(T->wh0[((size_t)y * Wp + x) * nd + i]);
+            for (int x = 7; x < w; x += p.partition_step)
+                if (ok(x, y)) vals.push_back(T->wh0[((size_t)y * Wp + x) * nd + i]);
         T->part.push_back(make_partition(vals, T->nslices[i], mn[i], mx[i]));
     }
     // Populate (patchtable.h:3121-3159), TABLE_POPULATE_FAST.
     T->table.assign(T->ncells, TABLE_UNUSED);
     std::vector<int32_t> order;
     for (int y = 7; y < h; y++)
-        for (int x = 7; x < w; x++) order.push_back(XY_TO_INT(x, y));
+        for (int x = 7; x < w; x++)
+            if (ok(x, y)) order.push_back(XY_TO_INT(x, y));
+    T->rand_used = rand_skip;
     if (p.populate_random) {
         GlibcRand rng(1);
+        for (long k = 0; k < rand_skip; k++) rng.next();
         random_shuffle(order.data(), (int)order.size(), rng);
+        if (order.size() > 1) T->rand_used += (long)order.size() - 1;
     }
     T->table_add_count = 0;
     for (int32_t pos : order) {
@@ -828,37 +934,118 @@ static Table *build(const pt_params &p_in, const float *b, int h, int w) {
             if (v == TABLE_UNUSED) v = vfill;
     }
     if (p.kcoherence > 0) build_kcoherence(*T);
+    // Further tables (patchtable.h:3359-3372): limit / table_ratio^i, no k-coherence.
+    long used = T->rand_used;
+    for (int i = 1; i < p.ntables; i++) {
+        pt_params pc = p;
+        pc.limit = (double)(int)(pc.limit * (1.0 / pow(p.table_ratio, i)));
+        pc.kcoherence = 0;
+        pc.ntables = 1;
+        Table *E = build(pc, b, h_in, w_in, c_in, allowed, used);
+        used = E->rand_used;
+        T->extra.push_back(E);
+    }
     return T;
 }
 
+// ---------------------------- query side ---------------------------------------------------
+// Descriptor of the query (get_padded_descriptor, patchtable.h:2492-2546): GCK of the colour-
+// converted RGB image, or the (padded) input descriptor in descriptor mode.
+struct QuerySide {
+    int ah = 0, aw = 0;          // "image" size (descriptor size + 7 in descriptor mode)
+    int C = 0;
+    const float *a = nullptr;    // RGB query (image mode)
+    fvec wh;                     // (ah+7) x (aw+7) x C
+    int ann_w() const { return aw - 7; }
+    int ann_h() const { return ah - 7; }
+    const float *qdesc(int x, int y) const { return &wh[((size_t)(y + 7) * (aw + 7) + x + 7) * C]; }
+};
+
+static void query_side(const Table &T, const float *a, int h_in, int w_in, int c_in, QuerySide &Q) {
+    Q.C = T.ndims;
+    if (T.desc) {
+        pad_descriptor(T.p, a, h_in, w_in, c_in, Q.wh, Q.ah, Q.aw);
+    } else {
+        Q.ah = h_in;
+        Q.aw = w_in;
+        Q.a = a;
+        descriptor(T.p, a, h_in, w_in, Q.wh);
+    }
+}
+
+// TABLE_LOOKUP_PATCH_DIST (patchtable.h:315-320): exact RGB SSD, or patch_dist_approx of the
+// descriptors in descriptor mode.
+static inline float lookup_dist(const Table &T, const QuerySide &Q, int x, int y, int xs, int ys) {
+    return T.desc ? desc_dist(Q.qdesc(x, y), T.bdesc(xs, ys), Q.C)
+                  : patch_ssd(Q.a, Q.aw, x, y, T.b0.data(), T.w, xs, ys);
+}
+
+static long rand_used_total(const Table &T) { return T.extra.empty() ? T.rand_used : T.extra.back()->rand_used; }
+
 // ---------------------------- lookup (reference raster order) -------------------------------
-// lookup_templated<is_descriptor=0, use_allowed_patches=0> (patchtable.h:3468-4217), no
-// ann_prev, no init_random, query_step 1, prop_dist 0, single table.
-static void lookup_raster(const Table &T, const float *a, int ah, int aw, double *ann) {
+// lookup_templated<is_descriptor, use_allowed_patches> (patchtable.h:3468-4217) with
+// kcoherence_step 1 and prop_dist as given: ann_prev initialisation and temporal rescaling,
+// table candidates of every table, k-coherence walk with the triangle test, spatial or random
+// search, and the propagation scatter (summed-area table in image mode, patch_dist_approx in
+// descriptor mode) with the border fill-in. rand() continues after the tables' shuffles.
+static void lookup_raster(const Table &T, const float *a_in, int h_in, int w_in, int c_in, double *ann,
+                          const double *ann_prev, const float *sv) {
     const pt_params &p = T.p;
-    fvec a_wh;
-    descriptor(p, a, ah, aw, a_wh);
-    const int ann_h = ah - 7, ann_w = aw - 7;
-    const int b0_ew = T.w - 7, b0_eh = T.h - 7;
-    const int a_w = aw, a_h = ah, b_w = T.w, b_h = T.h;
-    const int nd = T.ndims, AWp = aw + 7;
+    QuerySide Q;
+    query_side(T, a_in, h_in, w_in, c_in, Q);
+    const int ann_h = Q.ann_h(), ann_w = Q.ann_w();
+    const int b0_ew = T.bew(), b0_eh = T.beh();
+    const int a_w = Q.aw, a_h = Q.ah, b_w = T.w, b_h = T.h;
+    const bool is_desc = T.desc, use_allowed = !T.allowed.empty();
+    const float *a = Q.a;
     const float *b = T.b0.data();
-    for (int y = 0; y < ann_h; y++)
-        for (int x = 0; x < ann_w; x++) {
-            double *q = &ann[((size_t)y * ann_w + x) * 3];
-            q[0] = b0_ew + 1;
-            q[1] = 0;        // left uninitialised by the reference; any value behaves the same
-            q[2] = DIST_INFINITY;
-        }
+    GlibcRand rng(1);
+    for (long k = 0; k < rand_used_total(T); k++) rng.next();
+    auto allowed_at = [&](int xs, int ys) { return T.is_allowed(xs, ys, p.allowed_index); };
+    auto dist = [&](int x, int y, int xs, int ys) { return lookup_dist(T, Q, x, y, xs, ys); };
+    auto ANN = [&](int yy, int xx) -> double * { return &ann[((size_t)yy * ann_w + xx) * 3]; };
     const double prop_scale = 1.0 / (1 + p.coherence_spatial);
+    const double temporal_scale = 1.0 / (1 + p.coherence_temporal);
+    fvec dist_orig;
+    if (!ann_prev) {
+        for (int y = 0; y < ann_h; y++)
+            for (int x = 0; x < ann_w; x++) {
+                double *q = ANN(y, x);
+                q[0] = b0_ew + 1;
+                q[1] = 0;        // left uninitialised by the reference; any value behaves the same
+                q[2] = DIST_INFINITY;
+            }
+    } else {   // patchtable.h:3539-3585
+        if (p.recalc_dist_temporal) dist_orig.assign((size_t)ann_w * ann_h, 0.f);
+        for (int y = 0; y < ann_h; y++)
+            for (int x = 0; x < ann_w; x++) {
+                const size_t i = (size_t)y * ann_w + x;
+                const double *pp = &ann_prev[i * 3];
+                int xsrc = (int)pp[0], ysrc = (int)pp[1];
+                if (xsrc < 0) xsrc = 0;   // sanitize_input (the reference asserts in bounds otherwise)
+                else if (xsrc >= b0_ew) xsrc = b0_ew - 1;
+                if (ysrc < 0) ysrc = 0;
+                else if (ysrc >= b0_eh) ysrc = b0_eh - 1;
+                double *q = ANN(y, x);
+                q[0] = xsrc;
+                q[1] = ysrc;
+                if (p.recalc_dist_temporal) {
+                    float dc = dist(x, y, xsrc, ysrc);
+                    double tsc = sv ? (double)sv[i] : temporal_scale;
+                    dist_orig[i] = (float)(dc * tsc);
+                    q[2] = dist_orig[i];
+                } else {
+                    q[2] = pp[2] * temporal_scale;
+                }
+            }
+    }
+    static const int dx_table[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+    static const int dy_table[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
     const int qs = p.query_step;
     const int prop_w = (qs + p.prop_dist) * 2 + 1;
     const int sat_w = 8 + prop_w + 1 - 1;
     const int sat_w_nopad = sat_w - 1;
     const bool do_rs_or_spatial = p.do_rs || p.spatial;
-    auto A = [&](int yy, int xx, int c) -> float { return a[((size_t)yy * aw + xx) * 3 + c]; };
-    auto Bp = [&](int yy, int xx, int c) -> float { return b[((size_t)yy * b_w + xx) * 3 + c]; };
-    auto ANN = [&](int yy, int xx) -> double * { return &ann[((size_t)yy * ann_w + xx) * 3]; };
     for (int prop_iter = 0; prop_iter < p.prop_iters; prop_iter++) {
         bool reverse = prop_iter % 2;
         bool do_lookup = prop_iter == 0 && p.do_table_lookup;
@@ -871,57 +1058,57 @@ static void lookup_raster(const Table &T, const float *a, int ah, int aw, double
                 int xbest = (int)ANN(y, x)[0];
                 int ybest = (int)ANN(y, x)[1];
                 float dbest = (float)(ANN(y, x)[2] * prop_scale);
-                auto dist = [&](int xs, int ys) { return patch_ssd(a, aw, x, y, b, b_w, xs, ys); };
-                if (do_lookup) {
-                    const float *whp = &a_wh[((size_t)(y + 7) * AWp + x + 7) * nd];
+                auto consider = [&](int xs, int ys) {
+                    float d = dist(x, y, xs, ys);
+                    if (d < dbest) {
+                        dbest = d;
+                        xbest = xs;
+                        ybest = ys;
+                    }
+                };
+                if (do_lookup) {   // patchtable.h:3664-3737
+                    const float *whp = Q.qdesc(x, y);
                     int v = T.table[T.cell_of(whp)];
                     int xsrc = INT_TO_X(v) - 7, ysrc = INT_TO_Y(v) - 7;
-                    if (xsrc != xbest || ysrc != ybest) {
-                        float d = dist(xsrc, ysrc);
-                        if (d < dbest) {
-                            dbest = d;
-                            xbest = xsrc;
-                            ybest = ysrc;
-                        }
+                    if (xsrc != xbest || ysrc != ybest) consider(xsrc, ysrc);
+                    int xprev = xsrc, yprev = ysrc;
+                    for (const Table *E : T.extra) {
+                        int v1 = E->table[E->cell_of(whp)];
+                        int xs1 = INT_TO_X(v1) - 7, ys1 = INT_TO_Y(v1) - 7;
+                        if ((xs1 != xbest || ys1 != ybest) && (xs1 != xprev || ys1 != yprev)) consider(xs1, ys1);
+                        xprev = xs1;
+                        yprev = ys1;
                     }
                 }
-                if (do_kcoh) {
+                if (do_kcoh) {   // patchtable.h:3748-3851
                     const int32_t *kp = &T.kpos[((size_t)ybest * b0_ew + xbest) * T.K];
                     const float *kd = &T.kdist[((size_t)ybest * b0_ew + xbest) * T.K];
                     for (int k = 0; k < p.kcoherence; k++) {
                         if (kd[k] >= p.triangle_factor * dbest) break;
-                        int xsrc = INT_TO_X(kp[k]), ysrc = INT_TO_Y(kp[k]);
-                        float d = dist(xsrc, ysrc);
-                        if (d < dbest) {
-                            dbest = d;
-                            xbest = xsrc;
-                            ybest = ysrc;
-                        }
+                        consider(INT_TO_X(kp[k]), INT_TO_Y(kp[k]));
                     }
                 }
-                if (do_rs_or_spatial && p.spatial) {
-                    int symin = ybest - p.spatial, symax = ybest + p.spatial + 1;
-                    int sxmin = xbest - p.spatial, sxmax = xbest + p.spatial + 1;
-                    int xb0 = xbest, yb0 = ybest;
-                    if (symin < 0) symin = 0;
-                    else if (symax > b0_eh) symax = b0_eh;
-                    if (sxmin < 0) sxmin = 0;
-                    else if (sxmax > b0_ew) sxmax = b0_ew;
-                    if (symax > b0_eh) symax = b0_eh;   // only differs where the reference reads out of bounds
-                    if (sxmax > b0_ew) sxmax = b0_ew;
-                    for (int ys = symin; ys < symax; ys++)
-                        for (int xs = sxmin; xs < sxmax; xs++) {
-                            if (xs != xb0 || ys != yb0) {
-                                float d = dist(xs, ys);
-                                if (d < dbest) {
-                                    dbest = d;
-                                    xbest = xs;
-                                    ybest = ys;
-                                }
-                            }
-                        }
+                if (do_rs_or_spatial) {   // patchtable.h:3853-3900
+                    if (p.spatial) {
+                        int symin = ybest - p.spatial, symax = ybest + p.spatial + 1;
+                        int sxmin = xbest - p.spatial, sxmax = xbest + p.spatial + 1;
+                        int xb0 = xbest, yb0 = ybest;
+                        if (symin < 0) symin = 0;
+                        else if (symax > b0_eh) symax = b0_eh;
+                        if (sxmin < 0) sxmin = 0;
+                        else if (sxmax > b0_ew) sxmax = b0_ew;
+                        if (symax > b0_eh) symax = b0_eh;   // only differs where the reference reads out of bounds
+                        if (sxmax > b0_ew) sxmax = b0_ew;
+                        for (int ys = symin; ys < symax; ys++)
+                            for (int xs = sxmin; xs < sxmax; xs++)
+                                if ((xs != xb0 || ys != yb0) && allowed_at(xs, ys)) consider(xs, ys);
+                    } else {
+                        int r = rng.next() & 7;
+                        int xs = xbest + dx_table[r], ys = ybest + dy_table[r];
+                        if (xs >= 0 && xs < b0_ew && ys >= 0 && ys < b0_eh && allowed_at(xs, ys)) consider(xs, ys);
+                    }
                 }
-                if (p.do_prop) {
+                if (p.do_prop) {   // patchtable.h:3903-4148
                     int ax_shift = x - qs, ay_shift = y - qs;
                     int bx_shift = xbest - qs, by_shift = ybest - qs;
                     int dy_min = 0, dy_max = sat_w_nopad, dx_min = 0, dx_max = sat_w_nopad;
@@ -933,30 +1120,37 @@ static void lookup_raster(const Table &T, const float *a, int ah, int aw, double
                     else if (bx_shift + dx_max > b_w) dx_max = b_w - bx_shift;
                     if (by_shift + dy_min < 0) dy_min = -by_shift;
                     else if (by_shift + dy_max > b_h) dy_max = b_h - by_shift;
-                    if (dx_min > 0 || dy_min > 0) {
-                        if (dx_min > 0)
-                            for (int dy = dy_min > 0 ? dy_min - 1 : dy_min; dy < dy_max; dy++) sat[(dy + 1) * sat_w + dx_min] = 0;
-                        if (dy_min > 0)
-                            for (int dx = dx_min; dx < dx_max; dx++) sat[(dy_min)*sat_w + dx + 1] = 0;
-                    }
-                    for (int dy = dy_min; dy < dy_max; dy++) {
-                        float row_sum = 0;
-                        for (int dx = dx_min; dx < dx_max; dx++) {
-                            float d0 = A(ay_shift + dy, ax_shift + dx, 0) - Bp(by_shift + dy, bx_shift + dx, 0);
-                            float d1 = A(ay_shift + dy, ax_shift + dx, 1) - Bp(by_shift + dy, bx_shift + dx, 1);
-                            float d2 = A(ay_shift + dy, ax_shift + dx, 2) - Bp(by_shift + dy, bx_shift + dx, 2);
-                            row_sum += d0 * d0 + d1 * d1 + d2 * d2;
-                            sat[(dy + 1) * sat_w + 1 + dx] = row_sum + sat[dy * sat_w + 1 + dx];
+                    if (!is_desc) {
+                        if (dx_min > 0 || dy_min > 0) {
+                            if (dx_min > 0)
+                                for (int dy = dy_min > 0 ? dy_min - 1 : dy_min; dy < dy_max; dy++) sat[(dy + 1) * sat_w + dx_min] = 0;
+                            if (dy_min > 0)
+                                for (int dx = dx_min; dx < dx_max; dx++) sat[(dy_min)*sat_w + dx + 1] = 0;
+                        }
+                        for (int dy = dy_min; dy < dy_max; dy++) {
+                            float row_sum = 0;
+                            for (int dx = dx_min; dx < dx_max; dx++) {
+                                const float *ap = &a[((size_t)(ay_shift + dy) * a_w + ax_shift + dx) * 3];
+                                const float *bp = &b[((size_t)(by_shift + dy) * b_w + bx_shift + dx) * 3];
+                                float d0 = ap[0] - bp[0], d1 = ap[1] - bp[1], d2 = ap[2] - bp[2];
+                                row_sum += d0 * d0 + d1 * d1 + d2 * d2;
+                                sat[(dy + 1) * sat_w + 1 + dx] = row_sum + sat[dy * sat_w + 1 + dx];
+                            }
                         }
                     }
                     int dy_max_patch = dy_max - 8 + 1, dx_max_patch = dx_max - 8 + 1;
+                    bool missing = false;
                     for (int dy = dy_min; dy < dy_max_patch; dy++) {
                         const float *s1 = &sat[dy * sat_w], *s2 = &sat[(dy + 8) * sat_w];
                         int ysrc = dy + by_shift;
                         for (int dx = dx_min; dx < dx_max_patch; dx++) {
                             int xsrc = dx + bx_shift;
-                            int dx2 = dx + 8;
-                            float dcur = s1[dx] - s1[dx2] - s2[dx] + s2[dx2];
+                            if (use_allowed && !allowed_at(xsrc, ysrc)) {
+                                missing = true;
+                                continue;
+                            }
+                            float dcur = is_desc ? desc_dist(Q.qdesc(ax_shift + dx, ay_shift + dy), T.bdesc(xsrc, ysrc), Q.C)
+                                                 : s1[dx] - s1[dx + 8] - s2[dx] + s2[dx + 8];
                             double *ap = ANN(ay_shift + dy, ax_shift + dx);
                             if (dcur < ap[2]) {
                                 ap[0] = xsrc;
@@ -965,9 +1159,16 @@ static void lookup_raster(const Table &T, const float *a, int ah, int aw, double
                             }
                         }
                     }
-                    if (dx_max < sat_w_nopad || dy_max < sat_w_nopad) {
+                    if (dx_max < sat_w_nopad || dy_max < sat_w_nopad || missing) {
                         int dy_start, dy_end, dx_start, dx_end;
-                        if (reverse) {
+                        if (missing) {
+                            dx_start = 0; dx_end = qs * 2;
+                            dy_start = 0; dy_end = qs * 2;
+                            if (ax_shift + dx_start < 0) dx_start = -ax_shift;
+                            if (ay_shift + dy_start < 0) dy_start = -ay_shift;
+                            if (ax_shift + dx_end > ann_w) dx_end = ann_w - ax_shift;
+                            if (ay_shift + dy_end > ann_h) dy_end = ann_h - ay_shift;
+                        } else if (reverse) {
                             dx_start = 0; dx_end = qs;
                             dy_start = 0; dy_end = qs;
                             if (ax_shift + dx_start < 0) dx_start = -ax_shift;
@@ -988,7 +1189,12 @@ static void lookup_raster(const Table &T, const float *a, int ah, int aw, double
                                     int xsrc = xbest + (ax_shift + dx) - x;
                                     if (xsrc < 0) xsrc = 0;
                                     else if (xsrc >= b0_ew) xsrc = b0_ew - 1;
-                                    float d = patch_ssd(a, aw, ax_shift + dx, ay_shift + dy, b, b_w, xsrc, ysrc);
+                                    if (use_allowed)
+                                        while (!allowed_at(xsrc, ysrc)) {
+                                            xsrc = rng.next() % b0_ew;
+                                            ysrc = rng.next() % b0_eh;
+                                        }
+                                    float d = dist(ax_shift + dx, ay_shift + dy, xsrc, ysrc);
                                     ap[2] = d;
                                     ap[0] = xsrc;
                                     ap[1] = ysrc;
@@ -1005,19 +1211,41 @@ static void lookup_raster(const Table &T, const float *a, int ah, int aw, double
             }
         }
     }
+    if (ann_prev && p.coherence_temporal != 0.0) {   // patchtable.h:4178-4193
+        for (long i = 0; i < (long)ann_w * ann_h; i++) {
+            double *q = &ann[i * 3];
+            const double *pp = &ann_prev[i * 3];
+            if (q[0] == pp[0] && q[1] == pp[1]) {
+                if ((p.recalc_dist_temporal && q[2] == dist_orig[i]) ||
+                    (!p.recalc_dist_temporal && q[2] == pp[2] * temporal_scale))
+                    q[2] /= temporal_scale;
+            }
+        }
+    }
 }
 
 // ---------------------------- lookup (this build's parallel form) ---------------------------
-// Model of the GPU lookup, evaluated patch-parallel with double-buffered fields so that every
-// patch of a round reads the previous round's field only:
-//   round 0 (candidates): table cell candidate; then the k-coherence list of the current best
-//     with the triangle test (patchtable.h:3779-3784); then the (2*spatial+1)^2 window around
-//     the best (patchtable.h:3853-3885);
-//   rounds 1..R (propagation): for the 8 neighbours n = p + J*(dx,dy) the candidate
-//     NNF(n) - J*(dx,dy) if inside b; then the spatial window around the best. Jumps J run
-//     from large to 1 (jump flooding), followed by unit-jump rounds.
-// Candidates are tried in a fixed order and replace the best only when strictly closer, and
-// every distance is the exact SSD of patch_ssd, so NNF_DIST is the exact SSD of its match.
+// Model of the GPU lookup. The reference visits the query patches of the lattice
+// {x % query_step == 0, y % query_step == 0} in raster order and scatters each visited patch's
+// offset to its (2*query_step+1)^2 neighbourhood. Here:
+//   init: every patch starts from ann_prev (clamped; distance recomputed and scaled, or the
+//     previous distance scaled, patchtable.h:3539-3585) or from nothing;
+//   candidates (lattice patches, in parallel): the cell of every table (patchtable.h:3664-3737;
+//     with no previous field the first table's candidate is taken unconditionally), then the
+//     k-coherence walk of the best with the triangle test (patchtable.h:3779-3784), then the
+//     spatial window (allowed positions only) or one random neighbour (do_rs without spatial);
+//   rounds 1..R (lattice, double-buffered): for the 8 neighbours n = v + J*(dx,dy) of the
+//     lattice the candidate NNF(n) - J*query_step*(dx,dy) if inside b and allowed; then the
+//     k-coherence walk again when kcoherence_iter asks for it after pass 0; then the spatial
+//     window or a random neighbour. J runs 8, 4, 2, 1, then unit jumps;
+//   gather (query_step > 1): every patch takes the best of the offsets of the lattice patches
+//     within query_step of it (clamped into b, allowed only), starting from its initial match;
+//   temporal: a patch that kept its initial match and distance gets it divided by the
+//     temporal scale (patchtable.h:4178-4193).
+// Candidates replace the best only when strictly closer, so NNF_DIST is the distance of the
+// stored match. Skips that leave every result unchanged: a spatial window already searched
+// around an unchanged best, a k-coherence list already walked at an unchanged best, and a
+// neighbour's candidate that repeats the previous round's (same jump, neither changed).
 static void jf_jumps(int rounds, std::vector<int> &J) {
     // e.g. rounds=6 -> 8,4,2,1,1,1
     J.clear();
@@ -1026,27 +1254,81 @@ static void jf_jumps(int rounds, std::vector<int> &J) {
     while ((int)J.size() < rounds) J.push_back(1);
 }
 
-// Counters of SSD evaluations in the parallel model (diagnostics): [0] table + k-coherence,
-// [1] spatial window of the candidate pass, [2 + 2r] neighbours of round r, [3 + 2r] spatial
-// window of round r.
+// Random neighbour of the random search (do_rs without spatial): a hash of the patch index and
+// the round replaces the reference's rand() & 7 (patchtable.h:3887).
+static inline int rs_pick(uint32_t i, uint32_t round) {
+    uint32_t h = i * 0x9E3779B1u + round * 0x85EBCA77u + 0x165667B1u;
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return (int)(h & 7u);
+}
+
+// Counters of distance evaluations in the parallel model (diagnostics): [0] tables +
+// k-coherence, [1] spatial window of the candidate pass, [2 + 2r] neighbours (and k-coherence) of
+// round r, [3 + 2r] spatial window of round r, [63] gather.
 static long g_ssd_count[64];
 
-static void lookup_parallel(const Table &T, const float *a, int ah, int aw, double *ann) {
+static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_in, int c_in, double *ann,
+                            const double *ann_prev, const float *sv) {
     for (auto &c : g_ssd_count) c = 0;
     const pt_params &p = T.p;
-    fvec a_wh;
-    descriptor(p, a, ah, aw, a_wh);
-    const int ann_h = ah - 7, ann_w = aw - 7;
-    const int bew = T.w - 7, beh = T.h - 7;
-    const int nd = T.ndims, AWp = aw + 7;
-    const float *b = T.b0.data();
-    const long N = (long)ann_h * ann_w;
-    std::vector<int32_t> cx(N), cy(N), nx(N), ny(N);
-    fvec cd(N), ndist(N);
-    // Returns true if the window improved the best. A window whose search found nothing around
-    // an unchanged best would find nothing again, so callers may skip it (results unchanged).
+    QuerySide Q;
+    query_side(T, a_in, h_in, w_in, c_in, Q);
+    const int ann_h = Q.ann_h(), ann_w = Q.ann_w();
+    const int bew = T.bew(), beh = T.beh();
+    const int qs = p.query_step;
+    const int vw = (ann_w + qs - 1) / qs, vh = (ann_h + qs - 1) / qs;
+    const long N = (long)ann_h * ann_w, NV = (long)vw * vh;
+    auto dist = [&](int x, int y, int xs, int ys) { return lookup_dist(T, Q, x, y, xs, ys); };
+    auto allowed_at = [&](int xs, int ys) { return T.is_allowed(xs, ys, p.allowed_index); };
+    const double temporal_scale = 1.0 / (1 + p.coherence_temporal);
+    static const int DX[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+    static const int DY[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+    // initial field
+    std::vector<int32_t> ix(N, 0), iy(N, 0);
+    fvec id(N, 0.f);
+    const bool have_prev = ann_prev != nullptr;
+    if (have_prev)
+        for (long i = 0; i < N; i++) {
+            const int x = (int)(i % ann_w), y = (int)(i / ann_w);
+            int xs = (int)ann_prev[i * 3], ys = (int)ann_prev[i * 3 + 1];
+            xs = xs < 0 ? 0 : (xs >= bew ? bew - 1 : xs);
+            ys = ys < 0 ? 0 : (ys >= beh ? beh - 1 : ys);
+            ix[i] = xs;
+            iy[i] = ys;
+            if (p.recalc_dist_temporal) {
+                const double tsc = sv ? (double)sv[i] : temporal_scale;
+                id[i] = (float)(dist(x, y, xs, ys) * tsc);
+            } else {
+                id[i] = (float)(ann_prev[i * 3 + 2] * temporal_scale);
+            }
+        }
+    const bool kcoh0 = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == 0);
+    const bool kcohR = p.kcoherence > 0 && (p.kcoherence_iter < 0 || (p.kcoherence_iter >= 1 && p.kcoherence_iter < p.prop_iters));
+    const bool rs = !p.spatial && p.do_rs;
+    // the k-coherence walk of the best's list; returns true if it improved the best
+    auto kwalk = [&](int x, int y, int &xb, int &yb, float &db, int slot) -> bool {
+        const int32_t *kp = &T.kpos[((size_t)yb * bew + xb) * T.K];
+        const float *kd = &T.kdist[((size_t)yb * bew + xb) * T.K];
+        bool changed = false;
+        for (int k = 0; k < p.kcoherence; k++) {
+            if (kd[k] >= p.triangle_factor * db) break;
+            int xs = INT_TO_X(kp[k]), ys = INT_TO_Y(kp[k]);
+            float d = dist(x, y, xs, ys);
+            g_ssd_count[slot]++;
+            if (d < db) {
+                db = d;
+                xb = xs;
+                yb = ys;
+                changed = true;
+            }
+        }
+        return changed;
+    };
     auto spatial = [&](int x, int y, int &xb, int &yb, float &db, int slot) -> bool {
-        if (!p.spatial) return false;
         bool changed = false;
         int symin = yb - p.spatial, symax = yb + p.spatial + 1;
         int sxmin = xb - p.spatial, sxmax = xb + p.spatial + 1;
@@ -1059,8 +1341,8 @@ static void lookup_parallel(const Table &T, const float *a, int ah, int aw, doub
         if (sxmax > bew) sxmax = bew;
         for (int ys = symin; ys < symax; ys++)
             for (int xs = sxmin; xs < sxmax; xs++)
-                if (xs != xb0 || ys != yb0) {
-                    float d = patch_ssd(a, aw, x, y, b, T.w, xs, ys);
+                if ((xs != xb0 || ys != yb0) && allowed_at(xs, ys)) {
+                    float d = dist(x, y, xs, ys);
                     g_ssd_count[slot]++;
                     if (d < db) {
                         db = d;
@@ -1071,61 +1353,100 @@ static void lookup_parallel(const Table &T, const float *a, int ah, int aw, doub
                 }
         return changed;
     };
-    std::vector<uint8_t> cdone(N), ndone(N), cchg(N, 1), nchg(N);
-    for (int y = 0; y < ann_h; y++)
-        for (int x = 0; x < ann_w; x++) {
-            const float *whp = &a_wh[((size_t)(y + 7) * AWp + x + 7) * nd];
-            int v = T.table[T.cell_of(whp)];
-            int xb = INT_TO_X(v) - 7, yb = INT_TO_Y(v) - 7;
-            float db = patch_ssd(a, aw, x, y, b, T.w, xb, yb);
-            g_ssd_count[0]++;
-            if (p.kcoherence > 0) {
-                const int32_t *kp = &T.kpos[((size_t)yb * bew + xb) * T.K];
-                const float *kd = &T.kdist[((size_t)yb * bew + xb) * T.K];
-                for (int k = 0; k < p.kcoherence; k++) {
-                    if (kd[k] >= p.triangle_factor * db) break;
-                    int xs = INT_TO_X(kp[k]), ys = INT_TO_Y(kp[k]);
-                    float d = patch_ssd(a, aw, x, y, b, T.w, xs, ys);
+    auto rsearch = [&](int x, int y, int &xb, int &yb, float &db, int round, int slot) -> bool {
+        const int r = rs_pick((uint32_t)((long)y * ann_w + x), (uint32_t)round);
+        const int xs = xb + DX[r], ys = yb + DY[r];
+        if (xs < 0 || xs >= bew || ys < 0 || ys >= beh || !allowed_at(xs, ys)) return false;
+        float d = dist(x, y, xs, ys);
+        g_ssd_count[slot]++;
+        if (d < db) {
+            db = d;
+            xb = xs;
+            yb = ys;
+            return true;
+        }
+        return false;
+    };
+    std::vector<int32_t> cx(NV), cy(NV), nx(NV), ny(NV);
+    fvec cd(NV), ndist(NV);
+    std::vector<uint8_t> cdone(NV), ndone(NV), cchg(NV, 1), nchg(NV), ckd(NV), nkd(NV);
+    for (int vy = 0; vy < vh; vy++)
+        for (int vx = 0; vx < vw; vx++) {
+            const int x = vx * qs, y = vy * qs;
+            const long i = (long)y * ann_w + x;
+            bool have = have_prev;
+            int xb = ix[i], yb = iy[i];
+            float db = id[i];
+            if (p.do_table_lookup) {
+                const float *whp = Q.qdesc(x, y);
+                int v = T.table[T.cell_of(whp)];
+                int xs = INT_TO_X(v) - 7, ys = INT_TO_Y(v) - 7;
+                if (!have || xs != xb || ys != yb) {
+                    float d = dist(x, y, xs, ys);
                     g_ssd_count[0]++;
-                    if (d < db) {
+                    if (!have || d < db) {
                         db = d;
                         xb = xs;
                         yb = ys;
                     }
+                    have = true;
+                }
+                int xp = xs, yp = ys;
+                for (const Table *E : T.extra) {
+                    int v1 = E->table[E->cell_of(whp)];
+                    int x1 = INT_TO_X(v1) - 7, y1 = INT_TO_Y(v1) - 7;
+                    if ((x1 != xb || y1 != yb) && (x1 != xp || y1 != yp)) {
+                        float d = dist(x, y, x1, y1);
+                        g_ssd_count[0]++;
+                        if (d < db) {
+                            db = d;
+                            xb = x1;
+                            yb = y1;
+                        }
+                    }
+                    xp = x1;
+                    yp = y1;
                 }
             }
-            bool changed = spatial(x, y, xb, yb, db, 1);
-            long i = (long)y * ann_w + x;
-            cx[i] = xb;
-            cy[i] = yb;
-            cd[i] = db;
-            cdone[i] = p.spatial && !changed;
+            bool kd = false;
+            if (kcoh0) kd = !kwalk(x, y, xb, yb, db, 0);
+            bool done = false;
+            if (p.spatial) {
+                const bool changed = spatial(x, y, xb, yb, db, 1);
+                done = !changed;
+                if (changed) kd = false;
+            } else if (rs) {
+                if (rsearch(x, y, xb, yb, db, 0, 1)) kd = false;
+            }
+            const long v = (long)vy * vw + vx;
+            cx[v] = xb;
+            cy[v] = yb;
+            cd[v] = db;
+            cdone[v] = done;
+            ckd[v] = kd;
         }
     std::vector<int> J;
     jf_jumps(p.jf_rounds, J);
-    static const int DX[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
-    static const int DY[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
     for (size_t r = 0; r < J.size(); r++) {
-        int j = J[r];
-        // A candidate from neighbour n repeats the previous round's when that round had the same
-        // jump and neither the patch nor n changed in it; it was rejected against the same best
-        // then, so it is skipped (results unchanged).
+        const int j = J[r];
         const bool same_jump = r > 0 && J[r - 1] == j;
-        for (int y = 0; y < ann_h; y++)
-            for (int x = 0; x < ann_w; x++) {
-                long i = (long)y * ann_w + x;
-                int xb = cx[i], yb = cy[i];
-                float db = cd[i];
+        for (int vy = 0; vy < vh; vy++)
+            for (int vx = 0; vx < vw; vx++) {
+                const long v = (long)vy * vw + vx;
+                const int x = vx * qs, y = vy * qs;
+                int xb = cx[v], yb = cy[v];
+                float db = cd[v];
                 bool moved = false;
                 for (int t = 0; t < 8; t++) {
-                    int qx = x + j * DX[t], qy = y + j * DY[t];
-                    if (qx < 0 || qy < 0 || qx >= ann_w || qy >= ann_h) continue;
-                    long qi = (long)qy * ann_w + qx;
-                    if (same_jump && !cchg[i] && !cchg[qi]) continue;
-                    int xs = cx[qi] - j * DX[t], ys = cy[qi] - j * DY[t];
+                    const int qx = vx + j * DX[t], qy = vy + j * DY[t];
+                    if (qx < 0 || qy < 0 || qx >= vw || qy >= vh) continue;
+                    const long qv = (long)qy * vw + qx;
+                    if (same_jump && !cchg[v] && !cchg[qv]) continue;
+                    const int xs = cx[qv] - j * qs * DX[t], ys = cy[qv] - j * qs * DY[t];
                     if (xs < 0 || ys < 0 || xs >= bew || ys >= beh) continue;
                     if (xs == xb && ys == yb) continue;
-                    float d = patch_ssd(a, aw, x, y, b, T.w, xs, ys);
+                    if (!allowed_at(xs, ys)) continue;
+                    float d = dist(x, y, xs, ys);
                     g_ssd_count[2 + 2 * r]++;
                     if (d < db) {
                         db = d;
@@ -1134,28 +1455,89 @@ static void lookup_parallel(const Table &T, const float *a, int ah, int aw, doub
                         moved = true;
                     }
                 }
-                bool done = cdone[i] && !moved;
+                bool kd = ckd[v] && !moved;
+                bool kchg = false;
+                if (kcohR && !kd) {
+                    kchg = kwalk(x, y, xb, yb, db, 2 + 2 * (int)r);
+                    kd = !kchg;
+                }
+                bool done = cdone[v] && !moved && !kchg;
                 bool schg = false;
                 if (p.spatial && !done) {
                     schg = spatial(x, y, xb, yb, db, 3 + 2 * (int)r);
                     done = !schg;
+                } else if (rs) {
+                    schg = rsearch(x, y, xb, yb, db, (int)r + 1, 3 + 2 * (int)r);
                 }
-                nx[i] = xb;
-                ny[i] = yb;
-                ndist[i] = db;
-                ndone[i] = done;
-                nchg[i] = moved || schg;
+                if (schg) kd = false;
+                nx[v] = xb;
+                ny[v] = yb;
+                ndist[v] = db;
+                ndone[v] = done;
+                nchg[v] = moved || kchg || schg;
+                nkd[v] = kd;
             }
         cx.swap(nx);
         cy.swap(ny);
         cd.swap(ndist);
         cdone.swap(ndone);
         cchg.swap(nchg);
+        ckd.swap(nkd);
+    }
+    // output: the lattice itself, or the gather over the lattice neighbours of every patch
+    std::vector<int32_t> ox(N), oy(N);
+    fvec od(N);
+    if (qs == 1) {
+        for (long i = 0; i < N; i++) {
+            ox[i] = cx[i];
+            oy[i] = cy[i];
+            od[i] = cd[i];
+        }
+    } else {
+        for (int y = 0; y < ann_h; y++)
+            for (int x = 0; x < ann_w; x++) {
+                const long i = (long)y * ann_w + x;
+                bool have = have_prev;
+                int xb = ix[i], yb = iy[i];
+                float db = id[i];
+                const int vy0 = y / qs - (y % qs == 0 ? 1 : 0), vy1 = y / qs + 1;
+                const int vx0 = x / qs - (x % qs == 0 ? 1 : 0), vx1 = x / qs + 1;
+                for (int vy = vy0 < 0 ? 0 : vy0; vy <= vy1 && vy < vh; vy++)
+                    for (int vx = vx0 < 0 ? 0 : vx0; vx <= vx1 && vx < vw; vx++) {
+                        const long v = (long)vy * vw + vx;
+                        int xs = cx[v] + (x - vx * qs), ys = cy[v] + (y - vy * qs);
+                        xs = xs < 0 ? 0 : (xs >= bew ? bew - 1 : xs);
+                        ys = ys < 0 ? 0 : (ys >= beh ? beh - 1 : ys);
+                        if (!allowed_at(xs, ys)) continue;
+                        if (have && xs == xb && ys == yb) continue;
+                        float d = dist(x, y, xs, ys);
+                        g_ssd_count[63]++;
+                        if (!have || d < db) {
+                            db = d;
+                            xb = xs;
+                            yb = ys;
+                            have = true;
+                        }
+                    }
+                if (!have) {   // every proposal disallowed: the owner lattice patch's match
+                    const long v = (long)(y / qs) * vw + x / qs;
+                    xb = cx[v];
+                    yb = cy[v];
+                    db = dist(x, y, xb, yb);
+                }
+                ox[i] = xb;
+                oy[i] = yb;
+                od[i] = db;
+            }
     }
     for (long i = 0; i < N; i++) {
-        ann[i * 3 + 0] = cx[i];
-        ann[i * 3 + 1] = cy[i];
-        ann[i * 3 + 2] = cd[i];
+        double d = od[i];
+        if (have_prev && p.coherence_temporal != 0.0 && (double)ox[i] == ann_prev[i * 3] &&
+            (double)oy[i] == ann_prev[i * 3 + 1] && od[i] == id[i])
+            d = d / temporal_scale;
+        ann[i * 3 + 0] = ox[i];
+        ann[i * 3 + 1] = oy[i];
+        ann[i * 3 + 2] = d;
     }
 }
 
@@ -1245,7 +1627,9 @@ int orc_partition(const float *vals, int n, int nslices, float min_val, float ma
     return 0;
 }
 
-void *orc_build(const pt_params *p, const float *b, int h, int w) { return build(*p, b, h, w); }
+void *orc_build(const pt_params *p, const float *b, int h, int w, int c, const int32_t *allowed) {
+    return build(*p, b, h, w, c, allowed);
+}
 
 void orc_free(void *t) { delete (Table *)t; }
 
@@ -1256,6 +1640,14 @@ int orc_set_params(void *t, const pt_params *p) {
     int g = T->p.grid_ndims;
     T->p = *p;
     T->p.grid_ndims = g;
+    return 0;
+}
+
+int orc_shape(void *t, int *h, int *w, int *nd) {
+    Table *T = (Table *)t;
+    *h = T->h;
+    *w = T->w;
+    *nd = T->ndims;
     return 0;
 }
 
@@ -1365,8 +1757,8 @@ int orc_dt(int G, const int *nslices, int32_t *table, int algo, int dt_iters) {
     return 0;
 }
 
-int orc_lookup_raster(void *t, const float *a, int h, int w, double *ann) {
-    lookup_raster(*(Table *)t, a, h, w, ann);
+int orc_lookup_raster(void *t, const float *a, int h, int w, int c, double *ann, const double *prev, const float *sv) {
+    lookup_raster(*(Table *)t, a, h, w, c, ann, prev, sv);
     return 0;
 }
 
@@ -1375,9 +1767,35 @@ int orc_ssd_counts(long *out) {
     return 0;
 }
 
-int orc_lookup_parallel(void *t, const float *a, int h, int w, double *ann) {
-    lookup_parallel(*(Table *)t, a, h, w, ann);
+int orc_lookup_parallel(void *t, const float *a, int h, int w, int c, double *ann, const double *prev, const float *sv) {
+    lookup_parallel(*(Table *)t, a, h, w, c, ann, prev, sv);
     return 0;
+}
+
+float orc_desc_dist(const float *a, const float *b, int C) { return desc_dist(a, b, C); }
+
+// recompute_mean_dist<1> (patchtable_main.cpp:103-172, descriptor mode): float64 squared
+// distance of the descriptors of every match (clamped), mean of the square roots.
+double orc_recompute_mean_dist_desc(const float *a, int ah, int aw, const float *b, int bh, int bw, int C,
+                                    const double *ann, double *dist_out) {
+    double sum = 0.0;
+    for (int ay = 0; ay < ah; ay++)
+        for (int ax = 0; ax < aw; ax++) {
+            const double *q = &ann[((size_t)ay * aw + ax) * 3];
+            int bx = (int)q[0], by = (int)q[1];
+            if (bx < 0) bx = 0;
+            else if (bx >= bw) bx = bw - 1;
+            if (by < 0) by = 0;
+            else if (by >= bh) by = bh - 1;
+            double dist = 0.0;
+            for (int c = 0; c < C; c++) {
+                double delta = (double)a[((size_t)ay * aw + ax) * C + c] - (double)b[((size_t)by * bw + bx) * C + c];
+                dist += delta * delta;
+            }
+            if (dist_out) dist_out[(size_t)ay * aw + ax] = dist;
+            sum += sqrt(dist);
+        }
+    return sum / double(aw * ah);
 }
 
 // Mean of sqrt(max(SSD,0)) over the field (get_mean_dist, patchtable_main.cpp:90-101).

@@ -40,7 +40,8 @@ def lib():
             "orc_patch_ssd": (ctypes.c_float, [_f32p, _int, _int, _int, _f32p, _int, _int, _int]),
             "orc_scale_slices_to_limit": (_int, [_int, _P(_int), ctypes.c_double]),
             "orc_partition": (_int, [_f32p, _int, _int, ctypes.c_float, ctypes.c_float, _f32p, _i32p]),
-            "orc_build": (ctypes.c_void_p, [_P(PtParams), _f32p, _int, _int]),
+            "orc_build": (ctypes.c_void_p, [_P(PtParams), _f32p, _int, _int, _int, _i32p]),
+            "orc_shape": (_int, [ctypes.c_void_p, _P(_int), _P(_int), _P(_int)]),
             "orc_free": (None, [ctypes.c_void_p]),
             "orc_set_params": (_int, [ctypes.c_void_p, _P(PtParams)]),
             "orc_info": (_int, [ctypes.c_void_p, _P(_int), _P(_int), _P(ctypes.c_int64), _P(_int), _P(_int)]),
@@ -51,8 +52,10 @@ def lib():
             "orc_get_kcoherence": (_int, [ctypes.c_void_p, _i32p, _f32p]),
             "orc_save": (_int, [ctypes.c_void_p, ctypes.c_char_p]),
             "orc_dt": (_int, [_int, _P(_int), _i32p, _int, _int]),
-            "orc_lookup_raster": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f64p]),
-            "orc_lookup_parallel": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f64p]),
+            "orc_lookup_raster": (_int, [ctypes.c_void_p, _f32p, _int, _int, _int, _f64p, _f64p, _f32p]),
+            "orc_lookup_parallel": (_int, [ctypes.c_void_p, _f32p, _int, _int, _int, _f64p, _f64p, _f32p]),
+            "orc_desc_dist": (ctypes.c_float, [_f32p, _f32p, _int]),
+            "orc_recompute_mean_dist_desc": (ctypes.c_double, [_f32p, _int, _int, _f32p, _int, _int, _int, _f64p, _f64p]),
             "orc_mean_dist": (ctypes.c_double, [_f64p, ctypes.c_long]),
             "orc_recompute_mean_dist": (ctypes.c_double, [_f32p, _int, _int, _f32p, _int, _int, _f64p, _f64p]),
         }
@@ -83,12 +86,16 @@ def i32(a):
 class OracleTable:
     """CPU restatement of PatchTable construction (oracle/pt_oracle.cpp `build`)."""
 
-    def __init__(self, params, b):
+    def __init__(self, params, b, allowed=None):
         b = np.ascontiguousarray(b, dtype=np.float32)
-        self.h, self.w = b.shape[:2]
         self.b = b
         self.params = params
-        self.handle = lib().orc_build(ctypes.byref(params), f32(b), self.h, self.w)
+        self.allowed = None if allowed is None else np.ascontiguousarray(allowed, dtype=np.int32)
+        self.handle = lib().orc_build(ctypes.byref(params), f32(b), b.shape[0], b.shape[1], b.shape[2],
+                                      None if self.allowed is None else i32(self.allowed))
+        h, w, nd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().orc_shape(self.handle, ctypes.byref(h), ctypes.byref(w), ctypes.byref(nd))
+        self.h, self.w, self.nd = h.value, w.value, nd.value
         G = ctypes.c_int()
         ns = (ctypes.c_int * 64)()
         nc = ctypes.c_int64()
@@ -114,7 +121,7 @@ class OracleTable:
         assert lib().orc_save(self.handle, str(path).encode()) == 0
 
     def descriptor(self):
-        nd = self.params.ndims
+        nd = self.nd
         out = np.empty((self.h + 7, self.w + 7, nd), np.float32)
         lib().orc_get_descriptor(self.handle, f32(out))
         return out
@@ -143,17 +150,26 @@ class OracleTable:
         lib().orc_get_kcoherence(self.handle, i32(pos), f32(dist))
         return pos.reshape(self.h - 7, self.w - 7, self.K), dist.reshape(self.h - 7, self.w - 7, self.K)
 
-    def lookup_raster(self, a):
+    def _ann_shape(self, a):
+        if self.params.is_descriptor:
+            pad = 14 if self.params.descriptor_padded else 0
+            return (a.shape[0] - pad, a.shape[1] - pad, 3)
+        return (a.shape[0] - 7, a.shape[1] - 7, 3)
+
+    def _lookup(self, fn, a, prev, sv):
         a = np.ascontiguousarray(a, dtype=np.float32)
-        ann = np.empty((a.shape[0] - 7, a.shape[1] - 7, 3), np.float64)
-        lib().orc_lookup_raster(self.handle, f32(a), a.shape[0], a.shape[1], f64(ann))
+        ann = np.empty(self._ann_shape(a), np.float64)
+        pv = None if prev is None else np.ascontiguousarray(prev, dtype=np.float64)
+        s = None if sv is None else np.ascontiguousarray(sv, dtype=np.float32)
+        fn(self.handle, f32(a), a.shape[0], a.shape[1], a.shape[2], f64(ann),
+           None if pv is None else f64(pv), None if s is None else f32(s))
         return ann
 
-    def lookup_parallel(self, a):
-        a = np.ascontiguousarray(a, dtype=np.float32)
-        ann = np.empty((a.shape[0] - 7, a.shape[1] - 7, 3), np.float64)
-        lib().orc_lookup_parallel(self.handle, f32(a), a.shape[0], a.shape[1], f64(ann))
-        return ann
+    def lookup_raster(self, a, prev=None, sv=None):
+        return self._lookup(lib().orc_lookup_raster, a, prev, sv)
+
+    def lookup_parallel(self, a, prev=None, sv=None):
+        return self._lookup(lib().orc_lookup_parallel, a, prev, sv)
 
 
 def mean_dist(ann):
@@ -168,6 +184,17 @@ def recompute(a, b, ann):
     ann = np.ascontiguousarray(ann, dtype=np.float64)
     d = np.empty(ann.shape[:2], np.float64)
     m = lib().orc_recompute_mean_dist(f32(a), a.shape[0], a.shape[1], f32(b), b.shape[0], b.shape[1], f64(ann), f64(d))
+    return m, d
+
+
+def recompute_desc(a, b, ann):
+    """Descriptor mode: (mean sqrt of the float64 squared descriptor distance, per-patch values)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    ann = np.ascontiguousarray(ann, dtype=np.float64)
+    d = np.empty(ann.shape[:2], np.float64)
+    m = lib().orc_recompute_mean_dist_desc(f32(a), a.shape[0], a.shape[1], f32(b), b.shape[0], b.shape[1], a.shape[2],
+                                           f64(ann), f64(d))
     return m, d
 
 
