@@ -244,6 +244,10 @@ int pt_table_get_partition(const pt_table *t, int dim, float *min_val, float *ma
 int pt_table_get_cells(const pt_table *t, int32_t *table_out);
 int pt_table_get_populated(const pt_table *t, int32_t *table_out);
 int pt_table_get_kcoherence(const pt_table *t, int32_t *pos_out, float *dist_out);
+/* The i-th further table of ntables > 1 (i = 0 .. ntables-2; patchtable.h:3359-3372). */
+int pt_table_extra_info(const pt_table *t, int i, int *nslices /* [PT_MAX_DIMS] */, int64_t *ncells,
+                        int *table_add_count);
+int pt_table_get_extra_cells(const pt_table *t, int i, int32_t *table_out);
 /* Query-side descriptor of the most recent lookup, (h+7) x (w+7) x ndims. */
 int pt_table_get_query_descriptor(const pt_table *t, float *wh_out);
 /* Per-stage device times (milliseconds) of the last create / lookup, from HIP events on the
@@ -264,6 +268,9 @@ int pt_gck_dev(const float *img_dev, int h, int w, int n, int nchroma, int conve
 int pt_patch_ssd_dev(const float *a_dev, int aw, const float *b_dev, int bw,
                      const int32_t *ax, const int32_t *ay, const int32_t *bx, const int32_t *by,
                      int npairs, float *out_dev);
+/* Descriptor-mode distance (patch_dist_approx, patchtable.h:376-437) of n descriptor pairs
+ * a[i*C .. i*C+C-1], b[i*C ..] (device arrays), C >= 4, into out[n]. */
+int pt_desc_dist_dev(const float *a_dev, const float *b_dev, int C, int n, float *out_dev);
 
 /* ---- Seeded synthetic test images (the generator used by the tests and the benchmark) ----
  * a = 1/f^2 spectral noise plus 40 random coloured rectangles/ellipses; b = a warped by a

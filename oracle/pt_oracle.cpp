@@ -1236,7 +1236,8 @@ static void lookup_raster(const Table &T, const float *a_in, int h_in, int w_in,
 //     spatial window (allowed positions only) or one random neighbour (do_rs without spatial);
 //   rounds 1..R (lattice, double-buffered): for the 8 neighbours n = v + J*(dx,dy) of the
 //     lattice the candidate NNF(n) - J*query_step*(dx,dy) if inside b and allowed; then the
-//     k-coherence walk again when kcoherence_iter asks for it after pass 0; then the spatial
+//     k-coherence walk again when kcoherence_iter asks for it after pass 0 or when there are no
+//     table candidates (do_table_lookup 0); then the spatial
 //     window or a random neighbour. J runs 8, 4, 2, 1, then unit jumps;
 //   gather (query_step > 1): every patch takes the best of the offsets of the lattice patches
 //     within query_step of it (clamped into b, allowed only), starting from its initial match;
@@ -1307,7 +1308,10 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
             }
         }
     const bool kcoh0 = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == 0);
-    const bool kcohR = p.kcoherence > 0 && (p.kcoherence_iter < 0 || (p.kcoherence_iter >= 1 && p.kcoherence_iter < p.prop_iters));
+    // k-coherence walks in the rounds when kcoherence_iter asks for them after the first pass, and
+    // without table candidates (the walk then has to follow the propagated matches).
+    const bool kcohR = p.kcoherence > 0 && (p.kcoherence_iter < 0 || (p.kcoherence_iter >= 1 && p.kcoherence_iter < p.prop_iters) ||
+                                            !p.do_table_lookup);
     const bool rs = !p.spatial && p.do_rs;
     // the k-coherence walk of the best's list; returns true if it improved the best
     auto kwalk = [&](int x, int y, int &xb, int &yb, float &db, int slot) -> bool {
@@ -1426,7 +1430,7 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
             ckd[v] = kd;
         }
     std::vector<int> J;
-    jf_jumps(p.jf_rounds, J);
+    jf_jumps(p.do_prop ? p.jf_rounds : 0, J);   // do_prop 0: no propagation
     for (size_t r = 0; r < J.size(); r++) {
         const int j = J[r];
         const bool same_jump = r > 0 && J[r - 1] == j;
@@ -1658,6 +1662,23 @@ int orc_info(void *t, int *G, int *nslices, int64_t *ncells, int *add_count, int
     *ncells = T->ncells;
     *add_count = T->table_add_count;
     *K = T->K;
+    return 0;
+}
+
+int orc_extra_info(void *t, int i, int *nslices, int64_t *ncells, int *add_count) {
+    Table *T = (Table *)t;
+    if (i < 0 || i >= (int)T->extra.size()) return -1;
+    const Table *E = T->extra[i];
+    for (int d = 0; d < E->G; d++) nslices[d] = E->nslices[d];
+    *ncells = E->ncells;
+    *add_count = E->table_add_count;
+    return 0;
+}
+
+int orc_get_extra_table(void *t, int i, int32_t *out) {
+    Table *T = (Table *)t;
+    if (i < 0 || i >= (int)T->extra.size()) return -1;
+    std::copy(T->extra[i]->table.begin(), T->extra[i]->table.end(), out);
     return 0;
 }
 

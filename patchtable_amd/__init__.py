@@ -99,7 +99,22 @@ class PatchTable:
             h, w, c = b.shape
             check(load().pt_table_create(ctypes.byref(p.raw), b.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                          h, w, c, allowed, ctypes.byref(self._t)))
-        self.b_shape = (h, w, c)
+        self._shape()
+
+    def _shape(self):
+        """Table "image" size (b, or the descriptor size + 7 in descriptor mode) and dims."""
+        h, w, nd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(load().pt_table_shape(self._t, ctypes.byref(h), ctypes.byref(w), ctypes.byref(nd)))
+        self.b_shape = (h.value, w.value, 3)
+        self.ndims = nd.value
+
+    def ann_shape(self, qh, qw):
+        """NNF shape for a query of qh x qw: (qh-7, qw-7) images; (qh, qw) descriptors
+        ((qh-14, qw-14) when descriptor_padded)."""
+        if self.p.is_descriptor:
+            pad = 14 if self.p.descriptor_padded else 0
+            return (qh - pad, qw - pad, 3)
+        return (qh - 7, qw - 7, 3)
 
     @classmethod
     def load(cls, p, filename):
@@ -107,9 +122,7 @@ class PatchTable:
         self.p = p
         self._t = ctypes.c_void_p()
         check(load().pt_table_load(ctypes.byref(p.raw), filename.encode(), ctypes.byref(self._t)))
-        h, w, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        check(load().pt_table_shape(self._t, ctypes.byref(h), ctypes.byref(w), ctypes.byref(c)))
-        self.b_shape = (h.value, w.value, 3)
+        self._shape()
         return self
 
     def close(self):
@@ -133,7 +146,7 @@ class PatchTable:
             import torch
             h, w, c = (int(s) for s in a.shape)
             if ann is None:
-                ann = torch.empty((h - 7, w - 7, 3), dtype=torch.float64, device=a.device)
+                ann = torch.empty(self.ann_shape(h, w), dtype=torch.float64, device=a.device)
             prev = ctypes.c_void_p(ann_prev.data_ptr()) if ann_prev is not None else None
             sv = ctypes.c_void_p(coherence_temporal_sv.data_ptr()) if coherence_temporal_sv is not None else None
             check(load().pt_table_lookup_dev(self._t, ctypes.c_void_p(a.data_ptr()), h, w, c,
@@ -142,7 +155,7 @@ class PatchTable:
             a = np.ascontiguousarray(a, dtype=np.float32)
             h, w, c = a.shape
             if ann is None:
-                ann = np.empty((h - 7, w - 7, 3), np.float64)
+                ann = np.empty(self.ann_shape(h, w), np.float64)
             prev = None
             if ann_prev is not None:
                 self._prev = np.ascontiguousarray(ann_prev, dtype=np.float64)
@@ -173,12 +186,12 @@ class PatchTable:
 
     def descriptor(self):
         h, w, _ = self.b_shape
-        out = np.empty((h + 7, w + 7, self.p.ndims), np.float32)
+        out = np.empty((h + 7, w + 7, self.ndims), np.float32)
         check(load().pt_table_get_descriptor(self._t, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
 
     def query_descriptor(self, h, w):
-        out = np.empty((h + 7, w + 7, self.p.ndims), np.float32)
+        out = np.empty((h + 7, w + 7, self.ndims), np.float32)
         check(load().pt_table_get_query_descriptor(self._t, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
 
@@ -200,6 +213,20 @@ class PatchTable:
     def populated(self):
         out = np.empty(self.info()["ncells"], np.int32)
         check(load().pt_table_get_populated(self._t, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
+    def extra_info(self, i):
+        """nslices, ncells, table_add_count of further table i (ntables > 1)."""
+        ns = (ctypes.c_int * _ffi.PT_MAX_DIMS)()
+        nc = ctypes.c_int64()
+        add = ctypes.c_int()
+        check(load().pt_table_extra_info(self._t, i, ns, ctypes.byref(nc), ctypes.byref(add)))
+        G = self.info()["grid_ndims"]
+        return dict(nslices=[ns[k] for k in range(G)], ncells=nc.value, table_add_count=add.value)
+
+    def extra_cells(self, i):
+        out = np.empty(self.extra_info(i)["ncells"], np.int32)
+        check(load().pt_table_get_extra_cells(self._t, i, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return out
 
     def kcoherence(self):

@@ -134,11 +134,14 @@ SIGNATURES = {
     "pt_table_get_partition": (_int, [_vp, _int, _f32p, _f32p, _f32p, _i32p]),
     "pt_table_get_cells": (_int, [_vp, _i32p]),
     "pt_table_get_populated": (_int, [_vp, _i32p]),
+    "pt_table_extra_info": (_int, [_vp, _int, _P(_int), _i64p, _P(_int)]),
+    "pt_table_get_extra_cells": (_int, [_vp, _int, _i32p]),
     "pt_table_get_kcoherence": (_int, [_vp, _i32p, _f32p]),
     "pt_table_get_query_descriptor": (_int, [_vp, _f32p]),
     "pt_table_stage_ms": (_int, [_vp, _f64p]),
     "pt_gck_dev": (_int, [_vp, _int, _int, _int, _int, _int, _vp]),
     "pt_patch_ssd_dev": (_int, [_vp, _int, _vp, _int, _vp, _vp, _vp, _vp, _int, _vp]),
+    "pt_desc_dist_dev": (_int, [_vp, _vp, _int, _int, _vp]),
     "pt_synth_pair": (_int, [ctypes.c_uint64, _int, _int, _f32p, _f32p]),
     "pt_table_shape": (_int, [_vp, _P(_int), _P(_int), _P(_int)]),
     "pt_image_size": (_int, [ctypes.c_char_p, _P(_int), _P(_int), _P(_int)]),

@@ -11,9 +11,12 @@
 //      order; those at pixel distance^2 < min_dist^2 are dropped;
 //   3. the K candidates nearest in descriptor L2 (all dims, summed in index order; ties to the
 //      smaller raster index, duplicates removed) are kept in a register top-K list;
-//   4. they are re-scored by exact SSD and sorted by (SSD, XY_TO_INT) as the reference sorts its
-//      (dist, (index, valid)) triples (patchtable.h:2815-2835).
-// Lists shorter than K repeat their last entry (p itself with SSD 0 when empty).
+//   4. they are re-scored by the lookup distance (exact RGB SSD; patch_dist_approx of the
+//      descriptors in descriptor mode) and sorted by (distance, XY_TO_INT) as the reference sorts
+//      its (dist, (index, valid)) triples (patchtable.h:2815-2835).
+// Lists shorter than K repeat their last entry (p itself with distance 0 when empty). With an
+// allowed mask only allowed patches are listed and searched (TABLE_FOR_ALLOWED_PATCHES_STEP,
+// patchtable.h:2711); a patch outside the mask gets K entries (p, +inf), which end a walk at once.
 #include "pt_device.h"
 #include "pt_kernels.h"
 #include "pt_tile.h"
@@ -29,8 +32,10 @@ struct KeyArgs {
     int half[PT_MAX_DIMS];
 };
 
+// Keys carry a flag above the Morton bits for patches outside the mask, which sorts them last.
 __global__ void k_keys(const float *__restrict__ desc, size_t plane, int Wp, int bew, long N, DevQuant q, KeyArgs ka,
-                       uint64_t *__restrict__ k0, uint64_t *__restrict__ k1, int32_t *__restrict__ idx) {
+                       const int32_t *__restrict__ allowed, int aidx, uint64_t *__restrict__ k0,
+                       uint64_t *__restrict__ k1, int32_t *__restrict__ idx) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int px = (int)(i % bew), py = (int)(i / bew);
@@ -44,9 +49,20 @@ __global__ void k_keys(const float *__restrict__ desc, size_t plane, int Wp, int
             a = (a << 1) | ((bins[d] >> bit) & 1u);
             b = (b << 1) | (((bins[d] + (uint32_t)ka.half[d]) >> bit) & 1u);
         }
+    if (allowed && allowed[i] != aidx) {
+        a |= 1ull << (ka.Gk * ka.B);
+        b |= 1ull << (ka.Gk * ka.B);
+    }
     k0[i] = a;
     k1[i] = b;
     idx[i] = (int32_t)i;
+}
+
+__global__ void k_count_allowed(const int32_t *__restrict__ allowed, int aidx, long N, int *__restrict__ cnt) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = i < N && allowed[i] == aidx;
+    unsigned long long m = __ballot(ok);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(cnt, __popcll(m));
 }
 
 __global__ void k_positions(const int32_t *__restrict__ perm, long N, int32_t *__restrict__ pos) {
@@ -65,21 +81,29 @@ __global__ void k_gather_desc(const float *__restrict__ desc, size_t plane, int 
 }
 
 // One lane per database patch p, in 64 x 4 raster tiles whose pixels are staged in LDS for the
-// exact-SSD re-rank. ND is the descriptor width when known at compile time (0: runtime `nd`).
-template <int KMAX, int ND>
+// exact-SSD re-rank (DESC: re-rank by descriptor distance, nothing staged). ND is the descriptor
+// width when known at compile time (0: runtime `nd`). Na: allowed patches (sorted first).
+template <int KMAX, int ND, bool DESC>
 __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
     const int32_t *__restrict__ perm0, const int32_t *__restrict__ perm1, const int32_t *__restrict__ pos0,
-    const int32_t *__restrict__ pos1, const float *__restrict__ ds0, const float *__restrict__ ds1, long N, int bew,
-    int beh, int nd_rt, int K, int W, int md2, const float *__restrict__ b0, int bw, int bh, int32_t *__restrict__ kpos,
-    float *__restrict__ kdist) {
+    const int32_t *__restrict__ pos1, const float *__restrict__ ds0, const float *__restrict__ ds1, long Na, int bew,
+    int beh, int nd_rt, int K, int W, int md2, const float *__restrict__ b0, const float *__restrict__ bI, int bw,
+    int bh, const int32_t *__restrict__ allowed, int aidx, int32_t *__restrict__ kpos, float *__restrict__ kdist) {
     using namespace pt_tile;
-    __shared__ float4 tile[AX * AY];
-    stage_tile(b0, bh, bw, tile);
+    __shared__ float4 tile[DESC ? 1 : AX * AY];
+    if (!DESC) stage_tile(b0, bh, bw, tile);
     const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
     const int px = blockIdx.x * TILE_X + lx, py = blockIdx.y * TILE_Y + ly;
     if (px >= bew || py >= beh) return;
     const int nd = ND > 0 ? ND : nd_rt;
     const int i = py * bew + px;
+    if (allowed && allowed[i] != aidx) {
+        for (int k = 0; k < K; k++) {
+            kpos[(size_t)i * K + k] = PT_XY_TO_INT(px, py);
+            kdist[(size_t)i * K + k] = INFINITY;
+        }
+        return;
+    }
     float own[ND > 0 ? ND : PT_MAX_DIMS];
     const long r0 = pos0[i];
     for (int c = 0; c < nd; c++) own[c] = ds0[r0 * nd + c];
@@ -97,7 +121,7 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
         const int32_t *perm = o == 0 ? perm0 : perm1;
         const float *ds = o == 0 ? ds0 : ds1;
         const long r = o == 0 ? r0 : (long)pos1[i];
-        const long t0 = r - W < 0 ? 0 : r - W, t1 = r + W >= N ? N - 1 : r + W;
+        const long t0 = r - W < 0 ? 0 : r - W, t1 = r + W >= Na ? Na - 1 : r + W;
         for (long t = t0; t <= t1; t++) {
             if (t == r) continue;
             const int qi = perm[t];
@@ -144,8 +168,10 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
                 }
         }
     }
-    // exact SSD re-rank, then insertion by (SSD, XY_TO_INT)
-    const float4 *aq = tile + ly * AX + lx;
+    // exact re-rank, then insertion by (distance, XY_TO_INT)
+    const float4 *aq = tile + (DESC ? 0 : ly * AX + lx);
+    const int Wpb = bw + 7;
+    const float *own_i = bI ? bI + ((size_t)(py + 7) * Wpb + px + 7) * nd : nullptr;
     float ss[KMAX];
     int xy[KMAX];
 #pragma unroll
@@ -157,7 +183,8 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
     for (int a = 0; a < KMAX; a++) {
         if (a < cnt) {
             const int qx = bq[a] % bew, qy = bq[a] / bew;
-            float cs = ssd_staged<false>(aq, b0 + ((size_t)qy * bw + qx) * 3, bw * 3, 0.f);
+            float cs = DESC ? pt_desc_dist(own_i, bI + ((size_t)(qy + 7) * Wpb + qx + 7) * nd, nd)
+                            : ssd_staged<false>(aq, b0 + ((size_t)qy * bw + qx) * 3, bw * 3, 0.f);
             int cx = PT_XY_TO_INT(qx, qy);
 #pragma unroll
             for (int k = 0; k < KMAX; k++) {
@@ -194,9 +221,9 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
 
 }  // namespace
 
-int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, const float *b0, int bw, int bh,
-                         const DevQuant &q, const int *nslices_host, int K, int window, int min_dist2,
-                         int32_t *kpos, float *kdist, PtScratch &sc, hipStream_t s) {
+int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, const float *b0, const float *b_descI,
+                         int bw, int bh, const int32_t *allowed, int aidx, const DevQuant &q, const int *nslices_host,
+                         int K, int window, int min_dist2, int32_t *kpos, float *kdist, PtScratch &sc, hipStream_t s) {
     const int bew = bw - 7, beh = bh - 7;
     const long N = (long)bew * beh;
     if (K <= 0 || N <= 0) return PT_OK;
@@ -213,9 +240,9 @@ int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, con
     while ((1 << ka.B) < mx) ka.B++;
     if (ka.B == 0) ka.B = 1;
     ka.Gk = q.G;
-    while (ka.Gk * ka.B > 64) ka.Gk--;
+    while (ka.Gk * ka.B > 63) ka.Gk--;   // one bit above the key for the allowed flag
     for (int d = 0; d < PT_MAX_DIMS; d++) ka.half[d] = d < q.G ? (nslices_host[d] >> 1) : 0;
-    const int nbits = ka.Gk * ka.B;
+    const int nbits = ka.Gk * ka.B + 1;
 
     uint64_t *k0 = (uint64_t *)sc.get(20, N * 8), *k1 = (uint64_t *)sc.get(21, N * 8);
     uint64_t *k0s = (uint64_t *)sc.get(22, N * 8), *k1s = (uint64_t *)sc.get(23, N * 8);
@@ -226,7 +253,19 @@ int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, con
     if (!k0 || !k1 || !k0s || !k1s || !idx || !perm0 || !perm1 || !pos0 || !pos1 || !ds0 || !ds1)
         return pt_fail(PT_E_NOMEM, "kcoherence: scratch allocation failed");
     const int TB = 256;
-    hipLaunchKernelGGL(k_keys, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, N, q, ka, k0, k1, idx);
+    hipLaunchKernelGGL(k_keys, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, N, q, ka, allowed, aidx, k0,
+                       k1, idx);
+    long Na = N;   // allowed patches
+    if (allowed) {
+        int *cnt = (int *)sc.get(32, sizeof(int));
+        if (!cnt) return pt_fail(PT_E_NOMEM, "kcoherence: scratch allocation failed");
+        PT_HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_count_allowed, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, allowed, aidx, N, cnt);
+        int hc = 0;
+        PT_HIP_TRY(hipMemcpyAsync(&hc, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+        PT_HIP_TRY(hipStreamSynchronize(s));
+        Na = hc;
+    }
     size_t tmp_bytes = 0;
     PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k0s, idx, perm0, (int)N, 0, nbits, s));
     void *tmp = sc.get(30, tmp_bytes);
@@ -239,15 +278,18 @@ int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, con
     hipLaunchKernelGGL(k_gather_desc, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, ndims, perm1, N, ds1);
     dim3 grid((bew + pt_tile::TILE_X - 1) / pt_tile::TILE_X, (beh + pt_tile::TILE_Y - 1) / pt_tile::TILE_Y);
     dim3 blk(pt_tile::THREADS);
-#define PT_SELECT(KM, ND)                                                                                      \
-    hipLaunchKernelGGL((k_select<KM, ND>), grid, blk, 0, s, perm0, perm1, pos0, pos1, ds0, ds1, N, bew, beh, ndims, \
-                       K, window, min_dist2, b0, bw, bh, kpos, kdist)
-    if (K <= 16) {
-        if (ndims == 6) PT_SELECT(16, 6);
-        else PT_SELECT(16, 0);
+#define PT_SELECT(KM, ND, DESC)                                                                                 \
+    hipLaunchKernelGGL((k_select<KM, ND, DESC>), grid, blk, 0, s, perm0, perm1, pos0, pos1, ds0, ds1, Na, bew, beh, \
+                       ndims, K, window, min_dist2, b0, b_descI, bw, bh, allowed, aidx, kpos, kdist)
+    if (b_descI) {
+        if (K <= 16) PT_SELECT(16, 0, true);
+        else PT_SELECT(32, 0, true);
+    } else if (K <= 16) {
+        if (ndims == 6) PT_SELECT(16, 6, false);
+        else PT_SELECT(16, 0, false);
     } else {
-        if (ndims == 6) PT_SELECT(32, 6);
-        else PT_SELECT(32, 0);
+        if (ndims == 6) PT_SELECT(32, 6, false);
+        else PT_SELECT(32, 0, false);
     }
 #undef PT_SELECT
     PT_LAUNCH_CHECK();

@@ -1,28 +1,41 @@
-// Query lookup on gfx950 (replaces lookup_templated<0,0>, patchtable.h:3468-4217).
+// Query lookup on gfx950 (replaces lookup_templated<is_descriptor, use_allowed_patches>,
+// patchtable.h:3468-4217).
 //
-// The reference visits query patches in raster order (then reverse raster order) and lets each
-// patch push its best offset to its 3x3 neighbours before they are visited, so a good match
-// propagates along a whole row within one pass. Here every query patch is processed in
-// parallel and propagation runs in rounds over double-buffered fields:
-//   candidates: the table cell's patch (patchtable.h:3664-3704), then the k-coherence list of
-//     the best with the triangle early-out (patchtable.h:3748-3851), then the spatial window
-//     around the best (patchtable.h:3853-3885);
-//   rounds: jump flooding -- the 8 neighbours at distance J (J = 8, 4, 2, 1, then 1 ...) each
-//     propose their offset; then the spatial window around the best again.
-// Candidates replace the best only when strictly closer and every distance is the exact SSD
-// with the reference's lane order, so NNF_DIST is always the exact SSD of the stored match.
+// The reference visits the query patches of the lattice {x, y multiples of query_step} in raster
+// order (then reverse raster order) and lets each visited patch push its best offset to its
+// (2*query_step+1)^2 neighbourhood before those are visited, so a good match propagates along a
+// whole row within one pass. Here every lattice patch is processed in parallel and propagation
+// runs in rounds over double-buffered fields:
+//   init: nothing, or the previous field (clamped; distance recomputed and scaled by the temporal
+//     weight, or the previous distance scaled, patchtable.h:3539-3585);
+//   candidates: the cell of every table (patchtable.h:3664-3737), the k-coherence list of the
+//     best with the triangle early-out (patchtable.h:3748-3851), then the spatial window around
+//     the best (patchtable.h:3853-3885, allowed positions only) or one random neighbour (do_rs
+//     without spatial, patchtable.h:3886-3898);
+//   rounds: jump flooding -- the 8 lattice neighbours at distance J (J = 8, 4, 2, 1, then 1 ...)
+//     each propose their offset; the k-coherence walk again when kcoherence_iter asks for it after
+//     the first pass or when there are no table candidates; then the spatial window or a random
+//     neighbour;
+//   gather (query_step > 1): every patch takes the best of the offsets of the lattice patches
+//     within query_step of it, as the reference's propagation scatter does;
+//   output: NNF_X, NNF_Y, NNF_DIST as doubles, with the temporal rescaling of patchtable.h:4178-4193.
+// Candidates replace the best only when strictly closer and every distance is exact (the RGB SSD
+// with the reference's lane order, or patch_dist_approx of the descriptors in descriptor mode),
+// so NNF_DIST is always the distance of the stored match.
 //
-// Work layout: a workgroup of 256 lanes owns a 64 x 4 tile of query patches and stages the
-// tile's 71 x 11 query pixels in LDS (one float4 per pixel), so each SSD reads only the
-// database patch from global memory. The field is packed as XY_TO_INT position + distance, plus
-// a flag recording that the spatial window around the stored best has already been searched
-// without improvement; a later round whose best is unchanged skips that window (it would find
-// nothing strictly closer), which leaves every result unchanged. A second flag records whether
-// the best changed in the round that produced the field (used to skip repeated candidates).
+// Work layout: a workgroup of 256 lanes owns a 64 x 4 tile of lattice patches. With
+// query_step 1 in image mode the tile's 71 x 11 query pixels are staged in LDS (one float4 per
+// pixel); in descriptor mode each lane's query descriptor is staged; with query_step > 1 the
+// query patches are read from global memory. The field is packed as XY_TO_INT position +
+// distance + flags: the spatial window around the stored best was already searched without
+// improvement (a later round whose best is unchanged skips it), the best changed in the round
+// that produced the field (used to skip repeated candidates), and the k-coherence list of the
+// best was already walked. None of these skips changes a result.
 #include "pt_device.h"
 #include "pt_kernels.h"
 #include "pt_tile.h"
 
+#include <cmath>
 #include <vector>
 
 namespace {
@@ -32,19 +45,69 @@ using namespace pt_tile;
 __constant__ int c_dx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 __constant__ int c_dy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 
+enum { M_TILE = 0, M_GLOBAL = 1, M_DESC = 2 };   // where a lane's query patch is read from
+
+constexpr uint8_t F_DONE = 1, F_CHG = 2, F_KD = 4;
+
+// Floats of LDS that hold the block's query patches.
+template <int M>
+struct QBuf {
+    static constexpr int F4 = M == M_TILE ? AX * AY : (M == M_DESC ? THREADS * PT_LOOKUP_CMAX / 4 : 1);
+};
+
+__device__ __forceinline__ void lane_xy(const LookupArgs &L, int lane, int &x, int &y) {
+    x = (blockIdx.x * TILE_X + lane % TILE_X) * L.qs;
+    y = (blockIdx.y * TILE_Y + lane / TILE_X) * L.qs;
+}
+
+// Stages the block's query patches (every lane of the block must call this).
+template <int M>
+__device__ __forceinline__ void stage_query(const LookupArgs &L, float4 *qbuf, bool valid, int x, int y) {
+    if constexpr (M == M_TILE) {
+        stage_tile(L.a, L.ah, L.aw, qbuf);
+    } else if constexpr (M == M_DESC) {
+        float *qd = (float *)qbuf + threadIdx.x * L.C;
+        const float *src = L.a_descI + ((size_t)(y + 7) * (L.aw + 7) + x + 7) * L.C;
+        for (int c = 0; c < L.C; c++) qd[c] = valid ? src[c] : 0.f;
+        __syncthreads();
+    }
+}
+
+// Distance of lane `lane`'s query patch to database patch (xs, ys); with a finite bound the SSD
+// may stop early at a partial sum >= bound (then the result is >= bound, never a smaller value).
+template <int M>
+__device__ __forceinline__ float qdist(const LookupArgs &L, const float4 *qbuf, int lane, int xs, int ys, float bound) {
+    if constexpr (M == M_TILE) {
+        const float4 *aq = qbuf + (lane / TILE_X) * AX + (lane % TILE_X);
+        return ssd_staged<true>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, L.bw * 3, bound);
+    } else if constexpr (M == M_GLOBAL) {
+        int x, y;
+        lane_xy(L, lane, x, y);
+        return pt_patch_ssd_bounded(L.a + ((size_t)y * L.aw + x) * 3, L.aw * 3, L.b + ((size_t)ys * L.bw + xs) * 3,
+                                    L.bw * 3, bound);
+    } else {
+        const float *qd = (const float *)qbuf + lane * L.C;
+        return pt_desc_dist(qd, L.b_descI + ((size_t)(ys + 7) * (L.bw + 7) + xs + 7) * L.C, L.C);
+    }
+}
+
+__device__ __forceinline__ bool allowed_at(const LookupArgs &L, int xs, int ys) {
+    return !L.allowed || L.allowed[(size_t)ys * (L.bw - 7) + xs] == L.allowed_index;
+}
+
 // ---- Block work queue -------------------------------------------------------------------------
 // A lane owns one query patch. Instead of each lane looping over its own candidates (a wave then
-// waits for its slowest lane), the lanes of a block push their candidate SSDs to a queue in LDS,
-// all lanes of the block evaluate the queue, and each lane reduces its own results in candidate
-// order. Every task is bounded by its patch's best distance at the start of the phase, so a
-// result is exact whenever it is below that bound; reducing in order with strict "<" then picks
-// the same first minimum as evaluating the candidates one after another against the running best.
+// waits for its slowest lane), the lanes of a block push their candidate distances to a queue in
+// LDS, all lanes of the block evaluate the queue, and each lane reduces its own results in
+// candidate order. Every task is bounded by its patch's best distance at the start of the phase,
+// so a result is exact whenever it is below that bound; reducing in order with strict "<" then
+// picks the same first minimum as evaluating the candidates one after another.
 constexpr int SLOTS = 8;   // tasks per lane and phase
 
 struct BlockQueue {
     unsigned short id[THREADS * SLOTS];   // queued task ids: lane * SLOTS + slot
     int cxy[THREADS * SLOTS];             // candidate XY_TO_INT, by task id
-    float res[THREADS * SLOTS];           // SSD, by task id
+    float res[THREADS * SLOTS];           // distance, by task id
     float bound[THREADS];                 // per lane
     int n;
 };
@@ -68,17 +131,15 @@ __device__ __forceinline__ void q_push(BlockQueue &Q, bool pred, int slot, int c
 
 // Evaluates every queued task; on return the results are visible to all lanes and the queue is
 // empty again.
-__device__ __forceinline__ void q_run(BlockQueue &Q, const float4 *tile, const LookupArgs &L) {
+template <int M>
+__device__ __forceinline__ void q_run(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L) {
     __syncthreads();
     const int n = Q.n;
-    const int bpitch = L.bw * 3;
     for (int k = threadIdx.x; k < n; k += THREADS) {
         const int id = Q.id[k];
         const int lane = id / SLOTS;
         const int c = Q.cxy[id];
-        const float4 *aq = tile + (lane / TILE_X) * AX + (lane % TILE_X);
-        Q.res[id] = ssd_staged<true>(aq, L.b + ((size_t)PT_INT_TO_Y(c) * L.bw + PT_INT_TO_X(c)) * 3, bpitch,
-                                     Q.bound[lane]);
+        Q.res[id] = qdist<M>(L, qbuf, lane, PT_INT_TO_X(c), PT_INT_TO_Y(c), Q.bound[lane]);
     }
     __syncthreads();
     if (threadIdx.x == 0) Q.n = 0;
@@ -105,14 +166,14 @@ __device__ __forceinline__ Window spatial_window(const LookupArgs &L, int xb, in
 }
 
 // Sequential form, for spatial radius > 1 (more than SLOTS positions). Returns true if improved.
-__device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const float4 *aq, int &xb, int &yb, float &db) {
+template <int M>
+__device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const float4 *qbuf, int &xb, int &yb, float &db) {
     const Window w = spatial_window(L, xb, yb);
-    const int bpitch = L.bw * 3;
     bool changed = false;
     for (int ys = w.ymin; ys < w.ymax; ys++)
         for (int xs = w.xmin; xs < w.xmax; xs++) {
-            if (xs == w.xb0 && ys == w.yb0) continue;
-            float d = ssd_staged<true>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, bpitch, db);
+            if ((xs == w.xb0 && ys == w.yb0) || !allowed_at(L, xs, ys)) continue;
+            float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, db);
             if (d < db) {
                 db = d;
                 xb = xs;
@@ -125,162 +186,237 @@ __device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const fl
 
 // Spatial search through the block queue (radius 1: at most 8 positions). `want` is false for
 // lanes that skip it; every lane of the block must call this. Returns true if improved.
-__device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *tile, const LookupArgs &L, bool want,
+template <int M>
+__device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L, bool want,
                                                  int &xb, int &yb, float &db) {
     Window w = spatial_window(L, xb, yb);
     if (!want) w.ymax = w.ymin;   // empty
     if (want) Q.bound[threadIdx.x] = db;
+    unsigned pushed = 0;
     int slot = 0;
     for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++) {
             if (dx == 0 && dy == 0) continue;
             // slot order = row-by-row order of the window positions other than the centre
             const int ys = w.yb0 + dy, xs = w.xb0 + dx;
-            const bool in = ys >= w.ymin && ys < w.ymax && xs >= w.xmin && xs < w.xmax;
+            const bool in = ys >= w.ymin && ys < w.ymax && xs >= w.xmin && xs < w.xmax && allowed_at(L, xs, ys);
+            if (in) pushed |= 1u << slot;
             q_push(Q, in, slot, PT_XY_TO_INT(xs, ys));
             slot++;
         }
-    q_run(Q, tile, L);
+    q_run<M>(Q, qbuf, L);
     bool changed = false;
-    if (want) {
-        int s = 0;
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                if (dx == 0 && dy == 0) continue;
-                const int ys = w.yb0 + dy, xs = w.xb0 + dx;
-                if (ys >= w.ymin && ys < w.ymax && xs >= w.xmin && xs < w.xmax) {
-                    const float d = Q.res[threadIdx.x * SLOTS + s];
-                    if (d < db) {
-                        db = d;
-                        xb = xs;
-                        yb = ys;
-                        changed = true;
-                    }
-                }
-                s++;
-            }
+    for (int s = 0; s < 8; s++) {
+        if (!((pushed >> s) & 1u)) continue;
+        const float d = Q.res[threadIdx.x * SLOTS + s];
+        if (d < db) {
+            db = d;
+            const int c = Q.cxy[threadIdx.x * SLOTS + s];
+            xb = PT_INT_TO_X(c);
+            yb = PT_INT_TO_Y(c);
+            changed = true;
+        }
     }
     return changed;
 }
 
-__global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField out) {
-    __shared__ float4 tile[AX * AY];
-    __shared__ BlockQueue Q;
-    if (threadIdx.x == 0) Q.n = 0;
-    stage_tile(L.a, L.ah, L.aw, tile);
-    const int ann_w = L.aw - 7, ann_h = L.ah - 7;
-    const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
-    const int x = blockIdx.x * TILE_X + lx, y = blockIdx.y * TILE_Y + ly;
-    const bool valid = x < ann_w && y < ann_h;
-    const float4 *aq = tile + ly * AX + lx;
-    const int bpitch = L.bw * 3, bew = L.bw - 7;
-    int xb = 0, yb = 0;
-    float db = 0.f;
-    if (valid) {
-        // table cell of the query descriptor at (y+7, x+7)
-        const size_t plane = (size_t)(L.ah + 7) * (L.aw + 7);
-        const size_t off = (size_t)(y + 7) * (L.aw + 7) + x + 7;
-        int64_t cell = 0;
-        for (int d = 0; d < L.q.G; d++)
-            cell += (int64_t)pt_get_bin(L.a_desc[d * plane + off], L.q.min_val[d], L.q.range[d],
-                                        L.q.to_bin + (size_t)d * PT_NMAP) *
-                    L.q.stride[d];
-        const int v = L.table[cell];
-        xb = PT_INT_TO_X(v) - 7;
-        yb = PT_INT_TO_Y(v) - 7;
-        db = ssd_staged<false>(aq, L.b + ((size_t)yb * L.bw + xb) * 3, bpitch, 0.f);
-    }
-    if (L.K > 0 && L.kcoh_iter0) {
-        // The k-coherence list of the table candidate is walked in order and abandoned at the
-        // first entry with kd >= triangle_factor * dbest (patchtable.h:3779-3784); kd is sorted,
-        // so per chunk of SLOTS entries the queued superset is those with kd below the bound at
-        // the chunk's start, and the in-order reduce replays the early-out exactly.
-        const size_t lst = ((size_t)yb * bew + xb) * L.K;
-        bool broken = !valid;
-        for (int k0 = 0; k0 < L.K; k0 += SLOTS) {
-            const double tb = L.triangle_factor * (double)db;
-            if (!broken) Q.bound[threadIdx.x] = db;
-            for (int s = 0; s < SLOTS; s++) {
-                const int k = k0 + s;
-                bool pred = !broken && k < L.K && (double)L.kdist[lst + k] < tb;
-                int pos = pred ? L.kpos[lst + k] : 0;
-                q_push(Q, pred, s, pos);
+// Walk of the k-coherence list of the best at the walk's start, abandoned at the first entry with
+// kd >= triangle_factor * dbest (patchtable.h:3779-3784). kd is sorted, so per chunk of SLOTS
+// entries the queued superset is those with kd below the bound at the chunk's start, and the
+// in-order reduce replays the early-out exactly. Every lane of the block must call this.
+template <int M>
+__device__ __forceinline__ bool kcoh_walk_q(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L, bool want, int &xb,
+                                            int &yb, float &db) {
+    const int bew = L.bw - 7;
+    const size_t lst = ((size_t)yb * bew + xb) * L.K;
+    bool broken = !want, changed = false;
+    for (int k0 = 0; k0 < L.K; k0 += SLOTS) {
+        const double tb = L.triangle_factor * (double)db;
+        if (!broken) Q.bound[threadIdx.x] = db;
+        for (int s = 0; s < SLOTS; s++) {
+            const int k = k0 + s;
+            bool pred = !broken && k < L.K && (double)L.kdist[lst + k] < tb;
+            int pos = pred ? L.kpos[lst + k] : 0;
+            q_push(Q, pred, s, pos);
+        }
+        q_run<M>(Q, qbuf, L);
+        for (int s = 0; s < SLOTS && !broken; s++) {
+            const int k = k0 + s;
+            if (k >= L.K) break;
+            if ((double)L.kdist[lst + k] >= L.triangle_factor * (double)db) {
+                broken = true;
+                break;
             }
-            q_run(Q, tile, L);
-            for (int s = 0; s < SLOTS && !broken; s++) {
-                const int k = k0 + s;
-                if (k >= L.K) break;
-                if ((double)L.kdist[lst + k] >= L.triangle_factor * (double)db) {
-                    broken = true;
-                    break;
-                }
-                const float d = Q.res[threadIdx.x * SLOTS + s];
-                if (d < db) {
-                    db = d;
-                    const int c = Q.cxy[threadIdx.x * SLOTS + s];
-                    xb = PT_INT_TO_X(c);
-                    yb = PT_INT_TO_Y(c);
-                }
+            const float d = Q.res[threadIdx.x * SLOTS + s];
+            if (d < db) {
+                db = d;
+                const int c = Q.cxy[threadIdx.x * SLOTS + s];
+                xb = PT_INT_TO_X(c);
+                yb = PT_INT_TO_Y(c);
+                changed = true;
             }
         }
     }
+    return changed;
+}
+
+// Random neighbour of the random search (do_rs without spatial): a hash of the patch index and
+// the round replaces the reference's rand() & 7 (patchtable.h:3887).
+__device__ __forceinline__ int rs_pick(uint32_t i, uint32_t round) {
+    uint32_t h = i * 0x9E3779B1u + round * 0x85EBCA77u + 0x165667B1u;
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return (int)(h & 7u);
+}
+
+template <int M>
+__device__ __forceinline__ bool random_search(const LookupArgs &L, const float4 *qbuf, long pi, int round, int &xb,
+                                              int &yb, float &db) {
+    const int r = rs_pick((uint32_t)pi, (uint32_t)round);
+    const int xs = xb + c_dx[r], ys = yb + c_dy[r];
+    if (xs < 0 || ys < 0 || xs >= L.bw - 7 || ys >= L.bh - 7 || !allowed_at(L, xs, ys)) return false;
+    const float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, db);
+    if (d < db) {
+        db = d;
+        xb = xs;
+        yb = ys;
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ int table_cell_value(const LookupArgs &L, const DevQuant &q, const int32_t *table, int x,
+                                                int y) {
+    const size_t plane = (size_t)(L.ah + 7) * (L.aw + 7);
+    const size_t off = (size_t)(y + 7) * (L.aw + 7) + x + 7;
+    int64_t cell = 0;
+    for (int d = 0; d < q.G; d++)
+        cell += (int64_t)pt_get_bin(L.a_desc[d * plane + off], q.min_val[d], q.range[d], q.to_bin + (size_t)d * PT_NMAP) *
+                q.stride[d];
+    return table[cell];
+}
+
+template <int M>
+__global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField out) {
+    __shared__ float4 qbuf[QBuf<M>::F4];
+    __shared__ BlockQueue Q;
+    if (threadIdx.x == 0) Q.n = 0;
+    const int vx = blockIdx.x * TILE_X + threadIdx.x % TILE_X, vy = blockIdx.y * TILE_Y + threadIdx.x / TILE_X;
+    const bool valid = vx < L.vw && vy < L.vh;
+    const int x = vx * L.qs, y = vy * L.qs;
+    stage_query<M>(L, qbuf, valid, x, y);
+    const long pi = (long)y * L.ann_w + x;
+    bool have = false;
+    int xb = 0, yb = 0;
+    float db = 0.f;
+    if (valid && L.init_xy) {
+        const int v = L.init_xy[pi];
+        xb = PT_INT_TO_X(v);
+        yb = PT_INT_TO_Y(v);
+        db = L.init_d[pi];
+        have = true;
+    }
+    if (valid && L.do_table_lookup) {
+        const int v = table_cell_value(L, L.q, L.table, x, y);
+        const int xs = PT_INT_TO_X(v) - 7, ys = PT_INT_TO_Y(v) - 7;
+        if (!have || xs != xb || ys != yb) {
+            const float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, have ? db : INFINITY);
+            if (!have || d < db) {
+                db = d;
+                xb = xs;
+                yb = ys;
+            }
+            have = true;
+        }
+        int xp = xs, yp = ys;
+        for (int t = 0; t < L.nxt; t++) {   // further tables (patchtable.h:3711-3737)
+            const int v1 = table_cell_value(L, L.xq[t], L.xtable[t], x, y);
+            const int x1 = PT_INT_TO_X(v1) - 7, y1 = PT_INT_TO_Y(v1) - 7;
+            if ((x1 != xb || y1 != yb) && (x1 != xp || y1 != yp)) {
+                const float d = qdist<M>(L, qbuf, threadIdx.x, x1, y1, db);
+                if (d < db) {
+                    db = d;
+                    xb = x1;
+                    yb = y1;
+                }
+            }
+            xp = x1;
+            yp = y1;
+        }
+    }
+    bool kd = false;
+    if (L.K > 0 && L.kcoh_iter0) kd = !kcoh_walk_q<M>(Q, qbuf, L, valid, xb, yb, db);
     bool done = false;
-    if (L.spatial == 1) done = !spatial_search_q(Q, tile, L, valid, xb, yb, db);
-    else if (L.spatial > 1 && valid) done = !spatial_search_seq(L, aq, xb, yb, db);
+    if (L.spatial == 1) {
+        const bool changed = spatial_search_q<M>(Q, qbuf, L, valid, xb, yb, db);
+        done = !changed;
+        if (changed) kd = false;
+    } else if (L.spatial > 1 && valid) {
+        const bool changed = spatial_search_seq<M>(L, qbuf, xb, yb, db);
+        done = !changed;
+        if (changed) kd = false;
+    } else if (L.rs && valid) {
+        if (random_search<M>(L, qbuf, pi, 0, xb, yb, db)) kd = false;
+    }
     if (!valid) return;
-    const long i = (long)y * ann_w + x;
-    out.xy[i] = PT_XY_TO_INT(xb, yb);
-    out.d[i] = db;
-    out.done[i] = done;
-    out.chg[i] = 1;
+    const long v = (long)vy * L.vw + vx;
+    out.xy[v] = PT_XY_TO_INT(xb, yb);
+    out.d[v] = db;
+    out.fl[v] = (done ? F_DONE : 0) | F_CHG | (kd ? F_KD : 0);
 }
 
 // same_jump: the previous round used the same jump. A candidate from neighbour n then repeats
 // that round's when neither this patch nor n changed in it; it was rejected against the same
 // best, so it is skipped (results unchanged).
-__global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int same_jump, NnfField cur, NnfField nxt) {
-    __shared__ float4 tile[AX * AY];
+template <int M>
+__global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int same_jump, int round, NnfField cur,
+                                                      NnfField nxt) {
+    __shared__ float4 qbuf[QBuf<M>::F4];
     __shared__ BlockQueue Q;
     if (threadIdx.x == 0) Q.n = 0;
-    stage_tile(L.a, L.ah, L.aw, tile);
-    const int ann_w = L.aw - 7, ann_h = L.ah - 7;
-    const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
-    const int x = blockIdx.x * TILE_X + lx, y = blockIdx.y * TILE_Y + ly;
-    const bool valid = x < ann_w && y < ann_h;
-    const float4 *aq = tile + ly * AX + lx;
+    const int vx = blockIdx.x * TILE_X + threadIdx.x % TILE_X, vy = blockIdx.y * TILE_Y + threadIdx.x / TILE_X;
+    const bool valid = vx < L.vw && vy < L.vh;
+    const int x = vx * L.qs, y = vy * L.qs;
+    stage_query<M>(L, qbuf, valid, x, y);
     const int bew = L.bw - 7, beh = L.bh - 7;
-    const long i = (long)y * ann_w + x;
+    const long v = (long)vy * L.vw + vx;
+    const int js = J * L.qs;   // jump in patch units
     int xb = 0, yb = 0;
     float db = 0.f;
-    bool stable = false, was_done = false;
+    bool stable = false, was_done = false, kd = false;
     if (valid) {
-        const int v0 = cur.xy[i];
+        const int v0 = cur.xy[v];
         xb = PT_INT_TO_X(v0);
         yb = PT_INT_TO_Y(v0);
-        db = cur.d[i];
-        stable = same_jump && !cur.chg[i];
-        was_done = cur.done[i];
+        db = cur.d[v];
+        const uint8_t fl = cur.fl[v];
+        stable = same_jump && !(fl & F_CHG);
+        was_done = fl & F_DONE;
+        kd = fl & F_KD;
         Q.bound[threadIdx.x] = db;
     }
     unsigned pushed = 0;
     for (int t = 0; t < 8; t++) {
-        const int qx = x + J * c_dx[t], qy = y + J * c_dy[t];
-        bool pred = valid && qx >= 0 && qy >= 0 && qx < ann_w && qy < ann_h;
+        const int qx = vx + J * c_dx[t], qy = vy + J * c_dy[t];
+        bool pred = valid && qx >= 0 && qy >= 0 && qx < L.vw && qy < L.vh;
         int xs = 0, ys = 0;
         if (pred) {
-            const long qi = (long)qy * ann_w + qx;
-            if (stable && !cur.chg[qi]) pred = false;
+            const long qv = (long)qy * L.vw + qx;
+            if (stable && !(cur.fl[qv] & F_CHG)) pred = false;
             else {
-                const int vq = cur.xy[qi];
-                xs = PT_INT_TO_X(vq) - J * c_dx[t];
-                ys = PT_INT_TO_Y(vq) - J * c_dy[t];
-                pred = xs >= 0 && ys >= 0 && xs < bew && ys < beh && !(xs == xb && ys == yb);
+                const int vq = cur.xy[qv];
+                xs = PT_INT_TO_X(vq) - js * c_dx[t];
+                ys = PT_INT_TO_Y(vq) - js * c_dy[t];
+                pred = xs >= 0 && ys >= 0 && xs < bew && ys < beh && !(xs == xb && ys == yb) && allowed_at(L, xs, ys);
             }
         }
         if (pred) pushed |= 1u << t;
         q_push(Q, pred, t, PT_XY_TO_INT(xs, ys));
     }
-    q_run(Q, tile, L);
+    q_run<M>(Q, qbuf, L);
     bool moved = false;
     for (int t = 0; t < 8; t++) {
         if (!((pushed >> t) & 1u)) continue;
@@ -293,37 +429,174 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
             moved = true;
         }
     }
-    bool done = was_done && !moved;
+    kd = kd && !moved;
+    bool kchg = false;
+    if (L.kcoh_rounds) {
+        const bool want = valid && !kd;
+        kchg = kcoh_walk_q<M>(Q, qbuf, L, want, xb, yb, db);
+        if (want) kd = !kchg;
+    }
+    bool done = was_done && !moved && !kchg;
     bool schg = false;
     if (L.spatial == 1) {
-        schg = spatial_search_q(Q, tile, L, valid && !done, xb, yb, db);
+        schg = spatial_search_q<M>(Q, qbuf, L, valid && !done, xb, yb, db);
         if (valid && !done) done = !schg;
     } else if (L.spatial > 1 && valid && !done) {
-        schg = spatial_search_seq(L, aq, xb, yb, db);
+        schg = spatial_search_seq<M>(L, qbuf, xb, yb, db);
         done = !schg;
+    } else if (L.rs && valid) {
+        schg = random_search<M>(L, qbuf, (long)y * L.ann_w + x, round, xb, yb, db);
     }
+    if (schg) kd = false;
     if (!valid) return;
-    nxt.xy[i] = PT_XY_TO_INT(xb, yb);
-    nxt.d[i] = db;
-    nxt.done[i] = done;
-    nxt.chg[i] = moved || schg;
+    nxt.xy[v] = PT_XY_TO_INT(xb, yb);
+    nxt.d[v] = db;
+    nxt.fl[v] = (done ? F_DONE : 0) | ((moved || kchg || schg) ? F_CHG : 0) | (kd ? F_KD : 0);
 }
 
-__global__ void k_write_ann(NnfField f, long n, double *__restrict__ ann) {
+// query_step > 1: every patch takes the best offset of the lattice patches within query_step
+// (clamped into b, allowed positions only), starting from its initial match; a patch with no
+// allowed proposal takes its lattice owner's match. Launched over all patches (qs = 1 layout).
+template <int M>
+__global__ __launch_bounds__(THREADS) void k_gather(LookupArgs L, NnfField lat, int32_t *__restrict__ oxy,
+                                                    float *__restrict__ od) {
+    __shared__ float4 qbuf[QBuf<M>::F4];
+    const int x = blockIdx.x * TILE_X + threadIdx.x % TILE_X, y = blockIdx.y * TILE_Y + threadIdx.x / TILE_X;
+    const bool valid = x < L.ann_w && y < L.ann_h;
+    stage_query<M>(L, qbuf, valid, x, y);
+    if (!valid) return;
+    const int bew = L.bw - 7, beh = L.bh - 7, qs = L.gqs;
+    const long pi = (long)y * L.ann_w + x;
+    bool have = false;
+    int xb = 0, yb = 0;
+    float db = 0.f;
+    if (L.init_xy) {
+        const int v = L.init_xy[pi];
+        xb = PT_INT_TO_X(v);
+        yb = PT_INT_TO_Y(v);
+        db = L.init_d[pi];
+        have = true;
+    }
+    const int vy0 = y / qs - (y % qs == 0 ? 1 : 0), vy1 = y / qs + 1;
+    const int vx0 = x / qs - (x % qs == 0 ? 1 : 0), vx1 = x / qs + 1;
+    for (int vy = vy0 < 0 ? 0 : vy0; vy <= vy1 && vy < L.vh; vy++)
+        for (int vx = vx0 < 0 ? 0 : vx0; vx <= vx1 && vx < L.vw; vx++) {
+            const int c = lat.xy[(long)vy * L.vw + vx];
+            int xs = PT_INT_TO_X(c) + (x - vx * qs), ys = PT_INT_TO_Y(c) + (y - vy * qs);
+            xs = xs < 0 ? 0 : (xs >= bew ? bew - 1 : xs);
+            ys = ys < 0 ? 0 : (ys >= beh ? beh - 1 : ys);
+            if (!allowed_at(L, xs, ys)) continue;
+            if (have && xs == xb && ys == yb) continue;
+            const float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, have ? db : INFINITY);
+            if (!have || d < db) {
+                db = d;
+                xb = xs;
+                yb = ys;
+                have = true;
+            }
+        }
+    if (!have) {
+        const int c = lat.xy[(long)(y / qs) * L.vw + x / qs];
+        xb = PT_INT_TO_X(c);
+        yb = PT_INT_TO_Y(c);
+        db = qdist<M>(L, qbuf, threadIdx.x, xb, yb, INFINITY);
+    }
+    oxy[pi] = PT_XY_TO_INT(xb, yb);
+    od[pi] = db;
+}
+
+// Initial field from a previous one (TABLE_COPY_PREV_NNF and patchtable.h:3539-3585).
+template <int M>
+__global__ __launch_bounds__(256) void k_init_prev(LookupArgs L, const double *__restrict__ prev, const float *__restrict__ sv, int recalc,
+                            double temporal_scale, int sanitize, int32_t *__restrict__ ixy, float *__restrict__ id,
+                            int *__restrict__ bad) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)L.ann_w * L.ann_h) return;
+    const int x = (int)(i % L.ann_w), y = (int)(i / L.ann_w);
+    const int bew = L.bw - 7, beh = L.bh - 7;
+    int xs = (int)prev[i * 3], ys = (int)prev[i * 3 + 1];
+    if (!sanitize && (xs < 0 || ys < 0 || xs >= bew || ys >= beh)) atomicOr(bad, 1);
+    xs = xs < 0 ? 0 : (xs >= bew ? bew - 1 : xs);
+    ys = ys < 0 ? 0 : (ys >= beh ? beh - 1 : ys);
+    ixy[i] = PT_XY_TO_INT(xs, ys);
+    if (recalc) {
+        float d;
+        if constexpr (M == M_DESC)
+            d = pt_desc_dist(L.a_descI + ((size_t)(y + 7) * (L.aw + 7) + x + 7) * L.C,
+                          L.b_descI + ((size_t)(ys + 7) * (L.bw + 7) + xs + 7) * L.C, L.C);
+        else
+            d = pt_patch_ssd(L.a + ((size_t)y * L.aw + x) * 3, L.aw * 3, L.b + ((size_t)ys * L.bw + xs) * 3, L.bw * 3);
+        const double tsc = sv ? (double)sv[i] : temporal_scale;
+        id[i] = (float)(d * tsc);
+    } else {
+        id[i] = (float)(prev[i * 3 + 2] * temporal_scale);
+    }
+}
+
+// NNF_X, NNF_Y, NNF_DIST as doubles; a patch that kept its initial match and distance gets the
+// distance divided by the temporal scale (patchtable.h:4178-4193).
+__global__ void k_write_ann(const int32_t *__restrict__ xy, const float *__restrict__ d, long n, double *__restrict__ ann,
+                            const double *__restrict__ prev, const float *__restrict__ init_d, double temporal_scale,
+                            int fix) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int v = f.xy[i];
-    ann[i * 3 + 0] = (double)PT_INT_TO_X(v);
-    ann[i * 3 + 1] = (double)PT_INT_TO_Y(v);
-    ann[i * 3 + 2] = (double)f.d[i];
+    const int v = xy[i];
+    const int x = PT_INT_TO_X(v), y = PT_INT_TO_Y(v);
+    double dd = (double)d[i];
+    if (fix && (double)x == prev[i * 3] && (double)y == prev[i * 3 + 1] && d[i] == init_d[i]) dd = dd / temporal_scale;
+    ann[i * 3 + 0] = (double)x;
+    ann[i * 3 + 1] = (double)y;
+    ann[i * 3 + 2] = dd;
 }
 
-__global__ void k_ssd_pairs(const float *__restrict__ a, int aw, const float *__restrict__ b, int bw,
-                            const int32_t *__restrict__ ax, const int32_t *__restrict__ ay, const int32_t *__restrict__ bx,
-                            const int32_t *__restrict__ by, int n, float *__restrict__ out) {
+// Descriptor-mode input: an h_in x w_in x C descriptor image placed at rows/columns 7.. of the
+// padded (Hp x Wp) descriptor (get_padded_descriptor, patchtable.h:2498-2507), or a padded input
+// copied as is; written both as planes (quantiser, table cells) and interleaved (distances).
+__global__ void k_desc_pad(const float *__restrict__ in, int h_in, int w_in, int C, int padded, int Hp, int Wp,
+                           float *__restrict__ planes, float *__restrict__ inter) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)Hp * Wp) return;
+    const int y = (int)(i / Wp), x = (int)(i % Wp);
+    const int sy = padded ? y : y - 7, sx = padded ? x : x - 7;
+    const bool in_img = sy >= 0 && sx >= 0 && sy < h_in && sx < w_in;
+    const size_t plane = (size_t)Hp * Wp;
+    for (int c = 0; c < C; c++) {
+        const float v = in_img ? in[((size_t)sy * w_in + sx) * C + c] : 0.f;
+        planes[c * plane + i] = v;
+        inter[i * C + c] = v;
+    }
+}
+
+__global__ void k_ssd_pairs(const float *a, int aw, const float *b, int bw, const int32_t *__restrict__ ax,
+                            const int32_t *__restrict__ ay, const int32_t *__restrict__ bx, const int32_t *__restrict__ by,
+                            int n, float *__restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     out[i] = pt_patch_ssd(a + ((size_t)ay[i] * aw + ax[i]) * 3, aw * 3, b + ((size_t)by[i] * bw + bx[i]) * 3, bw * 3);
+}
+
+__global__ void k_desc_dist_pairs(const float *a, const float *b, int C, int n, float *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = pt_desc_dist(a + (size_t)i * C, b + (size_t)i * C, C);
+}
+
+template <int M>
+int launch_lookup(const LookupArgs &L, NnfField f0, NnfField f1, hipStream_t s, hipEvent_t ev_after_candidates,
+                  int &launches) {
+    dim3 grid((L.vw + TILE_X - 1) / TILE_X, (L.vh + TILE_Y - 1) / TILE_Y);
+    hipLaunchKernelGGL(k_candidates<M>, grid, dim3(THREADS), 0, s, L, f0);
+    if (ev_after_candidates) PT_HIP_TRY(hipEventRecord(ev_after_candidates, s));
+    std::vector<int> J;
+    pt_jump_schedule(L.rounds, J);
+    for (size_t r = 0; r < J.size(); r++) {
+        int same = r > 0 && J[r - 1] == J[r];
+        hipLaunchKernelGGL(k_propagate<M>, grid, dim3(THREADS), 0, s, L, J[r], same, (int)r + 1, f0, f1);
+        std::swap(f0, f1);
+    }
+    launches = (int)J.size();
+    PT_LAUNCH_CHECK();
+    return J.size() % 2 ? 1 : 0;   // which field holds the result
 }
 
 }  // namespace
@@ -336,22 +609,56 @@ void pt_jump_schedule(int rounds, std::vector<int> &J) {
     while ((int)J.size() < rounds) J.push_back(1);
 }
 
-int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, double *ann_out, hipStream_t s,
-                     hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds) {
-    const int ann_w = L.aw - 7, ann_h = L.ah - 7;
-    const long n = (long)ann_w * ann_h;
-    dim3 grid((ann_w + TILE_X - 1) / TILE_X, (ann_h + TILE_Y - 1) / TILE_Y);
-    hipLaunchKernelGGL(k_candidates, grid, dim3(THREADS), 0, s, L, f0);
-    if (ev_after_candidates) PT_HIP_TRY(hipEventRecord(ev_after_candidates, s));
-    std::vector<int> J;
-    pt_jump_schedule(L.rounds, J);
-    for (size_t r = 0; r < J.size(); r++) {
-        int same = r > 0 && J[r - 1] == J[r];
-        hipLaunchKernelGGL(k_propagate, grid, dim3(THREADS), 0, s, L, J[r], same, f0, f1);
-        std::swap(f0, f1);
+int pt_lookup_init_prev(const LookupArgs &L, const double *prev, const float *sv, int recalc, double temporal_scale,
+                        int sanitize, int32_t *ixy, float *id, int *bad_dev, hipStream_t s) {
+    const long n = (long)L.ann_w * L.ann_h;
+    if (L.C > 0)
+        hipLaunchKernelGGL(k_init_prev<M_DESC>, dim3(pt_blocks(n, 256)), dim3(256), 0, s, L, prev, sv, recalc,
+                           temporal_scale, sanitize, ixy, id, bad_dev);
+    else
+        hipLaunchKernelGGL(k_init_prev<M_GLOBAL>, dim3(pt_blocks(n, 256)), dim3(256), 0, s, L, prev, sv, recalc,
+                           temporal_scale, sanitize, ixy, id, bad_dev);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, int32_t *gxy, float *gd, double *ann_out,
+                     const double *prev, const float *init_d, double temporal_scale, int temporal_fix, hipStream_t s,
+                     hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds, int *launches) {
+    if (L.C > PT_LOOKUP_CMAX) return pt_failf(PT_E_UNSUPPORTED, "descriptor dims %d > %d", L.C, PT_LOOKUP_CMAX);
+    int which = 0, nl = 0;
+    if (L.C > 0) which = launch_lookup<M_DESC>(L, f0, f1, s, ev_after_candidates, nl);
+    else if (L.qs == 1) which = launch_lookup<M_TILE>(L, f0, f1, s, ev_after_candidates, nl);
+    else which = launch_lookup<M_GLOBAL>(L, f0, f1, s, ev_after_candidates, nl);
+    if (which < 0) return which;
+    NnfField res = which ? f1 : f0;
+    const long n = (long)L.ann_w * L.ann_h;
+    const int32_t *oxy = res.xy;
+    const float *od = res.d;
+    if (L.qs > 1) {
+        LookupArgs G = L;
+        G.gqs = L.qs;
+        G.qs = 1;   // k_gather lanes are single patches; stage_query / qdist use qs 1
+        dim3 grid((L.ann_w + TILE_X - 1) / TILE_X, (L.ann_h + TILE_Y - 1) / TILE_Y);
+        if (L.C > 0) hipLaunchKernelGGL(k_gather<M_DESC>, grid, dim3(THREADS), 0, s, G, res, gxy, gd);
+        else hipLaunchKernelGGL(k_gather<M_TILE>, grid, dim3(THREADS), 0, s, G, res, gxy, gd);
+        oxy = gxy;
+        od = gd;
+        nl++;
     }
     if (ev_after_rounds) PT_HIP_TRY(hipEventRecord(ev_after_rounds, s));
-    hipLaunchKernelGGL(k_write_ann, dim3(pt_blocks(n, 256)), dim3(256), 0, s, f0, n, ann_out);
+    hipLaunchKernelGGL(k_write_ann, dim3(pt_blocks(n, 256)), dim3(256), 0, s, oxy, od, n, ann_out, prev, init_d,
+                       temporal_scale, temporal_fix);
+    PT_LAUNCH_CHECK();
+    if (launches) *launches = nl;
+    return PT_OK;
+}
+
+int pt_desc_pad_launch(const float *in, int h_in, int w_in, int C, int padded, int Hp, int Wp, float *planes,
+                       float *inter, hipStream_t s) {
+    const long n = (long)Hp * Wp;
+    hipLaunchKernelGGL(k_desc_pad, dim3(pt_blocks(n, 256)), dim3(256), 0, s, in, h_in, w_in, C, padded, Hp, Wp, planes,
+                       inter);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }
@@ -360,6 +667,13 @@ int pt_ssd_pairs_launch(const float *a, int aw, const float *b, int bw, const in
                         const int32_t *bx, const int32_t *by, int n, float *out, hipStream_t s) {
     if (n <= 0) return PT_OK;
     hipLaunchKernelGGL(k_ssd_pairs, dim3(pt_blocks(n, 256)), dim3(256), 0, s, a, aw, b, bw, ax, ay, bx, by, n, out);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_desc_dist_pairs_launch(const float *a, const float *b, int C, int n, float *out, hipStream_t s) {
+    if (n <= 0) return PT_OK;
+    hipLaunchKernelGGL(k_desc_dist_pairs, dim3(pt_blocks(n, 256)), dim3(256), 0, s, a, b, C, n, out);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }

@@ -88,4 +88,35 @@ __device__ __forceinline__ float pt_patch_ssd_bounded(const float *__restrict__ 
     return ((L0 + L1) + L2) + L3;
 }
 
+// patch_dist_approx (patchtable.h:376-437) with the SSE lane order of descriptor mode: the first
+// four dimensions as one 4-lane (a-b)^2; for C = 8, 12, 16, 20 the other 4-wide chunks are added
+// lane-wise and the lanes summed last, otherwise the lanes are summed and the remaining
+// dimensions added one by one (C >= 4).
+__device__ __forceinline__ float pt_desc_dist(const float *a, const float *__restrict__ b, int C) {
+    float L0, L1, L2, L3;
+    {
+        float e0 = a[0] - b[0], e1 = a[1] - b[1], e2 = a[2] - b[2], e3 = a[3] - b[3];
+        L0 = e0 * e0;
+        L1 = e1 * e1;
+        L2 = e2 * e2;
+        L3 = e3 * e3;
+    }
+    if (C == 8 || C == 12 || C == 16 || C == 20) {
+        for (int n = 4; n < C; n += 4) {
+            float e0 = a[n] - b[n], e1 = a[n + 1] - b[n + 1], e2 = a[n + 2] - b[n + 2], e3 = a[n + 3] - b[n + 3];
+            L0 = L0 + e0 * e0;
+            L1 = L1 + e1 * e1;
+            L2 = L2 + e2 * e2;
+            L3 = L3 + e3 * e3;
+        }
+        return ((L0 + L1) + L2) + L3;
+    }
+    float d = ((L0 + L1) + L2) + L3;
+    for (int i = 4; i < C; i++) {
+        float e = a[i] - b[i];
+        d = d + e * e;
+    }
+    return d;
+}
+
 #endif

@@ -53,10 +53,11 @@ int pt_gck_launch(const float *img, int h, int w, int n, int nchroma, int conver
                   float *temp, float *desc, hipStream_t s);
 
 // quant.hip
-int pt_minmax_launch(const float *desc, size_t plane, int Wp, int h, int w, int G, float *partials, int nblocks,
-                     hipStream_t s);
-int pt_quantile_values(const float *desc, size_t plane, int Wp, int h, int w, int step, int dim, const int64_t *ranks_dev,
-                       int nranks, float *values_dev, PtScratch &sc, hipStream_t s);
+// allowed: NULL or the (h-7) x (w-7) device mask; only patches equal to aidx take part.
+int pt_minmax_launch(const float *desc, size_t plane, int Wp, int h, int w, int G, const int32_t *allowed, int aidx,
+                     float *partials, int nblocks, hipStream_t s);
+int pt_quantile_values(const float *desc, size_t plane, int Wp, int h, int w, int step, int dim, const int32_t *allowed,
+                       int aidx, const int64_t *ranks_dev, int nranks, float *values_dev, PtScratch &sc, hipStream_t s);
 
 // populate.hip
 int pt_list_launch(int bew, int n, int32_t *list, hipStream_t s);
@@ -81,39 +82,67 @@ int pt_dt_euclidean_launch(int32_t *table, int G, const int *nslices, int64_t nc
                            hipStream_t s);
 
 // kcoh.hip
-int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, const float *b0, int bw, int bh,
-                         const DevQuant &q, const int *nslices_host, int K, int window, int min_dist2,
-                         int32_t *kpos, float *kdist, PtScratch &sc, hipStream_t s);
+// b_descI: NULL in image mode (lists re-scored by exact RGB SSD on b0), else the interleaved
+// descriptor (re-scored by patch_dist_approx). allowed: NULL or the (bh-7) x (bw-7) device mask.
+int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, const float *b0, const float *b_descI,
+                         int bw, int bh, const int32_t *allowed, int aidx, const DevQuant &q, const int *nslices_host,
+                         int K, int window, int min_dist2, int32_t *kpos, float *kdist, PtScratch &sc, hipStream_t s);
 
 // lookup.hip
+#define PT_LOOKUP_CMAX 40    // descriptor dims supported in descriptor mode
+#define PT_MAX_XTABLES 15    // further tables (ntables - 1)
 struct LookupArgs {
-    const float *a = nullptr;       // query RGB, HWC
-    int ah = 0, aw = 0;
-    const float *a_desc = nullptr;  // query descriptor planes
-    const float *b = nullptr;       // database RGB, HWC
+    const float *a = nullptr;       // query RGB, HWC (image mode)
+    int ah = 0, aw = 0;             // query "image" size (descriptor size + 7 in descriptor mode)
+    int ann_w = 0, ann_h = 0;       // query patches
+    const float *a_desc = nullptr;  // query descriptor planes, (ah+7) x (aw+7) each
+    const float *b = nullptr;       // database RGB, HWC (image mode)
     int bh = 0, bw = 0;
     const int32_t *table = nullptr;
     DevQuant q;
+    int nxt = 0;                    // further tables
+    const int32_t *xtable[PT_MAX_XTABLES] = {};
+    DevQuant xq[PT_MAX_XTABLES];
     const int32_t *kpos = nullptr;
     const float *kdist = nullptr;
     int K = 0;
-    int kcoh_iter0 = 1;             // k-coherence enabled on the first pass
+    int kcoh_iter0 = 1;             // k-coherence walk in the candidate pass
+    int kcoh_rounds = 0;            // ... and again in the propagation rounds
     double triangle_factor = 2.0;
     int spatial = 1;
+    int rs = 0;                     // random search (do_rs without spatial)
     int do_table_lookup = 1;
     int rounds = 4;
+    int qs = 1;                     // query_step: lattice spacing of the processed patches
+    int gqs = 1;                    // query_step seen by the gather pass
+    int vw = 0, vh = 0;             // lattice size
+    int C = 0;                      // descriptor mode: descriptor dims (0: image mode)
+    const float *a_descI = nullptr; // descriptor mode: query descriptor, interleaved
+    const float *b_descI = nullptr; // descriptor mode: database descriptor, interleaved
+    const int32_t *allowed = nullptr;   // (bh-7) x (bw-7) mask of allowed database patches
+    int allowed_index = 1;
+    const int32_t *init_xy = nullptr;   // initial field from a previous one (per patch)
+    const float *init_d = nullptr;
 };
-// One nearest-neighbour field: XY_TO_INT(b patch x, y), exact SSD, whether the spatial window
-// around that match was already searched without improvement, and whether the match changed.
+// One nearest-neighbour field over the lattice: XY_TO_INT(b patch x, y), distance, and flags
+// (1: the spatial window around the match was searched without improvement; 2: the match changed
+// in the round that produced the field; 4: the match's k-coherence list was walked).
 struct NnfField {
     int32_t *xy = nullptr;
     float *d = nullptr;
-    uint8_t *done = nullptr;
-    uint8_t *chg = nullptr;   // best changed in the round that produced this field
+    uint8_t *fl = nullptr;
 };
-int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, double *ann_out, hipStream_t s,
-                     hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds);
+void pt_jump_schedule(int rounds, std::vector<int> &J);
+int pt_lookup_init_prev(const LookupArgs &L, const double *prev, const float *sv, int recalc, double temporal_scale,
+                        int sanitize, int32_t *ixy, float *id, int *bad_dev, hipStream_t s);
+// Candidates, propagation rounds, gather (query_step > 1, into gxy/gd) and the double output.
+int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, int32_t *gxy, float *gd, double *ann_out,
+                     const double *prev, const float *init_d, double temporal_scale, int temporal_fix, hipStream_t s,
+                     hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds, int *launches);
+int pt_desc_pad_launch(const float *in, int h_in, int w_in, int C, int padded, int Hp, int Wp, float *planes,
+                       float *inter, hipStream_t s);
 int pt_ssd_pairs_launch(const float *a, int aw, const float *b, int bw, const int32_t *ax, const int32_t *ay,
                         const int32_t *bx, const int32_t *by, int n, float *out, hipStream_t s);
+int pt_desc_dist_pairs_launch(const float *a, const float *b, int C, int n, float *out, hipStream_t s);
 
 #endif

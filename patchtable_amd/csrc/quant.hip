@@ -11,9 +11,10 @@ namespace {
 
 constexpr int MM_THREADS = 256;
 
-// Partial min/max of one dimension over the valid descriptor region [7,h) x [7,w).
+// Partial min/max of one dimension over the valid descriptor region [7,h) x [7,w), allowed
+// patches only (TABLE_FOR_ALLOWED_PATCHES, patchtable.h:1766) when a mask is given.
 __global__ void k_minmax(const float *__restrict__ desc, size_t plane, int Wp, int h, int w, int G,
-                         float *__restrict__ partials) {
+                         const int32_t *__restrict__ allowed, int aidx, float *__restrict__ partials) {
     __shared__ float smin[MM_THREADS], smax[MM_THREADS];
     const int ew = w - 7;
     const long n = (long)(h - 7) * ew;
@@ -22,6 +23,7 @@ __global__ void k_minmax(const float *__restrict__ desc, size_t plane, int Wp, i
         float mn = (float)1e20, mx = (float)-1e20;
         for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
             long y = k / ew, x = k - y * ew;
+            if (allowed && allowed[k] != aidx) continue;
             float v = P[(size_t)(y + 7) * Wp + x + 7];
             mn = mn < v ? mn : v;
             mx = mx > v ? mx : v;
@@ -48,12 +50,16 @@ __global__ void k_minmax(const float *__restrict__ desc, size_t plane, int Wp, i
 }
 
 // Values of one dimension over the patches of the partition grid (every `step`-th patch in x
-// and y, patchtable.h:1357-1370).
-__global__ void k_extract(const float *__restrict__ desc, int Wp, int nx, int ny, int step, float *__restrict__ out) {
+// and y, patchtable.h:1357-1370); patches outside the mask get +inf so that they sort after
+// every allowed value and the order statistics of the allowed ones come first.
+__global__ void k_extract(const float *__restrict__ desc, int Wp, int nx, int ny, int step, int bew,
+                          const int32_t *__restrict__ allowed, int aidx, float *__restrict__ out) {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)nx * ny) return;
     long yy = k / nx, xx = k - yy * nx;
-    out[k] = desc[(size_t)(7 + yy * step) * Wp + 7 + xx * step];
+    float v = desc[(size_t)(7 + yy * step) * Wp + 7 + xx * step];
+    if (allowed && allowed[(size_t)yy * step * bew + xx * step] != aidx) v = INFINITY;
+    out[k] = v;
 }
 
 __global__ void k_gather(const float *__restrict__ sorted, const int64_t *__restrict__ ranks, int nr, float *__restrict__ out) {
@@ -63,22 +69,23 @@ __global__ void k_gather(const float *__restrict__ sorted, const int64_t *__rest
 
 }  // namespace
 
-int pt_minmax_launch(const float *desc, size_t plane, int Wp, int h, int w, int G, float *partials, int nblocks,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(k_minmax, dim3(nblocks), dim3(MM_THREADS), 0, s, desc, plane, Wp, h, w, G, partials);
+int pt_minmax_launch(const float *desc, size_t plane, int Wp, int h, int w, int G, const int32_t *allowed, int aidx,
+                     float *partials, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_minmax, dim3(nblocks), dim3(MM_THREADS), 0, s, desc, plane, Wp, h, w, G, allowed, aidx,
+                       partials);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }
 
-int pt_quantile_values(const float *desc, size_t plane, int Wp, int h, int w, int step, int dim, const int64_t *ranks_dev,
-                       int nranks, float *values_dev, PtScratch &sc, hipStream_t s) {
+int pt_quantile_values(const float *desc, size_t plane, int Wp, int h, int w, int step, int dim, const int32_t *allowed,
+                       int aidx, const int64_t *ranks_dev, int nranks, float *values_dev, PtScratch &sc, hipStream_t s) {
     int nx = (w - 7 + step - 1) / step, ny = (h - 7 + step - 1) / step;
     long n = (long)nx * ny;
     float *vals = (float *)sc.get(10, (size_t)n * 4);
     float *sorted = (float *)sc.get(11, (size_t)n * 4);
     if (!vals || !sorted) return pt_fail(PT_E_NOMEM, "quantiser: scratch allocation failed");
     hipLaunchKernelGGL(k_extract, dim3(pt_blocks(n, 256)), dim3(256), 0, s, desc + (size_t)dim * plane, Wp, nx, ny, step,
-                       vals);
+                       w - 7, allowed, aidx, vals);
     size_t tmp_bytes = 0;
     PT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, vals, sorted, (int)n, 0, 32, s));
     void *tmp = sc.get(12, tmp_bytes);

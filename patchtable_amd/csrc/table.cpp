@@ -70,17 +70,22 @@ struct pt_table {
     pt_params build;          // parameters the table was built with
     hipStream_t stream = nullptr;
     hipEvent_t ev[12] = {};
-    int h = 0, w = 0;
+    int h = 0, w = 0;         // "image" size: b, or the descriptor size + 7 in descriptor mode
     int ndims = 0, nchroma = 0, G = 0;
+    bool desc_mode = false;   // is_descriptor: b is a descriptor image, distances are descriptor L2
     std::vector<int> nslices;
     std::vector<int64_t> stride;
     int64_t ncells = 0;
     std::vector<HostPartition> part;
     int table_add_count = 0;
+    long rand_used = 0;       // rand() outputs consumed by this table's shuffle and the earlier ones
     int K = 0;
     double stage_ms[12] = {};
+    std::vector<pt_table *> extra;   // the ntables-1 further tables (patchtable.h:3359-3372)
 
-    DevBuf<float> b0, desc;
+    DevBuf<float> b0, desc, descI;   // descI: interleaved descriptor (descriptor mode)
+    DevBuf<int32_t> allowed;         // (h-7) x (w-7) mask, empty without one
+    int allowed_index = 1;
     DevBuf<float> qmin, qrange;
     DevBuf<int32_t> qtobin;
     DevBuf<int64_t> qstride;
@@ -93,12 +98,15 @@ struct pt_table {
     DevBuf<uint64_t> dtkey;
     DevBuf<int32_t> kpos;
     DevBuf<float> kdist;
+    int n_pop = 0;          // patches listed for populate (allowed ones)
     // query side
-    DevBuf<float> a, adesc, yuv, temp;
-    DevBuf<int32_t> nxy, nxy2;
-    DevBuf<float> nd, nd2;
-    DevBuf<uint8_t> ndone, ndone2, nchg, nchg2;
-    DevBuf<double> ann;
+    DevBuf<float> a, adesc, adescI, yuv, temp;
+    DevBuf<int32_t> nxy, nxy2, gxy, ixy;
+    DevBuf<float> nd, nd2, gd, id;
+    DevBuf<uint8_t> nfl, nfl2;
+    DevBuf<double> ann, prevbuf;
+    DevBuf<float> svbuf;
+    DevBuf<int> bad;
     int qh = 0, qw = 0;
     PtScratch sc;
 
@@ -112,6 +120,7 @@ struct pt_table {
         return q;
     }
     ~pt_table() {
+        for (pt_table *e : extra) delete e;
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -128,15 +137,17 @@ int init_stream(pt_table *t) {
     return PT_OK;
 }
 
+// Descriptor sizes the reference instantiates (DECLARE_CONSTRUCTOR, patchtable.h:4405-4423),
+// from 4 on (patch_dist_approx reads four values at once).
+bool desc_dims_supported(int C) { return C >= 4 && C <= PT_LOOKUP_CMAX; }
+
 int check_build_params(const pt_params *p, int c) {
-    if (p->is_descriptor) return pt_fail(PT_E_UNSUPPORTED, "is_descriptor mode is not supported by this build yet");
     if (p->lookup_algo != PT_LOOKUP_ALGO_TABLE) return pt_fail(PT_E_UNSUPPORTED, "only lookup_algo table is supported");
-    if (p->dim_algo != PT_DIM_ALGO_WH) return pt_fail(PT_E_UNSUPPORTED, "only dim_algo wh is supported");
-    if (p->convert_colorspace && p->colorspace != PT_COLORSPACE_YUV)
-        return pt_fail(PT_E_UNSUPPORTED, "only the yuv and rgb colorspaces are supported");
     if (p->patch_w != 8) return pt_fail(PT_E_UNSUPPORTED, "gck: patch_w unsupported (only 8)");
     if (p->filter_dims_count) return pt_fail(PT_E_UNSUPPORTED, "filter_dims is not supported by this build yet");
-    if (p->ntables != 1) return pt_fail(PT_E_UNSUPPORTED, "ntables > 1 is not supported by this build yet");
+    if (p->ntables < 1 || p->ntables > PT_MAX_XTABLES + 1)
+        return pt_failf(PT_E_UNSUPPORTED, "ntables must be in 1..%d", PT_MAX_XTABLES + 1);
+    if (p->ntables > 1 && !(p->table_ratio > 0)) return pt_fail(PT_E_INVALID, "table_ratio must be > 0");
     if (p->kcoherence && p->kcoherence_algo != PT_KCOH_ALGO_FLANN)
         return pt_fail(PT_E_UNSUPPORTED, "only the default kcoherence_algo is supported");
     if (p->kcoherence_step != 1) return pt_fail(PT_E_UNSUPPORTED, "kcoherence_step > 1 is not supported");
@@ -146,9 +157,19 @@ int check_build_params(const pt_params *p, int c) {
                        p->dt_algo == PT_DT_ALGO_EUCLIDEAN))
         return pt_fail(PT_E_UNSUPPORTED, "dt_algo must be prop, downsample or euclidean");
     if (p->regular_grid) return pt_fail(PT_E_UNSUPPORTED, "regular_grid is not supported");
-    if (c != 3) return pt_fail(PT_E_INVALID, "expected 3 channel image as input to PatchTable");
     if (p->kcoherence < 0 || p->kcoherence > 32) return pt_fail(PT_E_UNSUPPORTED, "kcoherence must be in 0..32");
     if (p->partition_step < 1) return pt_fail(PT_E_INVALID, "partition_step must be >= 1");
+    if (p->is_descriptor) {
+        if (!desc_dims_supported(c))
+            return pt_failf(PT_E_UNSUPPORTED, "descriptor mode: %d channels unsupported (4..%d)", c, PT_LOOKUP_CMAX);
+        int gd = p->grid_ndims < 0 ? c : p->grid_ndims;
+        if (gd < 1 || gd > c) return pt_fail(PT_E_INVALID, "invalid grid_ndims");
+        return PT_OK;
+    }
+    if (p->dim_algo != PT_DIM_ALGO_WH) return pt_fail(PT_E_UNSUPPORTED, "only dim_algo wh is supported");
+    if (p->convert_colorspace && p->colorspace != PT_COLORSPACE_YUV)
+        return pt_fail(PT_E_UNSUPPORTED, "only the yuv and rgb colorspaces are supported");
+    if (c != 3) return pt_fail(PT_E_INVALID, "expected 3 channel image as input to PatchTable");
     int gd = p->grid_ndims < 0 ? p->ndims : p->grid_ndims;
     if (p->ndims < 1 || p->ndims > PT_MAX_DIMS || gd < 1 || gd > p->ndims)
         return pt_fail(PT_E_INVALID, "invalid ndims/grid_ndims");
@@ -228,7 +249,7 @@ void make_partition(HostPartition &P, const std::vector<float> &q) {
     }
 }
 
-int upload_quant(pt_table *t) {
+int upload_quant(pt_table *t, hipStream_t s) {
     const int G = t->G;
     std::vector<float> mn(G), rg(G);
     std::vector<int32_t> tb((size_t)G * PT_NMAP);
@@ -241,34 +262,52 @@ int upload_quant(pt_table *t) {
     TRY(t->qrange.ensure(G));
     TRY(t->qtobin.ensure(tb.size()));
     TRY(t->qstride.ensure(G));
-    HIPTRY(hipMemcpyAsync(t->qmin.p, mn.data(), G * 4, hipMemcpyHostToDevice, t->stream));
-    HIPTRY(hipMemcpyAsync(t->qrange.p, rg.data(), G * 4, hipMemcpyHostToDevice, t->stream));
-    HIPTRY(hipMemcpyAsync(t->qtobin.p, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, t->stream));
-    HIPTRY(hipMemcpyAsync(t->qstride.p, t->stride.data(), G * 8, hipMemcpyHostToDevice, t->stream));
-    HIPTRY(hipStreamSynchronize(t->stream));
+    HIPTRY(hipMemcpyAsync(t->qmin.p, mn.data(), G * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(t->qrange.p, rg.data(), G * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(t->qtobin.p, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(t->qstride.p, t->stride.data(), G * 8, hipMemcpyHostToDevice, s));
+    HIPTRY(hipStreamSynchronize(s));
     return PT_OK;
 }
 
-int build_table(pt_table *t, const float *b_dev) {
+// The glibc generator state after `skip` outputs from the state of a fresh process.
+void advanced_rand_state(long skip, uint32_t st[31]) {
+    pt_glibc_initial_state(1, st);
+    const uint32_t *jm = pt_glibc_jump_matrices(PT_RAND_CHUNK, PT_RAND_LEVELS);
+    long q = skip / PT_RAND_CHUNK, r = skip % PT_RAND_CHUNK;
+    for (int lv = 0; q; lv++, q >>= 1) {
+        if (!(q & 1)) continue;
+        const uint32_t *P = jm + (size_t)lv * 961;
+        uint32_t nx[31];
+        for (int i = 0; i < 31; i++) {
+            uint32_t acc = 0;
+            for (int k = 0; k < 31; k++) acc += P[i * 31 + k] * st[k];
+            nx[i] = acc;
+        }
+        std::memcpy(st, nx, sizeof(nx));
+    }
+    for (long k = 0; k < r; k++) {
+        uint32_t v = st[0] + st[28];
+        std::memmove(st, st + 1, 30 * 4);
+        st[30] = v;
+    }
+}
+
+// Quantiser, populate and DT of table t (PatchTableFixedN constructor, patchtable.h:2567-3295)
+// from the descriptor planes `desc` (owned by the main table). `mask_host` is the allowed mask
+// (or empty); `rand_skip` the rand() outputs used by earlier tables' shuffles. Records events
+// ev[1..4] of `evt` when given.
+int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_host, const int32_t *mask_dev,
+               long rand_skip, hipStream_t s, const hipEvent_t *evt) {
     const pt_params &p = t->build;
-    const int h = t->h, w = t->w, Hp = h + 7, Wp = w + 7;
-    const size_t plane = (size_t)Hp * Wp;
-    hipStream_t s = t->stream;
-    HIPTRY(hipEventRecord(t->ev[0], s));
-    // 1. keep a copy of b (patchtable.h:2628) and project it (get_padded_descriptor, :2626)
-    TRY(t->b0.ensure((size_t)h * w * 3));
-    HIPTRY(hipMemcpyAsync(t->b0.p, b_dev, (size_t)h * w * 3 * 4, hipMemcpyDeviceToDevice, s));
-    TRY(t->desc.ensure(plane * t->ndims));
-    TRY(t->yuv.ensure((size_t)h * w * 3));
-    TRY(t->temp.ensure(plane * 3));
-    TRY(pt_gck_launch(t->b0.p, h, w, t->ndims - 2 * t->nchroma, t->nchroma, p.convert_colorspace, t->yuv.p, t->temp.p,
-                      t->desc.p, s));
-    HIPTRY(hipEventRecord(t->ev[1], s));
-    // 2. quantiser: min/max (patchtable.h:1758-1778), slices, order statistics, to_bin
+    const int h = t->h, w = t->w, Wp = w + 7;
+    const size_t plane = (size_t)(h + 7) * Wp;
+    const int aidx = t->allowed_index;
+    // quantiser: min/max (patchtable.h:1758-1778), slices, order statistics, to_bin
     const int G = t->G, NB = 256;
     float *partials = (float *)t->sc.get(40, (size_t)NB * G * 2 * 4);
     if (!partials) return pt_fail(PT_E_NOMEM, "min/max scratch allocation failed");
-    TRY(pt_minmax_launch(t->desc.p, plane, Wp, h, w, G, partials, NB, s));
+    TRY(pt_minmax_launch(desc, plane, Wp, h, w, G, mask_dev, aidx, partials, NB, s));
     std::vector<float> hp((size_t)NB * G * 2);
     HIPTRY(hipMemcpyAsync(hp.data(), partials, hp.size() * 4, hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
@@ -301,7 +340,14 @@ int build_table(pt_table *t, const float *b_dev) {
     t->ncells = acc;
     if (acc > ((int64_t)1 << 31) - 1) return pt_fail(PT_E_UNSUPPORTED, "table larger than 2^31 cells");
     const int step = p.partition_step;
-    const int64_t npart = (int64_t)((w - 7 + step - 1) / step) * ((h - 7 + step - 1) / step);
+    const int bew = w - 7, beh = h - 7;
+    int64_t npart = (int64_t)((bew + step - 1) / step) * ((beh + step - 1) / step);
+    if (!mask_host.empty()) {   // allowed patches of the partition grid (patchtable.h:1357-1378)
+        npart = 0;
+        for (int y = 0; y < beh; y += step)
+            for (int x = 0; x < bew; x += step) npart += mask_host[(size_t)y * bew + x] == aidx;
+        if (npart == 0) return pt_fail(PT_E_INVALID, "no allowed patch on the partition grid");
+    }
     t->part.assign(G, HostPartition());
     {
         int maxns = *std::max_element(t->nslices.begin(), t->nslices.end());
@@ -317,48 +363,60 @@ int build_table(pt_table *t, const float *b_dev) {
             std::vector<int64_t> hr(P.nslices + 1);
             for (int j = 0; j <= P.nslices; j++) hr[j] = (npart - 1) * (int64_t)j / P.nslices;
             HIPTRY(hipMemcpyAsync(ranks, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, s));
-            TRY(pt_quantile_values(t->desc.p, plane, Wp, h, w, step, i, ranks, (int)hr.size(), vals, t->sc, s));
+            TRY(pt_quantile_values(desc, plane, Wp, h, w, step, i, mask_dev, aidx, ranks, (int)hr.size(), vals, t->sc, s));
             std::vector<float> q(hr.size());
             HIPTRY(hipMemcpyAsync(q.data(), vals, q.size() * 4, hipMemcpyDeviceToHost, s));
             HIPTRY(hipStreamSynchronize(s));
             make_partition(P, q);
         }
     }
-    TRY(upload_quant(t));
-    HIPTRY(hipEventRecord(t->ev[2], s));
-    // 3. populate (patchtable.h:3121-3159)
-    const int bew = w - 7, beh = h - 7;
-    const int n = bew * beh;
-    {
-        TRY(t->list_xy.ensure(n));
+    TRY(upload_quant(t, s));
+    if (evt) HIPTRY(hipEventRecord(evt[2], s));
+    // populate (patchtable.h:3121-3159): allowed patches in raster order, shuffled
+    int n = bew * beh;
+    TRY(t->list_xy.ensure(n));
+    if (mask_host.empty()) {
         TRY(pt_list_launch(bew, n, t->list_xy.p, s));
-        if (p.populate_random) {
-            if ((int64_t)n > ((int64_t)PT_RAND_CHUNK << PT_RAND_LEVELS))
-                return pt_fail(PT_E_UNSUPPORTED, "too many database patches for the rand() jump tables");
-            uint32_t st[31];
-            pt_glibc_initial_state(1, st);
-            const uint32_t *jm = pt_glibc_jump_matrices(PT_RAND_CHUNK, PT_RAND_LEVELS);
-            TRY(t->rnd.ensure(31 + (size_t)PT_RAND_LEVELS * 961));
-            HIPTRY(hipMemcpyAsync(t->rnd.p, st, 31 * 4, hipMemcpyHostToDevice, s));
-            HIPTRY(hipMemcpyAsync(t->rnd.p + 31, jm, (size_t)PT_RAND_LEVELS * 961 * 4, hipMemcpyHostToDevice, s));
-        }
-        TRY(t->rank.ensure(n));
-        TRY(pt_shuffle_ranks(t->rnd.p + 31, t->rnd.p, n, p.populate_random, t->sc, t->rank.p, s));
-        TRY(t->table.ensure((size_t)t->ncells));
-        TRY(t->cells.ensure(n));
-        TRY(t->win.ensure(n));
-        TRY(t->counter.ensure(1));
-        TRY(pt_populate_launch(t->desc.p, plane, Wp, t->list_xy.p, t->rank.p, n, t->quant(), t->table.p, t->ncells,
-                               t->cells.p, t->win.p, t->counter.p, s));
-        HIPTRY(hipMemcpyAsync(&t->table_add_count, t->counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    } else {
+        std::vector<int32_t> lst;
+        lst.reserve(n);
+        for (int y = 0; y < beh; y++)
+            for (int x = 0; x < bew; x++)
+                if (mask_host[(size_t)y * bew + x] == aidx) lst.push_back(PT_XY_TO_INT(x + 7, y + 7));
+        n = (int)lst.size();
+        HIPTRY(hipMemcpyAsync(t->list_xy.p, lst.data(), lst.size() * 4, hipMemcpyHostToDevice, s));
         HIPTRY(hipStreamSynchronize(s));
     }
-    HIPTRY(hipEventRecord(t->ev[3], s));
-    // 4. distance transform (patchtable.h:3254-3295) and fill_zeros (:2063-2079)
+    t->n_pop = n;
+    t->rand_used = rand_skip;
+    if (p.populate_random) {
+        if ((int64_t)n > ((int64_t)PT_RAND_CHUNK << PT_RAND_LEVELS))
+            return pt_fail(PT_E_UNSUPPORTED, "too many database patches for the rand() jump tables");
+        uint32_t st[31];
+        advanced_rand_state(rand_skip, st);
+        const uint32_t *jm = pt_glibc_jump_matrices(PT_RAND_CHUNK, PT_RAND_LEVELS);
+        TRY(t->rnd.ensure(31 + (size_t)PT_RAND_LEVELS * 961));
+        HIPTRY(hipMemcpyAsync(t->rnd.p, st, 31 * 4, hipMemcpyHostToDevice, s));
+        HIPTRY(hipMemcpyAsync(t->rnd.p + 31, jm, (size_t)PT_RAND_LEVELS * 961 * 4, hipMemcpyHostToDevice, s));
+        if (n > 1) t->rand_used += n - 1;
+    }
+    TRY(t->rank.ensure(n));
+    TRY(pt_shuffle_ranks(t->rnd.p + 31, t->rnd.p, n, p.populate_random, t->sc, t->rank.p, s));
+    TRY(t->table.ensure((size_t)t->ncells));
+    TRY(t->cells.ensure(n));
+    TRY(t->win.ensure(n));
+    TRY(t->counter.ensure(1));
+    TRY(pt_populate_launch(desc, plane, Wp, t->list_xy.p, t->rank.p, n, t->quant(), t->table.p, t->ncells, t->cells.p,
+                           t->win.p, t->counter.p, s));
+    HIPTRY(hipMemcpyAsync(&t->table_add_count, t->counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (evt) HIPTRY(hipEventRecord(evt[3], s));
+    // distance transform (patchtable.h:3254-3295) and fill_zeros (:2063-2079)
     const int32_t vfill = PT_XY_TO_INT(7, 7);
     if (p.run_dt && p.dt_algo == PT_DT_ALGO_PROP) {
         TRY(t->dtkey.ensure((size_t)t->ncells));
         TRY(pt_dt_prop_launch(t->table.p, G, t->nslices.data(), t->ncells, p.dt_iters, vfill, t->dtkey.p, s));
+        HIPTRY(hipStreamSynchronize(s));
         t->dtkey.release();
     } else if (p.run_dt && p.dt_algo == PT_DT_ALGO_DOWNSAMPLE) {
         TRY(pt_dt_downsample_launch(t->table.p, G, t->nslices.data(), t->sc, s));
@@ -371,15 +429,75 @@ int build_table(pt_table *t, const float *b_dev) {
         // run_dt off: fill_zeros replaces every empty cell (patchtable.h:3289-3293)
         TRY(pt_fill_unused_launch(t->table.p, t->ncells, vfill, s));
     }
-    HIPTRY(hipEventRecord(t->ev[4], s));
+    if (evt) HIPTRY(hipEventRecord(evt[4], s));
+    return PT_OK;
+}
+
+// PatchTableFixedN constructor (patchtable.h:2584-3373) from a device image (RGB, or a
+// descriptor image in descriptor mode) of h_in x w_in x c.
+int build_table(pt_table *t, const float *b_dev, int h_in, int w_in, int c, const std::vector<int32_t> &mask_host) {
+    const pt_params &p = t->build;
+    hipStream_t s = t->stream;
+    const int h = t->h, w = t->w, Hp = h + 7, Wp = w + 7;
+    const size_t plane = (size_t)Hp * Wp;
+    HIPTRY(hipEventRecord(t->ev[0], s));
+    // 1. descriptor (get_padded_descriptor, patchtable.h:2492-2546) and the copy of b (:2628)
+    TRY(t->desc.ensure(plane * t->ndims));
+    if (t->desc_mode) {
+        TRY(t->descI.ensure(plane * t->ndims));
+        TRY(pt_desc_pad_launch(b_dev, h_in, w_in, c, p.descriptor_padded, Hp, Wp, t->desc.p, t->descI.p, s));
+    } else {
+        TRY(t->b0.ensure((size_t)h * w * 3));
+        HIPTRY(hipMemcpyAsync(t->b0.p, b_dev, (size_t)h * w * 3 * 4, hipMemcpyDeviceToDevice, s));
+        TRY(t->yuv.ensure((size_t)h * w * 3));
+        TRY(t->temp.ensure(plane * 3));
+        TRY(pt_gck_launch(t->b0.p, h, w, t->ndims - 2 * t->nchroma, t->nchroma, p.convert_colorspace, t->yuv.p,
+                          t->temp.p, t->desc.p, s));
+    }
+    HIPTRY(hipEventRecord(t->ev[1], s));
+    if (!mask_host.empty()) {
+        TRY(t->allowed.ensure(mask_host.size()));
+        HIPTRY(hipMemcpyAsync(t->allowed.p, mask_host.data(), mask_host.size() * 4, hipMemcpyHostToDevice, s));
+    }
+    const int32_t *mask_dev = mask_host.empty() ? nullptr : t->allowed.p;
+    // 2-4. quantiser, populate, DT
+    TRY(build_grid(t, t->desc.p, mask_host, mask_dev, 0, s, t->ev));
     // 5. k-coherence (patchtable.h:2653-2987)
     t->K = p.kcoherence;
     if (t->K > 0) {
-        TRY(t->kpos.ensure((size_t)n * t->K));
-        TRY(t->kdist.ensure((size_t)n * t->K));
-        TRY(pt_kcoherence_launch(t->desc.p, plane, Wp, t->ndims, t->b0.p, w, h, t->quant(), t->nslices.data(), t->K,
-                                 p.kcoh_window, p.kcoherence_min_dist * p.kcoherence_min_dist, t->kpos.p, t->kdist.p,
-                                 t->sc, s));
+        const size_t n = (size_t)(w - 7) * (h - 7);
+        TRY(t->kpos.ensure(n * t->K));
+        TRY(t->kdist.ensure(n * t->K));
+        TRY(pt_kcoherence_launch(t->desc.p, plane, Wp, t->ndims, t->b0.p, t->desc_mode ? t->descI.p : nullptr, w, h,
+                                 mask_dev, t->allowed_index, t->quant(), t->nslices.data(), t->K, p.kcoh_window,
+                                 p.kcoherence_min_dist * p.kcoherence_min_dist, t->kpos.p, t->kdist.p, t->sc, s));
+    }
+    // 6. further tables (patchtable.h:3359-3372): limit / table_ratio^i, no k-coherence; their
+    // shuffles continue the rand() stream after the earlier tables'.
+    long used = t->rand_used;
+    for (int i = 1; i < p.ntables; i++) {
+        pt_table *E = new pt_table();
+        t->extra.push_back(E);
+        E->build = p;
+        E->build.limit = (double)(int)(p.limit * (1.0 / pow(p.table_ratio, i)));
+        E->build.kcoherence = 0;
+        E->build.ntables = 1;
+        E->h = h;
+        E->w = w;
+        E->ndims = t->ndims;
+        E->nchroma = t->nchroma;
+        E->G = t->G;
+        E->desc_mode = t->desc_mode;
+        E->allowed_index = t->allowed_index;
+        TRY(build_grid(E, t->desc.p, mask_host, mask_dev, used, s, nullptr));
+        used = E->rand_used;
+        HIPTRY(hipStreamSynchronize(s));
+        E->list_xy.release();
+        E->rank.release();
+        E->cells.release();
+        E->win.release();
+        E->rnd.release();
+        E->sc.release();
     }
     HIPTRY(hipEventRecord(t->ev[5], s));
     HIPTRY(hipStreamSynchronize(s));
@@ -400,29 +518,46 @@ int create_common(pt_params *p, const float *b, bool b_on_device, int h, int w, 
     *out = nullptr;
     if (p->load_filename[0]) return pt_table_load(p, p->load_filename, out);
     if (!b) return pt_fail(PT_E_INVALID, "pt_table_create: null image");
-    if (allowed) return pt_fail(PT_E_UNSUPPORTED, "allowed_patches is not supported by this build yet");
     TRY(check_build_params(p, c));
-    if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "database image must be at least 8x8");
-    if (h > 65536 - 8 || w > 65536 - 8) return pt_fail(PT_E_INVALID, "image too large for XY_TO_INT packing");
+    int vh = h, vw = w;   // "image" size
+    if (p->is_descriptor) {
+        vh = p->descriptor_padded ? h - 7 : h + 7;
+        vw = p->descriptor_padded ? w - 7 : w + 7;
+    }
+    if (vh < 8 || vw < 8) return pt_fail(PT_E_INVALID, "database image must be at least 8x8 (descriptor: 1x1)");
+    if (vh > 65536 - 8 || vw > 65536 - 8) return pt_fail(PT_E_INVALID, "image too large for XY_TO_INT packing");
+    if (p->is_descriptor) {   // patchtable.h:4398-4401, written back as the reference does
+        p->ndims = c;
+        p->nchroma = 0;
+    }
     if (p->grid_ndims < 0) p->grid_ndims = p->ndims;   // patchtable.h:2603
+    std::vector<int32_t> mask;
+    if (allowed) {
+        mask.assign(allowed, allowed + (size_t)(vh - 7) * (vw - 7));
+        bool any = false;
+        for (int32_t v : mask) any |= v == p->allowed_index;
+        if (!any) return pt_fail(PT_E_INVALID, "allowed_patches allows no patch");
+    }
     pt_table *t = new pt_table();
     t->p = p;
     t->build = *p;
-    t->h = h;
-    t->w = w;
+    t->h = vh;
+    t->w = vw;
     t->ndims = p->ndims;
     t->nchroma = p->nchroma;
     t->G = p->grid_ndims;
+    t->desc_mode = p->is_descriptor != 0;
+    t->allowed_index = p->allowed_index;
     int rc = init_stream(t);
     DevBuf<float> staging;
     const float *bd = b;
     if (rc == PT_OK && !b_on_device) {
-        rc = staging.ensure((size_t)h * w * 3);
-        if (rc == PT_OK && hipMemcpy(staging.p, b, (size_t)h * w * 3 * 4, hipMemcpyHostToDevice) != hipSuccess)
+        rc = staging.ensure((size_t)h * w * c);
+        if (rc == PT_OK && hipMemcpy(staging.p, b, (size_t)h * w * c * 4, hipMemcpyHostToDevice) != hipSuccess)
             rc = pt_fail(PT_E_HIP, "hipMemcpy of the database image failed");
         bd = staging.p;
     }
-    if (rc == PT_OK) rc = build_table(t, bd);
+    if (rc == PT_OK) rc = build_table(t, bd, h, w, c, mask);
     if (rc == PT_OK && p->save_filename[0]) rc = pt_table_save(t, p->save_filename);
     if (rc != PT_OK) {
         delete t;
@@ -432,68 +567,138 @@ int create_common(pt_params *p, const float *b, bool b_on_device, int h, int w, 
     return PT_OK;
 }
 
-int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, double *seconds) {
+// Query dims of a lookup input of h x w x c: "image" size and patch counts.
+int query_shape(const pt_table *t, int h, int w, int c, int &ah, int &aw) {
+    if (t->desc_mode) {
+        if (c != t->ndims) return pt_failf(PT_E_INVALID, "expected a %d channel query descriptor", t->ndims);
+        ah = t->build.descriptor_padded ? h - 7 : h + 7;
+        aw = t->build.descriptor_padded ? w - 7 : w + 7;
+    } else {
+        if (c != 3) return pt_fail(PT_E_INVALID, "expected 3 channel query image");
+        ah = h;
+        aw = w;
+    }
+    if (ah < 8 || aw < 8) return pt_fail(PT_E_INVALID, "query image must be at least 8x8 (descriptor: 1x1)");
+    if (ah > 65536 - 8 || aw > 65536 - 8) return pt_fail(PT_E_INVALID, "query image too large");
+    return PT_OK;
+}
+
+int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, const double *prev_dev,
+                  const float *sv_dev, double *seconds) {
     const pt_params &p = *t->p;
-    if (c != 3) return pt_fail(PT_E_INVALID, "expected 3 channel query image");
-    if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "query image must be at least 8x8");
-    if (!p.do_table_lookup) return pt_fail(PT_E_UNSUPPORTED, "do_table_lookup 0 requires ann_prev (not supported)");
+    int ah = 0, aw = 0;
+    TRY(query_shape(t, h, w, c, ah, aw));
+    if (!p.do_table_lookup && !prev_dev)
+        return pt_fail(PT_E_UNSUPPORTED, "do_table_lookup 0 requires a previous field (ann_prev)");
     if (p.init_random) return pt_fail(PT_E_UNSUPPORTED, "init_random is not supported");
-    if (p.query_step != 1) return pt_fail(PT_E_UNSUPPORTED, "query_step > 1 is not supported by this build yet");
+    if (p.query_step < 1) return pt_fail(PT_E_INVALID, "query_step must be >= 1");
+    if (p.prop_dist != 0) return pt_fail(PT_E_UNSUPPORTED, "prop_dist != 0 is not supported");
     if (p.prop_iters < 1) return pt_fail(PT_E_UNSUPPORTED, "prop_iters must be >= 1");
-    if (!p.spatial && p.do_rs) return pt_fail(PT_E_UNSUPPORTED, "random search without spatial search is not supported");
+    if (p.coherence_spatial != 0.0) return pt_fail(PT_E_UNSUPPORTED, "coherence_spatial != 0 is not supported");
+    if (p.kcoherence > t->K) return pt_fail(PT_E_INVALID, "kcoherence exceeds the table's k-coherence lists");
+    if (p.spatial < 0) return pt_fail(PT_E_INVALID, "spatial must be >= 0");
     hipStream_t s = t->stream;
-    const int Hp = h + 7, Wp = w + 7;
+    const int Hp = ah + 7, Wp = aw + 7;
     const size_t plane = (size_t)Hp * Wp;
-    const long n = (long)(h - 7) * (w - 7);
+    const int ann_w = aw - 7, ann_h = ah - 7;
+    const long n = (long)ann_w * ann_h;
+    const int qs = p.query_step;
+    const int vw = (ann_w + qs - 1) / qs, vh = (ann_h + qs - 1) / qs;
+    const long nv = (long)vw * vh;
     TRY(t->adesc.ensure(plane * t->ndims));
-    TRY(t->yuv.ensure((size_t)h * w * 3));
-    TRY(t->temp.ensure(plane * 3));
-    TRY(t->nxy.ensure(n));
-    TRY(t->nd.ensure(n));
-    TRY(t->ndone.ensure(n));
-    TRY(t->nxy2.ensure(n));
-    TRY(t->nd2.ensure(n));
-    TRY(t->ndone2.ensure(n));
-    TRY(t->nchg.ensure(n));
-    TRY(t->nchg2.ensure(n));
-    t->qh = h;
-    t->qw = w;
+    TRY(t->nxy.ensure(nv));
+    TRY(t->nd.ensure(nv));
+    TRY(t->nfl.ensure(nv));
+    TRY(t->nxy2.ensure(nv));
+    TRY(t->nd2.ensure(nv));
+    TRY(t->nfl2.ensure(nv));
+    if (qs > 1) {
+        TRY(t->gxy.ensure(n));
+        TRY(t->gd.ensure(n));
+    }
+    t->qh = ah;
+    t->qw = aw;
     double T0 = now_s();
     HIPTRY(hipEventRecord(t->ev[6], s));
-    TRY(pt_gck_launch(a_dev, h, w, t->ndims - 2 * t->nchroma, t->nchroma, t->build.convert_colorspace, t->yuv.p,
-                      t->temp.p, t->adesc.p, s));
-    HIPTRY(hipEventRecord(t->ev[7], s));
     LookupArgs L;
-    L.a = a_dev;
-    L.ah = h;
-    L.aw = w;
+    if (t->desc_mode) {
+        TRY(t->adescI.ensure(plane * t->ndims));
+        TRY(pt_desc_pad_launch(a_dev, h, w, c, t->build.descriptor_padded, Hp, Wp, t->adesc.p, t->adescI.p, s));
+        L.C = t->ndims;
+        L.a_descI = t->adescI.p;
+        L.b_descI = t->descI.p;
+    } else {
+        TRY(t->yuv.ensure((size_t)ah * aw * 3));
+        TRY(t->temp.ensure(plane * 3));
+        TRY(pt_gck_launch(a_dev, ah, aw, t->ndims - 2 * t->nchroma, t->nchroma, t->build.convert_colorspace, t->yuv.p,
+                          t->temp.p, t->adesc.p, s));
+        L.a = a_dev;
+        L.b = t->b0.p;
+    }
+    L.ah = ah;
+    L.aw = aw;
+    L.ann_w = ann_w;
+    L.ann_h = ann_h;
     L.a_desc = t->adesc.p;
-    L.b = t->b0.p;
     L.bh = t->h;
     L.bw = t->w;
     L.table = t->table.p;
     L.q = t->quant();
+    L.nxt = (int)t->extra.size();
+    for (int i = 0; i < L.nxt; i++) {
+        L.xtable[i] = t->extra[i]->table.p;
+        L.xq[i] = t->extra[i]->quant();
+    }
     L.kpos = t->kpos.p;
     L.kdist = t->kdist.p;
-    L.K = std::min(t->K, p.kcoherence);
+    L.K = p.kcoherence;
     L.kcoh_iter0 = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == 0);
+    // k-coherence walks in the rounds when kcoherence_iter asks for them after the first pass, and
+    // without table candidates (the walk then has to follow the propagated matches).
+    L.kcoh_rounds = p.kcoherence > 0 && (p.kcoherence_iter < 0 || (p.kcoherence_iter >= 1 && p.kcoherence_iter < p.prop_iters) ||
+                                         !p.do_table_lookup);
     L.triangle_factor = p.triangle_factor;
     L.spatial = p.spatial;
+    L.rs = !p.spatial && p.do_rs;
     L.do_table_lookup = p.do_table_lookup;
-    L.rounds = p.jf_rounds;
+    L.rounds = p.do_prop ? p.jf_rounds : 0;
+    L.qs = qs;
+    L.vw = vw;
+    L.vh = vh;
+    if (t->allowed.p) {
+        L.allowed = t->allowed.p;
+        L.allowed_index = p.allowed_index;
+    }
+    const double temporal_scale = 1.0 / (1 + p.coherence_temporal);
+    if (prev_dev) {
+        TRY(t->ixy.ensure(n));
+        TRY(t->id.ensure(n));
+        TRY(t->bad.ensure(1));
+        HIPTRY(hipMemsetAsync(t->bad.p, 0, sizeof(int), s));
+        TRY(pt_lookup_init_prev(L, prev_dev, sv_dev, p.recalc_dist_temporal, temporal_scale, p.sanitize_input, t->ixy.p,
+                                t->id.p, t->bad.p, s));
+        L.init_xy = t->ixy.p;
+        L.init_d = t->id.p;
+    }
+    HIPTRY(hipEventRecord(t->ev[7], s));
     NnfField f0, f1;
     f0.xy = t->nxy.p;
     f0.d = t->nd.p;
-    f0.done = t->ndone.p;
-    f0.chg = t->nchg.p;
+    f0.fl = t->nfl.p;
     f1.xy = t->nxy2.p;
     f1.d = t->nd2.p;
-    f1.done = t->ndone2.p;
-    f1.chg = t->nchg2.p;
-    TRY(pt_lookup_launch(L, f0, f1, ann_dev, s, t->ev[8], t->ev[10]));
+    f1.fl = t->nfl2.p;
+    int launches = 0;
+    TRY(pt_lookup_launch(L, f0, f1, t->gxy.p, t->gd.p, ann_dev, prev_dev, t->id.p, temporal_scale,
+                         prev_dev && p.coherence_temporal != 0.0, s, t->ev[8], t->ev[10], &launches));
     HIPTRY(hipEventRecord(t->ev[9], s));
     HIPTRY(hipStreamSynchronize(s));
     double T1 = now_s();
+    if (prev_dev && !p.sanitize_input) {
+        int bad = 0;
+        HIPTRY(hipMemcpy(&bad, t->bad.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (bad) return pt_fail(PT_E_INVALID, "expected xsrc, ysrc in bounds in prev_nnf mode (sanitize_input 0)");
+    }
     if (seconds) *seconds = T1 - T0;
     float ms = 0;
     HIPTRY(hipEventElapsedTime(&ms, t->ev[6], t->ev[7]));
@@ -502,9 +707,17 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     t->stage_ms[7] = ms;
     HIPTRY(hipEventElapsedTime(&ms, t->ev[8], t->ev[10]));
     t->stage_ms[8] = ms;
-    t->stage_ms[10] = L.rounds;
+    t->stage_ms[10] = launches;
     HIPTRY(hipEventElapsedTime(&ms, t->ev[6], t->ev[9]));
     t->stage_ms[9] = ms;
+    return PT_OK;
+}
+
+// Shape of the NNF of a query of h x w x c (reference: (h-7) x (w-7), or h x w in descriptor mode).
+int ann_size(const pt_table *t, int h, int w, int c, long &n) {
+    int ah = 0, aw = 0;
+    TRY(query_shape(t, h, w, c, ah, aw));
+    n = (long)(ah - 7) * (aw - 7);
     return PT_OK;
 }
 
@@ -523,13 +736,21 @@ extern "C" int pt_table_create_dev(pt_params *p, const float *b_dev, int h, int 
 extern "C" int pt_table_lookup(pt_table *t, const float *a, int h, int w, int c, double *ann_out,
                                const double *ann_prev, const float *sv, double *seconds) {
     if (!t || !a || !ann_out) return pt_fail(PT_E_INVALID, "pt_table_lookup: null argument");
-    if (ann_prev || sv) return pt_fail(PT_E_UNSUPPORTED, "ann_prev / coherence_temporal_sv are not supported yet");
-    if (c != 3 || h < 8 || w < 8) return pt_fail(PT_E_INVALID, "expected a 3 channel query image of at least 8x8");
-    TRY(t->a.ensure((size_t)h * w * 3));
-    const long n = (long)(h - 7) * (w - 7);
+    long n = 0;
+    TRY(ann_size(t, h, w, c, n));
+    TRY(t->a.ensure((size_t)h * w * c));
     TRY(t->ann.ensure((size_t)n * 3));
-    HIPTRY(hipMemcpyAsync(t->a.p, a, (size_t)h * w * 3 * 4, hipMemcpyHostToDevice, t->stream));
-    TRY(lookup_common(t, t->a.p, h, w, c, t->ann.p, seconds));
+    HIPTRY(hipMemcpyAsync(t->a.p, a, (size_t)h * w * c * 4, hipMemcpyHostToDevice, t->stream));
+    if (ann_prev) {
+        TRY(t->prevbuf.ensure((size_t)n * 3));
+        HIPTRY(hipMemcpyAsync(t->prevbuf.p, ann_prev, (size_t)n * 3 * 8, hipMemcpyHostToDevice, t->stream));
+    }
+    if (sv) {
+        TRY(t->svbuf.ensure((size_t)n));
+        HIPTRY(hipMemcpyAsync(t->svbuf.p, sv, (size_t)n * 4, hipMemcpyHostToDevice, t->stream));
+    }
+    TRY(lookup_common(t, t->a.p, h, w, c, t->ann.p, ann_prev ? t->prevbuf.p : nullptr, sv ? t->svbuf.p : nullptr,
+                      seconds));
     HIPTRY(hipMemcpyAsync(ann_out, t->ann.p, (size_t)n * 3 * 8, hipMemcpyDeviceToHost, t->stream));
     HIPTRY(hipStreamSynchronize(t->stream));
     return PT_OK;
@@ -538,8 +759,7 @@ extern "C" int pt_table_lookup(pt_table *t, const float *a, int h, int w, int c,
 extern "C" int pt_table_lookup_dev(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_out_dev,
                                    const double *ann_prev_dev, const float *sv_dev, double *seconds) {
     if (!t || !a_dev || !ann_out_dev) return pt_fail(PT_E_INVALID, "pt_table_lookup_dev: null argument");
-    if (ann_prev_dev || sv_dev) return pt_fail(PT_E_UNSUPPORTED, "ann_prev / coherence_temporal_sv are not supported yet");
-    return lookup_common(t, a_dev, h, w, c, ann_out_dev, seconds);
+    return lookup_common(t, a_dev, h, w, c, ann_out_dev, ann_prev_dev, sv_dev, seconds);
 }
 
 extern "C" int pt_table_destroy(pt_table *t) {
@@ -651,7 +871,8 @@ extern "C" int pt_table_save(pt_table *t, const char *filename) {
     std::vector<float> b0((size_t)t->h * t->w * 3), wh0((size_t)Hp * Wp * t->ndims);
     HIPTRY(hipStreamSynchronize(t->stream));
     HIPTRY(hipMemcpy(cells.data(), t->table.p, cells.size() * 4, hipMemcpyDeviceToHost));
-    HIPTRY(hipMemcpy(b0.data(), t->b0.p, b0.size() * 4, hipMemcpyDeviceToHost));
+    if (!t->desc_mode)   // descriptor mode keeps no RGB copy; the reference's b0 is unset there
+        HIPTRY(hipMemcpy(b0.data(), t->b0.p, b0.size() * 4, hipMemcpyDeviceToHost));
     TRY(planes_to_hwc(t->desc.p, Hp, Wp, t->ndims, wh0.data()));
     FILE *f = fopen(filename, "wb");
     if (!f) return pt_failf(PT_E_IO, "PatchTable: could not save to %s", filename);
@@ -752,13 +973,18 @@ int load_file(pt_table *t, const char *filename) {
     const size_t plane = (size_t)(t->h + 7) * (t->w + 7);
     std::vector<float> planes;
     hwc_to_planes(wh0.data(), plane, t->ndims, planes);
-    TRY(t->b0.ensure(b0.size()));
     TRY(t->desc.ensure(planes.size()));
     TRY(t->table.ensure(cells.size()));
-    HIPTRY(hipMemcpyAsync(t->b0.p, b0.data(), b0.size() * 4, hipMemcpyHostToDevice, s));
+    if (t->desc_mode) {   // descriptor mode: distances use the interleaved descriptor (= wh0)
+        TRY(t->descI.ensure(wh0.size()));
+        HIPTRY(hipMemcpyAsync(t->descI.p, wh0.data(), wh0.size() * 4, hipMemcpyHostToDevice, s));
+    } else {
+        TRY(t->b0.ensure(b0.size()));
+        HIPTRY(hipMemcpyAsync(t->b0.p, b0.data(), b0.size() * 4, hipMemcpyHostToDevice, s));
+    }
     HIPTRY(hipMemcpyAsync(t->desc.p, planes.data(), planes.size() * 4, hipMemcpyHostToDevice, s));
     HIPTRY(hipMemcpyAsync(t->table.p, cells.data(), cells.size() * 4, hipMemcpyHostToDevice, s));
-    TRY(upload_quant(t));
+    TRY(upload_quant(t, s));
     return PT_OK;
 }
 
@@ -770,9 +996,12 @@ extern "C" int pt_table_load(pt_params *p, const char *filename, pt_table **out)
     pt_table *t = new pt_table();
     t->p = p;
     t->build = *p;
+    t->desc_mode = p->is_descriptor != 0;
     int rc = init_stream(t);
     if (rc == PT_OK) rc = load_file(t, filename);
-    if (rc == PT_OK && t->ndims != p->ndims)
+    if (rc == PT_OK && t->desc_mode && !desc_dims_supported(t->ndims))
+        rc = pt_failf(PT_E_UNSUPPORTED, "descriptor mode: %d channels unsupported", t->ndims);
+    if (rc == PT_OK && !t->desc_mode && t->ndims != p->ndims)
         rc = pt_failf(PT_E_INVALID, "%s holds %d-dimensional descriptors, parameters say ndims %d", filename, t->ndims,
                       p->ndims);
     if (rc == PT_OK) {
@@ -788,7 +1017,8 @@ extern "C" int pt_table_load(pt_params *p, const char *filename, pt_table **out)
         rc = t->kpos.ensure(n * t->K);
         if (rc == PT_OK) rc = t->kdist.ensure(n * t->K);
         if (rc == PT_OK)
-            rc = pt_kcoherence_launch(t->desc.p, plane, t->w + 7, t->ndims, t->b0.p, t->w, t->h, t->quant(),
+            rc = pt_kcoherence_launch(t->desc.p, plane, t->w + 7, t->ndims, t->b0.p,
+                                      t->desc_mode ? t->descI.p : nullptr, t->w, t->h, nullptr, 1, t->quant(),
                                       t->nslices.data(), t->K, p->kcoh_window,
                                       p->kcoherence_min_dist * p->kcoherence_min_dist, t->kpos.p, t->kdist.p, t->sc,
                                       t->stream);
@@ -846,7 +1076,8 @@ extern "C" int pt_table_get_cells(const pt_table *t, int32_t *out) {
 
 extern "C" int pt_table_get_populated(const pt_table *t, int32_t *out) {
     if (!t || !out) return pt_fail(PT_E_INVALID, "null argument");
-    const long n = (long)(t->w - 7) * (t->h - 7);
+    if (!t->cells.p) return pt_fail(PT_E_INVALID, "no populate record (a loaded table)");
+    const long n = t->n_pop;
     std::vector<int64_t> cells(n);
     std::vector<uint8_t> win(n);
     std::vector<int32_t> xy(n);
@@ -856,6 +1087,22 @@ extern "C" int pt_table_get_populated(const pt_table *t, int32_t *out) {
     for (int64_t i = 0; i < t->ncells; i++) out[i] = PT_TABLE_UNUSED;
     for (long k = 0; k < n; k++)
         if (win[k]) out[cells[k]] = xy[k];
+    return PT_OK;
+}
+
+extern "C" int pt_table_extra_info(const pt_table *t, int i, int *nslices, int64_t *ncells, int *table_add_count) {
+    if (!t || i < 0 || i >= (int)t->extra.size()) return pt_fail(PT_E_INVALID, "no such further table");
+    const pt_table *E = t->extra[i];
+    if (nslices)
+        for (int d = 0; d < E->G; d++) nslices[d] = E->nslices[d];
+    if (ncells) *ncells = E->ncells;
+    if (table_add_count) *table_add_count = E->table_add_count;
+    return PT_OK;
+}
+
+extern "C" int pt_table_get_extra_cells(const pt_table *t, int i, int32_t *out) {
+    if (!t || !out || i < 0 || i >= (int)t->extra.size()) return pt_fail(PT_E_INVALID, "no such further table");
+    HIPTRY(hipMemcpy(out, t->extra[i]->table.p, (size_t)t->extra[i]->ncells * 4, hipMemcpyDeviceToHost));
     return PT_OK;
 }
 
@@ -890,6 +1137,13 @@ extern "C" int pt_gck_dev(const float *img_dev, int h, int w, int n, int nchroma
 extern "C" int pt_patch_ssd_dev(const float *a_dev, int aw, const float *b_dev, int bw, const int32_t *ax,
                                 const int32_t *ay, const int32_t *bx, const int32_t *by, int npairs, float *out_dev) {
     TRY(pt_ssd_pairs_launch(a_dev, aw, b_dev, bw, ax, ay, bx, by, npairs, out_dev, 0));
+    HIPTRY(hipStreamSynchronize(0));
+    return PT_OK;
+}
+
+extern "C" int pt_desc_dist_dev(const float *a_dev, const float *b_dev, int C, int n, float *out_dev) {
+    if (!a_dev || !b_dev || !out_dev || C < 4 || n < 0) return pt_fail(PT_E_INVALID, "pt_desc_dist_dev: bad arguments");
+    TRY(pt_desc_dist_pairs_launch(a_dev, b_dev, C, n, out_dev, 0));
     HIPTRY(hipStreamSynchronize(0));
     return PT_OK;
 }

@@ -50,6 +50,8 @@ def lib():
             "orc_get_table": (_int, [ctypes.c_void_p, _i32p]),
             "orc_get_populated": (_int, [ctypes.c_void_p, _i32p]),
             "orc_get_kcoherence": (_int, [ctypes.c_void_p, _i32p, _f32p]),
+            "orc_extra_info": (_int, [ctypes.c_void_p, _int, _P(_int), _P(ctypes.c_int64), _P(_int)]),
+            "orc_get_extra_table": (_int, [ctypes.c_void_p, _int, _i32p]),
             "orc_save": (_int, [ctypes.c_void_p, ctypes.c_char_p]),
             "orc_dt": (_int, [_int, _P(_int), _i32p, _int, _int]),
             "orc_lookup_raster": (_int, [ctypes.c_void_p, _f32p, _int, _int, _int, _f64p, _f64p, _f32p]),
@@ -116,6 +118,16 @@ class OracleTable:
     def set_params(self, params):
         self.params = params
         lib().orc_set_params(self.handle, ctypes.byref(params))
+
+    def extra(self, i):
+        """(nslices, table_add_count, cells) of further table i."""
+        ns = (ctypes.c_int * 64)()
+        nc = ctypes.c_int64()
+        add = ctypes.c_int()
+        assert lib().orc_extra_info(self.handle, i, ns, ctypes.byref(nc), ctypes.byref(add)) == 0
+        cells = np.empty(nc.value, np.int32)
+        lib().orc_get_extra_table(self.handle, i, i32(cells))
+        return [ns[k] for k in range(self.G)], add.value, cells
 
     def save(self, path):
         assert lib().orc_save(self.handle, str(path).encode()) == 0

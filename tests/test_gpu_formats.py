@@ -74,7 +74,7 @@ def test_save_switch_writes_file(pair_cfg1, tmp_path):
     assert np.array_equal(parse_table_file(path)["table"].reshape(-1), t.cells())
 
 
-@pytest.mark.parametrize("speed", [1])
+@pytest.mark.parametrize("speed", range(10))
 def test_vidpair0_mean_dist_vs_published(speed):
     from patchtable_amd import PatchTable, PatchTableParams, load_color_image
     a = load_color_image(os.path.join(GOLD, "a.png"))

@@ -99,8 +99,16 @@ def test_create_rejects_bad_input_before_gpu_work(lib):
     from patchtable_amd import PatchTable, PatchTableError
     b = np.zeros((20, 20, 3), np.float32)
     p = P()
+    p.is_descriptor = 1   # descriptor mode needs at least 4 channels (patch_dist_approx reads 4)
+    with pytest.raises(PatchTableError, match="3 channels unsupported"):
+        PatchTable(p, b)
+    p = P()
     p.is_descriptor = 1
-    with pytest.raises(PatchTableError, match="is_descriptor"):
+    with pytest.raises(PatchTableError, match="allows no patch"):
+        PatchTable(p, np.zeros((20, 20, 8), np.float32), allowed_patches=np.zeros((20, 20), np.int32))
+    p = P()
+    p.ntables = 40
+    with pytest.raises(PatchTableError, match="ntables"):
         PatchTable(p, b)
     p = P()
     with pytest.raises(PatchTableError, match="at least 8x8"):
