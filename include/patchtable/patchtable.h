@@ -144,9 +144,22 @@ class PatchTable {
     PatchTableParams *p = nullptr;
 
     PatchTable(PatchTableParams *p_, const Array<in_type> &b, Array<itype> *allowed_patches = NULL) : p(p_) {
-        const int32_t *allowed = allowed_patches ? (const int32_t *)allowed_patches->data : nullptr;
-        patchtable_amd_detail::check(
-            pt_table_create(p, (const float *)b.data, b.height(), b.width(), b.channels(), allowed, &t));
+        /* allowed_patches may be larger than the patch grid (the reference asserts >=,
+         * patchtable.h:2643-2646); its top-left patch-grid region is what is read. */
+        std::vector<int32_t> mask;
+        if (allowed_patches) {
+            const int pad = p->is_descriptor ? (p->descriptor_padded ? 14 : 0) : 7;
+            const int bh = b.height() - pad, bw = b.width() - pad;
+            if (allowed_patches->height() < bh || allowed_patches->width() < bw) {
+                fprintf(stderr, "expected allowed_patches to be at least the patch grid size %dx%d\n", bw, bh);
+                exit(1);
+            }
+            mask.resize((size_t)bh * bw);
+            for (int y = 0; y < bh; y++)
+                for (int x = 0; x < bw; x++) mask[(size_t)y * bw + x] = (int32_t)(*allowed_patches)(y, x);
+        }
+        patchtable_amd_detail::check(pt_table_create(p, (const float *)b.data, b.height(), b.width(), b.channels(),
+                                                     allowed_patches ? mask.data() : nullptr, &t));
     }
     ~PatchTable() {
         if (t) pt_table_destroy(t);
@@ -154,10 +167,12 @@ class PatchTable {
     PatchTable(const PatchTable &) = delete;
     PatchTable &operator=(const PatchTable &) = delete;
 
-    /* Fills ann ((h-7) x (w-7) x 3: NNF_X, NNF_Y, NNF_DIST); returns the lookup time in seconds. */
+    /* Fills ann ((h-7) x (w-7) x 3: NNF_X, NNF_Y, NNF_DIST; h x w in descriptor mode); returns the
+     * lookup time in seconds. */
     double lookup(const Array<in_type> &a, Array<double> &ann, Array<double> *ann_prev = NULL,
                   Array<float> *coherence_temporal_sv = NULL) {
-        ann.resize(a.height() - 7, a.width() - 7, 3);
+        const int pad = p->is_descriptor ? (p->descriptor_padded ? 14 : 0) : 7;
+        ann.resize(a.height() - pad, a.width() - pad, 3);
         double secs = 0;
         patchtable_amd_detail::check(pt_table_lookup(t, (const float *)a.data, a.height(), a.width(), a.channels(),
                                                      ann.data, ann_prev ? ann_prev->data : nullptr,

@@ -3,6 +3,7 @@
 //
 //   patchtable match a.png b.png out.pfm [switches] [-prev_nnf prev.pfm] [-allowed_patches mask.pfm]
 //   patchtable check_dist a.png b.png nnf.pfm [nnf_dist_recalc.pfm] [-check 1]
+//   patchtable test_descriptor
 //
 // Output lines and exit codes follow the reference: match prints
 // "total_time: T, query_time: Q, mean_dist: M, mean_dist_recomputed: R" (or the verbose report),
@@ -12,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <map>
+#include <random>
 #include <string>
 
 namespace {
@@ -92,6 +94,7 @@ double recompute_mean_dist(bool is_descriptor, const Array<float> &a, const Arra
 void usage() {
     printf("patchtable match a.png b.png out.pfm [-prev_nnf prev.pfm] [-allowed_patches mask.pfm] [-is_descriptor 0|1]\n");
     printf("patchtable check_dist a.png b.png nnf.pfm [nnf_dist_recalc.pfm] -- Prints average distance for a -> b nnf\n");
+    printf("patchtable test_descriptor\n");
     printf("\n");
     printf("Switches are those of PatchTableParams (-speed 0..9, -limit, -kcoherence, ...).\n");
     printf("\n");
@@ -166,6 +169,62 @@ void check_dist(const char *a_filename, const char *b_filename, const char *nnf_
     if (overwrite) save_color_image(ann, argv[4]);
 }
 
+// test_descriptor (patchtable_main.cpp:174-240): descriptor mode on uniform random 8-channel
+// descriptors with default parameters -- all 8 dims as the grid, then 6 of 8, then 6 of 6 -- must
+// give mean distances (over the full 8 dims) that grow in that order, the first below 0.77.
+void test_descriptor(int w = 400, int h = 300, int k = 8) {
+    Array<float> a0(h, w, k), b0(h, w, k);
+    std::mt19937 rng(1);
+    std::uniform_real_distribution<float> U(0.f, 1.f);
+    for (int i = 0; i < a0.nelems; i++) a0.data[i] = U(rng);
+    for (int i = 0; i < b0.nelems; i++) b0.data[i] = U(rng);
+    Array<float> a(a0), b(b0);
+    PatchTableParams *p = new PatchTableParams();
+    p->is_descriptor = true;
+    double dL[3];
+    for (int reduce = 0; reduce < 3; reduce++) {
+        if (reduce == 1) p->grid_ndims = k - 2;
+        else if (reduce == 2) {
+            p->grid_ndims = p->ndims = k - 2;
+            Array<float> ap(h, w, k - 2), bp(h, w, k - 2);
+            for (int y = 0; y < h; y++)
+                for (int x = 0; x < w; x++)
+                    for (int c = 0; c < k - 2; c++) {
+                        ap(y, x, c) = a0(y, x, c);
+                        bp(y, x, c) = b0(y, x, c);
+                    }
+            a.assign(ap);
+            b.assign(bp);
+        }
+        printf("test_descriptor: reduce=%d, ndims=%d, b channels: %d\n", reduce, p->grid_ndims, b.channels());
+        PatchTable<> btable(p, b);
+        Array<double> ann;
+        btable.lookup(a, ann);
+        if (ann.width() != a.width() || ann.height() != a.height()) {
+            fprintf(stderr, "expected ann size %dx%d to match a size %dx%d\n", ann.width(), ann.height(), a.width(),
+                    a.height());
+            exit(1);
+        }
+        for (int y = 0; y < ann.height(); y++)
+            for (int x = 0; x < ann.width(); x++) {
+                int bx = (int)ann(y, x, NNF_X), by = (int)ann(y, x, NNF_Y);
+                if ((unsigned)bx >= (unsigned)b.width() || (unsigned)by >= (unsigned)b.height()) {
+                    fprintf(stderr, "expected ax, ay=%d, %d => bx, by=%d, %d to be in bounds %dx%d\n", x, y, bx, by,
+                            b.width(), b.height());
+                    exit(1);
+                }
+            }
+        dL[reduce] = recompute_mean_dist(true, a0, b0, ann, p->patch_w, false, false);
+        printf("mean_dist: %f (reduce=%d)\n", dL[reduce], reduce);
+    }
+    if (dL[0] < dL[1] && dL[1] < dL[2] && dL[0] < 0.77) {
+        printf("test_descriptor: OK\n");
+    } else {
+        fprintf(stderr, "test_descriptor: Failed, distances not as expected\n");
+        exit(1);
+    }
+}
+
 }  // namespace
 
 int main(int argc0, char **argv0) {
@@ -178,6 +237,8 @@ int main(int argc0, char **argv0) {
             exit(1);
         }
         test_match(argv[1], argv[2], argv[3], argc, argv);
+    } else if (!strcmp(argv[0], "test_descriptor")) {
+        test_descriptor();
     } else if (!strcmp(argv[0], "check_dist")) {
         if (argc <= 3) {
             fprintf(stderr, "expected 3 args in check_dist mode\n");

@@ -108,3 +108,12 @@ def test_match_cli_on_vidpair0(tmp_path):
                         capture_output=True, text=True, timeout=120)
     assert r2.returncode == 0, r2.stderr
     assert abs(float(r2.stdout.strip()) - float(fields["mean_dist_recomputed"])) < 1e-5
+
+
+def test_cli_test_descriptor():
+    """`patchtable test_descriptor` (patchtable_main.cpp:174-240, 365-366) passes."""
+    exe = os.path.join(ROOT, "patchtable_amd", "patchtable")
+    r = subprocess.run([exe, "test_descriptor"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr
+    assert "test_descriptor: OK" in r.stdout
