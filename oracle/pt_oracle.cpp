@@ -1232,12 +1232,13 @@ static void lookup_raster(const Table &T, const float *a_in, int h_in, int w_in,
 //     previous distance scaled, patchtable.h:3539-3585) or from nothing;
 //   candidates (lattice patches, in parallel): the cell of every table (patchtable.h:3664-3737;
 //     with no previous field the first table's candidate is taken unconditionally), then the
-//     k-coherence walk of the best with the triangle test (patchtable.h:3779-3784), then the
-//     spatial window (allowed positions only) or one random neighbour (do_rs without spatial);
+//     k-coherence walk of the best with the triangle test (patchtable.h:3779-3784), then one
+//     random neighbour (do_rs without spatial);
 //   rounds 1..R (lattice, double-buffered): for the 8 neighbours n = v + J*(dx,dy) of the
 //     lattice the candidate NNF(n) - J*query_step*(dx,dy) if inside b and allowed; then the
 //     k-coherence walk again when kcoherence_iter asks for it after pass 0 or when there are no
-//     table candidates (do_table_lookup 0); then the spatial
+//     table candidates (do_table_lookup 0); then, from the second round on (in the candidate pass
+//     when there are no rounds, in the first round when there is one), the spatial
 //     window or a random neighbour. J runs 8, 4, 2, 1, then unit jumps;
 //   gather (query_step > 1): every patch takes the best of the offsets of the lattice patches
 //     within query_step of it (clamped into b, allowed only), starting from its initial match;
@@ -1313,6 +1314,11 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
     const bool kcohR = p.kcoherence > 0 && (p.kcoherence_iter < 0 || (p.kcoherence_iter >= 1 && p.kcoherence_iter < p.prop_iters) ||
                                             !p.do_table_lookup);
     const bool rs = !p.spatial && p.do_rs;
+    // The spatial window runs from the second round on (in the candidate pass only when there
+    // are no rounds, in the first round when there is one).
+    const int nrounds = p.do_prop ? p.jf_rounds : 0;
+    const bool spatial_cand = nrounds == 0;
+    const int spatial_from = nrounds >= 2 ? 2 : 1;   // 1-based round index
     // the k-coherence walk of the best's list; returns true if it improved the best
     auto kwalk = [&](int x, int y, int &xb, int &yb, float &db, int slot) -> bool {
         const int32_t *kp = &T.kpos[((size_t)yb * bew + xb) * T.K];
@@ -1332,6 +1338,11 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
         }
         return changed;
     };
+    // candidates evaluated for the current patch in the current phase: evaluating one again
+    // cannot change the result (rejected against a best that has only decreased since, or the
+    // best itself), so the model skips repeats like the kernels do
+    std::vector<int32_t> seen;
+    auto is_seen = [&](int xs, int ys) { return std::find(seen.begin(), seen.end(), XY_TO_INT(xs, ys)) != seen.end(); };
     auto spatial = [&](int x, int y, int &xb, int &yb, float &db, int slot) -> bool {
         bool changed = false;
         int symin = yb - p.spatial, symax = yb + p.spatial + 1;
@@ -1345,7 +1356,7 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
         if (sxmax > bew) sxmax = bew;
         for (int ys = symin; ys < symax; ys++)
             for (int xs = sxmin; xs < sxmax; xs++)
-                if ((xs != xb0 || ys != yb0) && allowed_at(xs, ys)) {
+                if ((xs != xb0 || ys != yb0) && allowed_at(xs, ys) && !is_seen(xs, ys)) {
                     float d = dist(x, y, xs, ys);
                     g_ssd_count[slot]++;
                     if (d < db) {
@@ -1415,7 +1426,8 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
             bool kd = false;
             if (kcoh0) kd = !kwalk(x, y, xb, yb, db, 0);
             bool done = false;
-            if (p.spatial) {
+            seen.clear();
+            if (p.spatial && spatial_cand) {
                 const bool changed = spatial(x, y, xb, yb, db, 1);
                 done = !changed;
                 if (changed) kd = false;
@@ -1441,6 +1453,7 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
                 int xb = cx[v], yb = cy[v];
                 float db = cd[v];
                 bool moved = false;
+                seen.clear();
                 for (int t = 0; t < 8; t++) {
                     const int qx = vx + j * DX[t], qy = vy + j * DY[t];
                     if (qx < 0 || qy < 0 || qx >= vw || qy >= vh) continue;
@@ -1450,6 +1463,8 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
                     if (xs < 0 || ys < 0 || xs >= bew || ys >= beh) continue;
                     if (xs == xb && ys == yb) continue;
                     if (!allowed_at(xs, ys)) continue;
+                    if (is_seen(xs, ys)) continue;
+                    seen.push_back(XY_TO_INT(xs, ys));
                     float d = dist(x, y, xs, ys);
                     g_ssd_count[2 + 2 * r]++;
                     if (d < db) {
@@ -1467,7 +1482,7 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
                 }
                 bool done = cdone[v] && !moved && !kchg;
                 bool schg = false;
-                if (p.spatial && !done) {
+                if (p.spatial && !done && (int)r + 1 >= spatial_from) {
                     schg = spatial(x, y, xb, yb, db, 3 + 2 * (int)r);
                     done = !schg;
                 } else if (rs) {

@@ -9,13 +9,15 @@
 //   init: nothing, or the previous field (clamped; distance recomputed and scaled by the temporal
 //     weight, or the previous distance scaled, patchtable.h:3539-3585);
 //   candidates: the cell of every table (patchtable.h:3664-3737), the k-coherence list of the
-//     best with the triangle early-out (patchtable.h:3748-3851), then the spatial window around
-//     the best (patchtable.h:3853-3885, allowed positions only) or one random neighbour (do_rs
+//     best with the triangle early-out (patchtable.h:3748-3851), and one random neighbour (do_rs
 //     without spatial, patchtable.h:3886-3898);
 //   rounds: jump flooding -- the 8 lattice neighbours at distance J (J = 8, 4, 2, 1, then 1 ...)
 //     each propose their offset; the k-coherence walk again when kcoherence_iter asks for it after
-//     the first pass or when there are no table candidates; then the spatial window or a random
-//     neighbour;
+//     the first pass or when there are no table candidates; then, from the second round on, the
+//     spatial window around the best (patchtable.h:3853-3885, allowed positions only), or a random
+//     neighbour. (The window runs in the candidate pass only when there are no rounds, and in the
+//     first round only when there is one round: searching it around the early, still-moving
+//     matches costs more than it gains);
 //   gather (query_step > 1): every patch takes the best of the offsets of the lattice patches
 //     within query_step of it, as the reference's propagation scatter does;
 //   output: NNF_X, NNF_Y, NNF_DIST as doubles, with the temporal rescaling of patchtable.h:4178-4193.
@@ -95,6 +97,26 @@ __device__ __forceinline__ bool allowed_at(const LookupArgs &L, int xs, int ys) 
     return !L.allowed || L.allowed[(size_t)ys * (L.bw - 7) + xs] == L.allowed_index;
 }
 
+// Candidates already evaluated for this patch in the current phase. Evaluating one again cannot
+// change the result: it was rejected against a best distance that has only decreased since, or it
+// is the best itself. So repeats are skipped (exactly result-preserving).
+struct Seen {
+    int c[8];
+    int n = 0;
+    __device__ __forceinline__ bool has(int v) const {
+        bool f = false;
+#pragma unroll
+        for (int i = 0; i < 8; i++) f |= i < n && c[i] == v;
+        return f;
+    }
+    __device__ __forceinline__ void add(int v) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (i == n) c[i] = v;
+        n++;
+    }
+};
+
 // ---- Block work queue -------------------------------------------------------------------------
 // A lane owns one query patch. Instead of each lane looping over its own candidates (a wave then
 // waits for its slowest lane), the lanes of a block push their candidate distances to a queue in
@@ -167,12 +189,13 @@ __device__ __forceinline__ Window spatial_window(const LookupArgs &L, int xb, in
 
 // Sequential form, for spatial radius > 1 (more than SLOTS positions). Returns true if improved.
 template <int M>
-__device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const float4 *qbuf, int &xb, int &yb, float &db) {
+__device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const float4 *qbuf, int &xb, int &yb, float &db,
+                                                   const Seen &seen) {
     const Window w = spatial_window(L, xb, yb);
     bool changed = false;
     for (int ys = w.ymin; ys < w.ymax; ys++)
         for (int xs = w.xmin; xs < w.xmax; xs++) {
-            if ((xs == w.xb0 && ys == w.yb0) || !allowed_at(L, xs, ys)) continue;
+            if ((xs == w.xb0 && ys == w.yb0) || !allowed_at(L, xs, ys) || seen.has(PT_XY_TO_INT(xs, ys))) continue;
             float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, db);
             if (d < db) {
                 db = d;
@@ -188,7 +211,7 @@ __device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const fl
 // lanes that skip it; every lane of the block must call this. Returns true if improved.
 template <int M>
 __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L, bool want,
-                                                 int &xb, int &yb, float &db) {
+                                                 int &xb, int &yb, float &db, const Seen &seen) {
     Window w = spatial_window(L, xb, yb);
     if (!want) w.ymax = w.ymin;   // empty
     if (want) Q.bound[threadIdx.x] = db;
@@ -199,7 +222,8 @@ __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qb
             if (dx == 0 && dy == 0) continue;
             // slot order = row-by-row order of the window positions other than the centre
             const int ys = w.yb0 + dy, xs = w.xb0 + dx;
-            const bool in = ys >= w.ymin && ys < w.ymax && xs >= w.xmin && xs < w.xmax && allowed_at(L, xs, ys);
+            const bool in = ys >= w.ymin && ys < w.ymax && xs >= w.xmin && xs < w.xmax && allowed_at(L, xs, ys) &&
+                            !seen.has(PT_XY_TO_INT(xs, ys));
             if (in) pushed |= 1u << slot;
             q_push(Q, in, slot, PT_XY_TO_INT(xs, ys));
             slot++;
@@ -350,12 +374,13 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
     bool kd = false;
     if (L.K > 0 && L.kcoh_iter0) kd = !kcoh_walk_q<M>(Q, qbuf, L, valid, xb, yb, db);
     bool done = false;
-    if (L.spatial == 1) {
-        const bool changed = spatial_search_q<M>(Q, qbuf, L, valid, xb, yb, db);
+    const Seen none{};
+    if (L.spatial_cand && L.spatial == 1) {
+        const bool changed = spatial_search_q<M>(Q, qbuf, L, valid, xb, yb, db, none);
         done = !changed;
         if (changed) kd = false;
-    } else if (L.spatial > 1 && valid) {
-        const bool changed = spatial_search_seq<M>(L, qbuf, xb, yb, db);
+    } else if (L.spatial_cand && L.spatial > 1 && valid) {
+        const bool changed = spatial_search_seq<M>(L, qbuf, xb, yb, db, none);
         done = !changed;
         if (changed) kd = false;
     } else if (L.rs && valid) {
@@ -399,6 +424,7 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
         Q.bound[threadIdx.x] = db;
     }
     unsigned pushed = 0;
+    Seen seen;
     for (int t = 0; t < 8; t++) {
         const int qx = vx + J * c_dx[t], qy = vy + J * c_dy[t];
         bool pred = valid && qx >= 0 && qy >= 0 && qx < L.vw && qy < L.vh;
@@ -410,10 +436,14 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
                 const int vq = cur.xy[qv];
                 xs = PT_INT_TO_X(vq) - js * c_dx[t];
                 ys = PT_INT_TO_Y(vq) - js * c_dy[t];
-                pred = xs >= 0 && ys >= 0 && xs < bew && ys < beh && !(xs == xb && ys == yb) && allowed_at(L, xs, ys);
+                pred = xs >= 0 && ys >= 0 && xs < bew && ys < beh && !(xs == xb && ys == yb) && allowed_at(L, xs, ys) &&
+                       !seen.has(PT_XY_TO_INT(xs, ys));
             }
         }
-        if (pred) pushed |= 1u << t;
+        if (pred) {
+            pushed |= 1u << t;
+            seen.add(PT_XY_TO_INT(xs, ys));
+        }
         q_push(Q, pred, t, PT_XY_TO_INT(xs, ys));
     }
     q_run<M>(Q, qbuf, L);
@@ -438,11 +468,12 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
     }
     bool done = was_done && !moved && !kchg;
     bool schg = false;
-    if (L.spatial == 1) {
-        schg = spatial_search_q<M>(Q, qbuf, L, valid && !done, xb, yb, db);
+    const bool sp = round >= L.spatial_from;   // rounds that run the spatial window
+    if (sp && L.spatial == 1) {
+        schg = spatial_search_q<M>(Q, qbuf, L, valid && !done, xb, yb, db, seen);
         if (valid && !done) done = !schg;
-    } else if (L.spatial > 1 && valid && !done) {
-        schg = spatial_search_seq<M>(L, qbuf, xb, yb, db);
+    } else if (sp && L.spatial > 1 && valid && !done) {
+        schg = spatial_search_seq<M>(L, qbuf, xb, yb, db, seen);
         done = !schg;
     } else if (L.rs && valid) {
         schg = random_search<M>(L, qbuf, (long)y * L.ann_w + x, round, xb, yb, db);

@@ -110,6 +110,8 @@ struct LookupArgs {
     int kcoh_rounds = 0;            // ... and again in the propagation rounds
     double triangle_factor = 2.0;
     int spatial = 1;
+    int spatial_cand = 0;           // spatial window in the candidate pass (no rounds)
+    int spatial_from = 2;           // first round (1-based) with the spatial window
     int rs = 0;                     // random search (do_rs without spatial)
     int do_table_lookup = 1;
     int rounds = 4;

@@ -662,6 +662,8 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     L.rs = !p.spatial && p.do_rs;
     L.do_table_lookup = p.do_table_lookup;
     L.rounds = p.do_prop ? p.jf_rounds : 0;
+    L.spatial_cand = L.rounds == 0;
+    L.spatial_from = L.rounds >= 2 ? 2 : 1;
     L.qs = qs;
     L.vw = vw;
     L.vh = vh;
