@@ -44,7 +44,8 @@ __device__ __forceinline__ float4 ldg4(const float *p) {
 // Exact SSD of the staged query patch at `aq` (row pitch AX pixels) against the database patch
 // at `b` (row pitch `bpitch` floats), lane order of patch_dist_exact (patchtable.h:330-373):
 // element e of a 24-float row goes to lane e & 3, lanes accumulate in increasing e then row.
-// With `bounded`, the sum is abandoned once a partial sum (after rows 2, 4, 6) reaches `bound`.
+// With `bounded`, the sum is abandoned once a partial sum (after any row but the last) reaches
+// `bound`.
 // Rows are processed one at a time (83 VGPRs, 5 waves per SIMD).
 template <bool bounded>
 __device__ __forceinline__ float ssd_staged(const float4 *aq, const float *__restrict__ b, int bpitch, float bound) {
@@ -69,7 +70,7 @@ __device__ __forceinline__ float ssd_staged(const float4 *aq, const float *__res
         PT_ACC(12, b3.x) PT_ACC(13, b3.y) PT_ACC(14, b3.z) PT_ACC(15, b3.w)
         PT_ACC(16, b4.x) PT_ACC(17, b4.y) PT_ACC(18, b4.z) PT_ACC(19, b4.w)
         PT_ACC(20, b5.x) PT_ACC(21, b5.y) PT_ACC(22, b5.z) PT_ACC(23, b5.w)
-        if (bounded && (dy & 1) && dy < 7) {
+        if (bounded && dy < 7) {
             float part = ((acc[0] + acc[1]) + acc[2]) + acc[3];
             if (part >= bound) return part;
         }
