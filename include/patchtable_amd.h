@@ -226,6 +226,25 @@ int pt_image_size(const char *path, int *h, int *w, int *c);
 int pt_image_load(const char *path, float *out, int h, int w, int c);
 int pt_pfm_save(const char *path, const double *data, int h, int w, int c);
 
+/* ---- One query against many databases ------------------------------------------------------
+ * The reference's one-vs-many benchmark runs one match per (query, database) pair
+ * (patchtable/bench.py:80-83); this batch keeps one table per database image in HBM.
+ * pt_batch_create: n RGB databases (device pointers, h[i] x w[i] x 3), built concurrently by
+ *   `threads` host threads (0: 4), each table on its own stream; p is shared and borrowed.
+ * pt_batch_lookup: the query (device, h x w x 3) against every table; ann_dev[i] receives the
+ *   (h-7) x (w-7) x 3 NNF for database i (device). The query descriptor is computed once; each
+ *   table's lookup runs on its own stream, replayed from a captured HIP graph after the first
+ *   call with the same buffers. ms_per_pair[i] (optional): device time of pair i from events on
+ *   its stream (pairs overlap); seconds (optional): wall time of the whole call. */
+typedef struct pt_batch pt_batch;
+int pt_batch_create(pt_params *p, int n, const float *const *b_dev, const int *h, const int *w, int threads,
+                    pt_batch **out);
+int pt_batch_lookup(pt_batch *bt, const float *a_dev, int h, int w, double *const *ann_dev, double *ms_per_pair,
+                    double *seconds);
+int pt_batch_size(const pt_batch *bt);
+pt_table *pt_batch_table(pt_batch *bt, int i);
+int pt_batch_destroy(pt_batch *bt);
+
 /* ---- Errors ------------------------------------------------------------------------------ */
 const char *pt_last_error(void);
 

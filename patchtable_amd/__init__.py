@@ -20,8 +20,8 @@ NNF_X = 0
 NNF_Y = 1
 NNF_DIST = 2
 
-__all__ = ["PatchTableParams", "PatchTable", "PatchTableError", "NNF_X", "NNF_Y", "NNF_DIST",
-           "synth_pair", "load_color_image", "save_color_image"]
+__all__ = ["PatchTableParams", "PatchTable", "PatchTableBatch", "PatchTableError", "NNF_X", "NNF_Y",
+           "NNF_DIST", "synth_pair", "load_color_image", "save_color_image"]
 
 
 class PatchTableParams:
@@ -245,6 +245,45 @@ class PatchTable:
                  "query_gck", "query_candidates", "query_propagation", "query_total",
                  "propagation_launches"]
         return dict(zip(names, list(out)))
+
+
+class PatchTableBatch:
+    """One query against many databases (torch CUDA tensors, HWC float32): one table per
+    database, built concurrently; lookups share the query descriptor and replay HIP graphs."""
+
+    def __init__(self, p, databases, threads=4):
+        self.p = p
+        self._dbs = [d.contiguous() for d in databases]
+        n = len(self._dbs)
+        ptrs = (ctypes.c_void_p * n)(*[d.data_ptr() for d in self._dbs])
+        hs = (ctypes.c_int * n)(*[int(d.shape[0]) for d in self._dbs])
+        ws = (ctypes.c_int * n)(*[int(d.shape[1]) for d in self._dbs])
+        self._bt = ctypes.c_void_p()
+        check(load().pt_batch_create(ctypes.byref(p.raw), n, ptrs, hs, ws, int(threads), ctypes.byref(self._bt)))
+        self.n = n
+
+    def lookup(self, a, anns=None):
+        """Returns (list of n NNFs as torch CUDA float64 tensors, per-pair device ms, wall s)."""
+        import torch
+        h, w = int(a.shape[0]), int(a.shape[1])
+        if anns is None:
+            anns = [torch.empty((h - 7, w - 7, 3), dtype=torch.float64, device=a.device) for _ in range(self.n)]
+        ptrs = (ctypes.c_void_p * self.n)(*[x.data_ptr() for x in anns])
+        ms = (ctypes.c_double * self.n)()
+        secs = ctypes.c_double()
+        check(load().pt_batch_lookup(self._bt, ctypes.c_void_p(a.data_ptr()), h, w, ptrs, ms, ctypes.byref(secs)))
+        return anns, list(ms), secs.value
+
+    def close(self):
+        if getattr(self, "_bt", None) and self._bt.value:
+            load().pt_batch_destroy(self._bt)
+            self._bt = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _is_device_tensor(x):

@@ -144,6 +144,11 @@ SIGNATURES = {
     "pt_desc_dist_dev": (_int, [_vp, _vp, _int, _int, _vp]),
     "pt_synth_pair": (_int, [ctypes.c_uint64, _int, _int, _f32p, _f32p]),
     "pt_table_shape": (_int, [_vp, _P(_int), _P(_int), _P(_int)]),
+    "pt_batch_create": (_int, [_P(PtParams), _int, _P(_vp), _P(_int), _P(_int), _int, _P(_vp)]),
+    "pt_batch_lookup": (_int, [_vp, _vp, _int, _int, _P(_vp), _f64p, _f64p]),
+    "pt_batch_size": (_int, [_vp]),
+    "pt_batch_table": (_vp, [_vp, _int]),
+    "pt_batch_destroy": (_int, [_vp]),
     "pt_image_size": (_int, [ctypes.c_char_p, _P(_int), _P(_int), _P(_int)]),
     "pt_image_load": (_int, [ctypes.c_char_p, _f32p, _int, _int, _int]),
     "pt_pfm_save": (_int, [ctypes.c_char_p, _f64p, _int, _int, _int]),

@@ -197,9 +197,8 @@ void wh_alpha(int level, int k, std::vector<int> &cur, std::vector<int> &sign_le
     }
 }
 
-const std::vector<GckStep> &gck_steps() {
-    static std::vector<GckStep> steps;
-    if (!steps.empty()) return steps;
+std::vector<GckStep> make_gck_steps() {
+    std::vector<GckStep> steps;
     const int w = 8, k = 3;
     std::vector<std::vector<int>> alphas;
     std::vector<int> cur, sign_level(k, 1);
@@ -222,6 +221,12 @@ const std::vector<GckStep> &gck_steps() {
         while (a1[idx] == a2[idx]) idx++;
         steps.push_back({horiz ? 0 : 1, 1 << idx, a2[idx]});
     }
+    return steps;
+}
+
+// Built once (thread-safe static initialisation): tables may be built from several threads.
+const std::vector<GckStep> &gck_steps() {
+    static const std::vector<GckStep> steps = make_gck_steps();
     return steps;
 }
 

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 
@@ -583,8 +584,13 @@ int query_shape(const pt_table *t, int h, int w, int c, int &ah, int &aw) {
     return PT_OK;
 }
 
-int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, const double *prev_dev,
-                  const float *sv_dev, double *seconds) {
+}  // namespace
+
+// Enqueues one lookup on the table's stream. adesc_pre: the query descriptor planes when the
+// caller already computed them (same parameters), else NULL. timing: record the stage events
+// (off while a graph is being captured).
+int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, const double *prev_dev,
+                      const float *sv_dev, const float *adesc_pre, bool timing, int *launches_out) {
     const pt_params &p = *t->p;
     int ah = 0, aw = 0;
     TRY(query_shape(t, h, w, c, ah, aw));
@@ -618,10 +624,12 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     }
     t->qh = ah;
     t->qw = aw;
-    double T0 = now_s();
-    HIPTRY(hipEventRecord(t->ev[6], s));
+    if (timing) HIPTRY(hipEventRecord(t->ev[6], s));
     LookupArgs L;
-    if (t->desc_mode) {
+    if (adesc_pre && !t->desc_mode) {
+        L.a = a_dev;
+        L.b = t->b0.p;
+    } else if (t->desc_mode) {
         TRY(t->adescI.ensure(plane * t->ndims));
         TRY(pt_desc_pad_launch(a_dev, h, w, c, t->build.descriptor_padded, Hp, Wp, t->adesc.p, t->adescI.p, s));
         L.C = t->ndims;
@@ -639,7 +647,7 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     L.aw = aw;
     L.ann_w = ann_w;
     L.ann_h = ann_h;
-    L.a_desc = t->adesc.p;
+    L.a_desc = adesc_pre && !t->desc_mode ? adesc_pre : t->adesc.p;
     L.bh = t->h;
     L.bw = t->w;
     L.table = t->table.p;
@@ -682,7 +690,7 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
         L.init_xy = t->ixy.p;
         L.init_d = t->id.p;
     }
-    HIPTRY(hipEventRecord(t->ev[7], s));
+    if (timing) HIPTRY(hipEventRecord(t->ev[7], s));
     NnfField f0, f1;
     f0.xy = t->nxy.p;
     f0.d = t->nd.p;
@@ -692,8 +700,22 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     f1.fl = t->nfl2.p;
     int launches = 0;
     TRY(pt_lookup_launch(L, f0, f1, t->gxy.p, t->gd.p, ann_dev, prev_dev, t->id.p, temporal_scale,
-                         prev_dev && p.coherence_temporal != 0.0, s, t->ev[8], t->ev[10], &launches));
-    HIPTRY(hipEventRecord(t->ev[9], s));
+                         prev_dev && p.coherence_temporal != 0.0, s, timing ? t->ev[8] : nullptr,
+                         timing ? t->ev[10] : nullptr, &launches));
+    if (timing) HIPTRY(hipEventRecord(t->ev[9], s));
+    if (launches_out) *launches_out = launches;
+    return PT_OK;
+}
+
+namespace {
+
+int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, const double *prev_dev,
+                  const float *sv_dev, double *seconds) {
+    const pt_params &p = *t->p;
+    hipStream_t s = t->stream;
+    double T0 = now_s();
+    int launches = 0;
+    TRY(pt_lookup_enqueue(t, a_dev, h, w, c, ann_dev, prev_dev, sv_dev, nullptr, true, &launches));
     HIPTRY(hipStreamSynchronize(s));
     double T1 = now_s();
     if (prev_dev && !p.sanitize_input) {
@@ -1147,5 +1169,145 @@ extern "C" int pt_desc_dist_dev(const float *a_dev, const float *b_dev, int C, i
     if (!a_dev || !b_dev || !out_dev || C < 4 || n < 0) return pt_fail(PT_E_INVALID, "pt_desc_dist_dev: bad arguments");
     TRY(pt_desc_dist_pairs_launch(a_dev, b_dev, C, n, out_dev, 0));
     HIPTRY(hipStreamSynchronize(0));
+    return PT_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// One query against many databases (the reference's one-vs-many benchmark runs its front end
+// once per pair, bench.py:80-83). The tables are built concurrently from host threads, each on
+// its own stream; a lookup computes the query descriptor once, then every table's lookup runs on
+// its own stream, replayed from a captured HIP graph after the first call with the same buffers.
+// ---------------------------------------------------------------------------------------------
+struct pt_batch {
+    pt_params *p = nullptr;
+    std::vector<pt_table *> tabs;
+    hipStream_t qs = nullptr;     // query-descriptor stream
+    hipEvent_t qdone = nullptr;
+    std::vector<hipEvent_t> e0, e1;
+    std::vector<hipGraphExec_t> gexec;
+    std::vector<std::vector<const void *>> gkey;   // buffers a graph was captured with
+    DevBuf<float> adesc, yuv, temp;
+    ~pt_batch() {
+        for (auto g : gexec)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (auto e : e0)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : e1)
+            if (e) (void)hipEventDestroy(e);
+        if (qdone) (void)hipEventDestroy(qdone);
+        if (qs) (void)hipStreamDestroy(qs);
+        for (pt_table *t : tabs) delete t;
+    }
+};
+
+extern "C" int pt_batch_create(pt_params *p, int n, const float *const *b_dev, const int *h, const int *w,
+                               int threads, pt_batch **out) {
+    if (!p || !out || n < 1 || !b_dev || !h || !w) return pt_fail(PT_E_INVALID, "pt_batch_create: bad arguments");
+    if (p->is_descriptor) return pt_fail(PT_E_UNSUPPORTED, "pt_batch: descriptor mode is not supported");
+    *out = nullptr;
+    pt_batch *bt = new pt_batch();
+    bt->p = p;
+    bt->tabs.assign(n, nullptr);
+    // Parameters are shared; the first create writes grid_ndims back, so do it once up front.
+    if (p->grid_ndims < 0) p->grid_ndims = p->ndims;
+    std::vector<int> rc(n, PT_OK);
+    std::vector<std::string> err(n);
+    const int nt = std::max(1, std::min(threads > 0 ? threads : 4, n));
+    std::vector<std::thread> pool;
+    for (int k = 0; k < nt; k++)
+        pool.emplace_back([&, k]() {
+            for (int i = k; i < n; i += nt) {
+                rc[i] = create_common(p, b_dev[i], true, h[i], w[i], 3, nullptr, &bt->tabs[i]);
+                if (rc[i] != PT_OK) err[i] = pt_last_error();
+            }
+        });
+    for (auto &th : pool) th.join();
+    for (int i = 0; i < n; i++)
+        if (rc[i] != PT_OK) {
+            delete bt;
+            return pt_failf(rc[i], "database %d: %s", i, err[i].c_str());
+        }
+    int r = PT_OK;
+    if (hipStreamCreateWithFlags(&bt->qs, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&bt->qdone) != hipSuccess)
+        r = pt_fail(PT_E_HIP, "pt_batch_create: stream/event creation failed");
+    bt->e0.assign(n, nullptr);
+    bt->e1.assign(n, nullptr);
+    bt->gexec.assign(n, nullptr);
+    bt->gkey.assign(n, {});
+    for (int i = 0; r == PT_OK && i < n; i++)
+        if (hipEventCreate(&bt->e0[i]) != hipSuccess || hipEventCreate(&bt->e1[i]) != hipSuccess)
+            r = pt_fail(PT_E_HIP, "pt_batch_create: event creation failed");
+    if (r != PT_OK) {
+        delete bt;
+        return r;
+    }
+    *out = bt;
+    return PT_OK;
+}
+
+extern "C" int pt_batch_size(const pt_batch *bt) { return bt ? (int)bt->tabs.size() : 0; }
+
+extern "C" pt_table *pt_batch_table(pt_batch *bt, int i) {
+    return bt && i >= 0 && i < (int)bt->tabs.size() ? bt->tabs[i] : nullptr;
+}
+
+extern "C" int pt_batch_lookup(pt_batch *bt, const float *a_dev, int h, int w, double *const *ann_dev,
+                               double *ms_per_pair, double *seconds) {
+    if (!bt || !a_dev || !ann_dev) return pt_fail(PT_E_INVALID, "pt_batch_lookup: null argument");
+    const int n = (int)bt->tabs.size();
+    pt_table *t0 = bt->tabs[0];
+    if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "query image must be at least 8x8");
+    const double T0 = now_s();
+    // the query descriptor once (all tables share the parameters)
+    const size_t plane = (size_t)(h + 7) * (w + 7);
+    TRY(bt->adesc.ensure(plane * t0->ndims));
+    TRY(bt->yuv.ensure((size_t)h * w * 3));
+    TRY(bt->temp.ensure(plane * 3));
+    TRY(pt_gck_launch(a_dev, h, w, t0->ndims - 2 * t0->nchroma, t0->nchroma, t0->build.convert_colorspace, bt->yuv.p,
+                      bt->temp.p, bt->adesc.p, bt->qs));
+    HIPTRY(hipEventRecord(bt->qdone, bt->qs));
+    for (int i = 0; i < n; i++) {
+        pt_table *t = bt->tabs[i];
+        hipStream_t s = t->stream;
+        HIPTRY(hipStreamWaitEvent(s, bt->qdone, 0));
+        HIPTRY(hipEventRecord(bt->e0[i], s));
+        const std::vector<const void *> key = {a_dev, (const void *)(intptr_t)h, (const void *)(intptr_t)w, ann_dev[i],
+                                               bt->adesc.p};
+        if (bt->gexec[i] && bt->gkey[i] == key) {
+            HIPTRY(hipGraphLaunch(bt->gexec[i], s));
+        } else {
+            // eager run (sizes the buffers), then capture the same launch sequence for replays
+            TRY(pt_lookup_enqueue(t, a_dev, h, w, 3, ann_dev[i], nullptr, nullptr, bt->adesc.p, false, nullptr));
+            if (bt->gexec[i]) {
+                HIPTRY(hipGraphExecDestroy(bt->gexec[i]));
+                bt->gexec[i] = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = pt_lookup_enqueue(t, a_dev, h, w, 3, ann_dev[i], nullptr, nullptr, bt->adesc.p, false, nullptr);
+            hipError_t ec = hipStreamEndCapture(s, &g);
+            if (rc != PT_OK) return rc;
+            if (ec != hipSuccess) return pt_failf(PT_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+            hipError_t ei = hipGraphInstantiate(&bt->gexec[i], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return pt_failf(PT_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+            bt->gkey[i] = key;
+        }
+        HIPTRY(hipEventRecord(bt->e1[i], s));
+    }
+    for (int i = 0; i < n; i++) HIPTRY(hipStreamSynchronize(bt->tabs[i]->stream));
+    const double T1 = now_s();
+    if (ms_per_pair)
+        for (int i = 0; i < n; i++) {
+            float ms = 0;
+            HIPTRY(hipEventElapsedTime(&ms, bt->e0[i], bt->e1[i]));
+            ms_per_pair[i] = ms;
+        }
+    if (seconds) *seconds = T1 - T0;
+    return PT_OK;
+}
+
+extern "C" int pt_batch_destroy(pt_batch *bt) {
+    delete bt;
     return PT_OK;
 }
