@@ -245,6 +245,43 @@ int pt_batch_size(const pt_batch *bt);
 pt_table *pt_batch_table(pt_batch *bt, int i);
 int pt_batch_destroy(pt_batch *bt);
 
+/* ---- Learned patch key (no counterpart in the reference; an alternative to the GCK
+ * descriptor as the table index). A 192 -> 64 -> 8 MLP with ReLU maps an 8x8 RGB patch,
+ * x[dy*24 + dx*3 + c] = img(y+dy, x+dx, c) - 0.5, to an 8-dim key. Weights are BF16 for the
+ * forward passes (FP32 masters for training), accumulation FP32. W1 is 64 x 192 row-major,
+ * W2 8 x 64.
+ * pt_key_create: He-uniform weights from `seed`, zero biases.
+ * pt_key_set_weights / pt_key_get_weights: FP32 masters (host arrays); get also returns the
+ *   BF16 copies (bit patterns; NULL to skip any output).
+ * pt_key_forward_dev: key planes of an RGB image (device, h x w x 3) into planes_dev,
+ *   8 planes of (h+7) x (w+7) floats, patch (x, y) at (y+7, x+7); other elements untouched.
+ * pt_key_gradients: gradients of the triplet loss mean(max(0, |ka-kp|^2 - |ka-kn|^2 + margin))
+ *   for nt triplets, pos_dev[3t + r] = XY_TO_INT(x, y) of the anchor (r=0, in a_dev) and the
+ *   positive and negative (r=1, 2, in b_dev); both images h x w x 3, every position inside the
+ *   patch grid (checked). Host outputs (NULL to skip).
+ * pt_key_train: triplets mined from the library's own lookups of npairs (a_i, b_i) device
+ *   image pairs with parameters p (tpp anchors per pair: positive = the match, negative = the
+ *   match moved 2..6 px per axis or a random patch), then `steps` Adam steps of `batch`
+ *   triplets; loss_per_step (optional, host, [steps]) receives each step's loss.
+ * pt_table_create_keyed: a table whose 8 grid dimensions are the key of b (device image,
+ *   h x w x 3), with the parameters p (its k-coherence lists are built in key space; distances
+ *   stay exact RGB SSD) and, as its one further table, the GCK table of b built with p; the
+ *   key network is copied, so `key` may change or go afterwards. Lookups use
+ *   pt_table_lookup(_dev) with an RGB query; such a table has no file format (save fails). */
+typedef struct pt_key pt_key;
+int pt_key_create(uint64_t seed, pt_key **out);
+int pt_key_destroy(pt_key *k);
+int pt_key_set_weights(pt_key *k, const float *w1, const float *b1, const float *w2, const float *b2);
+int pt_key_get_weights(const pt_key *k, float *w1, float *b1, float *w2, float *b2, uint16_t *w1_bf16,
+                       uint16_t *w2_bf16);
+int pt_key_forward_dev(const pt_key *k, const float *img_dev, int h, int w, float *planes_dev);
+int pt_key_gradients(pt_key *k, const float *a_dev, const float *b_dev, int h, int w, const int32_t *pos_dev,
+                     int nt, float margin, float *gW1, float *gb1, float *gW2, float *gb2, float *loss);
+int pt_key_train(pt_key *k, pt_params *p, int npairs, const float *const *a_dev, const float *const *b_dev,
+                 const int *h, const int *w, int tpp, int steps, int batch, float lr, float margin,
+                 uint64_t seed, float *loss_per_step);
+int pt_table_create_keyed(pt_params *p, const pt_key *key, const float *b_dev, int h, int w, pt_table **out);
+
 /* ---- Errors ------------------------------------------------------------------------------ */
 const char *pt_last_error(void);
 

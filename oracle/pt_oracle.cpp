@@ -405,6 +405,7 @@ struct Table {
     ivec kpos;                        // (h-7) x (w-7) x K
     fvec kdist;
     std::vector<Table *> extra;       // the ntables-1 further tables (patchtable.h:3359-3372)
+    bool keyed = false;               // learned-key table: wh0 is the key; extra[0] is the GCK fallback
     Table() {}
     Table(const Table &) = delete;
     ~Table() {
@@ -439,6 +440,10 @@ static void pad_descriptor(const pt_params &p, const float *in, int h_in, int w_
     for (int y = 0; y < h_in; y++)
         std::copy(in + (size_t)y * w_in * C, in + (size_t)(y + 1) * w_in * C, &wh[((size_t)(y + 7) * (w + 7) + 7) * C]);
 }
+
+// Key of the database of a learned-key table, (h+7) x (w+7) x 8 (computed on the GPU and handed
+// in by the caller: the table's construction from the key is what is checked).
+static const float *g_key_hwc = nullptr;
 
 // Descriptor of an RGB image: colour conversion then GCK (get_padded_descriptor,
 // patchtable.h:2492-2546, and reduce_dim, :1687-1728, for dim_algo WH without filter_dims).
@@ -841,7 +846,8 @@ static Table *build(const pt_params &p_in, const float *b, int h_in, int w_in, i
         T->h = h_in;
         T->w = w_in;
         T->b0.assign(b, b + (size_t)h_in * w_in * 3);
-        descriptor(p, b, h_in, w_in, T->wh0);
+        if (g_key_hwc) T->wh0.assign(g_key_hwc, g_key_hwc + (size_t)(h_in + 7) * (w_in + 7) * T->ndims);
+        else descriptor(p, b, h_in, w_in, T->wh0);
     }
     const int h = T->h, w = T->w;
     const int Wp = w + 7, nd = T->ndims, G = T->G;
@@ -956,14 +962,30 @@ struct QuerySide {
     int C = 0;
     const float *a = nullptr;    // RGB query (image mode)
     fvec wh;                     // (ah+7) x (aw+7) x C
+    fvec xwh;                    // learned-key tables: the GCK descriptor for the fallback table
+    int xC = 0;
     int ann_w() const { return aw - 7; }
     int ann_h() const { return ah - 7; }
     const float *qdesc(int x, int y) const { return &wh[((size_t)(y + 7) * (aw + 7) + x + 7) * C]; }
+    // descriptor indexing the further table E (its own for the fallback of a learned-key table)
+    const float *xdesc(int x, int y) const {
+        return xwh.empty() ? qdesc(x, y) : &xwh[((size_t)(y + 7) * (aw + 7) + x + 7) * xC];
+    }
 };
+
+// Query key of a learned-key table, (h+7) x (w+7) x 8 (computed on the GPU, handed in).
+static const float *g_query_key = nullptr;
 
 static void query_side(const Table &T, const float *a, int h_in, int w_in, int c_in, QuerySide &Q) {
     Q.C = T.ndims;
-    if (T.desc) {
+    if (T.keyed) {
+        Q.ah = h_in;
+        Q.aw = w_in;
+        Q.a = a;
+        Q.wh.assign(g_query_key, g_query_key + (size_t)(h_in + 7) * (w_in + 7) * T.ndims);
+        Q.xC = T.extra[0]->ndims;
+        descriptor(T.extra[0]->p, a, h_in, w_in, Q.xwh);
+    } else if (T.desc) {
         pad_descriptor(T.p, a, h_in, w_in, c_in, Q.wh, Q.ah, Q.aw);
     } else {
         Q.ah = h_in;
@@ -1073,7 +1095,7 @@ static void lookup_raster(const Table &T, const float *a_in, int h_in, int w_in,
                     if (xsrc != xbest || ysrc != ybest) consider(xsrc, ysrc);
                     int xprev = xsrc, yprev = ysrc;
                     for (const Table *E : T.extra) {
-                        int v1 = E->table[E->cell_of(whp)];
+                        int v1 = E->table[E->cell_of(Q.xdesc(x, y))];
                         int xs1 = INT_TO_X(v1) - 7, ys1 = INT_TO_Y(v1) - 7;
                         if ((xs1 != xbest || ys1 != ybest) && (xs1 != xprev || ys1 != yprev)) consider(xs1, ys1);
                         xprev = xs1;
@@ -1408,7 +1430,7 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
                 }
                 int xp = xs, yp = ys;
                 for (const Table *E : T.extra) {
-                    int v1 = E->table[E->cell_of(whp)];
+                    int v1 = E->table[E->cell_of(Q.xdesc(x, y))];
                     int x1 = INT_TO_X(v1) - 7, y1 = INT_TO_Y(v1) - 7;
                     if ((x1 != xb || y1 != yb) && (x1 != xp || y1 != yp)) {
                         float d = dist(x, y, x1, y1);
@@ -1560,6 +1582,60 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
     }
 }
 
+// ---------------------------- learned key (MLP 192 -> 64 -> 8) --------------------------------
+// The key of this build's learned-key table: x = patch - 0.5 in bf16 (HWC order), h =
+// ReLU(W1 x + b1), k = W2 h + b2 with W1, W2 rounded to bf16 and biases in float; sums here in
+// double (the GPU accumulates in FP32 on MFMA / VALU, so the two agree to FP32 rounding).
+static inline float bf16_round(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) u = (u | 0x400000u) & 0xffff0000u;
+    else {
+        u += 0x7fffu + ((u >> 16) & 1u);
+        u &= 0xffff0000u;
+    }
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+static const int KEY_IN = 192, KEY_H = 64, KEY_OUT = 8;
+
+static void key_patch(const float *img, int w, int x, int y, float *xv) {
+    for (int dy = 0; dy < 8; dy++)
+        for (int r = 0; r < 24; r++) xv[dy * 24 + r] = bf16_round(img[((size_t)(y + dy) * w + x) * 3 + r] - 0.5f);
+}
+
+// Weights are rounded to BF16 as the GPU stores them; a finite-difference check of the
+// gradients switches the rounding off (the rounded loss is piecewise constant in the weights).
+static bool g_key_round_weights = true;
+
+struct KeyNet {
+    std::vector<double> w1, b1, w2, b2;
+    KeyNet(const float *W1, const float *B1, const float *W2, const float *B2)
+        : w1(W1, W1 + KEY_H * KEY_IN), b1(B1, B1 + KEY_H), w2(W2, W2 + KEY_OUT * KEY_H), b2(B2, B2 + KEY_OUT) {
+        if (g_key_round_weights) {
+            for (auto &v : w1) v = bf16_round((float)v);
+            for (auto &v : w2) v = bf16_round((float)v);
+        }
+    }
+    // weights given in double (finite differences)
+    KeyNet(const double *W1, const double *B1, const double *W2, const double *B2)
+        : w1(W1, W1 + KEY_H * KEY_IN), b1(B1, B1 + KEY_H), w2(W2, W2 + KEY_OUT * KEY_H), b2(B2, B2 + KEY_OUT) {}
+    void forward(const float *x, double *z, double *k) const {
+        for (int j = 0; j < KEY_H; j++) {
+            double s = b1[j];
+            for (int i = 0; i < KEY_IN; i++) s += w1[j * KEY_IN + i] * x[i];
+            z[j] = s;
+        }
+        for (int o = 0; o < KEY_OUT; o++) {
+            double s = b2[o];
+            for (int j = 0; j < KEY_H; j++) s += w2[o * KEY_H + j] * (z[j] > 0 ? z[j] : 0.0);
+            k[o] = s;
+        }
+    }
+};
+
 }  // namespace orc
 
 // ============================================================================================
@@ -1651,6 +1727,43 @@ void *orc_build(const pt_params *p, const float *b, int h, int w, int c, const i
 }
 
 void orc_free(void *t) { delete (Table *)t; }
+
+// Learned-key table: the key of b ((h+7) x (w+7) x 8, computed by the GPU) indexes the table,
+// with default slices over its 8 dimensions; exact RGB SSD for every distance; the GCK table
+// built with the same parameters is the further table (its shuffle continues the rand() stream).
+void *orc_build_keyed(const pt_params *p_in, const float *b, int h, int w, const float *key_hwc) {
+    pt_params p = *p_in;
+    if (p.grid_ndims < 0) p.grid_ndims = p.ndims;
+    pt_params pk = p;
+    pk.ndims = KEY_OUT;
+    pk.nchroma = 0;
+    pk.grid_ndims = KEY_OUT;
+    pk.ntables = 1;
+    pk.nslices_count = 0;
+    g_key_hwc = key_hwc;
+    Table *T = build(pk, b, h, w, 3, nullptr);
+    g_key_hwc = nullptr;
+    T->keyed = true;
+    pt_params pg = p;
+    pg.kcoherence = 0;
+    pg.ntables = 1;
+    T->extra.push_back(build(pg, b, h, w, 3, nullptr, T->rand_used));
+    return T;
+}
+
+int orc_lookup_parallel_keyed(void *t, const float *a, int h, int w, const float *qkey, double *ann) {
+    g_query_key = qkey;
+    lookup_parallel(*(Table *)t, a, h, w, 3, ann, nullptr, nullptr);
+    g_query_key = nullptr;
+    return 0;
+}
+
+int orc_lookup_raster_keyed(void *t, const float *a, int h, int w, const float *qkey, double *ann) {
+    g_query_key = qkey;
+    lookup_raster(*(Table *)t, a, h, w, 3, ann, nullptr, nullptr);
+    g_query_key = nullptr;
+    return 0;
+}
 
 // The reference's table keeps a pointer to the caller's parameters (patchtable.h:2602), so
 // parameters changed after construction apply to later lookups; this mirrors that.
@@ -1809,6 +1922,103 @@ int orc_lookup_parallel(void *t, const float *a, int h, int w, int c, double *an
 }
 
 float orc_desc_dist(const float *a, const float *b, int C) { return desc_dist(a, b, C); }
+
+// Key planes (8 x (h+7) x (w+7), patch (x, y) at (y+7, x+7); other entries untouched).
+int orc_key_forward(const float *W1, const float *B1, const float *W2, const float *B2, const float *img, int h, int w,
+                    float *planes) {
+    KeyNet net(W1, B1, W2, B2);
+    const size_t plane = (size_t)(h + 7) * (w + 7);
+    float xv[KEY_IN];
+    double z[KEY_H], k[KEY_OUT];
+    for (int y = 0; y + 8 <= h; y++)
+        for (int x = 0; x + 8 <= w; x++) {
+            key_patch(img, w, x, y, xv);
+            net.forward(xv, z, k);
+            for (int o = 0; o < KEY_OUT; o++) planes[o * plane + (size_t)(y + 7) * (w + 7) + x + 7] = (float)k[o];
+        }
+    return 0;
+}
+
+// Gradients of the mean triplet margin loss on squared key distances (anchor in a, positive and
+// negative in b; pos[3t + role] = XY_TO_INT), in double.
+int orc_key_gradients(const float *W1, const float *B1, const float *W2, const float *B2, const float *a,
+                      const float *b, int w, const int32_t *pos, int nt, float margin, float *gW1, float *gb1,
+                      float *gW2, float *gb2, float *loss) {
+    KeyNet net(W1, B1, W2, B2);
+    std::vector<double> G1((size_t)KEY_H * KEY_IN, 0.0), Gb1(KEY_H, 0.0), G2((size_t)KEY_OUT * KEY_H, 0.0), Gb2(KEY_OUT, 0.0);
+    double L = 0.0;
+    float xv[3][KEY_IN];
+    double z[3][KEY_H], k[3][KEY_OUT];
+    for (int t = 0; t < nt; t++) {
+        for (int r = 0; r < 3; r++) {
+            const int v = pos[3 * t + r];
+            key_patch(r == 0 ? a : b, w, INT_TO_X(v), INT_TO_Y(v), xv[r]);
+            net.forward(xv[r], z[r], k[r]);
+        }
+        double dap = 0, dan = 0;
+        for (int o = 0; o < KEY_OUT; o++) {
+            dap += (k[0][o] - k[1][o]) * (k[0][o] - k[1][o]);
+            dan += (k[0][o] - k[2][o]) * (k[0][o] - k[2][o]);
+        }
+        const double l = dap - dan + margin;
+        if (l <= 0) continue;
+        L += l / nt;
+        double dk[3][KEY_OUT];
+        for (int o = 0; o < KEY_OUT; o++) {
+            dk[0][o] = 2.0 * (k[2][o] - k[1][o]) / nt;
+            dk[1][o] = -2.0 * (k[0][o] - k[1][o]) / nt;
+            dk[2][o] = 2.0 * (k[0][o] - k[2][o]) / nt;
+        }
+        for (int r = 0; r < 3; r++) {
+            for (int o = 0; o < KEY_OUT; o++) {
+                Gb2[o] += dk[r][o];
+                for (int j = 0; j < KEY_H; j++) G2[o * KEY_H + j] += dk[r][o] * (z[r][j] > 0 ? z[r][j] : 0.0);
+            }
+            for (int j = 0; j < KEY_H; j++) {
+                if (z[r][j] <= 0) continue;
+                double dh = 0;
+                for (int o = 0; o < KEY_OUT; o++) dh += net.w2[o * KEY_H + j] * dk[r][o];
+                Gb1[j] += dh;
+                for (int i = 0; i < KEY_IN; i++) G1[j * KEY_IN + i] += dh * xv[r][i];
+            }
+        }
+    }
+    for (size_t i = 0; i < G1.size(); i++) gW1[i] = (float)G1[i];
+    for (int i = 0; i < KEY_H; i++) gb1[i] = (float)Gb1[i];
+    for (size_t i = 0; i < G2.size(); i++) gW2[i] = (float)G2[i];
+    for (int i = 0; i < KEY_OUT; i++) gb2[i] = (float)Gb2[i];
+    *loss = (float)L;
+    return 0;
+}
+
+int orc_key_round_weights(int on) {
+    g_key_round_weights = on != 0;
+    return 0;
+}
+
+// The mean triplet loss in double for double weights (the finite-difference side of the
+// gradient check).
+double orc_key_loss(const double *W1, const double *B1, const double *W2, const double *B2, const float *a,
+                    const float *b, int w, const int32_t *pos, int nt, double margin) {
+    KeyNet net(W1, B1, W2, B2);
+    float xv[KEY_IN];
+    double z[KEY_H], k[3][KEY_OUT], L = 0.0;
+    for (int t = 0; t < nt; t++) {
+        for (int r = 0; r < 3; r++) {
+            const int v = pos[3 * t + r];
+            key_patch(r == 0 ? a : b, w, INT_TO_X(v), INT_TO_Y(v), xv);
+            net.forward(xv, z, k[r]);
+        }
+        double dap = 0, dan = 0;
+        for (int o = 0; o < KEY_OUT; o++) {
+            dap += (k[0][o] - k[1][o]) * (k[0][o] - k[1][o]);
+            dan += (k[0][o] - k[2][o]) * (k[0][o] - k[2][o]);
+        }
+        const double l = dap - dan + margin;
+        if (l > 0) L += l / nt;
+    }
+    return L;
+}
 
 // recompute_mean_dist<1> (patchtable_main.cpp:103-172, descriptor mode): float64 squared
 // distance of the descriptors of every match (clamped), mean of the square roots.

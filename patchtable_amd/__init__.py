@@ -20,8 +20,8 @@ NNF_X = 0
 NNF_Y = 1
 NNF_DIST = 2
 
-__all__ = ["PatchTableParams", "PatchTable", "PatchTableBatch", "PatchTableError", "NNF_X", "NNF_Y",
-           "NNF_DIST", "synth_pair", "load_color_image", "save_color_image"]
+__all__ = ["PatchTableParams", "PatchTable", "PatchTableBatch", "PatchKey", "PatchTableError", "NNF_X",
+           "NNF_Y", "NNF_DIST", "synth_pair", "load_color_image", "save_color_image"]
 
 
 class PatchTableParams:
@@ -115,6 +115,21 @@ class PatchTable:
             pad = 14 if self.p.descriptor_padded else 0
             return (qh - pad, qw - pad, 3)
         return (qh - 7, qw - 7, 3)
+
+    @classmethod
+    def keyed(cls, p, key, b):
+        """Learned-key table of the device image b (torch CUDA tensor, h x w x 3): the 8-dim key
+        of `key` (a PatchKey) indexes the table; the GCK table of b with p is its fallback."""
+        if not _is_device_tensor(b):
+            raise PatchTableError("PatchTable.keyed expects a torch CUDA tensor")
+        self = cls.__new__(cls)
+        self.p = p
+        self._t = ctypes.c_void_p()
+        h, w, _ = (int(s) for s in b.shape)
+        check(load().pt_table_create_keyed(ctypes.byref(p.raw), key._k, ctypes.c_void_p(b.data_ptr()), h, w,
+                                           ctypes.byref(self._t)))
+        self._shape()
+        return self
 
     @classmethod
     def load(cls, p, filename):
@@ -284,6 +299,96 @@ class PatchTableBatch:
             self.close()
         except Exception:
             pass
+
+
+class PatchKey:
+    """Learned patch key: a 192 -> 64 -> 8 MLP (ReLU) from an 8x8 RGB patch
+    (x[dy*24 + dx*3 + c] = img(y+dy, x+dx, c) - 0.5) to an 8-dim key, BF16 weights with FP32
+    accumulation, trained on the GPU from triplets mined by the library's own lookups.
+    Images are torch CUDA tensors (h x w x 3 float32)."""
+
+    KI, KH, KO = 192, 64, 8
+
+    def __init__(self, seed=1):
+        self._k = ctypes.c_void_p()
+        check(load().pt_key_create(int(seed), ctypes.byref(self._k)))
+
+    def close(self):
+        if getattr(self, "_k", None) and self._k.value:
+            load().pt_key_destroy(self._k)
+            self._k = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_weights(self, w1, b1, w2, b2):
+        arrs = [np.ascontiguousarray(x, dtype=np.float32) for x in (w1, b1, w2, b2)]
+        shapes = [(self.KH, self.KI), (self.KH,), (self.KO, self.KH), (self.KO,)]
+        for a, sh in zip(arrs, shapes):
+            if a.shape != sh:
+                raise PatchTableError(f"expected weights of shape {sh}, got {a.shape}")
+        check(load().pt_key_set_weights(self._k, *[_ffi_f32(a) for a in arrs]))
+
+    def weights(self):
+        """(w1, b1, w2, b2) FP32 masters and (w1, w2) BF16 bit patterns (uint16)."""
+        w1 = np.empty((self.KH, self.KI), np.float32)
+        b1 = np.empty(self.KH, np.float32)
+        w2 = np.empty((self.KO, self.KH), np.float32)
+        b2 = np.empty(self.KO, np.float32)
+        w1b = np.empty((self.KH, self.KI), np.uint16)
+        w2b = np.empty((self.KO, self.KH), np.uint16)
+        u16 = ctypes.POINTER(ctypes.c_uint16)
+        check(load().pt_key_get_weights(self._k, _ffi_f32(w1), _ffi_f32(b1), _ffi_f32(w2), _ffi_f32(b2),
+                                        w1b.ctypes.data_as(u16), w2b.ctypes.data_as(u16)))
+        return (w1, b1, w2, b2), (w1b, w2b)
+
+    def forward(self, img):
+        """Key planes of a device image: torch tensor 8 x (h+7) x (w+7), patch (x, y) at
+        (y+7, x+7), zeros elsewhere."""
+        import torch
+        h, w, _ = (int(s) for s in img.shape)
+        out = torch.zeros((self.KO, h + 7, w + 7), dtype=torch.float32, device=img.device)
+        check(load().pt_key_forward_dev(self._k, ctypes.c_void_p(img.data_ptr()), h, w,
+                                        ctypes.c_void_p(out.data_ptr())))
+        return out
+
+    def gradients(self, a, b, pos, margin):
+        """Triplet-loss gradients (gW1, gb1, gW2, gb2) and loss for triplet positions pos
+        (device int32 tensor [nt, 3] of XY_TO_INT: anchor in a, positive and negative in b)."""
+        nt = int(pos.shape[0])
+        g = [np.empty((self.KH, self.KI), np.float32), np.empty(self.KH, np.float32),
+             np.empty((self.KO, self.KH), np.float32), np.empty(self.KO, np.float32)]
+        loss = np.zeros(1, np.float32)
+        if tuple(a.shape) != tuple(b.shape):
+            raise PatchTableError("anchor and positive/negative images must have equal sizes")
+        check(load().pt_key_gradients(self._k, ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                      int(a.shape[0]), int(a.shape[1]), ctypes.c_void_p(pos.data_ptr()), nt,
+                                      float(margin),
+                                      *[_ffi_f32(x) for x in g], _ffi_f32(loss)))
+        return g, float(loss[0])
+
+    def train(self, p, pairs, tpp=2048, steps=300, batch=2048, lr=1e-3, margin=1.0, seed=1):
+        """Mines triplets from the library's lookups of the (a, b) device image pairs, then runs
+        `steps` Adam steps; returns the per-step losses (numpy)."""
+        n = len(pairs)
+        A = (ctypes.c_void_p * n)(*[a.data_ptr() for a, _ in pairs])
+        B = (ctypes.c_void_p * n)(*[b.data_ptr() for _, b in pairs])
+        H = (ctypes.c_int * n)(*[int(a.shape[0]) for a, _ in pairs])
+        W = (ctypes.c_int * n)(*[int(a.shape[1]) for a, _ in pairs])
+        for a, b in pairs:
+            if tuple(a.shape) != tuple(b.shape):
+                raise PatchTableError("training pairs must have equal image sizes")
+        losses = np.zeros(max(steps, 1), np.float32)
+        check(load().pt_key_train(self._k, ctypes.byref(p.raw), n, A, B, H, W, int(tpp), int(steps), int(batch),
+                                  float(lr), float(margin), int(seed), _ffi_f32(losses)))
+        return losses[:steps]
+
+
+def _ffi_f32(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def _is_device_tensor(x):

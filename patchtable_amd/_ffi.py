@@ -152,6 +152,16 @@ SIGNATURES = {
     "pt_image_size": (_int, [ctypes.c_char_p, _P(_int), _P(_int), _P(_int)]),
     "pt_image_load": (_int, [ctypes.c_char_p, _f32p, _int, _int, _int]),
     "pt_pfm_save": (_int, [ctypes.c_char_p, _f64p, _int, _int, _int]),
+    "pt_key_create": (_int, [ctypes.c_uint64, _P(_vp)]),
+    "pt_key_destroy": (_int, [_vp]),
+    "pt_key_set_weights": (_int, [_vp, _f32p, _f32p, _f32p, _f32p]),
+    "pt_key_get_weights": (_int, [_vp, _f32p, _f32p, _f32p, _f32p, _P(ctypes.c_uint16), _P(ctypes.c_uint16)]),
+    "pt_key_forward_dev": (_int, [_vp, _vp, _int, _int, _vp]),
+    "pt_key_gradients": (_int, [_vp, _vp, _vp, _int, _int, _vp, _int, ctypes.c_float, _f32p, _f32p, _f32p, _f32p,
+                                _f32p]),
+    "pt_key_train": (_int, [_vp, _P(PtParams), _int, _P(_vp), _P(_vp), _P(_int), _P(_int), _int, _int, _int,
+                            ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _f32p]),
+    "pt_table_create_keyed": (_int, [_P(PtParams), _vp, _vp, _int, _int, _P(_vp)]),
 }
 
 _lib = None

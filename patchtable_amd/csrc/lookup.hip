@@ -312,13 +312,13 @@ __device__ __forceinline__ bool random_search(const LookupArgs &L, const float4 
     return false;
 }
 
-__device__ __forceinline__ int table_cell_value(const LookupArgs &L, const DevQuant &q, const int32_t *table, int x,
-                                                int y) {
+__device__ __forceinline__ int table_cell_value(const LookupArgs &L, const float *desc, const DevQuant &q,
+                                                const int32_t *table, int x, int y) {
     const size_t plane = (size_t)(L.ah + 7) * (L.aw + 7);
     const size_t off = (size_t)(y + 7) * (L.aw + 7) + x + 7;
     int64_t cell = 0;
     for (int d = 0; d < q.G; d++)
-        cell += (int64_t)pt_get_bin(L.a_desc[d * plane + off], q.min_val[d], q.range[d], q.to_bin + (size_t)d * PT_NMAP) *
+        cell += (int64_t)pt_get_bin(desc[d * plane + off], q.min_val[d], q.range[d], q.to_bin + (size_t)d * PT_NMAP) *
                 q.stride[d];
     return table[cell];
 }
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
         have = true;
     }
     if (valid && L.do_table_lookup) {
-        const int v = table_cell_value(L, L.q, L.table, x, y);
+        const int v = table_cell_value(L, L.a_desc, L.q, L.table, x, y);
         const int xs = PT_INT_TO_X(v) - 7, ys = PT_INT_TO_Y(v) - 7;
         if (!have || xs != xb || ys != yb) {
             const float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, have ? db : INFINITY);
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
         }
         int xp = xs, yp = ys;
         for (int t = 0; t < L.nxt; t++) {   // further tables (patchtable.h:3711-3737)
-            const int v1 = table_cell_value(L, L.xq[t], L.xtable[t], x, y);
+            const int v1 = table_cell_value(L, L.xa_desc, L.xq[t], L.xtable[t], x, y);
             const int x1 = PT_INT_TO_X(v1) - 7, y1 = PT_INT_TO_Y(v1) - 7;
             if ((x1 != xb || y1 != yb) && (x1 != xp || y1 != yp)) {
                 const float d = qdist<M>(L, qbuf, threadIdx.x, x1, y1, db);

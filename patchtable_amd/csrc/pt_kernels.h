@@ -38,6 +38,32 @@ struct PtScratch {
     ~PtScratch() { release(); }
 };
 
+// Owned device array, grown on demand (contents are not kept across a regrow).
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t count) {
+        if (count <= n && p) return PT_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc((void **)&p, (count ? count : 1) * sizeof(T)) != hipSuccess)
+            return pt_failf(PT_E_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+        n = count;
+        return PT_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+};
+
 // Quantiser state on the device: per grid dimension min, range, 16384-entry bin map, stride.
 struct DevQuant {
     int G = 0;
@@ -103,6 +129,8 @@ struct LookupArgs {
     int nxt = 0;                    // further tables
     const int32_t *xtable[PT_MAX_XTABLES] = {};
     DevQuant xq[PT_MAX_XTABLES];
+    const float *xa_desc = nullptr; // query descriptor planes indexing the further tables (a_desc,
+                                    // or the GCK planes of a learned-key table's fallback)
     const int32_t *kpos = nullptr;
     const float *kdist = nullptr;
     int K = 0;
@@ -146,5 +174,37 @@ int pt_desc_pad_launch(const float *in, int h_in, int w_in, int C, int padded, i
 int pt_ssd_pairs_launch(const float *a, int aw, const float *b, int bw, const int32_t *ax, const int32_t *ay,
                         const int32_t *bx, const int32_t *by, int n, float *out, hipStream_t s);
 int pt_desc_dist_pairs_launch(const float *a, const float *b, int C, int n, float *out, hipStream_t s);
+
+// key.hip: the learned patch key (192 -> 64 -> 8 MLP, BF16 weights, FP32 accumulate).
+#define PT_KEY_IN 192
+#define PT_KEY_HIDDEN 64
+#define PT_KEY_OUT 8
+#define PT_KEY_NPARAM (PT_KEY_HIDDEN * PT_KEY_IN + PT_KEY_HIDDEN + PT_KEY_OUT * PT_KEY_HIDDEN + PT_KEY_OUT)
+struct KeyParamsDev {          // FP32 masters, BF16 copies, Adam moments (m, v: PT_KEY_NPARAM each)
+    float *w1, *b1, *w2, *b2;
+    unsigned short *w1bf, *w2bf;
+    float *m, *v;
+};
+struct KeyWorkDev {            // per-batch training buffers (n = 3 x triplets samples)
+    float *Z, *K, *dK, *dH;
+    float *gW1, *gb1, *gW2, *gb2;
+    float *loss;
+};
+struct KeyNetDev {             // BF16 weights and FP32 biases of one key network
+    const unsigned short *w1 = nullptr;
+    const float *b1 = nullptr;
+    const unsigned short *w2 = nullptr;
+    const float *b2 = nullptr;
+};
+struct pt_key;
+int pt_key_snapshot(const pt_key *k, unsigned short *w1, float *b1, unsigned short *w2, float *b2, hipStream_t s);
+int pt_key_forward_launch(const float *img, int h, int w, const unsigned short *w1, const float *b1,
+                          const unsigned short *w2, const float *b2, float *planes, hipStream_t s);
+int pt_key_extract_launch(const float *a, const float *b, int w, const int32_t *pos, const int64_t *slot, int nt,
+                          unsigned short *X, hipStream_t s);
+// One training step on nt triplets (lr 0: gradients only).
+int pt_key_train_step(const unsigned short *X, int nt, float margin, const KeyParamsDev &P, const KeyWorkDev &W,
+                      float lr, float beta1, float beta2, float eps, int step, float wd, hipStream_t s);
+int pt_key_to_bf16_launch(const float *in, int n, unsigned short *out, hipStream_t s);
 
 #endif

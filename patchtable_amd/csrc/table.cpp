@@ -31,28 +31,6 @@
 
 namespace {
 
-template <class T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    int ensure(size_t count) {
-        if (count <= n && p) return PT_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        if (hipMalloc((void **)&p, (count ? count : 1) * sizeof(T)) != hipSuccess)
-            return pt_failf(PT_E_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
-        n = count;
-        return PT_OK;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DevBuf() { release(); }
-};
-
 struct HostPartition {   // PatchPartition1D fields (patchtable.h:1133-1141)
     std::vector<int32_t> to_bin;
     std::vector<float> centers, lsizes, rsizes;
@@ -110,6 +88,12 @@ struct pt_table {
     DevBuf<int> bad;
     int qh = 0, qw = 0;
     PtScratch sc;
+    // learned-key table: the key network (copied at build) indexes the table; extra[0] is the GCK
+    // fallback table, indexed by the query's GCK planes (xadesc)
+    bool keyed = false;
+    KeyNetDev knet;
+    DevBuf<unsigned short> kw1, kw2;
+    DevBuf<float> kb1, kb2, xadesc;
 
     DevQuant quant() const {
         DevQuant q;
@@ -626,7 +610,19 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     t->qw = aw;
     if (timing) HIPTRY(hipEventRecord(t->ev[6], s));
     LookupArgs L;
-    if (adesc_pre && !t->desc_mode) {
+    if (t->keyed) {
+        // the key planes index the main table, the GCK planes the fallback table
+        const pt_table *E = t->extra[0];
+        HIPTRY(hipMemsetAsync(t->adesc.p, 0, plane * t->ndims * sizeof(float), s));
+        TRY(pt_key_forward_launch(a_dev, ah, aw, t->knet.w1, t->knet.b1, t->knet.w2, t->knet.b2, t->adesc.p, s));
+        TRY(t->xadesc.ensure(plane * E->ndims));
+        TRY(t->yuv.ensure((size_t)ah * aw * 3));
+        TRY(t->temp.ensure(plane * 3));
+        TRY(pt_gck_launch(a_dev, ah, aw, E->ndims - 2 * E->nchroma, E->nchroma, E->build.convert_colorspace, t->yuv.p,
+                          t->temp.p, t->xadesc.p, s));
+        L.a = a_dev;
+        L.b = t->b0.p;
+    } else if (adesc_pre && !t->desc_mode) {
         L.a = a_dev;
         L.b = t->b0.p;
     } else if (t->desc_mode) {
@@ -647,7 +643,8 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     L.aw = aw;
     L.ann_w = ann_w;
     L.ann_h = ann_h;
-    L.a_desc = adesc_pre && !t->desc_mode ? adesc_pre : t->adesc.p;
+    L.a_desc = adesc_pre && !t->desc_mode && !t->keyed ? adesc_pre : t->adesc.p;
+    L.xa_desc = t->keyed ? t->xadesc.p : L.a_desc;
     L.bh = t->h;
     L.bw = t->w;
     L.table = t->table.p;
@@ -755,6 +752,118 @@ extern "C" int pt_table_create(pt_params *p, const float *b, int h, int w, int c
 extern "C" int pt_table_create_dev(pt_params *p, const float *b_dev, int h, int w, int c, const int32_t *allowed,
                                    pt_table **out) {
     return create_common(p, b_dev, true, h, w, c, allowed, out);
+}
+
+namespace {
+
+// Learned-key table: main table over the 8 key planes of b (k-coherence in key space), then the
+// GCK table of b with the caller's parameters as the further table; its shuffle continues the
+// rand() stream after the key table's.
+int build_keyed(pt_table *t, const pt_key *key, const float *b_dev, int h, int w) {
+    const pt_params &p = t->build;
+    hipStream_t s = t->stream;
+    const int Hp = h + 7, Wp = w + 7;
+    const size_t plane = (size_t)Hp * Wp;
+    TRY(t->kw1.ensure(PT_KEY_HIDDEN * PT_KEY_IN));
+    TRY(t->kb1.ensure(PT_KEY_HIDDEN));
+    TRY(t->kw2.ensure(PT_KEY_OUT * PT_KEY_HIDDEN));
+    TRY(t->kb2.ensure(PT_KEY_OUT));
+    TRY(pt_key_snapshot(key, t->kw1.p, t->kb1.p, t->kw2.p, t->kb2.p, s));
+    t->knet.w1 = t->kw1.p;
+    t->knet.b1 = t->kb1.p;
+    t->knet.w2 = t->kw2.p;
+    t->knet.b2 = t->kb2.p;
+    HIPTRY(hipEventRecord(t->ev[0], s));
+    TRY(t->b0.ensure((size_t)h * w * 3));
+    HIPTRY(hipMemcpyAsync(t->b0.p, b_dev, (size_t)h * w * 3 * 4, hipMemcpyDeviceToDevice, s));
+    TRY(t->desc.ensure(plane * t->ndims));
+    HIPTRY(hipMemsetAsync(t->desc.p, 0, plane * t->ndims * sizeof(float), s));
+    TRY(pt_key_forward_launch(t->b0.p, h, w, t->knet.w1, t->knet.b1, t->knet.w2, t->knet.b2, t->desc.p, s));
+    HIPTRY(hipEventRecord(t->ev[1], s));
+    const std::vector<int32_t> no_mask;
+    TRY(build_grid(t, t->desc.p, no_mask, nullptr, 0, s, t->ev));
+    t->K = p.kcoherence;
+    if (t->K > 0) {
+        const size_t n = (size_t)(w - 7) * (h - 7);
+        TRY(t->kpos.ensure(n * t->K));
+        TRY(t->kdist.ensure(n * t->K));
+        TRY(pt_kcoherence_launch(t->desc.p, plane, Wp, t->ndims, t->b0.p, nullptr, w, h, nullptr, t->allowed_index,
+                                 t->quant(), t->nslices.data(), t->K, p.kcoh_window,
+                                 p.kcoherence_min_dist * p.kcoherence_min_dist, t->kpos.p, t->kdist.p, t->sc, s));
+    }
+    pt_table *E = new pt_table();
+    t->extra.push_back(E);
+    E->build = *t->p;
+    E->build.kcoherence = 0;
+    E->build.ntables = 1;
+    E->h = h;
+    E->w = w;
+    E->ndims = E->build.ndims;
+    E->nchroma = E->build.nchroma;
+    E->G = E->build.grid_ndims;
+    E->allowed_index = t->allowed_index;
+    TRY(E->desc.ensure(plane * E->ndims));
+    TRY(t->yuv.ensure((size_t)h * w * 3));
+    TRY(t->temp.ensure(plane * 3));
+    TRY(pt_gck_launch(t->b0.p, h, w, E->ndims - 2 * E->nchroma, E->nchroma, E->build.convert_colorspace, t->yuv.p,
+                      t->temp.p, E->desc.p, s));
+    TRY(build_grid(E, E->desc.p, no_mask, nullptr, t->rand_used, s, nullptr));
+    HIPTRY(hipEventRecord(t->ev[5], s));
+    HIPTRY(hipStreamSynchronize(s));
+    for (DevBuf<float> *b : {&E->desc}) b->release();
+    E->list_xy.release();
+    E->rank.release();
+    E->cells.release();
+    E->win.release();
+    E->rnd.release();
+    E->sc.release();
+    float ms = 0;
+    for (int k = 0; k < 5; k++) {
+        HIPTRY(hipEventElapsedTime(&ms, t->ev[k], t->ev[k + 1]));
+        t->stage_ms[k] = ms;
+    }
+    HIPTRY(hipEventElapsedTime(&ms, t->ev[0], t->ev[5]));
+    t->stage_ms[5] = ms;
+    t->rnd.release();
+    return PT_OK;
+}
+
+}  // namespace
+
+extern "C" int pt_table_create_keyed(pt_params *p, const pt_key *key, const float *b_dev, int h, int w,
+                                     pt_table **out) {
+    if (!p || !key || !b_dev || !out) return pt_fail(PT_E_INVALID, "pt_table_create_keyed: null argument");
+    *out = nullptr;
+    if (p->is_descriptor) return pt_fail(PT_E_UNSUPPORTED, "a learned-key table indexes RGB images (is_descriptor 0)");
+    if (p->ntables != 1) return pt_fail(PT_E_UNSUPPORTED, "a learned-key table has its own further table (ntables 1)");
+    if (p->load_filename[0] || p->save_filename[0])
+        return pt_fail(PT_E_UNSUPPORTED, "a learned-key table has no file format (load/save)");
+    TRY(check_build_params(p, 3));
+    if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "database image must be at least 8x8");
+    if (h > 65536 - 8 || w > 65536 - 8) return pt_fail(PT_E_INVALID, "image too large for XY_TO_INT packing");
+    if (p->grid_ndims < 0) p->grid_ndims = p->ndims;   // patchtable.h:2603
+    pt_table *t = new pt_table();
+    t->p = p;
+    t->build = *p;
+    t->build.ndims = PT_KEY_OUT;   // the key table: 8 grid dimensions, default slices
+    t->build.grid_ndims = PT_KEY_OUT;
+    t->build.nchroma = 0;
+    t->build.nslices_count = 0;
+    t->h = h;
+    t->w = w;
+    t->ndims = PT_KEY_OUT;
+    t->nchroma = 0;
+    t->G = PT_KEY_OUT;
+    t->keyed = true;
+    t->allowed_index = p->allowed_index;
+    int rc = init_stream(t);
+    if (rc == PT_OK) rc = build_keyed(t, key, b_dev, h, w);
+    if (rc != PT_OK) {
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return PT_OK;
 }
 
 extern "C" int pt_table_lookup(pt_table *t, const float *a, int h, int w, int c, double *ann_out,
@@ -889,6 +998,7 @@ int hwc_to_planes(const float *hwc, size_t plane, int C, std::vector<float> &pla
 }  // namespace
 
 extern "C" int pt_table_save(pt_table *t, const char *filename) {
+    if (t && t->keyed) return pt_fail(PT_E_UNSUPPORTED, "a learned-key table has no file format");
     if (!t || !filename) return pt_fail(PT_E_INVALID, "pt_table_save: null argument");
     const int Hp = t->h + 7, Wp = t->w + 7;
     std::vector<int32_t> cells((size_t)t->ncells);

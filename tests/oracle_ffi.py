@@ -60,6 +60,15 @@ def lib():
             "orc_recompute_mean_dist_desc": (ctypes.c_double, [_f32p, _int, _int, _f32p, _int, _int, _int, _f64p, _f64p]),
             "orc_mean_dist": (ctypes.c_double, [_f64p, ctypes.c_long]),
             "orc_recompute_mean_dist": (ctypes.c_double, [_f32p, _int, _int, _f32p, _int, _int, _f64p, _f64p]),
+            "orc_build_keyed": (ctypes.c_void_p, [_P(PtParams), _f32p, _int, _int, _f32p]),
+            "orc_lookup_parallel_keyed": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f32p, _f64p]),
+            "orc_lookup_raster_keyed": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f32p, _f64p]),
+            "orc_key_round_weights": (_int, [_int]),
+            "orc_key_loss": (ctypes.c_double, [_f64p, _f64p, _f64p, _f64p, _f32p, _f32p, _int, _i32p, _int,
+                                               ctypes.c_double]),
+            "orc_key_forward": (_int, [_f32p, _f32p, _f32p, _f32p, _f32p, _int, _int, _f32p]),
+            "orc_key_gradients": (_int, [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _int, _i32p, _int, ctypes.c_float,
+                                         _f32p, _f32p, _f32p, _f32p, _f32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -88,13 +97,18 @@ def i32(a):
 class OracleTable:
     """CPU restatement of PatchTable construction (oracle/pt_oracle.cpp `build`)."""
 
-    def __init__(self, params, b, allowed=None):
+    def __init__(self, params, b, allowed=None, key=None):
+        """key: for a learned-key table, the key of b ((h+7) x (w+7) x 8 float32)."""
         b = np.ascontiguousarray(b, dtype=np.float32)
         self.b = b
         self.params = params
         self.allowed = None if allowed is None else np.ascontiguousarray(allowed, dtype=np.int32)
-        self.handle = lib().orc_build(ctypes.byref(params), f32(b), b.shape[0], b.shape[1], b.shape[2],
-                                      None if self.allowed is None else i32(self.allowed))
+        if key is not None:
+            self._key = np.ascontiguousarray(key, dtype=np.float32)
+            self.handle = lib().orc_build_keyed(ctypes.byref(params), f32(b), b.shape[0], b.shape[1], f32(self._key))
+        else:
+            self.handle = lib().orc_build(ctypes.byref(params), f32(b), b.shape[0], b.shape[1], b.shape[2],
+                                          None if self.allowed is None else i32(self.allowed))
         h, w, nd = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         lib().orc_shape(self.handle, ctypes.byref(h), ctypes.byref(w), ctypes.byref(nd))
         self.h, self.w, self.nd = h.value, w.value, nd.value
@@ -182,6 +196,53 @@ class OracleTable:
 
     def lookup_parallel(self, a, prev=None, sv=None):
         return self._lookup(lib().orc_lookup_parallel, a, prev, sv)
+
+    def lookup_keyed(self, a, qkey, raster=False):
+        """Learned-key table: lookup of RGB a with its key qkey ((h+7) x (w+7) x 8)."""
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        q = np.ascontiguousarray(qkey, dtype=np.float32)
+        ann = np.empty((a.shape[0] - 7, a.shape[1] - 7, 3), np.float64)
+        fn = lib().orc_lookup_raster_keyed if raster else lib().orc_lookup_parallel_keyed
+        fn(self.handle, f32(a), a.shape[0], a.shape[1], f32(q), f64(ann))
+        return ann
+
+
+def key_forward(w1, b1, w2, b2, img):
+    """Key planes 8 x (h+7) x (w+7) of an RGB image (double accumulation over BF16-rounded
+    weights and inputs)."""
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    h, w = img.shape[:2]
+    out = np.zeros((8, h + 7, w + 7), np.float32)
+    ws = [np.ascontiguousarray(x, dtype=np.float32) for x in (w1, b1, w2, b2)]
+    lib().orc_key_forward(*[f32(x) for x in ws], f32(img), h, w, f32(out))
+    return out
+
+
+def key_loss(w1, b1, w2, b2, a, b, pos, margin):
+    """Mean triplet loss in double for double weights (no BF16 rounding)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    pos = np.ascontiguousarray(pos, dtype=np.int32)
+    ws = [np.ascontiguousarray(x, dtype=np.float64) for x in (w1, b1, w2, b2)]
+    return lib().orc_key_loss(*[f64(x) for x in ws], f32(a), f32(b), a.shape[1], i32(pos), pos.shape[0],
+                              float(margin))
+
+
+def key_round_weights(on):
+    lib().orc_key_round_weights(1 if on else 0)
+
+
+def key_gradients(w1, b1, w2, b2, a, b, pos, margin):
+    """(gW1, gb1, gW2, gb2), loss of the mean triplet loss; pos: [nt, 3] XY_TO_INT int32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    pos = np.ascontiguousarray(pos, dtype=np.int32)
+    ws = [np.ascontiguousarray(x, dtype=np.float32) for x in (w1, b1, w2, b2)]
+    g = [np.empty_like(x) for x in ws]
+    loss = np.zeros(1, np.float32)
+    lib().orc_key_gradients(*[f32(x) for x in ws], f32(a), f32(b), a.shape[1], i32(pos), pos.shape[0], float(margin),
+                            *[f32(x) for x in g], f32(loss))
+    return g, float(loss[0])
 
 
 def mean_dist(ann):
