@@ -126,6 +126,90 @@ def one_vs_many(args):
     print(json.dumps(out), flush=True)
 
 
+def timed_lookups(torch, table, a, ann, warmup, steps):
+    """Wall ms per lookup over `steps` calls (after `warmup`) and the last lookup's stage times."""
+    for _ in range(warmup):
+        table.lookup(a, ann)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        table.lookup(a, ann)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, table.stage_ms()
+
+
+def learned_key(args):
+    """Learned key: a 192 -> 64 -> 8 key network trained on the GPU (triplets mined from the
+    library's own lookups of 64 synthetic 1024x1024 pairs, seeds 2000..2063, 31250 per pair:
+    ~2M triplets; 2000 Adam steps of 4096), then the held-out 2048x2048 pair (seed 3) looked up in
+    the key table (with its GCK fallback) and in the plain GCK table, at speed 1 and at fewer
+    propagation rounds. Reports each point's query Mpatch/s and mean NNF SSD, and the fastest
+    keyed point whose SSD is at or below the plain table's at speed 1."""
+    import numpy as np
+    import torch
+    from patchtable_amd import PatchKey, PatchTable, PatchTableParams, synth_pair
+
+    def params(extra=""):
+        p = PatchTableParams()
+        p.set_speed(1)
+        if extra:
+            p.set_from_switches("-x 0 " + extra)
+        return p
+
+    npairs, S, tpp, steps, batch = 64, 1024, 31250, 2000, 4096
+    log(f"learned key: generating {npairs} training pairs of {S}x{S}")
+    pairs = [tuple(torch.from_numpy(x).cuda() for x in synth_pair(2000 + i, S, S)) for i in range(npairs)]
+    key = PatchKey(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = key.train(params(), pairs, tpp=tpp, steps=steps, batch=batch, lr=2e-3, margin=1.0, seed=9)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"trained in {train_s:.2f} s: loss {losses[:10].mean():.4f} -> {losses[-50:].mean():.4f}")
+    del pairs
+    H = W = 2048
+    a_np, b_np = synth_pair(3, H, W)
+    a, b = torch.from_numpy(a_np).cuda(), torch.from_numpy(b_np).cuda()
+    nq = (H - 7) * (W - 7)
+    ann = torch.empty((H - 7, W - 7, 3), dtype=torch.float64, device="cuda")
+    points = []
+    for extra in ["", "-jf_rounds 4", "-jf_rounds 2", "-jf_rounds 1", "-do_prop 0"]:
+        for kind in ("gck", "key"):
+            p = params(extra)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            t = PatchTable(p, b) if kind == "gck" else PatchTable.keyed(p, key, b)
+            torch.cuda.synchronize()
+            build_ms = (time.perf_counter() - t0) * 1e3
+            ms, st = timed_lookups(torch, t, a, ann, args.warmup, args.steps)
+            ssd = float(ann[..., 2].mean().item())
+            points.append({"table": kind, "switches": extra or "(speed 1)", "ms": round(ms, 3),
+                           "mpatch_s": round(nq / ms / 1e3, 2), "mean_ssd": round(ssd, 6),
+                           "build_ms": round(build_ms, 1), "query_key_ms": round(st["query_gck"], 3)})
+            log(json.dumps(points[-1]))
+            t.close()
+    base = points[0]
+    ok = [q for q in points if q["table"] == "key" and q["mean_ssd"] <= base["mean_ssd"]]
+    best = max(ok, key=lambda q: q["mpatch_s"]) if ok else None
+    key_default = points[1]
+    out = {
+        "metric": "learned-key query Mpatch/s (2048x2048 pair, speed 1) beside the plain GCK table",
+        "value": key_default["mpatch_s"], "unit": "Mpatch/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": key_default["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 key network, f32 SSD", "data": "synthetic",
+        "config": {"workload": "learned key: train on 64 1024x1024 pairs (seeds 2000..2063), evaluate the held-out "
+                               "2048x2048 pair (seed 3)", "triplets": npairs * tpp, "adam_steps": steps,
+                   "batch": batch},
+        "train_s": round(train_s, 2),
+        "loss_first_last": [round(float(losses[:10].mean()), 5), round(float(losses[-50:].mean()), 5)],
+        "points": points,
+        "plain_speed1": {"mpatch_s": base["mpatch_s"], "mean_ssd": base["mean_ssd"]},
+        "fastest_key_at_or_below_plain_ssd": best,
+        "speedup_at_equal_ssd": round(best["mpatch_s"] / base["mpatch_s"], 3) if best else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(h=768, w=1024, seed=2):
     """Oracle (CPU restatement of the reference, 1 thread) on a bounded pair; returns its lookup
     Mpatch/s, build ms and raster mean SSD, plus the GPU mean SSD on the same pair."""
@@ -157,9 +241,9 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--build-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="pair", choices=["pair", "one_vs_many"],
+    ap.add_argument("--workload", default="pair", choices=["pair", "one_vs_many", "learned_key"],
                     help="pair: the 4MP pair of the headline metric (default); one_vs_many: one query vs many "
-                         "databases")
+                         "databases; learned_key: train the key network, then key table vs GCK table")
     ap.add_argument("--databases", type=int, default=32)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration for the traffic figure)")
@@ -176,6 +260,9 @@ def main():
 
     if args.workload == "one_vs_many":
         one_vs_many(args)
+        return
+    if args.workload == "learned_key":
+        learned_key(args)
         return
 
     H, W = (int(v) for v in args.size.lower().split("x"))

@@ -84,9 +84,10 @@ int upload_weights(pt_key *k, const float *w1, const float *b1, const float *w2,
 }
 
 struct Work {
-    DevBuf<float> Z, K, dK, dH, gW1, gb1, gW2, gb2, loss;
+    DevBuf<float> Z, K, dK, dH, gW1, gb1, gW2, gb2, loss, part;
     int ensure(int nt) {
         const size_t n = (size_t)3 * nt;
+        TRY(part.ensure(pt_key_grad_part_floats(3 * nt)));
         TRY(Z.ensure(n * KH));
         TRY(K.ensure(n * KO));
         TRY(dK.ensure(n * KO));
@@ -109,6 +110,7 @@ struct Work {
         W.gW2 = gW2.p;
         W.gb2 = gb2.p;
         W.loss = loss.p;
+        W.part = part.p;
         return W;
     }
 };

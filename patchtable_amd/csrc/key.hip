@@ -3,15 +3,14 @@
 // BF16, activations accumulate in FP32. The patch vector is x[dy*24 + dx*3 + c] = img(y+dy,
 // x+dx, c) - 0.5 (the reference's HWC patch order, centred).
 //
-//   forward over an image (k_key_forward): one wave owns 16 patches of a row; layer 1 is
-//     [16 x 192] x [192 x 64] on MFMA (v_mfma_f32_16x16x16_bf16: 12 K-steps x 4 N-tiles) with the
-//     W1 fragments held in registers across the rows the block walks; bias + ReLU; layer 2
-//     (64 -> 8) on VALU from LDS; the keys are written as the 8 descriptor planes of the table
-//     ((h+7) x (w+7), patch (x, y) at (y+7, x+7)).
+//   forward over an image (k_key_forward): both layers on MFMA per 16 patches of a row (BF16
+//     layer 1, FP32 layer 2), weights in registers; the keys are written as the 8 descriptor
+//     planes of the table ((h+7) x (w+7), patch (x, y) at (y+7, x+7)).
 //   training: triplets (anchor, positive, negative) of patch vectors in BF16; k_train_forward
 //     (VALU, FP32 accumulate over BF16 weights), k_triplet_grad (margin loss on squared key
-//     distances), k_grad_w1 / k_grad_small (batch-reduced weight gradients), k_adam (FP32 master
-//     weights, BF16 copies for the forward passes).
+//     distances), k_grad_hidden, k_grad_partial (split over sample chunks; dW1 on FP32 MFMA) +
+//     k_grad_reduce (batch-summed weight gradients), k_adam (FP32 master weights, BF16 copies for
+//     the forward passes).
 #include "pt_device.h"
 #include "pt_kernels.h"
 
@@ -34,11 +33,21 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
 // ---- inference over an image ------------------------------------------------------------------
-// Block: 4 waves over 64 consecutive patches of a row (wave w: patches x0 + 16w .. + 15), looping
-// over rows y = blockIdx.y, blockIdx.y + gridDim.y, ... Each wave holds its W1 B-fragments in
-// registers (48 x 4 BF16 per lane), loaded once, so the weights are read once per wave rather
-// than once per row.
-constexpr int KEY_ROW_BLOCKS = 64;
+// One wave computes the keys of 16 consecutive patches of a row at a time (work items strided
+// over the grid's waves), with both layers on MFMA and nothing in LDS:
+//   layer 1, transposed: H^T (64 hidden x 16 patches) = W1 (64 x 192) . X^T (192 x 16) as
+//     4 row tiles x 6 K-steps of v_mfma_f32_16x16x32_bf16; the W1 A-fragments stay in registers
+//     for the wave's lifetime; the patch B-fragments are 8 consecutive RGB values of one patch row
+//     (8 divides 24) converted to BF16 (round to nearest even) as they are loaded.
+//   bias + ReLU in registers: lane l holds hidden units nb*16 + 4*(l/16) + j of patch l % 16.
+//   layer 2: K^T (8 x 16) = W2 (8 x 64) . relu(H^T) as 16 K-steps of v_mfma_f32_16x16x4_f32 in
+//     FP32, the K-step (nb, j) taking hidden unit nb*16 + 4*(l/16) + j from lane l: the layer-1
+//     accumulator layout is directly the B operand, and A holds the matching W2 column (rows 8..15
+//     zero). Lane l < 32 then holds outputs 4*(l/16) + j of patch l % 16.
+// A and B operands are filled under one convention (lane group l/16 owns the same K indices in
+// both), so the sum over K is independent of how the hardware numbers K inside a group.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float v8f __attribute__((ext_vector_type(8)));
 
 __global__ __launch_bounds__(256) void k_key_forward(const float *__restrict__ img, int h, int w,
                                                     const unsigned short *__restrict__ w1,   // KH x KI bf16
@@ -46,68 +55,81 @@ __global__ __launch_bounds__(256) void k_key_forward(const float *__restrict__ i
                                                     const unsigned short *__restrict__ w2,   // KO x KH bf16
                                                     const float *__restrict__ b2,            // KO
                                                     float *__restrict__ planes, size_t plane, int Wp) {
-    __shared__ float sh[4][16][KH + 1];   // hidden activations per wave
-    __shared__ float sw2[KO * KH];
-    for (int i = threadIdx.x; i < KO * KH; i += 256) sw2[i] = bf2f(w2[i]);
     const int ew = w - 7, eh = h - 7;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int x0 = blockIdx.x * 64 + wave * 16;
-    // MFMA 16x16x16 operand layout: A lane holds row lane % 16, k = 4*(lane/16) .. +3 of a 16-wide
-    // K block; B lane holds column lane % 16 and the same k; D lane holds rows 4*(lane/16) + j.
-    const int prow = lane & 15, kq = lane >> 4;
-    v4s bfrag[KI / 16][4];
+    const int lane = threadIdx.x & 63;
+    const int col = lane & 15, grp = lane >> 4;
+    bf8 afrag[4][KI / 32];
 #pragma unroll
-    for (int kb = 0; kb < KI / 16; kb++)
+    for (int nb = 0; nb < 4; nb++)
 #pragma unroll
-        for (int nb = 0; nb < 4; nb++) {
-            const unsigned short *wr = w1 + (size_t)(nb * 16 + prow) * KI + kb * 16 + 4 * kq;
-            bfrag[kb][nb] = v4s{(short)wr[0], (short)wr[1], (short)wr[2], (short)wr[3]};
+        for (int kb = 0; kb < KI / 32; kb++) {
+            const unsigned short *wr = w1 + (size_t)(nb * 16 + col) * KI + kb * 32 + 8 * grp;
+            bf8 v;
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = __builtin_bit_cast(__bf16, wr[i]);
+            afrag[nb][kb] = v;
         }
-    float bias[4];
+    float bias1[4][4];
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++) bias[nb] = b1[nb * 16 + prow];
-    __syncthreads();
-    const int px = x0 + prow;
-    const bool xvalid = px < ew;
-    for (int y = blockIdx.y; y < eh; y += gridDim.y) {
-        const float *pbase = img + ((size_t)y * w + (xvalid ? px : 0)) * 3;
+    for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) bias1[nb][j] = b1[nb * 16 + 4 * grp + j];
+    float w2c[4][4];   // W2[col][nb*16 + 4*grp + j], zero for rows col >= 8
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float v = bf2f(w2[(col & (KO - 1)) * KH + nb * 16 + 4 * grp + j]);
+            w2c[nb][j] = col < KO ? v : 0.f;
+        }
+    float bias2[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float v = b2[(4 * grp + j) & (KO - 1)];
+        bias2[j] = 4 * grp + j < KO ? v : 0.f;
+    }
+
+    const int ngx = (ew + 15) / 16;
+    const long items = (long)ngx * eh;
+    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+    for (long it = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); it < items; it += nwaves) {
+        const int y = (int)(it / ngx), x0 = (int)(it % ngx) * 16;
+        const int px = x0 + col;
+        const bool valid = px < ew;
+        // lanes past the row end read the last patch (in bounds) and skip the store
+        const float *pbase = img + ((size_t)y * w + (valid ? px : ew - 1)) * 3;
         v4f acc[4];
 #pragma unroll
-        for (int nb = 0; nb < 4; nb++) acc[nb] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int nb = 0; nb < 4; nb++) acc[nb] = v4f{bias1[nb][0], bias1[nb][1], bias1[nb][2], bias1[nb][3]};
 #pragma unroll
-        for (int kb = 0; kb < KI / 16; kb++) {
-            const int k = kb * 16 + 4 * kq;   // 4 consecutive elements inside one patch row of 24
-            const int dy = k / 24, r = k % 24;
-            v4s a = v4s{0, 0, 0, 0};
-            if (xvalid) {
-                const float *src = pbase + (size_t)dy * w * 3 + r;
-                a[0] = (short)f2bf(src[0] - 0.5f);
-                a[1] = (short)f2bf(src[1] - 0.5f);
-                a[2] = (short)f2bf(src[2] - 0.5f);
-                a[3] = (short)f2bf(src[3] - 0.5f);
-            }
+        for (int kb = 0; kb < KI / 32; kb++) {
+            const int k = kb * 32 + 8 * grp;   // 8 consecutive values inside one patch row of 24
+            const float *src = pbase + (size_t)(k / 24) * w * 3 + k % 24;
+            v8f xv;
+#pragma unroll
+            for (int i = 0; i < 8; i++) xv[i] = src[i] - 0.5f;
+            const bf8 bv = __builtin_convertvector(xv, bf8);
 #pragma unroll
             for (int nb = 0; nb < 4; nb++)
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bfrag[kb][nb], acc[nb], 0, 0, 0);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag[nb][kb], bv, acc[nb], 0, 0, 0);
         }
-        // bias + ReLU into LDS: lane holds patches 4*kq + j, hidden unit nb*16 + prow
+        // four independent accumulation chains (one per hidden tile), summed at the end
+        v4f o4[4];
 #pragma unroll
-        for (int nb = 0; nb < 4; nb++)
+        for (int nb = 0; nb < 4; nb++) {
+            o4[nb] = nb == 0 ? v4f{bias2[0], bias2[1], bias2[2], bias2[3]} : v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const float v = acc[nb][j] + bias[nb];
-                sh[wave][4 * kq + j][nb * 16 + prow] = v > 0.f ? v : 0.f;
+                const float hv = acc[nb][j] > 0.f ? acc[nb][j] : 0.f;
+                o4[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(w2c[nb][j], hv, o4[nb], 0, 0, 0);
             }
-        __syncthreads();
-        // layer 2: 16 patches x 8 outputs per wave, 2 per lane
-        for (int t = lane; t < 16 * KO; t += 64) {
-            const int pi = t / KO, o = t % KO;
-            float s = b2[o];
-            for (int n = 0; n < KH; n++) s += sw2[o * KH + n] * sh[wave][pi][n];
-            const int xx = x0 + pi;
-            if (xx < ew) planes[(size_t)o * plane + (size_t)(y + 7) * Wp + xx + 7] = s;
         }
-        __syncthreads();
+        const v4f out = (o4[0] + o4[1]) + (o4[2] + o4[3]);
+        if (grp < 2 && valid) {
+            float *dst = planes + (size_t)(y + 7) * Wp + px + 7;
+#pragma unroll
+            for (int j = 0; j < 4; j++) dst[(size_t)(4 * grp + j) * plane] = out[j];
+        }
     }
 }
 
@@ -192,39 +214,69 @@ __global__ void k_grad_hidden(const float *__restrict__ dK, const float *__restr
     dH[i] = g;
 }
 
-// dW1[j][k] = sum_s dH[s][j] * X[s][k]; one thread per weight, samples in chunks.
-__global__ void k_grad_w1(const float *__restrict__ dH, const unsigned short *__restrict__ X, int n,
-                          float *__restrict__ gW1) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= KH * KI) return;
-    const int j = i / KI, k = i % KI;
-    float g = 0.f;
-    for (int s = 0; s < n; s++) g += dH[(size_t)s * KH + j] * bf2f(X[(size_t)s * KI + k]);
-    gW1[i] = g;
+// Weight gradients, split over chunks of KEY_CHUNK samples: block c writes the partial sums of its
+// chunk to part[c * KEY_NGRAD ..] (dW1 64 x 192, then db1 64, dW2 8 x 64, db2 8), and
+// k_grad_reduce adds the chunks in order (deterministic).
+//   dW1 = dH^T . X on MFMA in FP32 (v_mfma_f32_16x16x4_f32): wave w owns hidden rows 16w..16w+15
+//   and all 12 column tiles; a K-step takes 4 samples (A: dH, B: X widened from BF16 exactly).
+//   db1, dW2, db2 on VALU, one output per thread.
+constexpr int KEY_CHUNK = 128;
+constexpr int KEY_NGRAD = KH * KI + KH + KO * KH + KO;
+
+__global__ __launch_bounds__(256) void k_grad_partial(const float *__restrict__ dH, const unsigned short *__restrict__ X,
+                                                     const float *__restrict__ Z, const float *__restrict__ dK, int n,
+                                                     float *__restrict__ part) {
+    const int c = blockIdx.x;
+    const int s0 = c * KEY_CHUNK, s1 = min(n, s0 + KEY_CHUNK);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, grp = lane >> 4;
+    float *P = part + (size_t)c * KEY_NGRAD;
+    v4f acc[KI / 16];
+#pragma unroll
+    for (int kt = 0; kt < KI / 16; kt++) acc[kt] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int s = s0; s < s1; s += 4) {
+        const int si = s + grp;
+        const bool ok = si < s1;
+        const float a = ok ? dH[(size_t)si * KH + wave * 16 + col] : 0.f;
+#pragma unroll
+        for (int kt = 0; kt < KI / 16; kt++) {
+            const float b = ok ? bf2f(X[(size_t)si * KI + kt * 16 + col]) : 0.f;
+            acc[kt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[kt], 0, 0, 0);
+        }
+    }
+    // D: lane holds rows (hidden) wave*16 + 4*grp + j, column kt*16 + col
+#pragma unroll
+    for (int kt = 0; kt < KI / 16; kt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) P[(size_t)(wave * 16 + 4 * grp + j) * KI + kt * 16 + col] = acc[kt][j];
+    for (int t = threadIdx.x; t < KH + KO * KH + KO; t += blockDim.x) {
+        float g = 0.f;
+        if (t < KH) {
+            for (int s = s0; s < s1; s++) g += dH[(size_t)s * KH + t];
+        } else if (t < KH + KO * KH) {
+            const int q = t - KH, o = q / KH, j = q % KH;
+            for (int s = s0; s < s1; s++) {
+                const float z = Z[(size_t)s * KH + j];
+                g += dK[(size_t)s * KO + o] * (z > 0.f ? z : 0.f);
+            }
+        } else {
+            const int o = t - KH - KO * KH;
+            for (int s = s0; s < s1; s++) g += dK[(size_t)s * KO + o];
+        }
+        P[KH * KI + t] = g;
+    }
 }
 
-// db1, dW2, db2 (one thread per parameter).
-__global__ void k_grad_small(const float *__restrict__ dH, const float *__restrict__ Z, const float *__restrict__ dK,
-                             int n, float *__restrict__ gb1, float *__restrict__ gW2, float *__restrict__ gb2) {
+__global__ void k_grad_reduce(const float *__restrict__ part, int nchunks, float *__restrict__ gW1,
+                              float *__restrict__ gb1, float *__restrict__ gW2, float *__restrict__ gb2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < KH) {
-        float g = 0.f;
-        for (int s = 0; s < n; s++) g += dH[(size_t)s * KH + i];
-        gb1[i] = g;
-    } else if (i < KH + KO * KH) {
-        const int q = i - KH, o = q / KH, j = q % KH;
-        float g = 0.f;
-        for (int s = 0; s < n; s++) {
-            const float z = Z[(size_t)s * KH + j];
-            g += dK[(size_t)s * KO + o] * (z > 0.f ? z : 0.f);
-        }
-        gW2[q] = g;
-    } else if (i < KH + KO * KH + KO) {
-        const int o = i - KH - KO * KH;
-        float g = 0.f;
-        for (int s = 0; s < n; s++) g += dK[(size_t)s * KO + o];
-        gb2[o] = g;
-    }
+    if (i >= KEY_NGRAD) return;
+    float g = 0.f;
+    for (int c = 0; c < nchunks; c++) g += part[(size_t)c * KEY_NGRAD + i];
+    if (i < KH * KI) gW1[i] = g;
+    else if (i < KH * KI + KH) gb1[i - KH * KI] = g;
+    else if (i < KH * KI + KH + KO * KH) gW2[i - KH * KI - KH] = g;
+    else gb2[i - KH * KI - KH - KO * KH] = g;
 }
 
 // Adam step on FP32 master parameters; refreshes the BF16 copy (when given).
@@ -256,8 +308,10 @@ int pt_key_forward_launch(const float *img, int h, int w, const unsigned short *
     const int ew = w - 7, eh = h - 7;
     if (ew <= 0 || eh <= 0) return pt_fail(PT_E_INVALID, "key: image smaller than a patch");
     const size_t plane = (size_t)(h + 7) * (w + 7);
-    hipLaunchKernelGGL(k_key_forward, dim3((ew + 63) / 64, std::min(eh, KEY_ROW_BLOCKS)), dim3(256), 0, s, img, h, w,
-                       w1, b1, w2, b2, planes, plane, w + 7);
+    // 16-patch work items strided over at most 2048 blocks of 4 waves
+    const long items = (long)((ew + 15) / 16) * eh;
+    const int blocks = (int)std::min<long>(2048, (items + 3) / 4);
+    hipLaunchKernelGGL(k_key_forward, dim3(blocks), dim3(256), 0, s, img, h, w, w1, b1, w2, b2, planes, plane, w + 7);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }
@@ -278,8 +332,9 @@ int pt_key_train_step(const unsigned short *X, int nt, float margin, const KeyPa
     hipLaunchKernelGGL(k_triplet_grad, dim3(pt_blocks(nt, 256)), dim3(256), 0, s, Wk.K, nt, margin, Wk.dK, Wk.loss);
     hipLaunchKernelGGL(k_grad_hidden, dim3(pt_blocks((long)n * KH, 256)), dim3(256), 0, s, Wk.dK, Wk.Z, n, P.w2bf,
                        Wk.dH);
-    hipLaunchKernelGGL(k_grad_w1, dim3(pt_blocks(KH * KI, 128)), dim3(128), 0, s, Wk.dH, X, n, Wk.gW1);
-    hipLaunchKernelGGL(k_grad_small, dim3(pt_blocks(KH + KO * KH + KO, 64)), dim3(64), 0, s, Wk.dH, Wk.Z, Wk.dK, n,
+    const int nchunks = pt_key_grad_chunks(n);
+    hipLaunchKernelGGL(k_grad_partial, dim3(nchunks), dim3(256), 0, s, Wk.dH, X, Wk.Z, Wk.dK, n, Wk.part);
+    hipLaunchKernelGGL(k_grad_reduce, dim3(pt_blocks(KEY_NGRAD, 256)), dim3(256), 0, s, Wk.part, nchunks, Wk.gW1,
                        Wk.gb1, Wk.gW2, Wk.gb2);
     if (lr > 0.f) {
         const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
@@ -296,6 +351,9 @@ int pt_key_train_step(const unsigned short *X, int nt, float margin, const KeyPa
     PT_LAUNCH_CHECK();
     return PT_OK;
 }
+
+int pt_key_grad_chunks(int nsamples) { return (nsamples + KEY_CHUNK - 1) / KEY_CHUNK; }
+size_t pt_key_grad_part_floats(int nsamples) { return (size_t)pt_key_grad_chunks(nsamples) * KEY_NGRAD; }
 
 int pt_key_to_bf16_launch(const float *in, int n, unsigned short *out, hipStream_t s) {
     hipLaunchKernelGGL(k_to_bf16, dim3(pt_blocks(n, 256)), dim3(256), 0, s, in, n, out);

@@ -189,7 +189,10 @@ struct KeyWorkDev {            // per-batch training buffers (n = 3 x triplets s
     float *Z, *K, *dK, *dH;
     float *gW1, *gb1, *gW2, *gb2;
     float *loss;
+    float *part;               // per-chunk partial gradients, pt_key_grad_part_floats(n) floats
 };
+int pt_key_grad_chunks(int nsamples);
+size_t pt_key_grad_part_floats(int nsamples);
 struct KeyNetDev {             // BF16 weights and FP32 biases of one key network
     const unsigned short *w1 = nullptr;
     const float *b1 = nullptr;
