@@ -245,6 +245,32 @@ int pt_batch_size(const pt_batch *bt);
 pt_table *pt_batch_table(pt_batch *bt, int i);
 int pt_batch_destroy(pt_batch *bt);
 
+/* ---- Exhaustive nearest neighbours (the reference names LOOKUP_ALGO_BRUTE, patchtable.h:230,
+ * "-lookup_algo brute" patchtable.cpp:276, but implements no search for it).
+ * pt_brute_lookup_dev: for each query patch of a (device RGB, ah x aw x 3), the nearest of every
+ *   8x8 patch of b (device RGB, bh x bw x 3). Candidate scoring is the dense contraction
+ *   |a|^2 + |b|^2 - 2 a.b over the 192-dim patch vectors (centred by -0.5) on MFMA: BF16 (round to
+ *   nearest even) or FP8 e4m3 with a per-patch scale (max |v| / 448), FP32 accumulation; each
+ *   query keeps its 8 best approximate scores (every lane of the search keeps the 2 best of the
+ *   database columns it sees, merged per query), which are rescored with the exact FP32 SSD
+ *   (patch_dist_exact's lane order, patchtable.h:330-373): the reported distance is exact.
+ *   qpos_dev: nq query positions XY_TO_INT(x, y) in patch coordinates (checked), or NULL for every
+ *   query patch in raster order (nq = (ah-7)*(aw-7)). Outputs (device): pos_out[nq] (XY_TO_INT of
+ *   the match, patch coordinates), dist_out[nq] (exact SSD); optional cand_out[nq*8] (database
+ *   patch indices y*(bw-7)+x of the survivors, best approximate score first) and
+ *   score_out[nq*8] (|b|^2 - 2 a.b). stage_ms (optional, [3]): patch vectors, MFMA search,
+ *   rescoring. Runs on stream 0 and returns when done.
+ * pt_brute_exact_f64_dev: validation only. The exact nearest neighbour of each query patch in
+ *   FP64 (sum over the 192 elements in row-major order, fused multiply-adds; ties to the smaller
+ *   database index): pos_out[nq], dist_out[nq] (double). */
+#define PT_BRUTE_BF16 0
+#define PT_BRUTE_FP8  1
+int pt_brute_lookup_dev(const float *a_dev, int ah, int aw, const float *b_dev, int bh, int bw,
+                        const int32_t *qpos_dev, int nq, int precision, int32_t *pos_out_dev, float *dist_out_dev,
+                        int32_t *cand_out_dev, float *score_out_dev, double *stage_ms);
+int pt_brute_exact_f64_dev(const float *a_dev, int ah, int aw, const float *b_dev, int bh, int bw,
+                           const int32_t *qpos_dev, int nq, int32_t *pos_out_dev, double *dist_out_dev);
+
 /* ---- Learned patch key (no counterpart in the reference; an alternative to the GCK
  * descriptor as the table index). A 192 -> 64 -> 8 MLP with ReLU maps an 8x8 RGB patch,
  * x[dy*24 + dx*3 + c] = img(y+dy, x+dx, c) - 0.5, to an 8-dim key. Weights are BF16 for the

@@ -1991,6 +1991,36 @@ int orc_key_gradients(const float *W1, const float *B1, const float *W2, const f
     return 0;
 }
 
+// Exact nearest neighbour in FP64 of each query patch (positions XY_TO_INT in patch coordinates)
+// among every patch of b: sum over the 192 elements in row-major order with fused multiply-adds,
+// ties to the smaller database index y*(bw-7)+x. pos_out is XY_TO_INT of the match.
+int orc_brute_f64(const float *a, int aw, const int32_t *qpos, int nq, const float *b, int bh, int bw,
+                  int32_t *pos_out, double *dist_out) {
+    const int dew = bw - 7, deh = bh - 7;
+    for (int q = 0; q < nq; q++) {
+        const int qx = INT_TO_X(qpos[q]), qy = INT_TO_Y(qpos[q]);
+        double qv[192];
+        for (int k = 0; k < 192; k++) qv[k] = a[((size_t)(qy + k / 24) * aw + qx) * 3 + k % 24];
+        double best = 1e300;
+        int bidx = -1;
+        for (int dy = 0; dy < deh; dy++)
+            for (int dx = 0; dx < dew; dx++) {
+                double acc = 0.0;
+                for (int k = 0; k < 192; k++) {
+                    const double d = qv[k] - (double)b[((size_t)(dy + k / 24) * bw + dx) * 3 + k % 24];
+                    acc = std::fma(d, d, acc);
+                }
+                if (acc < best) {
+                    best = acc;
+                    bidx = dy * dew + dx;
+                }
+            }
+        pos_out[q] = XY_TO_INT(bidx % dew, bidx / dew);
+        dist_out[q] = best;
+    }
+    return 0;
+}
+
 int orc_key_round_weights(int on) {
     g_key_round_weights = on != 0;
     return 0;

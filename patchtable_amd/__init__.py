@@ -21,7 +21,8 @@ NNF_Y = 1
 NNF_DIST = 2
 
 __all__ = ["PatchTableParams", "PatchTable", "PatchTableBatch", "PatchKey", "PatchTableError", "NNF_X",
-           "NNF_Y", "NNF_DIST", "synth_pair", "load_color_image", "save_color_image"]
+           "NNF_Y", "NNF_DIST", "synth_pair", "load_color_image", "save_color_image", "brute_lookup",
+           "brute_exact_f64"]
 
 
 class PatchTableParams:
@@ -385,6 +386,51 @@ class PatchKey:
         check(load().pt_key_train(self._k, ctypes.byref(p.raw), n, A, B, H, W, int(tpp), int(steps), int(batch),
                                   float(lr), float(margin), int(seed), _ffi_f32(losses)))
         return losses[:steps]
+
+
+def _queries(a, qpos):
+    import torch
+    if qpos is None:
+        return None, (int(a.shape[0]) - 7) * (int(a.shape[1]) - 7)
+    if not (_is_device_tensor(qpos) and qpos.dtype == torch.int32):
+        raise PatchTableError("query positions must be a CUDA int32 tensor of XY_TO_INT values")
+    return ctypes.c_void_p(qpos.data_ptr()), int(qpos.numel())
+
+
+def brute_lookup(a, b, qpos=None, precision="bf16"):
+    """Nearest neighbour of each query patch of a among all patches of b (torch CUDA tensors,
+    HWC float32): MFMA candidate scoring in BF16 or FP8, exact FP32 rescoring of the 8 best.
+    qpos: CUDA int32 XY_TO_INT query positions, or None for every query patch (raster order).
+    Returns dict(pos, dist, cand, score, stage_ms) with pos/dist/cand/score CUDA tensors."""
+    import torch
+    prec = {"bf16": _ffi.PT_BRUTE_BF16, "fp8": _ffi.PT_BRUTE_FP8}[precision]
+    qp, nq = _queries(a, qpos)
+    dev = a.device
+    pos = torch.empty(nq, dtype=torch.int32, device=dev)
+    dist = torch.empty(nq, dtype=torch.float32, device=dev)
+    cand = torch.empty((nq, 8), dtype=torch.int32, device=dev)
+    score = torch.empty((nq, 8), dtype=torch.float32, device=dev)
+    ms = (ctypes.c_double * 3)()
+    torch.cuda.synchronize()
+    check(load().pt_brute_lookup_dev(ctypes.c_void_p(a.data_ptr()), int(a.shape[0]), int(a.shape[1]),
+                                     ctypes.c_void_p(b.data_ptr()), int(b.shape[0]), int(b.shape[1]), qp, nq, prec,
+                                     ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(dist.data_ptr()),
+                                     ctypes.c_void_p(cand.data_ptr()), ctypes.c_void_p(score.data_ptr()), ms))
+    return dict(pos=pos, dist=dist, cand=cand, score=score,
+                stage_ms=dict(vectors=ms[0], search=ms[1], rescore=ms[2]))
+
+
+def brute_exact_f64(a, b, qpos=None):
+    """Validation: exact FP64 nearest neighbours (pos XY_TO_INT, dist float64), CUDA tensors."""
+    import torch
+    qp, nq = _queries(a, qpos)
+    pos = torch.empty(nq, dtype=torch.int32, device=a.device)
+    dist = torch.empty(nq, dtype=torch.float64, device=a.device)
+    torch.cuda.synchronize()
+    check(load().pt_brute_exact_f64_dev(ctypes.c_void_p(a.data_ptr()), int(a.shape[0]), int(a.shape[1]),
+                                        ctypes.c_void_p(b.data_ptr()), int(b.shape[0]), int(b.shape[1]), qp, nq,
+                                        ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(dist.data_ptr())))
+    return pos, dist
 
 
 def _ffi_f32(a):

@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(HERE, "libpatchtable_hip.so")
 
 PT_MAX_DIMS = 64
 PT_OK = 0
+PT_BRUTE_BF16 = 0
+PT_BRUTE_FP8 = 1
 
 
 class PtParams(ctypes.Structure):
@@ -162,17 +164,28 @@ SIGNATURES = {
     "pt_key_train": (_int, [_vp, _P(PtParams), _int, _P(_vp), _P(_vp), _P(_int), _P(_int), _int, _int, _int,
                             ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _f32p]),
     "pt_table_create_keyed": (_int, [_P(PtParams), _vp, _vp, _int, _int, _P(_vp)]),
+    "pt_brute_lookup_dev": (_int, [_vp, _int, _int, _vp, _int, _int, _vp, _int, _int, _vp, _vp, _vp, _vp, _f64p]),
+    "pt_brute_exact_f64_dev": (_int, [_vp, _int, _int, _vp, _int, _int, _vp, _int, _vp, _vp]),
 }
 
 _lib = None
 
 
 def load():
-    """Loads the in-tree HIP library (raises OSError if it has not been built)."""
+    """Loads the in-tree HIP library (raises OSError if it has not been built).
+
+    PyTorch-ROCm ships its own HIP runtime. Two HIP runtimes in one process do not share the
+    GPU: the one initialised second reports no device. Importing torch first makes the dynamic
+    loader bind the library's libamdhip64.so.7 to torch's already loaded copy, so both use one
+    runtime whichever initialises first."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise OSError(f"{LIB_PATH} is missing: run patchtable_amd/build.py (or __graft_entry__.build())")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

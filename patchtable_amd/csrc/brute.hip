@@ -1,0 +1,497 @@
+// Exhaustive nearest-neighbour search over every database patch, on gfx950.
+//
+// The reference names LOOKUP_ALGO_BRUTE (patchtable.h:230, "-lookup_algo brute",
+// patchtable.cpp:276) but has no search behind it; this file is the dense form of candidate
+// scoring: for query patch a and database patch b (192-dim RGB vectors, centred by -0.5),
+//     |a - b|^2 = |a|^2 + |b|^2 - 2 a.b,
+// so ranking all of b for a block of queries is a [queries x 192] x [192 x database] contraction.
+//
+//   k_patch_vectors   patch vectors in BF16 (round to nearest even) or FP8 e4m3 with a per-patch
+//                     scale (max |v| / 448), and |v|^2 of the rounded vector in FP32.
+//   k_brute_mfma      one block = 256 queries (4 waves x 4 row tiles of 16); the database streams
+//                     through two LDS buffers in chunks of 64 patches (the next chunk's global
+//                     loads overlap the current chunk's MFMAs); a.b on v_mfma_f32_16x16x32_bf16 (or
+//                     v_mfma_f32_16x16x32_fp8_fp8 times the two scales), FP32 accumulate; each
+//                     lane keeps the 2 best scores |b|^2 - 2 a.b of its 16 query rows over the
+//                     database columns it sees; at the end the 32 lane candidates of a query are
+//                     merged to its 8 best.
+//   k_rescore         exact FP32 SSD (the reference's lane order, patch_dist_exact,
+//                     patchtable.h:330-373) of the 8 survivors; the smallest is the match, so no
+//                     low-precision error reaches the reported distance.
+//   k_brute_f64 / k_f64_reduce
+//                     validation only: the exact nearest neighbour in FP64 (sum over the 192
+//                     elements in row-major order with fused multiply-adds), ties to the smaller
+//                     database index.
+#include "pt_device.h"
+#include "pt_kernels.h"
+
+#include <algorithm>
+
+namespace {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int DIM = 192;
+constexpr int QPB = 256;        // queries per block
+constexpr int DBC = 64;         // database patches per LDS chunk
+constexpr int KL = 2;           // candidates a lane keeps per query row
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// Patch p: positions pos[p] = XY_TO_INT(x, y) (patch coordinates), or raster order over an
+// ew-wide patch grid when pos is NULL.
+__device__ __forceinline__ void patch_xy(const int32_t *pos, long p, int ew, int &x, int &y) {
+    if (pos) {
+        x = PT_INT_TO_X(pos[p]);
+        y = PT_INT_TO_Y(pos[p]);
+    } else {
+        x = (int)(p % ew);
+        y = (int)(p / ew);
+    }
+}
+
+// One wave per patch; lane l holds elements l, l + 64, l + 128 (k = dy*24 + dx*3 + c).
+// fp8 = 0: vec is BF16 bits (2 bytes per element); fp8 = 1: vec is e4m3 bytes, scale[p] set.
+__global__ __launch_bounds__(256) void k_patch_vectors(const float *__restrict__ img, int w, const int32_t *__restrict__ pos,
+                                                      long n, int ew, int fp8, unsigned char *__restrict__ vec,
+                                                      float *__restrict__ norm, float *__restrict__ scale) {
+    const long p = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= n) return;
+    int x, y;
+    patch_xy(pos, p, ew, x, y);
+    float v[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int k = lane + 64 * i;
+        v[i] = img[((size_t)(y + k / 24) * w + x) * 3 + k % 24] - 0.5f;
+    }
+    float s = 0.f;
+    if (!fp8) {
+        unsigned short *o = (unsigned short *)vec + (size_t)p * DIM;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const unsigned short b = f2bf(v[i]);
+            o[lane + 64 * i] = b;
+            const float r = bf2f(b);
+            s += r * r;
+        }
+    } else {
+        float m = fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), fabsf(v[2])));
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        const float sc = m > 0.f ? m / 448.f : 1.f;
+        const float inv = 1.f / sc;
+        unsigned char *o = vec + (size_t)p * DIM;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            // v_cvt_pk_fp8_f32 converts two values; the first of the pair is the one kept
+            const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(v[i] * inv, 0.f, 0, false);
+            o[lane + 64 * i] = (unsigned char)(packed & 0xff);
+            const float r = __builtin_amdgcn_cvt_f32_fp8(packed, 0) * sc;
+            s += r * r;
+        }
+        if (lane == 0) scale[p] = sc;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) norm[p] = s;
+}
+
+// Keeps the KL smallest (score, index) pairs; equal scores keep the earlier index.
+__device__ __forceinline__ void keep_best(float s, int idx, float (&bs)[KL], int (&bi)[KL]) {
+    if (s < bs[1]) {
+        if (s < bs[0]) {
+            bs[1] = bs[0];
+            bi[1] = bi[0];
+            bs[0] = s;
+            bi[0] = idx;
+        } else {
+            bs[1] = s;
+            bi[1] = idx;
+        }
+    }
+}
+
+// FP8 = false: BF16 vectors (qvec/dvec hold 2-byte elements); true: e4m3 bytes with scales.
+// cand/cscore: nq x 8 database indices and scores |b|^2 - 2 a.b (ascending; -1 / +inf when the
+// database has fewer patches).
+template <bool FP8>
+__global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__restrict__ qvec, const float *__restrict__ qscale,
+                                                   int nq, const unsigned char *__restrict__ dvec,
+                                                   const float *__restrict__ dnorm, const float *__restrict__ dscale,
+                                                   int nd, int32_t *__restrict__ cand, float *__restrict__ cscore) {
+    constexpr int EB = FP8 ? 1 : 2;                 // bytes per element
+    constexpr int ROWB = DIM * EB + 16;             // LDS row pitch (bytes), padded
+    // two database chunks while streaming (the next one is written while the current one is
+    // read); at the end the candidate lists of two waves at a time
+    constexpr int MERGE = 2 * 64 * 16 * KL;
+    constexpr int SMEM = 2 * DBC * ROWB > MERGE * 8 ? 2 * DBC * ROWB : MERGE * 8;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
+    __shared__ float snorm[2][DBC];
+    __shared__ float sscale[2][DBC];
+    float (*mscore)[64][16 * KL] = (float (*)[64][16 * KL])smem;
+    int (*midx)[64][16 * KL] = (int (*)[64][16 * KL])(smem + MERGE * 4);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, grp = lane >> 4;
+    const int qw0 = blockIdx.x * QPB + wave * 64;   // first query of this wave
+    // A fragments: query row tile t, K block kb: 8 elements k = kb*32 + 8*grp .. + 7 of query
+    // qw0 + 16t + col (clamped to the last query; such rows are never written)
+    v4i afrag[4][DIM / 32];   // 16 bytes per fragment (8 BF16 or 8 FP8 in the low half)
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int q = min(qw0 + 16 * t + col, nq - 1);
+#pragma unroll
+        for (int kb = 0; kb < DIM / 32; kb++) {
+            const unsigned char *src = qvec + (size_t)q * DIM * EB + (size_t)(kb * 32 + 8 * grp) * EB;
+            if (FP8) {
+                const int2 v = *(const int2 *)src;
+                afrag[t][kb] = v4i{v.x, v.y, 0, 0};
+            } else {
+                afrag[t][kb] = *(const v4i *)src;
+            }
+        }
+    }
+    // scales of the query rows this lane's accumulators hold (rows 16t + 4grp + j)
+    float rs[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) rs[t][j] = FP8 ? qscale[min(qw0 + 16 * t + 4 * grp + j, nq - 1)] : 1.f;
+    float bs[16][KL];
+    int bi[16][KL];
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+#pragma unroll
+        for (int m = 0; m < KL; m++) {
+            bs[r][m] = __builtin_inff();
+            bi[r][m] = -1;
+        }
+    // staging: thread i moves 16-byte pieces i, i + 256, ... of a chunk (zero rows past the end)
+    constexpr int PIECES = DBC * DIM * EB / 16, PER = PIECES / 256, RP = DIM * EB / 16;
+    static_assert(PIECES % 256 == 0, "chunk pieces");
+    v4i stage[PER];
+    float stn = 0.f, sts = 1.f;
+    auto fetch = [&](int d0) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = threadIdx.x + 256 * k, row = i / RP, c16 = i % RP, d = d0 + row;
+            stage[k] = d < nd ? *(const v4i *)(dvec + (size_t)d * DIM * EB + c16 * 16) : v4i{0, 0, 0, 0};
+        }
+        if (threadIdx.x < DBC) {
+            const int d = d0 + threadIdx.x;
+            stn = d < nd ? dnorm[d] : __builtin_inff();
+            sts = FP8 && d < nd ? dscale[d] : 1.f;
+        }
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = threadIdx.x + 256 * k, row = i / RP, c16 = i % RP;
+            *(v4i *)(smem + (size_t)buf * DBC * ROWB + row * ROWB + c16 * 16) = stage[k];
+        }
+        if (threadIdx.x < DBC) {
+            snorm[buf][threadIdx.x] = stn;
+            sscale[buf][threadIdx.x] = sts;
+        }
+    };
+    fetch(0);
+    put(0);
+    __syncthreads();
+    for (int d0 = 0, cur = 0; d0 < nd; d0 += DBC, cur ^= 1) {
+        const bool more = d0 + DBC < nd;
+        if (more) fetch(d0 + DBC);   // loads in flight during this chunk's MFMAs
+        const unsigned char *sdb = smem + (size_t)cur * DBC * ROWB;
+#pragma unroll 1
+        for (int c = 0; c < DBC / 16; c++) {
+            v4f acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+            const unsigned char *brow = sdb + (c * 16 + col) * ROWB;
+#pragma unroll
+            for (int kb = 0; kb < DIM / 32; kb++) {
+                if (FP8) {
+                    const int2 bv = *(const int2 *)(brow + (kb * 32 + 8 * grp));
+                    const long bl = ((long)(unsigned)bv.y << 32) | (unsigned)bv.x;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const long al = ((long)(unsigned)afrag[t][kb][1] << 32) | (unsigned)afrag[t][kb][0];
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(al, bl, acc[t], 0, 0, 0);
+                    }
+                } else {
+                    const v4i bv = *(const v4i *)(brow + (kb * 32 + 8 * grp) * 2);
+                    const bf8 bb = __builtin_bit_cast(bf8, bv);
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, afrag[t][kb]), bb,
+                                                                        acc[t], 0, 0, 0);
+                }
+            }
+            // D: lane holds query rows 16t + 4grp + j against database column c*16 + col
+            const int dcol = c * 16 + col;
+            const int didx = d0 + dcol;
+            const float dn = snorm[cur][dcol];
+            const float dsc = sscale[cur][dcol];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float dot = FP8 ? acc[t][j] * (rs[t][j] * dsc) : acc[t][j];
+                    keep_best(dn - 2.f * dot, didx, bs[4 * t + j], bi[4 * t + j]);
+                }
+        }
+        if (more) put(cur ^ 1);   // the other buffer was last read before the previous barrier
+        __syncthreads();
+    }
+    // merge, waves 0-1 then waves 2-3: lane candidates of query row 16t + 4grp + j go to
+    // mscore[wave & 1][row][col*KL + m]; then lane l takes query qw0 + l and keeps the 8 best of
+    // its 32 candidates (ties: smaller index)
+    const int q = qw0 + lane;
+    float ts[8];
+    int ti[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        ts[i] = __builtin_inff();
+        ti[i] = -1;
+    }
+#pragma unroll 1
+    for (int phase = 0; phase < 2; phase++) {
+        __syncthreads();   // the database chunk, or the previous phase's lists, are no longer read
+        if ((wave >> 1) == phase) {
+            const int mw = wave & 1;
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int m = 0; m < KL; m++) {
+                        mscore[mw][16 * t + 4 * grp + j][col * KL + m] = bs[4 * t + j][m];
+                        midx[mw][16 * t + 4 * grp + j][col * KL + m] = bi[4 * t + j][m];
+                    }
+        }
+        __syncthreads();
+        if ((wave >> 1) == phase && q < nq) {
+            const int mw = wave & 1;
+            for (int e = 0; e < 16 * KL; e++) {
+                float s = mscore[mw][lane][e];
+                int id = midx[mw][lane][e];
+                if (id < 0) continue;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const bool better = s < ts[i] || (s == ts[i] && (ti[i] < 0 || id < ti[i]));
+                    if (better) {
+                        const float fs = ts[i];
+                        const int fi = ti[i];
+                        ts[i] = s;
+                        ti[i] = id;
+                        s = fs;
+                        id = fi;
+                        if (id < 0) break;
+                    }
+                }
+            }
+        }
+    }
+    if (q >= nq) return;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        cand[(size_t)q * 8 + i] = ti[i];
+        cscore[(size_t)q * 8 + i] = ts[i];
+    }
+}
+
+// Exact SSD of the 8 survivors of each query; the smallest (ties: better approximate rank) is
+// the match. out_pos = XY_TO_INT(x, y) in patch coordinates.
+__global__ void k_rescore(const float *__restrict__ a, int aw, const int32_t *__restrict__ qpos, int nq, int qew,
+                          const float *__restrict__ b, int bw, int dew, const int32_t *__restrict__ cand,
+                          int32_t *__restrict__ out_pos, float *__restrict__ out_dist) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    int qx, qy;
+    patch_xy(qpos, q, qew, qx, qy);
+    const float *ap = a + ((size_t)qy * aw + qx) * 3;
+    float best = __builtin_inff();
+    int bpos = -1;
+    for (int i = 0; i < 8; i++) {
+        const int d = cand[(size_t)q * 8 + i];
+        if (d < 0) continue;
+        const int dx = d % dew, dy = d / dew;
+        const float s = pt_patch_ssd(ap, 3 * aw, b + ((size_t)dy * bw + dx) * 3, 3 * bw);
+        if (s < best) {
+            best = s;
+            bpos = PT_XY_TO_INT(dx, dy);
+        }
+    }
+    out_pos[q] = bpos;
+    out_dist[q] = best;
+}
+
+// ---- FP64 exact nearest neighbour (validation) -------------------------------------------------
+constexpr int F64Q = 16;   // queries per block
+
+__global__ __launch_bounds__(256) void k_brute_f64(const float *__restrict__ a, int aw, const int32_t *__restrict__ qpos,
+                                                  int nq, int qew, const float *__restrict__ b, int bw, int dew,
+                                                  long nd, int per_block, double *__restrict__ pdist,
+                                                  int32_t *__restrict__ pidx) {
+    __shared__ double sq[F64Q][DIM];
+    __shared__ double rd[F64Q][4];
+    __shared__ int ri[F64Q][4];
+    const int q0 = blockIdx.y * F64Q;
+    for (int i = threadIdx.x; i < F64Q * DIM; i += blockDim.x) {
+        const int qi = i / DIM, k = i % DIM;
+        const int q = min(q0 + qi, nq - 1);
+        int qx, qy;
+        patch_xy(qpos, q, qew, qx, qy);
+        sq[qi][k] = (double)a[((size_t)(qy + k / 24) * aw + qx) * 3 + k % 24];
+    }
+    __syncthreads();
+    double best[F64Q];
+    int bidx[F64Q];
+#pragma unroll
+    for (int qi = 0; qi < F64Q; qi++) {
+        best[qi] = 1e300;
+        bidx[qi] = -1;
+    }
+    const long dbeg = (long)blockIdx.x * per_block, dend = min(nd, dbeg + per_block);
+    for (long d = dbeg + threadIdx.x; d < dend; d += blockDim.x) {
+        const int dx = (int)(d % dew), dy = (int)(d / dew);
+        double acc[F64Q];
+#pragma unroll
+        for (int qi = 0; qi < F64Q; qi++) acc[qi] = 0.0;
+        for (int r = 0; r < 8; r++) {
+            const float *br = b + ((size_t)(dy + r) * bw + dx) * 3;
+            double bv[24];
+#pragma unroll
+            for (int e = 0; e < 24; e++) bv[e] = (double)br[e];
+#pragma unroll
+            for (int qi = 0; qi < F64Q; qi++)
+#pragma unroll
+                for (int e = 0; e < 24; e++) {
+                    const double df = sq[qi][r * 24 + e] - bv[e];
+                    acc[qi] = __builtin_fma(df, df, acc[qi]);
+                }
+        }
+#pragma unroll
+        for (int qi = 0; qi < F64Q; qi++)
+            if (acc[qi] < best[qi]) {   // d increases along the loop: ties keep the smaller index
+                best[qi] = acc[qi];
+                bidx[qi] = (int)d;
+            }
+    }
+    // minimum over the wave (ties: smaller index), then over the 4 waves
+#pragma unroll
+    for (int qi = 0; qi < F64Q; qi++) {
+        double v = best[qi];
+        int ix = bidx[qi];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(v, off);
+            const int oi = __shfl_xor(ix, off);
+            if (oi >= 0 && (ix < 0 || ov < v || (ov == v && oi < ix))) {
+                v = ov;
+                ix = oi;
+            }
+        }
+        if ((threadIdx.x & 63) == 0) {
+            rd[qi][threadIdx.x >> 6] = v;
+            ri[qi][threadIdx.x >> 6] = ix;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < F64Q) {
+        const int qi = threadIdx.x;
+        double bd = 1e300;
+        int bx = -1;
+        for (int t = 0; t < 4; t++) {
+            const double v = rd[qi][t];
+            const int ix = ri[qi][t];
+            if (ix >= 0 && (v < bd || (v == bd && ix < bx))) {
+                bd = v;
+                bx = ix;
+            }
+        }
+        const int q = q0 + qi;
+        if (q < nq) {
+            pdist[(size_t)blockIdx.x * nq + q] = bd;
+            pidx[(size_t)blockIdx.x * nq + q] = bx;
+        }
+    }
+}
+
+__global__ void k_f64_reduce(const double *__restrict__ pdist, const int32_t *__restrict__ pidx, int nblk, int nq,
+                             int dew, int32_t *__restrict__ out_pos, double *__restrict__ out_dist) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    double bd = 1e300;
+    int bx = -1;
+    for (int k = 0; k < nblk; k++) {
+        const double v = pdist[(size_t)k * nq + q];
+        const int ix = pidx[(size_t)k * nq + q];
+        if (ix >= 0 && (v < bd || (v == bd && ix < bx))) {
+            bd = v;
+            bx = ix;
+        }
+    }
+    out_pos[q] = bx < 0 ? -1 : PT_XY_TO_INT(bx % dew, bx / dew);
+    out_dist[q] = bd;
+}
+
+}  // namespace
+
+int pt_brute_vectors_launch(const float *img, int w, const int32_t *pos, long n, int ew, int fp8, unsigned char *vec,
+                            float *norm, float *scale, hipStream_t s) {
+    if (n <= 0) return PT_OK;
+    hipLaunchKernelGGL(k_patch_vectors, dim3(pt_blocks(n, 4)), dim3(256), 0, s, img, w, pos, n, ew, fp8, vec, norm,
+                       scale);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_brute_mfma_launch(const unsigned char *qvec, const float *qscale, int nq, const unsigned char *dvec,
+                         const float *dnorm, const float *dscale, int nd, int fp8, int32_t *cand, float *cscore,
+                         hipStream_t s) {
+    if (nq <= 0) return PT_OK;
+    if (fp8)
+        hipLaunchKernelGGL(k_brute_mfma<true>, dim3(pt_blocks(nq, QPB)), dim3(256), 0, s, qvec, qscale, nq, dvec, dnorm,
+                           dscale, nd, cand, cscore);
+    else
+        hipLaunchKernelGGL(k_brute_mfma<false>, dim3(pt_blocks(nq, QPB)), dim3(256), 0, s, qvec, qscale, nq, dvec,
+                           dnorm, dscale, nd, cand, cscore);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_brute_rescore_launch(const float *a, int aw, const int32_t *qpos, int nq, int qew, const float *b, int bw,
+                            int dew, const int32_t *cand, int32_t *out_pos, float *out_dist, hipStream_t s) {
+    if (nq <= 0) return PT_OK;
+    hipLaunchKernelGGL(k_rescore, dim3(pt_blocks(nq, 256)), dim3(256), 0, s, a, aw, qpos, nq, qew, b, bw, dew, cand,
+                       out_pos, out_dist);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_brute_f64_blocks(long nd, int *per_block) {
+    const int nblk = (int)std::min<long>(64, (nd + 255) / 256);
+    *per_block = (int)((nd + nblk - 1) / nblk);
+    return nblk;
+}
+
+int pt_brute_f64_launch(const float *a, int aw, const int32_t *qpos, int nq, int qew, const float *b, int bw, int dew,
+                        long nd, double *pdist, int32_t *pidx, int32_t *out_pos, double *out_dist, hipStream_t s) {
+    if (nq <= 0) return PT_OK;
+    int per = 0;
+    const int nblk = pt_brute_f64_blocks(nd, &per);
+    hipLaunchKernelGGL(k_brute_f64, dim3(nblk, (nq + F64Q - 1) / F64Q), dim3(256), 0, s, a, aw, qpos, nq, qew, b, bw,
+                       dew, nd, per, pdist, pidx);
+    hipLaunchKernelGGL(k_f64_reduce, dim3(pt_blocks(nq, 256)), dim3(256), 0, s, pdist, pidx, nblk, nq, dew, out_pos,
+                       out_dist);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}

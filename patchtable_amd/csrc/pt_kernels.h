@@ -48,8 +48,11 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
-        if (hipMalloc((void **)&p, (count ? count : 1) * sizeof(T)) != hipSuccess)
-            return pt_failf(PT_E_NOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+        const hipError_t e = hipMalloc((void **)&p, (count ? count : 1) * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return pt_failf(PT_E_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        }
         n = count;
         return PT_OK;
     }
@@ -199,6 +202,18 @@ struct KeyNetDev {             // BF16 weights and FP32 biases of one key networ
     const unsigned short *w2 = nullptr;
     const float *b2 = nullptr;
 };
+// brute.hip: exhaustive search (MFMA candidate scoring + exact rescoring; FP64 validation).
+int pt_brute_vectors_launch(const float *img, int w, const int32_t *pos, long n, int ew, int fp8, unsigned char *vec,
+                            float *norm, float *scale, hipStream_t s);
+int pt_brute_mfma_launch(const unsigned char *qvec, const float *qscale, int nq, const unsigned char *dvec,
+                         const float *dnorm, const float *dscale, int nd, int fp8, int32_t *cand, float *cscore,
+                         hipStream_t s);
+int pt_brute_rescore_launch(const float *a, int aw, const int32_t *qpos, int nq, int qew, const float *b, int bw,
+                            int dew, const int32_t *cand, int32_t *out_pos, float *out_dist, hipStream_t s);
+int pt_brute_f64_blocks(long nd, int *per_block);
+int pt_brute_f64_launch(const float *a, int aw, const int32_t *qpos, int nq, int qew, const float *b, int bw, int dew,
+                        long nd, double *pdist, int32_t *pidx, int32_t *out_pos, double *out_dist, hipStream_t s);
+
 struct pt_key;
 int pt_key_snapshot(const pt_key *k, unsigned short *w1, float *b1, unsigned short *w2, float *b2, hipStream_t s);
 int pt_key_forward_launch(const float *img, int h, int w, const unsigned short *w1, const float *b1,

@@ -63,6 +63,7 @@ def lib():
             "orc_build_keyed": (ctypes.c_void_p, [_P(PtParams), _f32p, _int, _int, _f32p]),
             "orc_lookup_parallel_keyed": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f32p, _f64p]),
             "orc_lookup_raster_keyed": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f32p, _f64p]),
+            "orc_brute_f64": (_int, [_f32p, _int, _i32p, _int, _f32p, _int, _int, _i32p, _f64p]),
             "orc_key_round_weights": (_int, [_int]),
             "orc_key_loss": (ctypes.c_double, [_f64p, _f64p, _f64p, _f64p, _f32p, _f32p, _int, _i32p, _int,
                                                ctypes.c_double]),
@@ -216,6 +217,18 @@ def key_forward(w1, b1, w2, b2, img):
     ws = [np.ascontiguousarray(x, dtype=np.float32) for x in (w1, b1, w2, b2)]
     lib().orc_key_forward(*[f32(x) for x in ws], f32(img), h, w, f32(out))
     return out
+
+
+def brute_f64(a, b, qpos):
+    """(positions XY_TO_INT, FP64 distances) of the exact nearest neighbours of the query patches
+    at qpos (XY_TO_INT, patch coordinates)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    q = np.ascontiguousarray(qpos, dtype=np.int32)
+    pos = np.empty(len(q), np.int32)
+    dist = np.empty(len(q), np.float64)
+    lib().orc_brute_f64(f32(a), a.shape[1], i32(q), len(q), f32(b), b.shape[0], b.shape[1], i32(pos), f64(dist))
+    return pos, dist
 
 
 def key_loss(w1, b1, w2, b2, a, b, pos, margin):
