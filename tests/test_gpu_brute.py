@@ -108,3 +108,27 @@ def test_rejects_positions_outside_the_grid():
     bad = torch.tensor([xy(33, 0)], dtype=torch.int32).cuda()
     with pytest.raises(PatchTableError, match="outside"):
         brute_lookup(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), bad)
+
+
+@pytest.mark.parametrize("speed", [1, 3])
+def test_patchtable_lookup_against_exact_nn(speed):
+    """The table lookup's NNF against the FP64 exact nearest neighbours of every query patch:
+    each NNF_DIST is its match's SSD (FP64 recompute within 1e-5 relative), no match is closer
+    than the exact nearest neighbour, and the mean SSD stays within a few percent of it."""
+    import torch
+    from patchtable_amd import PatchTable, PatchTableParams, brute_exact_f64, synth_pair
+    a, b = synth_pair(51, 120, 160)
+    A, B = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    p = PatchTableParams()
+    p.set_speed(speed)
+    ann = PatchTable(p, b).lookup(a)
+    epos, edist = brute_exact_f64(A, B)
+    edist = edist.cpu().numpy().reshape(ann.shape[:2])
+    m, d64 = O.recompute(a, b, ann)
+    np.testing.assert_allclose(ann[..., 2], d64, rtol=1e-5, atol=1e-7)
+    assert np.all(d64 >= edist * (1 - 1e-9))
+    ratio = ann[..., 2].mean() / edist.mean()
+    found = np.mean(np.abs(d64 - edist) <= 1e-9 * np.maximum(edist, 1e-12))
+    print(f"speed {speed}: mean SSD {ann[..., 2].mean():.6f} vs exact {edist.mean():.6f} (ratio {ratio:.4f}); "
+          f"exact nearest neighbour for {found:.3f} of the patches", flush=True)
+    assert ratio < 1.10
