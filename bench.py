@@ -210,6 +210,42 @@ def learned_key(args):
     print(json.dumps(out), flush=True)
 
 
+def key_forward(args):
+    """The learned key's forward pass over the 2048x2048 query (seed 3): 4,165,681 patches x
+    25,600 FLOP (192x64 + 64x8 multiply-adds), BF16 layer 1 and FP32 layer 2 on MFMA. Times
+    `steps` calls of pt_key_forward_dev into a preallocated plane buffer."""
+    import ctypes
+    import torch
+    from patchtable_amd import PatchKey, _ffi, synth_pair
+    H = W = 2048
+    a = torch.from_numpy(synth_pair(3, H, W)[0]).cuda()
+    key = PatchKey(1)
+    planes = torch.zeros((8, H + 7, W + 7), dtype=torch.float32, device="cuda")
+    lib = _ffi.load()
+
+    def call():
+        _ffi.check(lib.pt_key_forward_dev(key._k, ctypes.c_void_p(a.data_ptr()), H, W,
+                                          ctypes.c_void_p(planes.data_ptr())))
+    for _ in range(args.warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        call()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    n = (H - 7) * (W - 7)
+    tf = n * 25600 / (ms / 1e3) / 1e12
+    out = {"metric": "learned-key forward Mpatch/s (2048x2048 image)", "value": round(n / ms / 1e3, 2),
+           "unit": "Mpatch/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (layer 2 f32)",
+           "data": "synthetic", "config": {"workload": "key forward over a 2048x2048 image (seed 3), 192-64-8 MLP"},
+           "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": round(tf / 2500.0, 4), "traffic": None,
+                        "note": "wall time per call including its stream synchronisation"}}
+    print(json.dumps(out), flush=True)
+
+
 def brute(args):
     """Exhaustive search on the 2048x2048 pair (seed 3): 65,536 query patches (every 8th in x and
     y) against all 4,165,681 database patches, candidate scoring |a|^2+|b|^2-2a.b on MFMA in BF16
@@ -316,10 +352,12 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--build-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="pair", choices=["pair", "one_vs_many", "learned_key", "brute"],
+    ap.add_argument("--workload", default="pair",
+                    choices=["pair", "one_vs_many", "learned_key", "key_forward", "brute"],
                     help="pair: the 4MP pair of the headline metric (default); one_vs_many: one query vs many "
-                         "databases; learned_key: train the key network, then key table vs GCK table; brute: "
-                         "exhaustive MFMA search against the FP64 exact nearest neighbours")
+                         "databases; learned_key: train the key network, then key table vs GCK table; "
+                         "key_forward: the key network's forward pass alone; brute: exhaustive MFMA search "
+                         "against the FP64 exact nearest neighbours")
     ap.add_argument("--databases", type=int, default=32)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration for the traffic figure)")
@@ -342,6 +380,9 @@ def main():
         return
     if args.workload == "brute":
         brute(args)
+        return
+    if args.workload == "key_forward":
+        key_forward(args)
         return
 
     H, W = (int(v) for v in args.size.lower().split("x"))
