@@ -97,23 +97,47 @@ __device__ __forceinline__ bool allowed_at(const LookupArgs &L, int xs, int ys) 
     return !L.allowed || L.allowed[(size_t)ys * (L.bw - 7) + xs] == L.allowed_index;
 }
 
-// Candidates already evaluated for this patch in the current phase. Evaluating one again cannot
-// change the result: it was rejected against a best distance that has only decreased since, or it
-// is the best itself. So repeats are skipped (exactly result-preserving).
-struct Seen {
-    int c[8];
+// Candidates already evaluated for this patch, in this lookup: the last HIST of them, in a ring.
+// Evaluating one again cannot change the result (it was rejected against a best distance that
+// has only decreased since, or it became the best and was replaced by a closer one, or it is the
+// best), so such repeats are skipped; keeping any subset of the evaluated candidates is exact.
+// Between the kernels of a lookup the ring lives in HBM, entry-major (entry k of lattice patch v
+// at hist[k * nv + v]) with the count of recorded candidates in hn[v].
+constexpr int HIST = PT_LOOKUP_HIST;
+struct Hist {
+    int c[HIST];
     int n = 0;
+    int adds = 0;   // candidates recorded since the last load
     __device__ __forceinline__ bool has(int v) const {
         bool f = false;
 #pragma unroll
-        for (int i = 0; i < 8; i++) f |= i < n && c[i] == v;
+        for (int i = 0; i < HIST; i++) f |= i < n && c[i] == v;
         return f;
     }
     __device__ __forceinline__ void add(int v) {
+        const int k = n % HIST;
 #pragma unroll
-        for (int i = 0; i < 8; i++)
-            if (i == n) c[i] = v;
-        n++;
+        for (int i = 0; i < HIST; i++)
+            if (i == k) c[i] = v;
+        n = n + 1 >= 2 * HIST ? n + 1 - HIST : n + 1;   // n % HIST is the next slot; n >= HIST once full
+        adds++;
+    }
+    __device__ __forceinline__ void load(const LookupArgs &L, long v, long nv) {
+        n = L.hn[v];
+        adds = 0;
+#pragma unroll
+        for (int i = 0; i < HIST; i++) c[i] = i < n ? L.hist[i * nv + v] : 0;
+    }
+    // Writes the count and the entries recorded since the load (the newest `adds` slots).
+    __device__ __forceinline__ void store(const LookupArgs &L, long v, long nv) const {
+        if (adds == 0) return;
+        L.hn[v] = (uint8_t)n;
+        const int added = adds < HIST ? adds : HIST;
+#pragma unroll
+        for (int i = 0; i < HIST; i++) {
+            const int age = ((n - 1 - i) % HIST + HIST) % HIST;   // 0: the newest slot
+            if (i < n && age < added) L.hist[i * nv + v] = c[i];
+        }
     }
 };
 
@@ -190,12 +214,13 @@ __device__ __forceinline__ Window spatial_window(const LookupArgs &L, int xb, in
 // Sequential form, for spatial radius > 1 (more than SLOTS positions). Returns true if improved.
 template <int M>
 __device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const float4 *qbuf, int &xb, int &yb, float &db,
-                                                   const Seen &seen) {
+                                                   Hist &hist) {
     const Window w = spatial_window(L, xb, yb);
     bool changed = false;
     for (int ys = w.ymin; ys < w.ymax; ys++)
         for (int xs = w.xmin; xs < w.xmax; xs++) {
-            if ((xs == w.xb0 && ys == w.yb0) || !allowed_at(L, xs, ys) || seen.has(PT_XY_TO_INT(xs, ys))) continue;
+            if ((xs == w.xb0 && ys == w.yb0) || !allowed_at(L, xs, ys) || hist.has(PT_XY_TO_INT(xs, ys))) continue;
+            hist.add(PT_XY_TO_INT(xs, ys));
             float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, db);
             if (d < db) {
                 db = d;
@@ -211,7 +236,7 @@ __device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const fl
 // lanes that skip it; every lane of the block must call this. Returns true if improved.
 template <int M>
 __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L, bool want,
-                                                 int &xb, int &yb, float &db, const Seen &seen) {
+                                                 int &xb, int &yb, float &db, Hist &hist) {
     Window w = spatial_window(L, xb, yb);
     if (!want) w.ymax = w.ymin;   // empty
     if (want) Q.bound[threadIdx.x] = db;
@@ -223,8 +248,11 @@ __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qb
             // slot order = row-by-row order of the window positions other than the centre
             const int ys = w.yb0 + dy, xs = w.xb0 + dx;
             const bool in = ys >= w.ymin && ys < w.ymax && xs >= w.xmin && xs < w.xmax && allowed_at(L, xs, ys) &&
-                            !seen.has(PT_XY_TO_INT(xs, ys));
-            if (in) pushed |= 1u << slot;
+                            !hist.has(PT_XY_TO_INT(xs, ys));
+            if (in) {
+                pushed |= 1u << slot;
+                hist.add(PT_XY_TO_INT(xs, ys));
+            }
             q_push(Q, in, slot, PT_XY_TO_INT(xs, ys));
             slot++;
         }
@@ -250,17 +278,20 @@ __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qb
 // in-order reduce replays the early-out exactly. Every lane of the block must call this.
 template <int M>
 __device__ __forceinline__ bool kcoh_walk_q(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L, bool want, int &xb,
-                                            int &yb, float &db) {
+                                            int &yb, float &db, Hist &hist) {
     const int bew = L.bw - 7;
     const size_t lst = ((size_t)yb * bew + xb) * L.K;
     bool broken = !want, changed = false;
     for (int k0 = 0; k0 < L.K; k0 += SLOTS) {
         const double tb = L.triangle_factor * (double)db;
         if (!broken) Q.bound[threadIdx.x] = db;
+        unsigned pushed = 0;
         for (int s = 0; s < SLOTS; s++) {
             const int k = k0 + s;
             bool pred = !broken && k < L.K && (double)L.kdist[lst + k] < tb;
             int pos = pred ? L.kpos[lst + k] : 0;
+            if (pred && hist.has(pos)) pred = false;   // evaluated before: cannot win
+            if (pred) pushed |= 1u << s;
             q_push(Q, pred, s, pos);
         }
         q_run<M>(Q, qbuf, L);
@@ -271,6 +302,10 @@ __device__ __forceinline__ bool kcoh_walk_q(BlockQueue &Q, const float4 *qbuf, c
                 broken = true;
                 break;
             }
+            if (!((pushed >> s) & 1u)) continue;
+            // recorded only once its result is used: an entry past the walk's end was evaluated
+            // but never compared, so it may still win later
+            hist.add(Q.cxy[threadIdx.x * SLOTS + s]);
             const float d = Q.res[threadIdx.x * SLOTS + s];
             if (d < db) {
                 db = d;
@@ -343,10 +378,12 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
         db = L.init_d[pi];
         have = true;
     }
+    Hist hist;
     if (valid && L.do_table_lookup) {
         const int v = table_cell_value(L, L.a_desc, L.q, L.table, x, y);
         const int xs = PT_INT_TO_X(v) - 7, ys = PT_INT_TO_Y(v) - 7;
         if (!have || xs != xb || ys != yb) {
+            hist.add(PT_XY_TO_INT(xs, ys));
             const float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, have ? db : INFINITY);
             if (!have || d < db) {
                 db = d;
@@ -360,6 +397,7 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
             const int v1 = table_cell_value(L, L.xa_desc, L.xq[t], L.xtable[t], x, y);
             const int x1 = PT_INT_TO_X(v1) - 7, y1 = PT_INT_TO_Y(v1) - 7;
             if ((x1 != xb || y1 != yb) && (x1 != xp || y1 != yp)) {
+                hist.add(PT_XY_TO_INT(x1, y1));
                 const float d = qdist<M>(L, qbuf, threadIdx.x, x1, y1, db);
                 if (d < db) {
                     db = d;
@@ -372,15 +410,14 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
         }
     }
     bool kd = false;
-    if (L.K > 0 && L.kcoh_iter0) kd = !kcoh_walk_q<M>(Q, qbuf, L, valid, xb, yb, db);
+    if (L.K > 0 && L.kcoh_iter0) kd = !kcoh_walk_q<M>(Q, qbuf, L, valid, xb, yb, db, hist);
     bool done = false;
-    const Seen none{};
     if (L.spatial_cand && L.spatial == 1) {
-        const bool changed = spatial_search_q<M>(Q, qbuf, L, valid, xb, yb, db, none);
+        const bool changed = spatial_search_q<M>(Q, qbuf, L, valid, xb, yb, db, hist);
         done = !changed;
         if (changed) kd = false;
     } else if (L.spatial_cand && L.spatial > 1 && valid) {
-        const bool changed = spatial_search_seq<M>(L, qbuf, xb, yb, db, none);
+        const bool changed = spatial_search_seq<M>(L, qbuf, xb, yb, db, hist);
         done = !changed;
         if (changed) kd = false;
     } else if (L.rs && valid) {
@@ -391,6 +428,10 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
     out.xy[v] = PT_XY_TO_INT(xb, yb);
     out.d[v] = db;
     out.fl[v] = (done ? F_DONE : 0) | F_CHG | (kd ? F_KD : 0);
+    if (L.hist && L.rounds > 0) {
+        if (hist.adds == 0) L.hn[v] = 0;   // nothing evaluated: the count must still be written
+        hist.store(L, v, (long)L.vw * L.vh);
+    }
 }
 
 // same_jump: the previous round used the same jump. A candidate from neighbour n then repeats
@@ -398,7 +439,7 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
 // best, so it is skipped (results unchanged).
 template <int M>
 __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int same_jump, int round, NnfField cur,
-                                                      NnfField nxt) {
+                                                      NnfField nxt, int keep_hist) {
     __shared__ float4 qbuf[QBuf<M>::F4];
     __shared__ BlockQueue Q;
     if (threadIdx.x == 0) Q.n = 0;
@@ -412,7 +453,10 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
     int xb = 0, yb = 0;
     float db = 0.f;
     bool stable = false, was_done = false, kd = false;
+    const long nv = (long)L.vw * L.vh;
+    Hist hist;
     if (valid) {
+        if (L.hist) hist.load(L, v, nv);
         const int v0 = cur.xy[v];
         xb = PT_INT_TO_X(v0);
         yb = PT_INT_TO_Y(v0);
@@ -424,7 +468,6 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
         Q.bound[threadIdx.x] = db;
     }
     unsigned pushed = 0;
-    Seen seen;
     for (int t = 0; t < 8; t++) {
         const int qx = vx + J * c_dx[t], qy = vy + J * c_dy[t];
         bool pred = valid && qx >= 0 && qy >= 0 && qx < L.vw && qy < L.vh;
@@ -437,12 +480,12 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
                 xs = PT_INT_TO_X(vq) - js * c_dx[t];
                 ys = PT_INT_TO_Y(vq) - js * c_dy[t];
                 pred = xs >= 0 && ys >= 0 && xs < bew && ys < beh && !(xs == xb && ys == yb) && allowed_at(L, xs, ys) &&
-                       !seen.has(PT_XY_TO_INT(xs, ys));
+                       !hist.has(PT_XY_TO_INT(xs, ys));
             }
         }
         if (pred) {
             pushed |= 1u << t;
-            seen.add(PT_XY_TO_INT(xs, ys));
+            hist.add(PT_XY_TO_INT(xs, ys));
         }
         q_push(Q, pred, t, PT_XY_TO_INT(xs, ys));
     }
@@ -463,17 +506,17 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
     bool kchg = false;
     if (L.kcoh_rounds) {
         const bool want = valid && !kd;
-        kchg = kcoh_walk_q<M>(Q, qbuf, L, want, xb, yb, db);
+        kchg = kcoh_walk_q<M>(Q, qbuf, L, want, xb, yb, db, hist);
         if (want) kd = !kchg;
     }
     bool done = was_done && !moved && !kchg;
     bool schg = false;
     const bool sp = round >= L.spatial_from;   // rounds that run the spatial window
     if (sp && L.spatial == 1) {
-        schg = spatial_search_q<M>(Q, qbuf, L, valid && !done, xb, yb, db, seen);
+        schg = spatial_search_q<M>(Q, qbuf, L, valid && !done, xb, yb, db, hist);
         if (valid && !done) done = !schg;
     } else if (sp && L.spatial > 1 && valid && !done) {
-        schg = spatial_search_seq<M>(L, qbuf, xb, yb, db, seen);
+        schg = spatial_search_seq<M>(L, qbuf, xb, yb, db, hist);
         done = !schg;
     } else if (L.rs && valid) {
         schg = random_search<M>(L, qbuf, (long)y * L.ann_w + x, round, xb, yb, db);
@@ -483,6 +526,7 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
     nxt.xy[v] = PT_XY_TO_INT(xb, yb);
     nxt.d[v] = db;
     nxt.fl[v] = (done ? F_DONE : 0) | ((moved || kchg || schg) ? F_CHG : 0) | (kd ? F_KD : 0);
+    if (L.hist && keep_hist) hist.store(L, v, nv);
 }
 
 // query_step > 1: every patch takes the best offset of the lattice patches within query_step
@@ -622,7 +666,8 @@ int launch_lookup(const LookupArgs &L, NnfField f0, NnfField f1, hipStream_t s, 
     pt_jump_schedule(L.rounds, J);
     for (size_t r = 0; r < J.size(); r++) {
         int same = r > 0 && J[r - 1] == J[r];
-        hipLaunchKernelGGL(k_propagate<M>, grid, dim3(THREADS), 0, s, L, J[r], same, (int)r + 1, f0, f1);
+        const int keep = (int)r + 1 < (int)J.size();   // the last round's history is not read again
+        hipLaunchKernelGGL(k_propagate<M>, grid, dim3(THREADS), 0, s, L, J[r], same, (int)r + 1, f0, f1, keep);
         std::swap(f0, f1);
     }
     launches = (int)J.size();

@@ -120,6 +120,7 @@ int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, con
 // lookup.hip
 #define PT_LOOKUP_CMAX 40    // descriptor dims supported in descriptor mode
 #define PT_MAX_XTABLES 15    // further tables (ntables - 1)
+#define PT_LOOKUP_HIST 16    // candidates remembered per lattice patch during a lookup
 struct LookupArgs {
     const float *a = nullptr;       // query RGB, HWC (image mode)
     int ah = 0, aw = 0;             // query "image" size (descriptor size + 7 in descriptor mode)
@@ -156,6 +157,8 @@ struct LookupArgs {
     int allowed_index = 1;
     const int32_t *init_xy = nullptr;   // initial field from a previous one (per patch)
     const float *init_d = nullptr;
+    int32_t *hist = nullptr;        // per lattice patch: candidates evaluated so far (16 x nv, entry-major)
+    uint8_t *hn = nullptr;          // ... and how many were recorded
 };
 // One nearest-neighbour field over the lattice: XY_TO_INT(b patch x, y), distance, and flags
 // (1: the spatial window around the match was searched without improvement; 2: the match changed

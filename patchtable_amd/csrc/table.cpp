@@ -86,6 +86,8 @@ struct pt_table {
     DevBuf<double> ann, prevbuf;
     DevBuf<float> svbuf;
     DevBuf<int> bad;
+    DevBuf<int32_t> hist;   // lookup: candidates evaluated per lattice patch (PT_LOOKUP_HIST each)
+    DevBuf<uint8_t> hn;
     int qh = 0, qw = 0;
     PtScratch sc;
     // learned-key table: the key network (copied at build) indexes the table; extra[0] is the GCK
@@ -602,6 +604,8 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     TRY(t->nxy2.ensure(nv));
     TRY(t->nd2.ensure(nv));
     TRY(t->nfl2.ensure(nv));
+    TRY(t->hist.ensure((size_t)nv * PT_LOOKUP_HIST));
+    TRY(t->hn.ensure(nv));
     if (qs > 1) {
         TRY(t->gxy.ensure(n));
         TRY(t->gd.ensure(n));
@@ -656,6 +660,8 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     }
     L.kpos = t->kpos.p;
     L.kdist = t->kdist.p;
+    L.hist = t->hist.p;
+    L.hn = t->hn.p;
     L.K = p.kcoherence;
     L.kcoh_iter0 = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == 0);
     // k-coherence walks in the rounds when kcoherence_iter asks for them after the first pass, and
