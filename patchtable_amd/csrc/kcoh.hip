@@ -122,17 +122,58 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
         const float *ds = o == 0 ? ds0 : ds1;
         const long r = o == 0 ? r0 : (long)pos1[i];
         const long t0 = r - W < 0 ? 0 : r - W, t1 = r + W >= Na ? Na - 1 : r + W;
-        for (long t = t0; t <= t1; t++) {
+        for (long t4 = t0; t4 <= t1; t4 += 4) {
+          // the window is contiguous in the sorted arrays: with ND known, four candidates' ranks
+          // and descriptors arrive in 16-byte loads (one for the ranks, ND for the values)
+          int qv[4];
+          float dv[4][ND > 0 ? ND : 1];
+          const bool full = ND > 0 && t4 + 3 <= t1;
+          if (full) {
+              const int4 p4 = pt_tile::ldg4i(perm + t4);
+              qv[0] = p4.x;
+              qv[1] = p4.y;
+              qv[2] = p4.z;
+              qv[3] = p4.w;
+              if constexpr (ND > 0) {
+                  float tmp[4 * ND];
+#pragma unroll
+                  for (int j = 0; j < ND; j++) {
+                      const float4 v = pt_tile::ldg4(ds + t4 * ND + 4 * j);
+                      tmp[4 * j] = v.x;
+                      tmp[4 * j + 1] = v.y;
+                      tmp[4 * j + 2] = v.z;
+                      tmp[4 * j + 3] = v.w;
+                  }
+#pragma unroll
+                  for (int u = 0; u < 4; u++)
+#pragma unroll
+                      for (int c = 0; c < ND; c++) dv[u][c] = tmp[u * ND + c];
+              }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const long t = t4 + u;
+            if (t > t1) break;
             if (t == r) continue;
-            const int qi = perm[t];
+            const int qi = full ? qv[u] : perm[t];
             const int qx = qi % bew, qy = qi / bew;
             const int dx = qx - px, dy = qy - py;
             if (dx * dx + dy * dy < md2) continue;
             float dd = 0.f;
-            const float *dq = ds + t * nd;
-            for (int c = 0; c < nd; c++) {
-                float e = own[c] - dq[c];
-                dd = dd + e * e;
+            if (full) {
+                if constexpr (ND > 0) {
+#pragma unroll
+                    for (int c = 0; c < ND; c++) {
+                        float e = own[c] - dv[u][c];
+                        dd = dd + e * e;
+                    }
+                }
+            } else {
+                const float *dq = ds + t * nd;
+                for (int c = 0; c < nd; c++) {
+                    float e = own[c] - dq[c];
+                    dd = dd + e * e;
+                }
             }
             // reject if not better than the current K-th entry
             if (cnt == K && !(dd < wd || (dd == wd && qi < wq))) continue;
@@ -166,6 +207,7 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
                     wd = bd[k];
                     wq = bq[k];
                 }
+          }
         }
     }
     // exact re-rank, then insertion by (distance, XY_TO_INT)

@@ -34,6 +34,11 @@ __device__ __forceinline__ float4 ldg4(const float *p) {
     __builtin_memcpy(&v, p, 16);
     return v;
 }
+__device__ __forceinline__ int4 ldg4i(const int32_t *p) {
+    int4 v;
+    __builtin_memcpy(&v, p, 16);
+    return v;
+}
 
 #define PT_ACC(e, bv)                                  \
     {                                                  \
