@@ -54,6 +54,39 @@ __global__ void k_dt_line(uint64_t *__restrict__ key, int64_t nlines, int len, i
     }
 }
 
+// The same two sweeps with the line held in registers (len <= LMAX): one read and one write
+// per cell instead of two of each.
+template <int LMAX>
+__global__ __launch_bounds__(64) void k_dt_line_reg(uint64_t *__restrict__ key, int64_t nlines, int len, int64_t st) {
+    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlines) return;
+    int64_t o = l / st, i = l - o * st;
+    uint64_t *p = key + o * (int64_t)len * st + i;
+    uint64_t v[LMAX];
+#pragma unroll
+    for (int t = 0; t < LMAX; t++)
+        if (t < len) v[t] = p[(int64_t)t * st];
+    uint64_t run = KEY_INF;
+#pragma unroll
+    for (int t = 0; t < LMAX; t++)
+        if (t < len) {
+            if (run != KEY_INF) run += KEY_STEP;
+            if (v[t] < run) run = v[t];
+            v[t] = run;
+        }
+    run = KEY_INF;
+#pragma unroll
+    for (int t = LMAX - 1; t >= 0; t--)
+        if (t < len) {
+            if (run != KEY_INF) run += KEY_STEP;
+            if (v[t] < run) run = v[t];
+            v[t] = run;
+        }
+#pragma unroll
+    for (int t = 0; t < LMAX; t++)
+        if (t < len) p[(int64_t)t * st] = v[t];
+}
+
 // The same sweeps for the last dimension, whose lines are contiguous: a block stages
 // `lpb` consecutive lines in LDS with coalesced loads, sweeps each line from LDS, writes back.
 __global__ void k_dt_line_contig(uint64_t *__restrict__ key, int64_t nlines, int len, int lpb) {
@@ -309,7 +342,14 @@ int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells,
             hipLaunchKernelGGL(k_dt_line_contig, dim3(pt_blocks(nlines, lpb)), dim3(256), (size_t)lpb * nslices[d] * 8,
                                s, key, nlines, nslices[d], lpb);
         } else {
-            hipLaunchKernelGGL(k_dt_line, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines, nslices[d], st);
+            if (nslices[d] <= 32)
+                hipLaunchKernelGGL(k_dt_line_reg<32>, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines,
+                                   nslices[d], st);
+            else if (nslices[d] <= 64)
+                hipLaunchKernelGGL(k_dt_line_reg<64>, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines,
+                                   nslices[d], st);
+            else
+                hipLaunchKernelGGL(k_dt_line, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines, nslices[d], st);
         }
         st *= nslices[d];
     }
