@@ -56,8 +56,11 @@ __global__ void k_dt_line(uint64_t *__restrict__ key, int64_t nlines, int len, i
 
 // The same two sweeps with the line held in registers (len <= LMAX): one read and one write
 // per cell instead of two of each.
-template <int LMAX>
-__global__ __launch_bounds__(64) void k_dt_line_reg(uint64_t *__restrict__ key, int64_t nlines, int len, int64_t st) {
+// With FINAL (the last dimension's pass) the lane writes the table values of k_dt_final for its
+// line instead of the keys.
+template <int LMAX, bool FINAL>
+__global__ __launch_bounds__(64) void k_dt_line_reg(uint64_t *__restrict__ key, int64_t nlines, int len, int64_t st,
+                                                    int32_t *__restrict__ table, int64_t max_dist, int32_t fill) {
     int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlines) return;
     int64_t o = l / st, i = l - o * st;
@@ -82,6 +85,22 @@ __global__ __launch_bounds__(64) void k_dt_line_reg(uint64_t *__restrict__ key, 
             if (v[t] < run) run = v[t];
             v[t] = run;
         }
+    if (FINAL) {
+        // as k_dt_final: source cells keep their own value, so reading table[src] is safe while
+        // other lanes rewrite other cells
+        const int64_t c0 = o * (int64_t)len * st + i;
+#pragma unroll
+        for (int t = 0; t < LMAX; t++)
+            if (t < len) {
+                const uint64_t k = v[t];
+                const int64_t c = c0 + (int64_t)t * st;
+                int32_t val;
+                if (k == KEY_INF || (max_dist >= 0 && (int64_t)(k >> 32) > max_dist)) val = fill;
+                else val = table[(int64_t)(k & 0xffffffffull)];
+                if (val != table[c]) table[c] = val;
+            }
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < LMAX; t++)
         if (t < len) p[(int64_t)t * st] = v[t];
@@ -89,13 +108,21 @@ __global__ __launch_bounds__(64) void k_dt_line_reg(uint64_t *__restrict__ key, 
 
 // The same sweeps for the last dimension, whose lines are contiguous: a block stages
 // `lpb` consecutive lines in LDS with coalesced loads, sweeps each line from LDS, writes back.
-__global__ void k_dt_line_contig(uint64_t *__restrict__ key, int64_t nlines, int len, int lpb) {
+// With `table` (the first pass) the keys are made from the table as k_dt_init makes them.
+__global__ void k_dt_line_contig(uint64_t *__restrict__ key, int64_t nlines, int len, int lpb,
+                                 const int32_t *__restrict__ table) {
     extern __shared__ uint64_t sh[];
     const int64_t l0 = (int64_t)blockIdx.x * lpb;
     const int nl = (int)(nlines - l0 < lpb ? nlines - l0 : lpb);
     uint64_t *g = key + l0 * len;
     const int total = nl * len;
-    for (int i = threadIdx.x; i < total; i += blockDim.x) sh[i] = g[i];
+    if (table) {
+        const int64_t c0 = l0 * len;
+        for (int i = threadIdx.x; i < total; i += blockDim.x)
+            sh[i] = table[c0 + i] != PT_TABLE_UNUSED ? (uint64_t)(c0 + i) : KEY_INF;
+    } else {
+        for (int i = threadIdx.x; i < total; i += blockDim.x) sh[i] = g[i];
+    }
     __syncthreads();
     for (int l = threadIdx.x; l < nl; l += blockDim.x) {
         uint64_t *p = sh + l * len;
@@ -333,28 +360,43 @@ int pt_dt_euclidean_launch(int32_t *table, int G, const int *nslices, int64_t nc
 int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
                       uint64_t *key, hipStream_t s) {
     const int TB = 256;
-    hipLaunchKernelGGL(k_dt_init, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, table, ncells, key);
+    const int64_t max_dist = dt_iters > 0 ? (int64_t)dt_iters + 1 : -1;
+    // passes run d = G-1 .. 0; the first (contiguous) pass makes the keys from the table and the
+    // last (strided, in registers) pass writes the table, when those kernels apply
+    const bool init_fused = nslices[G - 1] <= 256;
+    int64_t st0 = 1;   // stride of dimension 0
+    for (int d = 1; d < G; d++) st0 *= nslices[d];
+    const bool final_fused = st0 > 1 && nslices[0] <= 64;
+    if (!init_fused) hipLaunchKernelGGL(k_dt_init, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, table, ncells, key);
     int64_t st = 1;
     for (int d = G - 1; d >= 0; d--) {
-        int64_t nlines = ncells / nslices[d];
-        if (st == 1 && nslices[d] <= 256) {
-            const int lpb = 8192 / nslices[d] < 256 ? 8192 / nslices[d] : 256;   // <= 64 KB of keys
-            hipLaunchKernelGGL(k_dt_line_contig, dim3(pt_blocks(nlines, lpb)), dim3(256), (size_t)lpb * nslices[d] * 8,
-                               s, key, nlines, nslices[d], lpb);
-        } else {
-            if (nslices[d] <= 32)
-                hipLaunchKernelGGL(k_dt_line_reg<32>, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines,
-                                   nslices[d], st);
-            else if (nslices[d] <= 64)
-                hipLaunchKernelGGL(k_dt_line_reg<64>, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines,
-                                   nslices[d], st);
+        const int64_t nlines = ncells / nslices[d];
+        const int len = nslices[d];
+        const dim3 g64(pt_blocks(nlines, 64));
+        if (st == 1 && len <= 256) {
+            const int lpb = 8192 / len < 256 ? 8192 / len : 256;   // <= 64 KB of keys
+            hipLaunchKernelGGL(k_dt_line_contig, dim3(pt_blocks(nlines, lpb)), dim3(256), (size_t)lpb * len * 8, s,
+                               key, nlines, len, lpb, d == G - 1 && init_fused ? (const int32_t *)table : nullptr);
+        } else if (d == 0 && final_fused) {
+            if (len <= 32)
+                hipLaunchKernelGGL((k_dt_line_reg<32, true>), g64, dim3(64), 0, s, key, nlines, len, st, table,
+                                   max_dist, fill);
             else
-                hipLaunchKernelGGL(k_dt_line, dim3(pt_blocks(nlines, 64)), dim3(64), 0, s, key, nlines, nslices[d], st);
+                hipLaunchKernelGGL((k_dt_line_reg<64, true>), g64, dim3(64), 0, s, key, nlines, len, st, table,
+                                   max_dist, fill);
+        } else if (len <= 32) {
+            hipLaunchKernelGGL((k_dt_line_reg<32, false>), g64, dim3(64), 0, s, key, nlines, len, st, table, max_dist,
+                               fill);
+        } else if (len <= 64) {
+            hipLaunchKernelGGL((k_dt_line_reg<64, false>), g64, dim3(64), 0, s, key, nlines, len, st, table, max_dist,
+                               fill);
+        } else {
+            hipLaunchKernelGGL(k_dt_line, g64, dim3(64), 0, s, key, nlines, len, st);
         }
-        st *= nslices[d];
+        st *= len;
     }
-    int64_t max_dist = dt_iters > 0 ? (int64_t)dt_iters + 1 : -1;
-    hipLaunchKernelGGL(k_dt_final, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, key, ncells, table, max_dist, fill);
+    if (!final_fused)
+        hipLaunchKernelGGL(k_dt_final, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, key, ncells, table, max_dist, fill);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }

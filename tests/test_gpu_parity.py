@@ -82,7 +82,12 @@ def _build_both(p, b):
 
 @pytest.mark.parametrize("extra", ["-limit 1", "-limit 1 -dt_algo downsample", "-limit 1 -dt_algo euclidean",
                                    "-limit 0.5 -dt_iters 3", "-limit 1 -run_dt 0", "-limit 2 -partition_step 3",
-                                   "-limit 1 -populate_random 0"])
+                                   "-limit 1 -populate_random 0",
+                                   # DT line passes of every shape: single-slice dimensions (the first
+                                   # dimension's pass then has stride 1), lines longer than 64, a
+                                   # contiguous dimension longer than 256, and a distance cap
+                                   "-nslices 40,1,1,1,1,1", "-nslices 100,3,3,3,2,2",
+                                   "-nslices 3,3,3,3,2,300", "-nslices 50,4,4,4,4,4 -dt_iters 2"])
 def test_table_build_bit_exact(pair_cfg1, extra):
     print(f"table build cfg1 {extra}", flush=True)
     a, b = pair_cfg1
