@@ -60,9 +60,7 @@ __global__ void k_keys(const float *__restrict__ desc, size_t plane, int Wp, int
 
 __global__ void k_count_allowed(const int32_t *__restrict__ allowed, int aidx, long N, int *__restrict__ cnt) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool ok = i < N && allowed[i] == aidx;
-    unsigned long long m = __ballot(ok);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(cnt, __popcll(m));
+    pt_block_count_add(i < N && allowed[i] == aidx, cnt);
 }
 
 __global__ void k_positions(const int32_t *__restrict__ perm, long N, int32_t *__restrict__ pos) {

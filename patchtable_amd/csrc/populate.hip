@@ -154,9 +154,7 @@ __global__ void k_write_winners(const int64_t *__restrict__ cell, const uint8_t 
         table[cell[k]] = list_xy[k];
         won = 1;
     }
-    // wave-level reduction of the winner count
-    unsigned long long b = __ballot(won);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, __popcll(b));
+    pt_block_count_add(won, count);
 }
 
 __global__ void k_fill(int32_t *__restrict__ p, long n, int32_t v) {

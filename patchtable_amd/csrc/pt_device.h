@@ -19,6 +19,21 @@
 
 static inline unsigned pt_blocks(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
+// Adds the number of lanes of the block with `pred` set to *cnt: one atomic per block (atomics
+// to one address from every wave of a large grid serialise in L2). Every thread of the block must
+// call it (it synchronises the block); blockDim.x <= 1024.
+__device__ __forceinline__ void pt_block_count_add(bool pred, int *cnt) {
+    __shared__ int part[16];
+    const unsigned long long b = __ballot(pred);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < (int)((blockDim.x + 63) / 64); i++) s += part[i];
+        if (s) atomicAdd(cnt, s);
+    }
+}
+
 // int(float) with the x86 cvttss2si result for out-of-range and NaN inputs (INT_MIN), which is
 // what the reference's compiled get_bin computes (patchtable.h:1299).
 __device__ __forceinline__ int pt_x86_ftoi(float f) {
