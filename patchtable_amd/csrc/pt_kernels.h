@@ -67,6 +67,37 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Pinned host memory for small host<->device transfers that must not synchronise the stream
+// (a pageable source or destination makes the copy wait). Sized once per use site: a regrow
+// frees the old block, so it must not happen while a copy from or to it may be pending.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= n && p) return PT_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        const hipError_t e = hipHostMalloc(&p, bytes ? bytes : 16, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return pt_failf(PT_E_NOMEM, "hipHostMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        }
+        n = bytes;
+        return PT_OK;
+    }
+    template <class T>
+    T *at(size_t byte_off) const {
+        return (T *)((char *)p + byte_off);
+    }
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 // Quantiser state on the device: per grid dimension min, range, 16384-entry bin map, stride.
 struct DevQuant {
     int G = 0;
@@ -85,8 +116,18 @@ int pt_gck_launch(const float *img, int h, int w, int n, int nchroma, int conver
 // allowed: NULL or the (h-7) x (w-7) device mask; only patches equal to aidx take part.
 int pt_minmax_launch(const float *desc, size_t plane, int Wp, int h, int w, int G, const int32_t *allowed, int aidx,
                      float *partials, int nblocks, hipStream_t s);
+// The same order statistics for every dimension d with nslices[d] > 0 at once (an exact radix
+// select), into values_dev[d * ld + j]; needs nslices[d] + 1 <= PT_RS_SLOTS. ambig_dev[d] = 1
+// where a value is a zero whose sign the select cannot tell (the dimension holds -0.0 and +0.0):
+// such a dimension takes pt_quantile_values instead.
+constexpr int PT_RS_SLOTS = 48;
+int pt_quantile_select(const float *desc, size_t plane, int Wp, int h, int w, int step, int G, const int *nslices,
+                       const int32_t *allowed, int aidx, int64_t npart, int ld, float *values_dev, int *ambig_dev,
+                       PtScratch &sc, hipStream_t s);
+// The values at ranks (npart-1)*j/nslices, j = 0..nslices, of dimension `dim` over the partition
+// grid (ascending order, masked patches last), written to values_dev[0..nslices].
 int pt_quantile_values(const float *desc, size_t plane, int Wp, int h, int w, int step, int dim, const int32_t *allowed,
-                       int aidx, const int64_t *ranks_dev, int nranks, float *values_dev, PtScratch &sc, hipStream_t s);
+                       int aidx, int64_t npart, int nslices, float *values_dev, PtScratch &sc, hipStream_t s);
 
 // populate.hip
 int pt_list_launch(int bew, int n, int32_t *list, hipStream_t s);

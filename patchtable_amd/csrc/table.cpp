@@ -57,6 +57,8 @@ struct pt_table {
     int64_t ncells = 0;
     std::vector<HostPartition> part;
     int table_add_count = 0;
+    PinnedBuf pin;                   // staging of the build's small transfers
+    bool pin_count_valid = false;    // the populate count was copied into pin
     long rand_used = 0;       // rand() outputs consumed by this table's shuffle and the earlier ones
     int K = 0;
     double stage_ms[12] = {};
@@ -236,25 +238,41 @@ void make_partition(HostPartition &P, const std::vector<float> &q) {
     }
 }
 
+// Byte offsets of the quantiser upload in the table's pinned staging buffer: min, range and
+// stride per dimension, then the bin maps; the populate counter lands after them.
+size_t pin_upload_bytes(int G) { return (size_t)G * 16 + (size_t)G * PT_NMAP * 4; }
+
+// Device copy of the quantiser (min, range, bin maps, strides), staged through pinned memory so
+// that the copies do not wait for the stream. Called once per table: the staging buffer is not
+// rewritten while these copies may still be pending.
 int upload_quant(pt_table *t, hipStream_t s) {
     const int G = t->G;
-    std::vector<float> mn(G), rg(G);
-    std::vector<int32_t> tb((size_t)G * PT_NMAP);
+    TRY(t->pin.ensure(pin_upload_bytes(G) + 16));
+    float *mn = t->pin.at<float>(0), *rg = t->pin.at<float>((size_t)G * 4);
+    int64_t *st = t->pin.at<int64_t>((size_t)G * 8);
+    int32_t *tb = t->pin.at<int32_t>((size_t)G * 16);
     for (int i = 0; i < G; i++) {
         mn[i] = t->part[i].min_val;
         rg[i] = t->part[i].range;
-        std::copy(t->part[i].to_bin.begin(), t->part[i].to_bin.end(), tb.begin() + (size_t)i * PT_NMAP);
+        st[i] = t->stride[i];
+        std::copy(t->part[i].to_bin.begin(), t->part[i].to_bin.end(), tb + (size_t)i * PT_NMAP);
     }
     TRY(t->qmin.ensure(G));
     TRY(t->qrange.ensure(G));
-    TRY(t->qtobin.ensure(tb.size()));
+    TRY(t->qtobin.ensure((size_t)G * PT_NMAP));
     TRY(t->qstride.ensure(G));
-    HIPTRY(hipMemcpyAsync(t->qmin.p, mn.data(), G * 4, hipMemcpyHostToDevice, s));
-    HIPTRY(hipMemcpyAsync(t->qrange.p, rg.data(), G * 4, hipMemcpyHostToDevice, s));
-    HIPTRY(hipMemcpyAsync(t->qtobin.p, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, s));
-    HIPTRY(hipMemcpyAsync(t->qstride.p, t->stride.data(), G * 8, hipMemcpyHostToDevice, s));
-    HIPTRY(hipStreamSynchronize(s));
+    HIPTRY(hipMemcpyAsync(t->qmin.p, mn, G * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(t->qrange.p, rg, G * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(t->qtobin.p, tb, (size_t)G * PT_NMAP * 4, hipMemcpyHostToDevice, s));
+    HIPTRY(hipMemcpyAsync(t->qstride.p, st, G * 8, hipMemcpyHostToDevice, s));
     return PT_OK;
+}
+
+// After the build's final synchronisation: the populate count arrives in pinned memory, and the
+// DT work array (8 bytes per cell) is released (hipFree waits for the device, so not earlier).
+void finish_grid(pt_table *t) {
+    if (t->pin.p && t->pin_count_valid) t->table_add_count = *t->pin.at<int>(pin_upload_bytes(t->G));
+    t->dtkey.release();
 }
 
 // The glibc generator state after `skip` outputs from the state of a fresh process.
@@ -295,8 +313,10 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
     float *partials = (float *)t->sc.get(40, (size_t)NB * G * 2 * 4);
     if (!partials) return pt_fail(PT_E_NOMEM, "min/max scratch allocation failed");
     TRY(pt_minmax_launch(desc, plane, Wp, h, w, G, mask_dev, aidx, partials, NB, s));
-    std::vector<float> hp((size_t)NB * G * 2);
-    HIPTRY(hipMemcpyAsync(hp.data(), partials, hp.size() * 4, hipMemcpyDeviceToHost, s));
+    // one pinned block for the build's small transfers (the quantiser upload is the largest)
+    TRY(t->pin.ensure(std::max((size_t)NB * G * 2 * 4, pin_upload_bytes(G) + 16)));
+    const float *hp = t->pin.at<float>(0);
+    HIPTRY(hipMemcpyAsync(t->pin.p, partials, (size_t)NB * G * 2 * 4, hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
     std::vector<float> mn(G, (float)1e20), mx(G, (float)-1e20), range(G);
     for (int bk = 0; bk < NB; bk++)
@@ -337,24 +357,46 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
     }
     t->part.assign(G, HostPartition());
     {
-        int maxns = *std::max_element(t->nslices.begin(), t->nslices.end());
-        int64_t *ranks = (int64_t *)t->sc.get(41, (size_t)(maxns + 1) * 8);
-        float *vals = (float *)t->sc.get(42, (size_t)(maxns + 1) * 4);
-        if (!ranks || !vals) return pt_fail(PT_E_NOMEM, "quantile scratch allocation failed");
+        // Every dimension's order statistics, one copy back: the radix select for dimensions of
+        // up to PT_RS_SLOTS - 1 slices, a sort for the others and for a select that met zeros of
+        // both signs at a rank.
+        const size_t ld = (size_t)*std::max_element(t->nslices.begin(), t->nslices.end()) + 1;
+        float *vals = (float *)t->sc.get(42, (size_t)G * ld * 4 + (size_t)G * 4);
+        if (!vals) return pt_fail(PT_E_NOMEM, "quantile scratch allocation failed");
+        int *amb = (int *)(vals + (size_t)G * ld);
+        std::vector<int> sel(G, 0), sorted;
+        for (int i = 0; i < G; i++) {
+            if (t->nslices[i] + 1 <= PT_RS_SLOTS) sel[i] = t->nslices[i];
+            else sorted.push_back(i);
+        }
+        TRY(pt_quantile_select(desc, plane, Wp, h, w, step, G, sel.data(), mask_dev, aidx, npart, (int)ld, vals, amb,
+                               t->sc, s));
+        for (int i : sorted)
+            TRY(pt_quantile_values(desc, plane, Wp, h, w, step, i, mask_dev, aidx, npart, t->nslices[i], vals + i * ld,
+                                   t->sc, s));
+        const size_t back = (size_t)G * ld * 4 + (size_t)G * 4;
+        TRY(t->pin.ensure(std::max(back, pin_upload_bytes(G) + 16)));
+        HIPTRY(hipMemcpyAsync(t->pin.p, vals, back, hipMemcpyDeviceToHost, s));
+        HIPTRY(hipStreamSynchronize(s));
+        const int *hamb = t->pin.at<int>((size_t)G * ld * 4);
+        std::vector<int> redo;
+        for (int i = 0; i < G; i++)
+            if (sel[i] > 0 && hamb[i]) redo.push_back(i);
+        if (!redo.empty()) {
+            for (int i : redo)
+                TRY(pt_quantile_values(desc, plane, Wp, h, w, step, i, mask_dev, aidx, npart, t->nslices[i],
+                                       vals + i * ld, t->sc, s));
+            HIPTRY(hipMemcpyAsync(t->pin.p, vals, (size_t)G * ld * 4, hipMemcpyDeviceToHost, s));
+            HIPTRY(hipStreamSynchronize(s));
+        }
         for (int i = 0; i < G; i++) {
             HostPartition &P = t->part[i];
             P.nslices = t->nslices[i];
             P.min_val = mn[i];
             P.max_val = mx[i];
             P.range = mx[i] - mn[i];
-            std::vector<int64_t> hr(P.nslices + 1);
-            for (int j = 0; j <= P.nslices; j++) hr[j] = (npart - 1) * (int64_t)j / P.nslices;
-            HIPTRY(hipMemcpyAsync(ranks, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, s));
-            TRY(pt_quantile_values(desc, plane, Wp, h, w, step, i, mask_dev, aidx, ranks, (int)hr.size(), vals, t->sc, s));
-            std::vector<float> q(hr.size());
-            HIPTRY(hipMemcpyAsync(q.data(), vals, q.size() * 4, hipMemcpyDeviceToHost, s));
-            HIPTRY(hipStreamSynchronize(s));
-            make_partition(P, q);
+            const float *q = t->pin.at<float>(i * ld * 4);
+            make_partition(P, std::vector<float>(q, q + P.nslices + 1));
         }
     }
     TRY(upload_quant(t, s));
@@ -395,16 +437,15 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
     TRY(t->counter.ensure(1));
     TRY(pt_populate_launch(desc, plane, Wp, t->list_xy.p, t->rank.p, n, t->quant(), t->table.p, t->ncells, t->cells.p,
                            t->win.p, t->counter.p, s));
-    HIPTRY(hipMemcpyAsync(&t->table_add_count, t->counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPTRY(hipStreamSynchronize(s));
+    // read by finish_grid() once the build has synchronised
+    HIPTRY(hipMemcpyAsync(t->pin.at<int>(pin_upload_bytes(G)), t->counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    t->pin_count_valid = true;
     if (evt) HIPTRY(hipEventRecord(evt[3], s));
     // distance transform (patchtable.h:3254-3295) and fill_zeros (:2063-2079)
     const int32_t vfill = PT_XY_TO_INT(7, 7);
     if (p.run_dt && p.dt_algo == PT_DT_ALGO_PROP) {
         TRY(t->dtkey.ensure((size_t)t->ncells));
         TRY(pt_dt_prop_launch(t->table.p, G, t->nslices.data(), t->ncells, p.dt_iters, vfill, t->dtkey.p, s));
-        HIPTRY(hipStreamSynchronize(s));
-        t->dtkey.release();
     } else if (p.run_dt && p.dt_algo == PT_DT_ALGO_DOWNSAMPLE) {
         TRY(pt_dt_downsample_launch(t->table.p, G, t->nslices.data(), t->sc, s));
     } else if (p.run_dt && p.dt_algo == PT_DT_ALGO_EUCLIDEAN) {
@@ -479,6 +520,7 @@ int build_table(pt_table *t, const float *b_dev, int h_in, int w_in, int c, cons
         TRY(build_grid(E, t->desc.p, mask_host, mask_dev, used, s, nullptr));
         used = E->rand_used;
         HIPTRY(hipStreamSynchronize(s));
+        finish_grid(E);
         E->list_xy.release();
         E->rank.release();
         E->cells.release();
@@ -488,6 +530,7 @@ int build_table(pt_table *t, const float *b_dev, int h_in, int w_in, int c, cons
     }
     HIPTRY(hipEventRecord(t->ev[5], s));
     HIPTRY(hipStreamSynchronize(s));
+    finish_grid(t);
     float ms = 0;
     for (int k = 0; k < 5; k++) {
         HIPTRY(hipEventElapsedTime(&ms, t->ev[k], t->ev[k + 1]));
@@ -816,6 +859,8 @@ int build_keyed(pt_table *t, const pt_key *key, const float *b_dev, int h, int w
     TRY(build_grid(E, E->desc.p, no_mask, nullptr, t->rand_used, s, nullptr));
     HIPTRY(hipEventRecord(t->ev[5], s));
     HIPTRY(hipStreamSynchronize(s));
+    finish_grid(t);
+    finish_grid(E);
     for (DevBuf<float> *b : {&E->desc}) b->release();
     E->list_xy.release();
     E->rank.release();

@@ -237,6 +237,48 @@ def test_descriptor_mode_with_mask_and_tables():
     assert (m[ann[..., 1].astype(int), ann[..., 0].astype(int)] == 1).all()
 
 
+def test_quantiles_zeros_of_both_signs():
+    """A grid dimension whose quantile ranks fall inside a run of -0.0 and +0.0. The radix sort
+    the order statistics follow gives both zeros the same digit and keeps input order, so the
+    value at such a rank is the zero at that position; the build's radix select cannot see it and
+    sorts that dimension instead. The slice centres must match a stable sort by the same keys bit
+    for bit (the sign of a zero centre included); the other dimensions, the oracle's."""
+    from patchtable_amd import PatchTable
+    rng = np.random.default_rng(5)
+    h, w, k = 90, 120, 6
+    b = rng.random((h, w, k), dtype=np.float32) - np.float32(0.5)
+    zero = rng.random((h, w)) < 0.6
+    neg = rng.random((h, w)) < 0.5
+    c0 = b[..., 0]
+    c0[zero & neg] = np.float32(-0.0)
+    c0[zero & ~neg] = np.float32(0.0)
+    p = params("-speed 1 -limit 1", is_descriptor=1)
+    t = PatchTable(p, b)
+    o = O.OracleTable(p.raw, b)
+    ns = t.info()["nslices"]
+    assert ns == o.nslices and ns[0] + 1 <= 48   # dimension 0 goes through the select (48 slots)
+    H, W = t.b_shape[0], t.b_shape[1]
+    vals = np.ascontiguousarray(t.descriptor()[7:H, 7:W, 0]).ravel()
+    u = vals.view(np.uint32).copy()
+    u[u == 0x80000000] = 0
+    key = np.where(u & 0x80000000, ~u, u | np.uint32(0x80000000))
+    srt = vals[np.argsort(key, kind="stable")]
+    n, m = len(vals), ns[0]
+    q = np.array([srt[(n - 1) * j // m] for j in range(m + 1)], np.float32)
+    want = ((q[:-1] + q[1:]) / np.float32(2)).astype(np.float32)
+    _, _, gc, _ = t.partition(0)
+    assert np.array_equal(gc.view(np.uint32), want.view(np.uint32))
+    zc = want[want == 0]
+    assert len(zc) >= 2 and np.signbit(zc).any() and (~np.signbit(zc)).any()
+    _, _, oc, _ = o.partition(0)
+    assert np.array_equal(gc, oc)   # equal as numbers: std::sort orders the zeros arbitrarily
+    for d in range(1, len(ns)):
+        gm, gx, gcd, gb = t.partition(d)
+        om, ox, ocd, ob = o.partition(d)
+        assert (gm, gx) == (om, ox)
+        assert np.array_equal(gcd.view(np.uint32), ocd.view(np.uint32)) and np.array_equal(gb, ob)
+
+
 def test_descriptor_distance_lane_order(lib):
     """pt_desc_dist (descriptor-mode distance) equals the oracle's restatement bit for bit for the
     SSE-chunked widths and the scalar-tail widths."""
