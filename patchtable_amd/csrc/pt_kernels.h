@@ -120,7 +120,12 @@ int pt_minmax_launch(const float *desc, size_t plane, int Wp, int h, int w, int 
 // select), into values_dev[d * ld + j]; needs nslices[d] + 1 <= PT_RS_SLOTS. ambig_dev[d] = 1
 // where a value is a zero whose sign the select cannot tell (the dimension holds -0.0 and +0.0):
 // such a dimension takes pt_quantile_values instead.
+// The 16384-entry value->bin maps of G dimensions from their slice centres (centers[d * ldc + k],
+// k < nslices[d]) and min/range (device arrays), into to_bin[d * PT_NMAP + i].
+int pt_to_bin_launch(const float *centers, int ldc, const int *nslices, const float *min_val, const float *range, int G,
+                     int32_t *to_bin, hipStream_t s);
 constexpr int PT_RS_SLOTS = 48;
+constexpr int PT_RS_MAX_WIDTH = 65535;   // partition grid columns the select handles
 int pt_quantile_select(const float *desc, size_t plane, int Wp, int h, int w, int step, int G, const int *nslices,
                        const int32_t *allowed, int aidx, int64_t npart, int ld, float *values_dev, int *ambig_dev,
                        PtScratch &sc, hipStream_t s);

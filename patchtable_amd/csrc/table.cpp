@@ -59,6 +59,8 @@ struct pt_table {
     int table_add_count = 0;
     PinnedBuf pin;                   // staging of the build's small transfers
     bool pin_count_valid = false;    // the populate count was copied into pin
+    hipStream_t aux = nullptr;       // build: populate's shuffle, beside the descriptor work
+    bool shuffle_queued = false;     // ... enqueued there; ev[11] marks its end
     long rand_used = 0;       // rand() outputs consumed by this table's shuffle and the earlier ones
     int K = 0;
     double stage_ms[12] = {};
@@ -69,6 +71,7 @@ struct pt_table {
     int allowed_index = 1;
     DevBuf<float> qmin, qrange;
     DevBuf<int32_t> qtobin;
+    DevBuf<int32_t> qcent;   // slice centres and counts the device bin maps are made from
     DevBuf<int64_t> qstride;
     DevBuf<int32_t> table;
     DevBuf<int32_t> list_xy, rank;
@@ -113,6 +116,7 @@ struct pt_table {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+        if (aux) (void)hipStreamDestroy(aux);
     }
 };
 
@@ -200,9 +204,9 @@ std::vector<int> scale_slices(std::vector<int> ns, double limit) {
     return ns;
 }
 
-// Centres, half sizes and the 16384-entry value->bin map of one dimension
-// (PatchPartition1D constructor, patchtable.h:1193-1251), from the order statistics q of the
-// sorted values: q[i] is the value at rank (n-1)*i/nslices, i = 0..nslices.
+// Centres and half sizes of one dimension (PatchPartition1D constructor, patchtable.h:1193-1231)
+// from the order statistics q of the sorted values: q[i] is the value at rank (n-1)*i/nslices,
+// i = 0..nslices. The value->bin map is made on the device (pt_to_bin_launch) or by host_to_bin.
 void make_partition(HostPartition &P, const std::vector<float> &q) {
     const int ns = P.nslices;
     P.centers.resize(ns);
@@ -220,6 +224,13 @@ void make_partition(HostPartition &P, const std::vector<float> &q) {
         P.rsizes[i] = next_center - P.centers[i];
         if (i + 1 < ns) P.rsizes[i] /= 2;
     }
+    P.to_bin.clear();
+}
+
+// The 16384-entry value->bin map on the host (patchtable.h:1233-1251), for dimensions with more
+// centres than the device path stages.
+void host_to_bin(HostPartition &P) {
+    const int ns = P.nslices;
     P.to_bin.resize(PT_NMAP);
     int icenter = 0;
     for (int i = 0; i < PT_NMAP; i++) {
@@ -238,24 +249,55 @@ void make_partition(HostPartition &P, const std::vector<float> &q) {
     }
 }
 
+// Value->bin map of dimension d into out (PT_NMAP entries): the host copy where there is one
+// (a loaded table), else the device's.
+int fetch_to_bin(const pt_table *t, int d, int32_t *out) {
+    const HostPartition &P = t->part[d];
+    if ((int)P.to_bin.size() == PT_NMAP) {
+        std::copy(P.to_bin.begin(), P.to_bin.end(), out);
+        return PT_OK;
+    }
+    HIPTRY(hipStreamSynchronize(t->stream));
+    HIPTRY(hipMemcpy(out, t->qtobin.p + (size_t)d * PT_NMAP, PT_NMAP * 4, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
 // Byte offsets of the quantiser upload in the table's pinned staging buffer: min, range and
 // stride per dimension, then the bin maps; the populate counter lands after them.
 size_t pin_upload_bytes(int G) { return (size_t)G * 16 + (size_t)G * PT_NMAP * 4; }
 
-// Device copy of the quantiser (min, range, bin maps, strides), staged through pinned memory so
-// that the copies do not wait for the stream. Called once per table: the staging buffer is not
-// rewritten while these copies may still be pending.
+// Device copy of the quantiser (min, range, strides and bin maps), staged through pinned memory
+// so that the copies do not wait for the stream. The bin maps are made on the device from the
+// centres; a loaded table brings its maps along, and a dimension with PT_NMAP or more centres
+// gets its map on the host. Called once per table: the staging buffer is not rewritten while
+// these copies may still be pending.
 int upload_quant(pt_table *t, hipStream_t s) {
     const int G = t->G;
+    int ldc = 1;
+    bool host_maps = true;
+    for (int i = 0; i < G; i++) {
+        ldc = std::max(ldc, t->part[i].nslices);
+        host_maps &= (int)t->part[i].to_bin.size() == PT_NMAP;
+    }
+    if (!host_maps && ldc + 1 > PT_NMAP) {
+        for (int i = 0; i < G; i++) host_to_bin(t->part[i]);
+        host_maps = true;
+    }
     TRY(t->pin.ensure(pin_upload_bytes(G) + 16));
     float *mn = t->pin.at<float>(0), *rg = t->pin.at<float>((size_t)G * 4);
     int64_t *st = t->pin.at<int64_t>((size_t)G * 8);
-    int32_t *tb = t->pin.at<int32_t>((size_t)G * 16);
+    int32_t *tb = t->pin.at<int32_t>((size_t)G * 16);   // G * PT_NMAP words
     for (int i = 0; i < G; i++) {
-        mn[i] = t->part[i].min_val;
-        rg[i] = t->part[i].range;
+        const HostPartition &P = t->part[i];
+        mn[i] = P.min_val;
+        rg[i] = P.range;
         st[i] = t->stride[i];
-        std::copy(t->part[i].to_bin.begin(), t->part[i].to_bin.end(), tb + (size_t)i * PT_NMAP);
+        if (host_maps) {
+            std::copy(P.to_bin.begin(), P.to_bin.end(), tb + (size_t)i * PT_NMAP);
+        } else {   // centres [G][ldc], then the slice counts
+            std::copy(P.centers.begin(), P.centers.end(), (float *)tb + (size_t)i * ldc);
+            tb[(size_t)G * ldc + i] = P.nslices;
+        }
     }
     TRY(t->qmin.ensure(G));
     TRY(t->qrange.ensure(G));
@@ -263,8 +305,16 @@ int upload_quant(pt_table *t, hipStream_t s) {
     TRY(t->qstride.ensure(G));
     HIPTRY(hipMemcpyAsync(t->qmin.p, mn, G * 4, hipMemcpyHostToDevice, s));
     HIPTRY(hipMemcpyAsync(t->qrange.p, rg, G * 4, hipMemcpyHostToDevice, s));
-    HIPTRY(hipMemcpyAsync(t->qtobin.p, tb, (size_t)G * PT_NMAP * 4, hipMemcpyHostToDevice, s));
     HIPTRY(hipMemcpyAsync(t->qstride.p, st, G * 8, hipMemcpyHostToDevice, s));
+    if (host_maps) {
+        HIPTRY(hipMemcpyAsync(t->qtobin.p, tb, (size_t)G * PT_NMAP * 4, hipMemcpyHostToDevice, s));
+    } else {
+        const size_t words = (size_t)G * ldc + G;
+        TRY(t->qcent.ensure(words));
+        HIPTRY(hipMemcpyAsync(t->qcent.p, tb, words * 4, hipMemcpyHostToDevice, s));
+        TRY(pt_to_bin_launch((const float *)t->qcent.p, ldc, t->qcent.p + (size_t)G * ldc, t->qmin.p, t->qrange.p, G,
+                             t->qtobin.p, s));
+    }
     return PT_OK;
 }
 
@@ -298,6 +348,54 @@ void advanced_rand_state(long skip, uint32_t st[31]) {
     }
 }
 
+// Populate's random order (patchtable.h:3121-3159): the allowed patches in raster order and
+// their shuffle ranks, on stream q. It depends only on the patch count and the rand() position,
+// not on the descriptors or the quantiser.
+int enqueue_shuffle(pt_table *t, const std::vector<int32_t> &mask_host, long rand_skip, hipStream_t q) {
+    const pt_params &p = t->build;
+    const int bew = t->w - 7, beh = t->h - 7, aidx = t->allowed_index;
+    int n = bew * beh;
+    TRY(t->list_xy.ensure(n));
+    if (mask_host.empty()) {
+        TRY(pt_list_launch(bew, n, t->list_xy.p, q));
+    } else {
+        std::vector<int32_t> lst;
+        lst.reserve(n);
+        for (int y = 0; y < beh; y++)
+            for (int x = 0; x < bew; x++)
+                if (mask_host[(size_t)y * bew + x] == aidx) lst.push_back(PT_XY_TO_INT(x + 7, y + 7));
+        n = (int)lst.size();
+        HIPTRY(hipMemcpyAsync(t->list_xy.p, lst.data(), lst.size() * 4, hipMemcpyHostToDevice, q));
+        HIPTRY(hipStreamSynchronize(q));
+    }
+    t->n_pop = n;
+    t->rand_used = rand_skip;
+    if (p.populate_random) {
+        if ((int64_t)n > ((int64_t)PT_RAND_CHUNK << PT_RAND_LEVELS))
+            return pt_fail(PT_E_UNSUPPORTED, "too many database patches for the rand() jump tables");
+        uint32_t st[31];
+        advanced_rand_state(rand_skip, st);
+        const uint32_t *jm = pt_glibc_jump_matrices(PT_RAND_CHUNK, PT_RAND_LEVELS);
+        TRY(t->rnd.ensure(31 + (size_t)PT_RAND_LEVELS * 961));
+        HIPTRY(hipMemcpyAsync(t->rnd.p, st, 31 * 4, hipMemcpyHostToDevice, q));
+        HIPTRY(hipMemcpyAsync(t->rnd.p + 31, jm, (size_t)PT_RAND_LEVELS * 961 * 4, hipMemcpyHostToDevice, q));
+        if (n > 1) t->rand_used += n - 1;
+    }
+    TRY(t->rank.ensure(n));
+    return pt_shuffle_ranks(t->rnd.p + 31, t->rnd.p, n, p.populate_random, t->sc, t->rank.p, q);
+}
+
+// The main table's shuffle goes on its second stream before the descriptor work; the GPU runs
+// it beside the GCK's sequential recurrences and the quantiser's host round trips.
+int start_shuffle(pt_table *t, const std::vector<int32_t> &mask_host) {
+    if (!t->aux && hipStreamCreateWithFlags(&t->aux, hipStreamNonBlocking) != hipSuccess)
+        return pt_fail(PT_E_HIP, "hipStreamCreate failed");
+    TRY(enqueue_shuffle(t, mask_host, 0, t->aux));
+    HIPTRY(hipEventRecord(t->ev[11], t->aux));
+    t->shuffle_queued = true;
+    return PT_OK;
+}
+
 // Quantiser, populate and DT of table t (PatchTableFixedN constructor, patchtable.h:2567-3295)
 // from the descriptor planes `desc` (owned by the main table). `mask_host` is the allowed mask
 // (or empty); `rand_skip` the rand() outputs used by earlier tables' shuffles. Records events
@@ -308,8 +406,9 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
     const int h = t->h, w = t->w, Wp = w + 7;
     const size_t plane = (size_t)(h + 7) * Wp;
     const int aidx = t->allowed_index;
+    if (!t->shuffle_queued) TRY(enqueue_shuffle(t, mask_host, rand_skip, s));
     // quantiser: min/max (patchtable.h:1758-1778), slices, order statistics, to_bin
-    const int G = t->G, NB = 256;
+    const int G = t->G, NB = 512;
     float *partials = (float *)t->sc.get(40, (size_t)NB * G * 2 * 4);
     if (!partials) return pt_fail(PT_E_NOMEM, "min/max scratch allocation failed");
     TRY(pt_minmax_launch(desc, plane, Wp, h, w, G, mask_dev, aidx, partials, NB, s));
@@ -365,8 +464,9 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
         if (!vals) return pt_fail(PT_E_NOMEM, "quantile scratch allocation failed");
         int *amb = (int *)(vals + (size_t)G * ld);
         std::vector<int> sel(G, 0), sorted;
+        const bool narrow = (bew + step - 1) / step <= PT_RS_MAX_WIDTH;
         for (int i = 0; i < G; i++) {
-            if (t->nslices[i] + 1 <= PT_RS_SLOTS) sel[i] = t->nslices[i];
+            if (narrow && t->nslices[i] + 1 <= PT_RS_SLOTS) sel[i] = t->nslices[i];
             else sorted.push_back(i);
         }
         TRY(pt_quantile_select(desc, plane, Wp, h, w, step, G, sel.data(), mask_dev, aidx, npart, (int)ld, vals, amb,
@@ -374,6 +474,12 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
         for (int i : sorted)
             TRY(pt_quantile_values(desc, plane, Wp, h, w, step, i, mask_dev, aidx, npart, t->nslices[i], vals + i * ld,
                                    t->sc, s));
+        // buffers of the populate and DT stages, allocated while the GPU runs the select
+        TRY(t->table.ensure((size_t)t->ncells));
+        TRY(t->cells.ensure(t->n_pop));
+        TRY(t->win.ensure(t->n_pop));
+        TRY(t->counter.ensure(1));
+        if (p.run_dt && p.dt_algo == PT_DT_ALGO_PROP) TRY(t->dtkey.ensure((size_t)t->ncells));
         const size_t back = (size_t)G * ld * 4 + (size_t)G * 4;
         TRY(t->pin.ensure(std::max(back, pin_upload_bytes(G) + 16)));
         HIPTRY(hipMemcpyAsync(t->pin.p, vals, back, hipMemcpyDeviceToHost, s));
@@ -401,36 +507,9 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
     }
     TRY(upload_quant(t, s));
     if (evt) HIPTRY(hipEventRecord(evt[2], s));
-    // populate (patchtable.h:3121-3159): allowed patches in raster order, shuffled
-    int n = bew * beh;
-    TRY(t->list_xy.ensure(n));
-    if (mask_host.empty()) {
-        TRY(pt_list_launch(bew, n, t->list_xy.p, s));
-    } else {
-        std::vector<int32_t> lst;
-        lst.reserve(n);
-        for (int y = 0; y < beh; y++)
-            for (int x = 0; x < bew; x++)
-                if (mask_host[(size_t)y * bew + x] == aidx) lst.push_back(PT_XY_TO_INT(x + 7, y + 7));
-        n = (int)lst.size();
-        HIPTRY(hipMemcpyAsync(t->list_xy.p, lst.data(), lst.size() * 4, hipMemcpyHostToDevice, s));
-        HIPTRY(hipStreamSynchronize(s));
-    }
-    t->n_pop = n;
-    t->rand_used = rand_skip;
-    if (p.populate_random) {
-        if ((int64_t)n > ((int64_t)PT_RAND_CHUNK << PT_RAND_LEVELS))
-            return pt_fail(PT_E_UNSUPPORTED, "too many database patches for the rand() jump tables");
-        uint32_t st[31];
-        advanced_rand_state(rand_skip, st);
-        const uint32_t *jm = pt_glibc_jump_matrices(PT_RAND_CHUNK, PT_RAND_LEVELS);
-        TRY(t->rnd.ensure(31 + (size_t)PT_RAND_LEVELS * 961));
-        HIPTRY(hipMemcpyAsync(t->rnd.p, st, 31 * 4, hipMemcpyHostToDevice, s));
-        HIPTRY(hipMemcpyAsync(t->rnd.p + 31, jm, (size_t)PT_RAND_LEVELS * 961 * 4, hipMemcpyHostToDevice, s));
-        if (n > 1) t->rand_used += n - 1;
-    }
-    TRY(t->rank.ensure(n));
-    TRY(pt_shuffle_ranks(t->rnd.p + 31, t->rnd.p, n, p.populate_random, t->sc, t->rank.p, s));
+    // populate (patchtable.h:3121-3159): the shuffled allowed patches claim their cells
+    if (t->shuffle_queued) HIPTRY(hipStreamWaitEvent(s, t->ev[11], 0));
+    const int n = t->n_pop;
     TRY(t->table.ensure((size_t)t->ncells));
     TRY(t->cells.ensure(n));
     TRY(t->win.ensure(n));
@@ -483,6 +562,7 @@ int build_table(pt_table *t, const float *b_dev, int h_in, int w_in, int c, cons
                           t->temp.p, t->desc.p, s));
     }
     HIPTRY(hipEventRecord(t->ev[1], s));
+    TRY(start_shuffle(t, mask_host));   // its allocations overlap the descriptor kernels
     if (!mask_host.empty()) {
         TRY(t->allowed.ensure(mask_host.size()));
         HIPTRY(hipMemcpyAsync(t->allowed.p, mask_host.data(), mask_host.size() * 4, hipMemcpyHostToDevice, s));
@@ -829,6 +909,7 @@ int build_keyed(pt_table *t, const pt_key *key, const float *b_dev, int h, int w
     HIPTRY(hipMemsetAsync(t->desc.p, 0, plane * t->ndims * sizeof(float), s));
     TRY(pt_key_forward_launch(t->b0.p, h, w, t->knet.w1, t->knet.b1, t->knet.w2, t->knet.b2, t->desc.p, s));
     HIPTRY(hipEventRecord(t->ev[1], s));
+    TRY(start_shuffle(t, std::vector<int32_t>()));
     const std::vector<int32_t> no_mask;
     TRY(build_grid(t, t->desc.p, no_mask, nullptr, 0, s, t->ev));
     t->K = p.kcoherence;
@@ -1059,6 +1140,8 @@ extern "C" int pt_table_save(pt_table *t, const char *filename) {
     if (!t->desc_mode)   // descriptor mode keeps no RGB copy; the reference's b0 is unset there
         HIPTRY(hipMemcpy(b0.data(), t->b0.p, b0.size() * 4, hipMemcpyDeviceToHost));
     TRY(planes_to_hwc(t->desc.p, Hp, Wp, t->ndims, wh0.data()));
+    std::vector<std::vector<int32_t>> maps(t->G, std::vector<int32_t>(PT_NMAP));
+    for (int d = 0; d < t->G; d++) TRY(fetch_to_bin(t, d, maps[d].data()));
     FILE *f = fopen(filename, "wb");
     if (!f) return pt_failf(PT_E_IO, "PatchTable: could not save to %s", filename);
     Writer W{f};
@@ -1071,7 +1154,7 @@ extern "C" int pt_table_save(pt_table *t, const char *filename) {
     W.i32(t->G);
     for (const HostPartition &P : t->part) {
         W.str("PatchPartition1D");
-        W.vec(P.to_bin);
+        W.vec(maps[&P - t->part.data()]);
         W.vec(P.centers);
         W.vec(P.lsizes);
         W.vec(P.rsizes);
@@ -1249,7 +1332,7 @@ extern "C" int pt_table_get_partition(const pt_table *t, int dim, float *min_val
     if (min_val) *min_val = P.min_val;
     if (max_val) *max_val = P.max_val;
     if (centers) std::copy(P.centers.begin(), P.centers.end(), centers);
-    if (to_bin) std::copy(P.to_bin.begin(), P.to_bin.end(), to_bin);
+    if (to_bin) TRY(fetch_to_bin(t, dim, to_bin));
     return PT_OK;
 }
 
