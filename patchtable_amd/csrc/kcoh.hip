@@ -33,13 +33,15 @@ struct KeyArgs {
 };
 
 // Keys carry a flag above the Morton bits for patches outside the mask, which sorts them last.
+// Each patch's nd descriptor values are also written side by side (dI), for the sorted gathers.
 __global__ void k_keys(const float *__restrict__ desc, size_t plane, int Wp, int bew, long N, DevQuant q, KeyArgs ka,
                        const int32_t *__restrict__ allowed, int aidx, uint64_t *__restrict__ k0,
-                       uint64_t *__restrict__ k1, int32_t *__restrict__ idx) {
+                       uint64_t *__restrict__ k1, int32_t *__restrict__ idx, int nd, float *__restrict__ dI) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int px = (int)(i % bew), py = (int)(i / bew);
     size_t off = (size_t)(py + 7) * Wp + px + 7;
+    for (int c = 0; c < nd; c++) dI[i * nd + c] = desc[c * plane + off];
     uint32_t bins[PT_MAX_DIMS];
     for (int d = 0; d < ka.Gk; d++)
         bins[d] = (uint32_t)pt_get_bin(desc[d * plane + off], q.min_val[d], q.range[d], q.to_bin + (size_t)d * PT_NMAP);
@@ -68,14 +70,29 @@ __global__ void k_positions(const int32_t *__restrict__ perm, long N, int32_t *_
     if (r < N) pos[perm[r]] = (int32_t)r;
 }
 
-__global__ void k_gather_desc(const float *__restrict__ desc, size_t plane, int Wp, int bew, int nd,
-                              const int32_t *__restrict__ perm, long N, float *__restrict__ ds) {
-    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// Descriptors in both sorted orders: ds0[r] = dI[perm0[r]], ds1[r] = dI[perm1[r]] (nd floats each).
+template <int ND>
+__global__ void k_gather_desc(const float *__restrict__ dI, int nd_rt, const int32_t *__restrict__ perm0,
+                              const int32_t *__restrict__ perm1, long N, float *__restrict__ ds0,
+                              float *__restrict__ ds1) {
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= N) return;
-    int i = perm[r];
-    int px = i % bew, py = i / bew;
-    size_t off = (size_t)(py + 7) * Wp + px + 7;
-    for (int c = 0; c < nd; c++) ds[r * nd + c] = desc[c * plane + off];
+    const int nd = ND > 0 ? ND : nd_rt;
+    const float *a = dI + (size_t)perm0[r] * nd, *b = dI + (size_t)perm1[r] * nd;
+    float *oa = ds0 + (size_t)r * nd, *ob = ds1 + (size_t)r * nd;
+    if constexpr (ND == 6) {   // 24-byte rows: three 8-byte loads each
+#pragma unroll
+        for (int c = 0; c < 6; c += 2) {
+            const float2 va = *(const float2 *)(a + c), vb = *(const float2 *)(b + c);
+            *(float2 *)(oa + c) = va;
+            *(float2 *)(ob + c) = vb;
+        }
+    } else {
+        for (int c = 0; c < nd; c++) {
+            oa[c] = a[c];
+            ob[c] = b[c];
+        }
+    }
 }
 
 // One lane per database patch p, in 64 x 4 raster tiles whose pixels are staged in LDS for the
@@ -293,8 +310,10 @@ int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, con
     if (!k0 || !k1 || !k0s || !k1s || !idx || !perm0 || !perm1 || !pos0 || !pos1 || !ds0 || !ds1)
         return pt_fail(PT_E_NOMEM, "kcoherence: scratch allocation failed");
     const int TB = 256;
+    float *dI = (float *)sc.get(33, (size_t)N * ndims * 4);
+    if (!dI) return pt_fail(PT_E_NOMEM, "kcoherence: scratch allocation failed");
     hipLaunchKernelGGL(k_keys, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, N, q, ka, allowed, aidx, k0,
-                       k1, idx);
+                       k1, idx, ndims, dI);
     long Na = N;   // allowed patches
     if (allowed) {
         int *cnt = (int *)sc.get(32, sizeof(int));
@@ -314,8 +333,12 @@ int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, con
     PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k1, k1s, idx, perm1, (int)N, 0, nbits, s));
     hipLaunchKernelGGL(k_positions, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, perm0, N, pos0);
     hipLaunchKernelGGL(k_positions, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, perm1, N, pos1);
-    hipLaunchKernelGGL(k_gather_desc, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, ndims, perm0, N, ds0);
-    hipLaunchKernelGGL(k_gather_desc, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, desc, plane, Wp, bew, ndims, perm1, N, ds1);
+    if (ndims == 6)
+        hipLaunchKernelGGL(k_gather_desc<6>, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, dI, ndims, perm0, perm1, N, ds0,
+                           ds1);
+    else
+        hipLaunchKernelGGL(k_gather_desc<0>, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, dI, ndims, perm0, perm1, N, ds0,
+                           ds1);
     dim3 grid((bew + pt_tile::TILE_X - 1) / pt_tile::TILE_X, (beh + pt_tile::TILE_Y - 1) / pt_tile::TILE_Y);
     dim3 blk(pt_tile::THREADS);
 #define PT_SELECT(KM, ND, DESC)                                                                                 \
