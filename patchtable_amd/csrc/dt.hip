@@ -106,40 +106,53 @@ __global__ __launch_bounds__(64) void k_dt_line_reg(uint64_t *__restrict__ key, 
         if (t < len) p[(int64_t)t * st] = v[t];
 }
 
-// The same sweeps for the last dimension, whose lines are contiguous: a block stages
-// `lpb` consecutive lines in LDS with coalesced loads, sweeps each line from LDS, writes back.
-// With `table` (the first pass) the keys are made from the table as k_dt_init makes them.
-__global__ void k_dt_line_contig(uint64_t *__restrict__ key, int64_t nlines, int len, int lpb,
-                                 const int32_t *__restrict__ table) {
+// The innermost m dimensions at once. For each index of the outer dimensions their cells form a
+// contiguous slab; a block stages `spb` consecutive slabs (<= PT_DT_SLAB cells) in LDS with
+// coalesced loads, runs both sweeps along each slab dimension in turn, innermost first (the
+// order of the separate passes), and writes the keys back once. With `table` (the first pass)
+// the keys are made from the table as k_dt_init makes them.
+struct SlabDims {
+    int m;
+    int len[PT_MAX_DIMS];   // slab dimensions, innermost first
+    int st[PT_MAX_DIMS];    // their strides in cells
+};
+constexpr int PT_DT_SLAB = 6144;   // cells of the staged slabs: 48 KB of keys
+
+__global__ __launch_bounds__(256) void k_dt_slab(uint64_t *__restrict__ key, int64_t ncells, int P, int spb, SlabDims S,
+                                                 const int32_t *__restrict__ table) {
     extern __shared__ uint64_t sh[];
-    const int64_t l0 = (int64_t)blockIdx.x * lpb;
-    const int nl = (int)(nlines - l0 < lpb ? nlines - l0 : lpb);
-    uint64_t *g = key + l0 * len;
-    const int total = nl * len;
+    const int64_t c0 = (int64_t)blockIdx.x * spb * P;
+    const int total = (int)(ncells - c0 < (int64_t)spb * P ? ncells - c0 : (int64_t)spb * P);
+    uint64_t *g = key + c0;
     if (table) {
-        const int64_t c0 = l0 * len;
         for (int i = threadIdx.x; i < total; i += blockDim.x)
             sh[i] = table[c0 + i] != PT_TABLE_UNUSED ? (uint64_t)(c0 + i) : KEY_INF;
     } else {
         for (int i = threadIdx.x; i < total; i += blockDim.x) sh[i] = g[i];
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-        uint64_t *p = sh + l * len;
-        uint64_t run = KEY_INF;
-        for (int t = 0; t < len; t++) {
-            if (run != KEY_INF) run += KEY_STEP;
-            if (p[t] < run) run = p[t];
-            p[t] = run;
+    for (int k = 0; k < S.m; k++) {
+        const int len = S.len[k], st = S.st[k];
+        const int nl = total / len;   // lines of dimension k in the staged slabs
+        for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+            uint64_t *p = sh + (l / st) * (len * st) + (l % st);
+            uint64_t run = KEY_INF;
+            for (int t = 0; t < len; t++) {
+                if (run != KEY_INF) run += KEY_STEP;
+                const uint64_t v = p[t * st];
+                if (v < run) run = v;
+                p[t * st] = run;
+            }
+            run = KEY_INF;
+            for (int t = len - 1; t >= 0; t--) {
+                if (run != KEY_INF) run += KEY_STEP;
+                const uint64_t v = p[t * st];
+                if (v < run) run = v;
+                p[t * st] = run;
+            }
         }
-        run = KEY_INF;
-        for (int t = len - 1; t >= 0; t--) {
-            if (run != KEY_INF) run += KEY_STEP;
-            if (p[t] < run) run = p[t];
-            p[t] = run;
-        }
+        __syncthreads();
     }
-    __syncthreads();
     for (int i = threadIdx.x; i < total; i += blockDim.x) g[i] = sh[i];
 }
 
@@ -361,23 +374,37 @@ int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells,
                       uint64_t *key, hipStream_t s) {
     const int TB = 256;
     const int64_t max_dist = dt_iters > 0 ? (int64_t)dt_iters + 1 : -1;
-    // passes run d = G-1 .. 0; the first (contiguous) pass makes the keys from the table and the
-    // last (strided, in registers) pass writes the table, when those kernels apply
-    const bool init_fused = nslices[G - 1] <= 256;
+    // passes run d = G-1 .. 0. The innermost dimensions whose cells fit PT_DT_SLAB go in one
+    // slab pass, which also makes the keys from the table; the last (strided, in registers)
+    // pass writes the table when dimension 0 is outside the slab and short.
+    SlabDims S{};
+    int64_t P = 1;
+    while (S.m < G && P * nslices[G - 1 - S.m] <= PT_DT_SLAB) {
+        S.len[S.m] = nslices[G - 1 - S.m];
+        S.st[S.m] = (int)P;
+        P *= nslices[G - 1 - S.m];
+        S.m++;
+    }
     int64_t st0 = 1;   // stride of dimension 0
     for (int d = 1; d < G; d++) st0 *= nslices[d];
-    const bool final_fused = st0 > 1 && nslices[0] <= 64;
-    if (!init_fused) hipLaunchKernelGGL(k_dt_init, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, table, ncells, key);
+    const bool final_fused = st0 > 1 && nslices[0] <= 64 && S.m < G;
     int64_t st = 1;
-    for (int d = G - 1; d >= 0; d--) {
+    int dstart = G - 1;
+    if (S.m > 0) {
+        const int spb = (int)(PT_DT_SLAB / P);
+        const int64_t nblk = (ncells + (int64_t)spb * P - 1) / ((int64_t)spb * P);
+        hipLaunchKernelGGL(k_dt_slab, dim3((unsigned)nblk), dim3(256), (size_t)spb * P * 8, s, key, ncells, (int)P, spb,
+                           S, (const int32_t *)table);
+        st = P;
+        dstart = G - 1 - S.m;
+    } else {
+        hipLaunchKernelGGL(k_dt_init, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, table, ncells, key);
+    }
+    for (int d = dstart; d >= 0; d--) {
         const int64_t nlines = ncells / nslices[d];
         const int len = nslices[d];
         const dim3 g64(pt_blocks(nlines, 64));
-        if (st == 1 && len <= 256) {
-            const int lpb = 8192 / len < 256 ? 8192 / len : 256;   // <= 64 KB of keys
-            hipLaunchKernelGGL(k_dt_line_contig, dim3(pt_blocks(nlines, lpb)), dim3(256), (size_t)lpb * len * 8, s,
-                               key, nlines, len, lpb, d == G - 1 && init_fused ? (const int32_t *)table : nullptr);
-        } else if (d == 0 && final_fused) {
+        if (d == 0 && final_fused) {
             if (len <= 32)
                 hipLaunchKernelGGL((k_dt_line_reg<32, true>), g64, dim3(64), 0, s, key, nlines, len, st, table,
                                    max_dist, fill);
