@@ -232,14 +232,84 @@ __device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const fl
     return changed;
 }
 
-// Spatial search through the block queue (radius 1: at most 8 positions). `want` is false for
-// lanes that skip it; every lane of the block must call this. Returns true if improved.
-template <int M>
+// The 3 x 3 window around an interior best (1 <= xb <= bew-2, 1 <= yb <= beh-2) evaluated by
+// the calling lane alone, for the candidates in `todo` (slot order: row by row, centre left
+// out). The candidates of one window row (dy) read their 8 patch rows together: 30 floats, 8
+// loads per row instead of 6 per candidate row. Each candidate keeps the four lane-order
+// accumulators of ssd_staged in its order and stops once a partial sum after a row other than
+// the last reaches `bound`, so res[slot] is what qdist returns for it.
+__device__ __forceinline__ void spatial_local(const LookupArgs &L, const float4 *qbuf, int xb, int yb, float bound,
+                                              unsigned todo, float res[8]) {
+    const float4 *aq = qbuf + (threadIdx.x / TILE_X) * AX + (threadIdx.x % TILE_X);
+#pragma unroll
+    for (int g = 0; g < 3; g++) {
+        const int s0 = g == 0 ? 0 : (g == 1 ? 3 : 5);   // first slot of window row dy = g - 1
+        const int nc = g == 1 ? 2 : 3;                  // the centre row holds dx = -1 and +1
+        unsigned alive = (todo >> s0) & ((1u << nc) - 1u);
+        if (!alive) continue;
+        float acc[3][4];
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) acc[j][e] = 0.f;
+        const float *bw0 = L.b + ((size_t)(yb + g - 1) * L.bw + (xb - 1)) * 3;
+#pragma unroll 1
+        for (int r = 0; r < 8 && alive; r++) {
+            const float *br = bw0 + (size_t)r * L.bw * 3;
+            float wv[30];
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                const float4 v = ldg4(br + 4 * k);
+                wv[4 * k] = v.x;
+                wv[4 * k + 1] = v.y;
+                wv[4 * k + 2] = v.z;
+                wv[4 * k + 3] = v.w;
+            }
+            wv[28] = br[28];
+            wv[29] = br[29];
+            float av[24];
+            const float4 *ar = aq + r * AX;
+#pragma unroll
+            for (int px = 0; px < 8; px++) {
+                const float4 p = ar[px];
+                av[3 * px] = p.x;
+                av[3 * px + 1] = p.y;
+                av[3 * px + 2] = p.z;
+            }
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                if (j >= nc || !((alive >> j) & 1u)) continue;
+                const int off = g == 1 ? 6 * j : 3 * j;   // 3 * (dx + 1)
+#pragma unroll
+                for (int e = 0; e < 24; e++) {
+                    const float d = av[e] - wv[off + e];
+                    acc[j][e & 3] = acc[j][e & 3] + d * d;
+                }
+                const float part = ((acc[j][0] + acc[j][1]) + acc[j][2]) + acc[j][3];
+                if (r == 7 || part >= bound) {
+                    res[s0 + j] = part;
+                    alive &= ~(1u << j);
+                }
+            }
+        }
+    }
+}
+
+// Spatial search (radius 1: at most 8 positions). `want` is false for lanes that skip it; every
+// lane of the block must call this. Returns true if improved. In image mode with query_step 1,
+// when most lanes of the wave search, a lane whose window lies inside b evaluates it alone
+// (spatial_local); the others go through the block queue.
+template <int M, bool LOCAL>
 __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L, bool want,
                                                  int &xb, int &yb, float &db, Hist &hist) {
     Window w = spatial_window(L, xb, yb);
     if (!want) w.ymax = w.ymin;   // empty
     if (want) Q.bound[threadIdx.x] = db;
+    bool local = false;
+    if constexpr (M == M_TILE && LOCAL) {
+        const bool interior = want && xb >= 1 && yb >= 1 && xb <= L.bw - 9 && yb <= L.bh - 9;
+        local = interior && __popcll(__ballot(want)) >= 32;
+    }
     unsigned pushed = 0;
     int slot = 0;
     for (int dy = -1; dy <= 1; dy++)
@@ -253,17 +323,21 @@ __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qb
                 pushed |= 1u << slot;
                 hist.add(PT_XY_TO_INT(xs, ys));
             }
-            q_push(Q, in, slot, PT_XY_TO_INT(xs, ys));
+            q_push(Q, in && !local, slot, PT_XY_TO_INT(xs, ys));
             slot++;
         }
     q_run<M>(Q, qbuf, L);
+    float lres[8];
+    if constexpr (M == M_TILE && LOCAL)
+        if (local && pushed) spatial_local(L, qbuf, xb, yb, db, pushed, lres);
     bool changed = false;
     for (int s = 0; s < 8; s++) {
         if (!((pushed >> s) & 1u)) continue;
-        const float d = Q.res[threadIdx.x * SLOTS + s];
+        const float d = local ? lres[s] : Q.res[threadIdx.x * SLOTS + s];
         if (d < db) {
             db = d;
-            const int c = Q.cxy[threadIdx.x * SLOTS + s];
+            // slot s sits at (c_dx[s], c_dy[s]) from the window's centre
+            const int c = local ? PT_XY_TO_INT(w.xb0 + c_dx[s], w.yb0 + c_dy[s]) : Q.cxy[threadIdx.x * SLOTS + s];
             xb = PT_INT_TO_X(c);
             yb = PT_INT_TO_Y(c);
             changed = true;
@@ -413,7 +487,7 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
     if (L.K > 0 && L.kcoh_iter0) kd = !kcoh_walk_q<M>(Q, qbuf, L, valid, xb, yb, db, hist);
     bool done = false;
     if (L.spatial_cand && L.spatial == 1) {
-        const bool changed = spatial_search_q<M>(Q, qbuf, L, valid, xb, yb, db, hist);
+        const bool changed = spatial_search_q<M, false>(Q, qbuf, L, valid, xb, yb, db, hist);
         done = !changed;
         if (changed) kd = false;
     } else if (L.spatial_cand && L.spatial > 1 && valid) {
@@ -513,7 +587,7 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
     bool schg = false;
     const bool sp = round >= L.spatial_from;   // rounds that run the spatial window
     if (sp && L.spatial == 1) {
-        schg = spatial_search_q<M>(Q, qbuf, L, valid && !done, xb, yb, db, hist);
+        schg = spatial_search_q<M, true>(Q, qbuf, L, valid && !done, xb, yb, db, hist);
         if (valid && !done) done = !schg;
     } else if (sp && L.spatial > 1 && valid && !done) {
         schg = spatial_search_seq<M>(L, qbuf, xb, yb, db, hist);
