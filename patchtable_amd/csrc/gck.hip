@@ -142,7 +142,23 @@ __global__ void k_recur_x(float *__restrict__ desc, int Hp, int Wp, int chk, int
         float *K = desc + (size_t)(chk + c) * plane + (size_t)y * Wp;
         const float *P = desc + (size_t)(chp + c) * plane + (size_t)y * Wp;
         float kprev = zero, pprev = zero;
-        for (int x = r; x < Wp; x += delta) {
+        int x = r;
+        // eight steps' inputs are loaded ahead: they do not depend on the chain
+        for (; x + 7 * delta < Wp; x += 8 * delta) {
+            float cur[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) cur[u] = P[x + u * delta];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const float s1 = neg ? -kprev : kprev;
+                const float s2 = neg ? -pprev : pprev;
+                const float v = (s1 + cur[u]) + s2;
+                K[x + u * delta] = v;
+                kprev = v;
+                pprev = cur[u];
+            }
+        }
+        for (; x < Wp; x += delta) {
             float cur = P[x];
             float s1 = neg ? -kprev : kprev;
             float s2 = neg ? -pprev : pprev;
@@ -165,7 +181,22 @@ __global__ void k_recur_y(float *__restrict__ desc, int Hp, int Wp, int chk, int
         float *K = desc + (size_t)(chk + c) * plane + x;
         const float *P = desc + (size_t)(chp + c) * plane + x;
         float kprev = zero, pprev = zero;
-        for (int y = r; y < Hp; y += delta) {
+        int y = r;
+        for (; y + 7 * delta < Hp; y += 8 * delta) {
+            float cur[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) cur[u] = P[(size_t)(y + u * delta) * Wp];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const float s1 = neg ? -kprev : kprev;
+                const float s2 = neg ? -pprev : pprev;
+                const float v = (s1 + cur[u]) + s2;
+                K[(size_t)(y + u * delta) * Wp] = v;
+                kprev = v;
+                pprev = cur[u];
+            }
+        }
+        for (; y < Hp; y += delta) {
             float cur = P[(size_t)y * Wp];
             float s1 = neg ? -kprev : kprev;
             float s2 = neg ? -pprev : pprev;
