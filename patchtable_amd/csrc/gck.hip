@@ -38,6 +38,13 @@ __global__ void k_to_planes(const float *__restrict__ img, long npx, int convert
     planes[2 * npx + i] = V;
 }
 
+// Steps of a running sum whose inputs a lane loads before adding them up: the chains are
+// sequential, so loads in flight per lane are what hides the memory latency.
+constexpr int GCK_BATCH = 32;
+// ... along x, where a lane's steps are `delta` apart: 8 (32 spans a quarter row per batch and
+// measured slower)
+constexpr int GCK_BATCH_X = 8;
+
 // Horizontal running box sum (gck.h:54-165), one lane per (row, channel) chain:
 //   t(y,0) = in(y-3,0);  t(y,x) = (t(y,x-1) + in(y-3,x)) - in(y-3,x-8);
 // reads outside the image are get_zero(), i.e. in(0,0,0) * 0.
@@ -69,15 +76,15 @@ __global__ void k_hpass(const float *__restrict__ yuv, int h, int w, float *__re
     }
     // Interior: both terms inside the image.
     const int xend = w < Wp ? w : Wp;
-    for (; x + 8 <= xend; x += 8) {
-        float add[8], sub[8];
+    for (; x + GCK_BATCH <= xend; x += GCK_BATCH) {
+        float add[GCK_BATCH], sub[GCK_BATCH];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < GCK_BATCH; k++) {
             add[k] = row[x + k];
             sub[k] = row[x + k - 8];
         }
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < GCK_BATCH; k++) {
             t = (t + add[k]) - sub[k];
             out[x + k] = t;
         }
@@ -108,15 +115,15 @@ __global__ void k_vpass(const float *__restrict__ temp, int Hp, int Wp, float *_
         o = (o + add) - zero;
         O[(size_t)y * Wp] = o;
     }
-    for (; y + 8 <= Hp - 3; y += 8) {
-        float add[8], sub[8];
+    for (; y + GCK_BATCH <= Hp - 3; y += GCK_BATCH) {
+        float add[GCK_BATCH], sub[GCK_BATCH];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < GCK_BATCH; k++) {
             add[k] = T[(size_t)(y + k + 3) * Wp];
             sub[k] = T[(size_t)(y + k - 5) * Wp];
         }
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < GCK_BATCH; k++) {
             o = (o + add[k]) - sub[k];
             O[(size_t)(y + k) * Wp] = o;
         }
@@ -143,13 +150,13 @@ __global__ void k_recur_x(float *__restrict__ desc, int Hp, int Wp, int chk, int
         const float *P = desc + (size_t)(chp + c) * plane + (size_t)y * Wp;
         float kprev = zero, pprev = zero;
         int x = r;
-        // eight steps' inputs are loaded ahead: they do not depend on the chain
-        for (; x + 7 * delta < Wp; x += 8 * delta) {
-            float cur[8];
+        // GCK_BATCH_X steps' inputs are loaded ahead: they do not depend on the chain
+        for (; x + (GCK_BATCH_X - 1) * delta < Wp; x += GCK_BATCH_X * delta) {
+            float cur[GCK_BATCH_X];
 #pragma unroll
-            for (int u = 0; u < 8; u++) cur[u] = P[x + u * delta];
+            for (int u = 0; u < GCK_BATCH_X; u++) cur[u] = P[x + u * delta];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < GCK_BATCH_X; u++) {
                 const float s1 = neg ? -kprev : kprev;
                 const float s2 = neg ? -pprev : pprev;
                 const float v = (s1 + cur[u]) + s2;
@@ -182,12 +189,12 @@ __global__ void k_recur_y(float *__restrict__ desc, int Hp, int Wp, int chk, int
         const float *P = desc + (size_t)(chp + c) * plane + x;
         float kprev = zero, pprev = zero;
         int y = r;
-        for (; y + 7 * delta < Hp; y += 8 * delta) {
-            float cur[8];
+        for (; y + (GCK_BATCH - 1) * delta < Hp; y += GCK_BATCH * delta) {
+            float cur[GCK_BATCH];
 #pragma unroll
-            for (int u = 0; u < 8; u++) cur[u] = P[(size_t)(y + u * delta) * Wp];
+            for (int u = 0; u < GCK_BATCH; u++) cur[u] = P[(size_t)(y + u * delta) * Wp];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < GCK_BATCH; u++) {
                 const float s1 = neg ? -kprev : kprev;
                 const float s2 = neg ? -pprev : pprev;
                 const float v = (s1 + cur[u]) + s2;
