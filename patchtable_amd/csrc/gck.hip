@@ -74,7 +74,7 @@ __global__ void k_hpass(const float *__restrict__ yuv, int h, int w, float *__re
         t = (t + add) - zero;
         out[x] = t;
     }
-    // Interior: both terms inside the image.
+    // Interior: both terms inside the image (the subtracted one is in the lines just read).
     const int xend = w < Wp ? w : Wp;
     for (; x + GCK_BATCH <= xend; x += GCK_BATCH) {
         float add[GCK_BATCH], sub[GCK_BATCH];
@@ -115,18 +115,25 @@ __global__ void k_vpass(const float *__restrict__ temp, int Hp, int Wp, float *_
         o = (o + add) - zero;
         O[(size_t)y * Wp] = o;
     }
-    for (; y + GCK_BATCH <= Hp - 3; y += GCK_BATCH) {
+    // The term subtracted at y, t(y-5), was added at y - 8: after the first batch it comes from
+    // registers.
+    float prev[8];
+    for (int first = 1; y + GCK_BATCH <= Hp - 3; y += GCK_BATCH, first = 0) {
         float add[GCK_BATCH], sub[GCK_BATCH];
 #pragma unroll
+        for (int k = 0; k < GCK_BATCH; k++) add[k] = T[(size_t)(y + k + 3) * Wp];
+#pragma unroll
         for (int k = 0; k < GCK_BATCH; k++) {
-            add[k] = T[(size_t)(y + k + 3) * Wp];
-            sub[k] = T[(size_t)(y + k - 5) * Wp];
+            if (k < 8) sub[k] = first ? T[(size_t)(y + k - 5) * Wp] : prev[k];
+            else sub[k] = add[k - 8];
         }
 #pragma unroll
         for (int k = 0; k < GCK_BATCH; k++) {
             o = (o + add[k]) - sub[k];
             O[(size_t)(y + k) * Wp] = o;
         }
+#pragma unroll
+        for (int k = 0; k < 8; k++) prev[k] = add[GCK_BATCH - 8 + k];
     }
     for (; y < Hp; y++) {
         float add = (y + 3 < Hp) ? T[(size_t)(y + 3) * Wp] : zero;
