@@ -14,8 +14,9 @@
 //     G(k) = min{ m > k : j_m = j_k }     (next hit of the landing position)
 //     F(q) = min{ m > q : j_m = q }       (next hit of position q once occupied at step q)
 // the final position of element k is j_k if G(k) does not exist, else the end of the chain
-// G(k), F(G(k)), F(F(G(k))), ... . G and F come from one stable sort of (j_m, m); chain ends come
-// from pointer jumping.
+// G(k), F(G(k)), F(F(G(k))), ... . G and F come from one stable sort of (j_m, m). F(q) > q, so
+// every chain strictly increases and ends; position q is hit after step q about ln(n/q) times
+// (step m hits it with probability 1/(m+1)), so chains are short and each lane walks its own.
 #include "pt_device.h"
 #include "pt_kernels.h"
 
@@ -91,16 +92,18 @@ __global__ void k_next_hits(const uint32_t *__restrict__ js, const int32_t *__re
     }
 }
 
-__global__ void k_succ_init(const int32_t *__restrict__ F, int n, int32_t *__restrict__ succ) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
+// End of the chain q, F(q), F(F(q)), ...: the first position whose F is -1. F strictly
+// increases along the chain, so the walk takes fewer than n steps.
+__global__ void k_chain_end(const int32_t *__restrict__ F, int n, int32_t *__restrict__ fin) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    succ[q] = F[q] >= 0 ? F[q] : q;
-}
-
-__global__ void k_pointer_jump(const int32_t *__restrict__ src, int n, int32_t *__restrict__ dst) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    dst[q] = src[src[q]];
+    int32_t c = q;
+    for (int it = 0; it < n; it++) {
+        const int32_t f = F[c];
+        if (f < 0) break;
+        c = f;
+    }
+    fin[q] = c;
 }
 
 __global__ void k_ranks(const int32_t *__restrict__ Gk, const int32_t *__restrict__ fin, const uint32_t *__restrict__ j,
@@ -199,13 +202,8 @@ int pt_shuffle_ranks(const uint32_t *jump_dev, const uint32_t *state_dev, int n,
     PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, j, js, m, ms, n, 0, bits, s));
     PT_HIP_TRY(hipMemsetAsync(F, 0xff, (size_t)n * 4, s));
     hipLaunchKernelGGL(k_next_hits, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, js, ms, n, Gk, F);
-    hipLaunchKernelGGL(k_succ_init, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, F, n, s0);
-    int32_t *a = s0, *b = m;   // m is free after the sort
-    for (int r = 0; (1 << r) < 2 * n; r++) {
-        hipLaunchKernelGGL(k_pointer_jump, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, a, n, b);
-        std::swap(a, b);
-    }
-    hipLaunchKernelGGL(k_ranks, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, Gk, a, j, n, rank);
+    hipLaunchKernelGGL(k_chain_end, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, F, n, s0);
+    hipLaunchKernelGGL(k_ranks, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, Gk, s0, j, n, rank);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }
