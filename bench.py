@@ -366,7 +366,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the PatchTable hot path is built and timed on a single "
-              "MI355X; multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+              "MI355X; multi-GPU sharding is not run.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -39,7 +39,7 @@ extern "C" {
 
 #define PT_OK              0
 #define PT_E_INVALID      -1   /* invalid argument or parameter value                      */
-#define PT_E_UNSUPPORTED  -2   /* parameter combination outside this build's scope         */
+#define PT_E_UNSUPPORTED  -2   /* parameter combination outside this library's scope       */
 #define PT_E_HIP          -3   /* a HIP runtime call failed                                */
 #define PT_E_IO           -4   /* file could not be read or written                        */
 #define PT_E_NOMEM        -5   /* allocation failed                                        */
@@ -71,7 +71,7 @@ extern "C" {
 /* Plain-old-data mirror of PatchTableParams (patchtable_params.h:23-161). Field names and
  * meanings follow the reference; switch names accepted by pt_params_set_switches are the
  * reference's ("-speed", "-limit", "-ndims", ...). Fields that only select code paths outside
- * this build's scope are kept so that switch strings parse the same way; pt_table_create rejects
+ * this library's scope are kept so that switch strings parse the same way; pt_table_create rejects
  * values it does not implement with PT_E_UNSUPPORTED. */
 typedef struct pt_params {
     int    randomize_dt;
@@ -169,7 +169,7 @@ typedef struct pt_params {
     int    ann_bgrid;
     double ann_eps;
 
-    /* ---- Parameters of this build's GPU algorithms (no reference counterpart). ---------------
+    /* ---- Parameters of this library's GPU algorithms (no reference counterpart). -------------
      * kcoh_window: half-width of the sorted-key candidate window of the k-coherence kNN.
      * jf_rounds:   number of parallel propagation rounds that replace the raster-order passes.
      * Set by pt_params_default; accepted as switches "-kcoh_window" and "-jf_rounds". */

@@ -2,7 +2,7 @@
 // PatchTable ORACLE -- TEST INFRASTRUCTURE ONLY.
 //
 // A CPU restatement of the reference's hot path (uva-graphics/patchtable), written from the
-// reference's source as the checker for the HIP implementation. Only tests/, the smoke check
+// reference's source to check the HIP implementation against. Only tests/, the smoke check
 // and the CPU-baseline leg of bench.py load it; the product library never links or calls it.
 //
 // Every function cites the reference file:line it follows. Floating-point expressions keep the
@@ -21,7 +21,7 @@
 //   dt prop (BFS) .................. hand-worked known answers on tiny tables; invariants
 //   dt downsample / euclidean ...... brute-force definitions on tiny tables
 //   patch SSD ...................... float64 recompute, SSE lane order restated
-//   k-coherence .................... this build's own kNN (the reference uses FLANN, absent
+//   k-coherence .................... this library's own kNN (the reference uses FLANN, absent
 //                                    and unpinned): parity unpinned vs the reference; recall
 //                                    against exact kNN is measured instead
 //   lookup (raster) ................ invariants of stress.py:134-154; quality only
@@ -689,7 +689,7 @@ static void dt_euclidean(Table &T) {
 }
 
 // ---------------------------- k-coherence ---------------------------------------------------
-// This build's k-nearest-neighbour lists (replaces FLANN, patchtable.h:2653-2881). For every
+// This library's k-nearest-neighbour lists (replaces FLANN, patchtable.h:2653-2881). For every
 // database patch p:
 //   1. candidates: the patches within `window` positions of p in two sort orders of all patches,
 //      by the Morton interleave of their table bins and by the Morton interleave of the bins
@@ -721,7 +721,7 @@ static uint64_t morton_key(const Table &T, const int *bins, int shifted, int B) 
     return key;
 }
 
-// This build's k-coherence lists (the reference searches a FLANN kd-tree, patchtable.h:2653-2881):
+// This library's k-coherence lists (the reference searches a FLANN kd-tree, patchtable.h:2653-2881):
 // over the allowed database patches (TABLE_FOR_ALLOWED_PATCHES_STEP, patchtable.h:2711), sorted
 // by two Morton keys of their table bins, the candidates of p are the patches within W sorted
 // positions in either order at pixel distance >= kcoherence_min_dist; the K nearest in
@@ -1246,7 +1246,7 @@ static void lookup_raster(const Table &T, const float *a_in, int h_in, int w_in,
     }
 }
 
-// ---------------------------- lookup (this build's parallel form) ---------------------------
+// ---------------------------- lookup (this library's parallel form) ---------------------------
 // Model of the GPU lookup. The reference visits the query patches of the lattice
 // {x % query_step == 0, y % query_step == 0} in raster order and scatters each visited patch's
 // offset to its (2*query_step+1)^2 neighbourhood. Here:
@@ -1583,7 +1583,7 @@ static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_i
 }
 
 // ---------------------------- learned key (MLP 192 -> 64 -> 8) --------------------------------
-// The key of this build's learned-key table: x = patch - 0.5 in bf16 (HWC order), h =
+// The key of this library's learned-key table: x = patch - 0.5 in bf16 (HWC order), h =
 // ReLU(W1 x + b1), k = W2 h + b2 with W1, W2 rounded to bf16 and biases in float; sums here in
 // double (the GPU accumulates in FP32 on MFMA / VALU, so the two agree to FP32 rounding).
 static inline float bf16_round(float f) {

@@ -2,7 +2,7 @@
 //
 // For every database patch p the reference keeps its k nearest database patches in descriptor
 // space that lie at pixel distance >= kcoherence_min_dist, re-scored by exact RGB SSD and sorted
-// by it. FLANN's arithmetic is third-party and unpinned; this build finds candidates with a
+// by it. FLANN's arithmetic is third-party and unpinned; this library finds candidates with a
 // data-parallel approximate kNN instead:
 //   1. every patch gets two 64-bit keys, the Morton interleave of its table bins and of its
 //      bins offset by half the slice count; patches are radix-sorted by each key (ties by

@@ -233,7 +233,7 @@ int apply_switches(pt_params *p, const vector<string> &argv) {
         p->filter_dims_count = p->ndims;
         for (int i = 0; i < p->ndims; i++) p->filter_dims[i] = f[i];
     }
-    // Switches of this build's GPU algorithms.
+    // Switches of this library's GPU algorithms (no counterpart in the reference).
     if (has("-kcoh_window")) p->kcoh_window = to_int(sw["-kcoh_window"]);
     if (has("-jf_rounds")) p->jf_rounds = to_int(sw["-jf_rounds"]);
     return PT_OK;

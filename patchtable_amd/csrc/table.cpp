@@ -137,7 +137,7 @@ bool desc_dims_supported(int C) { return C >= 4 && C <= PT_LOOKUP_CMAX; }
 int check_build_params(const pt_params *p, int c) {
     if (p->lookup_algo != PT_LOOKUP_ALGO_TABLE) return pt_fail(PT_E_UNSUPPORTED, "only lookup_algo table is supported");
     if (p->patch_w != 8) return pt_fail(PT_E_UNSUPPORTED, "gck: patch_w unsupported (only 8)");
-    if (p->filter_dims_count) return pt_fail(PT_E_UNSUPPORTED, "filter_dims is not supported by this build yet");
+    if (p->filter_dims_count) return pt_fail(PT_E_UNSUPPORTED, "filter_dims is not supported");
     if (p->ntables < 1 || p->ntables > PT_MAX_XTABLES + 1)
         return pt_failf(PT_E_UNSUPPORTED, "ntables must be in 1..%d", PT_MAX_XTABLES + 1);
     if (p->ntables > 1 && !(p->table_ratio > 0)) return pt_fail(PT_E_INVALID, "table_ratio must be > 0");

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
 
 The oracle is test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's CPU
-baseline use it, as the checker.
+baseline use it, to check the HIP library against.
 """
 import ctypes
 import os
