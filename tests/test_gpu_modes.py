@@ -279,6 +279,21 @@ def test_quantiles_zeros_of_both_signs():
         assert np.array_equal(gcd.view(np.uint32), ocd.view(np.uint32)) and np.array_equal(gb, ob)
 
 
+def test_wide_database_build():
+    """A database wider than 16,383 patches (17,993 here): the quantile select counts in 16-bit LDS
+    halves, so its blocks take fewer rows of such a grid. Table and lookup must still match the
+    oracle bit for bit."""
+    from patchtable_amd import PatchTable
+    rng = np.random.default_rng(11)
+    b = rng.random((24, 18000, 3), dtype=np.float32)
+    a = rng.random((24, 700, 3), dtype=np.float32)
+    p = params("-speed 1 -limit 1")
+    print("wide database 24x18000", flush=True)
+    t, o = build_pair(p, b)
+    assert_tables_equal(t, o)
+    assert np.array_equal(t.lookup(a), o.lookup_parallel(a))
+
+
 def test_descriptor_distance_lane_order(lib):
     """pt_desc_dist (descriptor-mode distance) equals the oracle's restatement bit for bit for the
     SSE-chunked widths and the scalar-tail widths."""
