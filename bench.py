@@ -59,6 +59,21 @@ def measured_traffic(H, W):
     return None
 
 
+COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "lookup_counters_2048x2048_speed1.md")
+
+
+def measured_l1_rate(H, W):
+    """k_propagate's L1 tag lookups per CU per cycle (the TCP handles at most one), from the
+    committed rocprofv3 counter summary for this workload, or None."""
+    if (H, W) != (2048, 2048) or not os.path.exists(COUNTER_SUMMARY):
+        return None
+    for line in open(COUNTER_SUMMARY):
+        if line.startswith("| L1 tag lookups (TCP_TOTAL_CACHE_ACCESSES"):
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            return float(cols[2])   # columns: label, k_candidates, k_propagate, k_select
+    return None
+
+
 def copy_probe(torch, mib=1024, reps=3):
     """A plain device-to-device copy of known size, for calibrating HBM byte counters: each
     dispatch reads and writes exactly `mib` MiB."""
@@ -442,6 +457,11 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(H, W),
                 "alg_bytes_per_launch": prop_bytes, "avg_launch_ms": round(prop_launch_ms, 4),
                 "launches_per_step": launches}
+    l1 = measured_l1_rate(H, W)
+    if l1 is not None:   # the limit that binds: the L1 gathers of the exact SSDs, not HBM
+        roofline["l1_tag_lookups_per_cu_cycle"] = {"achieved": l1, "peak": 1.0, "frac": l1,
+                                                   "source": "rocprofv3 TCP_TOTAL_CACHE_ACCESSES, "
+                                                             "profiles/lookup_counters_2048x2048_speed1.md"}
     lk_bytes = lookup_alg_bytes(H, W, H, W, info["kcoherence"])
     lookup_ms = statistics.mean(total_ms)
 
