@@ -294,6 +294,35 @@ def test_wide_database_build():
     assert np.array_equal(t.lookup(a), o.lookup_parallel(a))
 
 
+@pytest.mark.parametrize("bshape,ashape,switches", [
+    ((8, 8), (8, 8), "-speed 0"),                    # one database patch, one query patch
+    ((8, 8), (40, 50), "-speed 1 -limit 1"),         # every query patch has the one candidate
+    ((9, 9), (8, 8), "-speed 1"),                    # 4 patches; dims of 61 and 45 slices
+    ((8, 300), (30, 20), "-speed 0"),                # database of one patch row
+    ((300, 8), (20, 30), "-speed 1 -limit 1"),       # database of one patch column
+    ((60, 60), (8, 300), "-speed 9 -limit 1"),       # one query row, query_step 10
+    ((12, 15), (300, 8), "-speed 5 -limit 1"),       # one query column, query_step 2
+    ((9, 9), (8, 8), "-speed 9 -limit 1"),
+])
+def test_smallest_and_thinnest_images(bshape, ashape, switches):
+    """Images at the 8x8 minimum and one patch thick: the table (cells, partitions, k-coherence
+    lists shorter than kcoherence) and the lookup equal the oracle's bit for bit, and every
+    NNF_DIST is the recomputed SSD of its in-bounds match."""
+    from patchtable_amd import synth_pair
+    print(f"database {bshape}, query {ashape}, {switches}", flush=True)
+    a, _ = synth_pair(11, *ashape)
+    _, b = synth_pair(12, *bshape)
+    p = params(switches)
+    t, o = build_pair(p, b)
+    assert_tables_equal(t, o)
+    ann = t.lookup(a)
+    assert np.array_equal(ann, o.lookup_parallel(a))
+    xs, ys = ann[..., 0].astype(int), ann[..., 1].astype(int)
+    assert (xs >= 0).all() and (ys >= 0).all() and (xs < bshape[1] - 7).all() and (ys < bshape[0] - 7).all()
+    _, d = O.recompute(a, b, ann)
+    assert np.abs(d - ann[..., 2]).max() <= 1e-4
+
+
 def test_descriptor_distance_lane_order(lib):
     """pt_desc_dist (descriptor-mode distance) equals the oracle's restatement bit for bit for the
     SSE-chunked widths and the scalar-tail widths."""
