@@ -11,7 +11,7 @@ from test_gpu_modes import assert_tables_equal, params, random_descriptors, shif
 
 pytestmark = pytest.mark.gpu
 
-NCASE = 40
+NCASE = 160
 
 
 def case(seed):
