@@ -95,6 +95,19 @@ __global__ void k_gather_desc(const float *__restrict__ dI, int nd_rt, const int
     }
 }
 
+// A (distance, index) pair as one 64-bit key that orders by distance, then by index. The
+// distances here are sums of squares (+0.0 or positive, never -0.0), whose bit patterns order as
+// the floats do; the index's sign bit is flipped so that signed indices keep their order. With a
+// single integer comparison per step the top-K insertions below are plain min/max swaps: the
+// equivalent compound test (d < d' || (d == d' && i < i')) was miscompiled for gfx950 in the
+// selection loop (on a distance tie the slot kept its old entry while the carry took it too,
+// leaving a duplicate in the list).
+__device__ __forceinline__ uint64_t dist_key(float d, int32_t i) {
+    return ((uint64_t)__float_as_uint(d) << 32) | ((uint32_t)i ^ 0x80000000u);
+}
+__device__ __forceinline__ int32_t key_index(uint64_t k) { return (int32_t)((uint32_t)k ^ 0x80000000u); }
+__device__ __forceinline__ float key_dist(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+
 // One lane per database patch p, in 64 x 4 raster tiles whose pixels are staged in LDS for the
 // exact-SSD re-rank (DESC: re-rank by descriptor distance, nothing staged). ND is the descriptor
 // width when known at compile time (0: runtime `nd`). Na: allowed patches (sorted first).
@@ -122,16 +135,11 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
     float own[ND > 0 ? ND : PT_MAX_DIMS];
     const long r0 = pos0[i];
     for (int c = 0; c < nd; c++) own[c] = ds0[r0 * nd + c];
-    float bd[KMAX];
-    int bq[KMAX];
+    uint64_t bk[KMAX];   // dist_key(descriptor distance, raster index), ascending; ~0 when empty
 #pragma unroll
-    for (int k = 0; k < KMAX; k++) {
-        bd[k] = 0.f;
-        bq[k] = 0x7fffffff;
-    }
+    for (int k = 0; k < KMAX; k++) bk[k] = ~0ull;
     int cnt = 0;
-    float wd = 0.f;   // K-th entry once the list is full
-    int wq = 0x7fffffff;
+    uint64_t wk = ~0ull;   // K-th entry once the list is full
     for (int o = 0; o < 2; o++) {
         const int32_t *perm = o == 0 ? perm0 : perm1;
         const float *ds = o == 0 ? ds0 : ds1;
@@ -191,37 +199,28 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
                 }
             }
             // reject if not better than the current K-th entry
-            if (cnt == K && !(dd < wd || (dd == wd && qi < wq))) continue;
+            const uint64_t key = dist_key(dd, qi);
+            if (cnt == K && !(key < wk)) continue;
             bool dup = false;
 #pragma unroll
-            for (int k = 0; k < KMAX; k++) dup |= (k < cnt && bq[k] == qi);
+            for (int k = 0; k < KMAX; k++) dup |= (k < cnt && key_index(bk[k]) == qi);
             if (dup) continue;
-            // insert keeping (dd, qi) ascending
+            // insert keeping the keys ascending (the empty slots hold ~0, above every key); a
+            // full list drops its last key
             const int n = cnt < K ? cnt + 1 : K;
-            float cd = dd;
-            int cq = qi;
+            uint64_t ck = key;
 #pragma unroll
             for (int k = 0; k < KMAX; k++) {
                 if (k < n) {
-                    bool empty = k >= cnt;
-                    bool less = empty || cd < bd[k] || (cd == bd[k] && cq < bq[k]);
-                    if (less) {
-                        float td = bd[k];
-                        int tq = bq[k];
-                        bd[k] = cd;
-                        bq[k] = cq;
-                        cd = td;
-                        cq = tq;
-                    }
+                    const uint64_t t = bk[k];
+                    bk[k] = ck < t ? ck : t;
+                    ck = ck < t ? t : ck;
                 }
             }
             cnt = n;
 #pragma unroll
             for (int k = 0; k < KMAX; k++)
-                if (k == K - 1) {
-                    wd = bd[k];
-                    wq = bq[k];
-                }
+                if (k == K - 1) wk = bk[k];
           }
         }
     }
@@ -229,32 +228,22 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
     const float4 *aq = tile + (DESC ? 0 : ly * AX + lx);
     const int Wpb = bw + 7;
     const float *own_i = bI ? bI + ((size_t)(py + 7) * Wpb + px + 7) * nd : nullptr;
-    float ss[KMAX];
-    int xy[KMAX];
+    uint64_t sk[KMAX];   // dist_key(lookup distance, XY_TO_INT), ascending; ~0 when empty
 #pragma unroll
-    for (int k = 0; k < KMAX; k++) {
-        ss[k] = 0.f;
-        xy[k] = 0x7fffffff;
-    }
+    for (int k = 0; k < KMAX; k++) sk[k] = ~0ull;
 #pragma unroll
     for (int a = 0; a < KMAX; a++) {
         if (a < cnt) {
-            const int qx = bq[a] % bew, qy = bq[a] / bew;
-            float cs = DESC ? pt_desc_dist(own_i, bI + ((size_t)(qy + 7) * Wpb + qx + 7) * nd, nd)
-                            : ssd_staged<false>(aq, b0 + ((size_t)qy * bw + qx) * 3, bw * 3, 0.f);
-            int cx = PT_XY_TO_INT(qx, qy);
+            const int q = key_index(bk[a]), qx = q % bew, qy = q / bew;
+            const float cs = DESC ? pt_desc_dist(own_i, bI + ((size_t)(qy + 7) * Wpb + qx + 7) * nd, nd)
+                                  : ssd_staged<false>(aq, b0 + ((size_t)qy * bw + qx) * 3, bw * 3, 0.f);
+            uint64_t ck = dist_key(cs, PT_XY_TO_INT(qx, qy));
 #pragma unroll
             for (int k = 0; k < KMAX; k++) {
                 if (k <= a) {
-                    bool less = k == a || cs < ss[k] || (cs == ss[k] && cx < xy[k]);
-                    if (less) {
-                        float ts = ss[k];
-                        int tx = xy[k];
-                        ss[k] = cs;
-                        xy[k] = cx;
-                        cs = ts;
-                        cx = tx;
-                    }
+                    const uint64_t t = sk[k];
+                    sk[k] = ck < t ? ck : t;
+                    ck = ck < t ? t : ck;
                 }
             }
         }
@@ -266,8 +255,8 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
         if (k < cnt) {
-            lastxy = xy[k];
-            lasts = ss[k];
+            lastxy = key_index(sk[k]);
+            lasts = key_dist(sk[k]);
         }
         if (k < K) {
             op[k] = lastxy;

@@ -323,6 +323,30 @@ def test_smallest_and_thinnest_images(bshape, ashape, switches):
     assert np.abs(d - ann[..., 2]).max() <= 1e-4
 
 
+@pytest.mark.parametrize("desc", [False, True])
+@pytest.mark.parametrize("switches", ["-speed 1 -limit 1", "-speed 0 -limit 1"])
+def test_kcoherence_ties_on_a_repeated_database(desc, switches):
+    """A database made of one tile repeated 3 x 3 times: every patch has exact copies, so its
+    k-coherence candidates tie in descriptor distance and in exact distance (0 for each copy, 6
+    to 8 such entries per list). The lists must equal the oracle's, which orders equal distances
+    by raster index when selecting and by XY_TO_INT when sorting."""
+    from patchtable_amd import synth_pair
+    rng = np.random.default_rng(5)
+    if desc:
+        b = np.tile(rng.random((20, 20, 6), dtype=np.float32), (3, 3, 1))
+        a = rng.random((50, 70, 6), dtype=np.float32)
+        p = params(switches, is_descriptor=1)
+    else:
+        _, tile = synth_pair(21, 30, 30)
+        b = np.tile(tile, (3, 3, 1))
+        a, _ = synth_pair(22, 60, 80)
+        p = params(switches)
+    print(f"repeated database {b.shape}, desc={desc}, {switches}", flush=True)
+    t, o = build_pair(p, b)
+    assert_tables_equal(t, o)
+    assert np.array_equal(t.lookup(a), o.lookup_parallel(a))
+
+
 def test_descriptor_distance_lane_order(lib):
     """pt_desc_dist (descriptor-mode distance) equals the oracle's restatement bit for bit for the
     SSE-chunked widths and the scalar-tail widths."""

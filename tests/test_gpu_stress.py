@@ -12,6 +12,10 @@ from test_gpu_modes import assert_tables_equal, params, random_descriptors, shif
 pytestmark = pytest.mark.gpu
 
 NCASE = 160
+# Further seeds whose k-coherence lists meet an exact distance tie in a way the first NCASE do not:
+# a later candidate with the same distance and a smaller index than a listed one. A compiled form
+# of the selection's tie test kept a duplicate there instead of the smaller index.
+TIE_SEEDS = [913, 1578, 1915, 3090, 3105, 3120, 3662]
 
 
 def case(seed):
@@ -38,7 +42,7 @@ def random_mask(rng, bh, bw):
     return m
 
 
-@pytest.mark.parametrize("seed", range(NCASE))
+@pytest.mark.parametrize("seed", list(range(NCASE)) + TIE_SEEDS)
 def test_stress_case(seed):
     from patchtable_amd import PatchTable, synth_pair
     c = case(seed)
