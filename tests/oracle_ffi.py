@@ -126,7 +126,9 @@ class OracleTable:
         self.K = K.value
 
     def __del__(self):
-        if getattr(self, "handle", None):
+        # at interpreter exit the module's globals may already be gone (lib is None); the
+        # process then frees the memory anyway
+        if getattr(self, "handle", None) and lib is not None:
             lib().orc_free(self.handle)
             self.handle = None
 
