@@ -121,7 +121,8 @@ __global__ __launch_bounds__(pt_tile::THREADS) void k_select(
     __shared__ float4 tile[DESC ? 1 : AX * AY];
     if (!DESC) stage_tile(b0, bh, bw, tile);
     const int lx = threadIdx.x % TILE_X, ly = threadIdx.x / TILE_X;
-    const int px = blockIdx.x * TILE_X + lx, py = blockIdx.y * TILE_Y + ly;
+    const int2 tc = tile_coords();
+    const int px = tc.x * TILE_X + lx, py = tc.y * TILE_Y + ly;
     if (px >= bew || py >= beh) return;
     const int nd = ND > 0 ? ND : nd_rt;
     const int i = py * bew + px;

@@ -58,8 +58,9 @@ struct QBuf {
 };
 
 __device__ __forceinline__ void lane_xy(const LookupArgs &L, int lane, int &x, int &y) {
-    x = (blockIdx.x * TILE_X + lane % TILE_X) * L.qs;
-    y = (blockIdx.y * TILE_Y + lane / TILE_X) * L.qs;
+    const int2 tc = tile_coords();
+    x = (tc.x * TILE_X + lane % TILE_X) * L.qs;
+    y = (tc.y * TILE_Y + lane / TILE_X) * L.qs;
 }
 
 // Stages the block's query patches (every lane of the block must call this).
@@ -437,7 +438,8 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
     __shared__ float4 qbuf[QBuf<M>::F4];
     __shared__ BlockQueue Q;
     if (threadIdx.x == 0) Q.n = 0;
-    const int vx = blockIdx.x * TILE_X + threadIdx.x % TILE_X, vy = blockIdx.y * TILE_Y + threadIdx.x / TILE_X;
+    const int2 tc = tile_coords();
+    const int vx = tc.x * TILE_X + threadIdx.x % TILE_X, vy = tc.y * TILE_Y + threadIdx.x / TILE_X;
     const bool valid = vx < L.vw && vy < L.vh;
     const int x = vx * L.qs, y = vy * L.qs;
     stage_query<M>(L, qbuf, valid, x, y);
@@ -517,7 +519,8 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
     __shared__ float4 qbuf[QBuf<M>::F4];
     __shared__ BlockQueue Q;
     if (threadIdx.x == 0) Q.n = 0;
-    const int vx = blockIdx.x * TILE_X + threadIdx.x % TILE_X, vy = blockIdx.y * TILE_Y + threadIdx.x / TILE_X;
+    const int2 tc = tile_coords();
+    const int vx = tc.x * TILE_X + threadIdx.x % TILE_X, vy = tc.y * TILE_Y + threadIdx.x / TILE_X;
     const bool valid = vx < L.vw && vy < L.vh;
     const int x = vx * L.qs, y = vy * L.qs;
     stage_query<M>(L, qbuf, valid, x, y);
@@ -610,7 +613,8 @@ template <int M>
 __global__ __launch_bounds__(THREADS) void k_gather(LookupArgs L, NnfField lat, int32_t *__restrict__ oxy,
                                                     float *__restrict__ od) {
     __shared__ float4 qbuf[QBuf<M>::F4];
-    const int x = blockIdx.x * TILE_X + threadIdx.x % TILE_X, y = blockIdx.y * TILE_Y + threadIdx.x / TILE_X;
+    const int2 tc = tile_coords();
+    const int x = tc.x * TILE_X + threadIdx.x % TILE_X, y = tc.y * TILE_Y + threadIdx.x / TILE_X;
     const bool valid = x < L.ann_w && y < L.ann_h;
     stage_query<M>(L, qbuf, valid, x, y);
     if (!valid) return;

@@ -13,9 +13,22 @@ namespace pt_tile {
 constexpr int TILE_X = 64, TILE_Y = 4, THREADS = TILE_X * TILE_Y;
 constexpr int AX = TILE_X + 7, AY = TILE_Y + 7;   // staged query pixels
 
+// Tile (column, row) of this workgroup. Workgroups go to the 8 XCDs round-robin in linear
+// dispatch order (x fastest), so plain blockIdx gives each XCD every 8th tile column. When the
+// columns divide evenly, XCD k instead takes the band of gridDim.x / 8 adjacent columns from
+// column k * gridDim.x / 8, row by row: neighbouring tiles, which share halo pixels and
+// candidate lines, then share one L2. The map is a bijection of the grid.
+__device__ __forceinline__ int2 tile_coords() {
+    const int gx = gridDim.x;
+    if (gx % 8 != 0) return make_int2(blockIdx.x, blockIdx.y);
+    const int lin = blockIdx.y * gx + blockIdx.x, band = gx / 8, j = lin / 8;
+    return make_int2((lin % 8) * band + j % band, j / band);
+}
+
 // Stages the pixels of this block's tile of patches into LDS, zero outside the image.
 __device__ __forceinline__ void stage_tile(const float *__restrict__ img, int ih, int iw, float4 *tile) {
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
+    const int2 tc = tile_coords();
+    const int x0 = tc.x * TILE_X, y0 = tc.y * TILE_Y;
     for (int i = threadIdx.x; i < AX * AY; i += THREADS) {
         int ly = i / AX, lx = i - ly * AX;
         int gx = x0 + lx, gy = y0 + ly;
