@@ -18,12 +18,19 @@
 //   quantiser ...................... brute force (sort, quantiles, nearest centre), limit
 //                                    invariants of scale_slices_to_limit
 //   populate ....................... invariant: each cell holds its minimum-rank patch
-//   dt prop (BFS) .................. hand-worked known answers on tiny tables; invariants
-//   dt downsample / euclidean ...... brute-force definitions on tiny tables
+//   set_default_nslices ............ closed form (patchtable.h:1806-1809) composed with the
+//                                    pinned scale_slices_to_limit
+//   dt prop (BFS) .................. hand-worked known answers on tiny tables; brute-force
+//                                    nearest source on 400 random lattices
+//   dt downsample .................. hand-worked known answers
+//   dt euclidean ................... nearest source in 1-D; a hand-worked 2-D case of the
+//                                    in-place index update (patchtable.h:4268)
 //   patch SSD ...................... float64 recompute, SSE lane order restated
 //   k-coherence .................... this library's own kNN (the reference uses FLANN, absent
-//                                    and unpinned): parity unpinned vs the reference; recall
-//                                    against exact kNN is measured instead
+//                                    and unpinned): parity unpinned vs the reference; pinned by
+//                                    recall@10 >= 0.25 against exact descriptor-space kNN
+//                                    (0.29 measured) and the list invariants (sorted, distinct,
+//                                    min_dist, float64 SSD, masked patches padded)
 //   lookup (raster) ................ invariants of stress.py:134-154; quality only
 // ============================================================================================
 #include "../include/patchtable_amd.h"

@@ -92,7 +92,12 @@ class PatchTable:
             self._allowed = np.ascontiguousarray(allowed_patches, dtype=np.int32)
             allowed = self._allowed.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         if _is_device_tensor(b):
+            import torch
+            if b.dim() != 3:
+                raise PatchTableError("b must be h x w x c")
+            _check_dev(b, "b", torch.float32)
             h, w, c = (int(s) for s in b.shape)
+            _after_torch(b.device)
             check(load().pt_table_create_dev(ctypes.byref(p.raw), ctypes.c_void_p(b.data_ptr()),
                                              h, w, c, allowed, ctypes.byref(self._t)))
         else:
@@ -123,10 +128,15 @@ class PatchTable:
         of `key` (a PatchKey) indexes the table; the GCK table of b with p is its fallback."""
         if not _is_device_tensor(b):
             raise PatchTableError("PatchTable.keyed expects a torch CUDA tensor")
+        import torch
+        if b.dim() != 3 or int(b.shape[2]) != 3:
+            raise PatchTableError("b must be h x w x 3")
+        _check_dev(b, "b", torch.float32)
         self = cls.__new__(cls)
         self.p = p
         self._t = ctypes.c_void_p()
         h, w, _ = (int(s) for s in b.shape)
+        _after_torch(b.device)
         check(load().pt_table_create_keyed(ctypes.byref(p.raw), key._k, ctypes.c_void_p(b.data_ptr()), h, w,
                                            ctypes.byref(self._t)))
         self._shape()
@@ -160,9 +170,19 @@ class PatchTable:
         secs = ctypes.c_double()
         if _is_device_tensor(a):
             import torch
+            if a.dim() != 3:
+                raise PatchTableError("a must be h x w x c")
+            _check_dev(a, "a", torch.float32)
             h, w, c = (int(s) for s in a.shape)
+            shp = self.ann_shape(h, w)
             if ann is None:
-                ann = torch.empty(self.ann_shape(h, w), dtype=torch.float64, device=a.device)
+                ann = torch.empty(shp, dtype=torch.float64, device=a.device)
+            _check_dev(ann, "ann", torch.float64, shp, a.device)
+            if ann_prev is not None:
+                _check_dev(ann_prev, "ann_prev", torch.float64, shp, a.device)
+            if coherence_temporal_sv is not None:
+                _check_dev(coherence_temporal_sv, "coherence_temporal_sv", torch.float32, shp[:2], a.device)
+            _after_torch(a.device)
             prev = ctypes.c_void_p(ann_prev.data_ptr()) if ann_prev is not None else None
             sv = ctypes.c_void_p(coherence_temporal_sv.data_ptr()) if coherence_temporal_sv is not None else None
             check(load().pt_table_lookup_dev(self._t, ctypes.c_void_p(a.data_ptr()), h, w, c,
@@ -268,9 +288,18 @@ class PatchTableBatch:
     database, built concurrently; lookups share the query descriptor and replay HIP graphs."""
 
     def __init__(self, p, databases, threads=4):
+        import torch
         self.p = p
-        self._dbs = [d.contiguous() for d in databases]
+        if not databases:
+            raise PatchTableError("PatchTableBatch needs at least one database")
+        dev = databases[0].device if _is_device_tensor(databases[0]) else None
+        for i, d in enumerate(databases):
+            if not _is_device_tensor(d) or d.dim() != 3 or int(d.shape[2]) != 3:
+                raise PatchTableError(f"database {i} must be an h x w x 3 torch CUDA tensor")
+            _check_dev(d, f"database {i}", torch.float32, device=dev)
+        self._dbs = list(databases)
         n = len(self._dbs)
+        _after_torch(dev)
         ptrs = (ctypes.c_void_p * n)(*[d.data_ptr() for d in self._dbs])
         hs = (ctypes.c_int * n)(*[int(d.shape[0]) for d in self._dbs])
         ws = (ctypes.c_int * n)(*[int(d.shape[1]) for d in self._dbs])
@@ -281,9 +310,17 @@ class PatchTableBatch:
     def lookup(self, a, anns=None):
         """Returns (list of n NNFs as torch CUDA float64 tensors, per-pair device ms, wall s)."""
         import torch
+        if not _is_device_tensor(a) or a.dim() != 3 or int(a.shape[2]) != 3:
+            raise PatchTableError("a must be an h x w x 3 torch CUDA tensor")
+        _check_dev(a, "a", torch.float32)
         h, w = int(a.shape[0]), int(a.shape[1])
         if anns is None:
             anns = [torch.empty((h - 7, w - 7, 3), dtype=torch.float64, device=a.device) for _ in range(self.n)]
+        if len(anns) != self.n:
+            raise PatchTableError(f"expected {self.n} output fields, got {len(anns)}")
+        for i, x in enumerate(anns):
+            _check_dev(x, f"anns[{i}]", torch.float64, (h - 7, w - 7, 3), a.device)
+        _after_torch(a.device)
         ptrs = (ctypes.c_void_p * self.n)(*[x.data_ptr() for x in anns])
         ms = (ctypes.c_double * self.n)()
         secs = ctypes.c_double()
@@ -350,8 +387,12 @@ class PatchKey:
         """Key planes of a device image: torch tensor 8 x (h+7) x (w+7), patch (x, y) at
         (y+7, x+7), zeros elsewhere."""
         import torch
+        if not _is_device_tensor(img) or img.dim() != 3 or int(img.shape[2]) != 3:
+            raise PatchTableError("img must be an h x w x 3 torch CUDA tensor")
+        _check_dev(img, "img", torch.float32)
         h, w, _ = (int(s) for s in img.shape)
         out = torch.zeros((self.KO, h + 7, w + 7), dtype=torch.float32, device=img.device)
+        _after_torch(img.device)
         check(load().pt_key_forward_dev(self._k, ctypes.c_void_p(img.data_ptr()), h, w,
                                         ctypes.c_void_p(out.data_ptr())))
         return out
@@ -363,8 +404,15 @@ class PatchKey:
         g = [np.empty((self.KH, self.KI), np.float32), np.empty(self.KH, np.float32),
              np.empty((self.KO, self.KH), np.float32), np.empty(self.KO, np.float32)]
         loss = np.zeros(1, np.float32)
+        import torch
         if tuple(a.shape) != tuple(b.shape):
             raise PatchTableError("anchor and positive/negative images must have equal sizes")
+        if a.dim() != 3 or int(a.shape[2]) != 3:
+            raise PatchTableError("images must be h x w x 3")
+        _check_dev(a, "a", torch.float32)
+        _check_dev(b, "b", torch.float32, device=a.device)
+        _check_dev(pos, "pos", torch.int32, (nt, 3), a.device)
+        _after_torch(a.device)
         check(load().pt_key_gradients(self._k, ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
                                       int(a.shape[0]), int(a.shape[1]), ctypes.c_void_p(pos.data_ptr()), nt,
                                       float(margin),
@@ -379,21 +427,33 @@ class PatchKey:
         B = (ctypes.c_void_p * n)(*[b.data_ptr() for _, b in pairs])
         H = (ctypes.c_int * n)(*[int(a.shape[0]) for a, _ in pairs])
         W = (ctypes.c_int * n)(*[int(a.shape[1]) for a, _ in pairs])
-        for a, b in pairs:
+        import torch
+        for i, (a, b) in enumerate(pairs):
             if tuple(a.shape) != tuple(b.shape):
                 raise PatchTableError("training pairs must have equal image sizes")
+            if a.dim() != 3 or int(a.shape[2]) != 3:
+                raise PatchTableError("training images must be h x w x 3")
+            _check_dev(a, f"pairs[{i}][0]", torch.float32)
+            _check_dev(b, f"pairs[{i}][1]", torch.float32, device=a.device)
+        if pairs:
+            _after_torch(pairs[0][0].device)
         losses = np.zeros(max(steps, 1), np.float32)
         check(load().pt_key_train(self._k, ctypes.byref(p.raw), n, A, B, H, W, int(tpp), int(steps), int(batch),
                                   float(lr), float(margin), int(seed), _ffi_f32(losses)))
         return losses[:steps]
 
 
-def _queries(a, qpos):
+def _queries(a, b, qpos):
     import torch
+    for name, x in (("a", a), ("b", b)):
+        if not _is_device_tensor(x) or x.dim() != 3 or int(x.shape[2]) != 3:
+            raise PatchTableError(f"{name} must be an h x w x 3 torch CUDA tensor")
+        _check_dev(x, name, torch.float32, device=a.device)
     if qpos is None:
         return None, (int(a.shape[0]) - 7) * (int(a.shape[1]) - 7)
     if not (_is_device_tensor(qpos) and qpos.dtype == torch.int32):
         raise PatchTableError("query positions must be a CUDA int32 tensor of XY_TO_INT values")
+    _check_dev(qpos, "qpos", torch.int32, device=a.device)
     return ctypes.c_void_p(qpos.data_ptr()), int(qpos.numel())
 
 
@@ -404,7 +464,7 @@ def brute_lookup(a, b, qpos=None, precision="bf16"):
     Returns dict(pos, dist, cand, score, stage_ms) with pos/dist/cand/score CUDA tensors."""
     import torch
     prec = {"bf16": _ffi.PT_BRUTE_BF16, "fp8": _ffi.PT_BRUTE_FP8}[precision]
-    qp, nq = _queries(a, qpos)
+    qp, nq = _queries(a, b, qpos)
     dev = a.device
     pos = torch.empty(nq, dtype=torch.int32, device=dev)
     dist = torch.empty(nq, dtype=torch.float32, device=dev)
@@ -423,7 +483,7 @@ def brute_lookup(a, b, qpos=None, precision="bf16"):
 def brute_exact_f64(a, b, qpos=None):
     """Validation: exact FP64 nearest neighbours (pos XY_TO_INT, dist float64), CUDA tensors."""
     import torch
-    qp, nq = _queries(a, qpos)
+    qp, nq = _queries(a, b, qpos)
     pos = torch.empty(nq, dtype=torch.int32, device=a.device)
     dist = torch.empty(nq, dtype=torch.float64, device=a.device)
     torch.cuda.synchronize()
@@ -439,6 +499,32 @@ def _ffi_f32(a):
 
 def _is_device_tensor(x):
     return hasattr(x, "is_cuda") and bool(getattr(x, "is_cuda"))
+
+
+def _check_dev(x, name, dtype, shape=None, device=None):
+    """A torch CUDA tensor argument handed to the library as a raw pointer: it must have the
+    dtype and shape the C entry point assumes, be contiguous (the library reads dense HWC
+    rows) and live on the device of the call. Anything else raises instead of reading or
+    writing out of bounds."""
+    if not _is_device_tensor(x):
+        raise PatchTableError(f"{name} must be a torch CUDA tensor")
+    if x.dtype != dtype:
+        raise PatchTableError(f"{name} must be {dtype}, got {x.dtype}")
+    if not x.is_contiguous():
+        raise PatchTableError(f"{name} must be contiguous")
+    if shape is not None and tuple(x.shape) != tuple(shape):
+        raise PatchTableError(f"{name} must have shape {tuple(shape)}, got {tuple(x.shape)}")
+    if device is not None and x.device != device:
+        raise PatchTableError(f"{name} is on {x.device}, expected {device}")
+    return x
+
+
+def _after_torch(device):
+    """The library works on its own non-blocking HIP streams and returns only once its work is
+    done. Torch's current stream may still be writing an input (or reading a buffer the call
+    will overwrite), so wait for it before handing the pointers over."""
+    import torch
+    torch.cuda.current_stream(device).synchronize()
 
 
 def load_color_image(filename):

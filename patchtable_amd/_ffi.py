@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libpatchtable_hip.so")
 
 PT_MAX_DIMS = 64
 PT_OK = 0
+PT_E_INVALID = -1
 PT_BRUTE_BF16 = 0
 PT_BRUTE_FP8 = 1
 

@@ -48,7 +48,7 @@ extern "C" int pt_brute_lookup_dev(const float *a_dev, int ah, int aw, const flo
     if (precision != PT_BRUTE_BF16 && precision != PT_BRUTE_FP8)
         return pt_fail(PT_E_INVALID, "precision must be PT_BRUTE_BF16 or PT_BRUTE_FP8");
     if (bh < 8 || bw < 8) return pt_fail(PT_E_INVALID, "database image must be at least 8x8");
-    if (bh > 65536 - 8 || bw > 65536 - 8 || ah > 65536 - 8 || aw > 65536 - 8)
+    if (!PT_SIZE_OK(bh, bw) || !PT_SIZE_OK(ah, aw))
         return pt_fail(PT_E_INVALID, "image too large for XY_TO_INT packing");
     TRY(check_queries(qpos_dev, nq, ah, aw));
     const long nd = (long)(bh - 7) * (bw - 7);
@@ -107,7 +107,7 @@ extern "C" int pt_brute_exact_f64_dev(const float *a_dev, int ah, int aw, const 
     if (!a_dev || !b_dev || !pos_out_dev || !dist_out_dev)
         return pt_fail(PT_E_INVALID, "pt_brute_exact_f64_dev: null argument");
     if (bh < 8 || bw < 8) return pt_fail(PT_E_INVALID, "database image must be at least 8x8");
-    if (bh > 65536 - 8 || bw > 65536 - 8 || ah > 65536 - 8 || aw > 65536 - 8)
+    if (!PT_SIZE_OK(bh, bw) || !PT_SIZE_OK(ah, aw))
         return pt_fail(PT_E_INVALID, "image too large for XY_TO_INT packing");
     TRY(check_queries(qpos_dev, nq, ah, aw));
     const long nd = (long)(bh - 7) * (bw - 7);

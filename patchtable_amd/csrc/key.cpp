@@ -187,7 +187,7 @@ int pt_key_forward(const pt_key *k, const float *img_dev, int h, int w, float *p
 }
 
 extern "C" int pt_key_forward_dev(const pt_key *k, const float *img_dev, int h, int w, float *planes_dev) {
-    if (!k || !img_dev || !planes_dev || h < 8 || w < 8) return pt_fail(PT_E_INVALID, "pt_key_forward_dev: bad arguments");
+    if (!k || !img_dev || !planes_dev || h < 8 || w < 8 || !PT_SIZE_OK(h, w)) return pt_fail(PT_E_INVALID, "pt_key_forward_dev: bad arguments");
     TRY(pt_key_forward(k, img_dev, h, w, planes_dev, k->s));
     HIPTRY(hipStreamSynchronize(k->s));
     return PT_OK;
@@ -198,7 +198,7 @@ extern "C" int pt_key_forward_dev(const pt_key *k, const float *img_dev, int h, 
 // positions are checked on the host before the gather.
 extern "C" int pt_key_gradients(pt_key *k, const float *a_dev, const float *b_dev, int h, int w, const int32_t *pos_dev,
                                 int nt, float margin, float *gW1, float *gb1, float *gW2, float *gb2, float *loss) {
-    if (!k || !a_dev || !b_dev || !pos_dev || nt < 1 || h < 8 || w < 8)
+    if (!k || !a_dev || !b_dev || !pos_dev || nt < 1 || h < 8 || w < 8 || !PT_SIZE_OK(h, w))
         return pt_fail(PT_E_INVALID, "pt_key_gradients: bad arguments");
     std::vector<int32_t> hp((size_t)nt * 3);
     HIPTRY(hipMemcpy(hp.data(), pos_dev, hp.size() * 4, hipMemcpyDeviceToHost));

@@ -23,6 +23,13 @@ int pt_failf(int code, const char *fmt, ...);
 #define PT_INT_TO_X(v) ((v) & 65535)
 #define PT_INT_TO_Y(v) ((v) >> 16)
 
+// Largest image sides the packing holds: x takes the low 16 bits (padded x <= 65535) and y the
+// bits above them in an int32, so a padded y must stay below 2^15 or INT_TO_Y decodes it as
+// negative. Heights (descriptor mode: descriptor height + 7) are therefore at most 32768.
+#define PT_MAX_H 32768
+#define PT_MAX_W (65536 - 8)
+#define PT_SIZE_OK(h, w) ((h) <= PT_MAX_H && (w) <= PT_MAX_W)
+
 // Empty cell marker (TABLE_UNUSED = 2^30 - 1, patchtable.h:109-112).
 #define PT_TABLE_UNUSED ((1 << 30) - 1)
 

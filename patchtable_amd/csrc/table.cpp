@@ -635,7 +635,8 @@ int create_common(pt_params *p, const float *b, bool b_on_device, int h, int w, 
         vw = p->descriptor_padded ? w - 7 : w + 7;
     }
     if (vh < 8 || vw < 8) return pt_fail(PT_E_INVALID, "database image must be at least 8x8 (descriptor: 1x1)");
-    if (vh > 65536 - 8 || vw > 65536 - 8) return pt_fail(PT_E_INVALID, "image too large for XY_TO_INT packing");
+    if (!PT_SIZE_OK(vh, vw))
+        return pt_failf(PT_E_INVALID, "image too large for XY_TO_INT packing (height <= %d, width <= %d)", PT_MAX_H, PT_MAX_W);
     if (p->is_descriptor) {   // patchtable.h:4398-4401, written back as the reference does
         p->ndims = c;
         p->nchroma = 0;
@@ -689,7 +690,8 @@ int query_shape(const pt_table *t, int h, int w, int c, int &ah, int &aw) {
         aw = w;
     }
     if (ah < 8 || aw < 8) return pt_fail(PT_E_INVALID, "query image must be at least 8x8 (descriptor: 1x1)");
-    if (ah > 65536 - 8 || aw > 65536 - 8) return pt_fail(PT_E_INVALID, "query image too large");
+    if (!PT_SIZE_OK(ah, aw))
+        return pt_failf(PT_E_INVALID, "query image too large (height <= %d, width <= %d)", PT_MAX_H, PT_MAX_W);
     return PT_OK;
 }
 
@@ -972,7 +974,8 @@ extern "C" int pt_table_create_keyed(pt_params *p, const pt_key *key, const floa
         return pt_fail(PT_E_UNSUPPORTED, "a learned-key table has no file format (load/save)");
     TRY(check_build_params(p, 3));
     if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "database image must be at least 8x8");
-    if (h > 65536 - 8 || w > 65536 - 8) return pt_fail(PT_E_INVALID, "image too large for XY_TO_INT packing");
+    if (!PT_SIZE_OK(h, w))
+        return pt_failf(PT_E_INVALID, "image too large for XY_TO_INT packing (height <= %d, width <= %d)", PT_MAX_H, PT_MAX_W);
     if (p->grid_ndims < 0) p->grid_ndims = p->ndims;   // patchtable.h:2603
     pt_table *t = new pt_table();
     t->p = p;
@@ -1196,7 +1199,7 @@ int load_file(pt_table *t, const char *filename) {
     t->w = bsz[1];
     t->ndims = wsz[2];
     t->G = (int)ns.size();
-    if (t->h < 8 || t->w < 8 || t->ndims < 1 || t->ndims > PT_MAX_DIMS || t->G > t->ndims)
+    if (t->h < 8 || t->w < 8 || !PT_SIZE_OK(t->h, t->w) || t->ndims < 1 || t->ndims > PT_MAX_DIMS || t->G > t->ndims)
         return pt_failf(PT_E_IO, "%s: bad table dimensions", filename);
     t->nslices = ns;
     t->stride.assign(t->G, 0);
@@ -1430,6 +1433,8 @@ struct pt_batch {
     std::vector<hipEvent_t> e0, e1;
     std::vector<hipGraphExec_t> gexec;
     std::vector<std::vector<const void *>> gkey;   // buffers a graph was captured with
+    std::vector<pt_params> gpar;                   // ... and the parameters (byte copy): the
+                                                   // caller may change p between lookups
     DevBuf<float> adesc, yuv, temp;
     ~pt_batch() {
         for (auto g : gexec)
@@ -1478,6 +1483,7 @@ extern "C" int pt_batch_create(pt_params *p, int n, const float *const *b_dev, c
     bt->e1.assign(n, nullptr);
     bt->gexec.assign(n, nullptr);
     bt->gkey.assign(n, {});
+    bt->gpar.assign(n, pt_params());
     for (int i = 0; r == PT_OK && i < n; i++)
         if (hipEventCreate(&bt->e0[i]) != hipSuccess || hipEventCreate(&bt->e1[i]) != hipSuccess)
             r = pt_fail(PT_E_HIP, "pt_batch_create: event creation failed");
@@ -1501,6 +1507,7 @@ extern "C" int pt_batch_lookup(pt_batch *bt, const float *a_dev, int h, int w, d
     const int n = (int)bt->tabs.size();
     pt_table *t0 = bt->tabs[0];
     if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "query image must be at least 8x8");
+    if (!PT_SIZE_OK(h, w)) return pt_fail(PT_E_INVALID, "query image too large for XY_TO_INT packing");
     const double T0 = now_s();
     // the query descriptor once (all tables share the parameters)
     const size_t plane = (size_t)(h + 7) * (w + 7);
@@ -1517,7 +1524,7 @@ extern "C" int pt_batch_lookup(pt_batch *bt, const float *a_dev, int h, int w, d
         HIPTRY(hipEventRecord(bt->e0[i], s));
         const std::vector<const void *> key = {a_dev, (const void *)(intptr_t)h, (const void *)(intptr_t)w, ann_dev[i],
                                                bt->adesc.p};
-        if (bt->gexec[i] && bt->gkey[i] == key) {
+        if (bt->gexec[i] && bt->gkey[i] == key && std::memcmp(&bt->gpar[i], bt->p, sizeof(pt_params)) == 0) {
             HIPTRY(hipGraphLaunch(bt->gexec[i], s));
         } else {
             // eager run (sizes the buffers), then capture the same launch sequence for replays
@@ -1536,6 +1543,7 @@ extern "C" int pt_batch_lookup(pt_batch *bt, const float *a_dev, int h, int w, d
             (void)hipGraphDestroy(g);
             if (ei != hipSuccess) return pt_failf(PT_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
             bt->gkey[i] = key;
+            std::memcpy(&bt->gpar[i], bt->p, sizeof(pt_params));
         }
         HIPTRY(hipEventRecord(bt->e1[i], s));
     }

@@ -33,3 +33,57 @@ def test_batch_matches_single_tables():
     o = O.OracleTable(p.raw, dbs[1])
     assert np.array_equal(singles[1], o.lookup_parallel(a_np))
     batch.close()
+
+
+def test_batch_replay_follows_parameter_changes():
+    """The tables read the caller's parameters at every lookup (patchtable.h:2602). Changing
+    lookup-time fields between calls with the same buffers must not replay a graph captured with
+    the old values: each call equals the single-table lookup under the current parameters."""
+    import torch
+    from patchtable_amd import PatchTable, PatchTableBatch, PatchTableParams, synth_pair
+    sizes = [(110, 140), (130, 120)]
+    dbs = [synth_pair(220 + i, h, w)[1] for i, (h, w) in enumerate(sizes)]
+    a_np = synth_pair(219, 120, 150)[0]
+    a = torch.from_numpy(a_np).cuda()
+    p = PatchTableParams()
+    p.set_speed(1)
+    p.set_from_switches("-x 0 -limit 1")
+    batch = PatchTableBatch(p, [torch.from_numpy(d).cuda() for d in dbs], threads=2)
+    singles = [PatchTable(p, d) for d in dbs]
+    anns = [torch.empty((113, 143, 3), dtype=torch.float64, device="cuda") for _ in dbs]
+    settings = [(6, 1, 10), (6, 1, 10), (2, 1, 10), (2, 0, 4), (6, 1, 10)]   # jf_rounds, spatial, kcoherence
+    for call, (jr, sp, kc) in enumerate(settings):
+        p.jf_rounds, p.spatial, p.kcoherence = jr, sp, kc
+        out, _, _ = batch.lookup(a, anns)
+        for i in range(len(dbs)):
+            ref = singles[i].lookup(a_np)
+            assert np.array_equal(out[i].cpu().numpy(), ref), (call, i)
+    batch.close()
+
+
+def test_device_arguments_are_checked():
+    """Device tensors handed to the library must have the dtype, shape and layout the C entry
+    points assume; anything else raises before a pointer is passed."""
+    import torch
+    from patchtable_amd import PatchTable, PatchTableError, PatchTableParams, synth_pair
+    a_np, b_np = synth_pair(230, 64, 72)
+    p = PatchTableParams()
+    p.set_speed(1)
+    p.set_from_switches("-x 0 -limit 0.1")
+    b = torch.from_numpy(b_np).cuda()
+    with pytest.raises(PatchTableError, match="float32"):
+        PatchTable(p, b.double())
+    with pytest.raises(PatchTableError, match="contiguous"):
+        PatchTable(p, b.transpose(0, 1))
+    t = PatchTable(p, b)
+    a = torch.from_numpy(a_np).cuda()
+    with pytest.raises(PatchTableError, match="float32"):
+        t.lookup(a.half())
+    with pytest.raises(PatchTableError, match="shape"):
+        t.lookup(a, ann=torch.empty((10, 10, 3), dtype=torch.float64, device="cuda"))
+    with pytest.raises(PatchTableError, match="float64"):
+        t.lookup(a, ann=torch.empty((57, 65, 3), dtype=torch.float32, device="cuda"))
+    # a field written by a torch kernel just before the call is read complete
+    a2 = torch.zeros_like(a)
+    a2.copy_(a)
+    assert np.array_equal(t.lookup(a2).cpu().numpy(), t.lookup(a_np))

@@ -133,7 +133,9 @@ int main(int argc, char **argv) {
     Array<float> img(10, 12, 3);
     img.clear(0.5f);
     printf("%d %d %d %d\n", img.height(), img.width(), img.channels(), img.nelems);
-    if (0) { PatchTable<> t(&p, img); Array<double> ann; t.lookup(img, ann); }
+    /* construct and lookup run on the GPU in tests/test_gpu_shim.py; here the code only has to
+       compile and link against the library */
+    if (argc < 0) { PatchTable<> t(&p, img); Array<double> ann; t.lookup(img, ann); }
     return 0;
 }
 '''
@@ -159,3 +161,22 @@ def test_synth_pair_deterministic():
     a3, _ = synth_pair(2, 30, 40)
     assert np.array_equal(a1, a2) and np.array_equal(b1, b2) and not np.array_equal(a1, a3)
     assert a1.min() >= 0 and a1.max() <= 1 and b1.min() >= 0 and b1.max() <= 1
+
+
+def test_packing_size_limits_rejected_before_launch():
+    """XY_TO_INT keeps y in the bits above 16 of an int32, so a padded y of 32768 or more would
+    decode as negative: heights above 32768 (descriptor mode: descriptor height + 7) are
+    rejected on the host, before any device work (this runs without a GPU)."""
+    from patchtable_amd import PatchTable, PatchTableError, _ffi
+    with pytest.raises(PatchTableError, match="too large"):
+        PatchTable(P(), np.zeros((32769, 8, 3), np.float32))
+    p = P()
+    p.is_descriptor = 1
+    with pytest.raises(PatchTableError, match="too large"):
+        PatchTable(p, np.zeros((32762, 1, 4), np.float32))
+    lib = _ffi.load()
+    fake = ctypes.c_void_p(256)   # never dereferenced: the size check comes first
+    pos, dist = ctypes.c_void_p(512), ctypes.c_void_p(768)
+    assert lib.pt_brute_exact_f64_dev(fake, 16, 16, fake, 32769, 8, None, 0, pos, dist) == _ffi.PT_E_INVALID
+    assert "too large" in lib.pt_last_error().decode()
+    assert lib.pt_brute_exact_f64_dev(fake, 32769, 16, fake, 16, 16, None, 0, pos, dist) == _ffi.PT_E_INVALID
