@@ -204,6 +204,18 @@ int pt_table_lookup(pt_table *t, const float *a, int h, int w, int c, double *an
 /* Same with a, ann_out (and ann_prev, sv if given) in device memory. */
 int pt_table_lookup_dev(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_out_dev,
                         const double *ann_prev_dev, const float *sv_dev, double *seconds);
+/* No counterpart in the reference: pt_table_lookup_dev without any device allocation, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, results and errors as pt_table_lookup_dev; the lookup works
+ * in a library-owned device array of PT_NOALLOC_MAX_BYTES per GPU (held until the process
+ * ends) instead of the table's buffers. A query whose working set exceeds it fails with
+ * PT_E_INVALID before any launch: about 131 bytes per query pixel at query_step 1 (a 1024 x 768
+ * query needs 102 MB), plus 16 per pixel with ann_prev. Learned-key tables are not supported
+ * (PT_E_UNSUPPORTED). The current device must be the table's. Calls on one GPU share the array
+ * and are serialised by a lock; the call returns when the lookup is done. */
+#define PT_NOALLOC_MAX_BYTES (128u << 20)
+int pt_table_lookup_dev_noalloc(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_out_dev,
+                                const double *ann_prev_dev, const float *sv_dev, double *seconds);
 
 /* ---- Persistence (patchtable.h:1574-1622; patchtable.cpp:499-573) ----------------------
  * The file is the reference's: "PatchTable", table, b0, wh0, nslices, PatchPartition of one

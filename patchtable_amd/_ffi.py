@@ -128,6 +128,7 @@ SIGNATURES = {
     "pt_table_create_dev": (_int, [_P(PtParams), _vp, _int, _int, _int, _i32p, _P(_vp)]),
     "pt_table_lookup": (_int, [_vp, _f32p, _int, _int, _int, _f64p, _f64p, _f32p, _f64p]),
     "pt_table_lookup_dev": (_int, [_vp, _vp, _int, _int, _int, _vp, _vp, _vp, _f64p]),
+    "pt_table_lookup_dev_noalloc": (_int, [_vp, _vp, _int, _int, _int, _vp, _vp, _vp, _f64p]),
     "pt_table_save": (_int, [_vp, ctypes.c_char_p]),
     "pt_table_load": (_int, [_P(PtParams), ctypes.c_char_p, _P(_vp)]),
     "pt_table_destroy": (_int, [_vp]),

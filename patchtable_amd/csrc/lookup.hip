@@ -740,28 +740,20 @@ int launch_lookup(const LookupArgs &L, NnfField f0, NnfField f1, hipStream_t s, 
     dim3 grid((L.vw + TILE_X - 1) / TILE_X, (L.vh + TILE_Y - 1) / TILE_Y);
     hipLaunchKernelGGL(k_candidates<M>, grid, dim3(THREADS), 0, s, L, f0);
     if (ev_after_candidates) PT_HIP_TRY(hipEventRecord(ev_after_candidates, s));
-    std::vector<int> J;
-    pt_jump_schedule(L.rounds, J);
-    for (size_t r = 0; r < J.size(); r++) {
-        int same = r > 0 && J[r - 1] == J[r];
-        const int keep = (int)r + 1 < (int)J.size();   // the last round's history is not read again
-        hipLaunchKernelGGL(k_propagate<M>, grid, dim3(THREADS), 0, s, L, J[r], same, (int)r + 1, f0, f1, keep);
+    const int R = L.rounds;
+    for (int r = 0; r < R; r++) {
+        const int J = pt_jump_at(R, r);
+        const int same = r > 0 && pt_jump_at(R, r - 1) == J;
+        const int keep = r + 1 < R;   // the last round's history is not read again
+        hipLaunchKernelGGL(k_propagate<M>, grid, dim3(THREADS), 0, s, L, J, same, r + 1, f0, f1, keep);
         std::swap(f0, f1);
     }
-    launches = (int)J.size();
+    launches = R;
     PT_LAUNCH_CHECK();
-    return J.size() % 2 ? 1 : 0;   // which field holds the result
+    return R % 2 ? 1 : 0;   // which field holds the result
 }
 
 }  // namespace
-
-// Jump schedule of the propagation rounds: 8, 4, 2, 1 then unit jumps (fewer rounds start lower).
-void pt_jump_schedule(int rounds, std::vector<int> &J) {
-    J.clear();
-    int big = rounds >= 4 ? 8 : (rounds == 3 ? 4 : (rounds == 2 ? 2 : 1));
-    for (int j = big; j >= 1 && (int)J.size() < rounds; j >>= 1) J.push_back(j);
-    while ((int)J.size() < rounds) J.push_back(1);
-}
 
 int pt_lookup_init_prev(const LookupArgs &L, const double *prev, const float *sv, int recalc, double temporal_scale,
                         int sanitize, int32_t *ixy, float *id, int *bad_dev, hipStream_t s) {

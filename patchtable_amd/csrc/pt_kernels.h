@@ -214,7 +214,40 @@ struct NnfField {
     float *d = nullptr;
     uint8_t *fl = nullptr;
 };
-void pt_jump_schedule(int rounds, std::vector<int> &J);
+// Jump of propagation round r (0-based) of `rounds`: 8, 4, 2, 1, then unit jumps (fewer rounds
+// start lower: 3 rounds 4, 2, 1; 2 rounds 2, 1; 1 round 1).
+static inline int pt_jump_at(int rounds, int r) {
+    const int big = rounds >= 4 ? 8 : (rounds == 3 ? 4 : (rounds == 2 ? 2 : 1));
+    const int j = big >> (r < 30 ? r : 30);
+    return j > 1 ? j : 1;
+}
+// Device buffers one lookup works in: the table's own (grown on demand), or carved from a
+// caller's workspace (pt_lookup_workspace_layout).
+struct LookupBufs {
+    float *adesc = nullptr, *adescI = nullptr, *yuv = nullptr, *temp = nullptr;
+    int32_t *nxy = nullptr, *nxy2 = nullptr, *gxy = nullptr, *ixy = nullptr, *hist = nullptr;
+    float *nd = nullptr, *nd2 = nullptr, *gd = nullptr, *id = nullptr;
+    uint8_t *nfl = nullptr, *nfl2 = nullptr, *hn = nullptr;
+    int *bad = nullptr;
+};
+struct pt_table;
+// Enqueues one lookup on the table's stream (table.cpp). ws: the buffers to work in, or NULL for
+// the table's own.
+int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, const double *prev_dev,
+                      const float *sv_dev, const float *adesc_pre, bool timing, int *launches_out,
+                      const LookupBufs *ws = nullptr);
+// Device the table was built on (its stream's device).
+int pt_table_device(const pt_table *t);
+// Bytes of the file-scope device array behind pt_table_lookup_dev_noalloc (noalloc.hip).
+#define PT_NOALLOC_SCRATCH_BYTES PT_NOALLOC_MAX_BYTES
+// Bytes of workspace a lookup of an h x w x c query needs (with a previous field or not), and,
+// when base is not NULL, the buffers carved from base (256-byte aligned pieces).
+int pt_lookup_workspace_layout(const pt_table *t, int h, int w, int c, bool prev, unsigned char *base,
+                               size_t *bytes, LookupBufs *bufs);
+// One lookup in a caller's workspace (no device allocation), synchronous like pt_table_lookup_dev.
+int pt_lookup_in_workspace(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev,
+                           const double *prev_dev, const float *sv_dev, unsigned char *ws, size_t ws_bytes,
+                           double *seconds);
 int pt_lookup_init_prev(const LookupArgs &L, const double *prev, const float *sv, int recalc, double temporal_scale,
                         int sanitize, int32_t *ixy, float *id, int *bad_dev, hipStream_t s);
 // Candidates, propagation rounds, gather (query_step > 1, into gxy/gd) and the double output.

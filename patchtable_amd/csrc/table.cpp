@@ -48,6 +48,7 @@ struct pt_table {
     pt_params *p = nullptr;   // the caller's parameters, read at lookup time (patchtable.h:2602)
     pt_params build;          // parameters the table was built with
     hipStream_t stream = nullptr;
+    int device = -1;          // the device current when the table was made (its stream's)
     hipEvent_t ev[12] = {};
     int h = 0, w = 0;         // "image" size: b, or the descriptor size + 7 in descriptor mode
     int ndims = 0, nchroma = 0, G = 0;
@@ -123,6 +124,7 @@ struct pt_table {
 namespace {
 
 int init_stream(pt_table *t) {
+    if (hipGetDevice(&t->device) != hipSuccess) return pt_fail(PT_E_HIP, "hipGetDevice failed");
     if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
         return pt_fail(PT_E_HIP, "hipStreamCreate failed");
     for (int i = 0; i < 12; i++)
@@ -699,9 +701,11 @@ int query_shape(const pt_table *t, int h, int w, int c, int &ah, int &aw) {
 
 // Enqueues one lookup on the table's stream. adesc_pre: the query descriptor planes when the
 // caller already computed them (same parameters), else NULL. timing: record the stage events
-// (off while a graph is being captured).
+// (off while a graph is being captured). ws: buffers to work in (no allocation), else the
+// table's own, grown as needed.
 int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, const double *prev_dev,
-                      const float *sv_dev, const float *adesc_pre, bool timing, int *launches_out) {
+                      const float *sv_dev, const float *adesc_pre, bool timing, int *launches_out,
+                      const LookupBufs *ws) {
     const pt_params &p = *t->p;
     int ah = 0, aw = 0;
     TRY(query_shape(t, h, w, c, ah, aw));
@@ -714,6 +718,7 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     if (p.coherence_spatial != 0.0) return pt_fail(PT_E_UNSUPPORTED, "coherence_spatial != 0 is not supported");
     if (p.kcoherence > t->K) return pt_fail(PT_E_INVALID, "kcoherence exceeds the table's k-coherence lists");
     if (p.spatial < 0) return pt_fail(PT_E_INVALID, "spatial must be >= 0");
+    if (ws && t->keyed) return pt_fail(PT_E_UNSUPPORTED, "learned-key tables look up in their own buffers");
     hipStream_t s = t->stream;
     const int Hp = ah + 7, Wp = aw + 7;
     const size_t plane = (size_t)Hp * Wp;
@@ -722,49 +727,78 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     const int qs = p.query_step;
     const int vw = (ann_w + qs - 1) / qs, vh = (ann_h + qs - 1) / qs;
     const long nv = (long)vw * vh;
-    TRY(t->adesc.ensure(plane * t->ndims));
-    TRY(t->nxy.ensure(nv));
-    TRY(t->nd.ensure(nv));
-    TRY(t->nfl.ensure(nv));
-    TRY(t->nxy2.ensure(nv));
-    TRY(t->nd2.ensure(nv));
-    TRY(t->nfl2.ensure(nv));
-    TRY(t->hist.ensure((size_t)nv * PT_LOOKUP_HIST));
-    TRY(t->hn.ensure(nv));
-    if (qs > 1) {
-        TRY(t->gxy.ensure(n));
-        TRY(t->gd.ensure(n));
+    LookupBufs B;
+    if (ws) {
+        B = *ws;
+    } else {
+        TRY(t->adesc.ensure(plane * t->ndims));
+        TRY(t->nxy.ensure(nv));
+        TRY(t->nd.ensure(nv));
+        TRY(t->nfl.ensure(nv));
+        TRY(t->nxy2.ensure(nv));
+        TRY(t->nd2.ensure(nv));
+        TRY(t->nfl2.ensure(nv));
+        TRY(t->hist.ensure((size_t)nv * PT_LOOKUP_HIST));
+        TRY(t->hn.ensure(nv));
+        if (qs > 1) {
+            TRY(t->gxy.ensure(n));
+            TRY(t->gd.ensure(n));
+        }
+        if (t->desc_mode) TRY(t->adescI.ensure(plane * t->ndims));
+        if (!t->desc_mode) {
+            TRY(t->yuv.ensure((size_t)ah * aw * 3));
+            TRY(t->temp.ensure(plane * 3));
+        }
+        if (prev_dev) {
+            TRY(t->ixy.ensure(n));
+            TRY(t->id.ensure(n));
+            TRY(t->bad.ensure(1));
+        }
+        B.adesc = t->adesc.p;
+        B.adescI = t->adescI.p;
+        B.yuv = t->yuv.p;
+        B.temp = t->temp.p;
+        B.nxy = t->nxy.p;
+        B.nxy2 = t->nxy2.p;
+        B.gxy = t->gxy.p;
+        B.ixy = t->ixy.p;
+        B.hist = t->hist.p;
+        B.nd = t->nd.p;
+        B.nd2 = t->nd2.p;
+        B.gd = t->gd.p;
+        B.id = t->id.p;
+        B.nfl = t->nfl.p;
+        B.nfl2 = t->nfl2.p;
+        B.hn = t->hn.p;
+        B.bad = t->bad.p;
     }
-    t->qh = ah;
-    t->qw = aw;
+    if (!ws) {   // what pt_table_get_query_descriptor reads back
+        t->qh = ah;
+        t->qw = aw;
+    }
     if (timing) HIPTRY(hipEventRecord(t->ev[6], s));
     LookupArgs L;
     if (t->keyed) {
         // the key planes index the main table, the GCK planes the fallback table
         const pt_table *E = t->extra[0];
-        HIPTRY(hipMemsetAsync(t->adesc.p, 0, plane * t->ndims * sizeof(float), s));
-        TRY(pt_key_forward_launch(a_dev, ah, aw, t->knet.w1, t->knet.b1, t->knet.w2, t->knet.b2, t->adesc.p, s));
+        HIPTRY(hipMemsetAsync(B.adesc, 0, plane * t->ndims * sizeof(float), s));
+        TRY(pt_key_forward_launch(a_dev, ah, aw, t->knet.w1, t->knet.b1, t->knet.w2, t->knet.b2, B.adesc, s));
         TRY(t->xadesc.ensure(plane * E->ndims));
-        TRY(t->yuv.ensure((size_t)ah * aw * 3));
-        TRY(t->temp.ensure(plane * 3));
-        TRY(pt_gck_launch(a_dev, ah, aw, E->ndims - 2 * E->nchroma, E->nchroma, E->build.convert_colorspace, t->yuv.p,
-                          t->temp.p, t->xadesc.p, s));
+        TRY(pt_gck_launch(a_dev, ah, aw, E->ndims - 2 * E->nchroma, E->nchroma, E->build.convert_colorspace, B.yuv,
+                          B.temp, t->xadesc.p, s));
         L.a = a_dev;
         L.b = t->b0.p;
     } else if (adesc_pre && !t->desc_mode) {
         L.a = a_dev;
         L.b = t->b0.p;
     } else if (t->desc_mode) {
-        TRY(t->adescI.ensure(plane * t->ndims));
-        TRY(pt_desc_pad_launch(a_dev, h, w, c, t->build.descriptor_padded, Hp, Wp, t->adesc.p, t->adescI.p, s));
+        TRY(pt_desc_pad_launch(a_dev, h, w, c, t->build.descriptor_padded, Hp, Wp, B.adesc, B.adescI, s));
         L.C = t->ndims;
-        L.a_descI = t->adescI.p;
+        L.a_descI = B.adescI;
         L.b_descI = t->descI.p;
     } else {
-        TRY(t->yuv.ensure((size_t)ah * aw * 3));
-        TRY(t->temp.ensure(plane * 3));
-        TRY(pt_gck_launch(a_dev, ah, aw, t->ndims - 2 * t->nchroma, t->nchroma, t->build.convert_colorspace, t->yuv.p,
-                          t->temp.p, t->adesc.p, s));
+        TRY(pt_gck_launch(a_dev, ah, aw, t->ndims - 2 * t->nchroma, t->nchroma, t->build.convert_colorspace, B.yuv,
+                          B.temp, B.adesc, s));
         L.a = a_dev;
         L.b = t->b0.p;
     }
@@ -772,7 +806,7 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     L.aw = aw;
     L.ann_w = ann_w;
     L.ann_h = ann_h;
-    L.a_desc = adesc_pre && !t->desc_mode && !t->keyed ? adesc_pre : t->adesc.p;
+    L.a_desc = adesc_pre && !t->desc_mode && !t->keyed ? adesc_pre : B.adesc;
     L.xa_desc = t->keyed ? t->xadesc.p : L.a_desc;
     L.bh = t->h;
     L.bw = t->w;
@@ -785,8 +819,8 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     }
     L.kpos = t->kpos.p;
     L.kdist = t->kdist.p;
-    L.hist = t->hist.p;
-    L.hn = t->hn.p;
+    L.hist = B.hist;
+    L.hn = B.hn;
     L.K = p.kcoherence;
     L.kcoh_iter0 = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == 0);
     // k-coherence walks in the rounds when kcoherence_iter asks for them after the first pass, and
@@ -809,25 +843,22 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     }
     const double temporal_scale = 1.0 / (1 + p.coherence_temporal);
     if (prev_dev) {
-        TRY(t->ixy.ensure(n));
-        TRY(t->id.ensure(n));
-        TRY(t->bad.ensure(1));
-        HIPTRY(hipMemsetAsync(t->bad.p, 0, sizeof(int), s));
-        TRY(pt_lookup_init_prev(L, prev_dev, sv_dev, p.recalc_dist_temporal, temporal_scale, p.sanitize_input, t->ixy.p,
-                                t->id.p, t->bad.p, s));
-        L.init_xy = t->ixy.p;
-        L.init_d = t->id.p;
+        HIPTRY(hipMemsetAsync(B.bad, 0, sizeof(int), s));
+        TRY(pt_lookup_init_prev(L, prev_dev, sv_dev, p.recalc_dist_temporal, temporal_scale, p.sanitize_input, B.ixy,
+                                B.id, B.bad, s));
+        L.init_xy = B.ixy;
+        L.init_d = B.id;
     }
     if (timing) HIPTRY(hipEventRecord(t->ev[7], s));
     NnfField f0, f1;
-    f0.xy = t->nxy.p;
-    f0.d = t->nd.p;
-    f0.fl = t->nfl.p;
-    f1.xy = t->nxy2.p;
-    f1.d = t->nd2.p;
-    f1.fl = t->nfl2.p;
+    f0.xy = B.nxy;
+    f0.d = B.nd;
+    f0.fl = B.nfl;
+    f1.xy = B.nxy2;
+    f1.d = B.nd2;
+    f1.fl = B.nfl2;
     int launches = 0;
-    TRY(pt_lookup_launch(L, f0, f1, t->gxy.p, t->gd.p, ann_dev, prev_dev, t->id.p, temporal_scale,
+    TRY(pt_lookup_launch(L, f0, f1, B.gxy, B.gd, ann_dev, prev_dev, B.id, temporal_scale,
                          prev_dev && p.coherence_temporal != 0.0, s, timing ? t->ev[8] : nullptr,
                          timing ? t->ev[10] : nullptr, &launches));
     if (timing) HIPTRY(hipEventRecord(t->ev[9], s));
@@ -864,6 +895,88 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
     t->stage_ms[9] = ms;
     return PT_OK;
 }
+
+}  // namespace
+
+int pt_table_device(const pt_table *t) { return t->device; }
+
+// The buffers pt_lookup_enqueue works in, laid out one after another from base in 256-byte
+// aligned pieces: query descriptor planes (and the interleaved copy in descriptor mode), the
+// GCK's colour planes and running sums (image mode), two lattice fields, the candidate history,
+// the gather output (query_step > 1) and the initial field (with a previous field).
+int pt_lookup_workspace_layout(const pt_table *t, int h, int w, int c, bool prev, unsigned char *base, size_t *bytes,
+                               LookupBufs *bufs) {
+    int ah = 0, aw = 0;
+    TRY(query_shape(t, h, w, c, ah, aw));
+    const pt_params &p = *t->p;
+    if (p.query_step < 1) return pt_fail(PT_E_INVALID, "query_step must be >= 1");
+    const size_t plane = (size_t)(ah + 7) * (aw + 7);
+    const int ann_w = aw - 7, ann_h = ah - 7, qs = p.query_step;
+    const size_t n = (size_t)ann_w * ann_h;
+    const size_t nv = (size_t)((ann_w + qs - 1) / qs) * ((ann_h + qs - 1) / qs);
+    size_t off = 0;
+    LookupBufs B;
+    auto take = [&](size_t nbytes) -> unsigned char * {
+        unsigned char *q = base ? base + off : nullptr;
+        off += (nbytes + 255) & ~(size_t)255;
+        return q;
+    };
+    B.adesc = (float *)take(plane * t->ndims * 4);
+    if (t->desc_mode) {
+        B.adescI = (float *)take(plane * t->ndims * 4);
+    } else {
+        B.yuv = (float *)take((size_t)ah * aw * 3 * 4);
+        B.temp = (float *)take(plane * 3 * 4);
+    }
+    B.nxy = (int32_t *)take(nv * 4);
+    B.nxy2 = (int32_t *)take(nv * 4);
+    B.nd = (float *)take(nv * 4);
+    B.nd2 = (float *)take(nv * 4);
+    B.nfl = take(nv);
+    B.nfl2 = take(nv);
+    B.hist = (int32_t *)take(nv * PT_LOOKUP_HIST * 4);
+    B.hn = take(nv);
+    if (qs > 1) {
+        B.gxy = (int32_t *)take(n * 4);
+        B.gd = (float *)take(n * 4);
+    }
+    if (prev) {
+        B.ixy = (int32_t *)take(n * 4);
+        B.id = (float *)take(n * 4);
+        B.bad = (int *)take(sizeof(int));
+    }
+    if (bytes) *bytes = off;
+    if (bufs) *bufs = B;
+    return PT_OK;
+}
+
+// One lookup in the caller's workspace ws (ws_bytes long, 256-byte aligned): no device memory is
+// allocated or freed. Synchronous, like pt_table_lookup_dev.
+int pt_lookup_in_workspace(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev,
+                           const double *prev_dev, const float *sv_dev, unsigned char *ws, size_t ws_bytes,
+                           double *seconds) {
+    if (t->keyed) return pt_fail(PT_E_UNSUPPORTED, "learned-key tables look up in their own buffers");
+    size_t need = 0;
+    TRY(pt_lookup_workspace_layout(t, h, w, c, prev_dev != nullptr, nullptr, &need, nullptr));
+    if (need > ws_bytes)
+        return pt_failf(PT_E_INVALID, "query needs %zu bytes of lookup workspace, more than the %zu available", need,
+                        ws_bytes);
+    LookupBufs B;
+    TRY(pt_lookup_workspace_layout(t, h, w, c, prev_dev != nullptr, ws, &need, &B));
+    const pt_params &p = *t->p;
+    const double T0 = now_s();
+    TRY(pt_lookup_enqueue(t, a_dev, h, w, c, ann_dev, prev_dev, sv_dev, nullptr, false, nullptr, &B));
+    HIPTRY(hipStreamSynchronize(t->stream));
+    if (seconds) *seconds = now_s() - T0;
+    if (prev_dev && !p.sanitize_input) {
+        int bad = 0;
+        HIPTRY(hipMemcpy(&bad, B.bad, sizeof(int), hipMemcpyDeviceToHost));
+        if (bad) return pt_fail(PT_E_INVALID, "expected xsrc, ysrc in bounds in prev_nnf mode (sanitize_input 0)");
+    }
+    return PT_OK;
+}
+
+namespace {
 
 // Shape of the NNF of a query of h x w x c (reference: (h-7) x (w-7), or h x w in descriptor mode).
 int ann_size(const pt_table *t, int h, int w, int c, long &n) {
