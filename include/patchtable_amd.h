@@ -172,9 +172,17 @@ typedef struct pt_params {
     /* ---- Parameters of this library's GPU algorithms (no reference counterpart). -------------
      * kcoh_window: half-width of the sorted-key candidate window of the k-coherence kNN.
      * jf_rounds:   number of parallel propagation rounds that replace the raster-order passes.
-     * Set by pt_params_default; accepted as switches "-kcoh_window" and "-jf_rounds". */
+     * mfma_rounds: 0 for the lookup above; n > 0 selects the MFMA lookup: table candidates
+     *              only, then n rounds of dense int8 search on matrix cores around up to
+     *              mfma_anchors shared offsets per 16 query patches, each query's best rescored
+     *              with the exact SSD (image mode, query_step 1, no allowed mask).
+     * mfma_anchors: most anchor offsets searched per group of 16 query patches and round.
+     * Set by pt_params_default; accepted as switches "-kcoh_window", "-jf_rounds",
+     * "-mfma_rounds" and "-mfma_anchors". */
     int    kcoh_window;
     int    jf_rounds;
+    int    mfma_rounds;
+    int    mfma_anchors;
 } pt_params;
 
 typedef struct pt_table pt_table;

@@ -51,6 +51,7 @@ typedef std::vector<int32_t> ivec;
 static const int TABLE_UNUSED = (1 << 30) - 1;   // patchtable.h:109-112
 static const int NMAP = 16384;                    // PATCH_PARTITION_NMAP, patchtable.h:216
 static const double DIST_INFINITY = 100000;       // patchtable.h:173
+static const int PT_MFMA_BAND_ORC = 1;            // candidate rows above/below an anchor (mfma.hip)
 
 static inline int XY_TO_INT(int x, int y) { return x | (y << 16); }   // patchtable.h:163
 static inline int INT_TO_X(int v) { return v & 65535; }
@@ -946,7 +947,8 @@ static Table *build(const pt_params &p_in, const float *b, int h_in, int w_in, i
         for (auto &v : T->table)
             if (v == TABLE_UNUSED) v = vfill;
     }
-    if (p.kcoherence > 0) build_kcoherence(*T);
+    // the MFMA lookup (mfma_rounds > 0) walks no k-coherence lists, so none are built
+    if (p.kcoherence > 0 && !(p.mfma_rounds > 0 && !T->desc)) build_kcoherence(*T);
     // Further tables (patchtable.h:3359-3372): limit / table_ratio^i, no k-coherence.
     long used = T->rand_used;
     for (int i = 1; i < p.ntables; i++) {
@@ -1302,8 +1304,220 @@ static inline int rs_pick(uint32_t i, uint32_t round) {
 // round r, [3 + 2r] spatial window of round r, [63] gather.
 static long g_ssd_count[64];
 
+// ---------------------------- lookup (this library's MFMA form) -----------------------------
+// Model of the GPU MFMA lookup (mfma_rounds > 0; image mode, query_step 1, no mask):
+//   init and candidates: as lookup_parallel with the cells of every table only (no k-coherence
+//     walk, no spatial or random search);
+//   rounds r = 0..R-1 with jump J = max(1, 2^min(4, R-1) >> r), double-buffered, for every
+//     group of 16 query patches x0 = 16k .. x0+15 of every row y:
+//     sources (slots 0..63): the members (x0+i, y), the patches (x0+i, y-J) and (x0+i, y+J),
+//       (x0+i-J, y) for i < 8 and (x0+i+J, y) for i >= 8, if inside the query; each proposes
+//       its offset (match - position);
+//     anchors: the distinct offsets (count, first slot); up to mfma_anchors times the
+//       uncovered one of largest count (ties: smallest first slot), which covers every offset
+//       within 8 columns and PT_MFMA_BAND rows of it;
+//     search: candidates (x0 + ox - 8 + j, y + oy + dy), j < 32, |dy| <= PT_MFMA_BAND, inside
+//       b, in order o = (anchor * (2 PT_MFMA_BAND + 1) + dy + PT_MFMA_BAND) * 32 + j; each member
+//       keeps the smallest (SSD, o), with SSD the integer SSD of the int8 images
+//       q = round(255 clamp(v, 0, 1)) - 128;
+//     rescoring: a member whose best differs from its match takes it if its exact SSD (the
+//       reference's lane order) is strictly smaller;
+//   output: the temporal rescaling as lookup_parallel.
+// The integer scores are exact, so this restates the matrix-core arithmetic bit for bit.
+static inline int quant8(float v) { return (int)rintf(fminf(fmaxf(v, 0.f), 1.f) * 255.f) - 128; }
+
+static int mfma_jump_at(int rounds, int r) {
+    const int big = 1 << (rounds >= 5 ? 4 : (rounds >= 1 ? rounds - 1 : 0));
+    const int j = big >> (r < 30 ? r : 30);
+    return j > 1 ? j : 1;
+}
+
+static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, int c_in, double *ann,
+                        const double *ann_prev, const float *sv) {
+    const pt_params &p = T.p;
+    const int R = PT_MFMA_BAND_ORC;
+    QuerySide Q;
+    query_side(T, a_in, h_in, w_in, c_in, Q);
+    const int ah = Q.ann_h(), aw = Q.ann_w(), AW = Q.aw, AH = Q.ah;
+    const int bew = T.bew(), beh = T.beh(), BW = T.w;
+    const long N = (long)ah * aw;
+    auto dist = [&](int x, int y, int xs, int ys) { return lookup_dist(T, Q, x, y, xs, ys); };
+    const double temporal_scale = 1.0 / (1 + p.coherence_temporal);
+    std::vector<int16_t> qa((size_t)AH * AW * 3), qb((size_t)T.h * BW * 3);
+    for (size_t i = 0; i < qa.size(); i++) qa[i] = (int16_t)quant8(Q.a[i]);
+    for (size_t i = 0; i < qb.size(); i++) qb[i] = (int16_t)quant8(T.b0[i]);
+    // init and candidates
+    std::vector<int32_t> cx(N), cy(N), ix(N), iy(N);
+    fvec cd(N), id(N);
+    const bool have_prev = ann_prev != nullptr;
+    for (long i = 0; i < N; i++) {
+        const int x = (int)(i % aw), y = (int)(i / aw);
+        bool have = false;
+        int xb = 0, yb = 0;
+        float db = 0.f;
+        if (have_prev) {
+            xb = (int)ann_prev[i * 3];
+            yb = (int)ann_prev[i * 3 + 1];
+            xb = xb < 0 ? 0 : (xb >= bew ? bew - 1 : xb);
+            yb = yb < 0 ? 0 : (yb >= beh ? beh - 1 : yb);
+            db = p.recalc_dist_temporal ? (float)(dist(x, y, xb, yb) * (sv ? (double)sv[i] : temporal_scale))
+                                        : (float)(ann_prev[i * 3 + 2] * temporal_scale);
+            have = true;
+        }
+        ix[i] = xb;
+        iy[i] = yb;
+        id[i] = db;
+        if (p.do_table_lookup) {
+            const int v = T.table[T.cell_of(Q.qdesc(x, y))];
+            const int xs = INT_TO_X(v) - 7, ys = INT_TO_Y(v) - 7;
+            if (!have || xs != xb || ys != yb) {
+                const float d = dist(x, y, xs, ys);
+                if (!have || d < db) {
+                    db = d;
+                    xb = xs;
+                    yb = ys;
+                }
+                have = true;
+            }
+            int xp = xs, yp = ys;
+            for (const Table *E : T.extra) {
+                const int v1 = E->table[E->cell_of(Q.xdesc(x, y))];
+                const int x1 = INT_TO_X(v1) - 7, y1 = INT_TO_Y(v1) - 7;
+                if ((x1 != xb || y1 != yb) && (x1 != xp || y1 != yp)) {
+                    const float d = dist(x, y, x1, y1);
+                    if (d < db) {
+                        db = d;
+                        xb = x1;
+                        yb = y1;
+                    }
+                }
+                xp = x1;
+                yp = y1;
+            }
+        }
+        cx[i] = xb;
+        cy[i] = yb;
+        cd[i] = db;
+    }
+    const int rounds = p.mfma_rounds;
+    std::vector<int32_t> nx(N), ny(N);
+    fvec nd(N);
+    for (int r = 0; r < rounds; r++) {
+        const int J = mfma_jump_at(rounds, r);
+        for (int y = 0; y < ah; y++)
+            for (int x0 = 0; x0 < aw; x0 += 16) {
+                int sox[64], soy[64];
+                bool sval[64];
+                for (int sl = 0; sl < 64; sl++) {
+                    int sx, sy;
+                    if (sl < 16) sx = x0 + sl, sy = y;
+                    else if (sl < 32) sx = x0 + sl - 16, sy = y - J;
+                    else if (sl < 48) sx = x0 + sl - 32, sy = y + J;
+                    else if (sl < 56) sx = x0 + (sl - 48) - J, sy = y;
+                    else sx = x0 + (sl - 56) + 8 + J, sy = y;
+                    sval[sl] = sx >= 0 && sy >= 0 && sx < aw && sy < ah;
+                    sox[sl] = soy[sl] = 0;
+                    if (sval[sl]) {
+                        const long k = (long)sy * aw + sx;
+                        sox[sl] = cx[k] - sx;
+                        soy[sl] = cy[k] - sy;
+                    }
+                }
+                // distinct offsets in order of first occurrence, with counts
+                std::vector<int> dfirst, dcnt;
+                for (int sl = 0; sl < 64; sl++) {
+                    if (!sval[sl]) continue;
+                    bool seen = false;
+                    for (size_t k = 0; k < dfirst.size() && !seen; k++)
+                        if (sox[dfirst[k]] == sox[sl] && soy[dfirst[k]] == soy[sl]) {
+                            dcnt[k]++;
+                            seen = true;
+                        }
+                    if (!seen) {
+                        dfirst.push_back(sl);
+                        dcnt.push_back(1);
+                    }
+                }
+                std::vector<char> covered(dfirst.size(), 0);
+                uint64_t best[16];
+                for (int i = 0; i < 16; i++) best[i] = ~0ull;
+                std::vector<int> aox, aoy;
+                for (int a = 0; a < p.mfma_anchors; a++) {
+                    int pick = -1;
+                    for (size_t k = 0; k < dfirst.size(); k++) {
+                        if (covered[k]) continue;
+                        if (pick < 0 || dcnt[k] > dcnt[pick] || (dcnt[k] == dcnt[pick] && dfirst[k] < dfirst[pick]))
+                            pick = (int)k;
+                    }
+                    if (pick < 0) break;
+                    const int ox = sox[dfirst[pick]], oy = soy[dfirst[pick]];
+                    aox.push_back(ox);
+                    aoy.push_back(oy);
+                    for (size_t k = 0; k < dfirst.size(); k++)
+                        if (std::abs(sox[dfirst[k]] - ox) <= 8 && std::abs(soy[dfirst[k]] - oy) <= R) covered[k] = 1;
+                    for (int dy = -R; dy <= R; dy++)
+                        for (int j = 0; j < 32; j++) {
+                            const int xs = x0 + ox - 8 + j, ys = y + oy + dy;
+                            if (xs < 0 || ys < 0 || xs >= bew || ys >= beh) continue;
+                            const uint32_t ord = (uint32_t)((a * (2 * R + 1) + dy + R) * 32 + j);
+                            for (int i = 0; i < 16 && x0 + i < aw; i++) {
+                                int ssd = 0;
+                                for (int rr = 0; rr < 8; rr++) {
+                                    const int16_t *qr = &qa[((size_t)(y + rr) * AW + x0 + i) * 3];
+                                    const int16_t *br = &qb[((size_t)(ys + rr) * BW + xs) * 3];
+                                    for (int e = 0; e < 24; e++) {
+                                        const int dd = qr[e] - br[e];
+                                        ssd += dd * dd;
+                                    }
+                                }
+                                const uint64_t key = ((uint64_t)(uint32_t)ssd << 11) | ord;
+                                if (key < best[i]) best[i] = key;
+                            }
+                        }
+                }
+                for (int i = 0; i < 16 && x0 + i < aw; i++) {
+                    const long k = (long)y * aw + x0 + i;
+                    int xb = cx[k], yb = cy[k];
+                    float db = cd[k];
+                    if (best[i] != ~0ull) {
+                        const int ord = (int)(best[i] & 2047u), j = ord & 31, dyi = (ord >> 5) % (2 * R + 1),
+                                  an = (ord >> 5) / (2 * R + 1);
+                        const int xs = x0 + aox[an] - 8 + j, ys = y + aoy[an] - R + dyi;
+                        if (xs != xb || ys != yb) {
+                            const float d = dist(x0 + i, y, xs, ys);
+                            if (d < db) {
+                                db = d;
+                                xb = xs;
+                                yb = ys;
+                            }
+                        }
+                    }
+                    nx[k] = xb;
+                    ny[k] = yb;
+                    nd[k] = db;
+                }
+            }
+        cx.swap(nx);
+        cy.swap(ny);
+        cd.swap(nd);
+    }
+    for (long i = 0; i < N; i++) {
+        double d = cd[i];
+        if (have_prev && p.coherence_temporal != 0.0 && (double)cx[i] == ann_prev[i * 3] &&
+            (double)cy[i] == ann_prev[i * 3 + 1] && cd[i] == id[i])
+            d = d / temporal_scale;
+        ann[i * 3 + 0] = cx[i];
+        ann[i * 3 + 1] = cy[i];
+        ann[i * 3 + 2] = d;
+    }
+}
+
 static void lookup_parallel(const Table &T, const float *a_in, int h_in, int w_in, int c_in, double *ann,
                             const double *ann_prev, const float *sv) {
+    if (T.p.mfma_rounds > 0) {
+        lookup_mfma(T, a_in, h_in, w_in, c_in, ann, ann_prev, sv);
+        return;
+    }
     for (auto &c : g_ssd_count) c = 0;
     const pt_params &p = T.p;
     QuerySide Q;

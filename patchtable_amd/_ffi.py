@@ -108,6 +108,8 @@ class PtParams(ctypes.Structure):
         ("ann_eps", ctypes.c_double),
         ("kcoh_window", ctypes.c_int),
         ("jf_rounds", ctypes.c_int),
+        ("mfma_rounds", ctypes.c_int),
+        ("mfma_anchors", ctypes.c_int),
     ]
 
 

@@ -768,16 +768,19 @@ int pt_lookup_init_prev(const LookupArgs &L, const double *prev, const float *sv
     return PT_OK;
 }
 
-int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, int32_t *gxy, float *gd, double *ann_out,
-                     const double *prev, const float *init_d, double temporal_scale, int temporal_fix, hipStream_t s,
-                     hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds, int *launches) {
+int pt_lookup_candidates(const LookupArgs &L, NnfField f0, hipStream_t s) {
     if (L.C > PT_LOOKUP_CMAX) return pt_failf(PT_E_UNSUPPORTED, "descriptor dims %d > %d", L.C, PT_LOOKUP_CMAX);
-    int which = 0, nl = 0;
-    if (L.C > 0) which = launch_lookup<M_DESC>(L, f0, f1, s, ev_after_candidates, nl);
-    else if (L.qs == 1) which = launch_lookup<M_TILE>(L, f0, f1, s, ev_after_candidates, nl);
-    else which = launch_lookup<M_GLOBAL>(L, f0, f1, s, ev_after_candidates, nl);
-    if (which < 0) return which;
-    NnfField res = which ? f1 : f0;
+    dim3 grid((L.vw + TILE_X - 1) / TILE_X, (L.vh + TILE_Y - 1) / TILE_Y);
+    if (L.C > 0) hipLaunchKernelGGL(k_candidates<M_DESC>, grid, dim3(THREADS), 0, s, L, f0);
+    else if (L.qs == 1) hipLaunchKernelGGL(k_candidates<M_TILE>, grid, dim3(THREADS), 0, s, L, f0);
+    else hipLaunchKernelGGL(k_candidates<M_GLOBAL>, grid, dim3(THREADS), 0, s, L, f0);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_lookup_output(const LookupArgs &L, NnfField res, int32_t *gxy, float *gd, double *ann_out, const double *prev,
+                     const float *init_d, double temporal_scale, int temporal_fix, hipStream_t s, int *launches,
+                     hipEvent_t ev_before_write) {
     const long n = (long)L.ann_w * L.ann_h;
     const int32_t *oxy = res.xy;
     const float *od = res.d;
@@ -790,12 +793,28 @@ int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, int32_t *gxy
         else hipLaunchKernelGGL(k_gather<M_TILE>, grid, dim3(THREADS), 0, s, G, res, gxy, gd);
         oxy = gxy;
         od = gd;
-        nl++;
+        if (launches) (*launches)++;
     }
-    if (ev_after_rounds) PT_HIP_TRY(hipEventRecord(ev_after_rounds, s));
+    if (ev_before_write) PT_HIP_TRY(hipEventRecord(ev_before_write, s));
     hipLaunchKernelGGL(k_write_ann, dim3(pt_blocks(n, 256)), dim3(256), 0, s, oxy, od, n, ann_out, prev, init_d,
                        temporal_scale, temporal_fix);
     PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, int32_t *gxy, float *gd, double *ann_out,
+                     const double *prev, const float *init_d, double temporal_scale, int temporal_fix, hipStream_t s,
+                     hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds, int *launches) {
+    if (L.C > PT_LOOKUP_CMAX) return pt_failf(PT_E_UNSUPPORTED, "descriptor dims %d > %d", L.C, PT_LOOKUP_CMAX);
+    int which = 0, nl = 0;
+    if (L.C > 0) which = launch_lookup<M_DESC>(L, f0, f1, s, ev_after_candidates, nl);
+    else if (L.qs == 1) which = launch_lookup<M_TILE>(L, f0, f1, s, ev_after_candidates, nl);
+    else which = launch_lookup<M_GLOBAL>(L, f0, f1, s, ev_after_candidates, nl);
+    if (which < 0) return which;
+    NnfField res = which ? f1 : f0;
+    const int rc = pt_lookup_output(L, res, gxy, gd, ann_out, prev, init_d, temporal_scale, temporal_fix, s, &nl,
+                                    ev_after_rounds);
+    if (rc != PT_OK) return rc;
     if (launches) *launches = nl;
     return PT_OK;
 }

@@ -1,0 +1,420 @@
+// MFMA lookup rounds on gfx950: dense int8 candidate scoring on matrix cores, exact rescoring.
+//
+// The plain lookup (lookup.hip) evaluates each candidate with one exact FP32 SSD per lane: 8
+// rows x 96 bytes gathered at an arbitrary place in b, about 48 tag lookups of L1, and the
+// lookup is bound by that gather rate. Here the candidates of 16 query patches that share a
+// nearby offset are scored together as a dense contraction on v_mfma_i32_16x16x64_i8:
+//
+//   for query q and database patch p (8x8 RGB quantised to int8: round(255 clamp(v, 0, 1)) - 128,
+//   laid out RGB0, so a patch is a 256-byte vector with 64 zero bytes),
+//       |q - p|^2 = |q|^2 + |p|^2 - 2 q.p ;
+//   an exact int32 (the MFMA accumulates int8 products in int32: no rounding anywhere).
+//
+// One round with jump J, for each group of 16 consecutive query patches of a row (x0 = 16k):
+//   sources: 64 positions -- the 16 members, the 16 patches J rows above and J rows below,
+//     8 patches J to the left of members 0-7 and 8 patches J to the right of members 8-15;
+//     each valid source proposes its current offset (match - position);
+//   anchors: the distinct offsets in slot order of first occurrence with their counts; up to
+//     mfma_anchors times take the uncovered offset of largest count (ties: first slot) and mark
+//     every offset within 8 columns and R rows of it covered;
+//   search: for anchor (ox, oy), database patches x0 + ox - 8 + j (j < 32) on rows
+//     y + oy + dy (|dy| <= R) -- each member sees every offset within 8 columns of the anchor;
+//     each member keeps the candidate of smallest (SSD, anchor, dy, j), SSD = |q|^2 + |p|^2 - 2 q.p
+//     the exact integer SSD of the int8 patches;
+//   rescoring: a member whose best candidate differs from its match computes the exact SSD in
+//     the reference's lane order (patch_dist_exact, patchtable.h:330-373) and takes it when
+//     strictly closer, so NNF_DIST is always the exact distance of the stored match.
+// The oracle (lookup_parallel with mfma_rounds > 0) restates this bit for bit: the integer
+// scores do not depend on summation order.
+//
+// Work layout: a workgroup of 4 waves owns a 64 x 4 tile of query patches; each wave takes one
+// row, its 4 groups in turn. A wave stages each anchor's band of b (39 x 10 pixels, int8 RGB0)
+// in its own LDS region; each lane loads the B fragments of one candidate column for the 9 row
+// pairs the band's 3 rows of candidates use, and the 3 x 4 MFMAs of a 16-candidate block read
+// them from registers. The float query tile staged for the rescoring is the lookup's own.
+#include "pt_device.h"
+#include "pt_kernels.h"
+#include "pt_tile.h"
+
+namespace {
+
+using namespace pt_tile;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int R = PT_MFMA_BAND;        // rows of candidates above and below the anchor
+constexpr int BR = 8 + 2 * R;          // band rows
+constexpr int BC = 32 + 7;             // band columns (32 candidates of 8 pixels)
+constexpr uint64_t NONE = ~0ull;
+
+__device__ __forceinline__ uint32_t quant_pixel(const float *p) {
+    uint32_t u = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float v = fminf(fmaxf(p[c], 0.f), 1.f) * 255.f;
+        const int q = (int)rintf(v) - 128;
+        u |= ((uint32_t)q & 0xffu) << (8 * c);
+    }
+    return u;
+}
+
+// int8 RGB0 copy of an RGB float image.
+__global__ void k_quant_rgb0(const float *__restrict__ img, long npx, uint32_t *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npx) out[i] = quant_pixel(img + i * 3);
+}
+
+__device__ __forceinline__ int sq_pixel(uint32_t u) {
+    const int r = (int)(int8_t)(u & 0xff), g = (int)(int8_t)((u >> 8) & 0xff), b = (int)(int8_t)((u >> 16) & 0xff);
+    return r * r + g * g + b * b;
+}
+
+constexpr int P = PT_MFMA_PAD;
+
+// int8 RGB0 copy of an h x w RGB image into the interior of a (h + 2P) x (w + 2P) array.
+__global__ void k_quant_rgb0_pad(const float *__restrict__ img, int h, int w, uint32_t *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)h * w) return;
+    const int x = (int)(i % w), y = (int)(i / w);
+    out[(size_t)(y + P) * (w + 2 * P) + x + P] = quant_pixel(img + i * 3);
+}
+
+// |p|^2 of every 8x8 patch of the padded int8 copy (pw x ph patches), into the interior of a
+// (ph + 2P) x (pw + 2P) array.
+__global__ void k_patch_norms_pad(const uint32_t *__restrict__ q, int w, int pw, int ph, int32_t *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)pw * ph) return;
+    const int x = (int)(i % pw), y = (int)(i / pw);
+    int s = 0;
+    for (int r = 0; r < 8; r++) {
+        const uint32_t *row = q + (size_t)(y + r + P) * (w + 2 * P) + x + P;
+#pragma unroll
+        for (int c = 0; c < 8; c++) s += sq_pixel(row[c]);
+    }
+    out[(size_t)(y + P) * (pw + 2 * P) + x + P] = s;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    const int lo = __shfl((int)(uint32_t)v, src), hi = __shfl((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)v, m), hi = __shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+// Minimum over each row of 16 lanes, left in every lane of the row: DPP swaps within quads, then
+// the half-row mirror (lane i <-> 7 - i) joins the two quads of each half, the row mirror
+// (i <-> 15 - i) the two halves.
+__device__ __forceinline__ uint32_t row16_min(uint32_t v) {
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true));   // quad_perm 1,0,3,2
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true));   // quad_perm 2,3,0,1
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true));  // row_half_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, true));  // row_mirror
+    return v;
+}
+__device__ __forceinline__ uint32_t row16_max(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true));
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true));
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true));
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, true));
+    return v;
+}
+// The 64-bit row minimum, by the same four DPP steps on both halves.
+#define PT_DPP64_STEP(v, ctrl)                                                                           \
+    {                                                                                                    \
+        const uint32_t lo_ = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(v), ctrl, 0xF, 0xF, true); \
+        const uint32_t hi_ = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)((v) >> 32), ctrl, 0xF, 0xF, true); \
+        const uint64_t o_ = ((uint64_t)hi_ << 32) | lo_;                                                 \
+        if (o_ < (v)) (v) = o_;                                                                          \
+    }
+__device__ __forceinline__ uint64_t row16_min64(uint64_t v) {
+    PT_DPP64_STEP(v, 0xB1)
+    PT_DPP64_STEP(v, 0x4E)
+    PT_DPP64_STEP(v, 0x141)
+    PT_DPP64_STEP(v, 0x140)
+    return v;
+}
+#undef PT_DPP64_STEP
+// Orders this wave's LDS writes before its later reads (and reads before later writes).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int NB = 32 * (2 * R + 1);               // candidate norms of one band
+constexpr int PER = (BR * BC + NB + 63) / 64;      // band words per lane
+constexpr int MAXITEMS = 4 * PT_MFMA_MAX_ANCHORS;  // (group, anchor) items of one wave
+
+// Loads the band of item (x0, ox, oy) into registers: band pixels (u0 + c, v0 + r), then the norms
+// of its 32 x (2R+1) candidates, from the padded copies. The band origin lies inside the margins
+// (see PT_MFMA_PAD); it is clamped to them anyway, so a load can never leave the arrays.
+__device__ __forceinline__ void band_fetch(const MfmaArgs &M, int lane, int x0, int y, int ox, int oy,
+                                           uint32_t (&reg)[PER]) {
+    const int bwp = M.bw + 2 * P, bnp = M.bw - 7 + 2 * P;
+    const int u0 = min(max(x0 + ox - 8, -P), M.bw + P - BC), v0 = min(max(y + oy - R, -P), M.bh + P - BR);
+    const uint32_t *bq = M.bq + (size_t)(v0 + P) * bwp + (u0 + P);
+    const int32_t *bn = M.bnorm + (size_t)(v0 + P) * bnp + (u0 + P);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int e = lane + 64 * k;
+        if (e < BR * BC) {
+            const int r = (e * 1681) >> 16, c = e - r * BC;   // e / 39 for e < 1000
+            reg[k] = bq[r * bwp + c];
+        } else if (e < BR * BC + NB) {
+            const int n = e - BR * BC;
+            reg[k] = (uint32_t)bn[(n >> 5) * bnp + (n & 31)];
+        } else {
+            reg[k] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(THREADS, 4) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J) {
+    __shared__ float4 qtile[AX * AY];           // float query pixels (exact rescoring)
+    __shared__ uint32_t qq[AX * AY];            // int8 RGB0 query pixels (A fragments)
+    __shared__ uint32_t band[THREADS / 64][BR * BC + NB];
+    __shared__ int items[THREADS / 64][MAXITEMS][3];
+    __shared__ int rq_n, rq_q[THREADS], rq_c[THREADS], rq_v[THREADS];   // rescoring queue
+    __shared__ float rq_d[THREADS];
+    if (threadIdx.x == 0) rq_n = 0;   // ordered before use by stage_tile's barrier
+    const int2 tc = tile_coords();
+    const int x00 = tc.x * TILE_X, y00 = tc.y * TILE_Y;
+    for (int i = threadIdx.x; i < AX * AY; i += THREADS) {
+        const int ly = i / AX, lx = i - ly * AX, gx = x00 + lx, gy = y00 + ly;
+        qq[i] = gx < M.aw && gy < M.ah ? M.aq[(size_t)gy * M.aw + gx] : 0u;
+    }
+    stage_tile(M.a, M.ah, M.aw, qtile);   // every thread; synchronises the block
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, jj = lane & 15;
+    const int y = y00 + wave, xw0 = x00;
+    const int aw = M.ann_w, ah = M.ann_h;
+    uint32_t *bnd = band[wave];
+    uint64_t mykey = NONE;   // best candidate of query (xw0 + lane, y)
+    int nitems = 0;
+    if (y < ah) {
+        // ---- phase 1: sources, distinct offsets and anchors of the wave's 4 groups. An offset is
+        // packed into one int (x in the low 16 bits): widths and heights are at most 32768.
+        int sxy[4], sx[4], sy[4];
+        bool sv[4];
+#pragma unroll
+        for (int gi = 0; gi < 4; gi++) {
+            const int x0 = xw0 + 16 * gi;
+            if (lane < 16) sx[gi] = x0 + lane, sy[gi] = y;
+            else if (lane < 32) sx[gi] = x0 + lane - 16, sy[gi] = y - J;
+            else if (lane < 48) sx[gi] = x0 + lane - 32, sy[gi] = y + J;
+            else if (lane < 56) sx[gi] = x0 + (lane - 48) - J, sy[gi] = y;
+            else sx[gi] = x0 + (lane - 56) + 8 + J, sy[gi] = y;
+            sv[gi] = x0 < aw && sx[gi] >= 0 && sy[gi] >= 0 && sx[gi] < aw && sy[gi] < ah;
+            sxy[gi] = sv[gi] ? cur.xy[(long)sy[gi] * aw + sx[gi]] : 0;   // all four loads in flight
+        }
+        for (int gi = 0; gi < 4; gi++) {
+            const int x0 = xw0 + 16 * gi;
+            if (x0 >= aw) break;
+            const bool v = sv[gi];
+            const int ox = PT_INT_TO_X(sxy[gi]) - sx[gi], oy = PT_INT_TO_Y(sxy[gi]) - sy[gi];
+            const int po = v ? (int)(((uint32_t)ox & 0xffffu) | ((uint32_t)oy << 16)) : 0;
+            // count and first slot of this lane's offset, one distinct offset per pass
+            int cnt = 0;
+            bool rep = false;
+            unsigned long long todo = __ballot(v);
+            while (todo) {
+                const int s0 = __builtin_ctzll(todo);
+                const int pv = __builtin_amdgcn_readlane(po, s0);
+                const bool eq = v && po == pv;
+                const unsigned long long m = __ballot(eq);
+                todo &= ~m;
+                if (eq) {
+                    cnt = __popcll(m);
+                    rep = lane == s0;   // first occurrence of its offset
+                }
+            }
+            bool covered = false;
+            for (int a = 0; a < M.nanchors; a++) {
+                // the uncovered offset of largest count, ties to the first slot (keys are unique)
+                const uint32_t key = rep && !covered ? (uint32_t)((cnt << 6) | (63 - lane)) + 1u : 0u;
+                const uint32_t r = row16_max(key);
+                const uint32_t kmax = max(max((uint32_t)__builtin_amdgcn_readlane((int)r, 0),
+                                              (uint32_t)__builtin_amdgcn_readlane((int)r, 16)),
+                                          max((uint32_t)__builtin_amdgcn_readlane((int)r, 32),
+                                              (uint32_t)__builtin_amdgcn_readlane((int)r, 48)));
+                if (kmax == 0u) break;
+                const int src = 63 - (int)((kmax - 1u) & 63u);
+                const int pa = __builtin_amdgcn_readlane(po, src);
+                const int ax = (int)(int16_t)(pa & 0xffff), ay = pa >> 16;
+                covered = covered || (v && abs(ox - ax) <= 8 && abs(oy - ay) <= R);
+                if (lane == 0) {
+                    items[wave][nitems][0] = gi;
+                    items[wave][nitems][1] = ax;
+                    items[wave][nitems][2] = ay;
+                }
+                nitems++;
+            }
+        }
+    }
+    wave_lds_sync();
+    // ---- phase 2: the items in order, the next item's band loading during this one's MFMAs.
+    // Each lane scores one candidate column jj against the 4 members 4g + t. Within a lane a
+    // candidate's key is (|p|^2 + OFF - 2 q.p) << 7 | ord with ord = (anchor * (2R+1) + dy + R) * 2
+    // + block: |q|^2 is the same for all candidates of a member, and OFF >= |q|^2 makes the key
+    // 128 (SSD - |q|^2 + OFF) + ord a non-negative int below 2^31 that ranks by (SSD, ord).
+    // Candidates outside b get keys above 2.4e9. At the end of a group the lanes' minima are
+    // reduced as (key, jj), i.e. (SSD, anchor, dy, j) with j = 16 block + jj.
+    constexpr uint32_t OFF = 192u * 128u * 128u;
+    constexpr uint32_t INV = 3200000000u, INV_LO = 2400000000u;
+    uint32_t reg[PER];
+    if (nitems > 0) band_fetch(M, lane, xw0 + 16 * items[wave][0][0], y, items[wave][0][1], items[wave][0][2], reg);
+    v4i A[4];
+    uint32_t best[4] = {~0u, ~0u, ~0u, ~0u};
+    int cur_g = -1, ga = 0, gfirst = 0;   // group of the current items, anchor index in it, its first item
+    const int bew = M.bw - 7, beh = M.bh - 7;
+    for (int it = 0; it < nitems; it++) {
+        const int gi = items[wave][it][0], ax = items[wave][it][1], ay = items[wave][it][2];
+        const int x0 = xw0 + 16 * gi;
+        wave_lds_sync();   // the previous item's reads are done
+#pragma unroll
+        for (int k = 0; k < PER; k++)
+            if (lane + 64 * k < BR * BC + NB) bnd[lane + 64 * k] = reg[k];
+        wave_lds_sync();
+        if (it + 1 < nitems)
+            band_fetch(M, lane, xw0 + 16 * items[wave][it + 1][0], y, items[wave][it + 1][1], items[wave][it + 1][2], reg);
+        if (gi != cur_g) {
+            cur_g = gi;
+            ga = 0;
+            gfirst = it;
+            // A fragments of member jj: k-step s covers patch rows 2s + (g >> 1), pixels 4(g & 1)..+3
+            const bool mv = x0 + jj < aw;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const uint32_t *src = qq + (wave + 2 * s + (g >> 1)) * AX + 16 * gi + jj + 4 * (g & 1);
+                A[s] = mv ? v4i{(int)src[0], (int)src[1], (int)src[2], (int)src[3]} : v4i{0, 0, 0, 0};
+            }
+        }
+        const int u0 = x0 + ax - 8, v0 = y + ay - R;
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int col = 16 * b + jj + 4 * (g & 1);
+            v4i B[9];   // band row pairs from row r0 (+ (g >> 1) for this lane's half)
+#pragma unroll
+            for (int r0 = 0; r0 < 9; r0++) {
+                const uint32_t *p = bnd + (r0 + (g >> 1)) * BC + col;
+                B[r0] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
+            }
+            const int xs = u0 + 16 * b + jj;
+            const bool xok = xs >= 0 && xs < bew;
+#pragma unroll
+            for (int dyi = 0; dyi < 2 * R + 1; dyi++) {
+                v4i C = v4i{0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < 4; s++) C = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s], B[dyi + 2 * s], C, 0, 0, 0);
+                const int ys = v0 + dyi;
+                const bool cv = xok && ys >= 0 && ys < beh;
+                const uint32_t nb = bnd[BR * BC + dyi * 32 + 16 * b + jj];   // read for every lane (no branch)
+                const uint32_t kv = ((nb + OFF) << 7) | (uint32_t)((ga * (2 * R + 1) + dyi) * 2 + b);
+                const uint32_t base = cv ? kv : INV;
+                // C[t]: member 4g + t against candidate column jj; |2 q.p| < 2^23 (24-bit operands)
+#pragma unroll
+                for (int t = 0; t < 4; t++) best[t] = min(best[t], base + (uint32_t)__mul24(C[t], -256));
+            }
+        }
+        ga++;
+        if (it + 1 == nitems || items[wave][it + 1][0] != gi) {
+            // best (key, jj) of member 4g + t over the group's 16 candidate lanes
+            uint64_t k64[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) k64[t] = row16_min64(((uint64_t)best[t] << 4) | (uint32_t)jj);
+            // member i = jj of lanes 16 gi .. takes it from lane 16 (i >> 2), register i & 3
+            const int srcl = 16 * (jj >> 2);
+            uint64_t v[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) v[t] = shfl64(k64[t], srcl);
+            const uint64_t k = (jj & 3) == 0 ? v[0] : (jj & 3) == 1 ? v[1] : (jj & 3) == 2 ? v[2] : v[3];
+            const uint32_t key = (uint32_t)(k >> 4);
+            if (g == gi && key < INV_LO) {
+                const int ord = (int)(key & 127u), j = 16 * (ord & 1) + (int)(k & 15u), dyi = (ord >> 1) % (2 * R + 1),
+                          an = (ord >> 1) / (2 * R + 1);
+                const int oxa = items[wave][gfirst + an][1], oya = items[wave][gfirst + an][2];
+                mykey = (uint64_t)(uint32_t)PT_XY_TO_INT(x0 + oxa - 8 + j, y + oya - R + dyi);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) best[t] = ~0u;
+        }
+    }
+    // ---- phase 3: exact rescoring. Queries whose best candidate differs from their match go to a
+    // block queue in LDS and are rescored densely (the first threads of the block take one entry
+    // each), so a round with few changes costs one wave of SSDs instead of four.
+    const int x = xw0 + lane;
+    const bool qv = y < ah && x < aw;
+    const long vi = (long)y * aw + x;
+    int v = 0;
+    float d = 0.f;
+    int cand = 0;
+    bool push = false;
+    if (qv) {
+        v = cur.xy[vi];
+        d = cur.d[vi];
+        if (mykey != NONE) {
+            cand = (int)(uint32_t)mykey;
+            push = PT_INT_TO_X(cand) != PT_INT_TO_X(v) || PT_INT_TO_Y(cand) != PT_INT_TO_Y(v);
+        }
+        if (!push) {
+            nxt.xy[vi] = v;
+            nxt.d[vi] = d;
+            nxt.fl[vi] = 0;
+        }
+    }
+    const unsigned long long pm = __ballot(push);
+    int base = 0;
+    if (lane == 0 && pm) base = atomicAdd(&rq_n, __popcll(pm));
+    base = __shfl(base, 0);
+    if (push) {
+        const int slot = base + __popcll(pm & ((1ull << lane) - 1ull));
+        rq_q[slot] = threadIdx.x;
+        rq_c[slot] = cand;
+        rq_v[slot] = v;
+        rq_d[slot] = d;
+    }
+    __syncthreads();
+    const int n = rq_n;
+    if ((int)threadIdx.x < n) {
+        const int k = threadIdx.x, q = rq_q[k], c = rq_c[k];
+        const int qw = q >> 6, ql = q & 63;
+        const float dq = rq_d[k];
+        const long qi = (long)(y00 + qw) * aw + (xw0 + ql);
+        const int xs = PT_INT_TO_X(c), ys = PT_INT_TO_Y(c);
+        const float dn = ssd_staged<true>(qtile + qw * AX + ql, M.b + ((size_t)ys * M.bw + xs) * 3, M.bw * 3, dq);
+        const bool take = dn < dq;
+        nxt.xy[qi] = take ? c : rq_v[k];
+        nxt.d[qi] = take ? dn : dq;
+        nxt.fl[qi] = 0;
+    }
+}
+
+}  // namespace
+
+int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t s) {
+    if (npx <= 0) return PT_OK;
+    hipLaunchKernelGGL(k_quant_rgb0, dim3(pt_blocks(npx, 256)), dim3(256), 0, s, img, npx, out);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, int32_t *bnorm, hipStream_t s) {
+    const long npx = (long)h * w, n = (long)(w - 7) * (h - 7);
+    hipLaunchKernelGGL(k_quant_rgb0_pad, dim3(pt_blocks(npx, 256)), dim3(256), 0, s, b, h, w, bq);
+    hipLaunchKernelGGL(k_patch_norms_pad, dim3(pt_blocks(n, 256)), dim3(256), 0, s, bq, w, w - 7, h - 7, bnorm);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f1, hipStream_t s, int *which) {
+    if (M.nanchors < 1 || M.nanchors > PT_MFMA_MAX_ANCHORS)
+        return pt_failf(PT_E_INVALID, "mfma_anchors must be in 1..%d", PT_MFMA_MAX_ANCHORS);
+    dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, (M.ann_h + TILE_Y - 1) / TILE_Y);
+    for (int r = 0; r < rounds; r++) {
+        hipLaunchKernelGGL(k_mfma_round, grid, dim3(THREADS), 0, s, M, f0, f1, pt_mfma_jump_at(rounds, r));
+        std::swap(f0, f1);
+    }
+    PT_LAUNCH_CHECK();
+    *which = rounds % 2;
+    return PT_OK;
+}

@@ -236,6 +236,8 @@ int apply_switches(pt_params *p, const vector<string> &argv) {
     // Switches of this library's GPU algorithms (no counterpart in the reference).
     if (has("-kcoh_window")) p->kcoh_window = to_int(sw["-kcoh_window"]);
     if (has("-jf_rounds")) p->jf_rounds = to_int(sw["-jf_rounds"]);
+    if (has("-mfma_rounds")) p->mfma_rounds = to_int(sw["-mfma_rounds"]);
+    if (has("-mfma_anchors")) p->mfma_anchors = to_int(sw["-mfma_anchors"]);
     return PT_OK;
 }
 
@@ -339,5 +341,7 @@ extern "C" int pt_params_default(pt_params *p) {
     p->ann_eps = 3.0;
     p->kcoh_window = 8;
     p->jf_rounds = 6;
+    p->mfma_rounds = 0;
+    p->mfma_anchors = 6;
     return pt_params_set_speed(p, 3);
 }

@@ -221,6 +221,41 @@ static inline int pt_jump_at(int rounds, int r) {
     const int j = big >> (r < 30 ? r : 30);
     return j > 1 ? j : 1;
 }
+// mfma.hip: the MFMA lookup rounds (dense int8 scoring of 16 queries x 32 x (2R+1) candidates
+// per anchor offset, exact rescoring of each query's best).
+#define PT_MFMA_BAND 1          // R: candidate rows above and below an anchor
+#define PT_MFMA_MAX_ANCHORS 16
+// Margin of zeros around the int8 copy of b and its norm table: every band a round reads lies
+// inside it (anchor offsets come from matches inside b and sources within 16 + 15 columns and 16
+// rows of the group), so the band loads need no bounds checks.
+#define PT_MFMA_PAD 48
+struct MfmaArgs {
+    const float *a = nullptr, *b = nullptr;         // RGB float images (exact rescoring)
+    const uint32_t *aq = nullptr;                   // int8 RGB0 query, ah x aw
+    const uint32_t *bq = nullptr;                   // int8 RGB0 database, (bh + 2 PAD) x (bw + 2 PAD)
+    const int32_t *bnorm = nullptr;                 // |p|^2 of every database patch, (beh + 2 PAD) x (bew + 2 PAD)
+    int ah = 0, aw = 0, bh = 0, bw = 0;             // image sizes
+    int ann_w = 0, ann_h = 0;                       // query patches (query_step 1)
+    int nanchors = 0;
+};
+// Jump of MFMA round r of `rounds`: 16, 8, 4, 2, 1, then unit jumps (fewer rounds start lower).
+static inline int pt_mfma_jump_at(int rounds, int r) {
+    const int big = 1 << (rounds >= 5 ? 4 : (rounds >= 1 ? rounds - 1 : 0));
+    const int j = big >> (r < 30 ? r : 30);
+    return j > 1 ? j : 1;
+}
+int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t s);
+// The padded int8 copy of b and its patch norms (margins of PT_MFMA_PAD zeros, written by the caller).
+int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, int32_t *bnorm, hipStream_t s);
+// Runs `rounds` rounds from f0; *which = 1 if the result is in f1.
+int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f1, hipStream_t s, int *which);
+// Output stage of a lookup (gather for query_step > 1, then the double field): lookup.hip.
+int pt_lookup_output(const LookupArgs &L, NnfField res, int32_t *gxy, float *gd, double *ann_out, const double *prev,
+                     const float *init_d, double temporal_scale, int temporal_fix, hipStream_t s, int *launches,
+                     hipEvent_t ev_before_write = nullptr);
+// Candidate pass only (tables, k-coherence walk per L) into f0: lookup.hip.
+int pt_lookup_candidates(const LookupArgs &L, NnfField f0, hipStream_t s);
+
 // Device buffers one lookup works in: the table's own (grown on demand), or carved from a
 // caller's workspace (pt_lookup_workspace_layout).
 struct LookupBufs {
@@ -229,6 +264,7 @@ struct LookupBufs {
     float *nd = nullptr, *nd2 = nullptr, *gd = nullptr, *id = nullptr;
     uint8_t *nfl = nullptr, *nfl2 = nullptr, *hn = nullptr;
     int *bad = nullptr;
+    uint32_t *aq = nullptr;   // int8 RGB0 query (MFMA lookup)
 };
 struct pt_table;
 // Enqueues one lookup on the table's stream (table.cpp). ws: the buffers to work in, or NULL for

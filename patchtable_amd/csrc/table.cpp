@@ -83,9 +83,12 @@ struct pt_table {
     DevBuf<uint64_t> dtkey;
     DevBuf<int32_t> kpos;
     DevBuf<float> kdist;
+    DevBuf<uint32_t> bq;      // MFMA lookup: int8 RGB0 copy of b0
+    DevBuf<int32_t> bnorm;    // ... and |p|^2 of every database patch
     int n_pop = 0;          // patches listed for populate (allowed ones)
     // query side
     DevBuf<float> a, adesc, adescI, yuv, temp;
+    DevBuf<uint32_t> aq;
     DevBuf<int32_t> nxy, nxy2, gxy, ixy;
     DevBuf<float> nd, nd2, gd, id;
     DevBuf<uint8_t> nfl, nfl2;
@@ -544,6 +547,19 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
 
 // PatchTableFixedN constructor (patchtable.h:2584-3373) from a device image (RGB, or a
 // descriptor image in descriptor mode) of h_in x w_in x c.
+// The MFMA lookup's view of b: an int8 RGB0 copy of b0 and the squared norm of every patch, both
+// inside margins of PT_MFMA_PAD zeros.
+int prepare_mfma(pt_table *t, hipStream_t s) {
+    const size_t nq = (size_t)(t->h + 2 * PT_MFMA_PAD) * (t->w + 2 * PT_MFMA_PAD);
+    const size_t nn = (size_t)(t->h - 7 + 2 * PT_MFMA_PAD) * (t->w - 7 + 2 * PT_MFMA_PAD);
+    TRY(t->bq.ensure(nq));
+    TRY(t->bnorm.ensure(nn));
+    HIPTRY(hipMemsetAsync(t->bq.p, 0, nq * 4, s));      // the margins stay zero
+    HIPTRY(hipMemsetAsync(t->bnorm.p, 0, nn * 4, s));
+    TRY(pt_mfma_prepare_launch(t->b0.p, t->h, t->w, t->bq.p, t->bnorm.p, s));
+    return PT_OK;
+}
+
 int build_table(pt_table *t, const float *b_dev, int h_in, int w_in, int c, const std::vector<int32_t> &mask_host) {
     const pt_params &p = t->build;
     hipStream_t s = t->stream;
@@ -572,8 +588,10 @@ int build_table(pt_table *t, const float *b_dev, int h_in, int w_in, int c, cons
     const int32_t *mask_dev = mask_host.empty() ? nullptr : t->allowed.p;
     // 2-4. quantiser, populate, DT
     TRY(build_grid(t, t->desc.p, mask_host, mask_dev, 0, s, t->ev));
-    // 5. k-coherence (patchtable.h:2653-2987)
-    t->K = p.kcoherence;
+    // 5. k-coherence (patchtable.h:2653-2987); the MFMA lookup walks no lists and takes the int8
+    // copy of b instead
+    t->K = p.mfma_rounds > 0 && !t->desc_mode ? 0 : p.kcoherence;
+    if (p.mfma_rounds > 0 && !t->desc_mode) TRY(prepare_mfma(t, s));
     if (t->K > 0) {
         const size_t n = (size_t)(w - 7) * (h - 7);
         TRY(t->kpos.ensure(n * t->K));
@@ -716,9 +734,24 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     if (p.prop_dist != 0) return pt_fail(PT_E_UNSUPPORTED, "prop_dist != 0 is not supported");
     if (p.prop_iters < 1) return pt_fail(PT_E_UNSUPPORTED, "prop_iters must be >= 1");
     if (p.coherence_spatial != 0.0) return pt_fail(PT_E_UNSUPPORTED, "coherence_spatial != 0 is not supported");
-    if (p.kcoherence > t->K) return pt_fail(PT_E_INVALID, "kcoherence exceeds the table's k-coherence lists");
+    if (p.kcoherence > t->K && p.mfma_rounds <= 0)
+        return pt_fail(PT_E_INVALID, "kcoherence exceeds the table's k-coherence lists");
     if (p.spatial < 0) return pt_fail(PT_E_INVALID, "spatial must be >= 0");
     if (ws && t->keyed) return pt_fail(PT_E_UNSUPPORTED, "learned-key tables look up in their own buffers");
+    const bool mfma = p.mfma_rounds > 0;
+    if (mfma) {
+        if (t->desc_mode) return pt_fail(PT_E_UNSUPPORTED, "the MFMA lookup scores RGB patches (is_descriptor 0)");
+        if (p.query_step != 1) return pt_fail(PT_E_UNSUPPORTED, "the MFMA lookup needs query_step 1");
+        if (t->allowed.p) return pt_fail(PT_E_UNSUPPORTED, "the MFMA lookup takes no allowed-patch mask");
+        if (t->w > 32768 || w > 32768)
+            return pt_fail(PT_E_UNSUPPORTED, "the MFMA lookup packs offsets in 16 bits: widths up to 32768");
+        if (p.mfma_anchors < 1 || p.mfma_anchors > PT_MFMA_MAX_ANCHORS)
+            return pt_failf(PT_E_INVALID, "mfma_anchors must be in 1..%d", PT_MFMA_MAX_ANCHORS);
+        if (!t->bq.p) {
+            if (ws) return pt_fail(PT_E_INVALID, "the table has no int8 copy of b: build it with mfma_rounds > 0");
+            TRY(prepare_mfma(t, t->stream));
+        }
+    }
     hipStream_t s = t->stream;
     const int Hp = ah + 7, Wp = aw + 7;
     const size_t plane = (size_t)Hp * Wp;
@@ -754,6 +787,8 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
             TRY(t->id.ensure(n));
             TRY(t->bad.ensure(1));
         }
+        if (mfma) TRY(t->aq.ensure((size_t)ah * aw));
+        B.aq = t->aq.p;
         B.adesc = t->adesc.p;
         B.adescI = t->adescI.p;
         B.yuv = t->yuv.p;
@@ -841,6 +876,16 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
         L.allowed = t->allowed.p;
         L.allowed_index = p.allowed_index;
     }
+    if (mfma) {   // candidates: the cells of every table only; then the MFMA rounds
+        L.K = 0;
+        L.kcoh_iter0 = false;
+        L.kcoh_rounds = false;
+        L.spatial_cand = false;
+        L.rs = false;
+        L.rounds = 0;
+        L.hist = nullptr;
+        L.hn = nullptr;
+    }
     const double temporal_scale = 1.0 / (1 + p.coherence_temporal);
     if (prev_dev) {
         HIPTRY(hipMemsetAsync(B.bad, 0, sizeof(int), s));
@@ -858,9 +903,33 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     f1.d = B.nd2;
     f1.fl = B.nfl2;
     int launches = 0;
-    TRY(pt_lookup_launch(L, f0, f1, B.gxy, B.gd, ann_dev, prev_dev, B.id, temporal_scale,
-                         prev_dev && p.coherence_temporal != 0.0, s, timing ? t->ev[8] : nullptr,
-                         timing ? t->ev[10] : nullptr, &launches));
+    const int tfix = prev_dev && p.coherence_temporal != 0.0;
+    if (mfma) {
+        TRY(pt_quant_rgb0_launch(a_dev, (long)ah * aw, B.aq, s));
+        TRY(pt_lookup_candidates(L, f0, s));
+        if (timing) HIPTRY(hipEventRecord(t->ev[8], s));
+        MfmaArgs M;
+        M.a = a_dev;
+        M.b = t->b0.p;
+        M.aq = B.aq;
+        M.bq = t->bq.p;
+        M.bnorm = t->bnorm.p;
+        M.ah = ah;
+        M.aw = aw;
+        M.bh = t->h;
+        M.bw = t->w;
+        M.ann_w = ann_w;
+        M.ann_h = ann_h;
+        M.nanchors = p.mfma_anchors;
+        int which = 0;
+        TRY(pt_mfma_rounds_launch(M, p.mfma_rounds, f0, f1, s, &which));
+        launches = p.mfma_rounds;
+        TRY(pt_lookup_output(L, which ? f1 : f0, B.gxy, B.gd, ann_dev, prev_dev, B.id, temporal_scale, tfix, s,
+                             &launches, timing ? t->ev[10] : nullptr));
+    } else {
+        TRY(pt_lookup_launch(L, f0, f1, B.gxy, B.gd, ann_dev, prev_dev, B.id, temporal_scale, tfix, s,
+                             timing ? t->ev[8] : nullptr, timing ? t->ev[10] : nullptr, &launches));
+    }
     if (timing) HIPTRY(hipEventRecord(t->ev[9], s));
     if (launches_out) *launches_out = launches;
     return PT_OK;
@@ -945,6 +1014,7 @@ int pt_lookup_workspace_layout(const pt_table *t, int h, int w, int c, bool prev
         B.id = (float *)take(n * 4);
         B.bad = (int *)take(sizeof(int));
     }
+    if (p.mfma_rounds > 0) B.aq = (uint32_t *)take((size_t)ah * aw * 4);
     if (bytes) *bytes = off;
     if (bufs) *bufs = B;
     return PT_OK;
@@ -1027,7 +1097,8 @@ int build_keyed(pt_table *t, const pt_key *key, const float *b_dev, int h, int w
     TRY(start_shuffle(t, std::vector<int32_t>()));
     const std::vector<int32_t> no_mask;
     TRY(build_grid(t, t->desc.p, no_mask, nullptr, 0, s, t->ev));
-    t->K = p.kcoherence;
+    t->K = p.mfma_rounds > 0 ? 0 : p.kcoherence;   // the MFMA lookup walks no lists
+    if (p.mfma_rounds > 0) TRY(prepare_mfma(t, s));
     if (t->K > 0) {
         const size_t n = (size_t)(w - 7) * (h - 7);
         TRY(t->kpos.ensure(n * t->K));
