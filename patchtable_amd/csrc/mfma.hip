@@ -27,8 +27,7 @@
 // The oracle (lookup_parallel with mfma_rounds > 0) restates this bit for bit: the integer
 // scores do not depend on summation order.
 //
-// Work layout: a workgroup of 4 waves owns a 64 x 4 tile of query patches; each wave takes one
-// row, its 4 groups in turn. A wave stages each anchor's band of b (39 x 10 pixels, int8 RGB0)
+// Work layout: a workgroup of one wave owns 64 query patches of a row, its 4 groups in turn. A wave stages each anchor's band of b (39 x 10 pixels, int8 RGB0)
 // in its own LDS region; each lane loads the B fragments of one candidate column for the 9 row
 // pairs the band's 3 rows of candidates use, and the 3 x 4 MFMAs of a 16-candidate block read
 // them from registers. The float query tile staged for the rescoring is the lookup's own.
@@ -46,6 +45,8 @@ constexpr int R = PT_MFMA_BAND;        // rows of candidates above and below the
 constexpr int BR = 8 + 2 * R;          // band rows
 constexpr int BC = 32 + 7;             // band columns (32 candidates of 8 pixels)
 constexpr uint64_t NONE = ~0ull;
+// No packed offset equals this: both halves would be -32768, and |offset| <= 32767.
+constexpr int HT_EMPTY = (int)0x80008000u;
 
 __device__ __forceinline__ uint32_t quant_pixel(const float *p) {
     uint32_t u = 0;
@@ -142,53 +143,69 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 constexpr int NB = 32 * (2 * R + 1);               // candidate norms of one band
-constexpr int PER = (BR * BC + NB + 63) / 64;      // band words per lane
+constexpr int BANDW = BR * BC + NB;                // words of one band buffer
 constexpr int MAXITEMS = 4 * PT_MFMA_MAX_ANCHORS;  // (group, anchor) items of one wave
 
-// Loads the band of item (x0, ox, oy) into registers: band pixels (u0 + c, v0 + r), then the norms
-// of its 32 x (2R+1) candidates, from the padded copies. The band origin lies inside the margins
-// (see PT_MFMA_PAD); it is clamped to them anyway, so a load can never leave the arrays.
-__device__ __forceinline__ void band_fetch(const MfmaArgs &M, int lane, int x0, int y, int ox, int oy,
-                                           uint32_t (&reg)[PER]) {
+constexpr int PER = (BANDW + 63) / 64;   // band words per lane
+
+// Per-lane offsets of the band words this lane loads: pixel (r, c) of the band at r * bwp + c in
+// the padded copy of b, or norm (dy, j) at dy * bnp + j in the padded norm table (-1: none).
+// They do not depend on the item, so they are computed once.
+__device__ __forceinline__ void band_offsets(const MfmaArgs &M, int lane, int (&off)[PER]) {
     const int bwp = M.bw + 2 * P, bnp = M.bw - 7 + 2 * P;
-    const int u0 = min(max(x0 + ox - 8, -P), M.bw + P - BC), v0 = min(max(y + oy - R, -P), M.bh + P - BR);
-    const uint32_t *bq = M.bq + (size_t)(v0 + P) * bwp + (u0 + P);
-    const int32_t *bn = M.bnorm + (size_t)(v0 + P) * bnp + (u0 + P);
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int e = lane + 64 * k;
         if (e < BR * BC) {
             const int r = (e * 1681) >> 16, c = e - r * BC;   // e / 39 for e < 1000
-            reg[k] = bq[r * bwp + c];
-        } else if (e < BR * BC + NB) {
+            off[k] = r * bwp + c;
+        } else if (e < BANDW) {
             const int n = e - BR * BC;
-            reg[k] = (uint32_t)bn[(n >> 5) * bnp + (n & 31)];
+            off[k] = (n >> 5) * bnp + (n & 31);
         } else {
-            reg[k] = 0u;
+            off[k] = -1;
         }
     }
 }
 
-__global__ __launch_bounds__(THREADS, 4) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J) {
-    __shared__ float4 qtile[AX * AY];           // float query pixels (exact rescoring)
-    __shared__ uint32_t qq[AX * AY];            // int8 RGB0 query pixels (A fragments)
-    __shared__ uint32_t band[THREADS / 64][BR * BC + NB];
-    __shared__ int items[THREADS / 64][MAXITEMS][3];
-    __shared__ int rq_n, rq_q[THREADS], rq_c[THREADS], rq_v[THREADS];   // rescoring queue
-    __shared__ float rq_d[THREADS];
-    if (threadIdx.x == 0) rq_n = 0;   // ordered before use by stage_tile's barrier
+// Loads the band of item (x0, ox, oy) into registers from the padded copies. The band origin lies
+// inside the margins (see PT_MFMA_PAD); it is clamped to them anyway, so a load can never leave
+// the arrays.
+__device__ __forceinline__ void band_fetch(const MfmaArgs &M, int lane, int x0, int y, int ox, int oy,
+                                           const int (&off)[PER], uint32_t (&reg)[PER]) {
+    const int bwp = M.bw + 2 * P, bnp = M.bw - 7 + 2 * P;
+    const int u0 = min(max(x0 + ox - 8, -P), M.bw + P - BC), v0 = min(max(y + oy - R, -P), M.bh + P - BR);
+    const uint32_t *bq = M.bq + (size_t)(v0 + P) * bwp + (u0 + P);
+    const uint32_t *bn = (const uint32_t *)M.bnorm + (size_t)(v0 + P) * bnp + (u0 + P);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int e = lane + 64 * k;   // e is known per k: the branch folds where all lanes agree
+        if (64 * k + 63 < BR * BC || (64 * k < BR * BC && e < BR * BC)) reg[k] = bq[off[k]];
+        else if (e < BANDW) reg[k] = bn[off[k]];
+        else reg[k] = 0u;
+    }
+}
+
+// One wave per workgroup: the workgroup owns 64 x 1 query patches, so a wave leaves as soon as its
+// own items are done (its row's item count varies with the image).
+constexpr int MW = 64, MQY = 1 + 7;   // threads; query rows staged (the row's patches)
+
+__global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J) {
+    __shared__ uint32_t qq[AX * MQY];   // int8 RGB0 query pixels (A fragments)
+    // the hash table of phase 1, then the two band buffers of phase 2
+    __shared__ uint32_t wsh[1][2 * BANDW];
+    __shared__ int items[1][MAXITEMS][3];
     const int2 tc = tile_coords();
-    const int x00 = tc.x * TILE_X, y00 = tc.y * TILE_Y;
-    for (int i = threadIdx.x; i < AX * AY; i += THREADS) {
+    const int x00 = tc.x * TILE_X, y00 = tc.y;
+    for (int i = threadIdx.x; i < AX * MQY; i += MW) {
         const int ly = i / AX, lx = i - ly * AX, gx = x00 + lx, gy = y00 + ly;
         qq[i] = gx < M.aw && gy < M.ah ? M.aq[(size_t)gy * M.aw + gx] : 0u;
     }
-    stage_tile(M.a, M.ah, M.aw, qtile);   // every thread; synchronises the block
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    const int wave = 0, lane = threadIdx.x;
     const int g = lane >> 4, jj = lane & 15;
     const int y = y00 + wave, xw0 = x00;
     const int aw = M.ann_w, ah = M.ann_h;
-    uint32_t *bnd = band[wave];
     uint64_t mykey = NONE;   // best candidate of query (xw0 + lane, y)
     int nitems = 0;
     if (y < ah) {
@@ -213,21 +230,31 @@ __global__ __launch_bounds__(THREADS, 4) void k_mfma_round(MfmaArgs M, NnfField 
             const bool v = sv[gi];
             const int ox = PT_INT_TO_X(sxy[gi]) - sx[gi], oy = PT_INT_TO_Y(sxy[gi]) - sy[gi];
             const int po = v ? (int)(((uint32_t)ox & 0xffffu) | ((uint32_t)oy << 16)) : 0;
-            // count and first slot of this lane's offset, one distinct offset per pass
-            int cnt = 0;
-            bool rep = false;
-            unsigned long long todo = __ballot(v);
-            while (todo) {
-                const int s0 = __builtin_ctzll(todo);
-                const int pv = __builtin_amdgcn_readlane(po, s0);
-                const bool eq = v && po == pv;
-                const unsigned long long m = __ballot(eq);
-                todo &= ~m;
-                if (eq) {
-                    cnt = __popcll(m);
-                    rep = lane == s0;   // first occurrence of its offset
+            // count and first slot of this lane's offset: a hash table of the wave's 64 offsets
+            // in LDS (open addressing, 128 slots, so probing ends); counts and first slots do not
+            // depend on the order in which the atomics land
+            int *hk = (int *)wsh[wave], *hc = hk + 128, *hf = hk + 256;
+            hk[lane] = HT_EMPTY;
+            hk[lane + 64] = HT_EMPTY;
+            hc[lane] = 0;
+            hc[lane + 64] = 0;
+            hf[lane] = 64;
+            hf[lane + 64] = 64;
+            wave_lds_sync();
+            int h = (int)(((uint32_t)po * 2654435761u) >> 25);
+            if (v) {
+                for (;;) {
+                    const int prev = atomicCAS(hk + h, HT_EMPTY, po);
+                    if (prev == HT_EMPTY || prev == po) break;
+                    h = (h + 1) & 127;
                 }
+                atomicAdd(hc + h, 1);
+                atomicMin(hf + h, lane);
             }
+            wave_lds_sync();
+            const int cnt = v ? hc[h] : 0;
+            const bool rep = v && hf[h] == lane;   // first occurrence of its offset
+            wave_lds_sync();   // the reads are done before the next group clears the table
             bool covered = false;
             for (int a = 0; a < M.nanchors; a++) {
                 // the uncovered offset of largest count, ties to the first slot (keys are unique)
@@ -261,22 +288,39 @@ __global__ __launch_bounds__(THREADS, 4) void k_mfma_round(MfmaArgs M, NnfField 
     // reduced as (key, jj), i.e. (SSD, anchor, dy, j) with j = 16 block + jj.
     constexpr uint32_t OFF = 192u * 128u * 128u;
     constexpr uint32_t INV = 3200000000u, INV_LO = 2400000000u;
+    // item fields are wave-uniform: read them into scalar registers
+    auto item = [&](int i, int f) { return __builtin_amdgcn_readfirstlane(items[wave][i][f]); };
+    int off[PER];
+    band_offsets(M, lane, off);
+    // Software pipeline: at item it the registers hold the band of item it + 1 (loaded during item
+    // it - 1); they go to the LDS buffer item it - 1 read, the loads of item it + 2 start, and item
+    // it is scored from the buffer written during item it - 1.
     uint32_t reg[PER];
-    if (nitems > 0) band_fetch(M, lane, xw0 + 16 * items[wave][0][0], y, items[wave][0][1], items[wave][0][2], reg);
+    uint32_t *bufs = wsh[wave];
+    if (nitems > 0) {
+        band_fetch(M, lane, xw0 + 16 * item(0, 0), y, item(0, 1), item(0, 2), off, reg);
+        wave_lds_sync();   // phase 1's reads of the hash table are done
+#pragma unroll
+        for (int k = 0; k < PER; k++)
+            if (lane + 64 * k < BANDW) bufs[lane + 64 * k] = reg[k];
+        if (nitems > 1) band_fetch(M, lane, xw0 + 16 * item(1, 0), y, item(1, 1), item(1, 2), off, reg);
+    }
     v4i A[4];
     uint32_t best[4] = {~0u, ~0u, ~0u, ~0u};
     int cur_g = -1, ga = 0, gfirst = 0;   // group of the current items, anchor index in it, its first item
     const int bew = M.bw - 7, beh = M.bh - 7;
     for (int it = 0; it < nitems; it++) {
-        const int gi = items[wave][it][0], ax = items[wave][it][1], ay = items[wave][it][2];
+        const int gi = item(it, 0), ax = item(it, 1), ay = item(it, 2);
         const int x0 = xw0 + 16 * gi;
-        wave_lds_sync();   // the previous item's reads are done
+        wave_lds_sync();   // item it - 1's reads are done, item it's band is written
+        if (it + 1 < nitems) {
+            uint32_t *nb1 = bufs + ((it + 1) & 1) * BANDW;
 #pragma unroll
-        for (int k = 0; k < PER; k++)
-            if (lane + 64 * k < BR * BC + NB) bnd[lane + 64 * k] = reg[k];
-        wave_lds_sync();
-        if (it + 1 < nitems)
-            band_fetch(M, lane, xw0 + 16 * items[wave][it + 1][0], y, items[wave][it + 1][1], items[wave][it + 1][2], reg);
+            for (int k = 0; k < PER; k++)
+                if (lane + 64 * k < BANDW) nb1[lane + 64 * k] = reg[k];
+            if (it + 2 < nitems) band_fetch(M, lane, xw0 + 16 * item(it + 2, 0), y, item(it + 2, 1), item(it + 2, 2), off, reg);
+        }
+        const uint32_t *bnd = bufs + (it & 1) * BANDW;
         if (gi != cur_g) {
             cur_g = gi;
             ga = 0;
@@ -317,7 +361,7 @@ __global__ __launch_bounds__(THREADS, 4) void k_mfma_round(MfmaArgs M, NnfField 
             }
         }
         ga++;
-        if (it + 1 == nitems || items[wave][it + 1][0] != gi) {
+        if (it + 1 == nitems || item(it + 1, 0) != gi) {
             // best (key, jj) of member 4g + t over the group's 16 candidate lanes
             uint64_t k64[4];
 #pragma unroll
@@ -332,61 +376,39 @@ __global__ __launch_bounds__(THREADS, 4) void k_mfma_round(MfmaArgs M, NnfField 
             if (g == gi && key < INV_LO) {
                 const int ord = (int)(key & 127u), j = 16 * (ord & 1) + (int)(k & 15u), dyi = (ord >> 1) % (2 * R + 1),
                           an = (ord >> 1) / (2 * R + 1);
-                const int oxa = items[wave][gfirst + an][1], oya = items[wave][gfirst + an][2];
-                mykey = (uint64_t)(uint32_t)PT_XY_TO_INT(x0 + oxa - 8 + j, y + oya - R + dyi);
+                if (gfirst + an < nitems) {
+                    const int oxa = items[wave][gfirst + an][1], oya = items[wave][gfirst + an][2];   // per lane
+                    const int xs = x0 + oxa - 8 + j, ys = y + oya - R + dyi;
+                    // a key below INV_LO is a candidate inside b; the check keeps the rescoring's
+                    // reads inside b whatever the key holds
+                    if (xs >= 0 && ys >= 0 && xs < bew && ys < beh) mykey = (uint64_t)(uint32_t)PT_XY_TO_INT(xs, ys);
+                }
             }
 #pragma unroll
             for (int t = 0; t < 4; t++) best[t] = ~0u;
         }
     }
-    // ---- phase 3: exact rescoring. Queries whose best candidate differs from their match go to a
-    // block queue in LDS and are rescored densely (the first threads of the block take one entry
-    // each), so a round with few changes costs one wave of SSDs instead of four.
+    // ---- phase 3: exact rescoring of each query's best candidate when it differs from its match
     const int x = xw0 + lane;
-    const bool qv = y < ah && x < aw;
+    if (y >= ah || x >= aw) return;
     const long vi = (long)y * aw + x;
-    int v = 0;
-    float d = 0.f;
-    int cand = 0;
-    bool push = false;
-    if (qv) {
-        v = cur.xy[vi];
-        d = cur.d[vi];
-        if (mykey != NONE) {
-            cand = (int)(uint32_t)mykey;
-            push = PT_INT_TO_X(cand) != PT_INT_TO_X(v) || PT_INT_TO_Y(cand) != PT_INT_TO_Y(v);
-        }
-        if (!push) {
-            nxt.xy[vi] = v;
-            nxt.d[vi] = d;
-            nxt.fl[vi] = 0;
-        }
-    }
-    const unsigned long long pm = __ballot(push);
-    int base = 0;
-    if (lane == 0 && pm) base = atomicAdd(&rq_n, __popcll(pm));
-    base = __shfl(base, 0);
-    if (push) {
-        const int slot = base + __popcll(pm & ((1ull << lane) - 1ull));
-        rq_q[slot] = threadIdx.x;
-        rq_c[slot] = cand;
-        rq_v[slot] = v;
-        rq_d[slot] = d;
-    }
-    __syncthreads();
-    const int n = rq_n;
-    if ((int)threadIdx.x < n) {
-        const int k = threadIdx.x, q = rq_q[k], c = rq_c[k];
-        const int qw = q >> 6, ql = q & 63;
-        const float dq = rq_d[k];
-        const long qi = (long)(y00 + qw) * aw + (xw0 + ql);
+    int v = cur.xy[vi];
+    float d = cur.d[vi];
+    if (mykey != NONE) {
+        const int c = (int)(uint32_t)mykey;
         const int xs = PT_INT_TO_X(c), ys = PT_INT_TO_Y(c);
-        const float dn = ssd_staged<true>(qtile + qw * AX + ql, M.b + ((size_t)ys * M.bw + xs) * 3, M.bw * 3, dq);
-        const bool take = dn < dq;
-        nxt.xy[qi] = take ? c : rq_v[k];
-        nxt.d[qi] = take ? dn : dq;
-        nxt.fl[qi] = 0;
+        if (xs != PT_INT_TO_X(v) || ys != PT_INT_TO_Y(v)) {
+            const float dn = pt_patch_ssd_bounded(M.a + ((size_t)y * M.aw + x) * 3, M.aw * 3,
+                                                  M.b + ((size_t)ys * M.bw + xs) * 3, M.bw * 3, d);
+            if (dn < d) {
+                d = dn;
+                v = c;
+            }
+        }
     }
+    nxt.xy[vi] = v;
+    nxt.d[vi] = d;
+    nxt.fl[vi] = 0;
 }
 
 }  // namespace
@@ -409,9 +431,9 @@ int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, int32_t *
 int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f1, hipStream_t s, int *which) {
     if (M.nanchors < 1 || M.nanchors > PT_MFMA_MAX_ANCHORS)
         return pt_failf(PT_E_INVALID, "mfma_anchors must be in 1..%d", PT_MFMA_MAX_ANCHORS);
-    dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, (M.ann_h + TILE_Y - 1) / TILE_Y);
+    dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, M.ann_h);
     for (int r = 0; r < rounds; r++) {
-        hipLaunchKernelGGL(k_mfma_round, grid, dim3(THREADS), 0, s, M, f0, f1, pt_mfma_jump_at(rounds, r));
+        hipLaunchKernelGGL(k_mfma_round, grid, dim3(MW), 0, s, M, f0, f1, pt_mfma_jump_at(rounds, r));
         std::swap(f0, f1);
     }
     PT_LAUNCH_CHECK();
