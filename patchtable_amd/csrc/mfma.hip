@@ -147,6 +147,10 @@ constexpr int BANDW = BR * BC + NB;                // words of one band buffer
 constexpr int MAXITEMS = 4 * PT_MFMA_MAX_ANCHORS;  // (group, anchor) items of one wave
 
 constexpr int PER = (BANDW + 63) / 64;   // band words per lane
+constexpr int HS = 96;                   // hash slots per group (64 sources)
+// phase 1 needs 4 groups x (3 HS table words + 64 source offsets) + the anchors
+constexpr int WSH = 12 * HS + 256 + 8 * PT_MFMA_MAX_ANCHORS > 2 * BANDW ? 12 * HS + 256 + 8 * PT_MFMA_MAX_ANCHORS
+                                                                        : 2 * BANDW;
 
 // Per-lane offsets of the band words this lane loads: pixel (r, c) of the band at r * bwp + c in
 // the padded copy of b, or norm (dy, j) at dy * bnp + j in the padded norm table (-1: none).
@@ -192,9 +196,10 @@ constexpr int MW = 64, MQY = 1 + 7;   // threads; query rows staged (the row's p
 
 __global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J) {
     __shared__ uint32_t qq[AX * MQY];   // int8 RGB0 query pixels (A fragments)
-    // the hash table of phase 1, then the two band buffers of phase 2
-    __shared__ uint32_t wsh[1][2 * BANDW];
+    // the hash tables, source offsets and anchors of phase 1, then the two band buffers of phase 2
+    __shared__ uint32_t wsh[1][WSH];
     __shared__ int items[1][MAXITEMS][3];
+    __shared__ uint32_t fin[16 * 2];      // (key, jj) of the 16 members at the end of a group
     const int2 tc = tile_coords();
     const int x00 = tc.x * TILE_X, y00 = tc.y;
     for (int i = threadIdx.x; i < AX * MQY; i += MW) {
@@ -209,74 +214,91 @@ __global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, 
     uint64_t mykey = NONE;   // best candidate of query (xw0 + lane, y)
     int nitems = 0;
     if (y < ah) {
-        // ---- phase 1: sources, distinct offsets and anchors of the wave's 4 groups. An offset is
-        // packed into one int (x in the low 16 bits): widths and heights are at most 32768.
-        int sxy[4], sx[4], sy[4];
+        // ---- phase 1: sources, distinct offsets and anchors of the wave's 4 groups, all four at
+        // once: row gq = lane >> 4 of the wave serves group gq, each lane 4 of its 64 source slots
+        // (slot 16 r + c in register r, c = lane & 15). An offset is packed into one int (x in
+        // the low 16 bits): widths and heights are at most 32768.
+        int *hk = (int *)wsh[wave], *hc = hk + 4 * HS, *hf = hk + 8 * HS, *pos = hk + 12 * HS, *itmp = pos + 256;
+        const int gq = lane >> 4, c = lane & 15, x0 = xw0 + 16 * gq;
+        int sxy[4], sx[4], sy[4], ox[4], oy[4], h[4];
         bool sv[4];
 #pragma unroll
-        for (int gi = 0; gi < 4; gi++) {
-            const int x0 = xw0 + 16 * gi;
-            if (lane < 16) sx[gi] = x0 + lane, sy[gi] = y;
-            else if (lane < 32) sx[gi] = x0 + lane - 16, sy[gi] = y - J;
-            else if (lane < 48) sx[gi] = x0 + lane - 32, sy[gi] = y + J;
-            else if (lane < 56) sx[gi] = x0 + (lane - 48) - J, sy[gi] = y;
-            else sx[gi] = x0 + (lane - 56) + 8 + J, sy[gi] = y;
-            sv[gi] = x0 < aw && sx[gi] >= 0 && sy[gi] >= 0 && sx[gi] < aw && sy[gi] < ah;
-            sxy[gi] = sv[gi] ? cur.xy[(long)sy[gi] * aw + sx[gi]] : 0;   // all four loads in flight
+        for (int r = 0; r < 4; r++) {
+            // slots 0-15 members, 16-31 J rows above, 32-47 J rows below, 48-55 J to the left of
+            // members 0-7, 56-63 J to the right of members 8-15
+            sx[r] = r < 3 ? x0 + c : (c < 8 ? x0 + c - J : x0 + c + J);
+            sy[r] = r == 1 ? y - J : (r == 2 ? y + J : y);
+            sv[r] = x0 < aw && sx[r] >= 0 && sy[r] >= 0 && sx[r] < aw && sy[r] < ah;
+            sxy[r] = sv[r] ? cur.xy[(long)sy[r] * aw + sx[r]] : 0;   // all four loads in flight
         }
-        for (int gi = 0; gi < 4; gi++) {
-            const int x0 = xw0 + 16 * gi;
-            if (x0 >= aw) break;
-            const bool v = sv[gi];
-            const int ox = PT_INT_TO_X(sxy[gi]) - sx[gi], oy = PT_INT_TO_Y(sxy[gi]) - sy[gi];
-            const int po = v ? (int)(((uint32_t)ox & 0xffffu) | ((uint32_t)oy << 16)) : 0;
-            // count and first slot of this lane's offset: a hash table of the wave's 64 offsets
-            // in LDS (open addressing, 128 slots, so probing ends); counts and first slots do not
-            // depend on the order in which the atomics land
-            int *hk = (int *)wsh[wave], *hc = hk + 128, *hf = hk + 256;
-            hk[lane] = HT_EMPTY;
-            hk[lane + 64] = HT_EMPTY;
-            hc[lane] = 0;
-            hc[lane + 64] = 0;
-            hf[lane] = 64;
-            hf[lane + 64] = 64;
-            wave_lds_sync();
-            int h = (int)(((uint32_t)po * 2654435761u) >> 25);
-            if (v) {
+        for (int i = lane; i < 4 * HS; i += 64) {
+            hk[i] = HT_EMPTY;
+            hc[i] = 0;
+            hf[i] = 64;
+        }
+        wave_lds_sync();
+        // count and first slot of each source's offset: one hash table per group in LDS (open
+        // addressing, HS > 64 slots, so probing ends); counts and first slots do not depend on
+        // the order in which the atomics land
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            ox[r] = PT_INT_TO_X(sxy[r]) - sx[r];
+            oy[r] = PT_INT_TO_Y(sxy[r]) - sy[r];
+            const int po = sv[r] ? (int)(((uint32_t)ox[r] & 0xffffu) | ((uint32_t)oy[r] << 16)) : 0;
+            pos[64 * gq + 16 * r + c] = po;
+            int hh = gq * HS + (int)__umulhi((uint32_t)po * 2654435761u, (uint32_t)HS);
+            if (sv[r]) {
                 for (;;) {
-                    const int prev = atomicCAS(hk + h, HT_EMPTY, po);
+                    const int prev = atomicCAS(hk + hh, HT_EMPTY, po);
                     if (prev == HT_EMPTY || prev == po) break;
-                    h = (h + 1) & 127;
+                    hh = hh + 1 == (gq + 1) * HS ? gq * HS : hh + 1;
                 }
-                atomicAdd(hc + h, 1);
-                atomicMin(hf + h, lane);
+                atomicAdd(hc + hh, 1);
+                atomicMin(hf + hh, 16 * r + c);
             }
-            wave_lds_sync();
-            const int cnt = v ? hc[h] : 0;
-            const bool rep = v && hf[h] == lane;   // first occurrence of its offset
-            wave_lds_sync();   // the reads are done before the next group clears the table
-            bool covered = false;
-            for (int a = 0; a < M.nanchors; a++) {
-                // the uncovered offset of largest count, ties to the first slot (keys are unique)
-                const uint32_t key = rep && !covered ? (uint32_t)((cnt << 6) | (63 - lane)) + 1u : 0u;
-                const uint32_t r = row16_max(key);
-                const uint32_t kmax = max(max((uint32_t)__builtin_amdgcn_readlane((int)r, 0),
-                                              (uint32_t)__builtin_amdgcn_readlane((int)r, 16)),
-                                          max((uint32_t)__builtin_amdgcn_readlane((int)r, 32),
-                                              (uint32_t)__builtin_amdgcn_readlane((int)r, 48)));
-                if (kmax == 0u) break;
-                const int src = 63 - (int)((kmax - 1u) & 63u);
-                const int pa = __builtin_amdgcn_readlane(po, src);
+            h[r] = hh;
+        }
+        wave_lds_sync();
+        // anchor key of a source that is its offset's first occurrence: (count, -slot) + 1
+        uint32_t kb[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int slot = 16 * r + c;
+            kb[r] = sv[r] && hf[h[r]] == slot ? (uint32_t)((hc[h[r]] << 6) | (63 - slot)) + 1u : 0u;
+        }
+        bool cov[4] = {false, false, false, false};
+        int na = 0;   // anchors of group gq so far (the same in all lanes of the row)
+        for (int a = 0; a < M.nanchors; a++) {
+            // per group, the uncovered offset of largest count, ties to the first slot (keys are unique)
+            uint32_t k = 0u;
+#pragma unroll
+            for (int r = 0; r < 4; r++) k = max(k, cov[r] ? 0u : kb[r]);
+            k = row16_max(k);
+            if (!__any(k != 0u)) break;
+            if (k != 0u) {
+                const int pa = pos[64 * gq + 63 - (int)((k - 1u) & 63u)];
                 const int ax = (int)(int16_t)(pa & 0xffff), ay = pa >> 16;
-                covered = covered || (v && abs(ox - ax) <= 8 && abs(oy - ay) <= R);
-                if (lane == 0) {
-                    items[wave][nitems][0] = gi;
-                    items[wave][nitems][1] = ax;
-                    items[wave][nitems][2] = ay;
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    cov[r] = cov[r] || ((uint32_t)(ox[r] - ax + 8) <= 16u && (uint32_t)(oy[r] - ay + R) <= (uint32_t)(2 * R));
+                if (c == 0) {
+                    itmp[(gq * PT_MFMA_MAX_ANCHORS + a) * 2] = ax;
+                    itmp[(gq * PT_MFMA_MAX_ANCHORS + a) * 2 + 1] = ay;
                 }
-                nitems++;
+                na++;
             }
         }
+        wave_lds_sync();
+        // the item list: group 0's anchors in order, then group 1's, ...
+        const int n0 = __builtin_amdgcn_readlane(na, 0), n1 = __builtin_amdgcn_readlane(na, 16),
+                  n2 = __builtin_amdgcn_readlane(na, 32), n3 = __builtin_amdgcn_readlane(na, 48);
+        const int pre = gq == 0 ? 0 : (gq == 1 ? n0 : (gq == 2 ? n0 + n1 : n0 + n1 + n2));
+        if (c < na) {
+            items[wave][pre + c][0] = gq;
+            items[wave][pre + c][1] = itmp[(gq * PT_MFMA_MAX_ANCHORS + c) * 2];
+            items[wave][pre + c][2] = itmp[(gq * PT_MFMA_MAX_ANCHORS + c) * 2 + 1];
+        }
+        nitems = n0 + n1 + n2 + n3;
     }
     wave_lds_sync();
     // ---- phase 2: the items in order, the next item's band loading during this one's MFMAs.
@@ -290,6 +312,10 @@ __global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, 
     constexpr uint32_t INV = 3200000000u, INV_LO = 2400000000u;
     // item fields are wave-uniform: read them into scalar registers
     auto item = [&](int i, int f) { return __builtin_amdgcn_readfirstlane(items[wave][i][f]); };
+    // -256 in a scalar register the compiler does not see as a constant: a constant factor would
+    // be strength-reduced to a shift and a subtract (two instructions instead of one v_mad_i32_i24)
+    int m256;
+    asm("s_mov_b32 %0, -256" : "=s"(m256));
     int off[PER];
     band_offsets(M, lane, off);
     // Software pipeline: at item it the registers hold the band of item it + 1 (loaded during item
@@ -355,26 +381,31 @@ __global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, 
                 const uint32_t nb = bnd[BR * BC + dyi * 32 + 16 * b + jj];   // read for every lane (no branch)
                 const uint32_t kv = ((nb + OFF) << 7) | (uint32_t)((ga * (2 * R + 1) + dyi) * 2 + b);
                 const uint32_t base = cv ? kv : INV;
-                // C[t]: member 4g + t against candidate column jj; |2 q.p| < 2^23 (24-bit operands)
+                // C[t]: member 4g + t against candidate column jj; |q.p| < 2^22 fits the signed
+                // 24-bit multiply, so base - 256 q.p is one multiply-add
 #pragma unroll
-                for (int t = 0; t < 4; t++) best[t] = min(best[t], base + (uint32_t)__mul24(C[t], -256));
+                for (int t = 0; t < 4; t++) best[t] = min(best[t], base + (uint32_t)__mul24(C[t], m256));
             }
         }
         ga++;
         if (it + 1 == nitems || item(it + 1, 0) != gi) {
-            // best (key, jj) of member 4g + t over the group's 16 candidate lanes
-            uint64_t k64[4];
+            // best (key, jj) of member 4g + t over the group's 16 candidate lanes: the row minimum
+            // of the keys, then the smallest jj holding it; row g's results go through LDS to
+            // member i = 4g + t, lane 16 gi + i
+            wave_lds_sync();   // the previous group's reads of fin are done
 #pragma unroll
-            for (int t = 0; t < 4; t++) k64[t] = row16_min64(((uint64_t)best[t] << 4) | (uint32_t)jj);
-            // member i = jj of lanes 16 gi .. takes it from lane 16 (i >> 2), register i & 3
-            const int srcl = 16 * (jj >> 2);
-            uint64_t v[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++) v[t] = shfl64(k64[t], srcl);
-            const uint64_t k = (jj & 3) == 0 ? v[0] : (jj & 3) == 1 ? v[1] : (jj & 3) == 2 ? v[2] : v[3];
-            const uint32_t key = (uint32_t)(k >> 4);
+            for (int t = 0; t < 4; t++) {
+                const uint32_t m = row16_min(best[t]);
+                const uint32_t jm = row16_min(best[t] == m ? (uint32_t)jj : 16u);
+                if (jj == 0) {
+                    fin[(4 * g + t) * 2] = m;
+                    fin[(4 * g + t) * 2 + 1] = jm;
+                }
+            }
+            wave_lds_sync();
+            const uint32_t key = fin[jj * 2], kj = fin[jj * 2 + 1];
             if (g == gi && key < INV_LO) {
-                const int ord = (int)(key & 127u), j = 16 * (ord & 1) + (int)(k & 15u), dyi = (ord >> 1) % (2 * R + 1),
+                const int ord = (int)(key & 127u), j = 16 * (ord & 1) + (int)kj, dyi = (ord >> 1) % (2 * R + 1),
                           an = (ord >> 1) / (2 * R + 1);
                 if (gfirst + an < nitems) {
                     const int oxa = items[wave][gfirst + an][1], oya = items[wave][gfirst + an][2];   // per lane
