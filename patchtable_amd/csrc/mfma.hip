@@ -419,24 +419,77 @@ __global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, 
             for (int t = 0; t < 4; t++) best[t] = ~0u;
         }
     }
-    // ---- phase 3: exact rescoring of each query's best candidate when it differs from its match
+    // ---- phase 3: exact rescoring of each query's best candidate when it differs from its match.
+    // The SSD's four lane sums L0..L3 (elements e = 4n + k of each row, rows in order) are
+    // independent, so four lanes can compute one rescoring, one sum each, and lane 4i adds them
+    // as ((L0 + L1) + L2) + L3: the same value as one lane computing all four. Few rescorings
+    // in a wave are packed 16 to a pass that way; many are done one per lane, each stopping
+    // early once a partial sum reaches the match's distance (the sums only grow, so stopping
+    // early never changes which of the two is smaller).
     const int x = xw0 + lane;
-    if (y >= ah || x >= aw) return;
+    const bool live = y < ah && x < aw;
     const long vi = (long)y * aw + x;
-    int v = cur.xy[vi];
-    float d = cur.d[vi];
-    if (mykey != NONE) {
-        const int c = (int)(uint32_t)mykey;
-        const int xs = PT_INT_TO_X(c), ys = PT_INT_TO_Y(c);
-        if (xs != PT_INT_TO_X(v) || ys != PT_INT_TO_Y(v)) {
+    int v = 0;
+    float d = 0.f;
+    if (live) {
+        v = cur.xy[vi];
+        d = cur.d[vi];
+    }
+    const int c = (int)(uint32_t)mykey;
+    const bool need = live && mykey != NONE && (PT_INT_TO_X(c) != PT_INT_TO_X(v) || PT_INT_TO_Y(c) != PT_INT_TO_Y(v));
+    const uint64_t nmask = __ballot(need);
+    const int nr = __popcll(nmask);
+    if (nr > 48) {
+        if (need) {
             const float dn = pt_patch_ssd_bounded(M.a + ((size_t)y * M.aw + x) * 3, M.aw * 3,
-                                                  M.b + ((size_t)ys * M.bw + xs) * 3, M.bw * 3, d);
+                                                  M.b + ((size_t)PT_INT_TO_Y(c) * M.bw + PT_INT_TO_X(c)) * 3, M.bw * 3, d);
+            if (dn < d) {
+                d = dn;
+                v = c;
+            }
+        }
+    } else if (nr > 0) {
+        int *tq = (int *)wsh[wave], *tc = tq + 64;
+        float *td = (float *)(tq + 128);
+        const int rank = __popcll(nmask & ((1ull << lane) - 1ull));
+        wave_lds_sync();   // phase 2's reads of the band buffers are done
+        if (need) {
+            tq[rank] = lane;
+            tc[rank] = c;
+        }
+        wave_lds_sync();
+        const int k = lane & 3, a3 = M.aw * 3, b3 = M.bw * 3;
+        for (int t0 = 0; t0 < nr; t0 += 16) {
+            const int ti = t0 + (lane >> 2);
+            float L = 0.f;
+            if (ti < nr) {
+                const int qc = tc[ti];
+                const float *ap = M.a + ((size_t)y * M.aw + xw0 + tq[ti]) * 3 + k;
+                const float *bp = M.b + ((size_t)PT_INT_TO_Y(qc) * M.bw + PT_INT_TO_X(qc)) * 3 + k;
+#pragma unroll
+                for (int dy = 0; dy < 8; dy++)
+#pragma unroll
+                    for (int n = 0; n < 6; n++) {
+                        const float e = ap[dy * a3 + 4 * n] - bp[dy * b3 + 4 * n];
+                        L = L + e * e;
+                    }
+            }
+            // lane 4i + j holds L_j of task i: quad_perm broadcasts of lanes 1, 2, 3 of each quad
+            const float L1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(L), 0x55, 0xF, 0xF, true));
+            const float L2 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(L), 0xAA, 0xF, 0xF, true));
+            const float L3 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(L), 0xFF, 0xF, 0xF, true));
+            if (k == 0 && ti < nr) td[ti] = ((L + L1) + L2) + L3;
+        }
+        wave_lds_sync();
+        if (need) {
+            const float dn = td[rank];
             if (dn < d) {
                 d = dn;
                 v = c;
             }
         }
     }
+    if (!live) return;
     nxt.xy[vi] = v;
     nxt.d[vi] = d;
     nxt.fl[vi] = 0;
