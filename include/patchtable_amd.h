@@ -357,6 +357,14 @@ int pt_table_get_query_descriptor(const pt_table *t, float *wh_out);
  * total, [6] query gck, [7] query candidate kernel, [8] query propagation rounds (all
  * launches), [9] query total, [10] number of propagation launches, [11] unused. */
 int pt_table_stage_ms(const pt_table *t, double *ms /* [12] */);
+/* Instrumentation of the MFMA lookup (mfma_rounds > 0), no counterpart in the reference: with
+ * pt_table_set_mfma_stats(t, 1) every later synchronous lookup (pt_table_lookup*) also counts,
+ * per round, the (group of 16 query patches, anchor) items the round scored, each 16 patches x
+ * 96 candidates; pt_table_mfma_items copies up to n of the last lookup's per-round counts and
+ * sets *rounds to their number (0 when nothing was counted). Counting adds one store per
+ * workgroup and a copy of one int per workgroup per round back to the host. */
+int pt_table_set_mfma_stats(pt_table *t, int on);
+int pt_table_mfma_items(const pt_table *t, int64_t *items, int n, int *rounds);
 
 /* ---- Stand-alone kernels (device pointers, stream 0 of the current device) ----------------
  * Used by the parity tests and the benchmark to exercise one kernel at a time. */

@@ -42,6 +42,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace orc {
 
@@ -56,6 +59,25 @@ static const int PT_MFMA_BAND_ORC = 1;            // candidate rows above/below 
 static inline int XY_TO_INT(int x, int y) { return x | (y << 16); }   // patchtable.h:163
 static inline int INT_TO_X(int v) { return v & 65535; }
 static inline int INT_TO_Y(int v) { return v >> 16; }
+
+// Host threads of the loops the reference runs under OpenMP (its -threads, default 1). Only the
+// OpenMP build of the oracle (liboracle_omp.so, for the CPU baseline's all-core timing) runs
+// more than one; the checker library is built without OpenMP.
+static inline int orc_threads(const pt_params &p) {
+#ifdef _OPENMP
+    return p.threads > 1 ? p.threads : 1;
+#else
+    (void)p;
+    return 1;
+#endif
+}
+static inline int orc_thread_num() {
+#ifdef _OPENMP
+    return omp_get_thread_num();
+#else
+    return 0;
+#endif
+}
 
 // --------------------------------------------------------------------------------------------
 // glibc random(3), TYPE_3 additive feedback generator, as rand() after the implicit srand(1).
@@ -773,7 +795,14 @@ static void build_kcoherence(Table &T) {
         for (long r = 0; r < Na; r++) pos[o][perm[o][r]] = (int32_t)r;
     }
     struct Cand { float d; int32_t q; };
+    // patches are independent: with several threads (OpenMP build) they are split between them,
+    // with the same lists as one thread
+    const int nthr = orc_threads(T.p);
+    (void)nthr;
+#pragma omp parallel num_threads(nthr) if (nthr > 1)
+    {
     std::vector<Cand> cands;
+#pragma omp for schedule(dynamic, 256)
     for (long i = 0; i < N; i++) {
         int px = (int)(i % bew), py = (int)(i / bew);
         if (!ok[i]) {
@@ -824,6 +853,7 @@ static void build_kcoherence(Table &T) {
             T.kpos[(size_t)i * K + k] = s.xy;
             T.kdist[(size_t)i * K + k] = s.ssd;
         }
+    }
     }
 }
 
@@ -1032,6 +1062,12 @@ static void lookup_raster(const Table &T, const float *a_in, int h_in, int w_in,
     const float *b = T.b0.data();
     GlibcRand rng(1);
     for (long k = 0; k < rand_used_total(T); k++) rng.next();
+    // With several threads the rows of a pass run concurrently, as under the reference's
+    // `omp parallel for schedule(dynamic, 16)` (patchtable.h:3617-3618): a row reads matches that
+    // other rows are writing, so the result depends on timing, and each thread draws from its
+    // own copy of the generator. With one thread this is the reference's raster order exactly.
+    const int nthr = orc_threads(p);
+    std::vector<GlibcRand> trng(nthr, rng);
     auto allowed_at = [&](int xs, int ys) { return T.is_allowed(xs, ys, p.allowed_index); };
     auto dist = [&](int x, int y, int xs, int ys) { return lookup_dist(T, Q, x, y, xs, ys); };
     auto ANN = [&](int yy, int xx) -> double * { return &ann[((size_t)yy * ann_w + xx) * 3]; };
@@ -1081,8 +1117,10 @@ static void lookup_raster(const Table &T, const float *a_in, int h_in, int w_in,
         bool reverse = prop_iter % 2;
         bool do_lookup = prop_iter == 0 && p.do_table_lookup;
         bool do_kcoh = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == prop_iter);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthr) if (nthr > 1)
         for (int y0 = 0; y0 < ann_h; y0 += qs) {
             int y = reverse ? (ann_h - 1 - y0) : y0;
+            GlibcRand &rg = nthr > 1 ? trng[orc_thread_num()] : rng;
             std::vector<float> sat(sat_w * sat_w, 0.f);
             for (int x0 = 0; x0 < ann_w; x0 += qs) {
                 int x = reverse ? (ann_w - 1 - x0) : x0;
@@ -1134,7 +1172,7 @@ static void lookup_raster(const Table &T, const float *a_in, int h_in, int w_in,
                             for (int xs = sxmin; xs < sxmax; xs++)
                                 if ((xs != xb0 || ys != yb0) && allowed_at(xs, ys)) consider(xs, ys);
                     } else {
-                        int r = rng.next() & 7;
+                        int r = rg.next() & 7;
                         int xs = xbest + dx_table[r], ys = ybest + dy_table[r];
                         if (xs >= 0 && xs < b0_ew && ys >= 0 && ys < b0_eh && allowed_at(xs, ys)) consider(xs, ys);
                     }
@@ -1222,8 +1260,8 @@ static void lookup_raster(const Table &T, const float *a_in, int h_in, int w_in,
                                     else if (xsrc >= b0_ew) xsrc = b0_ew - 1;
                                     if (use_allowed)
                                         while (!allowed_at(xsrc, ysrc)) {
-                                            xsrc = rng.next() % b0_ew;
-                                            ysrc = rng.next() % b0_eh;
+                                            xsrc = rg.next() % b0_ew;
+                                            ysrc = rg.next() % b0_eh;
                                         }
                                     float d = dist(ax_shift + dx, ay_shift + dy, xsrc, ysrc);
                                     ap[2] = d;
@@ -1326,6 +1364,9 @@ static long g_ssd_count[64];
 // The integer scores are exact, so this restates the matrix-core arithmetic bit for bit.
 static inline int quant8(float v) { return (int)rintf(fminf(fmaxf(v, 0.f), 1.f) * 255.f) - 128; }
 
+// (group, anchor) items of each round of the last lookup_mfma
+static std::vector<long> g_mfma_items;
+
 static int mfma_jump_at(int rounds, int r) {
     const int big = 1 << (rounds >= 5 ? 4 : (rounds >= 1 ? rounds - 1 : 0));
     const int j = big >> (r < 30 ? r : 30);
@@ -1402,6 +1443,7 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
     const int rounds = p.mfma_rounds;
     std::vector<int32_t> nx(N), ny(N);
     fvec nd(N);
+    g_mfma_items.assign(rounds, 0);
     for (int r = 0; r < rounds; r++) {
         const int J = mfma_jump_at(rounds, r);
         for (int y = 0; y < ah; y++)
@@ -1453,6 +1495,7 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
                     const int ox = sox[dfirst[pick]], oy = soy[dfirst[pick]];
                     aox.push_back(ox);
                     aoy.push_back(oy);
+                    g_mfma_items[r]++;
                     for (size_t k = 0; k < dfirst.size(); k++)
                         if (std::abs(sox[dfirst[k]] - ox) <= 8 && std::abs(soy[dfirst[k]] - oy) <= R) covered[k] = 1;
                     for (int dy = -R; dy <= R; dy++)
@@ -2130,6 +2173,12 @@ int orc_dt(int G, const int *nslices, int32_t *table, int algo, int dt_iters) {
 int orc_lookup_raster(void *t, const float *a, int h, int w, int c, double *ann, const double *prev, const float *sv) {
     lookup_raster(*(Table *)t, a, h, w, c, ann, prev, sv);
     return 0;
+}
+
+int orc_mfma_items(long *out, int n) {
+    const int m = (int)g_mfma_items.size();
+    for (int i = 0; i < n && i < m; i++) out[i] = g_mfma_items[i];
+    return m;
 }
 
 int orc_ssd_counts(long *out) {

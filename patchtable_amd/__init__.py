@@ -274,6 +274,18 @@ class PatchTable:
                                              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return pos, dist
 
+    def set_mfma_stats(self, on=True):
+        """Count the MFMA rounds' items in later lookups (instrumentation; see mfma_items)."""
+        check(load().pt_table_set_mfma_stats(self._t, 1 if on else 0))
+
+    def mfma_items(self):
+        """Per round of the last lookup (with set_mfma_stats on): the (16-patch group, anchor)
+        items scored, each 16 query patches x 96 candidates. Empty when nothing was counted."""
+        buf = (ctypes.c_int64 * 64)()
+        n = ctypes.c_int()
+        check(load().pt_table_mfma_items(self._t, buf, 64, ctypes.byref(n)))
+        return [int(buf[i]) for i in range(min(n.value, 64))]
+
     def stage_ms(self):
         out = (ctypes.c_double * 12)()
         check(load().pt_table_stage_ms(self._t, out))

@@ -145,6 +145,8 @@ SIGNATURES = {
     "pt_table_get_kcoherence": (_int, [_vp, _i32p, _f32p]),
     "pt_table_get_query_descriptor": (_int, [_vp, _f32p]),
     "pt_table_stage_ms": (_int, [_vp, _f64p]),
+    "pt_table_set_mfma_stats": (_int, [_vp, _int]),
+    "pt_table_mfma_items": (_int, [_vp, ctypes.POINTER(ctypes.c_int64), _int, ctypes.POINTER(_int)]),
     "pt_gck_dev": (_int, [_vp, _int, _int, _int, _int, _int, _vp]),
     "pt_patch_ssd_dev": (_int, [_vp, _int, _vp, _int, _vp, _vp, _vp, _vp, _int, _vp]),
     "pt_desc_dist_dev": (_int, [_vp, _vp, _int, _int, _vp]),

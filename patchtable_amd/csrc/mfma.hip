@@ -300,6 +300,7 @@ __global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, 
         }
         nitems = n0 + n1 + n2 + n3;
     }
+    if (M.items_out && lane == 0) M.items_out[(long)blockIdx.y * gridDim.x + blockIdx.x] = nitems;
     wave_lds_sync();
     // ---- phase 2: the items in order, the next item's band loading during this one's MFMAs.
     // Each lane scores one candidate column jj against the 4 members 4g + t. Within a lane a
@@ -512,12 +513,17 @@ int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, int32_t *
     return PT_OK;
 }
 
+long pt_mfma_round_workgroups(int ann_w, int ann_h) { return (long)((ann_w + TILE_X - 1) / TILE_X) * ann_h; }
+
 int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f1, hipStream_t s, int *which) {
     if (M.nanchors < 1 || M.nanchors > PT_MFMA_MAX_ANCHORS)
         return pt_failf(PT_E_INVALID, "mfma_anchors must be in 1..%d", PT_MFMA_MAX_ANCHORS);
     dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, M.ann_h);
+    const long nwg = pt_mfma_round_workgroups(M.ann_w, M.ann_h);
     for (int r = 0; r < rounds; r++) {
-        hipLaunchKernelGGL(k_mfma_round, grid, dim3(MW), 0, s, M, f0, f1, pt_mfma_jump_at(rounds, r));
+        MfmaArgs Mr = M;
+        if (M.items_out) Mr.items_out = M.items_out + r * nwg;
+        hipLaunchKernelGGL(k_mfma_round, grid, dim3(MW), 0, s, Mr, f0, f1, pt_mfma_jump_at(rounds, r));
         std::swap(f0, f1);
     }
     PT_LAUNCH_CHECK();

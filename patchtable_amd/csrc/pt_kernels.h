@@ -237,7 +237,12 @@ struct MfmaArgs {
     int ah = 0, aw = 0, bh = 0, bw = 0;             // image sizes
     int ann_w = 0, ann_h = 0;                       // query patches (query_step 1)
     int nanchors = 0;
+    // optional instrumentation: per round, the items (group, anchor) of every workgroup, one entry
+    // per workgroup (round r at items_out + r * workgroups); null: not written
+    int32_t *items_out = nullptr;
 };
+// Workgroups of one MFMA round launch.
+long pt_mfma_round_workgroups(int ann_w, int ann_h);
 // Jump of MFMA round r of `rounds`: 16, 8, 4, 2, 1, then unit jumps (fewer rounds start lower).
 static inline int pt_mfma_jump_at(int rounds, int r) {
     const int big = 1 << (rounds >= 5 ? 4 : (rounds >= 1 ? rounds - 1 : 0));

@@ -85,6 +85,9 @@ struct pt_table {
     DevBuf<float> kdist;
     DevBuf<uint32_t> bq;      // MFMA lookup: int8 RGB0 copy of b0
     DevBuf<int32_t> bnorm;    // ... and |p|^2 of every database patch
+    bool mfma_stats = false;              // count the MFMA rounds' items (instrumentation)
+    DevBuf<int32_t> mfma_items;           // per round, per workgroup
+    std::vector<int64_t> mfma_items_sum;  // per round, of the last synchronous lookup
     int n_pop = 0;          // patches listed for populate (allowed ones)
     // query side
     DevBuf<float> a, adesc, adescI, yuv, temp;
@@ -921,6 +924,11 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
         M.ann_w = ann_w;
         M.ann_h = ann_h;
         M.nanchors = p.mfma_anchors;
+        if (t->mfma_stats && !ws) {
+            const size_t n = (size_t)p.mfma_rounds * pt_mfma_round_workgroups(ann_w, ann_h);
+            TRY(t->mfma_items.ensure(n));
+            M.items_out = t->mfma_items.p;
+        }
         int which = 0;
         TRY(pt_mfma_rounds_launch(M, p.mfma_rounds, f0, f1, s, &which));
         launches = p.mfma_rounds;
@@ -952,6 +960,17 @@ int lookup_common(pt_table *t, const float *a_dev, int h, int w, int c, double *
         if (bad) return pt_fail(PT_E_INVALID, "expected xsrc, ysrc in bounds in prev_nnf mode (sanitize_input 0)");
     }
     if (seconds) *seconds = T1 - T0;
+    t->mfma_items_sum.clear();
+    if (t->mfma_stats && p.mfma_rounds > 0) {
+        const long nwg = pt_mfma_round_workgroups(t->qw - 7, t->qh - 7);
+        std::vector<int32_t> h((size_t)p.mfma_rounds * nwg);
+        HIPTRY(hipMemcpy(h.data(), t->mfma_items.p, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int r = 0; r < p.mfma_rounds; r++) {
+            int64_t sum = 0;
+            for (long i = 0; i < nwg; i++) sum += h[(size_t)r * nwg + i];
+            t->mfma_items_sum.push_back(sum);
+        }
+    }
     float ms = 0;
     HIPTRY(hipEventElapsedTime(&ms, t->ev[6], t->ev[7]));
     t->stage_ms[6] = ms;
@@ -1567,6 +1586,19 @@ extern "C" int pt_table_get_kcoherence(const pt_table *t, int32_t *pos_out, floa
     if (n == 0) return PT_OK;
     if (pos_out) HIPTRY(hipMemcpy(pos_out, t->kpos.p, n * 4, hipMemcpyDeviceToHost));
     if (dist_out) HIPTRY(hipMemcpy(dist_out, t->kdist.p, n * 4, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+extern "C" int pt_table_set_mfma_stats(pt_table *t, int on) {
+    if (!t) return pt_fail(PT_E_INVALID, "null argument");
+    t->mfma_stats = on != 0;
+    return PT_OK;
+}
+
+extern "C" int pt_table_mfma_items(const pt_table *t, int64_t *items, int n, int *rounds) {
+    if (!t || !rounds) return pt_fail(PT_E_INVALID, "null argument");
+    *rounds = (int)t->mfma_items_sum.size();
+    for (int r = 0; r < n && r < *rounds; r++) items[r] = t->mfma_items_sum[r];
     return PT_OK;
 }
 

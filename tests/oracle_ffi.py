@@ -11,6 +11,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+# the OpenMP build (pt_params.threads > 1 runs the raster lookup's rows and the k-coherence
+# patches on several threads): timing only, selected by use_openmp() before the first call
+LIB_OMP = os.path.join(ROOT, "oracle", "liboracle_omp.so")
 
 _P = ctypes.POINTER
 _f32p = _P(ctypes.c_float)
@@ -19,15 +22,24 @@ _i32p = _P(ctypes.c_int32)
 _int = ctypes.c_int
 
 _lib = None
+_path = LIB
+
+
+def use_openmp():
+    """Load the OpenMP build instead of the checker (before the first oracle call)."""
+    global _path
+    if _lib is not None and _path != LIB_OMP:
+        raise RuntimeError("the oracle library is already loaded")
+    _path = LIB_OMP
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        if not os.path.exists(_path):
             subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
         from patchtable_amd._ffi import PtParams
-        L = ctypes.CDLL(LIB)
+        L = ctypes.CDLL(_path)
         sig = {
             "orc_glibc_rand": (_int, [ctypes.c_uint32, _int, _i32p]),
             "orc_random_shuffle": (_int, [_int, _i32p]),
@@ -65,6 +77,7 @@ def lib():
             "orc_lookup_raster_keyed": (_int, [ctypes.c_void_p, _f32p, _int, _int, _f32p, _f64p]),
             "orc_brute_f64": (_int, [_f32p, _int, _i32p, _int, _f32p, _int, _int, _i32p, _f64p]),
             "orc_key_round_weights": (_int, [_int]),
+            "orc_mfma_items": (_int, [_P(ctypes.c_long), _int]),
             "orc_key_loss": (ctypes.c_double, [_f64p, _f64p, _f64p, _f64p, _f32p, _f32p, _int, _i32p, _int,
                                                ctypes.c_double]),
             "orc_key_forward": (_int, [_f32p, _f32p, _f32p, _f32p, _f32p, _int, _int, _f32p]),
@@ -292,3 +305,10 @@ def dt(nslices, table, algo, dt_iters=-1):
     rc = lib().orc_dt(len(nslices), ns, i32(t), algo, dt_iters)
     assert rc == 0
     return t
+
+
+def mfma_items():
+    """(group, anchor) items of each round of the oracle's last MFMA lookup."""
+    buf = (ctypes.c_long * 64)()
+    n = lib().orc_mfma_items(buf, 64)
+    return [int(buf[i]) for i in range(min(n, 64))]

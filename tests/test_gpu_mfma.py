@@ -41,6 +41,25 @@ def test_mfma_lookup_bit_exact(qh, qw, bh, bw, extra):
     assert np.abs(d64 - ann[..., 2]).max() <= 1e-4
 
 
+def test_mfma_item_counts_match_oracle():
+    """The instrumentation's per-round item counts (pt_table_mfma_items) equal the oracle's, and
+    are off until asked for."""
+    from patchtable_amd import synth_pair
+    a, _ = synth_pair(530, 90, 140)
+    _, b = synth_pair(531, 100, 120)
+    p = speed1("-limit 1 -mfma_rounds 4")
+    t, o = _both(p, b)
+    t.lookup(a)
+    assert t.mfma_items() == []
+    t.set_mfma_stats(True)
+    ann = t.lookup(a)
+    assert np.array_equal(ann, o.lookup_parallel(a))
+    items = t.mfma_items()
+    assert len(items) == 4 and items == O.mfma_items(), (items, O.mfma_items())
+    groups = (140 - 7 + 15) // 16 * (90 - 7)
+    assert all(groups <= n <= 6 * groups for n in items)
+
+
 def test_mfma_lookup_previous_field():
     """Starting from a previous field (with temporal weighting and recomputed distances)."""
     from patchtable_amd import synth_pair
