@@ -1,7 +1,8 @@
 """GPU: pt_table_lookup_dev_noalloc (the lookup without device allocation) through the C ABI
 against pt_table_lookup_dev, bit for bit, on consecutive calls whose second query is smaller
 (the shared device array keeps the first call's contents), with query_step 2, with a previous
-field, in descriptor mode, and a query too large for the array rejected before any launch."""
+field, in descriptor mode, in the MFMA form, and a query too large for the array rejected before
+any launch."""
 import ctypes
 
 import numpy as np
@@ -66,6 +67,19 @@ def test_noalloc_query_step_and_descriptor_mode(lib):
     dt = PatchTable(pd, torch.from_numpy(bd).cuda())
     ref, got = _both(lib, dt, torch.from_numpy(ad).cuda())
     assert np.array_equal(ref, got)
+
+
+def test_noalloc_mfma_form_two_calls(lib):
+    """The MFMA lookup form in the shared array (its int8 query copy lives there too)."""
+    import torch
+    from patchtable_amd import PatchTable, synth_pair
+    print("noalloc: MFMA form, 260x300 then 120x170 queries", flush=True)
+    a1, b = synth_pair(75, 260, 300)
+    a2, _ = synth_pair(76, 120, 170)
+    table = PatchTable(speed1("-limit 1 -mfma_rounds 4 -mfma_anchors 5"), torch.from_numpy(b).cuda())
+    for a in (a1, a2):
+        ref, got = _both(lib, table, torch.from_numpy(a).cuda())
+        assert np.array_equal(ref, got)
 
 
 def test_noalloc_rejects_queries_beyond_the_array(lib):
