@@ -326,6 +326,12 @@ int pt_key_gradients(pt_key *k, const float *a_dev, const float *b_dev, int h, i
 int pt_key_train(pt_key *k, pt_params *p, int npairs, const float *const *a_dev, const float *const *b_dev,
                  const int *h, const int *w, int tpp, int steps, int batch, float lr, float margin,
                  uint64_t seed, float *loss_per_step);
+/* pt_key_train_exact: as pt_key_train, with each anchor's positive its exact nearest neighbour
+ *   in b_i: the exhaustive search (pt_brute_lookup_dev, BF16 scoring, the 8 best rescored in
+ *   exact FP32) instead of a table lookup. */
+int pt_key_train_exact(pt_key *k, int npairs, const float *const *a_dev, const float *const *b_dev, const int *h,
+                       const int *w, int tpp, int steps, int batch, float lr, float margin, uint64_t seed,
+                       float *loss_per_step);
 int pt_table_create_keyed(pt_params *p, const pt_key *key, const float *b_dev, int h, int w, pt_table **out);
 
 /* ---- Errors ------------------------------------------------------------------------------ */

@@ -431,9 +431,12 @@ class PatchKey:
                                       *[_ffi_f32(x) for x in g], _ffi_f32(loss)))
         return g, float(loss[0])
 
-    def train(self, p, pairs, tpp=2048, steps=300, batch=2048, lr=1e-3, margin=1.0, seed=1):
-        """Mines triplets from the library's lookups of the (a, b) device image pairs, then runs
-        `steps` Adam steps; returns the per-step losses (numpy)."""
+    def train(self, p, pairs, tpp=2048, steps=300, batch=2048, lr=1e-3, margin=1.0, seed=1, positives="lookup"):
+        """Mines triplets from the (a, b) device image pairs -- positives from the library's
+        lookups with parameters p ("lookup") or the exhaustive search's exact nearest neighbours
+        ("exact", p unused) -- then runs `steps` Adam steps; returns the per-step losses (numpy)."""
+        if positives not in ("lookup", "exact"):
+            raise PatchTableError("positives must be 'lookup' or 'exact'")
         n = len(pairs)
         A = (ctypes.c_void_p * n)(*[a.data_ptr() for a, _ in pairs])
         B = (ctypes.c_void_p * n)(*[b.data_ptr() for _, b in pairs])
@@ -450,8 +453,12 @@ class PatchKey:
         if pairs:
             _after_torch(pairs[0][0].device)
         losses = np.zeros(max(steps, 1), np.float32)
-        check(load().pt_key_train(self._k, ctypes.byref(p.raw), n, A, B, H, W, int(tpp), int(steps), int(batch),
-                                  float(lr), float(margin), int(seed), _ffi_f32(losses)))
+        if positives == "exact":
+            check(load().pt_key_train_exact(self._k, n, A, B, H, W, int(tpp), int(steps), int(batch), float(lr),
+                                            float(margin), int(seed), _ffi_f32(losses)))
+        else:
+            check(load().pt_key_train(self._k, ctypes.byref(p.raw), n, A, B, H, W, int(tpp), int(steps), int(batch),
+                                      float(lr), float(margin), int(seed), _ffi_f32(losses)))
         return losses[:steps]
 
 

@@ -169,6 +169,8 @@ SIGNATURES = {
                                 _f32p]),
     "pt_key_train": (_int, [_vp, _P(PtParams), _int, _P(_vp), _P(_vp), _P(_int), _P(_int), _int, _int, _int,
                             ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _f32p]),
+    "pt_key_train_exact": (_int, [_vp, _int, _P(_vp), _P(_vp), _P(_int), _P(_int), _int, _int, _int,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _f32p]),
     "pt_table_create_keyed": (_int, [_P(PtParams), _vp, _vp, _int, _int, _P(_vp)]),
     "pt_brute_lookup_dev": (_int, [_vp, _int, _int, _vp, _int, _int, _vp, _int, _int, _vp, _vp, _vp, _vp, _f64p]),
     "pt_brute_exact_f64_dev": (_int, [_vp, _int, _int, _vp, _int, _int, _vp, _int, _vp, _vp]),

@@ -221,16 +221,23 @@ extern "C" int pt_key_gradients(pt_key *k, const float *a_dev, const float *b_de
     return PT_OK;
 }
 
-// Training: for every pair (a_i, b_i) the library's own lookup (parameters p) gives each query
-// patch its match in b_i; `tpp` anchors per pair are drawn uniformly, the positive is the match,
-// the negative the match moved by 2..6 pixels in each axis (half the triplets) or a uniformly
-// random database patch. The triplets of all pairs are shuffled, then `steps` Adam steps of
-// `batch` triplets each (epochs wrap around) minimise the margin loss on squared key distances.
-extern "C" int pt_key_train(pt_key *k, pt_params *p, int npairs, const float *const *a_dev, const float *const *b_dev,
-                            const int *h, const int *w, int tpp, int steps, int batch, float lr, float margin,
-                            uint64_t seed, float *loss_per_step) {
-    if (!k || !p || npairs < 1 || !a_dev || !b_dev || !h || !w || tpp < 1 || steps < 0 || batch < 1)
+// Training: for every pair (a_i, b_i), `tpp` anchors drawn uniformly from the query patches;
+// the positive is the anchor's match in b_i -- the library's own lookup with parameters p, or
+// (exact) the exhaustive search's nearest neighbour (BF16 scoring on the matrix cores, the 8 best
+// rescored in exact FP32) -- and the negative the positive moved by 2..6 pixels in each axis
+// (half the triplets) or a uniformly random database patch. The triplets of all pairs are
+// shuffled, then `steps` Adam steps of `batch` triplets each (epochs wrap around) minimise the
+// margin loss on squared key distances.
+namespace {
+
+int key_train(pt_key *k, pt_params *p, bool exact, int npairs, const float *const *a_dev, const float *const *b_dev,
+              const int *h, const int *w, int tpp, int steps, int batch, float lr, float margin, uint64_t seed,
+              float *loss_per_step) {
+    if (!k || (!exact && !p) || npairs < 1 || !a_dev || !b_dev || !h || !w || tpp < 1 || steps < 0 || batch < 1)
         return pt_fail(PT_E_INVALID, "pt_key_train: bad arguments");
+    for (int pi = 0; pi < npairs; pi++)
+        if (h[pi] < 8 || w[pi] < 8 || !PT_SIZE_OK(h[pi], w[pi]))
+            return pt_fail(PT_E_INVALID, "pt_key_train: bad image size");
     Pcg g(seed);
     const long T = (long)npairs * tpp;
     DevBuf<unsigned short> X;
@@ -240,27 +247,17 @@ extern "C" int pt_key_train(pt_key *k, pt_params *p, int npairs, const float *co
     for (long i = 0; i < T; i++) slot[i] = i;
     for (long i = T - 1; i > 0; i--) std::swap(slot[i], slot[(long)(g.next() % (uint64_t)(i + 1))]);
     DevBuf<double> ann;
-    DevBuf<int32_t> pos;
+    DevBuf<int32_t> pos, qpos, npos;
+    DevBuf<float> ndist;
     DevBuf<int64_t> dslot;
     TRY(dslot.ensure(T));
     HIPTRY(hipMemcpy(dslot.p, slot.data(), T * 8, hipMemcpyHostToDevice));
     std::vector<double> hann;
+    std::vector<int32_t> hq, hn;
     for (int pi = 0; pi < npairs; pi++) {
-        pt_table *t = nullptr;
-        TRY(pt_table_create_dev(p, b_dev[pi], h[pi], w[pi], 3, nullptr, &t));
         const int aw = w[pi] - 7, ah = h[pi] - 7;
-        TRY(ann.ensure((size_t)aw * ah * 3));
-        int rc = pt_table_lookup_dev(t, a_dev[pi], h[pi], w[pi], 3, ann.p, nullptr, nullptr, nullptr);
-        pt_table_destroy(t);
-        if (rc != PT_OK) return rc;
-        hann.resize((size_t)aw * ah * 3);
-        HIPTRY(hipMemcpy(hann.data(), ann.p, hann.size() * 8, hipMemcpyDeviceToHost));
         std::vector<int32_t> hp((size_t)tpp * 3);
-        for (int t2 = 0; t2 < tpp; t2++) {
-            const int x = g.below(aw), y = g.below(ah);
-            const double *q = &hann[((size_t)y * aw + x) * 3];
-            const int bx = (int)q[0], by = (int)q[1];
-            int nx, ny;
+        auto negative = [&](int bx, int by, int &nx, int &ny) {
             if (g.next() & 1u) {
                 const int dx = (2 + g.below(5)) * (g.next() & 1u ? 1 : -1), dy = (2 + g.below(5)) * (g.next() & 1u ? 1 : -1);
                 nx = std::min(std::max(bx + dx, 0), aw - 1);
@@ -270,9 +267,48 @@ extern "C" int pt_key_train(pt_key *k, pt_params *p, int npairs, const float *co
                 nx = g.below(aw);
                 ny = g.below(ah);
             }
-            hp[(size_t)t2 * 3 + 0] = PT_XY_TO_INT(x, y);
-            hp[(size_t)t2 * 3 + 1] = PT_XY_TO_INT(bx, by);
-            hp[(size_t)t2 * 3 + 2] = PT_XY_TO_INT(nx, ny);
+        };
+        if (!exact) {
+            pt_table *t = nullptr;
+            TRY(pt_table_create_dev(p, b_dev[pi], h[pi], w[pi], 3, nullptr, &t));
+            TRY(ann.ensure((size_t)aw * ah * 3));
+            int rc = pt_table_lookup_dev(t, a_dev[pi], h[pi], w[pi], 3, ann.p, nullptr, nullptr, nullptr);
+            pt_table_destroy(t);
+            if (rc != PT_OK) return rc;
+            hann.resize((size_t)aw * ah * 3);
+            HIPTRY(hipMemcpy(hann.data(), ann.p, hann.size() * 8, hipMemcpyDeviceToHost));
+            for (int t2 = 0; t2 < tpp; t2++) {
+                const int x = g.below(aw), y = g.below(ah);
+                const double *q = &hann[((size_t)y * aw + x) * 3];
+                const int bx = (int)q[0], by = (int)q[1];
+                int nx, ny;
+                negative(bx, by, nx, ny);
+                hp[(size_t)t2 * 3 + 0] = PT_XY_TO_INT(x, y);
+                hp[(size_t)t2 * 3 + 1] = PT_XY_TO_INT(bx, by);
+                hp[(size_t)t2 * 3 + 2] = PT_XY_TO_INT(nx, ny);
+            }
+        } else {
+            hq.resize(tpp);
+            for (int t2 = 0; t2 < tpp; t2++) {
+                const int x = g.below(aw), y = g.below(ah);
+                hq[t2] = PT_XY_TO_INT(x, y);
+            }
+            TRY(qpos.ensure(tpp));
+            TRY(npos.ensure(tpp));
+            TRY(ndist.ensure(tpp));
+            HIPTRY(hipMemcpy(qpos.p, hq.data(), (size_t)tpp * 4, hipMemcpyHostToDevice));
+            TRY(pt_brute_lookup_dev(a_dev[pi], h[pi], w[pi], b_dev[pi], h[pi], w[pi], qpos.p, tpp, PT_BRUTE_BF16,
+                                    npos.p, ndist.p, nullptr, nullptr, nullptr));
+            hn.resize(tpp);
+            HIPTRY(hipMemcpy(hn.data(), npos.p, (size_t)tpp * 4, hipMemcpyDeviceToHost));
+            for (int t2 = 0; t2 < tpp; t2++) {
+                const int bx = PT_INT_TO_X(hn[t2]), by = PT_INT_TO_Y(hn[t2]);
+                int nx, ny;
+                negative(bx, by, nx, ny);
+                hp[(size_t)t2 * 3 + 0] = hq[t2];
+                hp[(size_t)t2 * 3 + 1] = hn[t2];
+                hp[(size_t)t2 * 3 + 2] = PT_XY_TO_INT(nx, ny);
+            }
         }
         TRY(pos.ensure(hp.size()));
         HIPTRY(hipMemcpy(pos.p, hp.data(), hp.size() * 4, hipMemcpyHostToDevice));
@@ -298,4 +334,19 @@ extern "C" int pt_key_train(pt_key *k, pt_params *p, int npairs, const float *co
     }
     HIPTRY(hipStreamSynchronize(k->s));
     return PT_OK;
+}
+
+}  // namespace
+
+extern "C" int pt_key_train(pt_key *k, pt_params *p, int npairs, const float *const *a_dev, const float *const *b_dev,
+                            const int *h, const int *w, int tpp, int steps, int batch, float lr, float margin,
+                            uint64_t seed, float *loss_per_step) {
+    return key_train(k, p, false, npairs, a_dev, b_dev, h, w, tpp, steps, batch, lr, margin, seed, loss_per_step);
+}
+
+extern "C" int pt_key_train_exact(pt_key *k, int npairs, const float *const *a_dev, const float *const *b_dev,
+                                  const int *h, const int *w, int tpp, int steps, int batch, float lr, float margin,
+                                  uint64_t seed, float *loss_per_step) {
+    return key_train(k, nullptr, true, npairs, a_dev, b_dev, h, w, tpp, steps, batch, lr, margin, seed,
+                     loss_per_step);
 }

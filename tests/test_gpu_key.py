@@ -129,6 +129,40 @@ def test_keyed_table_bit_exact(extra):
     assert np.array_equal(np.moveaxis(planes, 0, -1), kb)
 
 
+def test_keyed_table_mfma_form_bit_exact():
+    """A key table in the MFMA lookup form: its candidates come from the key table and the GCK
+    fallback, then the MFMA rounds; bit-exact with the oracle given the GPU's keys."""
+    import torch
+    p = speed1("-limit 0.5 -mfma_rounds 4 -mfma_anchors 5")
+    a, b, key, t = keyed_case(p, (180, 260), 44)
+    assert t.info()["kcoherence"] == 0
+    o = O.OracleTable(p.raw, b, key=t.descriptor())
+    ann = t.lookup(torch.from_numpy(a).cuda()).cpu().numpy()
+    assert np.array_equal(ann, o.lookup_keyed(a, t.query_descriptor(*a.shape[:2])))
+    m, d = O.recompute(a, b, ann)
+    assert np.allclose(ann[..., 2], d, rtol=1e-5, atol=1e-6)
+
+
+def test_training_on_exact_nearest_neighbours():
+    """Positives from the exhaustive search (positives="exact"): the loss falls, and the trained
+    key's table candidates alone beat the untrained network's on a held-out pair."""
+    import torch
+    from patchtable_amd import PatchKey, PatchTable, synth_pair
+    pairs = [tuple(torch.from_numpy(x).cuda() for x in synth_pair(2000 + i, 192, 256)) for i in range(4)]
+    key = PatchKey(1)
+    losses = key.train(None, pairs, tpp=2048, steps=200, batch=512, lr=2e-3, margin=1.0, seed=9, positives="exact")
+    first, last = losses[:10].mean(), losses[-10:].mean()
+    print(f"exact positives: loss {first:.4f} -> {last:.4f}", flush=True)
+    assert np.isfinite(losses).all() and last < 0.7 * first
+    a, b = synth_pair(3000, 256, 320)
+    A, B = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    p = speed1("-kcoherence 0 -do_prop 0")
+    trained = PatchTable.keyed(p, key, B).lookup(A).cpu().numpy()[..., 2].mean()
+    untrained = PatchTable.keyed(p, PatchKey(1), B).lookup(A).cpu().numpy()[..., 2].mean()
+    print(f"table candidates: trained {trained:.4f} untrained {untrained:.4f}", flush=True)
+    assert trained < untrained
+
+
 def test_keyed_table_rejects_save_and_descriptor_mode(tmp_path):
     import torch
     from patchtable_amd import PatchKey, PatchTable, PatchTableError, synth_pair
