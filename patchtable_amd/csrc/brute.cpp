@@ -81,7 +81,8 @@ extern "C" int pt_brute_lookup_dev(const float *a_dev, int ah, int aw, const flo
     do {
         if (hipEventRecord(ev[0], s) != hipSuccess) { rc = pt_fail(PT_E_HIP, "event"); break; }
         rc = pt_brute_vectors_launch(a_dev, aw, qpos_dev, nq, aw - 7, fp8, qv.p, qn.p, qsc.p, s);
-        if (rc == PT_OK) rc = pt_brute_vectors_launch(b_dev, bw, nullptr, nd, bw - 7, fp8, dv.p, dn.p, dsc.p, s);
+        // FP8 database vectors are stored negated with a doubled scale (the search then yields -2 a.b)
+        if (rc == PT_OK) rc = pt_brute_vectors_launch(b_dev, bw, nullptr, nd, bw - 7, fp8 ? 2 : 0, dv.p, dn.p, dsc.p, s);
         if (rc != PT_OK) break;
         if (hipEventRecord(ev[1], s) != hipSuccess) { rc = pt_fail(PT_E_HIP, "event"); break; }
         rc = pt_brute_mfma_launch(qv.p, qsc.p, nq, dv.p, dn.p, dsc.p, (int)nd, fp8, cd, cs, s);

@@ -7,11 +7,14 @@
 // so ranking all of b for a block of queries is a [queries x 192] x [192 x database] contraction.
 //
 //   k_patch_vectors   patch vectors in BF16 (round to nearest even) or FP8 e4m3 with a per-patch
-//                     scale (max |v| / 448), and |v|^2 of the rounded vector in FP32.
+//                     power-of-two scale (an E8M0 block scale of the instruction), and |v|^2 of the
+//                     rounded vector in FP32.
 //   k_brute_mfma      one block = 256 queries (4 waves x 4 row tiles of 16); the database streams
 //                     through two LDS buffers in chunks of 64 patches (the next chunk's global
-//                     loads overlap the current chunk's MFMAs); a.b on v_mfma_f32_16x16x32_bf16 (or
-//                     v_mfma_f32_16x16x32_fp8_fp8 times the two scales), FP32 accumulate; each
+//                     loads overlap the current chunk's MFMAs); a.b on v_mfma_f32_16x16x32_bf16, or
+//                     |b|^2 - 2 a.b directly on v_mfma_scale_f32_16x16x128_f8f6f4 (two K steps of
+//                     128, the second half zero; accumulators start at |b|^2, the scales are
+//                     applied by the instruction), FP32 accumulate; each
 //                     lane keeps the 2 best scores |b|^2 - 2 a.b of its 16 query rows over the
 //                     database columns it sees; at the end the 32 lane candidates of a query are
 //                     merged to its 8 best.
@@ -32,6 +35,7 @@ namespace {
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 
 constexpr int DIM = 192;
 constexpr int QPB = 256;        // queries per block
@@ -58,7 +62,10 @@ __device__ __forceinline__ void patch_xy(const int32_t *pos, long p, int ew, int
 }
 
 // One wave per patch; lane l holds elements l, l + 64, l + 128 (k = dy*24 + dx*3 + c).
-// fp8 = 0: vec is BF16 bits (2 bytes per element); fp8 = 1: vec is e4m3 bytes, scale[p] set.
+// fp8 = 0: vec is BF16 bits (2 bytes per element). fp8 = 1 (queries) or 2 (database): vec is e4m3
+// bytes with a power-of-two scale[p] = 2^e, the smallest with max |v| / scale <= 448, so the
+// matrix cores can apply it as an E8M0 block scale; database vectors (fp8 = 2) hold -v / scale
+// and scale[p] = 2 * 2^e, so the contraction yields -2 a.b. norm[p] is |v|^2 of the rounded vector.
 __global__ __launch_bounds__(256) void k_patch_vectors(const float *__restrict__ img, int w, const int32_t *__restrict__ pos,
                                                       long n, int ew, int fp8, unsigned char *__restrict__ vec,
                                                       float *__restrict__ norm, float *__restrict__ scale) {
@@ -87,18 +94,23 @@ __global__ __launch_bounds__(256) void k_patch_vectors(const float *__restrict__
         float m = fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), fabsf(v[2])));
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        const float sc = m > 0.f ? m / 448.f : 1.f;
-        const float inv = 1.f / sc;
+        // m / 448 = f 2^e with f in [0.5, 1): scale 2^e keeps every |v| / scale <= 448
+        int e = 0;
+        (void)frexpf(m > 0.f ? m / 448.f : 1.f, &e);
+        e = min(max(e, -126), 127);
+        const float sc = ldexpf(1.f, e);
+        const float inv = ldexpf(1.f, -e);
+        const float sgn = fp8 == 2 ? -1.f : 1.f;
         unsigned char *o = vec + (size_t)p * DIM;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             // v_cvt_pk_fp8_f32 converts two values; the first of the pair is the one kept
-            const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(v[i] * inv, 0.f, 0, false);
+            const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(sgn * (v[i] * inv), 0.f, 0, false);
             o[lane + 64 * i] = (unsigned char)(packed & 0xff);
             const float r = __builtin_amdgcn_cvt_f32_fp8(packed, 0) * sc;
             s += r * r;
         }
-        if (lane == 0) scale[p] = sc;
+        if (lane == 0) scale[p] = fp8 == 2 ? 2.f * sc : sc;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -129,7 +141,9 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
                                                    const float *__restrict__ dnorm, const float *__restrict__ dscale,
                                                    int nd, int32_t *__restrict__ cand, float *__restrict__ cscore) {
     constexpr int EB = FP8 ? 1 : 2;                 // bytes per element
-    constexpr int ROWB = DIM * EB + 16;             // LDS row pitch (bytes), padded
+    // LDS row pitch (bytes): FP8 rows carry 64 zero bytes after the 192 elements, so the
+    // K = 128 instruction's second step reads 256 bytes; 16 bytes of padding against bank conflicts
+    constexpr int ROWB = FP8 ? 256 + 16 : DIM * EB + 16;
     // two database chunks while streaming (the next one is written while the current one is
     // read); at the end the candidate lists of two waves at a time
     constexpr int MERGE = 2 * 64 * 16 * KL;
@@ -142,29 +156,41 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 15, grp = lane >> 4;
     const int qw0 = blockIdx.x * QPB + wave * 64;   // first query of this wave
-    // A fragments: query row tile t, K block kb: 8 elements k = kb*32 + 8*grp .. + 7 of query
-    // qw0 + 16t + col (clamped to the last query; such rows are never written)
-    v4i afrag[4][DIM / 32];   // 16 bytes per fragment (8 BF16 or 8 FP8 in the low half)
+    // A fragments of query qw0 + 16t + col (clamped to the last query; such rows are never
+    // written). BF16: K block kb, elements k = kb*32 + 8*grp .. + 7 (v_mfma_f32_16x16x32_bf16).
+    // FP8: K step ks of 128, bytes k = 128*ks + 32*grp .. + 31 (v_mfma_f32_16x16x128_f8f6f4; the
+    // B fragments use the same k for the same lane group and byte, so the products pair up);
+    // k >= 192 is zero.
+    constexpr int NKB = FP8 ? 1 : DIM / 32;
+    v4i afrag[4][NKB];
+    v8i afr8[4][2];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const int q = min(qw0 + 16 * t + col, nq - 1);
+        if (FP8) {
 #pragma unroll
-        for (int kb = 0; kb < DIM / 32; kb++) {
-            const unsigned char *src = qvec + (size_t)q * DIM * EB + (size_t)(kb * 32 + 8 * grp) * EB;
-            if (FP8) {
-                const int2 v = *(const int2 *)src;
-                afrag[t][kb] = v4i{v.x, v.y, 0, 0};
-            } else {
-                afrag[t][kb] = *(const v4i *)src;
+            for (int ks = 0; ks < 2; ks++) {
+                const int k0 = 128 * ks + 32 * grp;
+                if (k0 < DIM) {
+                    const v4i lo = *(const v4i *)(qvec + (size_t)q * DIM + k0);
+                    const v4i hi = *(const v4i *)(qvec + (size_t)q * DIM + k0 + 16);
+                    afr8[t][ks] = v8i{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                } else {
+                    afr8[t][ks] = v8i{0, 0, 0, 0, 0, 0, 0, 0};
+                }
             }
+        } else {
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++)
+                afrag[t][kb] = *(const v4i *)(qvec + (size_t)q * DIM * EB + (size_t)(kb * 32 + 8 * grp) * EB);
         }
     }
-    // scales of the query rows this lane's accumulators hold (rows 16t + 4grp + j)
-    float rs[4][4];
+    // FP8: E8M0 scale (the biased exponent of the power-of-two scale) of the query row this
+    // lane's A fragments hold, row 16t + col of the wave's tile t
+    int qe[4];
 #pragma unroll
     for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) rs[t][j] = FP8 ? qscale[min(qw0 + 16 * t + 4 * grp + j, nq - 1)] : 1.f;
+        qe[t] = FP8 ? (int)((__float_as_uint(qscale[min(qw0 + 16 * t + col, nq - 1)]) >> 23) & 0xffu) : 0;
     float bs[16][KL];
     int bi[16][KL];
 #pragma unroll
@@ -202,6 +228,12 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
             sscale[buf][threadIdx.x] = sts;
         }
     };
+    if (FP8) {   // the zero bytes 192..255 of every row of both buffers (staging never writes them)
+        for (int i = threadIdx.x; i < 2 * DBC * 4; i += 256) {
+            const int row = i >> 2, piece = i & 3;
+            *(v4i *)(smem + (size_t)row * ROWB + DIM + piece * 16) = v4i{0, 0, 0, 0};
+        }
+    }
     fetch(0);
     put(0);
     __syncthreads();
@@ -211,21 +243,34 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
         const unsigned char *sdb = smem + (size_t)cur * DBC * ROWB;
 #pragma unroll 1
         for (int c = 0; c < DBC / 16; c++) {
+            // D: lane holds query rows 16t + 4grp + j against database column c*16 + col
+            const int dcol = c * 16 + col;
+            const int didx = d0 + dcol;
+            const float dn = snorm[cur][dcol];
             v4f acc[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
             const unsigned char *brow = sdb + (c * 16 + col) * ROWB;
+            if (FP8) {
+                // score = |b|^2 + sum (a / sa)(-b / sb) sa (2 sb) = |b|^2 - 2 a.b, straight from
+                // the matrix cores: the accumulators start at |b|^2 and the E8M0 block scales
+                // (one per row of A, one per column of B) are applied by the instruction
+                const int be = (int)((__float_as_uint(sscale[cur][dcol]) >> 23) & 0xffu);
 #pragma unroll
-            for (int kb = 0; kb < DIM / 32; kb++) {
-                if (FP8) {
-                    const int2 bv = *(const int2 *)(brow + (kb * 32 + 8 * grp));
-                    const long bl = ((long)(unsigned)bv.y << 32) | (unsigned)bv.x;
+                for (int t = 0; t < 4; t++) acc[t] = v4f{dn, dn, dn, dn};
 #pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const long al = ((long)(unsigned)afrag[t][kb][1] << 32) | (unsigned)afrag[t][kb][0];
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(al, bl, acc[t], 0, 0, 0);
-                    }
-                } else {
+                for (int ks = 0; ks < 2; ks++) {
+                    const v4i lo = *(const v4i *)(brow + 128 * ks + 32 * grp);
+                    const v4i hi = *(const v4i *)(brow + 128 * ks + 32 * grp + 16);
+                    const v8i b8 = v8i{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+#pragma unroll
+                    for (int t = 0; t < 4; t++)   // e4m3 operands (cbsz = blgp = 0), scale byte 0 of each
+                        acc[t] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(afr8[t][ks], b8, acc[t], 0, 0, 0,
+                                                                                  qe[t], 0, be);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kb = 0; kb < NKB; kb++) {
                     const v4i bv = *(const v4i *)(brow + (kb * 32 + 8 * grp) * 2);
                     const bf8 bb = __builtin_bit_cast(bf8, bv);
 #pragma unroll
@@ -234,18 +279,21 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
                                                                         acc[t], 0, 0, 0);
                 }
             }
-            // D: lane holds query rows 16t + 4grp + j against database column c*16 + col
-            const int dcol = c * 16 + col;
-            const int didx = d0 + dcol;
-            const float dn = snorm[cur][dcol];
-            const float dsc = sscale[cur][dcol];
+            auto score = [&](int t, int j) { return FP8 ? acc[t][j] : dn - 2.f * acc[t][j]; };
+            // a score enters a list only when it beats the list's second entry; once the lists
+            // have seen a few thousand columns that is rare, so the wave tests all 16 first and
+            // runs the insertions (the same scores, recomputed) only when some lane needs one
+            bool enter = false;
 #pragma unroll
             for (int t = 0; t < 4; t++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const float dot = FP8 ? acc[t][j] * (rs[t][j] * dsc) : acc[t][j];
-                    keep_best(dn - 2.f * dot, didx, bs[4 * t + j], bi[4 * t + j]);
-                }
+                for (int j = 0; j < 4; j++) enter = enter || score(t, j) < bs[4 * t + j][KL - 1];
+            if (__any(enter)) {
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) keep_best(score(t, j), didx, bs[4 * t + j], bi[4 * t + j]);
+            }
         }
         if (more) put(cur ^ 1);   // the other buffer was last read before the previous barrier
         __syncthreads();
