@@ -263,7 +263,7 @@ def key_forward(args):
     tf = n * 25600 / (ms / 1e3) / 1e12
     out = {"metric": "learned-key forward Mpatch/s (2048x2048 image)", "value": round(n / ms / 1e3, 2),
            "unit": "Mpatch/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (layer 2 f32)",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
            "data": "synthetic", "config": {"workload": "key forward over a 2048x2048 image (seed 3), 192-64-8 MLP"},
            "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": 2500.0, "unit": "TFLOP/s",
                         "frac": round(tf / 2500.0, 4), "traffic": None,

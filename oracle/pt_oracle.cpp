@@ -1886,7 +1886,9 @@ struct KeyNet {
     // weights given in double (finite differences)
     KeyNet(const double *W1, const double *B1, const double *W2, const double *B2)
         : w1(W1, W1 + KEY_H * KEY_IN), b1(B1, B1 + KEY_H), w2(W2, W2 + KEY_OUT * KEY_H), b2(B2, B2 + KEY_OUT) {}
-    void forward(const float *x, double *z, double *k) const {
+    // round_hidden: the ReLU'd hidden units enter layer 2 rounded to BF16 (nearest even), as the
+    // GPU's inference forward feeds them to its BF16 layer-2 MFMA; training keeps them exact
+    void forward(const float *x, double *z, double *k, bool round_hidden = false) const {
         for (int j = 0; j < KEY_H; j++) {
             double s = b1[j];
             for (int i = 0; i < KEY_IN; i++) s += w1[j * KEY_IN + i] * x[i];
@@ -1894,7 +1896,10 @@ struct KeyNet {
         }
         for (int o = 0; o < KEY_OUT; o++) {
             double s = b2[o];
-            for (int j = 0; j < KEY_H; j++) s += w2[o * KEY_H + j] * (z[j] > 0 ? z[j] : 0.0);
+            for (int j = 0; j < KEY_H; j++) {
+                const double hv = z[j] > 0 ? z[j] : 0.0;
+                s += w2[o * KEY_H + j] * (round_hidden ? (double)bf16_round((float)hv) : hv);
+            }
             k[o] = s;
         }
     }
@@ -2203,7 +2208,7 @@ int orc_key_forward(const float *W1, const float *B1, const float *W2, const flo
     for (int y = 0; y + 8 <= h; y++)
         for (int x = 0; x + 8 <= w; x++) {
             key_patch(img, w, x, y, xv);
-            net.forward(xv, z, k);
+            net.forward(xv, z, k, true);
             for (int o = 0; o < KEY_OUT; o++) planes[o * plane + (size_t)(y + 7) * (w + 7) + x + 7] = (float)k[o];
         }
     return 0;

@@ -3,8 +3,8 @@
 // BF16, activations accumulate in FP32. The patch vector is x[dy*24 + dx*3 + c] = img(y+dy,
 // x+dx, c) - 0.5 (the reference's HWC patch order, centred).
 //
-//   forward over an image (k_key_forward): both layers on MFMA per 16 patches of a row (BF16
-//     layer 1, FP32 layer 2), weights in registers; the keys are written as the 8 descriptor
+//   forward over an image (k_key_forward): both layers on BF16 MFMA per 16 patches of a row
+//     (the hidden units rounded to BF16 for layer 2), weights in registers; the keys are written as the 8 descriptor
 //     planes of the table ((h+7) x (w+7), patch (x, y) at (y+7, x+7)).
 //   training: triplets (anchor, positive, negative) of patch vectors in BF16; k_train_forward
 //     (VALU, FP32 accumulate over BF16 weights), k_triplet_grad (margin loss on squared key
@@ -40,10 +40,11 @@ __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float
 //     for the wave's lifetime; the patch B-fragments are 8 consecutive RGB values of one patch row
 //     (8 divides 24) converted to BF16 (round to nearest even) as they are loaded.
 //   bias + ReLU in registers: lane l holds hidden units nb*16 + 4*(l/16) + j of patch l % 16.
-//   layer 2: K^T (8 x 16) = W2 (8 x 64) . relu(H^T) as 16 K-steps of v_mfma_f32_16x16x4_f32 in
-//     FP32, the K-step (nb, j) taking hidden unit nb*16 + 4*(l/16) + j from lane l: the layer-1
-//     accumulator layout is directly the B operand, and A holds the matching W2 column (rows 8..15
-//     zero). Lane l < 32 then holds outputs 4*(l/16) + j of patch l % 16.
+//   layer 2: K^T (8 x 16) = W2 (8 x 64) . bf16(relu(H^T)) as 2 K-steps of 32 on
+//     v_mfma_f32_16x16x32_bf16: lane l's 8 B slots of K-step m are its own hidden units
+//     32m + 16(i/4) + 4*(l/16) + i%4 (the layer-1 accumulators of tiles 2m and 2m+1), and A holds
+//     W2 at the same units (rows 8..15 zero). Lane l < 32 then holds outputs 4*(l/16) + j of
+//     patch l % 16.
 // A and B operands are filled under one convention (lane group l/16 owns the same K indices in
 // both), so the sum over K is independent of how the hardware numbers K inside a group.
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
@@ -74,13 +75,15 @@ __global__ __launch_bounds__(256) void k_key_forward(const float *__restrict__ i
     for (int nb = 0; nb < 4; nb++)
 #pragma unroll
         for (int j = 0; j < 4; j++) bias1[nb][j] = b1[nb * 16 + 4 * grp + j];
-    float w2c[4][4];   // W2[col][nb*16 + 4*grp + j], zero for rows col >= 8
+    // layer-2 A fragments: W2 row col (zero for col >= 8) at the hidden units the B operand's
+    // slots hold: K-step m, lane group grp, slot i -> unit 32m + 16(i >> 2) + 4grp + (i & 3)
+    bf8 w2f[2];
 #pragma unroll
-    for (int nb = 0; nb < 4; nb++)
+    for (int m = 0; m < 2; m++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const float v = bf2f(w2[(col & (KO - 1)) * KH + nb * 16 + 4 * grp + j]);
-            w2c[nb][j] = col < KO ? v : 0.f;
+        for (int i = 0; i < 8; i++) {
+            const unsigned short v = w2[(col & (KO - 1)) * KH + 32 * m + 16 * (i >> 2) + 4 * grp + (i & 3)];
+            w2f[m][i] = __builtin_bit_cast(__bf16, col < KO ? v : (unsigned short)0);
         }
     float bias2[4];
 #pragma unroll
@@ -113,18 +116,20 @@ __global__ __launch_bounds__(256) void k_key_forward(const float *__restrict__ i
             for (int nb = 0; nb < 4; nb++)
                 acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag[nb][kb], bv, acc[nb], 0, 0, 0);
         }
-        // four independent accumulation chains (one per hidden tile), summed at the end
-        v4f o4[4];
+        // layer 2 on two BF16 K-steps of 32: the ReLU'd hidden units, rounded to BF16 (nearest
+        // even), are the B operand straight from the layer-1 accumulators (slot i of K-step m is
+        // acc[2m + (i >> 2)][i & 3])
+        v4f out = v4f{bias2[0], bias2[1], bias2[2], bias2[3]};
 #pragma unroll
-        for (int nb = 0; nb < 4; nb++) {
-            o4[nb] = nb == 0 ? v4f{bias2[0], bias2[1], bias2[2], bias2[3]} : v4f{0.f, 0.f, 0.f, 0.f};
+        for (int m = 0; m < 2; m++) {
+            v8f hv;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const float hv = acc[nb][j] > 0.f ? acc[nb][j] : 0.f;
-                o4[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(w2c[nb][j], hv, o4[nb], 0, 0, 0);
+            for (int i = 0; i < 8; i++) {
+                const float z = acc[2 * m + (i >> 2)][i & 3];
+                hv[i] = z > 0.f ? z : 0.f;
             }
+            out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2f[m], __builtin_convertvector(hv, bf8), out, 0, 0, 0);
         }
-        const v4f out = (o4[0] + o4[1]) + (o4[2] + o4[3]);
         if (grp < 2 && valid) {
             float *dst = planes + (size_t)(y + 7) * Wp + px + 7;
 #pragma unroll

@@ -17,17 +17,19 @@ def xy(x, y):
     return (y << 16) | x
 
 
-def selection_net(sel, w1_value=1.0):
-    """W1 row o picks input sel[o] (weight w1_value), b1 = 0.5 keeps the ReLU open for inputs
-    in [-0.5, 0.5], W2 picks hidden o, b2 = -0.5: key o = w1_value * x[sel[o]] (+0.5 - 0.5)."""
+def selection_net(sel, w1_value=1.0, offset=0.5):
+    """W1 row o picks input sel[o] (weight w1_value), b1 = offset (0.5 keeps the ReLU open for
+    inputs in [-0.5, 0.5]), W2 picks hidden o, b2 = -offset: key o = w1_value * x[sel[o]]. The
+    hidden unit w1_value * x + offset enters layer 2 rounded to BF16, so the known answers use
+    values that BF16 holds exactly."""
     w1 = np.zeros((KH, KI), np.float32)
     for o, k in enumerate(sel):
         w1[o, k] = w1_value
-    b1 = np.full(KH, 0.5, np.float32)
+    b1 = np.full(KH, offset, np.float32)
     w2 = np.zeros((KO, KH), np.float32)
     for o in range(KO):
         w2[o, o] = 1.0
-    b2 = np.full(KO, -0.5, np.float32)
+    b2 = np.full(KO, -offset, np.float32)
     return w1, b1, w2, b2
 
 
@@ -47,17 +49,29 @@ def test_forward_selection_known_answer():
 
 
 def test_forward_rounds_weights_to_bf16_nearest_even():
-    img = np.full((8, 8, 3), 0.75, np.float32)   # x = 0.25 everywhere
+    img = np.full((8, 8, 3), 0.75, np.float32)   # x = 0.25 everywhere: the ReLU is open without an offset
     sel = [0] * KO
     # 1 + 2^-8 is halfway between BF16 1 and 1 + 2^-7: ties to even -> 1.
-    k = key_forward(*selection_net(sel, 1.0 + 2.0 ** -8), img)[0, 7, 7]
+    k = key_forward(*selection_net(sel, 1.0 + 2.0 ** -8, 0.0), img)[0, 7, 7]
     assert k == np.float32(0.25)
     # 1 + 3 * 2^-8 is halfway between 1 + 2^-7 and 1 + 2^-6: ties to even -> 1 + 2^-6.
-    k = key_forward(*selection_net(sel, 1.0 + 3 * 2.0 ** -8), img)[0, 7, 7]
+    k = key_forward(*selection_net(sel, 1.0 + 3 * 2.0 ** -8, 0.0), img)[0, 7, 7]
     assert k == np.float32(0.25 * (1 + 2.0 ** -6))
     # 1 + 2^-8 + 2^-12 is above the half: rounds up to 1 + 2^-7.
-    k = key_forward(*selection_net(sel, 1.0 + 2.0 ** -8 + 2.0 ** -12), img)[0, 7, 7]
+    k = key_forward(*selection_net(sel, 1.0 + 2.0 ** -8 + 2.0 ** -12, 0.0), img)[0, 7, 7]
     assert k == np.float32(0.25 * (1 + 2.0 ** -7))
+
+
+def test_forward_rounds_hidden_units_to_bf16_nearest_even():
+    """The hidden units enter layer 2 in BF16: with b1 = 0.5 the hidden unit of x = 0.25 and
+    weight 1 + 2^-7 is 0.75 + 2^-9, halfway between the BF16 values 0.75 and 0.75 + 2^-8: ties to
+    even give 0.75, so the key is 0.25; weight 1 + 2^-6 gives 0.75 + 2^-8, exact."""
+    img = np.full((8, 8, 3), 0.75, np.float32)
+    sel = [0] * KO
+    k = key_forward(*selection_net(sel, 1.0 + 2.0 ** -7), img)[0, 7, 7]
+    assert k == np.float32(0.25)
+    k = key_forward(*selection_net(sel, 1.0 + 2.0 ** -6), img)[0, 7, 7]
+    assert k == np.float32(0.25 + 2.0 ** -8)
 
 
 def random_case(seed, nt=48, size=40):
