@@ -252,10 +252,19 @@ int pt_pfm_save(const char *path, const double *data, int h, int w, int c);
  * pt_batch_create: n RGB databases (device pointers, h[i] x w[i] x 3), built concurrently by
  *   `threads` host threads (0: 4), each table on its own stream; p is shared and borrowed.
  * pt_batch_lookup: the query (device, h x w x 3) against every table; ann_dev[i] receives the
- *   (h-7) x (w-7) x 3 NNF for database i (device). The query descriptor is computed once; each
- *   table's lookup runs on its own stream, replayed from a captured HIP graph after the first
- *   call with the same buffers. ms_per_pair[i] (optional): device time of pair i from events on
- *   its stream (pairs overlap); seconds (optional): wall time of the whole call. */
+ *   (h-7) x (w-7) x 3 NNF for database i (device). The query descriptor is computed once; then
+ *   each phase of the lookup (candidates, each propagation or MFMA round, output) is ONE launch
+ *   for all pairs (a grid of tiles x pairs, every pair's arguments in a device array), on the
+ *   batch's stream, replayed from one captured HIP graph after the first call with the same
+ *   buffers, tile width and parameters. Every pair's NNF equals its single-table lookup's.
+ *   ms_per_pair[i] (optional): device time of the whole batch (every pair ends with it);
+ *   seconds (optional): wall time of the whole call.
+ * pt_batch_set_tile / pt_batch_get_tile: the lattice tile of the batch's lookup kernels, 64 x 4,
+ *   32 x 8 or 16 x 16 patches (tile_x 64, 32, 16; results do not depend on it).
+ * pt_batch_autotune: times pt_batch_lookup of this query and these outputs with each tile width
+ *   (one eager run and capture, then `reps` replays each), keeps the fastest (median) and
+ *   returns it; ms_out[3] (optional): the medians for 64, 32 and 16. No counterpart in the
+ *   reference (an addition of this library). */
 typedef struct pt_batch pt_batch;
 int pt_batch_create(pt_params *p, int n, const float *const *b_dev, const int *h, const int *w, int threads,
                     pt_batch **out);
@@ -263,6 +272,10 @@ int pt_batch_lookup(pt_batch *bt, const float *a_dev, int h, int w, double *cons
                     double *seconds);
 int pt_batch_size(const pt_batch *bt);
 pt_table *pt_batch_table(pt_batch *bt, int i);
+int pt_batch_set_tile(pt_batch *bt, int tile_x);
+int pt_batch_get_tile(const pt_batch *bt);
+int pt_batch_autotune(pt_batch *bt, const float *a_dev, int h, int w, double *const *ann_dev, int reps,
+                      int *tile_x_out, double *ms_out);
 int pt_batch_destroy(pt_batch *bt);
 
 /* ---- Exhaustive nearest neighbours (the reference names LOOKUP_ALGO_BRUTE, patchtable.h:230,
@@ -371,6 +384,9 @@ int pt_table_stage_ms(const pt_table *t, double *ms /* [12] */);
  * workgroup and a copy of one int per workgroup per round back to the host. */
 int pt_table_set_mfma_stats(pt_table *t, int on);
 int pt_table_mfma_items(const pt_table *t, int64_t *items, int n, int *rounds);
+/* Lattice tile of this table's plain lookup kernels: 64 x 4 (default), 32 x 8 or 16 x 16 patches
+ * (tile_x 64, 32, 16); results do not depend on it. No counterpart in the reference. */
+int pt_table_set_tile(pt_table *t, int tile_x);
 
 /* ---- Stand-alone kernels (device pointers, stream 0 of the current device) ----------------
  * Used by the parity tests and the benchmark to exercise one kernel at a time. */

@@ -274,6 +274,11 @@ class PatchTable:
                                              dist.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return pos, dist
 
+    def set_tile(self, tile_x):
+        """Lattice tile of the plain lookup kernels: 64 x 4 (default), 32 x 8 or 16 x 16 patches
+        (tile_x 64, 32, 16); results do not depend on it."""
+        check(load().pt_table_set_tile(self._t, int(tile_x)))
+
     def set_mfma_stats(self, on=True):
         """Count the MFMA rounds' items in later lookups (instrumentation; see mfma_items)."""
         check(load().pt_table_set_mfma_stats(self._t, 1 if on else 0))
@@ -297,7 +302,8 @@ class PatchTable:
 
 class PatchTableBatch:
     """One query against many databases (torch CUDA tensors, HWC float32): one table per
-    database, built concurrently; lookups share the query descriptor and replay HIP graphs."""
+    database, built concurrently; a lookup computes the query descriptor once and runs each phase
+    as one launch over every pair, replayed from one captured HIP graph."""
 
     def __init__(self, p, databases, threads=4):
         import torch
@@ -338,6 +344,33 @@ class PatchTableBatch:
         secs = ctypes.c_double()
         check(load().pt_batch_lookup(self._bt, ctypes.c_void_p(a.data_ptr()), h, w, ptrs, ms, ctypes.byref(secs)))
         return anns, list(ms), secs.value
+
+    @property
+    def tile_x(self):
+        """Lattice tile width of the batch's lookup kernels (64 x 4, 32 x 8 or 16 x 16 patches)."""
+        return int(load().pt_batch_get_tile(self._bt))
+
+    @tile_x.setter
+    def tile_x(self, tx):
+        check(load().pt_batch_set_tile(self._bt, int(tx)))
+
+    def autotune(self, a, anns, reps=5):
+        """Times the batched lookup of query a into anns with each tile width and keeps the
+        fastest; returns {"tile_x": chosen, "median_ms": {64: .., 32: .., 16: ..}}."""
+        import torch
+        _check_dev(a, "a", torch.float32)
+        h, w = int(a.shape[0]), int(a.shape[1])
+        if len(anns) != self.n:
+            raise PatchTableError(f"expected {self.n} output fields, got {len(anns)}")
+        for i, x in enumerate(anns):
+            _check_dev(x, f"anns[{i}]", torch.float64, (h - 7, w - 7, 3), a.device)
+        _after_torch(a.device)
+        ptrs = (ctypes.c_void_p * self.n)(*[x.data_ptr() for x in anns])
+        tx = ctypes.c_int()
+        ms = (ctypes.c_double * 3)()
+        check(load().pt_batch_autotune(self._bt, ctypes.c_void_p(a.data_ptr()), h, w, ptrs, int(reps),
+                                       ctypes.byref(tx), ms))
+        return {"tile_x": tx.value, "median_ms": {64: ms[0], 32: ms[1], 16: ms[2]}}
 
     def close(self):
         if getattr(self, "_bt", None) and self._bt.value:

@@ -38,6 +38,7 @@
 #include "pt_tile.h"
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -52,22 +53,24 @@ enum { M_TILE = 0, M_GLOBAL = 1, M_DESC = 2 };   // where a lane's query patch i
 constexpr uint8_t F_DONE = 1, F_CHG = 2, F_KD = 4;
 
 // Floats of LDS that hold the block's query patches.
-template <int M>
+template <int M, int TX>
 struct QBuf {
-    static constexpr int F4 = M == M_TILE ? AX * AY : (M == M_DESC ? THREADS * PT_LOOKUP_CMAX / 4 : 1);
+    static constexpr int F4 =
+        M == M_TILE ? (TX + 7) * (THREADS / TX + 7) : (M == M_DESC ? THREADS * PT_LOOKUP_CMAX / 4 : 1);
 };
 
+template <int TX>
 __device__ __forceinline__ void lane_xy(const LookupArgs &L, int lane, int &x, int &y) {
     const int2 tc = tile_coords();
-    x = (tc.x * TILE_X + lane % TILE_X) * L.qs;
-    y = (tc.y * TILE_Y + lane / TILE_X) * L.qs;
+    x = (tc.x * TX + lane % TX) * L.qs;
+    y = (tc.y * (THREADS / TX) + lane / TX) * L.qs;
 }
 
 // Stages the block's query patches (every lane of the block must call this).
-template <int M>
+template <int M, int TX>
 __device__ __forceinline__ void stage_query(const LookupArgs &L, float4 *qbuf, bool valid, int x, int y) {
     if constexpr (M == M_TILE) {
-        stage_tile(L.a, L.ah, L.aw, qbuf);
+        stage_tile<TX>(L.a, L.ah, L.aw, qbuf);
     } else if constexpr (M == M_DESC) {
         float *qd = (float *)qbuf + threadIdx.x * L.C;
         const float *src = L.a_descI + ((size_t)(y + 7) * (L.aw + 7) + x + 7) * L.C;
@@ -78,14 +81,14 @@ __device__ __forceinline__ void stage_query(const LookupArgs &L, float4 *qbuf, b
 
 // Distance of lane `lane`'s query patch to database patch (xs, ys); with a finite bound the SSD
 // may stop early at a partial sum >= bound (then the result is >= bound, never a smaller value).
-template <int M>
+template <int M, int TX>
 __device__ __forceinline__ float qdist(const LookupArgs &L, const float4 *qbuf, int lane, int xs, int ys, float bound) {
     if constexpr (M == M_TILE) {
-        const float4 *aq = qbuf + (lane / TILE_X) * AX + (lane % TILE_X);
-        return ssd_staged<true>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, L.bw * 3, bound);
+        const float4 *aq = qbuf + (lane / TX) * (TX + 7) + (lane % TX);
+        return ssd_staged<true, TX + 7>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, L.bw * 3, bound);
     } else if constexpr (M == M_GLOBAL) {
         int x, y;
-        lane_xy(L, lane, x, y);
+        lane_xy<TX>(L, lane, x, y);
         return pt_patch_ssd_bounded(L.a + ((size_t)y * L.aw + x) * 3, L.aw * 3, L.b + ((size_t)ys * L.bw + xs) * 3,
                                     L.bw * 3, bound);
     } else {
@@ -178,7 +181,7 @@ __device__ __forceinline__ void q_push(BlockQueue &Q, bool pred, int slot, int c
 
 // Evaluates every queued task; on return the results are visible to all lanes and the queue is
 // empty again.
-template <int M>
+template <int M, int TX>
 __device__ __forceinline__ void q_run(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L) {
     __syncthreads();
     const int n = Q.n;
@@ -186,7 +189,7 @@ __device__ __forceinline__ void q_run(BlockQueue &Q, const float4 *qbuf, const L
         const int id = Q.id[k];
         const int lane = id / SLOTS;
         const int c = Q.cxy[id];
-        Q.res[id] = qdist<M>(L, qbuf, lane, PT_INT_TO_X(c), PT_INT_TO_Y(c), Q.bound[lane]);
+        Q.res[id] = qdist<M, TX>(L, qbuf, lane, PT_INT_TO_X(c), PT_INT_TO_Y(c), Q.bound[lane]);
     }
     __syncthreads();
     if (threadIdx.x == 0) Q.n = 0;
@@ -213,7 +216,7 @@ __device__ __forceinline__ Window spatial_window(const LookupArgs &L, int xb, in
 }
 
 // Sequential form, for spatial radius > 1 (more than SLOTS positions). Returns true if improved.
-template <int M>
+template <int M, int TX>
 __device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const float4 *qbuf, int &xb, int &yb, float &db,
                                                    Hist &hist) {
     const Window w = spatial_window(L, xb, yb);
@@ -222,7 +225,7 @@ __device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const fl
         for (int xs = w.xmin; xs < w.xmax; xs++) {
             if ((xs == w.xb0 && ys == w.yb0) || !allowed_at(L, xs, ys) || hist.has(PT_XY_TO_INT(xs, ys))) continue;
             hist.add(PT_XY_TO_INT(xs, ys));
-            float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, db);
+            float d = qdist<M, TX>(L, qbuf, threadIdx.x, xs, ys, db);
             if (d < db) {
                 db = d;
                 xb = xs;
@@ -239,9 +242,10 @@ __device__ __forceinline__ bool spatial_search_seq(const LookupArgs &L, const fl
 // loads per row instead of 6 per candidate row. Each candidate keeps the four lane-order
 // accumulators of ssd_staged in its order and stops once a partial sum after a row other than
 // the last reaches `bound`, so res[slot] is what qdist returns for it.
+template <int TX>
 __device__ __forceinline__ void spatial_local(const LookupArgs &L, const float4 *qbuf, int xb, int yb, float bound,
                                               unsigned todo, float res[8]) {
-    const float4 *aq = qbuf + (threadIdx.x / TILE_X) * AX + (threadIdx.x % TILE_X);
+    const float4 *aq = qbuf + (threadIdx.x / TX) * (TX + 7) + (threadIdx.x % TX);
 #pragma unroll
     for (int g = 0; g < 3; g++) {
         const int s0 = g == 0 ? 0 : (g == 1 ? 3 : 5);   // first slot of window row dy = g - 1
@@ -269,7 +273,7 @@ __device__ __forceinline__ void spatial_local(const LookupArgs &L, const float4 
             wv[28] = br[28];
             wv[29] = br[29];
             float av[24];
-            const float4 *ar = aq + r * AX;
+            const float4 *ar = aq + r * (TX + 7);
 #pragma unroll
             for (int px = 0; px < 8; px++) {
                 const float4 p = ar[px];
@@ -300,7 +304,7 @@ __device__ __forceinline__ void spatial_local(const LookupArgs &L, const float4 
 // lane of the block must call this. Returns true if improved. In image mode with query_step 1,
 // when most lanes of the wave search, a lane whose window lies inside b evaluates it alone
 // (spatial_local); the others go through the block queue.
-template <int M, bool LOCAL>
+template <int M, int TX, bool LOCAL>
 __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L, bool want,
                                                  int &xb, int &yb, float &db, Hist &hist) {
     Window w = spatial_window(L, xb, yb);
@@ -327,10 +331,10 @@ __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qb
             q_push(Q, in && !local, slot, PT_XY_TO_INT(xs, ys));
             slot++;
         }
-    q_run<M>(Q, qbuf, L);
+    q_run<M, TX>(Q, qbuf, L);
     float lres[8];
     if constexpr (M == M_TILE && LOCAL)
-        if (local && pushed) spatial_local(L, qbuf, xb, yb, db, pushed, lres);
+        if (local && pushed) spatial_local<TX>(L, qbuf, xb, yb, db, pushed, lres);
     bool changed = false;
     for (int s = 0; s < 8; s++) {
         if (!((pushed >> s) & 1u)) continue;
@@ -351,7 +355,7 @@ __device__ __forceinline__ bool spatial_search_q(BlockQueue &Q, const float4 *qb
 // kd >= triangle_factor * dbest (patchtable.h:3779-3784). kd is sorted, so per chunk of SLOTS
 // entries the queued superset is those with kd below the bound at the chunk's start, and the
 // in-order reduce replays the early-out exactly. Every lane of the block must call this.
-template <int M>
+template <int M, int TX>
 __device__ __forceinline__ bool kcoh_walk_q(BlockQueue &Q, const float4 *qbuf, const LookupArgs &L, bool want, int &xb,
                                             int &yb, float &db, Hist &hist) {
     const int bew = L.bw - 7;
@@ -369,7 +373,7 @@ __device__ __forceinline__ bool kcoh_walk_q(BlockQueue &Q, const float4 *qbuf, c
             if (pred) pushed |= 1u << s;
             q_push(Q, pred, s, pos);
         }
-        q_run<M>(Q, qbuf, L);
+        q_run<M, TX>(Q, qbuf, L);
         for (int s = 0; s < SLOTS && !broken; s++) {
             const int k = k0 + s;
             if (k >= L.K) break;
@@ -406,13 +410,13 @@ __device__ __forceinline__ int rs_pick(uint32_t i, uint32_t round) {
     return (int)(h & 7u);
 }
 
-template <int M>
+template <int M, int TX>
 __device__ __forceinline__ bool random_search(const LookupArgs &L, const float4 *qbuf, long pi, int round, int &xb,
                                               int &yb, float &db) {
     const int r = rs_pick((uint32_t)pi, (uint32_t)round);
     const int xs = xb + c_dx[r], ys = yb + c_dy[r];
     if (xs < 0 || ys < 0 || xs >= L.bw - 7 || ys >= L.bh - 7 || !allowed_at(L, xs, ys)) return false;
-    const float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, db);
+    const float d = qdist<M, TX>(L, qbuf, threadIdx.x, xs, ys, db);
     if (d < db) {
         db = d;
         xb = xs;
@@ -433,16 +437,16 @@ __device__ __forceinline__ int table_cell_value(const LookupArgs &L, const float
     return table[cell];
 }
 
-template <int M>
-__global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField out) {
-    __shared__ float4 qbuf[QBuf<M>::F4];
+template <int M, int TX>
+__device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField out) {
+    __shared__ float4 qbuf[QBuf<M, TX>::F4];
     __shared__ BlockQueue Q;
     if (threadIdx.x == 0) Q.n = 0;
     const int2 tc = tile_coords();
-    const int vx = tc.x * TILE_X + threadIdx.x % TILE_X, vy = tc.y * TILE_Y + threadIdx.x / TILE_X;
+    const int vx = tc.x * TX + threadIdx.x % TX, vy = tc.y * (THREADS / TX) + threadIdx.x / TX;
     const bool valid = vx < L.vw && vy < L.vh;
     const int x = vx * L.qs, y = vy * L.qs;
-    stage_query<M>(L, qbuf, valid, x, y);
+    stage_query<M, TX>(L, qbuf, valid, x, y);
     const long pi = (long)y * L.ann_w + x;
     bool have = false;
     int xb = 0, yb = 0;
@@ -460,7 +464,7 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
         const int xs = PT_INT_TO_X(v) - 7, ys = PT_INT_TO_Y(v) - 7;
         if (!have || xs != xb || ys != yb) {
             hist.add(PT_XY_TO_INT(xs, ys));
-            const float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, have ? db : INFINITY);
+            const float d = qdist<M, TX>(L, qbuf, threadIdx.x, xs, ys, have ? db : INFINITY);
             if (!have || d < db) {
                 db = d;
                 xb = xs;
@@ -474,7 +478,7 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
             const int x1 = PT_INT_TO_X(v1) - 7, y1 = PT_INT_TO_Y(v1) - 7;
             if ((x1 != xb || y1 != yb) && (x1 != xp || y1 != yp)) {
                 hist.add(PT_XY_TO_INT(x1, y1));
-                const float d = qdist<M>(L, qbuf, threadIdx.x, x1, y1, db);
+                const float d = qdist<M, TX>(L, qbuf, threadIdx.x, x1, y1, db);
                 if (d < db) {
                     db = d;
                     xb = x1;
@@ -486,18 +490,18 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
         }
     }
     bool kd = false;
-    if (L.K > 0 && L.kcoh_iter0) kd = !kcoh_walk_q<M>(Q, qbuf, L, valid, xb, yb, db, hist);
+    if (L.K > 0 && L.kcoh_iter0) kd = !kcoh_walk_q<M, TX>(Q, qbuf, L, valid, xb, yb, db, hist);
     bool done = false;
     if (L.spatial_cand && L.spatial == 1) {
-        const bool changed = spatial_search_q<M, false>(Q, qbuf, L, valid, xb, yb, db, hist);
+        const bool changed = spatial_search_q<M, TX, false>(Q, qbuf, L, valid, xb, yb, db, hist);
         done = !changed;
         if (changed) kd = false;
     } else if (L.spatial_cand && L.spatial > 1 && valid) {
-        const bool changed = spatial_search_seq<M>(L, qbuf, xb, yb, db, hist);
+        const bool changed = spatial_search_seq<M, TX>(L, qbuf, xb, yb, db, hist);
         done = !changed;
         if (changed) kd = false;
     } else if (L.rs && valid) {
-        if (random_search<M>(L, qbuf, pi, 0, xb, yb, db)) kd = false;
+        if (random_search<M, TX>(L, qbuf, pi, 0, xb, yb, db)) kd = false;
     }
     if (!valid) return;
     const long v = (long)vy * L.vw + vx;
@@ -513,17 +517,17 @@ __global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField o
 // same_jump: the previous round used the same jump. A candidate from neighbour n then repeats
 // that round's when neither this patch nor n changed in it; it was rejected against the same
 // best, so it is skipped (results unchanged).
-template <int M>
-__global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int same_jump, int round, NnfField cur,
-                                                      NnfField nxt, int keep_hist) {
-    __shared__ float4 qbuf[QBuf<M>::F4];
+template <int M, int TX>
+__device__ __forceinline__ void propagate_body(const LookupArgs &L, int J, int same_jump, int round, NnfField cur,
+                                               NnfField nxt, int keep_hist) {
+    __shared__ float4 qbuf[QBuf<M, TX>::F4];
     __shared__ BlockQueue Q;
     if (threadIdx.x == 0) Q.n = 0;
     const int2 tc = tile_coords();
-    const int vx = tc.x * TILE_X + threadIdx.x % TILE_X, vy = tc.y * TILE_Y + threadIdx.x / TILE_X;
+    const int vx = tc.x * TX + threadIdx.x % TX, vy = tc.y * (THREADS / TX) + threadIdx.x / TX;
     const bool valid = vx < L.vw && vy < L.vh;
     const int x = vx * L.qs, y = vy * L.qs;
-    stage_query<M>(L, qbuf, valid, x, y);
+    stage_query<M, TX>(L, qbuf, valid, x, y);
     const int bew = L.bw - 7, beh = L.bh - 7;
     const long v = (long)vy * L.vw + vx;
     const int js = J * L.qs;   // jump in patch units
@@ -566,7 +570,7 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
         }
         q_push(Q, pred, t, PT_XY_TO_INT(xs, ys));
     }
-    q_run<M>(Q, qbuf, L);
+    q_run<M, TX>(Q, qbuf, L);
     bool moved = false;
     for (int t = 0; t < 8; t++) {
         if (!((pushed >> t) & 1u)) continue;
@@ -583,20 +587,20 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
     bool kchg = false;
     if (L.kcoh_rounds) {
         const bool want = valid && !kd;
-        kchg = kcoh_walk_q<M>(Q, qbuf, L, want, xb, yb, db, hist);
+        kchg = kcoh_walk_q<M, TX>(Q, qbuf, L, want, xb, yb, db, hist);
         if (want) kd = !kchg;
     }
     bool done = was_done && !moved && !kchg;
     bool schg = false;
     const bool sp = round >= L.spatial_from;   // rounds that run the spatial window
     if (sp && L.spatial == 1) {
-        schg = spatial_search_q<M, true>(Q, qbuf, L, valid && !done, xb, yb, db, hist);
+        schg = spatial_search_q<M, TX, true>(Q, qbuf, L, valid && !done, xb, yb, db, hist);
         if (valid && !done) done = !schg;
     } else if (sp && L.spatial > 1 && valid && !done) {
-        schg = spatial_search_seq<M>(L, qbuf, xb, yb, db, hist);
+        schg = spatial_search_seq<M, TX>(L, qbuf, xb, yb, db, hist);
         done = !schg;
     } else if (L.rs && valid) {
-        schg = random_search<M>(L, qbuf, (long)y * L.ann_w + x, round, xb, yb, db);
+        schg = random_search<M, TX>(L, qbuf, (long)y * L.ann_w + x, round, xb, yb, db);
     }
     if (schg) kd = false;
     if (!valid) return;
@@ -609,14 +613,14 @@ __global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int 
 // query_step > 1: every patch takes the best offset of the lattice patches within query_step
 // (clamped into b, allowed positions only), starting from its initial match; a patch with no
 // allowed proposal takes its lattice owner's match. Launched over all patches (qs = 1 layout).
-template <int M>
-__global__ __launch_bounds__(THREADS) void k_gather(LookupArgs L, NnfField lat, int32_t *__restrict__ oxy,
-                                                    float *__restrict__ od) {
-    __shared__ float4 qbuf[QBuf<M>::F4];
+template <int M, int TX>
+__device__ __forceinline__ void gather_body(const LookupArgs &L, NnfField lat, int32_t *__restrict__ oxy,
+                                            float *__restrict__ od) {
+    __shared__ float4 qbuf[QBuf<M, TX>::F4];
     const int2 tc = tile_coords();
-    const int x = tc.x * TILE_X + threadIdx.x % TILE_X, y = tc.y * TILE_Y + threadIdx.x / TILE_X;
+    const int x = tc.x * TX + threadIdx.x % TX, y = tc.y * (THREADS / TX) + threadIdx.x / TX;
     const bool valid = x < L.ann_w && y < L.ann_h;
-    stage_query<M>(L, qbuf, valid, x, y);
+    stage_query<M, TX>(L, qbuf, valid, x, y);
     if (!valid) return;
     const int bew = L.bw - 7, beh = L.bh - 7, qs = L.gqs;
     const long pi = (long)y * L.ann_w + x;
@@ -640,7 +644,7 @@ __global__ __launch_bounds__(THREADS) void k_gather(LookupArgs L, NnfField lat, 
             ys = ys < 0 ? 0 : (ys >= beh ? beh - 1 : ys);
             if (!allowed_at(L, xs, ys)) continue;
             if (have && xs == xb && ys == yb) continue;
-            const float d = qdist<M>(L, qbuf, threadIdx.x, xs, ys, have ? db : INFINITY);
+            const float d = qdist<M, TX>(L, qbuf, threadIdx.x, xs, ys, have ? db : INFINITY);
             if (!have || d < db) {
                 db = d;
                 xb = xs;
@@ -652,10 +656,44 @@ __global__ __launch_bounds__(THREADS) void k_gather(LookupArgs L, NnfField lat, 
         const int c = lat.xy[(long)(y / qs) * L.vw + x / qs];
         xb = PT_INT_TO_X(c);
         yb = PT_INT_TO_Y(c);
-        db = qdist<M>(L, qbuf, threadIdx.x, xb, yb, INFINITY);
+        db = qdist<M, TX>(L, qbuf, threadIdx.x, xb, yb, INFINITY);
     }
     oxy[pi] = PT_XY_TO_INT(xb, yb);
     od[pi] = db;
+}
+
+// Kernels of one lookup (arguments by value) and of a batch of lookups of one query against many
+// tables (blockIdx.z = pair; each pair's arguments and fields in a device array). A pair's lanes
+// compute exactly what its single lookup computes.
+template <int M, int TX>
+__global__ __launch_bounds__(THREADS) void k_candidates(LookupArgs L, NnfField out) {
+    candidates_body<M, TX>(L, out);
+}
+template <int M, int TX>
+__global__ __launch_bounds__(THREADS) void k_propagate(LookupArgs L, int J, int same_jump, int round, NnfField cur,
+                                                      NnfField nxt, int keep_hist) {
+    propagate_body<M, TX>(L, J, same_jump, round, cur, nxt, keep_hist);
+}
+template <int M, int TX>
+__global__ __launch_bounds__(THREADS) void k_gather(LookupArgs L, NnfField lat, int32_t *__restrict__ oxy,
+                                                    float *__restrict__ od) {
+    gather_body<M, TX>(L, lat, oxy, od);
+}
+template <int M, int TX>
+__global__ __launch_bounds__(THREADS) void k_candidates_batch(const PairLookup *__restrict__ P) {
+    const PairLookup &p = P[blockIdx.z];
+    candidates_body<M, TX>(p.L, p.f[0]);
+}
+template <int M, int TX>
+__global__ __launch_bounds__(THREADS) void k_propagate_batch(const PairLookup *__restrict__ P, int J, int same_jump,
+                                                            int round, int src, int keep_hist) {
+    const PairLookup &p = P[blockIdx.z];
+    propagate_body<M, TX>(p.L, J, same_jump, round, p.f[src], p.f[src ^ 1], keep_hist);
+}
+template <int M, int TX>
+__global__ __launch_bounds__(THREADS) void k_gather_batch(const PairLookup *__restrict__ P, int src) {
+    const PairLookup &p = P[blockIdx.z];
+    gather_body<M, TX>(p.G, p.f[src], p.gxy, p.gd);
 }
 
 // Initial field from a previous one (TABLE_COPY_PREV_NNF and patchtable.h:3539-3585).
@@ -702,6 +740,19 @@ __global__ void k_write_ann(const int32_t *__restrict__ xy, const float *__restr
     ann[i * 3 + 2] = dd;
 }
 
+// NNF output of every pair of a batch (no previous field): blockIdx.y = pair; field `src`, or the
+// gathered field with query_step > 1.
+__global__ void k_write_ann_batch(const PairLookup *__restrict__ P, int src, int gathered, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PairLookup &p = P[blockIdx.y];
+    const int v = gathered ? p.gxy[i] : p.f[src].xy[i];
+    const float d = gathered ? p.gd[i] : p.f[src].d[i];
+    p.ann[i * 3 + 0] = (double)PT_INT_TO_X(v);
+    p.ann[i * 3 + 1] = (double)PT_INT_TO_Y(v);
+    p.ann[i * 3 + 2] = (double)d;
+}
+
 // Descriptor-mode input: an h_in x w_in x C descriptor image placed at rows/columns 7.. of the
 // padded (Hp x Wp) descriptor (get_padded_descriptor, patchtable.h:2498-2507), or a padded input
 // copied as is; written both as planes (quantiser, table cells) and interleaved (distances).
@@ -734,18 +785,18 @@ __global__ void k_desc_dist_pairs(const float *a, const float *b, int C, int n, 
     out[i] = pt_desc_dist(a + (size_t)i * C, b + (size_t)i * C, C);
 }
 
-template <int M>
+template <int M, int TX>
 int launch_lookup(const LookupArgs &L, NnfField f0, NnfField f1, hipStream_t s, hipEvent_t ev_after_candidates,
                   int &launches) {
-    dim3 grid((L.vw + TILE_X - 1) / TILE_X, (L.vh + TILE_Y - 1) / TILE_Y);
-    hipLaunchKernelGGL(k_candidates<M>, grid, dim3(THREADS), 0, s, L, f0);
+    dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX));
+    hipLaunchKernelGGL((k_candidates<M, TX>), grid, dim3(THREADS), 0, s, L, f0);
     if (ev_after_candidates) PT_HIP_TRY(hipEventRecord(ev_after_candidates, s));
     const int R = L.rounds;
     for (int r = 0; r < R; r++) {
         const int J = pt_jump_at(R, r);
         const int same = r > 0 && pt_jump_at(R, r - 1) == J;
         const int keep = r + 1 < R;   // the last round's history is not read again
-        hipLaunchKernelGGL(k_propagate<M>, grid, dim3(THREADS), 0, s, L, J, same, r + 1, f0, f1, keep);
+        hipLaunchKernelGGL((k_propagate<M, TX>), grid, dim3(THREADS), 0, s, L, J, same, r + 1, f0, f1, keep);
         std::swap(f0, f1);
     }
     launches = R;
@@ -753,7 +804,58 @@ int launch_lookup(const LookupArgs &L, NnfField f0, NnfField f1, hipStream_t s, 
     return R % 2 ? 1 : 0;   // which field holds the result
 }
 
+template <int M, int TX>
+void launch_candidates(const LookupArgs &L, NnfField f0, hipStream_t s) {
+    dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX));
+    hipLaunchKernelGGL((k_candidates<M, TX>), grid, dim3(THREADS), 0, s, L, f0);
+}
+
+// Phases of a batch: the candidates, the rounds and the gather of every pair, one launch each.
+template <int M, int TX>
+int launch_batch(const PairLookup *P, const PairLookup &P0, int npairs, hipStream_t s, int &src, int &launches) {
+    const LookupArgs &L = P0.L;
+    dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX), npairs);
+    hipLaunchKernelGGL((k_candidates_batch<M, TX>), grid, dim3(THREADS), 0, s, P);
+    launches++;
+    const int R = L.rounds;
+    src = 0;
+    for (int r = 0; r < R; r++) {
+        const int J = pt_jump_at(R, r);
+        const int same = r > 0 && pt_jump_at(R, r - 1) == J;
+        const int keep = r + 1 < R;
+        hipLaunchKernelGGL((k_propagate_batch<M, TX>), grid, dim3(THREADS), 0, s, P, J, same, r + 1, src, keep);
+        src ^= 1;
+        launches++;
+    }
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+template <int M, int TX>
+void launch_gather(const PairLookup *P, const LookupArgs &G, int npairs, int src, hipStream_t s) {
+    dim3 grid((G.ann_w + TX - 1) / TX, (G.ann_h + THREADS / TX - 1) / (THREADS / TX), npairs);
+    hipLaunchKernelGGL((k_gather_batch<M, TX>), grid, dim3(THREADS), 0, s, P, src);
+}
+
+template <int M, int TX>
+void launch_gather1(const LookupArgs &G, NnfField res, int32_t *gxy, float *gd, hipStream_t s) {
+    dim3 grid((G.ann_w + TX - 1) / TX, (G.ann_h + THREADS / TX - 1) / (THREADS / TX));
+    hipLaunchKernelGGL((k_gather<M, TX>), grid, dim3(THREADS), 0, s, G, res, gxy, gd);
+}
+
+// Calls f(std::integral_constant<int, TX>) for the tile width `tile_x` (64, 32 or 16; descriptor
+// mode always 64: its LDS holds one descriptor per lane whatever the shape).
+template <class F>
+int with_tile(int tile_x, int C, F &&f) {
+    if (C > 0 || tile_x == 64) return f(std::integral_constant<int, 64>{});
+    if (tile_x == 32) return f(std::integral_constant<int, 32>{});
+    if (tile_x == 16) return f(std::integral_constant<int, 16>{});
+    return pt_failf(PT_E_INVALID, "tile width %d is not one of 64, 32, 16", tile_x);
+}
+
 }  // namespace
+
+int pt_lookup_tile_ok(int tile_x) { return tile_x == 64 || tile_x == 32 || tile_x == 16; }
 
 int pt_lookup_init_prev(const LookupArgs &L, const double *prev, const float *sv, int recalc, double temporal_scale,
                         int sanitize, int32_t *ixy, float *id, int *bad_dev, hipStream_t s) {
@@ -770,12 +872,15 @@ int pt_lookup_init_prev(const LookupArgs &L, const double *prev, const float *sv
 
 int pt_lookup_candidates(const LookupArgs &L, NnfField f0, hipStream_t s) {
     if (L.C > PT_LOOKUP_CMAX) return pt_failf(PT_E_UNSUPPORTED, "descriptor dims %d > %d", L.C, PT_LOOKUP_CMAX);
-    dim3 grid((L.vw + TILE_X - 1) / TILE_X, (L.vh + TILE_Y - 1) / TILE_Y);
-    if (L.C > 0) hipLaunchKernelGGL(k_candidates<M_DESC>, grid, dim3(THREADS), 0, s, L, f0);
-    else if (L.qs == 1) hipLaunchKernelGGL(k_candidates<M_TILE>, grid, dim3(THREADS), 0, s, L, f0);
-    else hipLaunchKernelGGL(k_candidates<M_GLOBAL>, grid, dim3(THREADS), 0, s, L, f0);
-    PT_LAUNCH_CHECK();
-    return PT_OK;
+    return with_tile(L.tile_x, L.C, [&](auto tx_) {
+        constexpr int TX = decltype(tx_)::value;
+        if (L.C > 0) {
+            if constexpr (TX == 64) launch_candidates<M_DESC, TX>(L, f0, s);
+        } else if (L.qs == 1) launch_candidates<M_TILE, TX>(L, f0, s);
+        else launch_candidates<M_GLOBAL, TX>(L, f0, s);
+        PT_LAUNCH_CHECK();
+        return PT_OK;
+    });
 }
 
 int pt_lookup_output(const LookupArgs &L, NnfField res, int32_t *gxy, float *gd, double *ann_out, const double *prev,
@@ -788,9 +893,15 @@ int pt_lookup_output(const LookupArgs &L, NnfField res, int32_t *gxy, float *gd,
         LookupArgs G = L;
         G.gqs = L.qs;
         G.qs = 1;   // k_gather lanes are single patches; stage_query / qdist use qs 1
-        dim3 grid((L.ann_w + TILE_X - 1) / TILE_X, (L.ann_h + TILE_Y - 1) / TILE_Y);
-        if (L.C > 0) hipLaunchKernelGGL(k_gather<M_DESC>, grid, dim3(THREADS), 0, s, G, res, gxy, gd);
-        else hipLaunchKernelGGL(k_gather<M_TILE>, grid, dim3(THREADS), 0, s, G, res, gxy, gd);
+        TRY_PT(with_tile(L.tile_x, L.C, [&](auto tx_) {
+            constexpr int TX = decltype(tx_)::value;
+            if (L.C > 0) {
+                if constexpr (TX == 64) launch_gather1<M_DESC, TX>(G, res, gxy, gd, s);
+            } else {
+                launch_gather1<M_TILE, TX>(G, res, gxy, gd, s);
+            }
+            return PT_OK;
+        }));
         oxy = gxy;
         od = gd;
         if (launches) (*launches)++;
@@ -807,15 +918,67 @@ int pt_lookup_launch(const LookupArgs &L, NnfField f0, NnfField f1, int32_t *gxy
                      hipEvent_t ev_after_candidates, hipEvent_t ev_after_rounds, int *launches) {
     if (L.C > PT_LOOKUP_CMAX) return pt_failf(PT_E_UNSUPPORTED, "descriptor dims %d > %d", L.C, PT_LOOKUP_CMAX);
     int which = 0, nl = 0;
-    if (L.C > 0) which = launch_lookup<M_DESC>(L, f0, f1, s, ev_after_candidates, nl);
-    else if (L.qs == 1) which = launch_lookup<M_TILE>(L, f0, f1, s, ev_after_candidates, nl);
-    else which = launch_lookup<M_GLOBAL>(L, f0, f1, s, ev_after_candidates, nl);
-    if (which < 0) return which;
+    TRY_PT(with_tile(L.tile_x, L.C, [&](auto tx_) {
+        constexpr int TX = decltype(tx_)::value;
+        if (L.C > 0) {
+            if constexpr (TX == 64) which = launch_lookup<M_DESC, TX>(L, f0, f1, s, ev_after_candidates, nl);
+        } else if (L.qs == 1) which = launch_lookup<M_TILE, TX>(L, f0, f1, s, ev_after_candidates, nl);
+        else which = launch_lookup<M_GLOBAL, TX>(L, f0, f1, s, ev_after_candidates, nl);
+        return which < 0 ? which : PT_OK;
+    }));
     NnfField res = which ? f1 : f0;
     const int rc = pt_lookup_output(L, res, gxy, gd, ann_out, prev, init_d, temporal_scale, temporal_fix, s, &nl,
                                     ev_after_rounds);
     if (rc != PT_OK) return rc;
     if (launches) *launches = nl;
+    return PT_OK;
+}
+
+int pt_lookup_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, hipStream_t s,
+                           hipEvent_t ev_after_rounds, int *launches) {
+    const LookupArgs &L = P0.L;
+    if (L.C > 0) return pt_fail(PT_E_UNSUPPORTED, "batched lookups take images, not descriptors");
+    if (npairs < 1 || npairs > 65535) return pt_failf(PT_E_INVALID, "batch of %d pairs", npairs);
+    int src = 0, nl = 0;
+    TRY_PT(with_tile(L.tile_x, 0, [&](auto tx_) {
+        constexpr int TX = decltype(tx_)::value;
+        if (L.qs == 1) return launch_batch<M_TILE, TX>(P_dev, P0, npairs, s, src, nl);
+        return launch_batch<M_GLOBAL, TX>(P_dev, P0, npairs, s, src, nl);
+    }));
+    TRY_PT(pt_lookup_batch_output(P_dev, P0, npairs, src, s, ev_after_rounds, &nl));
+    if (launches) *launches = nl;
+    return PT_OK;
+}
+
+int pt_lookup_batch_candidates(const PairLookup *P_dev, const PairLookup &P0, int npairs, hipStream_t s) {
+    const LookupArgs &L = P0.L;
+    if (L.C > 0) return pt_fail(PT_E_UNSUPPORTED, "batched lookups take images, not descriptors");
+    return with_tile(L.tile_x, 0, [&](auto tx_) {
+        constexpr int TX = decltype(tx_)::value;
+        dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX), npairs);
+        if (L.qs == 1) hipLaunchKernelGGL((k_candidates_batch<M_TILE, TX>), grid, dim3(THREADS), 0, s, P_dev);
+        else hipLaunchKernelGGL((k_candidates_batch<M_GLOBAL, TX>), grid, dim3(THREADS), 0, s, P_dev);
+        PT_LAUNCH_CHECK();
+        return PT_OK;
+    });
+}
+
+int pt_lookup_batch_output(const PairLookup *P_dev, const PairLookup &P0, int npairs, int src, hipStream_t s,
+                           hipEvent_t ev_before_write, int *launches) {
+    const LookupArgs &L = P0.L;
+    const int gathered = L.qs > 1;
+    if (gathered) {
+        TRY_PT(with_tile(L.tile_x, 0, [&](auto tx_) {
+            constexpr int TX = decltype(tx_)::value;
+            launch_gather<M_TILE, TX>(P_dev, P0.G, npairs, src, s);
+            return PT_OK;
+        }));
+        if (launches) (*launches)++;
+    }
+    if (ev_before_write) PT_HIP_TRY(hipEventRecord(ev_before_write, s));
+    const long n = (long)L.ann_w * L.ann_h;
+    hipLaunchKernelGGL(k_write_ann_batch, dim3(pt_blocks(n, 256), npairs), dim3(256), 0, s, P_dev, src, gathered, n);
+    PT_LAUNCH_CHECK();
     return PT_OK;
 }
 

@@ -194,7 +194,7 @@ __device__ __forceinline__ void band_fetch(const MfmaArgs &M, int lane, int x0, 
 // own items are done (its row's item count varies with the image).
 constexpr int MW = 64, MQY = 1 + 7;   // threads; query rows staged (the row's patches)
 
-__global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J) {
+__device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur, NnfField nxt, int J) {
     __shared__ uint32_t qq[AX * MQY];   // int8 RGB0 query pixels (A fragments)
     // the hash tables, source offsets and anchors of phase 1, then the two band buffers of phase 2
     __shared__ uint32_t wsh[1][WSH];
@@ -496,6 +496,16 @@ __global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, 
     nxt.fl[vi] = 0;
 }
 
+__global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J) {
+    mfma_round_body(M, cur, nxt, J);
+}
+
+// One round of every pair of a batch (blockIdx.z = pair), from field src into field src ^ 1.
+__global__ __launch_bounds__(MW, 4) void k_mfma_round_batch(const PairLookup *__restrict__ P, int src, int J) {
+    const PairLookup &p = P[blockIdx.z];
+    mfma_round_body(p.M, p.f[src], p.f[src ^ 1], J);
+}
+
 }  // namespace
 
 int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t s) {
@@ -528,5 +538,22 @@ int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f
     }
     PT_LAUNCH_CHECK();
     *which = rounds % 2;
+    return PT_OK;
+}
+
+int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, int rounds, hipStream_t s,
+                                int *which) {
+    const MfmaArgs &M = P0.M;
+    if (M.nanchors < 1 || M.nanchors > PT_MFMA_MAX_ANCHORS)
+        return pt_failf(PT_E_INVALID, "mfma_anchors must be in 1..%d", PT_MFMA_MAX_ANCHORS);
+    if (M.items_out) return pt_fail(PT_E_INVALID, "batched MFMA rounds keep no item counts");
+    dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, M.ann_h, npairs);
+    int src = 0;
+    for (int r = 0; r < rounds; r++) {
+        hipLaunchKernelGGL(k_mfma_round_batch, grid, dim3(MW), 0, s, P_dev, src, pt_mfma_jump_at(rounds, r));
+        src ^= 1;
+    }
+    PT_LAUNCH_CHECK();
+    *which = src;
     return PT_OK;
 }

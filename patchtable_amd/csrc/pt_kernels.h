@@ -205,6 +205,8 @@ struct LookupArgs {
     const float *init_d = nullptr;
     int32_t *hist = nullptr;        // per lattice patch: candidates evaluated so far (16 x nv, entry-major)
     uint8_t *hn = nullptr;          // ... and how many were recorded
+    int tile_x = 64;                // lattice tile of a workgroup: 64 x 4, 32 x 8 or 16 x 16 patches
+                                    // (image mode; results do not depend on it)
 };
 // One nearest-neighbour field over the lattice: XY_TO_INT(b patch x, y), distance, and flags
 // (1: the spatial window around the match was searched without improvement; 2: the match changed
@@ -254,6 +256,32 @@ int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t 
 int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, int32_t *bnorm, hipStream_t s);
 // Runs `rounds` rounds from f0; *which = 1 if the result is in f1.
 int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f1, hipStream_t s, int *which);
+// One pair of a batch (one query against many tables), read by the batched kernels from a device
+// array indexed by the pair: the lookup's arguments (L; G: the same for the query_step > 1 gather,
+// whose lanes are single patches), its two fields, its gathered field and its NNF output.
+struct PairLookup {
+    LookupArgs L, G;
+    MfmaArgs M;
+    NnfField f[2];
+    int32_t *gxy = nullptr;
+    float *gd = nullptr;
+    double *ann = nullptr;
+};
+// Candidates, propagation rounds, gather and output of every pair of a batch, one launch per phase
+// (P_dev: device copy of the pairs; P0: host copy of pair 0, whose shape and parameters every pair
+// shares): lookup.hip.
+int pt_lookup_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, hipStream_t s,
+                           hipEvent_t ev_after_rounds, int *launches);
+// The output phase alone, from field `src` of every pair.
+int pt_lookup_batch_output(const PairLookup *P_dev, const PairLookup &P0, int npairs, int src, hipStream_t s,
+                           hipEvent_t ev_before_write, int *launches);
+// The table candidates of every pair (MFMA batches): field 0 of each pair.
+int pt_lookup_batch_candidates(const PairLookup *P_dev, const PairLookup &P0, int npairs, hipStream_t s);
+// MFMA rounds of every pair of a batch, one launch per round; *which = the field with the result.
+int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, int rounds, hipStream_t s,
+                                int *which);
+// Tile widths the lookup kernels are built for.
+int pt_lookup_tile_ok(int tile_x);
 // Output stage of a lookup (gather for query_step > 1, then the double field): lookup.hip.
 int pt_lookup_output(const LookupArgs &L, NnfField res, int32_t *gxy, float *gd, double *ann_out, const double *prev,
                      const float *init_d, double temporal_scale, int temporal_fix, hipStream_t s, int *launches,

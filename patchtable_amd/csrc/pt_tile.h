@@ -25,12 +25,15 @@ __device__ __forceinline__ int2 tile_coords() {
     return make_int2((lin % 8) * band + j % band, j / band);
 }
 
-// Stages the pixels of this block's tile of patches into LDS, zero outside the image.
+// Stages the pixels of this block's tile of patches into LDS, zero outside the image. The tile
+// is TX x (THREADS / TX) patches, (TX + 7) x (THREADS / TX + 7) pixels.
+template <int TX = TILE_X>
 __device__ __forceinline__ void stage_tile(const float *__restrict__ img, int ih, int iw, float4 *tile) {
+    constexpr int TY = THREADS / TX, SX = TX + 7, SY = TY + 7;
     const int2 tc = tile_coords();
-    const int x0 = tc.x * TILE_X, y0 = tc.y * TILE_Y;
-    for (int i = threadIdx.x; i < AX * AY; i += THREADS) {
-        int ly = i / AX, lx = i - ly * AX;
+    const int x0 = tc.x * TX, y0 = tc.y * TY;
+    for (int i = threadIdx.x; i < SX * SY; i += THREADS) {
+        int ly = i / SX, lx = i - ly * SX;
         int gx = x0 + lx, gy = y0 + ly;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (gx < iw && gy < ih) {
@@ -59,18 +62,18 @@ __device__ __forceinline__ int4 ldg4i(const int32_t *p) {
         acc[(e) & 3] = acc[(e) & 3] + d_ * d_;         \
     }
 
-// Exact SSD of the staged query patch at `aq` (row pitch AX pixels) against the database patch
+// Exact SSD of the staged query patch at `aq` (row pitch PITCH pixels) against the database patch
 // at `b` (row pitch `bpitch` floats), lane order of patch_dist_exact (patchtable.h:330-373):
 // element e of a 24-float row goes to lane e & 3, lanes accumulate in increasing e then row.
 // With `bounded`, the sum is abandoned once a partial sum (after any row but the last) reaches
 // `bound`.
 // Rows are processed one at a time (83 VGPRs, 5 waves per SIMD).
-template <bool bounded>
+template <bool bounded, int PITCH = AX>
 __device__ __forceinline__ float ssd_staged(const float4 *aq, const float *__restrict__ b, int bpitch, float bound) {
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
     for (int dy = 0; dy < 8; dy++) {
-        const float4 *ar = aq + dy * AX;
+        const float4 *ar = aq + dy * PITCH;
         const float *br = b + (size_t)dy * bpitch;
         float4 b0 = ldg4(br), b1 = ldg4(br + 4), b2 = ldg4(br + 8), b3 = ldg4(br + 12), b4 = ldg4(br + 16),
                b5 = ldg4(br + 20);

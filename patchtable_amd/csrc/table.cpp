@@ -101,6 +101,7 @@ struct pt_table {
     DevBuf<int32_t> hist;   // lookup: candidates evaluated per lattice patch (PT_LOOKUP_HIST each)
     DevBuf<uint8_t> hn;
     int qh = 0, qw = 0;
+    int tile_x = 64;        // lookup kernels' lattice tile: 64 x 4, 32 x 8 or 16 x 16 patches
     PtScratch sc;
     // learned-key table: the key network (copied at build) indexes the table; extra[0] is the GCK
     // fallback table, indexed by the query's GCK planes (xadesc)
@@ -724,13 +725,15 @@ int query_shape(const pt_table *t, int h, int w, int c, int &ah, int &aw) {
 // caller already computed them (same parameters), else NULL. timing: record the stage events
 // (off while a graph is being captured). ws: buffers to work in (no allocation), else the
 // table's own, grown as needed.
-int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, const double *prev_dev,
-                      const float *sv_dev, const float *adesc_pre, bool timing, int *launches_out,
-                      const LookupBufs *ws) {
-    const pt_params &p = *t->p;
-    int ah = 0, aw = 0;
+namespace {
+
+// Checks a lookup of an h x w x c query on table t with parameters p (prev: a previous field is
+// given; ws: in a caller's workspace) and sets the query "image" size; makes the table's int8
+// copy of b when an MFMA lookup needs it first.
+int check_lookup(pt_table *t, const pt_params &p, int h, int w, int c, bool prev, const LookupBufs *ws, int &ah,
+                 int &aw) {
     TRY(query_shape(t, h, w, c, ah, aw));
-    if (!p.do_table_lookup && !prev_dev)
+    if (!p.do_table_lookup && !prev)
         return pt_fail(PT_E_UNSUPPORTED, "do_table_lookup 0 requires a previous field (ann_prev)");
     if (p.init_random) return pt_fail(PT_E_UNSUPPORTED, "init_random is not supported");
     if (p.query_step < 1) return pt_fail(PT_E_INVALID, "query_step must be >= 1");
@@ -755,6 +758,93 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
             TRY(prepare_mfma(t, t->stream));
         }
     }
+    return PT_OK;
+}
+
+// The lookup arguments every form shares: the table(s), the parameters of the candidate pass and
+// the rounds, the lattice, the mask. The caller sets the query images (a / descriptors).
+void fill_lookup_args(const pt_table *t, const pt_params &p, int ah, int aw, const float *adesc, const float *xadesc,
+                      const LookupBufs &B, LookupArgs &L) {
+    const int qs = p.query_step;
+    const int ann_w = aw - 7, ann_h = ah - 7;
+    const bool mfma = p.mfma_rounds > 0;
+    L.ah = ah;
+    L.aw = aw;
+    L.ann_w = ann_w;
+    L.ann_h = ann_h;
+    L.a_desc = adesc;
+    L.xa_desc = xadesc;
+    L.bh = t->h;
+    L.bw = t->w;
+    L.table = t->table.p;
+    L.q = t->quant();
+    L.nxt = (int)t->extra.size();
+    for (int i = 0; i < L.nxt; i++) {
+        L.xtable[i] = t->extra[i]->table.p;
+        L.xq[i] = t->extra[i]->quant();
+    }
+    L.kpos = t->kpos.p;
+    L.kdist = t->kdist.p;
+    L.hist = B.hist;
+    L.hn = B.hn;
+    L.K = p.kcoherence;
+    L.kcoh_iter0 = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == 0);
+    // k-coherence walks in the rounds when kcoherence_iter asks for them after the first pass, and
+    // without table candidates (the walk then has to follow the propagated matches).
+    L.kcoh_rounds = p.kcoherence > 0 && (p.kcoherence_iter < 0 || (p.kcoherence_iter >= 1 && p.kcoherence_iter < p.prop_iters) ||
+                                         !p.do_table_lookup);
+    L.triangle_factor = p.triangle_factor;
+    L.spatial = p.spatial;
+    L.rs = !p.spatial && p.do_rs;
+    L.do_table_lookup = p.do_table_lookup;
+    L.rounds = p.do_prop ? p.jf_rounds : 0;
+    L.spatial_cand = L.rounds == 0;
+    L.spatial_from = L.rounds >= 2 ? 2 : 1;
+    L.qs = qs;
+    L.vw = (ann_w + qs - 1) / qs;
+    L.vh = (ann_h + qs - 1) / qs;
+    L.tile_x = t->tile_x;
+    if (t->allowed.p) {
+        L.allowed = t->allowed.p;
+        L.allowed_index = p.allowed_index;
+    }
+    if (mfma) {   // candidates: the cells of every table only; then the MFMA rounds
+        L.K = 0;
+        L.kcoh_iter0 = false;
+        L.kcoh_rounds = false;
+        L.spatial_cand = false;
+        L.rs = false;
+        L.rounds = 0;
+        L.hist = nullptr;
+        L.hn = nullptr;
+    }
+}
+
+void fill_mfma_args(const pt_table *t, const pt_params &p, const float *a_dev, const uint32_t *aq, int ah, int aw,
+                    MfmaArgs &M) {
+    M.a = a_dev;
+    M.b = t->b0.p;
+    M.aq = aq;
+    M.bq = t->bq.p;
+    M.bnorm = t->bnorm.p;
+    M.ah = ah;
+    M.aw = aw;
+    M.bh = t->h;
+    M.bw = t->w;
+    M.ann_w = aw - 7;
+    M.ann_h = ah - 7;
+    M.nanchors = p.mfma_anchors;
+}
+
+}  // namespace
+
+int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, const double *prev_dev,
+                      const float *sv_dev, const float *adesc_pre, bool timing, int *launches_out,
+                      const LookupBufs *ws) {
+    const pt_params &p = *t->p;
+    int ah = 0, aw = 0;
+    TRY(check_lookup(t, p, h, w, c, prev_dev != nullptr, ws, ah, aw));
+    const bool mfma = p.mfma_rounds > 0;
     hipStream_t s = t->stream;
     const int Hp = ah + 7, Wp = aw + 7;
     const size_t plane = (size_t)Hp * Wp;
@@ -840,54 +930,9 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
         L.a = a_dev;
         L.b = t->b0.p;
     }
-    L.ah = ah;
-    L.aw = aw;
-    L.ann_w = ann_w;
-    L.ann_h = ann_h;
-    L.a_desc = adesc_pre && !t->desc_mode && !t->keyed ? adesc_pre : B.adesc;
-    L.xa_desc = t->keyed ? t->xadesc.p : L.a_desc;
-    L.bh = t->h;
-    L.bw = t->w;
-    L.table = t->table.p;
-    L.q = t->quant();
-    L.nxt = (int)t->extra.size();
-    for (int i = 0; i < L.nxt; i++) {
-        L.xtable[i] = t->extra[i]->table.p;
-        L.xq[i] = t->extra[i]->quant();
-    }
-    L.kpos = t->kpos.p;
-    L.kdist = t->kdist.p;
-    L.hist = B.hist;
-    L.hn = B.hn;
-    L.K = p.kcoherence;
-    L.kcoh_iter0 = p.kcoherence > 0 && (p.kcoherence_iter < 0 || p.kcoherence_iter == 0);
-    // k-coherence walks in the rounds when kcoherence_iter asks for them after the first pass, and
-    // without table candidates (the walk then has to follow the propagated matches).
-    L.kcoh_rounds = p.kcoherence > 0 && (p.kcoherence_iter < 0 || (p.kcoherence_iter >= 1 && p.kcoherence_iter < p.prop_iters) ||
-                                         !p.do_table_lookup);
-    L.triangle_factor = p.triangle_factor;
-    L.spatial = p.spatial;
-    L.rs = !p.spatial && p.do_rs;
-    L.do_table_lookup = p.do_table_lookup;
-    L.rounds = p.do_prop ? p.jf_rounds : 0;
-    L.spatial_cand = L.rounds == 0;
-    L.spatial_from = L.rounds >= 2 ? 2 : 1;
-    L.qs = qs;
-    L.vw = vw;
-    L.vh = vh;
-    if (t->allowed.p) {
-        L.allowed = t->allowed.p;
-        L.allowed_index = p.allowed_index;
-    }
-    if (mfma) {   // candidates: the cells of every table only; then the MFMA rounds
-        L.K = 0;
-        L.kcoh_iter0 = false;
-        L.kcoh_rounds = false;
-        L.spatial_cand = false;
-        L.rs = false;
-        L.rounds = 0;
-        L.hist = nullptr;
-        L.hn = nullptr;
+    {
+        const float *adesc = adesc_pre && !t->desc_mode && !t->keyed ? adesc_pre : B.adesc;
+        fill_lookup_args(t, p, ah, aw, adesc, t->keyed ? t->xadesc.p : adesc, B, L);
     }
     const double temporal_scale = 1.0 / (1 + p.coherence_temporal);
     if (prev_dev) {
@@ -912,18 +957,7 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
         TRY(pt_lookup_candidates(L, f0, s));
         if (timing) HIPTRY(hipEventRecord(t->ev[8], s));
         MfmaArgs M;
-        M.a = a_dev;
-        M.b = t->b0.p;
-        M.aq = B.aq;
-        M.bq = t->bq.p;
-        M.bnorm = t->bnorm.p;
-        M.ah = ah;
-        M.aw = aw;
-        M.bh = t->h;
-        M.bw = t->w;
-        M.ann_w = ann_w;
-        M.ann_h = ann_h;
-        M.nanchors = p.mfma_anchors;
+        fill_mfma_args(t, p, a_dev, B.aq, ah, aw, M);
         if (t->mfma_stats && !ws) {
             const size_t n = (size_t)p.mfma_rounds * pt_mfma_round_workgroups(ann_w, ann_h);
             TRY(t->mfma_items.ensure(n));
@@ -1595,6 +1629,13 @@ extern "C" int pt_table_set_mfma_stats(pt_table *t, int on) {
     return PT_OK;
 }
 
+extern "C" int pt_table_set_tile(pt_table *t, int tile_x) {
+    if (!t) return pt_fail(PT_E_INVALID, "null argument");
+    if (!pt_lookup_tile_ok(tile_x)) return pt_failf(PT_E_INVALID, "tile width %d is not one of 64, 32, 16", tile_x);
+    t->tile_x = tile_x;
+    return PT_OK;
+}
+
 extern "C" int pt_table_mfma_items(const pt_table *t, int64_t *items, int n, int *rounds) {
     if (!t || !rounds) return pt_fail(PT_E_INVALID, "null argument");
     *rounds = (int)t->mfma_items_sum.size();
@@ -1644,22 +1685,21 @@ extern "C" int pt_desc_dist_dev(const float *a_dev, const float *b_dev, int C, i
 struct pt_batch {
     pt_params *p = nullptr;
     std::vector<pt_table *> tabs;
-    hipStream_t qs = nullptr;     // query-descriptor stream
-    hipEvent_t qdone = nullptr;
-    std::vector<hipEvent_t> e0, e1;
-    std::vector<hipGraphExec_t> gexec;
-    std::vector<std::vector<const void *>> gkey;   // buffers a graph was captured with
-    std::vector<pt_params> gpar;                   // ... and the parameters (byte copy): the
-                                                   // caller may change p between lookups
+    hipStream_t qs = nullptr;     // the batch's stream: the query's descriptor and every phase
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<const void *> gkey;   // buffers and tile width the graph was captured with
+    pt_params gpar;                   // ... and the parameters (byte copy): the caller may change
+                                      // p between lookups
+    int tile_x = 64;                  // lattice tile of the lookup kernels (pt_batch_autotune)
     DevBuf<float> adesc, yuv, temp;
+    DevBuf<uint32_t> aq;              // int8 RGB0 query (MFMA lookups)
+    DevBuf<PairLookup> pairs;         // every pair's arguments, read by the batched kernels
+    std::vector<PairLookup> hpairs;   // ... and their host copy
     ~pt_batch() {
-        for (auto g : gexec)
-            if (g) (void)hipGraphExecDestroy(g);
-        for (auto e : e0)
-            if (e) (void)hipEventDestroy(e);
-        for (auto e : e1)
-            if (e) (void)hipEventDestroy(e);
-        if (qdone) (void)hipEventDestroy(qdone);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
         if (qs) (void)hipStreamDestroy(qs);
         for (pt_table *t : tabs) delete t;
     }
@@ -1669,6 +1709,7 @@ extern "C" int pt_batch_create(pt_params *p, int n, const float *const *b_dev, c
                                int threads, pt_batch **out) {
     if (!p || !out || n < 1 || !b_dev || !h || !w) return pt_fail(PT_E_INVALID, "pt_batch_create: bad arguments");
     if (p->is_descriptor) return pt_fail(PT_E_UNSUPPORTED, "pt_batch: descriptor mode is not supported");
+    if (n > 65535) return pt_fail(PT_E_INVALID, "pt_batch_create: at most 65535 databases");
     *out = nullptr;
     pt_batch *bt = new pt_batch();
     bt->p = p;
@@ -1692,20 +1733,10 @@ extern "C" int pt_batch_create(pt_params *p, int n, const float *const *b_dev, c
             delete bt;
             return pt_failf(rc[i], "database %d: %s", i, err[i].c_str());
         }
-    int r = PT_OK;
-    if (hipStreamCreateWithFlags(&bt->qs, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&bt->qdone) != hipSuccess)
-        r = pt_fail(PT_E_HIP, "pt_batch_create: stream/event creation failed");
-    bt->e0.assign(n, nullptr);
-    bt->e1.assign(n, nullptr);
-    bt->gexec.assign(n, nullptr);
-    bt->gkey.assign(n, {});
-    bt->gpar.assign(n, pt_params());
-    for (int i = 0; r == PT_OK && i < n; i++)
-        if (hipEventCreate(&bt->e0[i]) != hipSuccess || hipEventCreate(&bt->e1[i]) != hipSuccess)
-            r = pt_fail(PT_E_HIP, "pt_batch_create: event creation failed");
-    if (r != PT_OK) {
+    if (hipStreamCreateWithFlags(&bt->qs, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&bt->e0) != hipSuccess ||
+        hipEventCreate(&bt->e1) != hipSuccess) {
         delete bt;
-        return r;
+        return pt_fail(PT_E_HIP, "pt_batch_create: stream/event creation failed");
     }
     *out = bt;
     return PT_OK;
@@ -1717,61 +1748,191 @@ extern "C" pt_table *pt_batch_table(pt_batch *bt, int i) {
     return bt && i >= 0 && i < (int)bt->tabs.size() ? bt->tabs[i] : nullptr;
 }
 
-extern "C" int pt_batch_lookup(pt_batch *bt, const float *a_dev, int h, int w, double *const *ann_dev,
-                               double *ms_per_pair, double *seconds) {
-    if (!bt || !a_dev || !ann_dev) return pt_fail(PT_E_INVALID, "pt_batch_lookup: null argument");
+namespace {
+
+// Every pair's arguments for a lookup of the h x w query a_dev (buffers grown on each table), and
+// their device copy; the tables' builds and earlier work on their streams are waited for.
+int batch_prepare(pt_batch *bt, const float *a_dev, int h, int w, double *const *ann_dev) {
+    const pt_params &p = *bt->p;
     const int n = (int)bt->tabs.size();
     pt_table *t0 = bt->tabs[0];
-    if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "query image must be at least 8x8");
-    if (!PT_SIZE_OK(h, w)) return pt_fail(PT_E_INVALID, "query image too large for XY_TO_INT packing");
-    const double T0 = now_s();
-    // the query descriptor once (all tables share the parameters)
     const size_t plane = (size_t)(h + 7) * (w + 7);
     TRY(bt->adesc.ensure(plane * t0->ndims));
     TRY(bt->yuv.ensure((size_t)h * w * 3));
     TRY(bt->temp.ensure(plane * 3));
-    TRY(pt_gck_launch(a_dev, h, w, t0->ndims - 2 * t0->nchroma, t0->nchroma, t0->build.convert_colorspace, bt->yuv.p,
-                      bt->temp.p, bt->adesc.p, bt->qs));
-    HIPTRY(hipEventRecord(bt->qdone, bt->qs));
+    const bool mfma = p.mfma_rounds > 0;
+    if (mfma) TRY(bt->aq.ensure((size_t)h * w));
+    bt->hpairs.assign(n, PairLookup());
     for (int i = 0; i < n; i++) {
         pt_table *t = bt->tabs[i];
-        hipStream_t s = t->stream;
-        HIPTRY(hipStreamWaitEvent(s, bt->qdone, 0));
-        HIPTRY(hipEventRecord(bt->e0[i], s));
-        const std::vector<const void *> key = {a_dev, (const void *)(intptr_t)h, (const void *)(intptr_t)w, ann_dev[i],
-                                               bt->adesc.p};
-        if (bt->gexec[i] && bt->gkey[i] == key && std::memcmp(&bt->gpar[i], bt->p, sizeof(pt_params)) == 0) {
-            HIPTRY(hipGraphLaunch(bt->gexec[i], s));
-        } else {
-            // eager run (sizes the buffers), then capture the same launch sequence for replays
-            TRY(pt_lookup_enqueue(t, a_dev, h, w, 3, ann_dev[i], nullptr, nullptr, bt->adesc.p, false, nullptr));
-            if (bt->gexec[i]) {
-                HIPTRY(hipGraphExecDestroy(bt->gexec[i]));
-                bt->gexec[i] = nullptr;
-            }
-            hipGraph_t g = nullptr;
-            HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            int rc = pt_lookup_enqueue(t, a_dev, h, w, 3, ann_dev[i], nullptr, nullptr, bt->adesc.p, false, nullptr);
-            hipError_t ec = hipStreamEndCapture(s, &g);
-            if (rc != PT_OK) return rc;
-            if (ec != hipSuccess) return pt_failf(PT_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
-            hipError_t ei = hipGraphInstantiate(&bt->gexec[i], g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ei != hipSuccess) return pt_failf(PT_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
-            bt->gkey[i] = key;
-            std::memcpy(&bt->gpar[i], bt->p, sizeof(pt_params));
+        if (t->ndims != t0->ndims || t->nchroma != t0->nchroma || t->extra.size() != t0->extra.size())
+            return pt_failf(PT_E_INVALID, "database %d: tables of a batch share their descriptor", i);
+        int ah = 0, aw = 0;
+        TRY(check_lookup(t, p, h, w, 3, false, nullptr, ah, aw));
+        const int ann_w = aw - 7, ann_h = ah - 7, qs = p.query_step;
+        const long nv = (long)((ann_w + qs - 1) / qs) * ((ann_h + qs - 1) / qs), np = (long)ann_w * ann_h;
+        TRY(t->nxy.ensure(nv));
+        TRY(t->nd.ensure(nv));
+        TRY(t->nfl.ensure(nv));
+        TRY(t->nxy2.ensure(nv));
+        TRY(t->nd2.ensure(nv));
+        TRY(t->nfl2.ensure(nv));
+        if (!mfma) {
+            TRY(t->hist.ensure((size_t)nv * PT_LOOKUP_HIST));
+            TRY(t->hn.ensure(nv));
         }
-        HIPTRY(hipEventRecord(bt->e1[i], s));
+        if (qs > 1) {
+            TRY(t->gxy.ensure(np));
+            TRY(t->gd.ensure(np));
+        }
+        LookupBufs B;
+        B.hist = t->hist.p;
+        B.hn = t->hn.p;
+        PairLookup &P = bt->hpairs[i];
+        fill_lookup_args(t, p, ah, aw, bt->adesc.p, bt->adesc.p, B, P.L);
+        P.L.a = a_dev;
+        P.L.b = t->b0.p;
+        P.L.tile_x = bt->tile_x;
+        P.G = P.L;
+        P.G.gqs = qs;
+        P.G.qs = 1;   // gather lanes are single patches
+        if (mfma) fill_mfma_args(t, p, a_dev, bt->aq.p, ah, aw, P.M);
+        P.f[0].xy = t->nxy.p;
+        P.f[0].d = t->nd.p;
+        P.f[0].fl = t->nfl.p;
+        P.f[1].xy = t->nxy2.p;
+        P.f[1].d = t->nd2.p;
+        P.f[1].fl = t->nfl2.p;
+        P.gxy = t->gxy.p;
+        P.gd = t->gd.p;
+        P.ann = ann_dev[i];
+        HIPTRY(hipStreamSynchronize(t->stream));   // its build (and any single lookup) is done
     }
-    for (int i = 0; i < n; i++) HIPTRY(hipStreamSynchronize(bt->tabs[i]->stream));
-    const double T1 = now_s();
-    if (ms_per_pair)
-        for (int i = 0; i < n; i++) {
-            float ms = 0;
-            HIPTRY(hipEventElapsedTime(&ms, bt->e0[i], bt->e1[i]));
-            ms_per_pair[i] = ms;
+    TRY(bt->pairs.ensure(n));
+    HIPTRY(hipMemcpy(bt->pairs.p, bt->hpairs.data(), n * sizeof(PairLookup), hipMemcpyHostToDevice));
+    return PT_OK;
+}
+
+// The launches of one batched lookup, on the batch's stream (captured into a graph after the first
+// eager run): the query's descriptor once, then each phase once for all pairs.
+int batch_enqueue(pt_batch *bt, const float *a_dev, int h, int w, int *launches) {
+    const pt_params &p = *bt->p;
+    const int n = (int)bt->tabs.size();
+    pt_table *t0 = bt->tabs[0];
+    const PairLookup &P0 = bt->hpairs[0];
+    hipStream_t s = bt->qs;
+    TRY(pt_gck_launch(a_dev, h, w, t0->ndims - 2 * t0->nchroma, t0->nchroma, t0->build.convert_colorspace, bt->yuv.p,
+                      bt->temp.p, bt->adesc.p, s));
+    int nl = 1;
+    if (p.mfma_rounds > 0) {
+        TRY(pt_quant_rgb0_launch(a_dev, (long)h * w, bt->aq.p, s));
+        TRY(pt_lookup_batch_candidates(bt->pairs.p, P0, n, s));
+        int which = 0;
+        TRY(pt_mfma_rounds_batch_launch(bt->pairs.p, P0, n, p.mfma_rounds, s, &which));
+        TRY(pt_lookup_batch_output(bt->pairs.p, P0, n, which, s, nullptr, &nl));
+        nl += 3 + p.mfma_rounds;
+    } else {
+        int l = 0;
+        TRY(pt_lookup_batch_launch(bt->pairs.p, P0, n, s, nullptr, &l));
+        nl += l + 1;
+    }
+    if (launches) *launches = nl;
+    return PT_OK;
+}
+
+}  // namespace
+
+extern "C" int pt_batch_lookup(pt_batch *bt, const float *a_dev, int h, int w, double *const *ann_dev,
+                               double *ms_per_pair, double *seconds) {
+    if (!bt || !a_dev || !ann_dev) return pt_fail(PT_E_INVALID, "pt_batch_lookup: null argument");
+    const int n = (int)bt->tabs.size();
+    for (int i = 0; i < n; i++)
+        if (!ann_dev[i]) return pt_fail(PT_E_INVALID, "pt_batch_lookup: null output");
+    if (h < 8 || w < 8) return pt_fail(PT_E_INVALID, "query image must be at least 8x8");
+    if (!PT_SIZE_OK(h, w)) return pt_fail(PT_E_INVALID, "query image too large for XY_TO_INT packing");
+    const double T0 = now_s();
+    std::vector<const void *> key = {a_dev, (const void *)(intptr_t)h, (const void *)(intptr_t)w,
+                                     (const void *)(intptr_t)bt->tile_x};
+    for (int i = 0; i < n; i++) key.push_back(ann_dev[i]);
+    hipStream_t s = bt->qs;
+    if (bt->gexec && bt->gkey == key && std::memcmp(&bt->gpar, bt->p, sizeof(pt_params)) == 0) {
+        HIPTRY(hipEventRecord(bt->e0, s));
+        HIPTRY(hipGraphLaunch(bt->gexec, s));
+        HIPTRY(hipEventRecord(bt->e1, s));
+    } else {
+        // eager run (sizes the buffers, uploads the pairs), then capture the same launches
+        if (bt->gexec) {
+            HIPTRY(hipGraphExecDestroy(bt->gexec));
+            bt->gexec = nullptr;
         }
+        TRY(batch_prepare(bt, a_dev, h, w, ann_dev));
+        HIPTRY(hipEventRecord(bt->e0, s));
+        TRY(batch_enqueue(bt, a_dev, h, w, nullptr));
+        HIPTRY(hipEventRecord(bt->e1, s));
+        hipGraph_t g = nullptr;
+        HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_enqueue(bt, a_dev, h, w, nullptr);
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc != PT_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return pt_failf(PT_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&bt->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            bt->gexec = nullptr;
+            return pt_failf(PT_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        }
+        bt->gkey = key;
+        std::memcpy(&bt->gpar, bt->p, sizeof(pt_params));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    const double T1 = now_s();
+    if (ms_per_pair) {
+        float ms = 0;
+        HIPTRY(hipEventElapsedTime(&ms, bt->e0, bt->e1));
+        for (int i = 0; i < n; i++) ms_per_pair[i] = ms;   // every pair ends with the batch
+    }
     if (seconds) *seconds = T1 - T0;
+    return PT_OK;
+}
+
+extern "C" int pt_batch_set_tile(pt_batch *bt, int tile_x) {
+    if (!bt) return pt_fail(PT_E_INVALID, "pt_batch_set_tile: null batch");
+    if (!pt_lookup_tile_ok(tile_x)) return pt_failf(PT_E_INVALID, "tile width %d is not one of 64, 32, 16", tile_x);
+    bt->tile_x = tile_x;
+    return PT_OK;
+}
+
+extern "C" int pt_batch_get_tile(const pt_batch *bt) { return bt ? bt->tile_x : 0; }
+
+extern "C" int pt_batch_autotune(pt_batch *bt, const float *a_dev, int h, int w, double *const *ann_dev, int reps,
+                                 int *tile_x_out, double *ms_out) {
+    if (!bt || reps < 1) return pt_fail(PT_E_INVALID, "pt_batch_autotune: bad arguments");
+    static const int shapes[3] = {64, 32, 16};
+    double best = 0;
+    int best_tx = 64;
+    for (int k = 0; k < 3; k++) {
+        bt->tile_x = shapes[k];
+        TRY(pt_batch_lookup(bt, a_dev, h, w, ann_dev, nullptr, nullptr));   // eager run + capture
+        std::vector<double> ms;
+        for (int r = 0; r < reps; r++) {
+            TRY(pt_batch_lookup(bt, a_dev, h, w, ann_dev, nullptr, nullptr));
+            float e = 0;
+            HIPTRY(hipEventElapsedTime(&e, bt->e0, bt->e1));
+            ms.push_back(e);
+        }
+        std::sort(ms.begin(), ms.end());
+        const double med = ms[ms.size() / 2];
+        if (ms_out) ms_out[k] = med;
+        if (k == 0 || med < best) {
+            best = med;
+            best_tx = shapes[k];
+        }
+    }
+    bt->tile_x = best_tx;
+    if (tile_x_out) *tile_x_out = best_tx;
     return PT_OK;
 }
 

@@ -1,6 +1,7 @@
-"""GPU: one query against many databases (pt_batch_*). Every pair's NNF equals the single-table
-lookup of the same pair bit for bit, on the first call (eager, then captured) and on graph
-replays, and matches the oracle's model of the parallel lookup."""
+"""GPU: one query against many databases (pt_batch_*): each phase one launch over every pair,
+captured in one graph. Every pair's NNF equals the single-table lookup of the same pair bit for
+bit, on the first call (eager, then captured) and on graph replays, with every lattice tile, and
+matches the oracle's model of the parallel lookup."""
 import numpy as np
 import pytest
 
@@ -32,6 +33,46 @@ def test_batch_matches_single_tables():
         assert all(m > 0 for m in ms)
     o = O.OracleTable(p.raw, dbs[1])
     assert np.array_equal(singles[1], o.lookup_parallel(a_np))
+    batch.close()
+
+
+@pytest.mark.parametrize("switches", ["-x 0 -limit 1", "-x 0 -limit 1 -query_step 2",
+                                      "-x 0 -limit 1 -mfma_rounds 4 -mfma_anchors 5"])
+def test_batch_one_launch_per_phase_every_tile(switches):
+    """The batched kernels (one launch per phase for all pairs) with each lattice tile give every
+    pair the single-table lookup's NNF bit for bit, also with query_step 2 (the gather phase) and
+    in the MFMA form; a single table gives the same NNF with every tile. Databases of uneven
+    sizes, among them 8 and 16 tile columns wide (the XCD column bands)."""
+    import torch
+    from patchtable_amd import PatchTable, PatchTableBatch, PatchTableParams, synth_pair
+    sizes = [(120, 160), (200, 150), (96, 260), (170, 170), (90, 1031)]
+    dbs = [synth_pair(240 + i, h, w)[1] for i, (h, w) in enumerate(sizes)]
+    a_np = synth_pair(239, 140, 519)[0]          # 512 patches wide: 8 tile columns of 64
+    a = torch.from_numpy(a_np).cuda()
+    p = PatchTableParams()
+    p.set_speed(1)
+    p.set_from_switches(switches)
+    print(f"batch tiles: {switches}", flush=True)
+    batch = PatchTableBatch(p, [torch.from_numpy(d).cuda() for d in dbs], threads=3)
+    singles = [PatchTable(p, d) for d in dbs]
+    refs = [t.lookup(a_np) for t in singles]
+    for tx in (64, 32, 16):
+        singles[0].set_tile(tx)
+        assert np.array_equal(singles[0].lookup(a_np), refs[0]), tx
+        batch.tile_x = tx
+        for call in range(2):                    # eager + capture, then a replay
+            anns, ms, _ = batch.lookup(a)
+            for i, ann in enumerate(anns):
+                assert np.array_equal(ann.cpu().numpy(), refs[i]), (tx, call, i)
+            assert len(set(ms)) == 1 and ms[0] > 0
+    anns = [torch.empty((133, 512, 3), dtype=torch.float64, device="cuda") for _ in dbs]
+    res = batch.autotune(a, anns, reps=2)
+    print(f"autotune: {res}", flush=True)
+    assert res["tile_x"] in (64, 32, 16) and batch.tile_x == res["tile_x"]
+    assert all(v > 0 for v in res["median_ms"].values())
+    out, _, _ = batch.lookup(a, anns)
+    for i in range(len(dbs)):
+        assert np.array_equal(out[i].cpu().numpy(), refs[i]), i
     batch.close()
 
 
