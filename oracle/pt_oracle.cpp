@@ -1343,24 +1343,36 @@ static inline int rs_pick(uint32_t i, uint32_t round) {
 static long g_ssd_count[64];
 
 // ---------------------------- lookup (this library's MFMA form) -----------------------------
-// Model of the GPU MFMA lookup (mfma_rounds > 0; image mode, query_step 1, no mask):
-//   init and candidates: as lookup_parallel with the cells of every table only (no k-coherence
-//     walk, no spatial or random search);
+// Model of the GPU MFMA lookup (mfma_rounds > 0; image mode, query_step 1, no mask). Matches are
+// ranked by the integer SSD of the int8 images, q = round(255 clamp(v, 0, 1)) - 128 per value
+// ("int8 SSD"); each patch's field holds its match and the match's score, and the exact FP32 SSD
+// (the reference's lane order) of the final match is computed once at the end:
+//   init: nothing, or the previous field's match (clamped) with score int8 SSD x w, w the
+//     temporal weight (coherence_temporal_sv of the patch, else 1 / (1 + coherence_temporal));
+//     its output distance is the reference's initial distance (the previous distance x w, or
+//     with recalc_dist_temporal the exact SSD x w);
+//   candidates: the cell of every table as lookup_parallel visits them (a cell equal to the
+//     current match, or to the previous table's cell, is skipped), each taken when its int8 SSD
+//     is strictly below the current score;
 //   rounds r = 0..R-1 with jump J = max(1, 2^min(4, R-1) >> r), double-buffered, for every
 //     group of 16 query patches x0 = 16k .. x0+15 of every row y:
 //     sources (slots 0..63): the members (x0+i, y), the patches (x0+i, y-J) and (x0+i, y+J),
 //       (x0+i-J, y) for i < 8 and (x0+i+J, y) for i >= 8, if inside the query; each proposes
 //       its offset (match - position);
+//     a round whose jump equals the previous round's skips every group none of whose valid
+//       sources changed its match in the previous round (it would repeat that round's work and
+//       keep every match; the skip only shows in the item counts);
 //     anchors: the distinct offsets (count, first slot); up to mfma_anchors times the
 //       uncovered one of largest count (ties: smallest first slot), which covers every offset
 //       within 8 columns and PT_MFMA_BAND rows of it;
 //     search: candidates (x0 + ox - 8 + j, y + oy + dy), j < 32, |dy| <= PT_MFMA_BAND, inside
 //       b, in order o = (anchor * (2 PT_MFMA_BAND + 1) + dy + PT_MFMA_BAND) * 32 + j; each member
-//       keeps the smallest (SSD, o), with SSD the integer SSD of the int8 images
-//       q = round(255 clamp(v, 0, 1)) - 128;
-//     rescoring: a member whose best differs from its match takes it if its exact SSD (the
-//       reference's lane order) is strictly smaller;
-//   output: the temporal rescaling as lookup_parallel.
+//       keeps the smallest (int8 SSD, o);
+//     acceptance: a member whose best differs from its match takes it when its int8 SSD is
+//       strictly below the match's score;
+//   output: x, y and the exact FP32 SSD of the match; a patch still holding its initial
+//     (previous-field) match keeps its initial distance, with the temporal rescaling of
+//     lookup_parallel.
 // The integer scores are exact, so this restates the matrix-core arithmetic bit for bit.
 static inline int quant8(float v) { return (int)rintf(fminf(fmaxf(v, 0.f), 1.f) * 255.f) - 128; }
 
@@ -1387,9 +1399,22 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
     std::vector<int16_t> qa((size_t)AH * AW * 3), qb((size_t)T.h * BW * 3);
     for (size_t i = 0; i < qa.size(); i++) qa[i] = (int16_t)quant8(Q.a[i]);
     for (size_t i = 0; i < qb.size(); i++) qb[i] = (int16_t)quant8(T.b0[i]);
+    auto ssd8 = [&](int x, int y, int xs, int ys) {
+        int s = 0;
+        for (int rr = 0; rr < 8; rr++) {
+            const int16_t *qr = &qa[((size_t)(y + rr) * AW + x) * 3];
+            const int16_t *br = &qb[((size_t)(ys + rr) * BW + xs) * 3];
+            for (int e = 0; e < 24; e++) {
+                const int dd = qr[e] - br[e];
+                s += dd * dd;
+            }
+        }
+        return s;
+    };
     // init and candidates
     std::vector<int32_t> cx(N), cy(N), ix(N), iy(N);
-    fvec cd(N), id(N);
+    fvec cd(N), id(N);                     // scores; initial output distances
+    std::vector<char> moved(N, 1);         // the match is not the initial (previous-field) one
     const bool have_prev = ann_prev != nullptr;
     for (long i = 0; i < N; i++) {
         const int x = (int)(i % aw), y = (int)(i / aw);
@@ -1401,22 +1426,24 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
             yb = (int)ann_prev[i * 3 + 1];
             xb = xb < 0 ? 0 : (xb >= bew ? bew - 1 : xb);
             yb = yb < 0 ? 0 : (yb >= beh ? beh - 1 : yb);
-            db = p.recalc_dist_temporal ? (float)(dist(x, y, xb, yb) * (sv ? (double)sv[i] : temporal_scale))
-                                        : (float)(ann_prev[i * 3 + 2] * temporal_scale);
+            const double w = sv ? (double)sv[i] : temporal_scale;
+            id[i] = p.recalc_dist_temporal ? (float)(dist(x, y, xb, yb) * w) : (float)(ann_prev[i * 3 + 2] * temporal_scale);
+            db = (float)((double)ssd8(x, y, xb, yb) * w);
             have = true;
+            moved[i] = 0;
         }
         ix[i] = xb;
         iy[i] = yb;
-        id[i] = db;
         if (p.do_table_lookup) {
             const int v = T.table[T.cell_of(Q.qdesc(x, y))];
             const int xs = INT_TO_X(v) - 7, ys = INT_TO_Y(v) - 7;
             if (!have || xs != xb || ys != yb) {
-                const float d = dist(x, y, xs, ys);
+                const float d = (float)ssd8(x, y, xs, ys);
                 if (!have || d < db) {
                     db = d;
                     xb = xs;
                     yb = ys;
+                    moved[i] = 1;
                 }
                 have = true;
             }
@@ -1425,11 +1452,12 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
                 const int v1 = E->table[E->cell_of(Q.xdesc(x, y))];
                 const int x1 = INT_TO_X(v1) - 7, y1 = INT_TO_Y(v1) - 7;
                 if ((x1 != xb || y1 != yb) && (x1 != xp || y1 != yp)) {
-                    const float d = dist(x, y, x1, y1);
+                    const float d = (float)ssd8(x, y, x1, y1);
                     if (d < db) {
                         db = d;
                         xb = x1;
                         yb = y1;
+                        moved[i] = 1;
                     }
                 }
                 xp = x1;
@@ -1444,12 +1472,15 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
     std::vector<int32_t> nx(N), ny(N);
     fvec nd(N);
     g_mfma_items.assign(rounds, 0);
+    std::vector<char> chg(N, 1);   // the previous round changed the match
     for (int r = 0; r < rounds; r++) {
         const int J = mfma_jump_at(rounds, r);
+        const bool stable = r > 0 && mfma_jump_at(rounds, r - 1) == J;
         for (int y = 0; y < ah; y++)
             for (int x0 = 0; x0 < aw; x0 += 16) {
                 int sox[64], soy[64];
                 bool sval[64];
+                bool any_moved = false;
                 for (int sl = 0; sl < 64; sl++) {
                     int sx, sy;
                     if (sl < 16) sx = x0 + sl, sy = y;
@@ -1463,8 +1494,11 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
                         const long k = (long)sy * aw + sx;
                         sox[sl] = cx[k] - sx;
                         soy[sl] = cy[k] - sy;
+                        any_moved = any_moved || chg[k];
                     }
                 }
+                if (stable && !any_moved)
+                    for (int sl = 0; sl < 64; sl++) sval[sl] = false;
                 // distinct offsets in order of first occurrence, with counts
                 std::vector<int> dfirst, dcnt;
                 for (int sl = 0; sl < 64; sl++) {
@@ -1504,16 +1538,7 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
                             if (xs < 0 || ys < 0 || xs >= bew || ys >= beh) continue;
                             const uint32_t ord = (uint32_t)((a * (2 * R + 1) + dy + R) * 32 + j);
                             for (int i = 0; i < 16 && x0 + i < aw; i++) {
-                                int ssd = 0;
-                                for (int rr = 0; rr < 8; rr++) {
-                                    const int16_t *qr = &qa[((size_t)(y + rr) * AW + x0 + i) * 3];
-                                    const int16_t *br = &qb[((size_t)(ys + rr) * BW + xs) * 3];
-                                    for (int e = 0; e < 24; e++) {
-                                        const int dd = qr[e] - br[e];
-                                        ssd += dd * dd;
-                                    }
-                                }
-                                const uint64_t key = ((uint64_t)(uint32_t)ssd << 11) | ord;
+                                const uint64_t key = ((uint64_t)(uint32_t)ssd8(x0 + i, y, xs, ys) << 11) | ord;
                                 if (key < best[i]) best[i] = key;
                             }
                         }
@@ -1526,13 +1551,12 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
                         const int ord = (int)(best[i] & 2047u), j = ord & 31, dyi = (ord >> 5) % (2 * R + 1),
                                   an = (ord >> 5) / (2 * R + 1);
                         const int xs = x0 + aox[an] - 8 + j, ys = y + aoy[an] - R + dyi;
-                        if (xs != xb || ys != yb) {
-                            const float d = dist(x0 + i, y, xs, ys);
-                            if (d < db) {
-                                db = d;
-                                xb = xs;
-                                yb = ys;
-                            }
+                        const float s8 = (float)(uint32_t)(best[i] >> 11);
+                        if ((xs != xb || ys != yb) && s8 < db) {
+                            db = s8;
+                            xb = xs;
+                            yb = ys;
+                            moved[k] = 1;
                         }
                     }
                     nx[k] = xb;
@@ -1540,14 +1564,17 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
                     nd[k] = db;
                 }
             }
+        for (long k = 0; k < N; k++) chg[k] = nx[k] != cx[k] || ny[k] != cy[k];
         cx.swap(nx);
         cy.swap(ny);
         cd.swap(nd);
     }
     for (long i = 0; i < N; i++) {
-        double d = cd[i];
+        const int x = (int)(i % aw), y = (int)(i / aw);
+        const float d0 = moved[i] ? dist(x, y, cx[i], cy[i]) : id[i];
+        double d = d0;
         if (have_prev && p.coherence_temporal != 0.0 && (double)cx[i] == ann_prev[i * 3] &&
-            (double)cy[i] == ann_prev[i * 3 + 1] && cd[i] == id[i])
+            (double)cy[i] == ann_prev[i * 3 + 1] && d0 == id[i])
             d = d / temporal_scale;
         ann[i * 3 + 0] = cx[i];
         ann[i * 3 + 1] = cy[i];

@@ -48,15 +48,20 @@ using namespace pt_tile;
 __constant__ int c_dx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 __constant__ int c_dy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 
-enum { M_TILE = 0, M_GLOBAL = 1, M_DESC = 2 };   // where a lane's query patch is read from
+// Where a lane's query patch is read from: the float tile in LDS, global memory, the descriptor
+// in LDS; M_INT8 (MFMA form's candidate pass): the int8 RGB0 tile in LDS, distances being int8
+// SSDs |q|^2 + |p|^2 - 2 q.p (exact integers) instead of FP32 SSDs.
+enum { M_TILE = 0, M_GLOBAL = 1, M_DESC = 2, M_INT8 = 3 };
 
-constexpr uint8_t F_DONE = 1, F_CHG = 2, F_KD = 4;
+constexpr uint8_t F_DONE = 1, F_CHG = 2, F_KD = 4, F_MOVED = PT_FIELD_MOVED;
 
 // Floats of LDS that hold the block's query patches.
 template <int M, int TX>
 struct QBuf {
-    static constexpr int F4 =
-        M == M_TILE ? (TX + 7) * (THREADS / TX + 7) : (M == M_DESC ? THREADS * PT_LOOKUP_CMAX / 4 : 1);
+    static constexpr int F4 = M == M_TILE   ? (TX + 7) * (THREADS / TX + 7)
+                              : M == M_INT8 ? ((TX + 7) * (THREADS / TX + 7) + 3) / 4
+                              : M == M_DESC ? THREADS * PT_LOOKUP_CMAX / 4
+                                            : 1;
 };
 
 template <int TX>
@@ -71,6 +76,16 @@ template <int M, int TX>
 __device__ __forceinline__ void stage_query(const LookupArgs &L, float4 *qbuf, bool valid, int x, int y) {
     if constexpr (M == M_TILE) {
         stage_tile<TX>(L.a, L.ah, L.aw, qbuf);
+    } else if constexpr (M == M_INT8) {
+        constexpr int SX = TX + 7, SY = THREADS / TX + 7;
+        uint32_t *t = (uint32_t *)qbuf;
+        const int2 tc = tile_coords();
+        const int x0 = tc.x * TX, y0 = tc.y * (THREADS / TX);
+        for (int i = threadIdx.x; i < SX * SY; i += THREADS) {
+            const int ly = i / SX, lx = i - ly * SX, gx = x0 + lx, gy = y0 + ly;
+            t[i] = gx < L.aw && gy < L.ah ? L.aq8[(size_t)gy * L.aw + gx] : 0u;
+        }
+        __syncthreads();
     } else if constexpr (M == M_DESC) {
         float *qd = (float *)qbuf + threadIdx.x * L.C;
         const float *src = L.a_descI + ((size_t)(y + 7) * (L.aw + 7) + x + 7) * L.C;
@@ -86,6 +101,31 @@ __device__ __forceinline__ float qdist(const LookupArgs &L, const float4 *qbuf, 
     if constexpr (M == M_TILE) {
         const float4 *aq = qbuf + (lane / TX) * (TX + 7) + (lane % TX);
         return ssd_staged<true, TX + 7>(aq, L.b + ((size_t)ys * L.bw + xs) * 3, L.bw * 3, bound);
+    } else if constexpr (M == M_INT8) {
+        // |q|^2 + |p|^2 - 2 q.p over the RGB0 words (the fourth byte is zero in both)
+        constexpr int SX = TX + 7;
+        const uint32_t *qt = (const uint32_t *)qbuf + (lane / TX) * SX + (lane % TX);
+        const int pitch = L.bw + 2 * PT_MFMA_PAD;
+        const int32_t *bp = (const int32_t *)L.bq8 + (size_t)(ys + PT_MFMA_PAD) * pitch + xs + PT_MFMA_PAD;
+        int dot = 0;
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int4 b0 = ldg4i(bp + (size_t)r * pitch), b1 = ldg4i(bp + (size_t)r * pitch + 4);
+            const uint32_t *q = qt + r * SX;
+            dot = __builtin_amdgcn_sdot4((int)q[0], b0.x, dot, false);
+            dot = __builtin_amdgcn_sdot4((int)q[1], b0.y, dot, false);
+            dot = __builtin_amdgcn_sdot4((int)q[2], b0.z, dot, false);
+            dot = __builtin_amdgcn_sdot4((int)q[3], b0.w, dot, false);
+            dot = __builtin_amdgcn_sdot4((int)q[4], b1.x, dot, false);
+            dot = __builtin_amdgcn_sdot4((int)q[5], b1.y, dot, false);
+            dot = __builtin_amdgcn_sdot4((int)q[6], b1.z, dot, false);
+            dot = __builtin_amdgcn_sdot4((int)q[7], b1.w, dot, false);
+        }
+        int x, y;
+        lane_xy<TX>(L, lane, x, y);
+        const int nkp = L.bw - 7 + 2 * PT_MFMA_PAD;
+        const int pn = (int)(L.bnk[(size_t)(ys + PT_MFMA_PAD) * nkp + xs + PT_MFMA_PAD] >> 7) - (int)PT_MFMA_OFF;
+        return (float)(L.qnorm[(size_t)y * L.ann_w + x] + pn - 2 * dot);
     } else if constexpr (M == M_GLOBAL) {
         int x, y;
         lane_xy<TX>(L, lane, x, y);
@@ -448,15 +488,20 @@ __device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField ou
     const int x = vx * L.qs, y = vy * L.qs;
     stage_query<M, TX>(L, qbuf, valid, x, y);
     const long pi = (long)y * L.ann_w + x;
-    bool have = false;
+    bool have = false, moved = true;   // moved: the match is not the initial (previous-field) one
     int xb = 0, yb = 0;
     float db = 0.f;
     if (valid && L.init_xy) {
         const int v = L.init_xy[pi];
         xb = PT_INT_TO_X(v);
         yb = PT_INT_TO_Y(v);
-        db = L.init_d[pi];
+        if constexpr (M == M_INT8)
+            db = (float)((double)qdist<M, TX>(L, qbuf, threadIdx.x, xb, yb, INFINITY) *
+                         (L.init_sv ? (double)L.init_sv[pi] : L.init_ts));
+        else
+            db = L.init_d[pi];
         have = true;
+        moved = false;
     }
     Hist hist;
     if (valid && L.do_table_lookup) {
@@ -469,6 +514,7 @@ __device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField ou
                 db = d;
                 xb = xs;
                 yb = ys;
+                moved = true;
             }
             have = true;
         }
@@ -483,6 +529,7 @@ __device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField ou
                     db = d;
                     xb = x1;
                     yb = y1;
+                    moved = true;
                 }
             }
             xp = x1;
@@ -507,7 +554,7 @@ __device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField ou
     const long v = (long)vy * L.vw + vx;
     out.xy[v] = PT_XY_TO_INT(xb, yb);
     out.d[v] = db;
-    out.fl[v] = (done ? F_DONE : 0) | F_CHG | (kd ? F_KD : 0);
+    out.fl[v] = (done ? F_DONE : 0) | F_CHG | (kd ? F_KD : 0) | (M == M_INT8 && moved ? F_MOVED : 0);
     if (L.hist && L.rounds > 0) {
         if (hist.adds == 0) L.hn[v] = 0;   // nothing evaluated: the count must still be written
         hist.store(L, v, (long)L.vw * L.vh);
@@ -662,6 +709,26 @@ __device__ __forceinline__ void gather_body(const LookupArgs &L, NnfField lat, i
     od[pi] = db;
 }
 
+// MFMA form's output stage: the field holds int8 scores; a patch whose match is not its initial
+// one gets the exact FP32 SSD of the match (the reference's lane order), a patch holding its
+// initial (previous-field) match its initial distance.
+template <int TX>
+__device__ __forceinline__ void exact_moved_body(const LookupArgs &L, NnfField f) {
+    __shared__ float4 qbuf[QBuf<M_TILE, TX>::F4];
+    const int2 tc = tile_coords();
+    const int vx = tc.x * TX + threadIdx.x % TX, vy = tc.y * (THREADS / TX) + threadIdx.x / TX;
+    const bool valid = vx < L.vw && vy < L.vh;
+    stage_query<M_TILE, TX>(L, qbuf, valid, vx, vy);
+    if (!valid) return;
+    const long v = (long)vy * L.vw + vx;
+    if (!(f.fl[v] & F_MOVED)) {   // the initial (previous-field) match keeps its initial distance
+        if (L.init_d) f.d[v] = L.init_d[v];
+        return;
+    }
+    const int c = f.xy[v];
+    f.d[v] = qdist<M_TILE, TX>(L, qbuf, threadIdx.x, PT_INT_TO_X(c), PT_INT_TO_Y(c), INFINITY);
+}
+
 // Kernels of one lookup (arguments by value) and of a batch of lookups of one query against many
 // tables (blockIdx.z = pair; each pair's arguments and fields in a device array). A pair's lanes
 // compute exactly what its single lookup computes.
@@ -694,6 +761,15 @@ template <int M, int TX>
 __global__ __launch_bounds__(THREADS) void k_gather_batch(const PairLookup *__restrict__ P, int src) {
     const PairLookup &p = P[blockIdx.z];
     gather_body<M, TX>(p.G, p.f[src], p.gxy, p.gd);
+}
+template <int TX>
+__global__ __launch_bounds__(THREADS) void k_exact_moved(LookupArgs L, NnfField f) {
+    exact_moved_body<TX>(L, f);
+}
+template <int TX>
+__global__ __launch_bounds__(THREADS) void k_exact_moved_batch(const PairLookup *__restrict__ P, int src) {
+    const PairLookup &p = P[blockIdx.z];
+    exact_moved_body<TX>(p.L, p.f[src]);
 }
 
 // Initial field from a previous one (TABLE_COPY_PREV_NNF and patchtable.h:3539-3585).
@@ -876,7 +952,8 @@ int pt_lookup_candidates(const LookupArgs &L, NnfField f0, hipStream_t s) {
         constexpr int TX = decltype(tx_)::value;
         if (L.C > 0) {
             if constexpr (TX == 64) launch_candidates<M_DESC, TX>(L, f0, s);
-        } else if (L.qs == 1) launch_candidates<M_TILE, TX>(L, f0, s);
+        } else if (L.int8) launch_candidates<M_INT8, TX>(L, f0, s);
+        else if (L.qs == 1) launch_candidates<M_TILE, TX>(L, f0, s);
         else launch_candidates<M_GLOBAL, TX>(L, f0, s);
         PT_LAUNCH_CHECK();
         return PT_OK;
@@ -956,8 +1033,32 @@ int pt_lookup_batch_candidates(const PairLookup *P_dev, const PairLookup &P0, in
     return with_tile(L.tile_x, 0, [&](auto tx_) {
         constexpr int TX = decltype(tx_)::value;
         dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX), npairs);
-        if (L.qs == 1) hipLaunchKernelGGL((k_candidates_batch<M_TILE, TX>), grid, dim3(THREADS), 0, s, P_dev);
+        if (L.int8) hipLaunchKernelGGL((k_candidates_batch<M_INT8, TX>), grid, dim3(THREADS), 0, s, P_dev);
+        else if (L.qs == 1) hipLaunchKernelGGL((k_candidates_batch<M_TILE, TX>), grid, dim3(THREADS), 0, s, P_dev);
         else hipLaunchKernelGGL((k_candidates_batch<M_GLOBAL, TX>), grid, dim3(THREADS), 0, s, P_dev);
+        PT_LAUNCH_CHECK();
+        return PT_OK;
+    });
+}
+
+int pt_lookup_exact_moved(const LookupArgs &L, NnfField f, hipStream_t s) {
+    if (L.C > 0 || L.qs != 1) return pt_fail(PT_E_UNSUPPORTED, "int8 scores are kept in image mode with query_step 1");
+    return with_tile(L.tile_x, 0, [&](auto tx_) {
+        constexpr int TX = decltype(tx_)::value;
+        dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX));
+        hipLaunchKernelGGL((k_exact_moved<TX>), grid, dim3(THREADS), 0, s, L, f);
+        PT_LAUNCH_CHECK();
+        return PT_OK;
+    });
+}
+
+int pt_lookup_batch_exact_moved(const PairLookup *P_dev, const PairLookup &P0, int npairs, int src, hipStream_t s) {
+    const LookupArgs &L = P0.L;
+    if (L.C > 0 || L.qs != 1) return pt_fail(PT_E_UNSUPPORTED, "int8 scores are kept in image mode with query_step 1");
+    return with_tile(L.tile_x, 0, [&](auto tx_) {
+        constexpr int TX = decltype(tx_)::value;
+        dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX), npairs);
+        hipLaunchKernelGGL((k_exact_moved_batch<TX>), grid, dim3(THREADS), 0, s, P_dev, src);
         PT_LAUNCH_CHECK();
         return PT_OK;
     });

@@ -1,4 +1,4 @@
-// MFMA lookup rounds on gfx950: dense int8 candidate scoring on matrix cores, exact rescoring.
+// MFMA lookup rounds on gfx950: dense int8 candidate scoring on matrix cores.
 //
 // The plain lookup (lookup.hip) evaluates each candidate with one exact FP32 SSD per lane: 8
 // rows x 96 bytes gathered at an arbitrary place in b, about 48 tag lookups of L1, and the
@@ -10,7 +10,9 @@
 //       |q - p|^2 = |q|^2 + |p|^2 - 2 q.p ;
 //   an exact int32 (the MFMA accumulates int8 products in int32: no rounding anywhere).
 //
-// One round with jump J, for each group of 16 consecutive query patches of a row (x0 = 16k):
+// The field of the MFMA form holds each patch's match and the match's int8 SSD (its score; the
+// candidate pass scores the table cells the same way, lookup.hip M_INT8). One round with jump J,
+// for each group of 16 consecutive query patches of a row (x0 = 16k):
 //   sources: 64 positions -- the 16 members, the 16 patches J rows above and J rows below,
 //     8 patches J to the left of members 0-7 and 8 patches J to the right of members 8-15;
 //     each valid source proposes its current offset (match - position);
@@ -21,16 +23,17 @@
 //     y + oy + dy (|dy| <= R) -- each member sees every offset within 8 columns of the anchor;
 //     each member keeps the candidate of smallest (SSD, anchor, dy, j), SSD = |q|^2 + |p|^2 - 2 q.p
 //     the exact integer SSD of the int8 patches;
-//   rescoring: a member whose best candidate differs from its match computes the exact SSD in
-//     the reference's lane order (patch_dist_exact, patchtable.h:330-373) and takes it when
-//     strictly closer, so NNF_DIST is always the exact distance of the stored match.
-// The oracle (lookup_parallel with mfma_rounds > 0) restates this bit for bit: the integer
-// scores do not depend on summation order.
+//   acceptance: a member whose best candidate differs from its match takes it when its int8 SSD
+//     is strictly below the match's score.
+// After the rounds the exact FP32 SSD of every final match (the reference's lane order,
+// patch_dist_exact, patchtable.h:330-373) replaces its score (lookup.hip), so NNF_DIST is always
+// the exact distance of the stored match. The oracle (lookup_parallel with mfma_rounds > 0)
+// restates this bit for bit: the integer scores do not depend on summation order.
 //
-// Work layout: a workgroup of one wave owns 64 query patches of a row, its 4 groups in turn. A wave stages each anchor's band of b (39 x 10 pixels, int8 RGB0)
-// in its own LDS region; each lane loads the B fragments of one candidate column for the 9 row
-// pairs the band's 3 rows of candidates use, and the 3 x 4 MFMAs of a 16-candidate block read
-// them from registers. The float query tile staged for the rescoring is the lookup's own.
+// Work layout: a workgroup of one wave owns 64 query patches of a row, its 4 groups in turn. A
+// wave stages each anchor's band of b (39 x 10 pixels, int8 RGB0) in its own LDS region; each
+// lane loads the B fragments of one candidate column for the 9 row pairs the band's 3 rows of
+// candidates use, and the 3 x 4 MFMAs of a 16-candidate block read them from registers.
 #include "pt_device.h"
 #include "pt_kernels.h"
 #include "pt_tile.h"
@@ -45,6 +48,8 @@ constexpr int R = PT_MFMA_BAND;        // rows of candidates above and below the
 constexpr int BR = 8 + 2 * R;          // band rows
 constexpr int BC = 32 + 7;             // band columns (32 candidates of 8 pixels)
 constexpr uint64_t NONE = ~0ull;
+constexpr uint8_t F_CHG = 2;           // field flag: the round that wrote the field changed the match
+constexpr uint8_t F_MOVED = PT_FIELD_MOVED;   // ... the match is not the initial one (int8 score)
 // No packed offset equals this: both halves would be -32768, and |offset| <= 32767.
 constexpr int HT_EMPTY = (int)0x80008000u;
 
@@ -80,9 +85,10 @@ __global__ void k_quant_rgb0_pad(const float *__restrict__ img, int h, int w, ui
     out[(size_t)(y + P) * (w + 2 * P) + x + P] = quant_pixel(img + i * 3);
 }
 
-// |p|^2 of every 8x8 patch of the padded int8 copy (pw x ph patches), into the interior of a
-// (ph + 2P) x (pw + 2P) array.
-__global__ void k_patch_norms_pad(const uint32_t *__restrict__ q, int w, int pw, int ph, int32_t *__restrict__ out) {
+// Norm key of every 8x8 patch of the padded int8 copy (pw x ph patches), (|p|^2 + OFF) << 7
+// (see phase 2 of the round), into the interior of a (ph + 2P) x (pw + 2P) array whose margins
+// hold PT_MFMA_NK_PAD.
+__global__ void k_patch_norms_pad(const uint32_t *__restrict__ q, int w, int pw, int ph, uint32_t *__restrict__ out) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)pw * ph) return;
     const int x = (int)(i % pw), y = (int)(i / pw);
@@ -92,7 +98,21 @@ __global__ void k_patch_norms_pad(const uint32_t *__restrict__ q, int w, int pw,
 #pragma unroll
         for (int c = 0; c < 8; c++) s += sq_pixel(row[c]);
     }
-    out[(size_t)(y + P) * (pw + 2 * P) + x + P] = s;
+    out[(size_t)(y + P) * (pw + 2 * P) + x + P] = ((uint32_t)s + PT_MFMA_OFF) << 7;
+}
+
+// |q|^2 of every 8x8 patch of an unpadded int8 RGB0 image (aw pixels wide; pw x ph patches).
+__global__ void k_query_norms(const uint32_t *__restrict__ q, int aw, int pw, int ph, int32_t *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)pw * ph) return;
+    const int x = (int)(i % pw), y = (int)(i / pw);
+    int s = 0;
+    for (int r = 0; r < 8; r++) {
+        const uint32_t *row = q + (size_t)(y + r) * aw + x;
+#pragma unroll
+        for (int c = 0; c < 8; c++) s += sq_pixel(row[c]);
+    }
+    out[i] = s;
 }
 
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
@@ -152,8 +172,9 @@ constexpr int HS = 96;                   // hash slots per group (64 sources)
 constexpr int WSH = 12 * HS + 256 + 8 * PT_MFMA_MAX_ANCHORS > 2 * BANDW ? 12 * HS + 256 + 8 * PT_MFMA_MAX_ANCHORS
                                                                         : 2 * BANDW;
 
-// Per-lane offsets of the band words this lane loads: pixel (r, c) of the band at r * bwp + c in
-// the padded copy of b, or norm (dy, j) at dy * bnp + j in the padded norm table (-1: none).
+// Per-lane byte offsets of the band words this lane loads: pixel (r, c) of the band at
+// r * bwp + c in the padded copy of b, or norm key (dy, j) at dy * bnp + j in the padded norm key
+// table (-1: none).
 // They do not depend on the item, so they are computed once.
 __device__ __forceinline__ void band_offsets(const MfmaArgs &M, int lane, int (&off)[PER]) {
     const int bwp = M.bw + 2 * P, bnp = M.bw - 7 + 2 * P;
@@ -162,31 +183,46 @@ __device__ __forceinline__ void band_offsets(const MfmaArgs &M, int lane, int (&
         const int e = lane + 64 * k;
         if (e < BR * BC) {
             const int r = (e * 1681) >> 16, c = e - r * BC;   // e / 39 for e < 1000
-            off[k] = r * bwp + c;
+            off[k] = (r * bwp + c) * 4;
         } else if (e < BANDW) {
             const int n = e - BR * BC;
-            off[k] = (n >> 5) * bnp + (n & 31);
+            off[k] = ((n >> 5) * bnp + (n & 31)) * 4;
         } else {
             off[k] = -1;
         }
     }
 }
 
-// Loads the band of item (x0, ox, oy) into registers from the padded copies. The band origin lies
-// inside the margins (see PT_MFMA_PAD); it is clamped to them anyway, so a load can never leave
-// the arrays.
-__device__ __forceinline__ void band_fetch(const MfmaArgs &M, int lane, int x0, int y, int ox, int oy,
-                                           const int (&off)[PER], uint32_t (&reg)[PER]) {
+// Loads the band of item (x0, ox, oy) into registers from the padded copies, as buffer loads
+// whose base is the band's origin (scalar) and whose offsets are the lane's fixed band offsets.
+// The band origin lies inside the margins (see PT_MFMA_PAD); it is clamped to them anyway, and
+// each resource ends where its array ends, so a load can never leave the arrays.
+__device__ __forceinline__ void band_fetch(const MfmaArgs &M, int x0, int y, int ox, int oy, const int (&off)[PER],
+                                           uint32_t (&reg)[PER]) {
     const int bwp = M.bw + 2 * P, bnp = M.bw - 7 + 2 * P;
     const int u0 = min(max(x0 + ox - 8, -P), M.bw + P - BC), v0 = min(max(y + oy - R, -P), M.bh + P - BR);
-    const uint32_t *bq = M.bq + (size_t)(v0 + P) * bwp + (u0 + P);
-    const uint32_t *bn = (const uint32_t *)M.bnorm + (size_t)(v0 + P) * bnp + (u0 + P);
+    const size_t oq = (size_t)(v0 + P) * bwp + (u0 + P), on = (size_t)(v0 + P) * bnp + (u0 + P);
+    const size_t nq = (size_t)(M.bh + 2 * P) * bwp, nn = (size_t)(M.bh - 7 + 2 * P) * bnp;
+    const __amdgpu_buffer_rsrc_t rq = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(M.bq + oq), 0, (int)min((nq - oq) * 4, (size_t)0x7fffffff), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rn = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(M.bnorm + on), 0, (int)min((nn - on) * 4, (size_t)0x7fffffff), 0x00020000);
 #pragma unroll
     for (int k = 0; k < PER; k++) {
-        const int e = lane + 64 * k;   // e is known per k: the branch folds where all lanes agree
-        if (64 * k + 63 < BR * BC || (64 * k < BR * BC && e < BR * BC)) reg[k] = bq[off[k]];
-        else if (e < BANDW) reg[k] = bn[off[k]];
-        else reg[k] = 0u;
+        const int e = threadIdx.x + 64 * k;
+        if (64 * k + 63 < BR * BC) {               // pixels in every lane
+            reg[k] = __builtin_amdgcn_raw_buffer_load_b32(rq, off[k], 0, 0);
+        } else if (64 * k < BR * BC) {             // pixels, then norm keys: both loads in every
+            const bool px = e < BR * BC;           // lane (a per-lane choice of resource would
+            const uint32_t vq = __builtin_amdgcn_raw_buffer_load_b32(rq, px ? off[k] : 0, 0, 0);   // loop)
+            const uint32_t vn = __builtin_amdgcn_raw_buffer_load_b32(rn, px ? 0 : off[k], 0, 0);
+            reg[k] = px ? vq : vn;
+        } else if (64 * k < BANDW) {               // norm keys (lanes past the band load word 0)
+            const uint32_t vn = __builtin_amdgcn_raw_buffer_load_b32(rn, e < BANDW ? off[k] : 0, 0, 0);
+            reg[k] = e < BANDW ? vn : 0u;
+        } else {
+            reg[k] = 0u;
+        }
     }
 }
 
@@ -194,7 +230,10 @@ __device__ __forceinline__ void band_fetch(const MfmaArgs &M, int lane, int x0, 
 // own items are done (its row's item count varies with the image).
 constexpr int MW = 64, MQY = 1 + 7;   // threads; query rows staged (the row's patches)
 
-__device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur, NnfField nxt, int J) {
+// `stable`: the previous round had the same jump J and its field's flags mark the patches whose
+// match it changed. A group none of whose valid sources changed would then compute the anchors,
+// winners and rescorings of the previous round again and keep every match: it takes no items.
+__device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur, NnfField nxt, int J, int stable) {
     __shared__ uint32_t qq[AX * MQY];   // int8 RGB0 query pixels (A fragments)
     // the hash tables, source offsets and anchors of phase 1, then the two band buffers of phase 2
     __shared__ uint32_t wsh[1][WSH];
@@ -212,6 +251,7 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
     const int y = y00 + wave, xw0 = x00;
     const int aw = M.ann_w, ah = M.ann_h;
     uint64_t mykey = NONE;   // best candidate of query (xw0 + lane, y)
+    uint32_t mys = 0;        // ... its key without the order bits: |p|^2 + OFF - 2 q.p
     int nitems = 0;
     if (y < ah) {
         // ---- phase 1: sources, distinct offsets and anchors of the wave's 4 groups, all four at
@@ -230,6 +270,14 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
             sy[r] = r == 1 ? y - J : (r == 2 ? y + J : y);
             sv[r] = x0 < aw && sx[r] >= 0 && sy[r] >= 0 && sx[r] < aw && sy[r] < ah;
             sxy[r] = sv[r] ? cur.xy[(long)sy[r] * aw + sx[r]] : 0;   // all four loads in flight
+        }
+        if (stable) {
+            bool moved = false;
+#pragma unroll
+            for (int r = 0; r < 4; r++) moved = moved || (sv[r] && (cur.fl[(long)sy[r] * aw + sx[r]] & F_CHG));
+            if (((__ballot(moved) >> (16 * gq)) & 0xffffull) == 0)   // no source of this group moved
+#pragma unroll
+                for (int r = 0; r < 4; r++) sv[r] = false;
         }
         for (int i = lane; i < 4 * HS; i += 64) {
             hk[i] = HT_EMPTY;
@@ -306,11 +354,15 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
     // Each lane scores one candidate column jj against the 4 members 4g + t. Within a lane a
     // candidate's key is (|p|^2 + OFF - 2 q.p) << 7 | ord with ord = (anchor * (2R+1) + dy + R) * 2
     // + block: |q|^2 is the same for all candidates of a member, and OFF >= |q|^2 makes the key
-    // 128 (SSD - |q|^2 + OFF) + ord a non-negative int below 2^31 that ranks by (SSD, ord).
-    // Candidates outside b get keys above 2.4e9. At the end of a group the lanes' minima are
-    // reduced as (key, jj), i.e. (SSD, anchor, dy, j) with j = 16 block + jj.
-    constexpr uint32_t OFF = 192u * 128u * 128u;
-    constexpr uint32_t INV = 3200000000u, INV_LO = 2400000000u;
+    // 128 (SSD - |q|^2 + OFF) + ord a non-negative int below 2^31 that ranks by (SSD, ord). The
+    // norm key table holds (|p|^2 + OFF) << 7, and PT_MFMA_NK_PAD for positions outside b, whose
+    // keys then lie in [2.48e9, 4.1e9]: above every key inside b, and at or above INV_LO. At the
+    // end of a group the lanes' minima are reduced as (key, jj), i.e. (SSD, anchor, dy, j) with
+    // j = 16 block + jj.
+    constexpr uint32_t INV_LO = 2400000000u;
+    static_assert((uint64_t)PT_MFMA_NK_PAD - 256ull * 192 * 128 * 128 >= INV_LO, "pad keys stay invalid");
+    static_assert((uint64_t)PT_MFMA_NK_PAD + 127 + 256ull * 192 * 128 * 128 < (1ull << 32), "pad keys do not wrap");
+    static_assert(((uint64_t)4 * PT_MFMA_OFF << 7) + 127 < INV_LO, "keys inside b stay below INV_LO");
     // item fields are wave-uniform: read them into scalar registers
     auto item = [&](int i, int f) { return __builtin_amdgcn_readfirstlane(items[wave][i][f]); };
     // -256 in a scalar register the compiler does not see as a constant: a constant factor would
@@ -325,19 +377,19 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
     uint32_t reg[PER];
     uint32_t *bufs = wsh[wave];
     if (nitems > 0) {
-        band_fetch(M, lane, xw0 + 16 * item(0, 0), y, item(0, 1), item(0, 2), off, reg);
+        band_fetch(M, xw0 + 16 * item(0, 0), y, item(0, 1), item(0, 2), off, reg);
         wave_lds_sync();   // phase 1's reads of the hash table are done
 #pragma unroll
         for (int k = 0; k < PER; k++)
             if (lane + 64 * k < BANDW) bufs[lane + 64 * k] = reg[k];
-        if (nitems > 1) band_fetch(M, lane, xw0 + 16 * item(1, 0), y, item(1, 1), item(1, 2), off, reg);
+        if (nitems > 1) band_fetch(M, xw0 + 16 * item(1, 0), y, item(1, 1), item(1, 2), off, reg);
     }
     v4i A[4];
     uint32_t best[4] = {~0u, ~0u, ~0u, ~0u};
     int cur_g = -1, ga = 0, gfirst = 0;   // group of the current items, anchor index in it, its first item
     const int bew = M.bw - 7, beh = M.bh - 7;
     for (int it = 0; it < nitems; it++) {
-        const int gi = item(it, 0), ax = item(it, 1), ay = item(it, 2);
+        const int gi = item(it, 0);
         const int x0 = xw0 + 16 * gi;
         wave_lds_sync();   // item it - 1's reads are done, item it's band is written
         if (it + 1 < nitems) {
@@ -345,7 +397,7 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
 #pragma unroll
             for (int k = 0; k < PER; k++)
                 if (lane + 64 * k < BANDW) nb1[lane + 64 * k] = reg[k];
-            if (it + 2 < nitems) band_fetch(M, lane, xw0 + 16 * item(it + 2, 0), y, item(it + 2, 1), item(it + 2, 2), off, reg);
+            if (it + 2 < nitems) band_fetch(M, xw0 + 16 * item(it + 2, 0), y, item(it + 2, 1), item(it + 2, 2), off, reg);
         }
         const uint32_t *bnd = bufs + (it & 1) * BANDW;
         if (gi != cur_g) {
@@ -360,7 +412,6 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
                 A[s] = mv ? v4i{(int)src[0], (int)src[1], (int)src[2], (int)src[3]} : v4i{0, 0, 0, 0};
             }
         }
-        const int u0 = x0 + ax - 8, v0 = y + ay - R;
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             const int col = 16 * b + jj + 4 * (g & 1);
@@ -370,18 +421,13 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
                 const uint32_t *p = bnd + (r0 + (g >> 1)) * BC + col;
                 B[r0] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
             }
-            const int xs = u0 + 16 * b + jj;
-            const bool xok = xs >= 0 && xs < bew;
 #pragma unroll
             for (int dyi = 0; dyi < 2 * R + 1; dyi++) {
                 v4i C = v4i{0, 0, 0, 0};
 #pragma unroll
                 for (int s = 0; s < 4; s++) C = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s], B[dyi + 2 * s], C, 0, 0, 0);
-                const int ys = v0 + dyi;
-                const bool cv = xok && ys >= 0 && ys < beh;
-                const uint32_t nb = bnd[BR * BC + dyi * 32 + 16 * b + jj];   // read for every lane (no branch)
-                const uint32_t kv = ((nb + OFF) << 7) | (uint32_t)((ga * (2 * R + 1) + dyi) * 2 + b);
-                const uint32_t base = cv ? kv : INV;
+                const uint32_t nk = bnd[BR * BC + dyi * 32 + 16 * b + jj];   // read for every lane (no branch)
+                const uint32_t base = nk | (uint32_t)((ga * (2 * R + 1) + dyi) * 2 + b);
                 // C[t]: member 4g + t against candidate column jj; |q.p| < 2^22 fits the signed
                 // 24-bit multiply, so base - 256 q.p is one multiply-add
 #pragma unroll
@@ -411,99 +457,57 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
                 if (gfirst + an < nitems) {
                     const int oxa = items[wave][gfirst + an][1], oya = items[wave][gfirst + an][2];   // per lane
                     const int xs = x0 + oxa - 8 + j, ys = y + oya - R + dyi;
-                    // a key below INV_LO is a candidate inside b; the check keeps the rescoring's
-                    // reads inside b whatever the key holds
-                    if (xs >= 0 && ys >= 0 && xs < bew && ys < beh) mykey = (uint64_t)(uint32_t)PT_XY_TO_INT(xs, ys);
+                    // a key below INV_LO is a candidate inside b; the check keeps the stored
+                    // match inside b whatever the key holds
+                    if (xs >= 0 && ys >= 0 && xs < bew && ys < beh) {
+                        mykey = (uint64_t)(uint32_t)PT_XY_TO_INT(xs, ys);
+                        mys = key >> 7;   // |p|^2 + OFF - 2 q.p
+                    }
                 }
             }
 #pragma unroll
             for (int t = 0; t < 4; t++) best[t] = ~0u;
         }
     }
-    // ---- phase 3: exact rescoring of each query's best candidate when it differs from its match.
-    // The SSD's four lane sums L0..L3 (elements e = 4n + k of each row, rows in order) are
-    // independent, so four lanes can compute one rescoring, one sum each, and lane 4i adds them
-    // as ((L0 + L1) + L2) + L3: the same value as one lane computing all four. Few rescorings
-    // in a wave are packed 16 to a pass that way; many are done one per lane, each stopping
-    // early once a partial sum reaches the match's distance (the sums only grow, so stopping
-    // early never changes which of the two is smaller).
+    // ---- phase 3: a member whose best candidate differs from its match takes it when the
+    // candidate's int8 SSD, |q|^2 + |p|^2 - 2 q.p, is strictly below the match's score (the field
+    // holds int8 scores; the exact FP32 SSD of the final match is computed after the rounds).
     const int x = xw0 + lane;
     const bool live = y < ah && x < aw;
     const long vi = (long)y * aw + x;
     int v = 0;
     float d = 0.f;
+    uint8_t fl = 0;
     if (live) {
         v = cur.xy[vi];
         d = cur.d[vi];
+        fl = cur.fl[vi] & F_MOVED;
     }
+    const int v_in = v;
     const int c = (int)(uint32_t)mykey;
-    const bool need = live && mykey != NONE && (PT_INT_TO_X(c) != PT_INT_TO_X(v) || PT_INT_TO_Y(c) != PT_INT_TO_Y(v));
-    const uint64_t nmask = __ballot(need);
-    const int nr = __popcll(nmask);
-    if (nr > 48) {
-        if (need) {
-            const float dn = pt_patch_ssd_bounded(M.a + ((size_t)y * M.aw + x) * 3, M.aw * 3,
-                                                  M.b + ((size_t)PT_INT_TO_Y(c) * M.bw + PT_INT_TO_X(c)) * 3, M.bw * 3, d);
-            if (dn < d) {
-                d = dn;
-                v = c;
-            }
-        }
-    } else if (nr > 0) {
-        int *tq = (int *)wsh[wave], *tc = tq + 64;
-        float *td = (float *)(tq + 128);
-        const int rank = __popcll(nmask & ((1ull << lane) - 1ull));
-        wave_lds_sync();   // phase 2's reads of the band buffers are done
-        if (need) {
-            tq[rank] = lane;
-            tc[rank] = c;
-        }
-        wave_lds_sync();
-        const int k = lane & 3, a3 = M.aw * 3, b3 = M.bw * 3;
-        for (int t0 = 0; t0 < nr; t0 += 16) {
-            const int ti = t0 + (lane >> 2);
-            float L = 0.f;
-            if (ti < nr) {
-                const int qc = tc[ti];
-                const float *ap = M.a + ((size_t)y * M.aw + xw0 + tq[ti]) * 3 + k;
-                const float *bp = M.b + ((size_t)PT_INT_TO_Y(qc) * M.bw + PT_INT_TO_X(qc)) * 3 + k;
-#pragma unroll
-                for (int dy = 0; dy < 8; dy++)
-#pragma unroll
-                    for (int n = 0; n < 6; n++) {
-                        const float e = ap[dy * a3 + 4 * n] - bp[dy * b3 + 4 * n];
-                        L = L + e * e;
-                    }
-            }
-            // lane 4i + j holds L_j of task i: quad_perm broadcasts of lanes 1, 2, 3 of each quad
-            const float L1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(L), 0x55, 0xF, 0xF, true));
-            const float L2 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(L), 0xAA, 0xF, 0xF, true));
-            const float L3 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(L), 0xFF, 0xF, 0xF, true));
-            if (k == 0 && ti < nr) td[ti] = ((L + L1) + L2) + L3;
-        }
-        wave_lds_sync();
-        if (need) {
-            const float dn = td[rank];
-            if (dn < d) {
-                d = dn;
-                v = c;
-            }
+    if (live && mykey != NONE && c != v) {
+        const int s8 = (int)mys - (int)PT_MFMA_OFF + M.qnorm[vi];
+        if ((float)s8 < d) {
+            d = (float)s8;
+            v = c;
+            fl = F_MOVED;
         }
     }
     if (!live) return;
     nxt.xy[vi] = v;
     nxt.d[vi] = d;
-    nxt.fl[vi] = 0;
+    nxt.fl[vi] = fl | (v != v_in ? F_CHG : 0);
 }
 
-__global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J) {
-    mfma_round_body(M, cur, nxt, J);
+__global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J, int stable) {
+    mfma_round_body(M, cur, nxt, J, stable);
 }
 
 // One round of every pair of a batch (blockIdx.z = pair), from field src into field src ^ 1.
-__global__ __launch_bounds__(MW, 4) void k_mfma_round_batch(const PairLookup *__restrict__ P, int src, int J) {
+__global__ __launch_bounds__(MW, 4) void k_mfma_round_batch(const PairLookup *__restrict__ P, int src, int J,
+                                                            int stable) {
     const PairLookup &p = P[blockIdx.z];
-    mfma_round_body(p.M, p.f[src], p.f[src ^ 1], J);
+    mfma_round_body(p.M, p.f[src], p.f[src ^ 1], J, stable);
 }
 
 }  // namespace
@@ -515,7 +519,15 @@ int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t 
     return PT_OK;
 }
 
-int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, int32_t *bnorm, hipStream_t s) {
+int pt_query_norms_launch(const uint32_t *aq, int ah, int aw, int32_t *qnorm, hipStream_t s) {
+    const long n = (long)(aw - 7) * (ah - 7);
+    if (n <= 0) return PT_OK;
+    hipLaunchKernelGGL(k_query_norms, dim3(pt_blocks(n, 256)), dim3(256), 0, s, aq, aw, aw - 7, ah - 7, qnorm);
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, uint32_t *bnorm, hipStream_t s) {
     const long npx = (long)h * w, n = (long)(w - 7) * (h - 7);
     hipLaunchKernelGGL(k_quant_rgb0_pad, dim3(pt_blocks(npx, 256)), dim3(256), 0, s, b, h, w, bq);
     hipLaunchKernelGGL(k_patch_norms_pad, dim3(pt_blocks(n, 256)), dim3(256), 0, s, bq, w, w - 7, h - 7, bnorm);
@@ -533,7 +545,8 @@ int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f
     for (int r = 0; r < rounds; r++) {
         MfmaArgs Mr = M;
         if (M.items_out) Mr.items_out = M.items_out + r * nwg;
-        hipLaunchKernelGGL(k_mfma_round, grid, dim3(MW), 0, s, Mr, f0, f1, pt_mfma_jump_at(rounds, r));
+        const int J = pt_mfma_jump_at(rounds, r), stable = r > 0 && pt_mfma_jump_at(rounds, r - 1) == J;
+        hipLaunchKernelGGL(k_mfma_round, grid, dim3(MW), 0, s, Mr, f0, f1, J, stable);
         std::swap(f0, f1);
     }
     PT_LAUNCH_CHECK();
@@ -550,7 +563,8 @@ int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, i
     dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, M.ann_h, npairs);
     int src = 0;
     for (int r = 0; r < rounds; r++) {
-        hipLaunchKernelGGL(k_mfma_round_batch, grid, dim3(MW), 0, s, P_dev, src, pt_mfma_jump_at(rounds, r));
+        const int J = pt_mfma_jump_at(rounds, r), stable = r > 0 && pt_mfma_jump_at(rounds, r - 1) == J;
+        hipLaunchKernelGGL(k_mfma_round_batch, grid, dim3(MW), 0, s, P_dev, src, J, stable);
         src ^= 1;
     }
     PT_LAUNCH_CHECK();

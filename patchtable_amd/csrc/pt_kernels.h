@@ -207,10 +207,19 @@ struct LookupArgs {
     uint8_t *hn = nullptr;          // ... and how many were recorded
     int tile_x = 64;                // lattice tile of a workgroup: 64 x 4, 32 x 8 or 16 x 16 patches
                                     // (image mode; results do not depend on it)
+    // MFMA form: the candidate pass scores by int8 SSD (image mode, query_step 1), from the int8
+    // RGB0 query (ah x aw), the padded int8 copy of b and its norm keys and |q|^2 of every query
+    // patch; a previous field's initial score is its int8 SSD x (init_sv ? init_sv[i] : init_ts)
+    int int8 = 0;
+    const uint32_t *aq8 = nullptr, *bq8 = nullptr, *bnk = nullptr;
+    const int32_t *qnorm = nullptr;
+    const float *init_sv = nullptr;
+    double init_ts = 1.0;
 };
 // One nearest-neighbour field over the lattice: XY_TO_INT(b patch x, y), distance, and flags
 // (1: the spatial window around the match was searched without improvement; 2: the match changed
-// in the round that produced the field; 4: the match's k-coherence list was walked).
+// in the round that produced the field; 4: the match's k-coherence list was walked; 8, MFMA form:
+// the match is not the initial one and the distance is its int8 score, not yet the exact SSD).
 struct NnfField {
     int32_t *xy = nullptr;
     float *d = nullptr;
@@ -231,14 +240,20 @@ static inline int pt_jump_at(int rounds, int r) {
 // inside it (anchor offsets come from matches inside b and sources within 16 + 15 columns and 16
 // rows of the group), so the band loads need no bounds checks.
 #define PT_MFMA_PAD 48
+// Norm keys of the MFMA scoring: (|p|^2 + PT_MFMA_OFF) << 7 for a database patch p inside b (int8
+// RGB0, |p|^2 <= PT_MFMA_OFF = 192 * 128^2), PT_MFMA_NK_PAD in the margins.
+#define PT_MFMA_OFF (192u * 128u * 128u)
+#define PT_MFMA_NK_PAD 0xC4000000u
+#define PT_FIELD_MOVED 8
 struct MfmaArgs {
     const float *a = nullptr, *b = nullptr;         // RGB float images (exact rescoring)
     const uint32_t *aq = nullptr;                   // int8 RGB0 query, ah x aw
     const uint32_t *bq = nullptr;                   // int8 RGB0 database, (bh + 2 PAD) x (bw + 2 PAD)
-    const int32_t *bnorm = nullptr;                 // |p|^2 of every database patch, (beh + 2 PAD) x (bew + 2 PAD)
+    const uint32_t *bnorm = nullptr;                // norm key of every database patch, (beh + 2 PAD) x (bew + 2 PAD)
     int ah = 0, aw = 0, bh = 0, bw = 0;             // image sizes
     int ann_w = 0, ann_h = 0;                       // query patches (query_step 1)
     int nanchors = 0;
+    const int32_t *qnorm = nullptr;                 // |q|^2 of every query patch (int8 RGB0)
     // optional instrumentation: per round, the items (group, anchor) of every workgroup, one entry
     // per workgroup (round r at items_out + r * workgroups); null: not written
     int32_t *items_out = nullptr;
@@ -252,8 +267,12 @@ static inline int pt_mfma_jump_at(int rounds, int r) {
     return j > 1 ? j : 1;
 }
 int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t s);
-// The padded int8 copy of b and its patch norms (margins of PT_MFMA_PAD zeros, written by the caller).
-int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, int32_t *bnorm, hipStream_t s);
+// |q|^2 of every 8x8 patch of the int8 RGB0 query (ah x aw pixels), (ah-7) x (aw-7) int32.
+int pt_query_norms_launch(const uint32_t *aq, int ah, int aw, int32_t *qnorm, hipStream_t s);
+
+// The padded int8 copy of b and its patch norm keys (margins written by the caller: zeros in the
+// copy, PT_MFMA_NK_PAD in the norm keys).
+int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, uint32_t *bnorm, hipStream_t s);
 // Runs `rounds` rounds from f0; *which = 1 if the result is in f1.
 int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f1, hipStream_t s, int *which);
 // One pair of a batch (one query against many tables), read by the batched kernels from a device
@@ -280,6 +299,10 @@ int pt_lookup_batch_candidates(const PairLookup *P_dev, const PairLookup &P0, in
 // MFMA rounds of every pair of a batch, one launch per round; *which = the field with the result.
 int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, int rounds, hipStream_t s,
                                 int *which);
+// MFMA form: the exact FP32 SSD (the reference's lane order) of every patch whose field flags
+// mark a match other than its initial one (F_MOVED), into the field's distances: lookup.hip.
+int pt_lookup_exact_moved(const LookupArgs &L, NnfField f, hipStream_t s);
+int pt_lookup_batch_exact_moved(const PairLookup *P_dev, const PairLookup &P0, int npairs, int src, hipStream_t s);
 // Tile widths the lookup kernels are built for.
 int pt_lookup_tile_ok(int tile_x);
 // Output stage of a lookup (gather for query_step > 1, then the double field): lookup.hip.
@@ -298,6 +321,7 @@ struct LookupBufs {
     uint8_t *nfl = nullptr, *nfl2 = nullptr, *hn = nullptr;
     int *bad = nullptr;
     uint32_t *aq = nullptr;   // int8 RGB0 query (MFMA lookup)
+    int32_t *qnorm = nullptr; // ... and |q|^2 of its patches
 };
 struct pt_table;
 // Enqueues one lookup on the table's stream (table.cpp). ws: the buffers to work in, or NULL for

@@ -84,7 +84,7 @@ struct pt_table {
     DevBuf<int32_t> kpos;
     DevBuf<float> kdist;
     DevBuf<uint32_t> bq;      // MFMA lookup: int8 RGB0 copy of b0
-    DevBuf<int32_t> bnorm;    // ... and |p|^2 of every database patch
+    DevBuf<uint32_t> bnorm;   // ... and the norm key of every database patch
     bool mfma_stats = false;              // count the MFMA rounds' items (instrumentation)
     DevBuf<int32_t> mfma_items;           // per round, per workgroup
     std::vector<int64_t> mfma_items_sum;  // per round, of the last synchronous lookup
@@ -92,6 +92,7 @@ struct pt_table {
     // query side
     DevBuf<float> a, adesc, adescI, yuv, temp;
     DevBuf<uint32_t> aq;
+    DevBuf<int32_t> qnorm;   // MFMA lookup: |q|^2 of every query patch (int8 RGB0)
     DevBuf<int32_t> nxy, nxy2, gxy, ixy;
     DevBuf<float> nd, nd2, gd, id;
     DevBuf<uint8_t> nfl, nfl2;
@@ -551,15 +552,15 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
 
 // PatchTableFixedN constructor (patchtable.h:2584-3373) from a device image (RGB, or a
 // descriptor image in descriptor mode) of h_in x w_in x c.
-// The MFMA lookup's view of b: an int8 RGB0 copy of b0 and the squared norm of every patch, both
-// inside margins of PT_MFMA_PAD zeros.
+// The MFMA lookup's view of b: an int8 RGB0 copy of b0 and the norm key of every patch, both
+// inside margins of PT_MFMA_PAD (zeros in the copy, PT_MFMA_NK_PAD in the norm keys).
 int prepare_mfma(pt_table *t, hipStream_t s) {
     const size_t nq = (size_t)(t->h + 2 * PT_MFMA_PAD) * (t->w + 2 * PT_MFMA_PAD);
     const size_t nn = (size_t)(t->h - 7 + 2 * PT_MFMA_PAD) * (t->w - 7 + 2 * PT_MFMA_PAD);
     TRY(t->bq.ensure(nq));
     TRY(t->bnorm.ensure(nn));
     HIPTRY(hipMemsetAsync(t->bq.p, 0, nq * 4, s));      // the margins stay zero
-    HIPTRY(hipMemsetAsync(t->bnorm.p, 0, nn * 4, s));
+    TRY(pt_fill_launch((int32_t *)t->bnorm.p, (int64_t)nn, (int32_t)PT_MFMA_NK_PAD, s));
     TRY(pt_mfma_prepare_launch(t->b0.p, t->h, t->w, t->bq.p, t->bnorm.p, s));
     return PT_OK;
 }
@@ -820,6 +821,20 @@ void fill_lookup_args(const pt_table *t, const pt_params &p, int ah, int aw, con
     }
 }
 
+// The MFMA form's candidate pass scores by int8 SSD: the int8 query, the table's int8 copy of b
+// and norm keys, the query norms; a previous field's initial score is its int8 SSD times the
+// temporal weight (sv per patch, else ts).
+void set_int8_args(const pt_table *t, const uint32_t *aq, const int32_t *qnorm, const float *sv, double ts,
+                   LookupArgs &L) {
+    L.int8 = 1;
+    L.aq8 = aq;
+    L.bq8 = t->bq.p;
+    L.bnk = t->bnorm.p;
+    L.qnorm = qnorm;
+    L.init_sv = sv;
+    L.init_ts = ts;
+}
+
 void fill_mfma_args(const pt_table *t, const pt_params &p, const float *a_dev, const uint32_t *aq, int ah, int aw,
                     MfmaArgs &M) {
     M.a = a_dev;
@@ -880,8 +895,12 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
             TRY(t->id.ensure(n));
             TRY(t->bad.ensure(1));
         }
-        if (mfma) TRY(t->aq.ensure((size_t)ah * aw));
+        if (mfma) {
+            TRY(t->aq.ensure((size_t)ah * aw));
+            TRY(t->qnorm.ensure(n));
+        }
         B.aq = t->aq.p;
+        B.qnorm = t->qnorm.p;
         B.adesc = t->adesc.p;
         B.adescI = t->adescI.p;
         B.yuv = t->yuv.p;
@@ -954,10 +973,13 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     const int tfix = prev_dev && p.coherence_temporal != 0.0;
     if (mfma) {
         TRY(pt_quant_rgb0_launch(a_dev, (long)ah * aw, B.aq, s));
+        TRY(pt_query_norms_launch(B.aq, ah, aw, B.qnorm, s));
+        set_int8_args(t, B.aq, B.qnorm, sv_dev, temporal_scale, L);
         TRY(pt_lookup_candidates(L, f0, s));
         if (timing) HIPTRY(hipEventRecord(t->ev[8], s));
         MfmaArgs M;
         fill_mfma_args(t, p, a_dev, B.aq, ah, aw, M);
+        M.qnorm = B.qnorm;
         if (t->mfma_stats && !ws) {
             const size_t n = (size_t)p.mfma_rounds * pt_mfma_round_workgroups(ann_w, ann_h);
             TRY(t->mfma_items.ensure(n));
@@ -965,6 +987,7 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
         }
         int which = 0;
         TRY(pt_mfma_rounds_launch(M, p.mfma_rounds, f0, f1, s, &which));
+        TRY(pt_lookup_exact_moved(L, which ? f1 : f0, s));
         launches = p.mfma_rounds;
         TRY(pt_lookup_output(L, which ? f1 : f0, B.gxy, B.gd, ann_dev, prev_dev, B.id, temporal_scale, tfix, s,
                              &launches, timing ? t->ev[10] : nullptr));
@@ -1067,7 +1090,10 @@ int pt_lookup_workspace_layout(const pt_table *t, int h, int w, int c, bool prev
         B.id = (float *)take(n * 4);
         B.bad = (int *)take(sizeof(int));
     }
-    if (p.mfma_rounds > 0) B.aq = (uint32_t *)take((size_t)ah * aw * 4);
+    if (p.mfma_rounds > 0) {
+        B.aq = (uint32_t *)take((size_t)ah * aw * 4);
+        B.qnorm = (int32_t *)take(n * 4);
+    }
     if (bytes) *bytes = off;
     if (bufs) *bufs = B;
     return PT_OK;
@@ -1694,6 +1720,7 @@ struct pt_batch {
     int tile_x = 64;                  // lattice tile of the lookup kernels (pt_batch_autotune)
     DevBuf<float> adesc, yuv, temp;
     DevBuf<uint32_t> aq;              // int8 RGB0 query (MFMA lookups)
+    DevBuf<int32_t> qnorm;            // ... and |q|^2 of its patches
     DevBuf<PairLookup> pairs;         // every pair's arguments, read by the batched kernels
     std::vector<PairLookup> hpairs;   // ... and their host copy
     ~pt_batch() {
@@ -1761,7 +1788,10 @@ int batch_prepare(pt_batch *bt, const float *a_dev, int h, int w, double *const 
     TRY(bt->yuv.ensure((size_t)h * w * 3));
     TRY(bt->temp.ensure(plane * 3));
     const bool mfma = p.mfma_rounds > 0;
-    if (mfma) TRY(bt->aq.ensure((size_t)h * w));
+    if (mfma) {
+        TRY(bt->aq.ensure((size_t)h * w));
+        TRY(bt->qnorm.ensure((size_t)(h - 7) * (w - 7)));
+    }
     bt->hpairs.assign(n, PairLookup());
     for (int i = 0; i < n; i++) {
         pt_table *t = bt->tabs[i];
@@ -1796,7 +1826,11 @@ int batch_prepare(pt_batch *bt, const float *a_dev, int h, int w, double *const 
         P.G = P.L;
         P.G.gqs = qs;
         P.G.qs = 1;   // gather lanes are single patches
-        if (mfma) fill_mfma_args(t, p, a_dev, bt->aq.p, ah, aw, P.M);
+        if (mfma) {
+            set_int8_args(t, bt->aq.p, bt->qnorm.p, nullptr, 1.0, P.L);
+            fill_mfma_args(t, p, a_dev, bt->aq.p, ah, aw, P.M);
+            P.M.qnorm = bt->qnorm.p;
+        }
         P.f[0].xy = t->nxy.p;
         P.f[0].d = t->nd.p;
         P.f[0].fl = t->nfl.p;
@@ -1826,11 +1860,13 @@ int batch_enqueue(pt_batch *bt, const float *a_dev, int h, int w, int *launches)
     int nl = 1;
     if (p.mfma_rounds > 0) {
         TRY(pt_quant_rgb0_launch(a_dev, (long)h * w, bt->aq.p, s));
+        TRY(pt_query_norms_launch(bt->aq.p, h, w, bt->qnorm.p, s));
         TRY(pt_lookup_batch_candidates(bt->pairs.p, P0, n, s));
         int which = 0;
         TRY(pt_mfma_rounds_batch_launch(bt->pairs.p, P0, n, p.mfma_rounds, s, &which));
+        TRY(pt_lookup_batch_exact_moved(bt->pairs.p, P0, n, which, s));
         TRY(pt_lookup_batch_output(bt->pairs.p, P0, n, which, s, nullptr, &nl));
-        nl += 3 + p.mfma_rounds;
+        nl += 5 + p.mfma_rounds;
     } else {
         int l = 0;
         TRY(pt_lookup_batch_launch(bt->pairs.p, P0, n, s, nullptr, &l));
