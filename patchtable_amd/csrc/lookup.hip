@@ -108,7 +108,7 @@ __device__ __forceinline__ float qdist(const LookupArgs &L, const float4 *qbuf, 
         const int pitch = L.bw + 2 * PT_MFMA_PAD;
         const int32_t *bp = (const int32_t *)L.bq8 + (size_t)(ys + PT_MFMA_PAD) * pitch + xs + PT_MFMA_PAD;
         int dot = 0;
-#pragma unroll
+#pragma unroll 2
         for (int r = 0; r < 8; r++) {
             const int4 b0 = ldg4i(bp + (size_t)r * pitch), b1 = ldg4i(bp + (size_t)r * pitch + 4);
             const uint32_t *q = qt + r * SX;
@@ -477,20 +477,19 @@ __device__ __forceinline__ int table_cell_value(const LookupArgs &L, const float
     return table[cell];
 }
 
-template <int M, int TX>
-__device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField out) {
-    __shared__ float4 qbuf[QBuf<M, TX>::F4];
-    __shared__ BlockQueue Q;
-    if (threadIdx.x == 0) Q.n = 0;
-    const int2 tc = tile_coords();
-    const int vx = tc.x * TX + threadIdx.x % TX, vy = tc.y * (THREADS / TX) + threadIdx.x / TX;
-    const bool valid = vx < L.vw && vy < L.vh;
-    const int x = vx * L.qs, y = vy * L.qs;
-    stage_query<M, TX>(L, qbuf, valid, x, y);
-    const long pi = (long)y * L.ann_w + x;
-    bool have = false, moved = true;   // moved: the match is not the initial (previous-field) one
-    int xb = 0, yb = 0;
-    float db = 0.f;
+// The initial match (a previous field's, scored by distance or, in M_INT8, by int8 SSD times the
+// temporal weight) and the cells of every table (patchtable.h:3664-3737): a cell equal to the
+// current match, or to the previous table's cell, is skipped; the others replace the match when
+// strictly closer. `moved`: the match is not the initial one.
+template <int M, int TX, bool REC = true>
+__device__ __forceinline__ void init_and_tables(const LookupArgs &L, const float4 *qbuf, bool valid, int x, int y,
+                                                long pi, bool &have, bool &moved, int &xb, int &yb, float &db,
+                                                Hist &hist) {
+    have = false;
+    moved = true;
+    xb = 0;
+    yb = 0;
+    db = 0.f;
     if (valid && L.init_xy) {
         const int v = L.init_xy[pi];
         xb = PT_INT_TO_X(v);
@@ -503,12 +502,11 @@ __device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField ou
         have = true;
         moved = false;
     }
-    Hist hist;
     if (valid && L.do_table_lookup) {
         const int v = table_cell_value(L, L.a_desc, L.q, L.table, x, y);
         const int xs = PT_INT_TO_X(v) - 7, ys = PT_INT_TO_Y(v) - 7;
         if (!have || xs != xb || ys != yb) {
-            hist.add(PT_XY_TO_INT(xs, ys));
+            if (REC) hist.add(PT_XY_TO_INT(xs, ys));
             const float d = qdist<M, TX>(L, qbuf, threadIdx.x, xs, ys, have ? db : INFINITY);
             if (!have || d < db) {
                 db = d;
@@ -523,7 +521,7 @@ __device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField ou
             const int v1 = table_cell_value(L, L.xa_desc, L.xq[t], L.xtable[t], x, y);
             const int x1 = PT_INT_TO_X(v1) - 7, y1 = PT_INT_TO_Y(v1) - 7;
             if ((x1 != xb || y1 != yb) && (x1 != xp || y1 != yp)) {
-                hist.add(PT_XY_TO_INT(x1, y1));
+                if (REC) hist.add(PT_XY_TO_INT(x1, y1));
                 const float d = qdist<M, TX>(L, qbuf, threadIdx.x, x1, y1, db);
                 if (d < db) {
                     db = d;
@@ -536,6 +534,24 @@ __device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField ou
             yp = y1;
         }
     }
+}
+
+template <int M, int TX>
+__device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField out) {
+    __shared__ float4 qbuf[QBuf<M, TX>::F4];
+    __shared__ BlockQueue Q;
+    if (threadIdx.x == 0) Q.n = 0;
+    const int2 tc = tile_coords();
+    const int vx = tc.x * TX + threadIdx.x % TX, vy = tc.y * (THREADS / TX) + threadIdx.x / TX;
+    const bool valid = vx < L.vw && vy < L.vh;
+    const int x = vx * L.qs, y = vy * L.qs;
+    stage_query<M, TX>(L, qbuf, valid, x, y);
+    const long pi = (long)y * L.ann_w + x;
+    bool have, moved;
+    int xb, yb;
+    float db;
+    Hist hist;
+    init_and_tables<M, TX>(L, qbuf, valid, x, y, pi, have, moved, xb, yb, db, hist);
     bool kd = false;
     if (L.K > 0 && L.kcoh_iter0) kd = !kcoh_walk_q<M, TX>(Q, qbuf, L, valid, xb, yb, db, hist);
     bool done = false;
@@ -554,7 +570,7 @@ __device__ __forceinline__ void candidates_body(const LookupArgs &L, NnfField ou
     const long v = (long)vy * L.vw + vx;
     out.xy[v] = PT_XY_TO_INT(xb, yb);
     out.d[v] = db;
-    out.fl[v] = (done ? F_DONE : 0) | F_CHG | (kd ? F_KD : 0) | (M == M_INT8 && moved ? F_MOVED : 0);
+    out.fl[v] = (done ? F_DONE : 0) | F_CHG | (kd ? F_KD : 0);
     if (L.hist && L.rounds > 0) {
         if (hist.adds == 0) L.hn[v] = 0;   // nothing evaluated: the count must still be written
         hist.store(L, v, (long)L.vw * L.vh);
@@ -709,6 +725,28 @@ __device__ __forceinline__ void gather_body(const LookupArgs &L, NnfField lat, i
     od[pi] = db;
 }
 
+// MFMA form's candidate pass (image mode, query_step 1): the initial match and the cells of every
+// table, scored by int8 SSD; the MFMA form walks no k-coherence lists and runs no spatial or
+// random search in this pass, so none of candidates_body's queue machinery is needed.
+template <int TX>
+__device__ __forceinline__ void cand_int8_body(const LookupArgs &L, NnfField out) {
+    __shared__ float4 qbuf[QBuf<M_INT8, TX>::F4];
+    const int2 tc = tile_coords();
+    const int x = tc.x * TX + threadIdx.x % TX, y = tc.y * (THREADS / TX) + threadIdx.x / TX;
+    const bool valid = x < L.vw && y < L.vh;
+    stage_query<M_INT8, TX>(L, qbuf, valid, x, y);
+    if (!valid) return;
+    const long pi = (long)y * L.ann_w + x;
+    bool have, moved;
+    int xb, yb;
+    float db;
+    Hist hist;
+    init_and_tables<M_INT8, TX, false>(L, qbuf, true, x, y, pi, have, moved, xb, yb, db, hist);
+    out.xy[pi] = PT_XY_TO_INT(xb, yb);
+    out.d[pi] = db;
+    out.fl[pi] = F_CHG | (moved ? F_MOVED : 0);
+}
+
 // MFMA form's output stage: the field holds int8 scores; a patch whose match is not its initial
 // one gets the exact FP32 SSD of the match (the reference's lane order), a patch holding its
 // initial (previous-field) match its initial distance.
@@ -761,6 +799,15 @@ template <int M, int TX>
 __global__ __launch_bounds__(THREADS) void k_gather_batch(const PairLookup *__restrict__ P, int src) {
     const PairLookup &p = P[blockIdx.z];
     gather_body<M, TX>(p.G, p.f[src], p.gxy, p.gd);
+}
+template <int TX>
+__global__ __launch_bounds__(THREADS) void k_cand_int8(LookupArgs L, NnfField out) {
+    cand_int8_body<TX>(L, out);
+}
+template <int TX>
+__global__ __launch_bounds__(THREADS) void k_cand_int8_batch(const PairLookup *__restrict__ P) {
+    const PairLookup &p = P[blockIdx.z];
+    cand_int8_body<TX>(p.L, p.f[0]);
 }
 template <int TX>
 __global__ __launch_bounds__(THREADS) void k_exact_moved(LookupArgs L, NnfField f) {
@@ -952,8 +999,10 @@ int pt_lookup_candidates(const LookupArgs &L, NnfField f0, hipStream_t s) {
         constexpr int TX = decltype(tx_)::value;
         if (L.C > 0) {
             if constexpr (TX == 64) launch_candidates<M_DESC, TX>(L, f0, s);
-        } else if (L.int8) launch_candidates<M_INT8, TX>(L, f0, s);
-        else if (L.qs == 1) launch_candidates<M_TILE, TX>(L, f0, s);
+        } else if (L.int8) {
+            dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX));
+            hipLaunchKernelGGL((k_cand_int8<TX>), grid, dim3(THREADS), 0, s, L, f0);
+        } else if (L.qs == 1) launch_candidates<M_TILE, TX>(L, f0, s);
         else launch_candidates<M_GLOBAL, TX>(L, f0, s);
         PT_LAUNCH_CHECK();
         return PT_OK;
@@ -1033,7 +1082,7 @@ int pt_lookup_batch_candidates(const PairLookup *P_dev, const PairLookup &P0, in
     return with_tile(L.tile_x, 0, [&](auto tx_) {
         constexpr int TX = decltype(tx_)::value;
         dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX), npairs);
-        if (L.int8) hipLaunchKernelGGL((k_candidates_batch<M_INT8, TX>), grid, dim3(THREADS), 0, s, P_dev);
+        if (L.int8) hipLaunchKernelGGL((k_cand_int8_batch<TX>), grid, dim3(THREADS), 0, s, P_dev);
         else if (L.qs == 1) hipLaunchKernelGGL((k_candidates_batch<M_TILE, TX>), grid, dim3(THREADS), 0, s, P_dev);
         else hipLaunchKernelGGL((k_candidates_batch<M_GLOBAL, TX>), grid, dim3(THREADS), 0, s, P_dev);
         PT_LAUNCH_CHECK();

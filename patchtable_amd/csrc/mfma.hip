@@ -228,24 +228,18 @@ __device__ __forceinline__ void band_fetch(const MfmaArgs &M, int x0, int y, int
 
 // One wave per workgroup: the workgroup owns 64 x 1 query patches, so a wave leaves as soon as its
 // own items are done (its row's item count varies with the image).
-constexpr int MW = 64, MQY = 1 + 7;   // threads; query rows staged (the row's patches)
+constexpr int MW = 64;   // threads
 
 // `stable`: the previous round had the same jump J and its field's flags mark the patches whose
 // match it changed. A group none of whose valid sources changed would then compute the anchors,
 // winners and rescorings of the previous round again and keep every match: it takes no items.
 __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur, NnfField nxt, int J, int stable) {
-    __shared__ uint32_t qq[AX * MQY];   // int8 RGB0 query pixels (A fragments)
     // the hash tables, source offsets and anchors of phase 1, then the two band buffers of phase 2
     __shared__ uint32_t wsh[1][WSH];
     __shared__ int items[1][MAXITEMS][3];
     __shared__ uint32_t fin[16 * 2];      // (key, jj) of the 16 members at the end of a group
     const int2 tc = tile_coords();
     const int x00 = tc.x * TILE_X, y00 = tc.y;
-    for (int i = threadIdx.x; i < AX * MQY; i += MW) {
-        const int ly = i / AX, lx = i - ly * AX, gx = x00 + lx, gy = y00 + ly;
-        qq[i] = gx < M.aw && gy < M.ah ? M.aq[(size_t)gy * M.aw + gx] : 0u;
-    }
-    __syncthreads();
     const int wave = 0, lane = threadIdx.x;
     const int g = lane >> 4, jj = lane & 15;
     const int y = y00 + wave, xw0 = x00;
@@ -404,35 +398,47 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
             cur_g = gi;
             ga = 0;
             gfirst = it;
-            // A fragments of member jj: k-step s covers patch rows 2s + (g >> 1), pixels 4(g & 1)..+3
+            // A fragments of member jj: k-step s covers patch rows 2s + (g >> 1), pixels 4(g & 1)..+3,
+            // read from the int8 query (a valid member's patch lies inside it)
             const bool mv = x0 + jj < aw;
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                const uint32_t *src = qq + (wave + 2 * s + (g >> 1)) * AX + 16 * gi + jj + 4 * (g & 1);
-                A[s] = mv ? v4i{(int)src[0], (int)src[1], (int)src[2], (int)src[3]} : v4i{0, 0, 0, 0};
+                const uint32_t *src = M.aq + (size_t)(y + 2 * s + (g >> 1)) * M.aw + x0 + jj + 4 * (g & 1);
+                A[s] = mv ? *(const v4i *)__builtin_assume_aligned(src, 4) : v4i{0, 0, 0, 0};
             }
         }
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             const int col = 16 * b + jj + 4 * (g & 1);
-            v4i B[9];   // band row pairs from row r0 (+ (g >> 1) for this lane's half)
-#pragma unroll
-            for (int r0 = 0; r0 < 9; r0++) {
-                const uint32_t *p = bnd + (r0 + (g >> 1)) * BC + col;
-                B[r0] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
-            }
-#pragma unroll
-            for (int dyi = 0; dyi < 2 * R + 1; dyi++) {
+            // band row pairs r0 (+ (g >> 1) for this lane's half): candidate row dyi uses r0 = dyi + 2s.
+            // Rows 0 and 2 use the even pairs, row 1 the odd ones: the even pairs are loaded and
+            // used first, then the odd ones, so at most 5 pairs are held at a time.
+            auto score = [&](int dyi, const v4i (&Bp)[5], int first) {
                 v4i C = v4i{0, 0, 0, 0};
 #pragma unroll
-                for (int s = 0; s < 4; s++) C = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s], B[dyi + 2 * s], C, 0, 0, 0);
+                for (int s = 0; s < 4; s++) C = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s], Bp[first + s], C, 0, 0, 0);
                 const uint32_t nk = bnd[BR * BC + dyi * 32 + 16 * b + jj];   // read for every lane (no branch)
                 const uint32_t base = nk | (uint32_t)((ga * (2 * R + 1) + dyi) * 2 + b);
                 // C[t]: member 4g + t against candidate column jj; |q.p| < 2^22 fits the signed
                 // 24-bit multiply, so base - 256 q.p is one multiply-add
 #pragma unroll
                 for (int t = 0; t < 4; t++) best[t] = min(best[t], base + (uint32_t)__mul24(C[t], m256));
+            };
+            static_assert(R == 1, "the even/odd split assumes candidate rows -1, 0, +1");
+            v4i Bp[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {   // even pairs 0, 2, 4, 6, 8
+                const uint32_t *p = bnd + (2 * k + (g >> 1)) * BC + col;
+                Bp[k] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
             }
+            score(0, Bp, 0);
+            score(2, Bp, 1);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {   // odd pairs 1, 3, 5, 7
+                const uint32_t *p = bnd + (2 * k + 1 + (g >> 1)) * BC + col;
+                Bp[k] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
+            }
+            score(1, Bp, 0);
         }
         ga++;
         if (it + 1 == nitems || item(it + 1, 0) != gi) {
@@ -499,12 +505,12 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
     nxt.fl[vi] = fl | (v != v_in ? F_CHG : 0);
 }
 
-__global__ __launch_bounds__(MW, 4) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J, int stable) {
+__global__ __launch_bounds__(MW, 5) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J, int stable) {
     mfma_round_body(M, cur, nxt, J, stable);
 }
 
 // One round of every pair of a batch (blockIdx.z = pair), from field src into field src ^ 1.
-__global__ __launch_bounds__(MW, 4) void k_mfma_round_batch(const PairLookup *__restrict__ P, int src, int J,
+__global__ __launch_bounds__(MW, 5) void k_mfma_round_batch(const PairLookup *__restrict__ P, int src, int J,
                                                             int stable) {
     const PairLookup &p = P[blockIdx.z];
     mfma_round_body(p.M, p.f[src], p.f[src ^ 1], J, stable);
