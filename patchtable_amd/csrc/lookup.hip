@@ -737,6 +737,15 @@ __device__ __forceinline__ void cand_int8_body(const LookupArgs &L, NnfField out
     stage_query<M_INT8, TX>(L, qbuf, valid, x, y);
     if (!valid) return;
     const long pi = (long)y * L.ann_w + x;
+    {   // |q|^2 of the lane's patch, kept for the rounds (and read back by qdist below)
+        const uint32_t *qt = (const uint32_t *)qbuf + (threadIdx.x / TX) * (TX + 7) + (threadIdx.x % TX);
+        int qn = 0;
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int c = 0; c < 8; c++) qn = __builtin_amdgcn_sdot4((int)qt[r * (TX + 7) + c], (int)qt[r * (TX + 7) + c], qn, false);
+        const_cast<int32_t *>(L.qnorm)[pi] = qn;
+    }
     bool have, moved;
     int xb, yb;
     float db;
@@ -747,11 +756,13 @@ __device__ __forceinline__ void cand_int8_body(const LookupArgs &L, NnfField out
     out.fl[pi] = F_CHG | (moved ? F_MOVED : 0);
 }
 
-// MFMA form's output stage: the field holds int8 scores; a patch whose match is not its initial
-// one gets the exact FP32 SSD of the match (the reference's lane order), a patch holding its
-// initial (previous-field) match its initial distance.
+// MFMA form's output stage (NNF_X, NNF_Y, NNF_DIST as doubles, as k_write_ann): the field holds
+// int8 scores; a patch whose match is not its initial one gets the exact FP32 SSD of the match
+// (the reference's lane order), a patch holding its initial (previous-field) match its initial
+// distance, with the temporal rescaling of patchtable.h:4178-4193.
 template <int TX>
-__device__ __forceinline__ void exact_moved_body(const LookupArgs &L, NnfField f) {
+__device__ __forceinline__ void exact_out_body(const LookupArgs &L, NnfField f, double *__restrict__ ann,
+                                               const double *__restrict__ prev, double temporal_scale, int fix) {
     __shared__ float4 qbuf[QBuf<M_TILE, TX>::F4];
     const int2 tc = tile_coords();
     const int vx = tc.x * TX + threadIdx.x % TX, vy = tc.y * (THREADS / TX) + threadIdx.x / TX;
@@ -759,12 +770,15 @@ __device__ __forceinline__ void exact_moved_body(const LookupArgs &L, NnfField f
     stage_query<M_TILE, TX>(L, qbuf, valid, vx, vy);
     if (!valid) return;
     const long v = (long)vy * L.vw + vx;
-    if (!(f.fl[v] & F_MOVED)) {   // the initial (previous-field) match keeps its initial distance
-        if (L.init_d) f.d[v] = L.init_d[v];
-        return;
-    }
     const int c = f.xy[v];
-    f.d[v] = qdist<M_TILE, TX>(L, qbuf, threadIdx.x, PT_INT_TO_X(c), PT_INT_TO_Y(c), INFINITY);
+    const int x = PT_INT_TO_X(c), y = PT_INT_TO_Y(c);
+    const float d = (f.fl[v] & F_MOVED) || !L.init_d ? qdist<M_TILE, TX>(L, qbuf, threadIdx.x, x, y, INFINITY)
+                                                     : L.init_d[v];
+    double dd = (double)d;
+    if (fix && (double)x == prev[v * 3] && (double)y == prev[v * 3 + 1] && d == L.init_d[v]) dd = dd / temporal_scale;
+    ann[v * 3 + 0] = (double)x;
+    ann[v * 3 + 1] = (double)y;
+    ann[v * 3 + 2] = dd;
 }
 
 // Kernels of one lookup (arguments by value) and of a batch of lookups of one query against many
@@ -810,13 +824,14 @@ __global__ __launch_bounds__(THREADS) void k_cand_int8_batch(const PairLookup *_
     cand_int8_body<TX>(p.L, p.f[0]);
 }
 template <int TX>
-__global__ __launch_bounds__(THREADS) void k_exact_moved(LookupArgs L, NnfField f) {
-    exact_moved_body<TX>(L, f);
+__global__ __launch_bounds__(THREADS) void k_exact_out(LookupArgs L, NnfField f, double *__restrict__ ann,
+                                                      const double *__restrict__ prev, double temporal_scale, int fix) {
+    exact_out_body<TX>(L, f, ann, prev, temporal_scale, fix);
 }
 template <int TX>
-__global__ __launch_bounds__(THREADS) void k_exact_moved_batch(const PairLookup *__restrict__ P, int src) {
+__global__ __launch_bounds__(THREADS) void k_exact_out_batch(const PairLookup *__restrict__ P, int src) {
     const PairLookup &p = P[blockIdx.z];
-    exact_moved_body<TX>(p.L, p.f[src]);
+    exact_out_body<TX>(p.L, p.f[src], p.ann, nullptr, 1.0, 0);
 }
 
 // Initial field from a previous one (TABLE_COPY_PREV_NNF and patchtable.h:3539-3585).
@@ -1090,24 +1105,26 @@ int pt_lookup_batch_candidates(const PairLookup *P_dev, const PairLookup &P0, in
     });
 }
 
-int pt_lookup_exact_moved(const LookupArgs &L, NnfField f, hipStream_t s) {
+int pt_lookup_exact_out(const LookupArgs &L, NnfField f, double *ann_out, const double *prev, double temporal_scale,
+                        int temporal_fix, hipStream_t s) {
     if (L.C > 0 || L.qs != 1) return pt_fail(PT_E_UNSUPPORTED, "int8 scores are kept in image mode with query_step 1");
     return with_tile(L.tile_x, 0, [&](auto tx_) {
         constexpr int TX = decltype(tx_)::value;
         dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX));
-        hipLaunchKernelGGL((k_exact_moved<TX>), grid, dim3(THREADS), 0, s, L, f);
+        hipLaunchKernelGGL((k_exact_out<TX>), grid, dim3(THREADS), 0, s, L, f, ann_out, prev, temporal_scale,
+                           temporal_fix);
         PT_LAUNCH_CHECK();
         return PT_OK;
     });
 }
 
-int pt_lookup_batch_exact_moved(const PairLookup *P_dev, const PairLookup &P0, int npairs, int src, hipStream_t s) {
+int pt_lookup_batch_exact_out(const PairLookup *P_dev, const PairLookup &P0, int npairs, int src, hipStream_t s) {
     const LookupArgs &L = P0.L;
     if (L.C > 0 || L.qs != 1) return pt_fail(PT_E_UNSUPPORTED, "int8 scores are kept in image mode with query_step 1");
     return with_tile(L.tile_x, 0, [&](auto tx_) {
         constexpr int TX = decltype(tx_)::value;
         dim3 grid((L.vw + TX - 1) / TX, (L.vh + THREADS / TX - 1) / (THREADS / TX), npairs);
-        hipLaunchKernelGGL((k_exact_moved_batch<TX>), grid, dim3(THREADS), 0, s, P_dev, src);
+        hipLaunchKernelGGL((k_exact_out_batch<TX>), grid, dim3(THREADS), 0, s, P_dev, src);
         PT_LAUNCH_CHECK();
         return PT_OK;
     });

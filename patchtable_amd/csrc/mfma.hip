@@ -101,20 +101,6 @@ __global__ void k_patch_norms_pad(const uint32_t *__restrict__ q, int w, int pw,
     out[(size_t)(y + P) * (pw + 2 * P) + x + P] = ((uint32_t)s + PT_MFMA_OFF) << 7;
 }
 
-// |q|^2 of every 8x8 patch of an unpadded int8 RGB0 image (aw pixels wide; pw x ph patches).
-__global__ void k_query_norms(const uint32_t *__restrict__ q, int aw, int pw, int ph, int32_t *__restrict__ out) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)pw * ph) return;
-    const int x = (int)(i % pw), y = (int)(i / pw);
-    int s = 0;
-    for (int r = 0; r < 8; r++) {
-        const uint32_t *row = q + (size_t)(y + r) * aw + x;
-#pragma unroll
-        for (int c = 0; c < 8; c++) s += sq_pixel(row[c]);
-    }
-    out[i] = s;
-}
-
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
     const int lo = __shfl((int)(uint32_t)v, src), hi = __shfl((int)(uint32_t)(v >> 32), src);
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
@@ -521,14 +507,6 @@ __global__ __launch_bounds__(MW, 5) void k_mfma_round_batch(const PairLookup *__
 int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t s) {
     if (npx <= 0) return PT_OK;
     hipLaunchKernelGGL(k_quant_rgb0, dim3(pt_blocks(npx, 256)), dim3(256), 0, s, img, npx, out);
-    PT_LAUNCH_CHECK();
-    return PT_OK;
-}
-
-int pt_query_norms_launch(const uint32_t *aq, int ah, int aw, int32_t *qnorm, hipStream_t s) {
-    const long n = (long)(aw - 7) * (ah - 7);
-    if (n <= 0) return PT_OK;
-    hipLaunchKernelGGL(k_query_norms, dim3(pt_blocks(n, 256)), dim3(256), 0, s, aq, aw, aw - 7, ah - 7, qnorm);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }

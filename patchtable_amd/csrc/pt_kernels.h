@@ -208,8 +208,9 @@ struct LookupArgs {
     int tile_x = 64;                // lattice tile of a workgroup: 64 x 4, 32 x 8 or 16 x 16 patches
                                     // (image mode; results do not depend on it)
     // MFMA form: the candidate pass scores by int8 SSD (image mode, query_step 1), from the int8
-    // RGB0 query (ah x aw), the padded int8 copy of b and its norm keys and |q|^2 of every query
-    // patch; a previous field's initial score is its int8 SSD x (init_sv ? init_sv[i] : init_ts)
+    // RGB0 query (ah x aw), the padded int8 copy of b and its norm keys; the pass writes |q|^2 of
+    // every query patch to qnorm (read again by the rounds); a previous field's initial score is
+    // its int8 SSD x (init_sv ? init_sv[i] : init_ts)
     int int8 = 0;
     const uint32_t *aq8 = nullptr, *bq8 = nullptr, *bnk = nullptr;
     const int32_t *qnorm = nullptr;
@@ -267,8 +268,6 @@ static inline int pt_mfma_jump_at(int rounds, int r) {
     return j > 1 ? j : 1;
 }
 int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t s);
-// |q|^2 of every 8x8 patch of the int8 RGB0 query (ah x aw pixels), (ah-7) x (aw-7) int32.
-int pt_query_norms_launch(const uint32_t *aq, int ah, int aw, int32_t *qnorm, hipStream_t s);
 
 // The padded int8 copy of b and its patch norm keys (margins written by the caller: zeros in the
 // copy, PT_MFMA_NK_PAD in the norm keys).
@@ -299,10 +298,13 @@ int pt_lookup_batch_candidates(const PairLookup *P_dev, const PairLookup &P0, in
 // MFMA rounds of every pair of a batch, one launch per round; *which = the field with the result.
 int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, int rounds, hipStream_t s,
                                 int *which);
-// MFMA form: the exact FP32 SSD (the reference's lane order) of every patch whose field flags
-// mark a match other than its initial one (F_MOVED), into the field's distances: lookup.hip.
-int pt_lookup_exact_moved(const LookupArgs &L, NnfField f, hipStream_t s);
-int pt_lookup_batch_exact_moved(const PairLookup *P_dev, const PairLookup &P0, int npairs, int src, hipStream_t s);
+// MFMA form's output (lookup.hip): NNF_X, NNF_Y and NNF_DIST as doubles, NNF_DIST the exact FP32
+// SSD (the reference's lane order) of every match other than the initial one (field flag
+// F_MOVED), the initial distance for a patch that kept its initial match, and the temporal
+// rescaling of a previous field; the batch form writes every pair's output (no previous field).
+int pt_lookup_exact_out(const LookupArgs &L, NnfField f, double *ann_out, const double *prev, double temporal_scale,
+                        int temporal_fix, hipStream_t s);
+int pt_lookup_batch_exact_out(const PairLookup *P_dev, const PairLookup &P0, int npairs, int src, hipStream_t s);
 // Tile widths the lookup kernels are built for.
 int pt_lookup_tile_ok(int tile_x);
 // Output stage of a lookup (gather for query_step > 1, then the double field): lookup.hip.

@@ -973,7 +973,6 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
     const int tfix = prev_dev && p.coherence_temporal != 0.0;
     if (mfma) {
         TRY(pt_quant_rgb0_launch(a_dev, (long)ah * aw, B.aq, s));
-        TRY(pt_query_norms_launch(B.aq, ah, aw, B.qnorm, s));
         set_int8_args(t, B.aq, B.qnorm, sv_dev, temporal_scale, L);
         TRY(pt_lookup_candidates(L, f0, s));
         if (timing) HIPTRY(hipEventRecord(t->ev[8], s));
@@ -987,10 +986,11 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
         }
         int which = 0;
         TRY(pt_mfma_rounds_launch(M, p.mfma_rounds, f0, f1, s, &which));
-        TRY(pt_lookup_exact_moved(L, which ? f1 : f0, s));
         launches = p.mfma_rounds;
-        TRY(pt_lookup_output(L, which ? f1 : f0, B.gxy, B.gd, ann_dev, prev_dev, B.id, temporal_scale, tfix, s,
-                             &launches, timing ? t->ev[10] : nullptr));
+        if (timing) HIPTRY(hipEventRecord(t->ev[10], s));
+        // the exact distances and the double output in one pass (k_init_prev's initial distances,
+        // L.init_d, serve the patches that kept a previous field's match)
+        TRY(pt_lookup_exact_out(L, which ? f1 : f0, ann_dev, prev_dev, temporal_scale, tfix, s));
     } else {
         TRY(pt_lookup_launch(L, f0, f1, B.gxy, B.gd, ann_dev, prev_dev, B.id, temporal_scale, tfix, s,
                              timing ? t->ev[8] : nullptr, timing ? t->ev[10] : nullptr, &launches));
@@ -1860,13 +1860,11 @@ int batch_enqueue(pt_batch *bt, const float *a_dev, int h, int w, int *launches)
     int nl = 1;
     if (p.mfma_rounds > 0) {
         TRY(pt_quant_rgb0_launch(a_dev, (long)h * w, bt->aq.p, s));
-        TRY(pt_query_norms_launch(bt->aq.p, h, w, bt->qnorm.p, s));
         TRY(pt_lookup_batch_candidates(bt->pairs.p, P0, n, s));
         int which = 0;
         TRY(pt_mfma_rounds_batch_launch(bt->pairs.p, P0, n, p.mfma_rounds, s, &which));
-        TRY(pt_lookup_batch_exact_moved(bt->pairs.p, P0, n, which, s));
-        TRY(pt_lookup_batch_output(bt->pairs.p, P0, n, which, s, nullptr, &nl));
-        nl += 5 + p.mfma_rounds;
+        TRY(pt_lookup_batch_exact_out(bt->pairs.p, P0, n, which, s));
+        nl += 4 + p.mfma_rounds;
     } else {
         int l = 0;
         TRY(pt_lookup_batch_launch(bt->pairs.p, P0, n, s, nullptr, &l));
