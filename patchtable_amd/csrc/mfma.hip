@@ -384,47 +384,43 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
             cur_g = gi;
             ga = 0;
             gfirst = it;
-            // A fragments of member jj: k-step s covers patch rows 2s + (g >> 1), pixels 4(g & 1)..+3,
+            // A fragments of member jj: k-step s covers patch row s + 4 (g >> 1), pixels 4(g & 1)..+3,
             // read from the int8 query (a valid member's patch lies inside it)
             const bool mv = x0 + jj < aw;
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                const uint32_t *src = M.aq + (size_t)(y + 2 * s + (g >> 1)) * M.aw + x0 + jj + 4 * (g & 1);
+                const uint32_t *src = M.aq + (size_t)(y + s + 4 * (g >> 1)) * M.aw + x0 + jj + 4 * (g & 1);
                 A[s] = mv ? *(const v4i *)__builtin_assume_aligned(src, 4) : v4i{0, 0, 0, 0};
             }
         }
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             const int col = 16 * b + jj + 4 * (g & 1);
-            // band row pairs r0 (+ (g >> 1) for this lane's half): candidate row dyi uses r0 = dyi + 2s.
-            // Rows 0 and 2 use the even pairs, row 1 the odd ones: the even pairs are loaded and
-            // used first, then the odd ones, so at most 5 pairs are held at a time.
-            auto score = [&](int dyi, const v4i (&Bp)[5], int first) {
+            // B fragment k: band row k + 4 (g >> 1), words col..col+3; candidate row dyi at k-step s
+            // uses fragment dyi + s, so the three rows share six fragments
+            // (fragment k is loaded just before row k - 3 first needs it and dropped after row k,
+            // so at most five are held)
+            v4i F[6];
+            auto frag = [&](int k) {
+                const uint32_t *p = bnd + (k + 4 * (g >> 1)) * BC + col;
+                F[k] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
+            };
+            static_assert(R == 1, "six fragments cover candidate rows -1, 0, +1");
+#pragma unroll
+            for (int k = 0; k < 4; k++) frag(k);
+#pragma unroll
+            for (int dyi = 0; dyi < 2 * R + 1; dyi++) {
+                if (dyi > 0) frag(dyi + 3);
                 v4i C = v4i{0, 0, 0, 0};
 #pragma unroll
-                for (int s = 0; s < 4; s++) C = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s], Bp[first + s], C, 0, 0, 0);
+                for (int s = 0; s < 4; s++) C = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s], F[dyi + s], C, 0, 0, 0);
                 const uint32_t nk = bnd[BR * BC + dyi * 32 + 16 * b + jj];   // read for every lane (no branch)
                 const uint32_t base = nk | (uint32_t)((ga * (2 * R + 1) + dyi) * 2 + b);
                 // C[t]: member 4g + t against candidate column jj; |q.p| < 2^22 fits the signed
                 // 24-bit multiply, so base - 256 q.p is one multiply-add
 #pragma unroll
                 for (int t = 0; t < 4; t++) best[t] = min(best[t], base + (uint32_t)__mul24(C[t], m256));
-            };
-            static_assert(R == 1, "the even/odd split assumes candidate rows -1, 0, +1");
-            v4i Bp[5];
-#pragma unroll
-            for (int k = 0; k < 5; k++) {   // even pairs 0, 2, 4, 6, 8
-                const uint32_t *p = bnd + (2 * k + (g >> 1)) * BC + col;
-                Bp[k] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
             }
-            score(0, Bp, 0);
-            score(2, Bp, 1);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {   // odd pairs 1, 3, 5, 7
-                const uint32_t *p = bnd + (2 * k + 1 + (g >> 1)) * BC + col;
-                Bp[k] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
-            }
-            score(1, Bp, 0);
         }
         ga++;
         if (it + 1 == nitems || item(it + 1, 0) != gi) {
