@@ -459,6 +459,12 @@ def time_form(torch, switches, a, b, H, W, args, form):
     info = table.info()
     log(f"{form}: build {statistics.median(build_ms):.2f} ms")
     ann = torch.empty((H - 7, W - 7, 3), dtype=torch.float64, device="cuda")
+    if args.tile:
+        table.set_tile(args.tile)
+        tune = {"tile_x": args.tile}
+    else:   # the library's autotuner picks the lattice tile (results do not depend on it)
+        tune = table.autotune(a, ann, reps=3)
+        log(f"{form}: autotune {tune}")
     for _ in range(args.warmup):
         table.lookup(a, ann)
     torch.cuda.synchronize()
@@ -478,7 +484,9 @@ def time_form(torch, switches, a, b, H, W, args, form):
            "mean_dist": float(ann[..., 2].clamp(min=0).sqrt().mean().item()),
            "stage_ms": {"query_gck": mean("query_gck"), "candidates": mean("query_candidates"),
                         "propagation": mean("query_propagation"), "total": mean("query_total")},
-           "launches": int(st_all[-1]["propagation_launches"])}
+           "launches": int(st_all[-1]["propagation_launches"]),
+           "tile_x": tune["tile_x"],
+           "autotune_median_ms": {str(k): round(v, 3) for k, v in tune.get("median_ms", {}).items()}}
     if "-mfma_rounds" in switches:   # one more (untimed) lookup counting the rounds' items
         table.set_mfma_stats(True)
         table.lookup(a, ann)
@@ -494,7 +502,8 @@ def summary(f, nq):
             "mean_ssd": round(f["mean_ssd"], 6), "mean_dist": round(f["mean_dist"], 6),
             "table_build_ms": round(f["build_ms"], 3), "build_stage_ms": f["build_stages"],
             "lookup_stage_ms": {k: round(v, 4) for k, v in f["stage_ms"].items()},
-            "launches_per_step": f["launches"],
+            "launches_per_step": f["launches"], "tile_x": f["tile_x"],
+            **({"autotune_median_ms": f["autotune_median_ms"]} if f["autotune_median_ms"] else {}),
             "params": "PatchTableParams()+set_speed(1)" + (" " + f["switches"] if f["switches"] else ""),
             **({"mfma_items_per_round": f["mfma_items"]} if "mfma_items" in f else {})}
 
@@ -584,6 +593,9 @@ def main():
                     help="also run a 1 GiB device copy (counter calibration for the traffic figure)")
     ap.add_argument("--mfma-grid", default=";".join(MFMA_GRID),
                     help="MFMA-form switch sets timed beside the plain form, separated by ';'")
+    ap.add_argument("--tile", type=int, default=0, choices=(0, 64, 32, 16),
+                    help="lattice tile width of the lookup kernels (64 x 4, 32 x 8 or 16 x 16 patches); "
+                         "0: autotuned per form")
     ap.add_argument("--cpu-baseline-child", type=int, default=0, metavar="THREADS", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_baseline_child:

@@ -387,6 +387,12 @@ int pt_table_mfma_items(const pt_table *t, int64_t *items, int n, int *rounds);
 /* Lattice tile of this table's plain lookup kernels: 64 x 4 (default), 32 x 8 or 16 x 16 patches
  * (tile_x 64, 32, 16); results do not depend on it. No counterpart in the reference. */
 int pt_table_set_tile(pt_table *t, int tile_x);
+/* Times lookups of this query (device, h x w x c, into ann_dev) with each tile width (one
+ * warm-up and `reps` timed lookups each, device time from the table's events), keeps the
+ * fastest (median) and returns it; ms_out[3] (optional): the medians for 64, 32 and 16. No
+ * counterpart in the reference (an addition of this library). */
+int pt_table_autotune(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, int reps,
+                      int *tile_x_out, double *ms_out);
 
 /* ---- Stand-alone kernels (device pointers, stream 0 of the current device) ----------------
  * Used by the parity tests and the benchmark to exercise one kernel at a time. */

@@ -279,6 +279,20 @@ class PatchTable:
         (tile_x 64, 32, 16); results do not depend on it."""
         check(load().pt_table_set_tile(self._t, int(tile_x)))
 
+    def autotune(self, a, ann, reps=5):
+        """Times lookups of the device query a (into the device field ann) with each lattice tile
+        and keeps the fastest; returns {"tile_x": chosen, "median_ms": {64: .., 32: .., 16: ..}}."""
+        import torch
+        _check_dev(a, "a", torch.float32)
+        h, w = int(a.shape[0]), int(a.shape[1])
+        _check_dev(ann, "ann", torch.float64, (h - 7, w - 7, 3), a.device)
+        _after_torch(a.device)
+        tx = ctypes.c_int()
+        ms = (ctypes.c_double * 3)()
+        check(load().pt_table_autotune(self._t, ctypes.c_void_p(a.data_ptr()), h, w, int(a.shape[2]),
+                                       ctypes.c_void_p(ann.data_ptr()), int(reps), ctypes.byref(tx), ms))
+        return {"tile_x": tx.value, "median_ms": {64: ms[0], 32: ms[1], 16: ms[2]}}
+
     def set_mfma_stats(self, on=True):
         """Count the MFMA rounds' items in later lookups (instrumentation; see mfma_items)."""
         check(load().pt_table_set_mfma_stats(self._t, 1 if on else 0))

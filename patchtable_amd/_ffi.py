@@ -161,6 +161,7 @@ SIGNATURES = {
     "pt_batch_get_tile": (_int, [_vp]),
     "pt_batch_autotune": (_int, [_vp, _vp, _int, _int, _P(_vp), _int, _P(_int), _f64p]),
     "pt_table_set_tile": (_int, [_vp, _int]),
+    "pt_table_autotune": (_int, [_vp, _vp, _int, _int, _int, _vp, _int, _P(_int), _f64p]),
     "pt_image_size": (_int, [ctypes.c_char_p, _P(_int), _P(_int), _P(_int)]),
     "pt_image_load": (_int, [ctypes.c_char_p, _f32p, _int, _int, _int]),
     "pt_pfm_save": (_int, [ctypes.c_char_p, _f64p, _int, _int, _int]),

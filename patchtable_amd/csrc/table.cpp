@@ -1662,6 +1662,38 @@ extern "C" int pt_table_set_tile(pt_table *t, int tile_x) {
     return PT_OK;
 }
 
+extern "C" int pt_table_autotune(pt_table *t, const float *a_dev, int h, int w, int c, double *ann_dev, int reps,
+                                 int *tile_x_out, double *ms_out) {
+    if (!t || !a_dev || !ann_dev || reps < 1) return pt_fail(PT_E_INVALID, "pt_table_autotune: bad arguments");
+    static const int shapes[3] = {64, 32, 16};
+    const int keep = t->tile_x;
+    double best = 0;
+    int best_tx = keep;
+    for (int k = 0; k < 3; k++) {
+        t->tile_x = shapes[k];
+        int rc = lookup_common(t, a_dev, h, w, c, ann_dev, nullptr, nullptr, nullptr);   // warm-up
+        std::vector<double> ms;
+        for (int r = 0; rc == PT_OK && r < reps; r++) {
+            rc = lookup_common(t, a_dev, h, w, c, ann_dev, nullptr, nullptr, nullptr);
+            ms.push_back(t->stage_ms[9]);   // device time of the whole lookup
+        }
+        if (rc != PT_OK) {
+            t->tile_x = keep;
+            return rc;
+        }
+        std::sort(ms.begin(), ms.end());
+        const double med = ms[ms.size() / 2];
+        if (ms_out) ms_out[k] = med;
+        if (k == 0 || med < best) {
+            best = med;
+            best_tx = shapes[k];
+        }
+    }
+    t->tile_x = best_tx;
+    if (tile_x_out) *tile_x_out = best_tx;
+    return PT_OK;
+}
+
 extern "C" int pt_table_mfma_items(const pt_table *t, int64_t *items, int n, int *rounds) {
     if (!t || !rounds) return pt_fail(PT_E_INVALID, "null argument");
     *rounds = (int)t->mfma_items_sum.size();

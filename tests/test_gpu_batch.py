@@ -76,6 +76,25 @@ def test_batch_one_launch_per_phase_every_tile(switches):
     batch.close()
 
 
+def test_table_autotune_keeps_results():
+    """The single-table autotuner times each lattice tile and keeps the fastest; the lookup after
+    it gives the same NNF as with the default tile."""
+    import torch
+    from patchtable_amd import PatchTable, synth_pair
+    from conftest import speed1
+    a_np, b_np = synth_pair(260, 150, 530)
+    for switches in ("-limit 1", "-limit 1 -mfma_rounds 3 -mfma_anchors 4"):
+        p = speed1(switches)
+        t = PatchTable(p, torch.from_numpy(b_np).cuda())
+        ref = t.lookup(a_np)
+        a = torch.from_numpy(a_np).cuda()
+        ann = torch.empty((143, 523, 3), dtype=torch.float64, device="cuda")
+        res = t.autotune(a, ann, reps=2)
+        print(f"autotune {switches}: {res}", flush=True)
+        assert res["tile_x"] in (64, 32, 16)
+        assert np.array_equal(t.lookup(a, ann).cpu().numpy(), ref)
+
+
 def test_batch_replay_follows_parameter_changes():
     """The tables read the caller's parameters at every lookup (patchtable.h:2602). Changing
     lookup-time fields between calls with the same buffers must not replay a graph captured with
