@@ -21,7 +21,7 @@
 #include "pt_kernels.h"
 #include "pt_tile.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace {
 
@@ -316,11 +316,11 @@ int pt_kcoherence_launch(const float *desc, size_t plane, int Wp, int ndims, con
         Na = hc;
     }
     size_t tmp_bytes = 0;
-    PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k0s, idx, perm0, (int)N, 0, nbits, s));
+    PT_HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, k0, k0s, idx, perm0, (int)N, 0, nbits, s));
     void *tmp = sc.get(30, tmp_bytes);
     if (!tmp) return pt_fail(PT_E_NOMEM, "kcoherence: sort scratch allocation failed");
-    PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k0s, idx, perm0, (int)N, 0, nbits, s));
-    PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k1, k1s, idx, perm1, (int)N, 0, nbits, s));
+    PT_HIP_TRY(rocprim::radix_sort_pairs(tmp, tmp_bytes, k0, k0s, idx, perm0, (int)N, 0, nbits, s));
+    PT_HIP_TRY(rocprim::radix_sort_pairs(tmp, tmp_bytes, k1, k1s, idx, perm1, (int)N, 0, nbits, s));
     hipLaunchKernelGGL(k_positions, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, perm0, N, pos0);
     hipLaunchKernelGGL(k_positions, dim3(pt_blocks(N, TB)), dim3(TB), 0, s, perm1, N, pos1);
     if (ndims == 6)

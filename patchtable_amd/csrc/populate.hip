@@ -20,7 +20,7 @@
 #include "pt_device.h"
 #include "pt_kernels.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace {
 
@@ -196,10 +196,10 @@ int pt_shuffle_ranks(const uint32_t *jump_dev, const uint32_t *state_dev, int n,
     int bits = 1;
     while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)n) bits++;
     size_t tmp_bytes = 0;
-    PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, j, js, m, ms, n, 0, bits, s));
+    PT_HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, j, js, m, ms, n, 0, bits, s));
     void *tmp = sc.get(7, tmp_bytes);
     if (!tmp) return pt_fail(PT_E_NOMEM, "populate: sort scratch allocation failed");
-    PT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, j, js, m, ms, n, 0, bits, s));
+    PT_HIP_TRY(rocprim::radix_sort_pairs(tmp, tmp_bytes, j, js, m, ms, n, 0, bits, s));
     PT_HIP_TRY(hipMemsetAsync(F, 0xff, (size_t)n * 4, s));
     hipLaunchKernelGGL(k_next_hits, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, js, ms, n, Gk, F);
     hipLaunchKernelGGL(k_chain_end, dim3(pt_blocks(n, TB)), dim3(TB), 0, s, F, n, s0);

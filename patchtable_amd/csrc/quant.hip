@@ -7,7 +7,7 @@
 #include "pt_device.h"
 #include "pt_kernels.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace {
 
@@ -385,10 +385,10 @@ int pt_quantile_values(const float *desc, size_t plane, int Wp, int h, int w, in
     hipLaunchKernelGGL(k_extract, dim3(pt_blocks(n, 256)), dim3(256), 0, s, desc + (size_t)dim * plane, Wp, nx, ny, step,
                        w - 7, allowed, aidx, vals);
     size_t tmp_bytes = 0;
-    PT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, vals, sorted, (int)n, 0, 32, s));
+    PT_HIP_TRY(rocprim::radix_sort_keys(nullptr, tmp_bytes, vals, sorted, (int)n, 0, 32, s));
     void *tmp = sc.get(12, tmp_bytes);
     if (!tmp) return pt_fail(PT_E_NOMEM, "quantiser: sort scratch allocation failed");
-    PT_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, vals, sorted, (int)n, 0, 32, s));
+    PT_HIP_TRY(rocprim::radix_sort_keys(tmp, tmp_bytes, vals, sorted, (int)n, 0, 32, s));
     hipLaunchKernelGGL(k_gather_ranks, dim3(pt_blocks(nslices + 1, 256)), dim3(256), 0, s, sorted, npart, nslices,
                        values_dev);
     PT_LAUNCH_CHECK();
