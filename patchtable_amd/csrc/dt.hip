@@ -14,7 +14,18 @@
 // sweep along each dimension. dt_iters > 0 (the reference stops after dt_iters+1 levels) keeps
 // cells farther than dt_iters+1 empty, to be filled by fill_zeros (patchtable.h:2063-2079).
 //
-// Keys are 64-bit: distance in the high word, source cell index in the low word; empty = ~0.
+// Keys order the pair (distance, source index) as one unsigned integer, empty = all ones:
+//   32-bit keys (the default): distance << 27 | source, for tables of at most 2^27 cells. A key
+//     holds distances up to K32_DMAX = 30; a step that would go beyond drops the key (it becomes
+//     empty). Distances only grow along a path, so the result is the exact fill for every cell
+//     whose distance is at most 30. A cell left empty at the end (distance above 30, or a table
+//     with no filled cell) is not written: the last pass raises a device flag, and the 64-bit
+//     passes below, launched behind that flag (they return at once when it is clear), first
+//     restore the cells the last pass wrote (a cell is a source exactly when its key's distance
+//     is 0) and then compute the whole fill again.
+//   64-bit keys: distance << 32 | source, with no distance limit (the fallback, and the path for
+//     lattices whose shape the 32-bit passes do not cover).
+// Keys of both widths live in one buffer of 8-byte words; the flag is the int after it.
 #include "pt_device.h"
 #include "pt_kernels.h"
 
@@ -23,150 +34,211 @@
 
 namespace {
 
-constexpr uint64_t KEY_INF = ~0ull;
-constexpr uint64_t KEY_STEP = 1ull << 32;
+template <typename K> struct DtKey;
+template <> struct DtKey<uint64_t> {
+    static constexpr uint64_t INF = ~0ull;
+    static __device__ __forceinline__ uint64_t step(uint64_t r) { return r == INF ? r : r + (1ull << 32); }
+    static __device__ __forceinline__ int64_t src(uint64_t k) { return (int64_t)(k & 0xffffffffull); }
+    static __device__ __forceinline__ int64_t dist(uint64_t k) { return (int64_t)(k >> 32); }
+};
+constexpr int K32_SHIFT = 27;
+constexpr uint32_t K32_DMAX = 30;
+template <> struct DtKey<uint32_t> {
+    static constexpr uint32_t INF = ~0u;
+    // INF >> 27 is 31, so INF stays INF
+    static __device__ __forceinline__ uint32_t step(uint32_t r) {
+        return (r >> K32_SHIFT) >= K32_DMAX ? INF : r + (1u << K32_SHIFT);
+    }
+    static __device__ __forceinline__ int64_t src(uint32_t k) { return (int64_t)(k & ((1u << K32_SHIFT) - 1)); }
+    static __device__ __forceinline__ int64_t dist(uint32_t k) { return (int64_t)(k >> K32_SHIFT); }
+};
 
-__global__ void k_dt_init(const int32_t *__restrict__ table, int64_t n, uint64_t *__restrict__ key) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    key[c] = table[c] != PT_TABLE_UNUSED ? (uint64_t)c : KEY_INF;
+// run_if: null, or a device flag; the kernel does nothing while the flag is 0.
+__device__ __forceinline__ bool dt_skip(const int *run_if) { return run_if && *(volatile const int *)run_if == 0; }
+
+__global__ void k_dt_init(const int32_t *__restrict__ table, int64_t n, uint64_t *__restrict__ key, const int *run_if) {
+    if (dt_skip(run_if)) return;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+        key[c] = table[c] != PT_TABLE_UNUSED ? (uint64_t)c : DtKey<uint64_t>::INF;
 }
 
-// One line per lane along a dimension of size len and stride st (in cells).
-__global__ void k_dt_line(uint64_t *__restrict__ key, int64_t nlines, int len, int64_t st) {
-    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlines) return;
-    int64_t o = l / st, i = l - o * st;
-    uint64_t *p = key + o * (int64_t)len * st + i;
-    uint64_t run = KEY_INF;
-    for (int t = 0; t < len; t++) {
-        uint64_t k = p[(int64_t)t * st];
-        if (run != KEY_INF) run += KEY_STEP;
-        if (k < run) run = k;
-        p[(int64_t)t * st] = run;
-    }
-    run = KEY_INF;
-    for (int t = len - 1; t >= 0; t--) {
-        uint64_t k = p[(int64_t)t * st];
-        if (run != KEY_INF) run += KEY_STEP;
-        if (k < run) run = k;
-        p[(int64_t)t * st] = run;
+// One line per lane along a dimension of size len and stride st (in cells), lines of any length.
+__global__ void k_dt_line(uint64_t *__restrict__ key, int64_t nlines, int len, int64_t st, const int *run_if) {
+    if (dt_skip(run_if)) return;
+    typedef DtKey<uint64_t> D;
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < nlines; l += (int64_t)gridDim.x * blockDim.x) {
+        int64_t o = l / st, i = l - o * st;
+        uint64_t *p = key + o * (int64_t)len * st + i;
+        uint64_t run = D::INF;
+        for (int t = 0; t < len; t++) {
+            uint64_t k = p[(int64_t)t * st];
+            run = D::step(run);
+            if (k < run) run = k;
+            p[(int64_t)t * st] = run;
+        }
+        run = D::INF;
+        for (int t = len - 1; t >= 0; t--) {
+            uint64_t k = p[(int64_t)t * st];
+            run = D::step(run);
+            if (k < run) run = k;
+            p[(int64_t)t * st] = run;
+        }
     }
 }
 
-// The same two sweeps with the line held in registers (len <= LMAX): one read and one write
-// per cell instead of two of each.
-// With FINAL (the last dimension's pass) the lane writes the table values of k_dt_final for its
-// line instead of the keys.
-template <int LMAX, bool FINAL>
-__global__ __launch_bounds__(64) void k_dt_line_reg(uint64_t *__restrict__ key, int64_t nlines, int len, int64_t st,
-                                                    int32_t *__restrict__ table, int64_t max_dist, int32_t fill) {
-    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlines) return;
-    int64_t o = l / st, i = l - o * st;
-    uint64_t *p = key + o * (int64_t)len * st + i;
-    uint64_t v[LMAX];
-#pragma unroll
-    for (int t = 0; t < LMAX; t++)
-        if (t < len) v[t] = p[(int64_t)t * st];
-    uint64_t run = KEY_INF;
+// Table value of cell c from its final key; returns false when the cell is not written: a
+// source (it keeps its own value) or, with 32-bit keys, a cell the keys left empty (the flag is
+// raised unless a distance cap at or below K32_DMAX makes it a `fill` cell anyway).
+template <typename K>
+__device__ __forceinline__ bool dt_value(K k, int64_t c, const int32_t *table, int64_t max_dist, int32_t fill,
+                                         int *flag, int32_t &val) {
+    typedef DtKey<K> D;
+    if (k == D::INF) {
+        if constexpr (sizeof(K) == 4) {
+            if (!(max_dist >= 0 && max_dist <= (int64_t)K32_DMAX)) {
+                *flag = 1;
+                return false;
+            }
+        }
+        val = fill;
+        return true;
+    }
+    if (max_dist >= 0 && D::dist(k) > max_dist) {
+        val = fill;
+        return true;
+    }
+    const int64_t src = D::src(k);
+    if (src == c) return false;
+    val = table[src];
+    return true;
+}
+
+// Both sweeps over a line of len <= LMAX keys held in registers.
+template <typename K, int LMAX>
+__device__ __forceinline__ void sweep_regs(K (&v)[LMAX], int len) {
+    typedef DtKey<K> D;
+    K run = D::INF;
 #pragma unroll
     for (int t = 0; t < LMAX; t++)
         if (t < len) {
-            if (run != KEY_INF) run += KEY_STEP;
+            run = D::step(run);
             if (v[t] < run) run = v[t];
             v[t] = run;
         }
-    run = KEY_INF;
+    run = D::INF;
 #pragma unroll
     for (int t = LMAX - 1; t >= 0; t--)
         if (t < len) {
-            if (run != KEY_INF) run += KEY_STEP;
+            run = D::step(run);
             if (v[t] < run) run = v[t];
             v[t] = run;
         }
-    if (FINAL) {
-        // as k_dt_final: source cells keep their own value, so reading table[src] is safe while
-        // other lanes rewrite other cells
+}
+
+// The two sweeps with the line held in registers (len <= LMAX): one read and one write per cell.
+// With FINAL (the last dimension's pass) the lane writes its line's table values instead of the
+// keys: source cells are never written, so reading table[src] is safe while other lanes write.
+template <typename K, int LMAX, bool FINAL>
+__global__ __launch_bounds__(64) void k_dt_line_reg(K *__restrict__ key, int64_t nlines, int len, int64_t st,
+                                                    int32_t *__restrict__ table, int64_t max_dist, int32_t fill,
+                                                    int *flag, const int *run_if) {
+    if (dt_skip(run_if)) return;
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < nlines; l += (int64_t)gridDim.x * blockDim.x) {
+        int64_t o = l / st, i = l - o * st;
         const int64_t c0 = o * (int64_t)len * st + i;
+        K *p = key + c0;
+        K v[LMAX];
 #pragma unroll
         for (int t = 0; t < LMAX; t++)
-            if (t < len) {
-                const uint64_t k = v[t];
-                const int64_t c = c0 + (int64_t)t * st;
-                int32_t val;
-                if (k == KEY_INF || (max_dist >= 0 && (int64_t)(k >> 32) > max_dist)) val = fill;
-                else val = table[(int64_t)(k & 0xffffffffull)];
-                if (val != table[c]) table[c] = val;
-            }
-        return;
-    }
+            if (t < len) v[t] = p[(int64_t)t * st];
+        sweep_regs<K, LMAX>(v, len);
+        if (FINAL) {
 #pragma unroll
-    for (int t = 0; t < LMAX; t++)
-        if (t < len) p[(int64_t)t * st] = v[t];
+            for (int t = 0; t < LMAX; t++)
+                if (t < len) {
+                    const int64_t c = c0 + (int64_t)t * st;
+                    int32_t val;
+                    if (dt_value<K>(v[t], c, table, max_dist, fill, flag, val)) table[c] = val;
+                }
+        } else {
+#pragma unroll
+            for (int t = 0; t < LMAX; t++)
+                if (t < len) p[(int64_t)t * st] = v[t];
+        }
+    }
 }
 
 // The innermost m dimensions at once. For each index of the outer dimensions their cells form a
-// contiguous slab; a block stages `spb` consecutive slabs (<= PT_DT_SLAB cells) in LDS with
-// coalesced loads, runs both sweeps along each slab dimension in turn, innermost first (the
-// order of the separate passes), and writes the keys back once. With `table` (the first pass)
-// the keys are made from the table as k_dt_init makes them.
+// contiguous slab; a block stages `spb` consecutive slabs (P cells each) in LDS with coalesced
+// loads, runs both sweeps along each slab dimension in turn, innermost first (the order of the
+// separate passes), and writes the keys back once. The keys are made from the table (cell c:
+// c if filled, else empty), as k_dt_init makes them.
 struct SlabDims {
     int m;
     int len[PT_MAX_DIMS];   // slab dimensions, innermost first
     int st[PT_MAX_DIMS];    // their strides in cells
 };
-constexpr int PT_DT_SLAB = 6144;   // cells of the staged slabs: 48 KB of keys
+constexpr int PT_DT_SLAB = 6144;   // cells of the staged slabs (24 or 48 KB of keys)
 
-__global__ __launch_bounds__(256) void k_dt_slab(uint64_t *__restrict__ key, int64_t ncells, int P, int spb, SlabDims S,
-                                                 const int32_t *__restrict__ table) {
-    extern __shared__ uint64_t sh[];
-    const int64_t c0 = (int64_t)blockIdx.x * spb * P;
-    const int total = (int)(ncells - c0 < (int64_t)spb * P ? ncells - c0 : (int64_t)spb * P);
-    uint64_t *g = key + c0;
-    if (table) {
+template <typename K>
+__global__ __launch_bounds__(256) void k_dt_slab(K *__restrict__ key, int64_t ncells, int P, int spb, SlabDims S,
+                                                 const int32_t *__restrict__ table, const int *run_if) {
+    if (dt_skip(run_if)) return;
+    typedef DtKey<K> D;
+    extern __shared__ __align__(8) unsigned char dt_lds[];
+    K *sh = (K *)dt_lds;
+    const int64_t per = (int64_t)spb * P;
+    const int64_t nblk = (ncells + per - 1) / per;
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int64_t c0 = blk * per;
+        const int total = (int)(ncells - c0 < per ? ncells - c0 : per);
         for (int i = threadIdx.x; i < total; i += blockDim.x)
-            sh[i] = table[c0 + i] != PT_TABLE_UNUSED ? (uint64_t)(c0 + i) : KEY_INF;
-    } else {
-        for (int i = threadIdx.x; i < total; i += blockDim.x) sh[i] = g[i];
-    }
-    __syncthreads();
-    for (int k = 0; k < S.m; k++) {
-        const int len = S.len[k], st = S.st[k];
-        const int nl = total / len;   // lines of dimension k in the staged slabs
-        for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-            uint64_t *p = sh + (l / st) * (len * st) + (l % st);
-            uint64_t run = KEY_INF;
-            for (int t = 0; t < len; t++) {
-                if (run != KEY_INF) run += KEY_STEP;
-                const uint64_t v = p[t * st];
-                if (v < run) run = v;
-                p[t * st] = run;
+            sh[i] = table[c0 + i] != PT_TABLE_UNUSED ? (K)(c0 + i) : D::INF;
+        __syncthreads();
+        for (int k = 0; k < S.m; k++) {
+            const int len = S.len[k], st = S.st[k];
+            const int nl = total / len;   // lines of dimension k in the staged slabs
+            for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+                K *p = sh + (l / st) * (len * st) + (l % st);
+                K run = D::INF;
+                for (int t = 0; t < len; t++) {
+                    run = D::step(run);
+                    const K v = p[t * st];
+                    if (v < run) run = v;
+                    p[t * st] = run;
+                }
+                run = D::INF;
+                for (int t = len - 1; t >= 0; t--) {
+                    run = D::step(run);
+                    const K v = p[t * st];
+                    if (v < run) run = v;
+                    p[t * st] = run;
+                }
             }
-            run = KEY_INF;
-            for (int t = len - 1; t >= 0; t--) {
-                if (run != KEY_INF) run += KEY_STEP;
-                const uint64_t v = p[t * st];
-                if (v < run) run = v;
-                p[t * st] = run;
-            }
+            __syncthreads();
         }
+        for (int i = threadIdx.x; i < total; i += blockDim.x) key[c0 + i] = sh[i];
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < total; i += blockDim.x) g[i] = sh[i];
 }
 
-// table[c] = value of the winning source, or `fill` for cells beyond max_dist / unreached.
-// Source cells keep their own value, so reading table[src] while writing other cells is safe.
+// Writes the table from 64-bit keys (when the last pass is not a register pass).
 __global__ void k_dt_final(const uint64_t *__restrict__ key, int64_t n, int32_t *__restrict__ table, int64_t max_dist,
-                           int32_t fill) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    uint64_t k = key[c];
-    int32_t v;
-    if (k == KEY_INF || (max_dist >= 0 && (int64_t)(k >> 32) > max_dist)) v = fill;
-    else v = table[(int64_t)(k & 0xffffffffull)];
-    if (v != table[c]) table[c] = v;
+                           int32_t fill, const int *run_if) {
+    if (dt_skip(run_if)) return;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        int32_t val;
+        if (dt_value<uint64_t>(key[c], c, table, max_dist, fill, nullptr, val)) table[c] = val;
+    }
+}
+
+// Before the 64-bit fallback: every cell the 32-bit last pass may have written (the cells that are
+// not sources: their key's distance is not 0) is empty again.
+__global__ void k_dt_restore(const uint32_t *__restrict__ key, int64_t n, int32_t *__restrict__ table,
+                             const int *run_if) {
+    if (dt_skip(run_if)) return;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+        if ((key[c] >> K32_SHIFT) != 0) table[c] = PT_TABLE_UNUSED;
 }
 
 // fill_zeros (patchtable.h:2063-2079): every empty cell takes `fill`.
@@ -370,60 +442,103 @@ int pt_dt_euclidean_launch(int32_t *table, int G, const int *nslices, int64_t nc
     return PT_OK;
 }
 
-int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
-                      uint64_t *key, hipStream_t s) {
-    const int TB = 256;
-    const int64_t max_dist = dt_iters > 0 ? (int64_t)dt_iters + 1 : -1;
-    // passes run d = G-1 .. 0. The innermost dimensions whose cells fit PT_DT_SLAB go in one
-    // slab pass, which also makes the keys from the table; the last (strided, in registers)
-    // pass writes the table when dimension 0 is outside the slab and short.
+namespace {
+
+// The innermost dimensions whose cells fit `cap` cells, innermost first.
+SlabDims slab_dims(int G, const int *nslices, int64_t cap, int64_t &P) {
     SlabDims S{};
-    int64_t P = 1;
-    while (S.m < G && P * nslices[G - 1 - S.m] <= PT_DT_SLAB) {
+    P = 1;
+    while (S.m < G && P * nslices[G - 1 - S.m] <= cap) {
         S.len[S.m] = nslices[G - 1 - S.m];
         S.st[S.m] = (int)P;
         P *= nslices[G - 1 - S.m];
         S.m++;
     }
-    int64_t st0 = 1;   // stride of dimension 0
-    for (int d = 1; d < G; d++) st0 *= nslices[d];
+    return S;
+}
+
+// One sweep pass per dimension outside the slab, the last one (dimension 0) writing the table.
+// grid_cap > 0 bounds the grids (the fallback's launches, which usually return at once).
+template <typename K>
+void dt_passes(int32_t *table, int G, const int *nslices, int64_t ncells, int64_t max_dist, int32_t fill, K *key,
+               const SlabDims &S, int64_t P, int *flag, const int *run_if, unsigned grid_cap, hipStream_t s) {
+    auto cap = [&](unsigned g) { return grid_cap && g > grid_cap ? grid_cap : g; };
+    const int64_t st0 = ncells / nslices[0];
     const bool final_fused = st0 > 1 && nslices[0] <= 64 && S.m < G;
     int64_t st = 1;
     int dstart = G - 1;
     if (S.m > 0) {
         const int spb = (int)(PT_DT_SLAB / P);
         const int64_t nblk = (ncells + (int64_t)spb * P - 1) / ((int64_t)spb * P);
-        hipLaunchKernelGGL(k_dt_slab, dim3((unsigned)nblk), dim3(256), (size_t)spb * P * 8, s, key, ncells, (int)P, spb,
-                           S, (const int32_t *)table);
+        hipLaunchKernelGGL(k_dt_slab<K>, dim3(cap((unsigned)nblk)), dim3(256), (size_t)spb * P * sizeof(K), s, key,
+                           ncells, (int)P, spb, S, (const int32_t *)table, run_if);
         st = P;
         dstart = G - 1 - S.m;
     } else {
-        hipLaunchKernelGGL(k_dt_init, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, table, ncells, key);
+        // only 64-bit keys get here
+        hipLaunchKernelGGL(k_dt_init, dim3(cap(pt_blocks(ncells, 256))), dim3(256), 0, s, table, ncells,
+                           (uint64_t *)key, run_if);
     }
     for (int d = dstart; d >= 0; d--) {
         const int64_t nlines = ncells / nslices[d];
         const int len = nslices[d];
-        const dim3 g64(pt_blocks(nlines, 64));
+        const dim3 g64(cap(pt_blocks(nlines, 64)));
         if (d == 0 && final_fused) {
             if (len <= 32)
-                hipLaunchKernelGGL((k_dt_line_reg<32, true>), g64, dim3(64), 0, s, key, nlines, len, st, table,
-                                   max_dist, fill);
+                hipLaunchKernelGGL((k_dt_line_reg<K, 32, true>), g64, dim3(64), 0, s, key, nlines, len, st, table,
+                                   max_dist, fill, flag, run_if);
             else
-                hipLaunchKernelGGL((k_dt_line_reg<64, true>), g64, dim3(64), 0, s, key, nlines, len, st, table,
-                                   max_dist, fill);
+                hipLaunchKernelGGL((k_dt_line_reg<K, 64, true>), g64, dim3(64), 0, s, key, nlines, len, st, table,
+                                   max_dist, fill, flag, run_if);
         } else if (len <= 32) {
-            hipLaunchKernelGGL((k_dt_line_reg<32, false>), g64, dim3(64), 0, s, key, nlines, len, st, table, max_dist,
-                               fill);
+            hipLaunchKernelGGL((k_dt_line_reg<K, 32, false>), g64, dim3(64), 0, s, key, nlines, len, st, table,
+                               max_dist, fill, flag, run_if);
         } else if (len <= 64) {
-            hipLaunchKernelGGL((k_dt_line_reg<64, false>), g64, dim3(64), 0, s, key, nlines, len, st, table, max_dist,
-                               fill);
+            hipLaunchKernelGGL((k_dt_line_reg<K, 64, false>), g64, dim3(64), 0, s, key, nlines, len, st, table,
+                               max_dist, fill, flag, run_if);
         } else {
-            hipLaunchKernelGGL(k_dt_line, g64, dim3(64), 0, s, key, nlines, len, st);
+            // only 64-bit keys get here
+            hipLaunchKernelGGL(k_dt_line, g64, dim3(64), 0, s, (uint64_t *)key, nlines, len, st, run_if);
         }
         st *= len;
     }
-    if (!final_fused)
-        hipLaunchKernelGGL(k_dt_final, dim3(pt_blocks(ncells, TB)), dim3(TB), 0, s, key, ncells, table, max_dist, fill);
+    if (!final_fused)   // only 64-bit keys get here
+        hipLaunchKernelGGL(k_dt_final, dim3(cap(pt_blocks(ncells, 256))), dim3(256), 0, s, (const uint64_t *)key,
+                           ncells, table, max_dist, fill, run_if);
+}
+
+// The 32-bit passes cover: at most 2^27 cells, a slab of at least one and fewer than all
+// dimensions, and the other dimensions at most 64 long (register passes).
+bool dt_narrow(int G, const int *nslices, int64_t ncells) {
+    if (ncells > ((int64_t)1 << K32_SHIFT) || G < 2) return false;
+    int64_t P;
+    const SlabDims S = slab_dims(G, nslices, PT_DT_SLAB, P);
+    if (S.m == 0 || S.m == G) return false;
+    for (int d = 0; d < G - S.m; d++)
+        if (nslices[d] > 64) return false;
+    return true;
+}
+
+}  // namespace
+
+int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
+                      uint64_t *key, hipStream_t s) {
+    const int64_t max_dist = dt_iters > 0 ? (int64_t)dt_iters + 1 : -1;
+    int *flag = (int *)(key + ncells);
+    int64_t P;
+    const SlabDims S = slab_dims(G, nslices, PT_DT_SLAB, P);
+    if (!dt_narrow(G, nslices, ncells)) {
+        dt_passes<uint64_t>(table, G, nslices, ncells, max_dist, fill, key, S, P, nullptr, nullptr, 0, s);
+        PT_LAUNCH_CHECK();
+        return PT_OK;
+    }
+    if (int rc = pt_fill_launch(flag, 1, 0, s)) return rc;
+    dt_passes<uint32_t>(table, G, nslices, ncells, max_dist, fill, (uint32_t *)key, S, P, flag, nullptr, 0, s);
+    // the fallback: returns at once unless the 32-bit passes left a cell empty
+    const unsigned FB_GRID = 512;
+    hipLaunchKernelGGL(k_dt_restore, dim3(FB_GRID), dim3(256), 0, s, (const uint32_t *)key, ncells, table,
+                       (const int *)flag);
+    dt_passes<uint64_t>(table, G, nslices, ncells, max_dist, fill, key, S, P, nullptr, flag, FB_GRID, s);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }

@@ -149,6 +149,7 @@ int pt_populate_launch(const float *desc, size_t plane, int Wp, const int32_t *l
 int pt_fill_launch(int32_t *p, int64_t n, int32_t v, hipStream_t s);
 
 // dt.hip
+// key: ncells + 1 eight-byte words of scratch (the keys, then a flag).
 int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
                       uint64_t *key, hipStream_t s);
 int pt_fill_unused_launch(int32_t *table, int64_t ncells, int32_t fill, hipStream_t s);

@@ -489,7 +489,7 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
         TRY(t->cells.ensure(t->n_pop));
         TRY(t->win.ensure(t->n_pop));
         TRY(t->counter.ensure(1));
-        if (p.run_dt && p.dt_algo == PT_DT_ALGO_PROP) TRY(t->dtkey.ensure((size_t)t->ncells));
+        if (p.run_dt && p.dt_algo == PT_DT_ALGO_PROP) TRY(t->dtkey.ensure((size_t)t->ncells + 1));
         const size_t back = (size_t)G * ld * 4 + (size_t)G * 4;
         TRY(t->pin.ensure(std::max(back, pin_upload_bytes(G) + 16)));
         HIPTRY(hipMemcpyAsync(t->pin.p, vals, back, hipMemcpyDeviceToHost, s));
@@ -533,7 +533,7 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
     // distance transform (patchtable.h:3254-3295) and fill_zeros (:2063-2079)
     const int32_t vfill = PT_XY_TO_INT(7, 7);
     if (p.run_dt && p.dt_algo == PT_DT_ALGO_PROP) {
-        TRY(t->dtkey.ensure((size_t)t->ncells));
+        TRY(t->dtkey.ensure((size_t)t->ncells + 1));
         TRY(pt_dt_prop_launch(t->table.p, G, t->nslices.data(), t->ncells, p.dt_iters, vfill, t->dtkey.p, s));
     } else if (p.run_dt && p.dt_algo == PT_DT_ALGO_DOWNSAMPLE) {
         TRY(pt_dt_downsample_launch(t->table.p, G, t->nslices.data(), t->sc, s));

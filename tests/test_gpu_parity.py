@@ -112,6 +112,22 @@ def test_table_build_bit_exact(pair_cfg1, extra):
     assert np.array_equal(gd.view(np.uint32), od.view(np.uint32))
 
 
+@pytest.mark.parametrize("extra", ["-nslices 40,40,40,2,2,1", "-nslices 30,30,30,30,1,1",
+                                   "-nslices 40,40,40,2,2,1 -dt_iters 10", "-nslices 40,40,40,2,2,1 -dt_iters 40"])
+def test_table_build_sparse_far_cells(extra):
+    """A 24x24 database (289 patches) in a lattice of about 1e8 cells: cells up to 54-58 steps
+    (L1, in cells) from the nearest filled cell, beyond what a 32-bit DT key holds (30), so the
+    fill takes the 64-bit passes after the 32-bit ones; with a distance cap of 11 the 32-bit
+    passes finish alone, with a cap of 41 they do not."""
+    from patchtable_amd import synth_pair
+    print(f"table build sparse {extra}", flush=True)
+    _, b = synth_pair(5, 24, 24)
+    t, o = _build_both(speed1(extra), b)
+    assert t.info()["nslices"] == o.nslices
+    assert np.array_equal(t.populated(), o.populated())
+    assert np.array_equal(t.cells(), o.table())
+
+
 def test_table_build_speed1_full_size_cfg1(pair_cfg1):
     """The default speed-1 table (limit 1e8 cells) at 400x300, all stages bit-exact."""
     a, b = pair_cfg1
