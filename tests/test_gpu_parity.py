@@ -37,7 +37,7 @@ def oracle_gck(img, n, nchroma, convert=True):
 
 
 @pytest.mark.parametrize("h,w,n,nchroma", [(300, 400, 4, 1), (37, 53, 6, 1), (64, 40, 8, 0), (20, 30, 2, 2),
-                                           (9, 8, 10, 1), (90, 70, 4, 3)])
+                                           (9, 8, 10, 1), (90, 70, 4, 3), (1030, 2100, 4, 1), (130, 257, 10, 1)])
 def test_gck_bit_exact(lib, h, w, n, nchroma):
     print(f"gck {h}x{w} n={n} nchroma={nchroma}", flush=True)
     img = seeded((h, w, 3), 100 + h)
