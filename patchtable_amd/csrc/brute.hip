@@ -9,15 +9,16 @@
 //   k_patch_vectors   patch vectors in BF16 (round to nearest even) or FP8 e4m3 with a per-patch
 //                     power-of-two scale (an E8M0 block scale of the instruction), and |v|^2 of the
 //                     rounded vector in FP32.
-//   k_brute_mfma      one block = 256 queries (4 waves x 4 row tiles of 16); the database streams
-//                     through two LDS buffers in chunks of 64 patches (the next chunk's global
-//                     loads overlap the current chunk's MFMAs); a.b on v_mfma_f32_16x16x32_bf16, or
-//                     |b|^2 - 2 a.b directly on v_mfma_scale_f32_16x16x128_f8f6f4 (two K steps of
-//                     128, the second half zero; accumulators start at |b|^2, the scales are
-//                     applied by the instruction), FP32 accumulate; each
-//                     lane keeps the 2 best scores |b|^2 - 2 a.b of its 16 query rows over the
-//                     database columns it sees; at the end the 32 lane candidates of a query are
-//                     merged to its 8 best.
+//   k_brute_mfma      BF16: one block = 256 queries (4 waves x 4 row tiles of 16); the database
+//                     streams through two LDS buffers in chunks of 64 patches (the next chunk's
+//                     global loads overlap the current chunk's MFMAs); a.b on
+//                     v_mfma_f32_16x16x32_bf16, FP32 accumulate; each lane keeps the 2 best scores
+//                     |b|^2 - 2 a.b of its 16 query rows over the database columns it sees; at the
+//                     end the 32 lane candidates of a query are merged to its 8 best.
+//   k_brute_fp8       FP8: one block = 256 queries (8 waves x 32), chunks of 128 patches;
+//                     |b|^2 - 2 a.b directly on v_mfma_scale_f32_32x32x64_f8f6f4 (K = 192 in three
+//                     steps of 64; accumulators start at |b|^2, the scales are applied by the
+//                     instruction); a query's 64 lane candidates are merged to its 8 best.
 //   k_rescore         exact FP32 SSD (the reference's lane order, patch_dist_exact,
 //                     patchtable.h:330-373) of the 8 survivors; the smallest is the match, so no
 //                     low-precision error reaches the reported distance.
@@ -36,6 +37,7 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int DIM = 192;
 constexpr int QPB = 256;        // queries per block
@@ -132,65 +134,59 @@ __device__ __forceinline__ void keep_best(float s, int idx, float (&bs)[KL], int
     }
 }
 
-// FP8 = false: BF16 vectors (qvec/dvec hold 2-byte elements); true: e4m3 bytes with scales.
-// cand/cscore: nq x 8 database indices and scores |b|^2 - 2 a.b (ascending; -1 / +inf when the
-// database has fewer patches).
-template <bool FP8>
-__global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__restrict__ qvec, const float *__restrict__ qscale,
-                                                   int nq, const unsigned char *__restrict__ dvec,
-                                                   const float *__restrict__ dnorm, const float *__restrict__ dscale,
-                                                   int nd, int32_t *__restrict__ cand, float *__restrict__ cscore) {
-    constexpr int EB = FP8 ? 1 : 2;                 // bytes per element
-    // LDS row pitch (bytes): FP8 rows carry 64 zero bytes after the 192 elements, so the
-    // K = 128 instruction's second step reads 256 bytes; 16 bytes of padding against bank conflicts
-    constexpr int ROWB = FP8 ? 256 + 16 : DIM * EB + 16;
+// The 8 best of a lane's query: its candidates (mscore/midx row `row`, n entries, -1 = none) merged
+// into (ts, ti), ascending, ties to the smaller index.
+__device__ __forceinline__ void merge_best8(const float *mrow, const int *irow, int n, float (&ts)[8], int (&ti)[8]) {
+    for (int e = 0; e < n; e++) {
+        float s = mrow[e];
+        int id = irow[e];
+        if (id < 0) continue;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const bool better = s < ts[i] || (s == ts[i] && (ti[i] < 0 || id < ti[i]));
+            if (better) {
+                const float fs = ts[i];
+                const int fi = ti[i];
+                ts[i] = s;
+                ti[i] = id;
+                s = fs;
+                id = fi;
+                if (id < 0) break;
+            }
+        }
+    }
+}
+
+// BF16 search. cand/cscore: nq x 8 database indices and scores |b|^2 - 2 a.b (ascending; -1 / +inf
+// when the database has fewer patches).
+__global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__restrict__ qvec, int nq,
+                                                       const unsigned char *__restrict__ dvec,
+                                                       const float *__restrict__ dnorm, int nd,
+                                                       int32_t *__restrict__ cand, float *__restrict__ cscore) {
+    constexpr int EB = 2;                           // bytes per element
+    constexpr int ROWB = DIM * EB + 16;             // LDS row pitch: 16 bytes against bank conflicts
     // two database chunks while streaming (the next one is written while the current one is
     // read); at the end the candidate lists of two waves at a time
     constexpr int MERGE = 2 * 64 * 16 * KL;
     constexpr int SMEM = 2 * DBC * ROWB > MERGE * 8 ? 2 * DBC * ROWB : MERGE * 8;
     __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
     __shared__ float snorm[2][DBC];
-    __shared__ float sscale[2][DBC];
     float (*mscore)[64][16 * KL] = (float (*)[64][16 * KL])smem;
     int (*midx)[64][16 * KL] = (int (*)[64][16 * KL])(smem + MERGE * 4);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 15, grp = lane >> 4;
     const int qw0 = blockIdx.x * QPB + wave * 64;   // first query of this wave
     // A fragments of query qw0 + 16t + col (clamped to the last query; such rows are never
-    // written). BF16: K block kb, elements k = kb*32 + 8*grp .. + 7 (v_mfma_f32_16x16x32_bf16).
-    // FP8: K step ks of 128, bytes k = 128*ks + 32*grp .. + 31 (v_mfma_f32_16x16x128_f8f6f4; the
-    // B fragments use the same k for the same lane group and byte, so the products pair up);
-    // k >= 192 is zero.
-    constexpr int NKB = FP8 ? 1 : DIM / 32;
+    // written): K block kb, elements k = kb*32 + 8*grp .. + 7 (v_mfma_f32_16x16x32_bf16)
+    constexpr int NKB = DIM / 32;
     v4i afrag[4][NKB];
-    v8i afr8[4][2];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const int q = min(qw0 + 16 * t + col, nq - 1);
-        if (FP8) {
 #pragma unroll
-            for (int ks = 0; ks < 2; ks++) {
-                const int k0 = 128 * ks + 32 * grp;
-                if (k0 < DIM) {
-                    const v4i lo = *(const v4i *)(qvec + (size_t)q * DIM + k0);
-                    const v4i hi = *(const v4i *)(qvec + (size_t)q * DIM + k0 + 16);
-                    afr8[t][ks] = v8i{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-                } else {
-                    afr8[t][ks] = v8i{0, 0, 0, 0, 0, 0, 0, 0};
-                }
-            }
-        } else {
-#pragma unroll
-            for (int kb = 0; kb < NKB; kb++)
-                afrag[t][kb] = *(const v4i *)(qvec + (size_t)q * DIM * EB + (size_t)(kb * 32 + 8 * grp) * EB);
-        }
+        for (int kb = 0; kb < NKB; kb++)
+            afrag[t][kb] = *(const v4i *)(qvec + (size_t)q * DIM * EB + (size_t)(kb * 32 + 8 * grp) * EB);
     }
-    // FP8: E8M0 scale (the biased exponent of the power-of-two scale) of the query row this
-    // lane's A fragments hold, row 16t + col of the wave's tile t
-    int qe[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-        qe[t] = FP8 ? (int)((__float_as_uint(qscale[min(qw0 + 16 * t + col, nq - 1)]) >> 23) & 0xffu) : 0;
     float bs[16][KL];
     int bi[16][KL];
 #pragma unroll
@@ -204,7 +200,7 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
     constexpr int PIECES = DBC * DIM * EB / 16, PER = PIECES / 256, RP = DIM * EB / 16;
     static_assert(PIECES % 256 == 0, "chunk pieces");
     v4i stage[PER];
-    float stn = 0.f, sts = 1.f;
+    float stn = 0.f;
     auto fetch = [&](int d0) {
 #pragma unroll
         for (int k = 0; k < PER; k++) {
@@ -214,7 +210,6 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
         if (threadIdx.x < DBC) {
             const int d = d0 + threadIdx.x;
             stn = d < nd ? dnorm[d] : __builtin_inff();
-            sts = FP8 && d < nd ? dscale[d] : 1.f;
         }
     };
     auto put = [&](int buf) {
@@ -223,17 +218,8 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
             const int i = threadIdx.x + 256 * k, row = i / RP, c16 = i % RP;
             *(v4i *)(smem + (size_t)buf * DBC * ROWB + row * ROWB + c16 * 16) = stage[k];
         }
-        if (threadIdx.x < DBC) {
-            snorm[buf][threadIdx.x] = stn;
-            sscale[buf][threadIdx.x] = sts;
-        }
+        if (threadIdx.x < DBC) snorm[buf][threadIdx.x] = stn;
     };
-    if (FP8) {   // the zero bytes 192..255 of every row of both buffers (staging never writes them)
-        for (int i = threadIdx.x; i < 2 * DBC * 4; i += 256) {
-            const int row = i >> 2, piece = i & 3;
-            *(v4i *)(smem + (size_t)row * ROWB + DIM + piece * 16) = v4i{0, 0, 0, 0};
-        }
-    }
     fetch(0);
     put(0);
     __syncthreads();
@@ -249,37 +235,18 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
             const float dn = snorm[cur][dcol];
             v4f acc[4];
             const unsigned char *brow = sdb + (c * 16 + col) * ROWB;
-            if (FP8) {
-                // score = |b|^2 + sum (a / sa)(-b / sb) sa (2 sb) = |b|^2 - 2 a.b, straight from
-                // the matrix cores: the accumulators start at |b|^2 and the E8M0 block scales
-                // (one per row of A, one per column of B) are applied by the instruction
-                const int be = (int)((__float_as_uint(sscale[cur][dcol]) >> 23) & 0xffu);
 #pragma unroll
-                for (int t = 0; t < 4; t++) acc[t] = v4f{dn, dn, dn, dn};
+            for (int t = 0; t < 4; t++) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int ks = 0; ks < 2; ks++) {
-                    const v4i lo = *(const v4i *)(brow + 128 * ks + 32 * grp);
-                    const v4i hi = *(const v4i *)(brow + 128 * ks + 32 * grp + 16);
-                    const v8i b8 = v8i{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            for (int kb = 0; kb < NKB; kb++) {
+                const v4i bv = *(const v4i *)(brow + (kb * 32 + 8 * grp) * 2);
+                const bf8 bb = __builtin_bit_cast(bf8, bv);
 #pragma unroll
-                    for (int t = 0; t < 4; t++)   // e4m3 operands (cbsz = blgp = 0), scale byte 0 of each
-                        acc[t] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(afr8[t][ks], b8, acc[t], 0, 0, 0,
-                                                                                  qe[t], 0, be);
-                }
-            } else {
-#pragma unroll
-                for (int t = 0; t < 4; t++) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kb = 0; kb < NKB; kb++) {
-                    const v4i bv = *(const v4i *)(brow + (kb * 32 + 8 * grp) * 2);
-                    const bf8 bb = __builtin_bit_cast(bf8, bv);
-#pragma unroll
-                    for (int t = 0; t < 4; t++)
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, afrag[t][kb]), bb,
-                                                                        acc[t], 0, 0, 0);
-                }
+                for (int t = 0; t < 4; t++)
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, afrag[t][kb]), bb, acc[t],
+                                                                    0, 0, 0);
             }
-            auto score = [&](int t, int j) { return FP8 ? acc[t][j] : dn - 2.f * acc[t][j]; };
+            auto score = [&](int t, int j) { return dn - 2.f * acc[t][j]; };
             // a score enters a list only when it beats the list's second entry; once the lists
             // have seen a few thousand columns that is rare, so the wave tests all 16 first and
             // runs the insertions (the same scores, recomputed) only when some lane needs one
@@ -327,27 +294,160 @@ __global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__re
         __syncthreads();
         if ((wave >> 1) == phase && q < nq) {
             const int mw = wave & 1;
-            for (int e = 0; e < 16 * KL; e++) {
-                float s = mscore[mw][lane][e];
-                int id = midx[mw][lane][e];
-                if (id < 0) continue;
-#pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    const bool better = s < ts[i] || (s == ts[i] && (ti[i] < 0 || id < ti[i]));
-                    if (better) {
-                        const float fs = ts[i];
-                        const int fi = ti[i];
-                        ts[i] = s;
-                        ti[i] = id;
-                        s = fs;
-                        id = fi;
-                        if (id < 0) break;
-                    }
-                }
-            }
+            merge_best8(mscore[mw][lane], midx[mw][lane], 16 * KL, ts, ti);
         }
     }
     if (q >= nq) return;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        cand[(size_t)q * 8 + i] = ti[i];
+        cscore[(size_t)q * 8 + i] = ts[i];
+    }
+}
+
+// FP8 search: one block = 256 queries (8 waves x 32); the database streams through two LDS
+// buffers in chunks of 128 patches (4 column tiles of 32). |b|^2 - 2 a.b comes straight from
+// v_mfma_scale_f32_32x32x64_f8f6f4: K = 192 is three steps of 64 with no padding, the accumulators
+// start at |b|^2 and the E8M0 block scales (one per row of A, one per column of B) are applied by
+// the instruction. Lane l holds A row / B column l % 32, bytes 32 (l / 32) .. + 31 of each K step
+// (the same bytes of A and B, so the products pair up); accumulator r holds query row
+// 8 (r / 4) + 4 (l / 32) + r % 4 against database column l % 32. Each lane keeps the KL best scores
+// of its 16 query rows over the columns it sees; a query's 32 lanes x KL candidates are merged to
+// its 8 best.
+constexpr int F8W = 8;            // waves per block
+constexpr int F8DBC = 128;        // database patches per chunk
+__global__ __launch_bounds__(64 * F8W, 1) void k_brute_fp8(const unsigned char *__restrict__ qvec,
+                                                           const float *__restrict__ qscale, int nq,
+                                                           const unsigned char *__restrict__ dvec,
+                                                           const float *__restrict__ dnorm,
+                                                           const float *__restrict__ dscale, int nd,
+                                                           int32_t *__restrict__ cand, float *__restrict__ cscore) {
+    constexpr int NT = 64 * F8W;
+    constexpr int ROWB = DIM + 16;             // LDS row pitch: 16 bytes against bank conflicts
+    constexpr int MW8 = F8W / 2;               // waves whose lists are merged at a time
+    constexpr int MERGE = MW8 * 32 * 32 * KL;  // their lists: 32 rows x 32 lanes x KL each
+    constexpr int SMEM = 2 * F8DBC * ROWB > MERGE * 8 ? 2 * F8DBC * ROWB : MERGE * 8;
+    static_assert(QPB == 32 * F8W, "one row tile of 32 queries per wave");
+    __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
+    __shared__ float snorm[2][F8DBC];
+    __shared__ float sscale[2][F8DBC];
+    float (*mscore)[32][32 * KL] = (float (*)[32][32 * KL])smem;
+    int (*midx)[32][32 * KL] = (int (*)[32][32 * KL])(smem + MERGE * 4);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l32 = lane & 31, kh = lane >> 5;
+    const int qw0 = blockIdx.x * QPB + wave * 32;   // first query of this wave
+    // A fragments of query qw0 + l32 (clamped to the last query; such rows are never written)
+    // and the E8M0 exponent of its scale
+    v8i afr[3];
+    const int qa = min(qw0 + l32, nq - 1);
+#pragma unroll
+    for (int ks = 0; ks < 3; ks++) {
+        const unsigned char *src = qvec + (size_t)qa * DIM + 64 * ks + 32 * kh;
+        const v4i lo = *(const v4i *)src, hi = *(const v4i *)(src + 16);
+        afr[ks] = v8i{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    }
+    const int qe = (int)((__float_as_uint(qscale[qa]) >> 23) & 0xffu);
+    float bs[16][KL];
+    int bi[16][KL];
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+#pragma unroll
+        for (int m = 0; m < KL; m++) {
+            bs[r][m] = __builtin_inff();
+            bi[r][m] = -1;
+        }
+    // staging: thread i moves 16-byte pieces i, i + NT, ... of a chunk (zero rows past the end)
+    constexpr int PIECES = F8DBC * DIM / 16, PER = PIECES / NT, RP = DIM / 16;
+    static_assert(PIECES % NT == 0, "chunk pieces");
+    v4i stage[PER];
+    float stn = 0.f, sts = 1.f;
+    auto fetch = [&](int d0) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = threadIdx.x + NT * k, row = i / RP, c16 = i % RP, d = d0 + row;
+            stage[k] = d < nd ? *(const v4i *)(dvec + (size_t)d * DIM + c16 * 16) : v4i{0, 0, 0, 0};
+        }
+        if (threadIdx.x < F8DBC) {
+            const int d = d0 + threadIdx.x;
+            stn = d < nd ? dnorm[d] : __builtin_inff();
+            sts = d < nd ? dscale[d] : 1.f;
+        }
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = threadIdx.x + NT * k, row = i / RP, c16 = i % RP;
+            *(v4i *)(smem + (size_t)buf * F8DBC * ROWB + row * ROWB + c16 * 16) = stage[k];
+        }
+        if (threadIdx.x < F8DBC) {
+            snorm[buf][threadIdx.x] = stn;
+            sscale[buf][threadIdx.x] = sts;
+        }
+    };
+    fetch(0);
+    put(0);
+    __syncthreads();
+    for (int d0 = 0, cur = 0; d0 < nd; d0 += F8DBC, cur ^= 1) {
+        const bool more = d0 + F8DBC < nd;
+        if (more) fetch(d0 + F8DBC);   // loads in flight during this chunk's MFMAs
+        const unsigned char *sdb = smem + (size_t)cur * F8DBC * ROWB;
+#pragma unroll 1
+        for (int c = 0; c < F8DBC / 32; c++) {
+            const int dcol = c * 32 + l32;
+            const int didx = d0 + dcol;
+            const float dn = snorm[cur][dcol];
+            const int be = (int)((__float_as_uint(sscale[cur][dcol]) >> 23) & 0xffu);
+            v16f acc;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[r] = dn;
+#pragma unroll
+            for (int ks = 0; ks < 3; ks++) {   // e4m3 operands (cbsz = blgp = 0), scale byte 0 of each
+                const unsigned char *src = sdb + dcol * ROWB + 64 * ks + 32 * kh;
+                const v4i lo = *(const v4i *)src, hi = *(const v4i *)(src + 16);
+                const v8i b8 = v8i{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(afr[ks], b8, acc, 0, 0, 0, qe, 0, be);
+            }
+            // as in the BF16 search: test all 16 first, insert only when some lane needs to
+            bool enter = false;
+#pragma unroll
+            for (int r = 0; r < 16; r++) enter = enter || acc[r] < bs[r][KL - 1];
+            if (__any(enter)) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) keep_best(acc[r], didx, bs[r], bi[r]);
+            }
+        }
+        if (more) put(cur ^ 1);   // the other buffer was last read before the previous barrier
+        __syncthreads();
+    }
+    // merge, MW8 waves at a time: the lane lists of query row 8 (r / 4) + 4 kh + r % 4 go to
+    // mscore[wave % MW8][row][l32 * KL + m]; then lane l < 32 takes query qw0 + l
+    const int q = qw0 + l32;
+    float ts[8];
+    int ti[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        ts[i] = __builtin_inff();
+        ti[i] = -1;
+    }
+#pragma unroll 1
+    for (int phase = 0; phase < F8W / MW8; phase++) {
+        __syncthreads();   // the database chunk, or the previous phase's lists, are no longer read
+        const bool mine = wave / MW8 == phase;
+        const int mw = wave % MW8;
+        if (mine) {
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+#pragma unroll
+                for (int m = 0; m < KL; m++) {
+                    const int row = 8 * (r >> 2) + 4 * kh + (r & 3);
+                    mscore[mw][row][l32 * KL + m] = bs[r][m];
+                    midx[mw][row][l32 * KL + m] = bi[r][m];
+                }
+        }
+        __syncthreads();
+        if (mine && kh == 0 && q < nq) merge_best8(mscore[mw][l32], midx[mw][l32], 32 * KL, ts, ti);
+    }
+    if (kh != 0 || q >= nq) return;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         cand[(size_t)q * 8 + i] = ti[i];
@@ -507,11 +607,11 @@ int pt_brute_mfma_launch(const unsigned char *qvec, const float *qscale, int nq,
                          hipStream_t s) {
     if (nq <= 0) return PT_OK;
     if (fp8)
-        hipLaunchKernelGGL(k_brute_mfma<true>, dim3(pt_blocks(nq, QPB)), dim3(256), 0, s, qvec, qscale, nq, dvec, dnorm,
-                           dscale, nd, cand, cscore);
-    else
-        hipLaunchKernelGGL(k_brute_mfma<false>, dim3(pt_blocks(nq, QPB)), dim3(256), 0, s, qvec, qscale, nq, dvec,
+        hipLaunchKernelGGL(k_brute_fp8, dim3(pt_blocks(nq, QPB)), dim3(64 * F8W), 0, s, qvec, qscale, nq, dvec,
                            dnorm, dscale, nd, cand, cscore);
+    else
+        hipLaunchKernelGGL(k_brute_mfma, dim3(pt_blocks(nq, QPB)), dim3(256), 0, s, qvec, nq, dvec, dnorm, nd, cand,
+                           cscore);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }
