@@ -9,16 +9,15 @@
 //   k_patch_vectors   patch vectors in BF16 (round to nearest even) or FP8 e4m3 with a per-patch
 //                     power-of-two scale (an E8M0 block scale of the instruction), and |v|^2 of the
 //                     rounded vector in FP32.
-//   k_brute_mfma      BF16: one block = 256 queries (4 waves x 4 row tiles of 16); the database
-//                     streams through two LDS buffers in chunks of 64 patches (the next chunk's
-//                     global loads overlap the current chunk's MFMAs); a.b on
-//                     v_mfma_f32_16x16x32_bf16, FP32 accumulate; each lane keeps the 2 best scores
-//                     |b|^2 - 2 a.b of its 16 query rows over the database columns it sees; at the
-//                     end the 32 lane candidates of a query are merged to its 8 best.
-//   k_brute_fp8       FP8: one block = 256 queries (8 waves x 32), chunks of 128 patches;
-//                     |b|^2 - 2 a.b directly on v_mfma_scale_f32_32x32x64_f8f6f4 (K = 192 in three
-//                     steps of 64; accumulators start at |b|^2, the scales are applied by the
-//                     instruction); a query's 64 lane candidates are merged to its 8 best.
+//   k_brute_wide      one block = 256 queries (8 waves x 32); the database streams through two
+//                     LDS buffers in chunks of 128 patches (the next chunk's global loads overlap
+//                     the current chunk's MFMAs); 32 x 32 output tiles: BF16 a.b on
+//                     v_mfma_f32_32x32x16_bf16, or FP8 |b|^2 - 2 a.b directly on
+//                     v_mfma_scale_f32_32x32x64_f8f6f4 (K = 192 in three steps of 64; accumulators
+//                     start at |b|^2, the scales are applied by the instruction), FP32 accumulate;
+//                     each lane keeps the 2 best scores |b|^2 - 2 a.b of its 16 query rows over the
+//                     database columns it sees; at the end the 64 lane candidates of a query are
+//                     merged to its 8 best.
 //   k_rescore         exact FP32 SSD (the reference's lane order, patch_dist_exact,
 //                     patchtable.h:330-373) of the 8 survivors; the smallest is the match, so no
 //                     low-precision error reaches the reported distance.
@@ -41,7 +40,6 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int DIM = 192;
 constexpr int QPB = 256;        // queries per block
-constexpr int DBC = 64;         // database patches per LDS chunk
 constexpr int KL = 2;           // candidates a lane keeps per query row
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -157,196 +155,57 @@ __device__ __forceinline__ void merge_best8(const float *mrow, const int *irow, 
     }
 }
 
-// BF16 search. cand/cscore: nq x 8 database indices and scores |b|^2 - 2 a.b (ascending; -1 / +inf
-// when the database has fewer patches).
-__global__ __launch_bounds__(256, 2) void k_brute_mfma(const unsigned char *__restrict__ qvec, int nq,
-                                                       const unsigned char *__restrict__ dvec,
-                                                       const float *__restrict__ dnorm, int nd,
-                                                       int32_t *__restrict__ cand, float *__restrict__ cscore) {
-    constexpr int EB = 2;                           // bytes per element
-    constexpr int ROWB = DIM * EB + 16;             // LDS row pitch: 16 bytes against bank conflicts
-    // two database chunks while streaming (the next one is written while the current one is
-    // read); at the end the candidate lists of two waves at a time
-    constexpr int MERGE = 2 * 64 * 16 * KL;
-    constexpr int SMEM = 2 * DBC * ROWB > MERGE * 8 ? 2 * DBC * ROWB : MERGE * 8;
-    __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
-    __shared__ float snorm[2][DBC];
-    float (*mscore)[64][16 * KL] = (float (*)[64][16 * KL])smem;
-    int (*midx)[64][16 * KL] = (int (*)[64][16 * KL])(smem + MERGE * 4);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int col = lane & 15, grp = lane >> 4;
-    const int qw0 = blockIdx.x * QPB + wave * 64;   // first query of this wave
-    // A fragments of query qw0 + 16t + col (clamped to the last query; such rows are never
-    // written): K block kb, elements k = kb*32 + 8*grp .. + 7 (v_mfma_f32_16x16x32_bf16)
-    constexpr int NKB = DIM / 32;
-    v4i afrag[4][NKB];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const int q = min(qw0 + 16 * t + col, nq - 1);
-#pragma unroll
-        for (int kb = 0; kb < NKB; kb++)
-            afrag[t][kb] = *(const v4i *)(qvec + (size_t)q * DIM * EB + (size_t)(kb * 32 + 8 * grp) * EB);
-    }
-    float bs[16][KL];
-    int bi[16][KL];
-#pragma unroll
-    for (int r = 0; r < 16; r++)
-#pragma unroll
-        for (int m = 0; m < KL; m++) {
-            bs[r][m] = __builtin_inff();
-            bi[r][m] = -1;
-        }
-    // staging: thread i moves 16-byte pieces i, i + 256, ... of a chunk (zero rows past the end)
-    constexpr int PIECES = DBC * DIM * EB / 16, PER = PIECES / 256, RP = DIM * EB / 16;
-    static_assert(PIECES % 256 == 0, "chunk pieces");
-    v4i stage[PER];
-    float stn = 0.f;
-    auto fetch = [&](int d0) {
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int i = threadIdx.x + 256 * k, row = i / RP, c16 = i % RP, d = d0 + row;
-            stage[k] = d < nd ? *(const v4i *)(dvec + (size_t)d * DIM * EB + c16 * 16) : v4i{0, 0, 0, 0};
-        }
-        if (threadIdx.x < DBC) {
-            const int d = d0 + threadIdx.x;
-            stn = d < nd ? dnorm[d] : __builtin_inff();
-        }
-    };
-    auto put = [&](int buf) {
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int i = threadIdx.x + 256 * k, row = i / RP, c16 = i % RP;
-            *(v4i *)(smem + (size_t)buf * DBC * ROWB + row * ROWB + c16 * 16) = stage[k];
-        }
-        if (threadIdx.x < DBC) snorm[buf][threadIdx.x] = stn;
-    };
-    fetch(0);
-    put(0);
-    __syncthreads();
-    for (int d0 = 0, cur = 0; d0 < nd; d0 += DBC, cur ^= 1) {
-        const bool more = d0 + DBC < nd;
-        if (more) fetch(d0 + DBC);   // loads in flight during this chunk's MFMAs
-        const unsigned char *sdb = smem + (size_t)cur * DBC * ROWB;
-#pragma unroll 1
-        for (int c = 0; c < DBC / 16; c++) {
-            // D: lane holds query rows 16t + 4grp + j against database column c*16 + col
-            const int dcol = c * 16 + col;
-            const int didx = d0 + dcol;
-            const float dn = snorm[cur][dcol];
-            v4f acc[4];
-            const unsigned char *brow = sdb + (c * 16 + col) * ROWB;
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kb = 0; kb < NKB; kb++) {
-                const v4i bv = *(const v4i *)(brow + (kb * 32 + 8 * grp) * 2);
-                const bf8 bb = __builtin_bit_cast(bf8, bv);
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, afrag[t][kb]), bb, acc[t],
-                                                                    0, 0, 0);
-            }
-            auto score = [&](int t, int j) { return dn - 2.f * acc[t][j]; };
-            // a score enters a list only when it beats the list's second entry; once the lists
-            // have seen a few thousand columns that is rare, so the wave tests all 16 first and
-            // runs the insertions (the same scores, recomputed) only when some lane needs one
-            bool enter = false;
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) enter = enter || score(t, j) < bs[4 * t + j][KL - 1];
-            if (__any(enter)) {
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) keep_best(score(t, j), didx, bs[4 * t + j], bi[4 * t + j]);
-            }
-        }
-        if (more) put(cur ^ 1);   // the other buffer was last read before the previous barrier
-        __syncthreads();
-    }
-    // merge, waves 0-1 then waves 2-3: lane candidates of query row 16t + 4grp + j go to
-    // mscore[wave & 1][row][col*KL + m]; then lane l takes query qw0 + l and keeps the 8 best of
-    // its 32 candidates (ties: smaller index)
-    const int q = qw0 + lane;
-    float ts[8];
-    int ti[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        ts[i] = __builtin_inff();
-        ti[i] = -1;
-    }
-#pragma unroll 1
-    for (int phase = 0; phase < 2; phase++) {
-        __syncthreads();   // the database chunk, or the previous phase's lists, are no longer read
-        if ((wave >> 1) == phase) {
-            const int mw = wave & 1;
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-#pragma unroll
-                    for (int m = 0; m < KL; m++) {
-                        mscore[mw][16 * t + 4 * grp + j][col * KL + m] = bs[4 * t + j][m];
-                        midx[mw][16 * t + 4 * grp + j][col * KL + m] = bi[4 * t + j][m];
-                    }
-        }
-        __syncthreads();
-        if ((wave >> 1) == phase && q < nq) {
-            const int mw = wave & 1;
-            merge_best8(mscore[mw][lane], midx[mw][lane], 16 * KL, ts, ti);
-        }
-    }
-    if (q >= nq) return;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        cand[(size_t)q * 8 + i] = ti[i];
-        cscore[(size_t)q * 8 + i] = ts[i];
-    }
-}
-
-// FP8 search: one block = 256 queries (8 waves x 32); the database streams through two LDS
-// buffers in chunks of 128 patches (4 column tiles of 32). |b|^2 - 2 a.b comes straight from
-// v_mfma_scale_f32_32x32x64_f8f6f4: K = 192 is three steps of 64 with no padding, the accumulators
-// start at |b|^2 and the E8M0 block scales (one per row of A, one per column of B) are applied by
-// the instruction. Lane l holds A row / B column l % 32, bytes 32 (l / 32) .. + 31 of each K step
-// (the same bytes of A and B, so the products pair up); accumulator r holds query row
-// 8 (r / 4) + 4 (l / 32) + r % 4 against database column l % 32. Each lane keeps the KL best scores
-// of its 16 query rows over the columns it sees; a query's 32 lanes x KL candidates are merged to
-// its 8 best.
-constexpr int F8W = 8;            // waves per block
-constexpr int F8DBC = 128;        // database patches per chunk
-__global__ __launch_bounds__(64 * F8W, 1) void k_brute_fp8(const unsigned char *__restrict__ qvec,
+// One block = 256 queries (8 waves x 32); the database streams through two LDS buffers in chunks
+// of 128 patches (4 column tiles of 32; the next chunk's global loads overlap the current chunk's
+// MFMAs). Lane l holds A row / B column l % 32 and, of each K step, the second half of the step's
+// elements when l >= 32 (the same elements of A and B, so the products pair up); accumulator r
+// holds query row 8 (r / 4) + 4 (l / 32) + r % 4 against database column l % 32.
+//   BF16 (FP8 = false): a.b on v_mfma_f32_32x32x16_bf16, twelve K steps of 16, FP32 accumulate;
+//     score |b|^2 - 2 a.b.
+//   FP8: |b|^2 - 2 a.b straight from v_mfma_scale_f32_32x32x64_f8f6f4: K = 192 is three steps of
+//     64 with no padding, the accumulators start at |b|^2 and the E8M0 block scales (one per row
+//     of A, one per column of B) are applied by the instruction.
+// Each lane keeps the KL best scores of its 16 query rows over the columns it sees; a query's
+// 32 lanes x KL candidates are merged to its 8 best.
+constexpr int WW = 8;            // waves per block
+constexpr int WDBC = 128;        // database patches per chunk
+template <bool FP8>
+__global__ __launch_bounds__(64 * WW, 2) void k_brute_wide(const unsigned char *__restrict__ qvec,
                                                            const float *__restrict__ qscale, int nq,
                                                            const unsigned char *__restrict__ dvec,
                                                            const float *__restrict__ dnorm,
                                                            const float *__restrict__ dscale, int nd,
                                                            int32_t *__restrict__ cand, float *__restrict__ cscore) {
-    constexpr int NT = 64 * F8W;
-    constexpr int ROWB = DIM + 16;             // LDS row pitch: 16 bytes against bank conflicts
-    constexpr int MW8 = F8W / 2;               // waves whose lists are merged at a time
-    constexpr int MERGE = MW8 * 32 * 32 * KL;  // their lists: 32 rows x 32 lanes x KL each
-    constexpr int SMEM = 2 * F8DBC * ROWB > MERGE * 8 ? 2 * F8DBC * ROWB : MERGE * 8;
-    static_assert(QPB == 32 * F8W, "one row tile of 32 queries per wave");
+    constexpr int NT = 64 * WW;
+    constexpr int EB = FP8 ? 1 : 2;            // bytes per element
+    constexpr int NKS = FP8 ? 3 : 12;          // K steps
+    constexpr int SB = DIM * EB / NKS;         // bytes of one K step (64 or 32), half per lane group
+    constexpr int ROWB = DIM * EB + 16;        // LDS row pitch: 16 bytes against bank conflicts
+    constexpr int MWW = WW / 2;                // waves whose lists are merged at a time
+    constexpr int MERGE = MWW * 32 * 32 * KL;  // their lists: 32 rows x 32 lanes x KL each
+    constexpr int SMEM = 2 * WDBC * ROWB > MERGE * 8 ? 2 * WDBC * ROWB : MERGE * 8;
+    static_assert(QPB == 32 * WW, "one row tile of 32 queries per wave");
+    typedef int frag_t __attribute__((ext_vector_type(SB / 8)));   // a lane's half step
     __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
-    __shared__ float snorm[2][F8DBC];
-    __shared__ float sscale[2][F8DBC];
+    __shared__ float snorm[2][WDBC];
+    __shared__ float sscale[2][FP8 ? WDBC : 1];
     float (*mscore)[32][32 * KL] = (float (*)[32][32 * KL])smem;
     int (*midx)[32][32 * KL] = (int (*)[32][32 * KL])(smem + MERGE * 4);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int l32 = lane & 31, kh = lane >> 5;
     const int qw0 = blockIdx.x * QPB + wave * 32;   // first query of this wave
+    auto half_step = [&](const unsigned char *row, int ks) {
+        frag_t f;
+        __builtin_memcpy(&f, row + SB * ks + (SB / 2) * kh, SB / 2);
+        return f;
+    };
     // A fragments of query qw0 + l32 (clamped to the last query; such rows are never written)
-    // and the E8M0 exponent of its scale
-    v8i afr[3];
+    // and, for FP8, the E8M0 exponent of its scale
     const int qa = min(qw0 + l32, nq - 1);
+    frag_t afr[NKS];
 #pragma unroll
-    for (int ks = 0; ks < 3; ks++) {
-        const unsigned char *src = qvec + (size_t)qa * DIM + 64 * ks + 32 * kh;
-        const v4i lo = *(const v4i *)src, hi = *(const v4i *)(src + 16);
-        afr[ks] = v8i{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-    }
-    const int qe = (int)((__float_as_uint(qscale[qa]) >> 23) & 0xffu);
+    for (int ks = 0; ks < NKS; ks++) afr[ks] = half_step(qvec + (size_t)qa * DIM * EB, ks);
+    const int qe = FP8 ? (int)((__float_as_uint(qscale[qa]) >> 23) & 0xffu) : 0;
     float bs[16][KL];
     int bi[16][KL];
 #pragma unroll
@@ -357,7 +216,7 @@ __global__ __launch_bounds__(64 * F8W, 1) void k_brute_fp8(const unsigned char *
             bi[r][m] = -1;
         }
     // staging: thread i moves 16-byte pieces i, i + NT, ... of a chunk (zero rows past the end)
-    constexpr int PIECES = F8DBC * DIM / 16, PER = PIECES / NT, RP = DIM / 16;
+    constexpr int PIECES = WDBC * DIM * EB / 16, PER = PIECES / NT, RP = DIM * EB / 16;
     static_assert(PIECES % NT == 0, "chunk pieces");
     v4i stage[PER];
     float stn = 0.f, sts = 1.f;
@@ -365,49 +224,61 @@ __global__ __launch_bounds__(64 * F8W, 1) void k_brute_fp8(const unsigned char *
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int i = threadIdx.x + NT * k, row = i / RP, c16 = i % RP, d = d0 + row;
-            stage[k] = d < nd ? *(const v4i *)(dvec + (size_t)d * DIM + c16 * 16) : v4i{0, 0, 0, 0};
+            stage[k] = d < nd ? *(const v4i *)(dvec + (size_t)d * DIM * EB + c16 * 16) : v4i{0, 0, 0, 0};
         }
-        if (threadIdx.x < F8DBC) {
+        if (threadIdx.x < WDBC) {
             const int d = d0 + threadIdx.x;
             stn = d < nd ? dnorm[d] : __builtin_inff();
-            sts = d < nd ? dscale[d] : 1.f;
+            if (FP8) sts = d < nd ? dscale[d] : 1.f;
         }
     };
     auto put = [&](int buf) {
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int i = threadIdx.x + NT * k, row = i / RP, c16 = i % RP;
-            *(v4i *)(smem + (size_t)buf * F8DBC * ROWB + row * ROWB + c16 * 16) = stage[k];
+            *(v4i *)(smem + (size_t)buf * WDBC * ROWB + row * ROWB + c16 * 16) = stage[k];
         }
-        if (threadIdx.x < F8DBC) {
+        if (threadIdx.x < WDBC) {
             snorm[buf][threadIdx.x] = stn;
-            sscale[buf][threadIdx.x] = sts;
+            if (FP8) sscale[buf][threadIdx.x] = sts;
         }
     };
     fetch(0);
     put(0);
     __syncthreads();
-    for (int d0 = 0, cur = 0; d0 < nd; d0 += F8DBC, cur ^= 1) {
-        const bool more = d0 + F8DBC < nd;
-        if (more) fetch(d0 + F8DBC);   // loads in flight during this chunk's MFMAs
-        const unsigned char *sdb = smem + (size_t)cur * F8DBC * ROWB;
+    for (int d0 = 0, cur = 0; d0 < nd; d0 += WDBC, cur ^= 1) {
+        const bool more = d0 + WDBC < nd;
+        if (more) fetch(d0 + WDBC);   // loads in flight during this chunk's MFMAs
+        const unsigned char *sdb = smem + (size_t)cur * WDBC * ROWB;
 #pragma unroll 1
-        for (int c = 0; c < F8DBC / 32; c++) {
+        for (int c = 0; c < WDBC / 32; c++) {
             const int dcol = c * 32 + l32;
             const int didx = d0 + dcol;
             const float dn = snorm[cur][dcol];
-            const int be = (int)((__float_as_uint(sscale[cur][dcol]) >> 23) & 0xffu);
+            const unsigned char *brow = sdb + dcol * ROWB;
             v16f acc;
+            if constexpr (FP8) {
+                const int be = (int)((__float_as_uint(sscale[cur][dcol]) >> 23) & 0xffu);
 #pragma unroll
-            for (int r = 0; r < 16; r++) acc[r] = dn;
+                for (int r = 0; r < 16; r++) acc[r] = dn;
 #pragma unroll
-            for (int ks = 0; ks < 3; ks++) {   // e4m3 operands (cbsz = blgp = 0), scale byte 0 of each
-                const unsigned char *src = sdb + dcol * ROWB + 64 * ks + 32 * kh;
-                const v4i lo = *(const v4i *)src, hi = *(const v4i *)(src + 16);
-                const v8i b8 = v8i{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-                acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(afr[ks], b8, acc, 0, 0, 0, qe, 0, be);
+                for (int ks = 0; ks < NKS; ks++)   // e4m3 operands (cbsz = blgp = 0), scale byte 0 of each
+                    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(afr[ks], half_step(brow, ks), acc, 0, 0, 0,
+                                                                          qe, 0, be);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[r] = 0.f;
+#pragma unroll
+                for (int ks = 0; ks < NKS; ks++)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, afr[ks]),
+                                                                 __builtin_bit_cast(bf8, half_step(brow, ks)), acc, 0, 0,
+                                                                 0);
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[r] = dn - 2.f * acc[r];
             }
-            // as in the BF16 search: test all 16 first, insert only when some lane needs to
+            // a score enters a list only when it beats the list's second entry; once the lists
+            // have seen a few thousand columns that is rare, so the wave tests all 16 first and
+            // runs the insertions only when some lane needs one
             bool enter = false;
 #pragma unroll
             for (int r = 0; r < 16; r++) enter = enter || acc[r] < bs[r][KL - 1];
@@ -419,8 +290,8 @@ __global__ __launch_bounds__(64 * F8W, 1) void k_brute_fp8(const unsigned char *
         if (more) put(cur ^ 1);   // the other buffer was last read before the previous barrier
         __syncthreads();
     }
-    // merge, MW8 waves at a time: the lane lists of query row 8 (r / 4) + 4 kh + r % 4 go to
-    // mscore[wave % MW8][row][l32 * KL + m]; then lane l < 32 takes query qw0 + l
+    // merge, MWW waves at a time: the lane lists of query row 8 (r / 4) + 4 kh + r % 4 go to
+    // mscore[wave % MWW][row][l32 * KL + m]; then lane l < 32 takes query qw0 + l
     const int q = qw0 + l32;
     float ts[8];
     int ti[8];
@@ -430,10 +301,10 @@ __global__ __launch_bounds__(64 * F8W, 1) void k_brute_fp8(const unsigned char *
         ti[i] = -1;
     }
 #pragma unroll 1
-    for (int phase = 0; phase < F8W / MW8; phase++) {
+    for (int phase = 0; phase < WW / MWW; phase++) {
         __syncthreads();   // the database chunk, or the previous phase's lists, are no longer read
-        const bool mine = wave / MW8 == phase;
-        const int mw = wave % MW8;
+        const bool mine = wave / MWW == phase;
+        const int mw = wave % MWW;
         if (mine) {
 #pragma unroll
             for (int r = 0; r < 16; r++)
@@ -607,11 +478,11 @@ int pt_brute_mfma_launch(const unsigned char *qvec, const float *qscale, int nq,
                          hipStream_t s) {
     if (nq <= 0) return PT_OK;
     if (fp8)
-        hipLaunchKernelGGL(k_brute_fp8, dim3(pt_blocks(nq, QPB)), dim3(64 * F8W), 0, s, qvec, qscale, nq, dvec,
+        hipLaunchKernelGGL(k_brute_wide<true>, dim3(pt_blocks(nq, QPB)), dim3(64 * WW), 0, s, qvec, qscale, nq, dvec,
                            dnorm, dscale, nd, cand, cscore);
     else
-        hipLaunchKernelGGL(k_brute_mfma, dim3(pt_blocks(nq, QPB)), dim3(256), 0, s, qvec, nq, dvec, dnorm, nd, cand,
-                           cscore);
+        hipLaunchKernelGGL(k_brute_wide<false>, dim3(pt_blocks(nq, QPB)), dim3(64 * WW), 0, s, qvec, qscale, nq, dvec,
+                           dnorm, dscale, nd, cand, cscore);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }
