@@ -19,10 +19,11 @@
 //     holds distances up to K32_DMAX = 30; a step that would go beyond drops the key (it becomes
 //     empty). Distances only grow along a path, so the result is the exact fill for every cell
 //     whose distance is at most 30. A cell left empty at the end (distance above 30, or a table
-//     with no filled cell) is not written: the last pass raises a device flag, and the 64-bit
-//     passes below, launched behind that flag (they return at once when it is clear), first
-//     restore the cells the last pass wrote (a cell is a source exactly when its key's distance
-//     is 0) and then compute the whole fill again.
+//     with no filled cell) is not written: the last pass raises a flag, and the 64-bit passes then
+//     first restore the cells the last pass wrote (a cell is a source exactly when its key's
+//     distance is 0) and compute the whole fill again. The build reads the flag at its final
+//     synchronisation and runs them only when it is set; without a host copy of the flag they are
+//     launched behind it on the device and return at once when it is clear.
 //   64-bit keys: distance << 32 | source, with no distance limit (the fallback, and the path for
 //     lattices whose shape the 32-bit passes do not cover).
 // Keys of both widths live in one buffer of 8-byte words; the flag is the int after it.
@@ -522,23 +523,42 @@ bool dt_narrow(int G, const int *nslices, int64_t ncells) {
 }  // namespace
 
 int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
-                      uint64_t *key, hipStream_t s) {
+                      uint64_t *key, hipStream_t s, int *flag_host) {
     const int64_t max_dist = dt_iters > 0 ? (int64_t)dt_iters + 1 : -1;
     int *flag = (int *)(key + ncells);
     int64_t P;
     const SlabDims S = slab_dims(G, nslices, PT_DT_SLAB, P);
     if (!dt_narrow(G, nslices, ncells)) {
         dt_passes<uint64_t>(table, G, nslices, ncells, max_dist, fill, key, S, P, nullptr, nullptr, 0, s);
+        if (flag_host) *flag_host = 0;
         PT_LAUNCH_CHECK();
         return PT_OK;
     }
     if (int rc = pt_fill_launch(flag, 1, 0, s)) return rc;
     dt_passes<uint32_t>(table, G, nslices, ncells, max_dist, fill, (uint32_t *)key, S, P, flag, nullptr, 0, s);
-    // the fallback: returns at once unless the 32-bit passes left a cell empty
-    const unsigned FB_GRID = 512;
-    hipLaunchKernelGGL(k_dt_restore, dim3(FB_GRID), dim3(256), 0, s, (const uint32_t *)key, ncells, table,
-                       (const int *)flag);
-    dt_passes<uint64_t>(table, G, nslices, ncells, max_dist, fill, key, S, P, nullptr, flag, FB_GRID, s);
+    if (flag_host) {
+        // the caller reads the flag once the stream has synchronised and, when it is set, runs
+        // pt_dt_prop_fallback_launch
+        PT_HIP_TRY(hipMemcpyAsync(flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    } else {
+        // the fallback on the device: its launches return at once unless the flag is set
+        const unsigned FB_GRID = 512;
+        hipLaunchKernelGGL(k_dt_restore, dim3(FB_GRID), dim3(256), 0, s, (const uint32_t *)key, ncells, table,
+                           (const int *)flag);
+        dt_passes<uint64_t>(table, G, nslices, ncells, max_dist, fill, key, S, P, nullptr, flag, FB_GRID, s);
+    }
+    PT_LAUNCH_CHECK();
+    return PT_OK;
+}
+
+int pt_dt_prop_fallback_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
+                               uint64_t *key, hipStream_t s) {
+    const int64_t max_dist = dt_iters > 0 ? (int64_t)dt_iters + 1 : -1;
+    int64_t P;
+    const SlabDims S = slab_dims(G, nslices, PT_DT_SLAB, P);
+    hipLaunchKernelGGL(k_dt_restore, dim3(pt_blocks(ncells, 256)), dim3(256), 0, s, (const uint32_t *)key, ncells, table,
+                       nullptr);
+    dt_passes<uint64_t>(table, G, nslices, ncells, max_dist, fill, key, S, P, nullptr, nullptr, 0, s);
     PT_LAUNCH_CHECK();
     return PT_OK;
 }

@@ -149,9 +149,13 @@ int pt_populate_launch(const float *desc, size_t plane, int Wp, const int32_t *l
 int pt_fill_launch(int32_t *p, int64_t n, int32_t v, hipStream_t s);
 
 // dt.hip
-// key: ncells + 1 eight-byte words of scratch (the keys, then a flag).
+// key: ncells + 1 eight-byte words of scratch (the keys, then a flag). flag_host (pinned, or
+// null): receives the flag on the stream; when it reads nonzero after the stream has
+// synchronised, pt_dt_prop_fallback_launch completes the fill (key must still hold the keys).
 int pt_dt_prop_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
-                      uint64_t *key, hipStream_t s);
+                      uint64_t *key, hipStream_t s, int *flag_host);
+int pt_dt_prop_fallback_launch(int32_t *table, int G, const int *nslices, int64_t ncells, int dt_iters, int32_t fill,
+                               uint64_t *key, hipStream_t s);
 int pt_fill_unused_launch(int32_t *table, int64_t ncells, int32_t fill, hipStream_t s);
 int pt_dt_downsample_launch(int32_t *table, int G, const int *nslices, PtScratch &sc, hipStream_t s);
 int pt_dt_euclidean_launch(int32_t *table, int G, const int *nslices, int64_t ncells, float *dist_scratch,

@@ -60,6 +60,8 @@ struct pt_table {
     int table_add_count = 0;
     PinnedBuf pin;                   // staging of the build's small transfers
     bool pin_count_valid = false;    // the populate count was copied into pin
+    bool dt_check = false;           // ... and the 32-bit DT's flag beside it
+    int dt_iters = -1;
     hipStream_t aux = nullptr;       // build: populate's shuffle, beside the descriptor work
     bool shuffle_queued = false;     // ... enqueued there; ev[11] marks its end
     long rand_used = 0;       // rand() outputs consumed by this table's shuffle and the earlier ones
@@ -328,11 +330,19 @@ int upload_quant(pt_table *t, hipStream_t s) {
     return PT_OK;
 }
 
-// After the build's final synchronisation: the populate count arrives in pinned memory, and the
-// DT work array (8 bytes per cell) is released (hipFree waits for the device, so not earlier).
-void finish_grid(pt_table *t) {
+// After the build's final synchronisation: the populate count arrives in pinned memory; a DT whose
+// 32-bit keys left cells empty is completed by the 64-bit passes; the DT work array (8 bytes per
+// cell) is released (hipFree waits for the device, so not earlier).
+int finish_grid(pt_table *t, hipStream_t s) {
     if (t->pin.p && t->pin_count_valid) t->table_add_count = *t->pin.at<int>(pin_upload_bytes(t->G));
+    if (t->dt_check && t->pin.p && t->pin.at<int>(pin_upload_bytes(t->G))[1] != 0) {
+        TRY(pt_dt_prop_fallback_launch(t->table.p, t->G, t->nslices.data(), t->ncells, t->dt_iters,
+                                       PT_XY_TO_INT(7, 7), t->dtkey.p, s));
+        HIPTRY(hipStreamSynchronize(s));
+    }
+    t->dt_check = false;
     t->dtkey.release();
+    return PT_OK;
 }
 
 // The glibc generator state after `skip` outputs from the state of a fresh process.
@@ -534,7 +544,11 @@ int build_grid(pt_table *t, const float *desc, const std::vector<int32_t> &mask_
     const int32_t vfill = PT_XY_TO_INT(7, 7);
     if (p.run_dt && p.dt_algo == PT_DT_ALGO_PROP) {
         TRY(t->dtkey.ensure((size_t)t->ncells + 1));
-        TRY(pt_dt_prop_launch(t->table.p, G, t->nslices.data(), t->ncells, p.dt_iters, vfill, t->dtkey.p, s));
+        int *flag_host = t->pin.at<int>(pin_upload_bytes(G)) + 1;   // beside the populate count
+        TRY(pt_dt_prop_launch(t->table.p, G, t->nslices.data(), t->ncells, p.dt_iters, vfill, t->dtkey.p, s,
+                              flag_host));
+        t->dt_check = true;
+        t->dt_iters = p.dt_iters;
     } else if (p.run_dt && p.dt_algo == PT_DT_ALGO_DOWNSAMPLE) {
         TRY(pt_dt_downsample_launch(t->table.p, G, t->nslices.data(), t->sc, s));
     } else if (p.run_dt && p.dt_algo == PT_DT_ALGO_EUCLIDEAN) {
@@ -625,7 +639,7 @@ int build_table(pt_table *t, const float *b_dev, int h_in, int w_in, int c, cons
         TRY(build_grid(E, t->desc.p, mask_host, mask_dev, used, s, nullptr));
         used = E->rand_used;
         HIPTRY(hipStreamSynchronize(s));
-        finish_grid(E);
+        TRY(finish_grid(E, s));
         E->list_xy.release();
         E->rank.release();
         E->cells.release();
@@ -635,7 +649,7 @@ int build_table(pt_table *t, const float *b_dev, int h_in, int w_in, int c, cons
     }
     HIPTRY(hipEventRecord(t->ev[5], s));
     HIPTRY(hipStreamSynchronize(s));
-    finish_grid(t);
+    TRY(finish_grid(t, s));
     float ms = 0;
     for (int k = 0; k < 5; k++) {
         HIPTRY(hipEventElapsedTime(&ms, t->ev[k], t->ev[k + 1]));
@@ -1205,8 +1219,8 @@ int build_keyed(pt_table *t, const pt_key *key, const float *b_dev, int h, int w
     TRY(build_grid(E, E->desc.p, no_mask, nullptr, t->rand_used, s, nullptr));
     HIPTRY(hipEventRecord(t->ev[5], s));
     HIPTRY(hipStreamSynchronize(s));
-    finish_grid(t);
-    finish_grid(E);
+    TRY(finish_grid(t, s));
+    TRY(finish_grid(E, s));
     for (DevBuf<float> *b : {&E->desc}) b->release();
     E->list_xy.release();
     E->rank.release();
