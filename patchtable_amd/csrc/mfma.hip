@@ -251,82 +251,88 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
             sv[r] = x0 < aw && sx[r] >= 0 && sy[r] >= 0 && sx[r] < aw && sy[r] < ah;
             sxy[r] = sv[r] ? cur.xy[(long)sy[r] * aw + sx[r]] : 0;   // all four loads in flight
         }
+        // a stable round in which no source of the wave's four groups moved: the wave has no items
+        bool idle = false;
         if (stable) {
             bool moved = false;
 #pragma unroll
             for (int r = 0; r < 4; r++) moved = moved || (sv[r] && (cur.fl[(long)sy[r] * aw + sx[r]] & F_CHG));
-            if (((__ballot(moved) >> (16 * gq)) & 0xffffull) == 0)   // no source of this group moved
+            const uint64_t mv = __ballot(moved);
+            if (((mv >> (16 * gq)) & 0xffffull) == 0)   // no source of this group moved
 #pragma unroll
                 for (int r = 0; r < 4; r++) sv[r] = false;
+            idle = mv == 0;
         }
-        for (int i = lane; i < 4 * HS; i += 64) {
-            hk[i] = HT_EMPTY;
-            hc[i] = 0;
-            hf[i] = 64;
-        }
-        wave_lds_sync();
-        // count and first slot of each source's offset: one hash table per group in LDS (open
-        // addressing, HS > 64 slots, so probing ends); counts and first slots do not depend on
-        // the order in which the atomics land
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            ox[r] = PT_INT_TO_X(sxy[r]) - sx[r];
-            oy[r] = PT_INT_TO_Y(sxy[r]) - sy[r];
-            const int po = sv[r] ? (int)(((uint32_t)ox[r] & 0xffffu) | ((uint32_t)oy[r] << 16)) : 0;
-            pos[64 * gq + 16 * r + c] = po;
-            int hh = gq * HS + (int)__umulhi((uint32_t)po * 2654435761u, (uint32_t)HS);
-            if (sv[r]) {
-                for (;;) {
-                    const int prev = atomicCAS(hk + hh, HT_EMPTY, po);
-                    if (prev == HT_EMPTY || prev == po) break;
-                    hh = hh + 1 == (gq + 1) * HS ? gq * HS : hh + 1;
-                }
-                atomicAdd(hc + hh, 1);
-                atomicMin(hf + hh, 16 * r + c);
+        if (!idle) {
+            for (int i = lane; i < 4 * HS; i += 64) {
+                hk[i] = HT_EMPTY;
+                hc[i] = 0;
+                hf[i] = 64;
             }
-            h[r] = hh;
-        }
-        wave_lds_sync();
-        // anchor key of a source that is its offset's first occurrence: (count, -slot) + 1
-        uint32_t kb[4];
+            wave_lds_sync();
+            // count and first slot of each source's offset: one hash table per group in LDS (open
+            // addressing, HS > 64 slots, so probing ends); counts and first slots do not depend on
+            // the order in which the atomics land
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int slot = 16 * r + c;
-            kb[r] = sv[r] && hf[h[r]] == slot ? (uint32_t)((hc[h[r]] << 6) | (63 - slot)) + 1u : 0u;
-        }
-        bool cov[4] = {false, false, false, false};
-        int na = 0;   // anchors of group gq so far (the same in all lanes of the row)
-        for (int a = 0; a < M.nanchors; a++) {
-            // per group, the uncovered offset of largest count, ties to the first slot (keys are unique)
-            uint32_t k = 0u;
-#pragma unroll
-            for (int r = 0; r < 4; r++) k = max(k, cov[r] ? 0u : kb[r]);
-            k = row16_max(k);
-            if (!__any(k != 0u)) break;
-            if (k != 0u) {
-                const int pa = pos[64 * gq + 63 - (int)((k - 1u) & 63u)];
-                const int ax = (int)(int16_t)(pa & 0xffff), ay = pa >> 16;
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    cov[r] = cov[r] || ((uint32_t)(ox[r] - ax + 8) <= 16u && (uint32_t)(oy[r] - ay + R) <= (uint32_t)(2 * R));
-                if (c == 0) {
-                    itmp[(gq * PT_MFMA_MAX_ANCHORS + a) * 2] = ax;
-                    itmp[(gq * PT_MFMA_MAX_ANCHORS + a) * 2 + 1] = ay;
+            for (int r = 0; r < 4; r++) {
+                ox[r] = PT_INT_TO_X(sxy[r]) - sx[r];
+                oy[r] = PT_INT_TO_Y(sxy[r]) - sy[r];
+                const int po = sv[r] ? (int)(((uint32_t)ox[r] & 0xffffu) | ((uint32_t)oy[r] << 16)) : 0;
+                pos[64 * gq + 16 * r + c] = po;
+                int hh = gq * HS + (int)__umulhi((uint32_t)po * 2654435761u, (uint32_t)HS);
+                if (sv[r]) {
+                    for (;;) {
+                        const int prev = atomicCAS(hk + hh, HT_EMPTY, po);
+                        if (prev == HT_EMPTY || prev == po) break;
+                        hh = hh + 1 == (gq + 1) * HS ? gq * HS : hh + 1;
+                    }
+                    atomicAdd(hc + hh, 1);
+                    atomicMin(hf + hh, 16 * r + c);
                 }
-                na++;
+                h[r] = hh;
             }
+            wave_lds_sync();
+            // anchor key of a source that is its offset's first occurrence: (count, -slot) + 1
+            uint32_t kb[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int slot = 16 * r + c;
+                kb[r] = sv[r] && hf[h[r]] == slot ? (uint32_t)((hc[h[r]] << 6) | (63 - slot)) + 1u : 0u;
+            }
+            bool cov[4] = {false, false, false, false};
+            int na = 0;   // anchors of group gq so far (the same in all lanes of the row)
+            for (int a = 0; a < M.nanchors; a++) {
+                // per group, the uncovered offset of largest count, ties to the first slot (keys are unique)
+                uint32_t k = 0u;
+#pragma unroll
+                for (int r = 0; r < 4; r++) k = max(k, cov[r] ? 0u : kb[r]);
+                k = row16_max(k);
+                if (!__any(k != 0u)) break;
+                if (k != 0u) {
+                    const int pa = pos[64 * gq + 63 - (int)((k - 1u) & 63u)];
+                    const int ax = (int)(int16_t)(pa & 0xffff), ay = pa >> 16;
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        cov[r] = cov[r] || ((uint32_t)(ox[r] - ax + 8) <= 16u && (uint32_t)(oy[r] - ay + R) <= (uint32_t)(2 * R));
+                    if (c == 0) {
+                        itmp[(gq * PT_MFMA_MAX_ANCHORS + a) * 2] = ax;
+                        itmp[(gq * PT_MFMA_MAX_ANCHORS + a) * 2 + 1] = ay;
+                    }
+                    na++;
+                }
+            }
+            wave_lds_sync();
+            // the item list: group 0's anchors in order, then group 1's, ...
+            const int n0 = __builtin_amdgcn_readlane(na, 0), n1 = __builtin_amdgcn_readlane(na, 16),
+                      n2 = __builtin_amdgcn_readlane(na, 32), n3 = __builtin_amdgcn_readlane(na, 48);
+            const int pre = gq == 0 ? 0 : (gq == 1 ? n0 : (gq == 2 ? n0 + n1 : n0 + n1 + n2));
+            if (c < na) {
+                items[wave][pre + c][0] = gq;
+                items[wave][pre + c][1] = itmp[(gq * PT_MFMA_MAX_ANCHORS + c) * 2];
+                items[wave][pre + c][2] = itmp[(gq * PT_MFMA_MAX_ANCHORS + c) * 2 + 1];
+            }
+            nitems = n0 + n1 + n2 + n3;
         }
-        wave_lds_sync();
-        // the item list: group 0's anchors in order, then group 1's, ...
-        const int n0 = __builtin_amdgcn_readlane(na, 0), n1 = __builtin_amdgcn_readlane(na, 16),
-                  n2 = __builtin_amdgcn_readlane(na, 32), n3 = __builtin_amdgcn_readlane(na, 48);
-        const int pre = gq == 0 ? 0 : (gq == 1 ? n0 : (gq == 2 ? n0 + n1 : n0 + n1 + n2));
-        if (c < na) {
-            items[wave][pre + c][0] = gq;
-            items[wave][pre + c][1] = itmp[(gq * PT_MFMA_MAX_ANCHORS + c) * 2];
-            items[wave][pre + c][2] = itmp[(gq * PT_MFMA_MAX_ANCHORS + c) * 2 + 1];
-        }
-        nitems = n0 + n1 + n2 + n3;
     }
     if (M.items_out && lane == 0) M.items_out[(long)blockIdx.y * gridDim.x + blockIdx.x] = nitems;
     wave_lds_sync();
