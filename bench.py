@@ -424,10 +424,14 @@ def cpu_baseline(H, W, seed):
     return out
 
 
-# MFMA-form settings timed beside the plain form (rounds x anchors per 16-patch group): the
-# headline is the fastest whose mean NNF SSD is at or below the plain form's
+# MFMA-form settings timed beside the plain form (rounds x anchors per 16-patch group, and an
+# explicit jump per round where given): the headline is the fastest whose mean NNF SSD is at or
+# below the plain form's
 MFMA_GRID = ["-mfma_rounds 6 -mfma_anchors 5", "-mfma_rounds 5", "-mfma_rounds 5 -mfma_anchors 7",
-             "-mfma_rounds 6", "-mfma_rounds 4 -mfma_anchors 8", "-mfma_rounds 6 -mfma_anchors 4"]
+             "-mfma_rounds 6", "-mfma_rounds 4 -mfma_anchors 8", "-mfma_rounds 6 -mfma_anchors 4",
+             "-mfma_rounds 4 -mfma_anchors 7 -mfma_jumps 24,6,1,1",
+             "-mfma_rounds 4 -mfma_anchors 8 -mfma_jumps 16,4,1,1",
+             "-mfma_rounds 5 -mfma_anchors 6 -mfma_jumps 20,8,3,1,1"]
 
 
 def params(extra=""):

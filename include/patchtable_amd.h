@@ -67,6 +67,7 @@ extern "C" {
 #define PT_DIM_ALGO_PCA        1
 
 #define PT_MAX_DIMS 64
+#define PT_MFMA_MAX_JUMPS 16   /* entries of pt_params.mfma_jumps */
 
 /* Plain-old-data mirror of PatchTableParams (patchtable_params.h:23-161). Field names and
  * meanings follow the reference; switch names accepted by pt_params_set_switches are the
@@ -177,12 +178,18 @@ typedef struct pt_params {
      *              mfma_anchors shared offsets per 16 query patches, each query's best rescored
      *              with the exact SSD (image mode, query_step 1, no allowed mask).
      * mfma_anchors: most anchor offsets searched per group of 16 query patches and round.
+     * mfma_jumps: the jump of each MFMA round, in patches (the distance of the neighbours whose
+     *              offsets a group takes as sources); a 0 ends the list and rounds past it use 1.
+     *              All 0 (the default): 16, 8, 4, 2, 1, then 1 (fewer than 5 rounds start at
+     *              2^(rounds-1)). A round whose jump equals the previous round's skips the groups
+     *              none of whose sources changed in that round.
      * Set by pt_params_default; accepted as switches "-kcoh_window", "-jf_rounds",
-     * "-mfma_rounds" and "-mfma_anchors". */
+     * "-mfma_rounds", "-mfma_anchors" and "-mfma_jumps" (comma-separated, e.g. 16,4,1,1). */
     int    kcoh_window;
     int    jf_rounds;
     int    mfma_rounds;
     int    mfma_anchors;
+    int    mfma_jumps[PT_MFMA_MAX_JUMPS];
 } pt_params;
 
 typedef struct pt_table pt_table;

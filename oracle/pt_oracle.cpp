@@ -1354,7 +1354,8 @@ static long g_ssd_count[64];
 //   candidates: the cell of every table as lookup_parallel visits them (a cell equal to the
 //     current match, or to the previous table's cell, is skipped), each taken when its int8 SSD
 //     is strictly below the current score;
-//   rounds r = 0..R-1 with jump J = max(1, 2^min(4, R-1) >> r), double-buffered, for every
+//   rounds r = 0..R-1 with jump J = mfma_jumps[r] (1 past the list's end) or, with no list,
+//     J = max(1, 2^min(4, R-1) >> r), double-buffered, for every
 //     group of 16 query patches x0 = 16k .. x0+15 of every row y:
 //     sources (slots 0..63): the members (x0+i, y), the patches (x0+i, y-J) and (x0+i, y+J),
 //       (x0+i-J, y) for i < 8 and (x0+i+J, y) for i >= 8, if inside the query; each proposes
@@ -1379,7 +1380,13 @@ static inline int quant8(float v) { return (int)rintf(fminf(fmaxf(v, 0.f), 1.f) 
 // (group, anchor) items of each round of the last lookup_mfma
 static std::vector<long> g_mfma_items;
 
-static int mfma_jump_at(int rounds, int r) {
+// jumps: pt_params.mfma_jumps (0-terminated; rounds past it use 1); empty: 16, 8, 4, 2, 1, then 1
+static int mfma_jump_at(const int *jumps, int rounds, int r) {
+    if (jumps[0] > 0) {
+        for (int k = 0; k < PT_MFMA_MAX_JUMPS && jumps[k] > 0; k++)
+            if (k == r) return jumps[k];
+        return 1;
+    }
     const int big = 1 << (rounds >= 5 ? 4 : (rounds >= 1 ? rounds - 1 : 0));
     const int j = big >> (r < 30 ? r : 30);
     return j > 1 ? j : 1;
@@ -1474,8 +1481,8 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
     g_mfma_items.assign(rounds, 0);
     std::vector<char> chg(N, 1);   // the previous round changed the match
     for (int r = 0; r < rounds; r++) {
-        const int J = mfma_jump_at(rounds, r);
-        const bool stable = r > 0 && mfma_jump_at(rounds, r - 1) == J;
+        const int J = mfma_jump_at(p.mfma_jumps, rounds, r);
+        const bool stable = r > 0 && mfma_jump_at(p.mfma_jumps, rounds, r - 1) == J;
         for (int y = 0; y < ah; y++)
             for (int x0 = 0; x0 < aw; x0 += 16) {
                 int sox[64], soy[64];

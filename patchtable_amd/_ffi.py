@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpatchtable_hip.so")
 
 PT_MAX_DIMS = 64
+PT_MFMA_MAX_JUMPS = 16
 PT_OK = 0
 PT_E_INVALID = -1
 PT_BRUTE_BF16 = 0
@@ -110,6 +111,7 @@ class PtParams(ctypes.Structure):
         ("jf_rounds", ctypes.c_int),
         ("mfma_rounds", ctypes.c_int),
         ("mfma_anchors", ctypes.c_int),
+        ("mfma_jumps", ctypes.c_int * PT_MFMA_MAX_JUMPS),
     ]
 
 

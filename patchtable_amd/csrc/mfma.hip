@@ -523,7 +523,8 @@ int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, uint32_t 
 
 long pt_mfma_round_workgroups(int ann_w, int ann_h) { return (long)((ann_w + TILE_X - 1) / TILE_X) * ann_h; }
 
-int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f1, hipStream_t s, int *which) {
+int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, const int *jumps, NnfField f0, NnfField f1, hipStream_t s,
+                          int *which) {
     if (M.nanchors < 1 || M.nanchors > PT_MFMA_MAX_ANCHORS)
         return pt_failf(PT_E_INVALID, "mfma_anchors must be in 1..%d", PT_MFMA_MAX_ANCHORS);
     dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, M.ann_h);
@@ -531,7 +532,7 @@ int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f
     for (int r = 0; r < rounds; r++) {
         MfmaArgs Mr = M;
         if (M.items_out) Mr.items_out = M.items_out + r * nwg;
-        const int J = pt_mfma_jump_at(rounds, r), stable = r > 0 && pt_mfma_jump_at(rounds, r - 1) == J;
+        const int J = pt_mfma_jump_at(jumps, rounds, r), stable = r > 0 && pt_mfma_jump_at(jumps, rounds, r - 1) == J;
         hipLaunchKernelGGL(k_mfma_round, grid, dim3(MW), 0, s, Mr, f0, f1, J, stable);
         std::swap(f0, f1);
     }
@@ -540,7 +541,8 @@ int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f
     return PT_OK;
 }
 
-int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, int rounds, hipStream_t s,
+int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, int rounds, const int *jumps,
+                                hipStream_t s,
                                 int *which) {
     const MfmaArgs &M = P0.M;
     if (M.nanchors < 1 || M.nanchors > PT_MFMA_MAX_ANCHORS)
@@ -549,7 +551,7 @@ int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, i
     dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, M.ann_h, npairs);
     int src = 0;
     for (int r = 0; r < rounds; r++) {
-        const int J = pt_mfma_jump_at(rounds, r), stable = r > 0 && pt_mfma_jump_at(rounds, r - 1) == J;
+        const int J = pt_mfma_jump_at(jumps, rounds, r), stable = r > 0 && pt_mfma_jump_at(jumps, rounds, r - 1) == J;
         hipLaunchKernelGGL(k_mfma_round_batch, grid, dim3(MW), 0, s, P_dev, src, J, stable);
         src ^= 1;
     }

@@ -238,6 +238,25 @@ int apply_switches(pt_params *p, const vector<string> &argv) {
     if (has("-jf_rounds")) p->jf_rounds = to_int(sw["-jf_rounds"]);
     if (has("-mfma_rounds")) p->mfma_rounds = to_int(sw["-mfma_rounds"]);
     if (has("-mfma_anchors")) p->mfma_anchors = to_int(sw["-mfma_anchors"]);
+    if (has("-mfma_jumps")) {
+        // comma-separated jumps, at most PT_MFMA_MAX_JUMPS, each in 1..1024
+        int v[PT_MFMA_MAX_JUMPS] = {0}, n = 0;
+        const string &js = sw["-mfma_jumps"];
+        size_t i = 0;
+        while (i <= js.size()) {
+            size_t j = js.find(',', i);
+            if (j == string::npos) j = js.size();
+            const string tok = js.substr(i, j - i);
+            char *end = nullptr;
+            const long x = strtol(tok.c_str(), &end, 10);
+            if (tok.empty() || *end != '\0' || x < 1 || x > 1024 || n == PT_MFMA_MAX_JUMPS)
+                return pt_failf(PT_E_INVALID, "-mfma_jumps: expected up to %d comma-separated jumps in 1..1024",
+                                PT_MFMA_MAX_JUMPS);
+            v[n++] = (int)x;
+            i = j + 1;
+        }
+        for (int k = 0; k < PT_MFMA_MAX_JUMPS; k++) p->mfma_jumps[k] = v[k];
+    }
     return PT_OK;
 }
 
@@ -343,5 +362,6 @@ extern "C" int pt_params_default(pt_params *p) {
     p->jf_rounds = 6;
     p->mfma_rounds = 0;
     p->mfma_anchors = 6;
+    for (int k = 0; k < PT_MFMA_MAX_JUMPS; k++) p->mfma_jumps[k] = 0;
     return pt_params_set_speed(p, 3);
 }

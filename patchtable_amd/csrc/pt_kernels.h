@@ -266,8 +266,15 @@ struct MfmaArgs {
 };
 // Workgroups of one MFMA round launch.
 long pt_mfma_round_workgroups(int ann_w, int ann_h);
-// Jump of MFMA round r of `rounds`: 16, 8, 4, 2, 1, then unit jumps (fewer rounds start lower).
-static inline int pt_mfma_jump_at(int rounds, int r) {
+// Jump of MFMA round r of `rounds`: jumps[r] when the list (pt_params.mfma_jumps, 0-terminated)
+// is given, 1 past its end; with an empty list 16, 8, 4, 2, 1, then unit jumps (fewer rounds
+// start lower).
+static inline int pt_mfma_jump_at(const int *jumps, int rounds, int r) {
+    if (jumps && jumps[0] > 0) {
+        for (int k = 0; k < PT_MFMA_MAX_JUMPS && jumps[k] > 0; k++)
+            if (k == r) return jumps[k];
+        return 1;
+    }
     const int big = 1 << (rounds >= 5 ? 4 : (rounds >= 1 ? rounds - 1 : 0));
     const int j = big >> (r < 30 ? r : 30);
     return j > 1 ? j : 1;
@@ -278,7 +285,8 @@ int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t 
 // copy, PT_MFMA_NK_PAD in the norm keys).
 int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, uint32_t *bnorm, hipStream_t s);
 // Runs `rounds` rounds from f0; *which = 1 if the result is in f1.
-int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, NnfField f0, NnfField f1, hipStream_t s, int *which);
+int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, const int *jumps, NnfField f0, NnfField f1, hipStream_t s,
+                          int *which);
 // One pair of a batch (one query against many tables), read by the batched kernels from a device
 // array indexed by the pair: the lookup's arguments (L; G: the same for the query_step > 1 gather,
 // whose lanes are single patches), its two fields, its gathered field and its NNF output.
@@ -301,7 +309,8 @@ int pt_lookup_batch_output(const PairLookup *P_dev, const PairLookup &P0, int np
 // The table candidates of every pair (MFMA batches): field 0 of each pair.
 int pt_lookup_batch_candidates(const PairLookup *P_dev, const PairLookup &P0, int npairs, hipStream_t s);
 // MFMA rounds of every pair of a batch, one launch per round; *which = the field with the result.
-int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, int rounds, hipStream_t s,
+int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, int npairs, int rounds, const int *jumps,
+                                hipStream_t s,
                                 int *which);
 // MFMA form's output (lookup.hip): NNF_X, NNF_Y and NNF_DIST as doubles, NNF_DIST the exact FP32
 // SSD (the reference's lane order) of every match other than the initial one (field flag

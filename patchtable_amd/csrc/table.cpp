@@ -999,7 +999,7 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
             M.items_out = t->mfma_items.p;
         }
         int which = 0;
-        TRY(pt_mfma_rounds_launch(M, p.mfma_rounds, f0, f1, s, &which));
+        TRY(pt_mfma_rounds_launch(M, p.mfma_rounds, p.mfma_jumps, f0, f1, s, &which));
         launches = p.mfma_rounds;
         if (timing) HIPTRY(hipEventRecord(t->ev[10], s));
         // the exact distances and the double output in one pass (k_init_prev's initial distances,
@@ -1908,7 +1908,7 @@ int batch_enqueue(pt_batch *bt, const float *a_dev, int h, int w, int *launches)
         TRY(pt_quant_rgb0_launch(a_dev, (long)h * w, bt->aq.p, s));
         TRY(pt_lookup_batch_candidates(bt->pairs.p, P0, n, s));
         int which = 0;
-        TRY(pt_mfma_rounds_batch_launch(bt->pairs.p, P0, n, p.mfma_rounds, s, &which));
+        TRY(pt_mfma_rounds_batch_launch(bt->pairs.p, P0, n, p.mfma_rounds, p.mfma_jumps, s, &which));
         TRY(pt_lookup_batch_exact_out(bt->pairs.p, P0, n, which, s));
         nl += 4 + p.mfma_rounds;
     } else {

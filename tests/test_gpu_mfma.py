@@ -25,6 +25,8 @@ def _both(p, b):
     (80, 119, 90, 100, "-mfma_rounds 3 -mfma_anchors 16"),
     (70, 90, 120, 130, "-mfma_rounds 5 -ntables 2"),
     (40, 519, 60, 300, "-mfma_rounds 4"),                     # 512 columns: 8 tile columns (XCD bands)
+    (80, 119, 90, 100, "-mfma_rounds 4 -mfma_anchors 8 -mfma_jumps 16,4,1,1"),
+    (96, 150, 110, 160, "-mfma_rounds 6 -mfma_jumps 24,8,3"),  # rounds past the list: unit jumps
 ])
 def test_mfma_lookup_bit_exact(qh, qw, bh, bw, extra):
     from patchtable_amd import synth_pair

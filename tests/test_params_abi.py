@@ -50,6 +50,13 @@ def test_cli_argv_and_switch_errors():
     p.set_from_switches("-x 0 -nslices 10,20,30")
     assert p.ndims == 3 and p.nslices == [10, 20, 30]
     p = P()
+    p.set_from_switches("-x 0 -mfma_rounds 4 -mfma_jumps 16,4,1,1")
+    assert p.mfma_rounds == 4 and list(p.raw.mfma_jumps)[:5] == [16, 4, 1, 1, 0]
+    assert list(P().raw.mfma_jumps) == [0] * 16
+    for bad in ("16,,1", "16,x", "0,1", "2000", ",".join(["1"] * 17)):
+        with pytest.raises(PatchTableError, match="mfma_jumps"):
+            P().set_from_switches("-x 0 -mfma_jumps " + bad)
+    p = P()
     p.set_from_switches("-x 0 -auto_filter 1")
     assert p.filter_dims == [2, 1, 5, 0, 3, 10]
     p = P()
