@@ -86,7 +86,7 @@ def copy_probe(torch, mib=1024, reps=3):
     del src, dst
 
 
-OVM_FORMS = [("plain", ""), ("mfma", "-mfma_rounds 6 -mfma_anchors 5")]
+OVM_MFMA = "-mfma_rounds 4 -mfma_anchors 7 -mfma_jumps 24,6,1,1"   # the MFMA form's switches in the one-vs-many workload
 
 
 def one_vs_many(args):
@@ -108,7 +108,7 @@ def one_vs_many(args):
     dbs = [torch.from_numpy(synth_pair(1000 + i, h, w)[1]).cuda() for i, (h, w) in enumerate(sizes)]
     nq = (1024 - 7) * (1024 - 7) * n
     forms = {}
-    for name, switches in OVM_FORMS:
+    for name, switches in [("plain", ""), ("mfma", args.ovm_mfma)]:
         p = params(switches)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -593,6 +593,7 @@ def main():
                          "key_forward: the key network's forward pass alone; brute: exhaustive MFMA search "
                          "against the FP64 exact nearest neighbours")
     ap.add_argument("--databases", type=int, default=32)
+    ap.add_argument("--ovm-mfma", default=OVM_MFMA, help="the MFMA form's switches in one_vs_many")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration for the traffic figure)")
     ap.add_argument("--mfma-grid", default=";".join(MFMA_GRID),
