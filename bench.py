@@ -176,9 +176,9 @@ def learned_key(args):
     anchor's exact nearest neighbour (the exhaustive search: BF16 scoring on the matrix cores, the
     8 best rescored in exact FP32), 2000 Adam steps of 4096; then the held-out 2048x2048 pair (seed
     3000) looked up in the key table (with its GCK fallback table) and in the plain GCK table, in
-    the plain form and in MFMA forms. Reports each point's query Mpatch/s and mean NNF SSD, the
-    Pareto line, and the fastest key + MFMA point whose SSD is at or below the plain GCK table's
-    at speed 1."""
+    the plain form and in MFMA forms, each table's lattice tile autotuned. Reports each point's
+    query Mpatch/s and mean NNF SSD, the Pareto line, and the fastest key + MFMA point whose SSD
+    is at or below the plain GCK table's at speed 1."""
     import numpy as np
     import torch
     from patchtable_amd import PatchKey, PatchTable, synth_pair
@@ -200,9 +200,10 @@ def learned_key(args):
     nq = (H - 7) * (W - 7)
     ann = torch.empty((H - 7, W - 7, 3), dtype=torch.float64, device="cuda")
     points = []
-    settings = ["", "-mfma_rounds 6 -mfma_anchors 5", "-mfma_rounds 5", "-mfma_rounds 5 -mfma_anchors 4",
-                "-mfma_rounds 4 -mfma_anchors 8", "-mfma_rounds 4 -mfma_anchors 6", "-mfma_rounds 4 -mfma_anchors 4",
-                "-mfma_rounds 3 -mfma_anchors 8"]
+    settings = ["", "-mfma_rounds 6 -mfma_anchors 5", "-mfma_rounds 5 -mfma_anchors 7", "-mfma_rounds 4 -mfma_anchors 8",
+                "-mfma_rounds 4 -mfma_anchors 7 -mfma_jumps 24,6,1,1", "-mfma_rounds 4 -mfma_anchors 8 -mfma_jumps 16,4,1,1",
+                "-mfma_rounds 5 -mfma_anchors 6 -mfma_jumps 20,8,3,1,1", "-mfma_rounds 4 -mfma_anchors 5 -mfma_jumps 24,6,1,1",
+                "-mfma_rounds 3 -mfma_anchors 7 -mfma_jumps 24,4,1", "-mfma_rounds 3 -mfma_anchors 6 -mfma_jumps 8,1,1"]
     for extra in settings:
         for kind in ("gck", "key"):
             p = params(extra)
@@ -211,11 +212,12 @@ def learned_key(args):
             t = PatchTable(p, b) if kind == "gck" else PatchTable.keyed(p, key, b)
             torch.cuda.synchronize()
             build_ms = (time.perf_counter() - t0) * 1e3
+            tune = t.autotune(a, ann, reps=3)
             ms, st = timed_lookups(torch, t, a, ann, args.warmup, args.steps)
             ssd = float(ann[..., 2].mean().item())
             points.append({"table": kind, "switches": extra or "(speed 1)", "ms": round(ms, 3),
                            "mpatch_s": round(nq / ms / 1e3, 2), "mean_ssd": round(ssd, 6),
-                           "build_ms": round(build_ms, 1), "query_key_ms": round(st["query_gck"], 3),
+                           "build_ms": round(build_ms, 1), "tile_x": tune["tile_x"], "query_key_ms": round(st["query_gck"], 3),
                            "candidates_ms": round(st["query_candidates"], 3)})
             log(json.dumps(points[-1]))
             t.close()
