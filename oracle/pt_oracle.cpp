@@ -1368,7 +1368,11 @@ static long g_ssd_count[64];
 //       within 8 columns and PT_MFMA_BAND rows of it;
 //     search: candidates (x0 + ox - 8 + j, y + oy + dy), j < 32, |dy| <= PT_MFMA_BAND, inside
 //       b, in order o = (anchor * (2 PT_MFMA_BAND + 1) + dy + PT_MFMA_BAND) * 32 + j; each member
-//       keeps the smallest (int8 SSD, o);
+//       keeps the smallest (int8 SSD, o); when every score is at most the int8 SSD of its match
+//       (no previous field, or no per-patch weights and coherence_temporal >= 0), a round after
+//       the first does not search an anchor that the group searched in its last round with
+//       anchors (the anchor keeps its index): those candidates cannot be taken, so only the
+//       item counts change (pinned against the full search in the tests);
 //     acceptance: a member whose best differs from its match takes it when its int8 SSD is
 //       strictly below the match's score;
 //   output: x, y and the exact FP32 SSD of the match; a patch still holding its initial
@@ -1379,6 +1383,8 @@ static inline int quant8(float v) { return (int)rintf(fminf(fmaxf(v, 0.f), 1.f) 
 
 // (group, anchor) items of each round of the last lookup_mfma
 static std::vector<long> g_mfma_items;
+// 0: search every anchor (the full search the skip of searched anchors is checked against)
+static int g_mfma_seen_skip = 1;
 
 // jumps: pt_params.mfma_jumps (0-terminated; rounds past it use 1); empty: 16, 8, 4, 2, 1, then 1
 static int mfma_jump_at(const int *jumps, int rounds, int r) {
@@ -1480,6 +1486,10 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
     fvec nd(N);
     g_mfma_items.assign(rounds, 0);
     std::vector<char> chg(N, 1);   // the previous round changed the match
+    // each group's anchors of the last round in which it had any
+    const int ngx = (aw + 15) / 16;
+    std::vector<std::vector<std::pair<int, int>>> lastA((size_t)ngx * ah);
+    const bool seen_ok = g_mfma_seen_skip && (!have_prev || (!sv && p.coherence_temporal >= 0.0));
     for (int r = 0; r < rounds; r++) {
         const int J = mfma_jump_at(p.mfma_jumps, rounds, r);
         const bool stable = r > 0 && mfma_jump_at(p.mfma_jumps, rounds, r - 1) == J;
@@ -1536,9 +1546,14 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
                     const int ox = sox[dfirst[pick]], oy = soy[dfirst[pick]];
                     aox.push_back(ox);
                     aoy.push_back(oy);
-                    g_mfma_items[r]++;
                     for (size_t k = 0; k < dfirst.size(); k++)
                         if (std::abs(sox[dfirst[k]] - ox) <= 8 && std::abs(soy[dfirst[k]] - oy) <= R) covered[k] = 1;
+                    if (seen_ok && r > 0) {
+                        bool seen = false;
+                        for (const auto &q : lastA[(size_t)y * ngx + x0 / 16]) seen = seen || (q.first == ox && q.second == oy);
+                        if (seen) continue;
+                    }
+                    g_mfma_items[r]++;
                     for (int dy = -R; dy <= R; dy++)
                         for (int j = 0; j < 32; j++) {
                             const int xs = x0 + ox - 8 + j, ys = y + oy + dy;
@@ -1549,6 +1564,11 @@ static void lookup_mfma(const Table &T, const float *a_in, int h_in, int w_in, i
                                 if (key < best[i]) best[i] = key;
                             }
                         }
+                }
+                if (!aox.empty()) {
+                    auto &la = lastA[(size_t)y * ngx + x0 / 16];
+                    la.clear();
+                    for (size_t q = 0; q < aox.size(); q++) la.push_back({aox[q], aoy[q]});
                 }
                 for (int i = 0; i < 16 && x0 + i < aw; i++) {
                     const long k = (long)y * aw + x0 + i;
@@ -2213,6 +2233,8 @@ int orc_lookup_raster(void *t, const float *a, int h, int w, int c, double *ann,
     lookup_raster(*(Table *)t, a, h, w, c, ann, prev, sv);
     return 0;
 }
+
+void orc_set_mfma_seen_skip(int on) { g_mfma_seen_skip = on; }
 
 int orc_mfma_items(long *out, int n) {
     const int m = (int)g_mfma_items.size();

@@ -219,10 +219,18 @@ constexpr int MW = 64;   // threads
 // `stable`: the previous round had the same jump J and its field's flags mark the patches whose
 // match it changed. A group none of whose valid sources changed would then compute the anchors,
 // winners and rescorings of the previous round again and keep every match: it takes no items.
-__device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur, NnfField nxt, int J, int stable) {
-    // the hash tables, source offsets and anchors of phase 1, then the two band buffers of phase 2
+// `seen`: every match's score is at most the int8 SSD of the match, so after a round each
+// member's score is at most that of every candidate the round scored for it, and scores only
+// fall. An anchor the group searched in its last searched round (M.anch) then holds no candidate
+// strictly below any member's score: the round keeps its place in the anchor order (the order
+// bits of the other anchors' keys) but scores none of its candidates. A winner strictly below a
+// member's score lies outside every such band, so the decisions are those of the full search.
+__device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur, NnfField nxt, int J, int stable,
+                                                int seen) {
+    // the hash tables, source offsets and anchor indices of phase 1, then the two band buffers of phase 2
     __shared__ uint32_t wsh[1][WSH];
-    __shared__ int items[1][MAXITEMS][3];
+    __shared__ int items[1][MAXITEMS];                 // group | anchor index << 2
+    __shared__ int anch[4 * PT_MFMA_MAX_ANCHORS];      // the 4 groups' anchors (packed offsets)
     __shared__ uint32_t fin[16 * 2];      // (key, jj) of the 16 members at the end of a group
     const int2 tc = tile_coords();
     const int x00 = tc.x * TILE_X, y00 = tc.y;
@@ -240,6 +248,8 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
         // the low 16 bits): widths and heights are at most 32768.
         int *hk = (int *)wsh[wave], *hc = hk + 4 * HS, *hf = hk + 8 * HS, *pos = hk + 12 * HS, *itmp = pos + 256;
         const int gq = lane >> 4, c = lane & 15, x0 = xw0 + 16 * gq;
+        // first anchor word of the group
+        auto gbase = [&]() { return ((long)y * ((aw + 15) >> 4) + (x0 >> 4)) * M.nanchors; };
         int sxy[4], sx[4], sy[4], ox[4], oy[4], h[4];
         bool sv[4];
 #pragma unroll
@@ -299,8 +309,11 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
                 const int slot = 16 * r + c;
                 kb[r] = sv[r] && hf[h[r]] == slot ? (uint32_t)((hc[h[r]] << 6) | (63 - slot)) + 1u : 0u;
             }
+            // the group's anchors of its last searched round, one per lane of the row
+            const int pv = seen && x0 < aw && c < M.nanchors ? M.anch[gbase() + c] : PT_MFMA_NO_ANCHOR;
             bool cov[4] = {false, false, false, false};
             int na = 0;   // anchors of group gq so far (the same in all lanes of the row)
+            int ns = 0;   // ... of them to search
             for (int a = 0; a < M.nanchors; a++) {
                 // per group, the uncovered offset of largest count, ties to the first slot (keys are unique)
                 uint32_t k = 0u;
@@ -314,23 +327,26 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
 #pragma unroll
                     for (int r = 0; r < 4; r++)
                         cov[r] = cov[r] || ((uint32_t)(ox[r] - ax + 8) <= 16u && (uint32_t)(oy[r] - ay + R) <= (uint32_t)(2 * R));
+                    // searched in the group's last searched round (rows whose k is 0 are inactive here)
+                    const bool was = ((__ballot(pv == pa) >> (16 * gq)) & 0xffffull) != 0;
                     if (c == 0) {
-                        itmp[(gq * PT_MFMA_MAX_ANCHORS + a) * 2] = ax;
-                        itmp[(gq * PT_MFMA_MAX_ANCHORS + a) * 2 + 1] = ay;
+                        anch[gq * PT_MFMA_MAX_ANCHORS + a] = pa;
+                        if (!was) itmp[gq * PT_MFMA_MAX_ANCHORS + ns] = a;
                     }
+                    ns += was ? 0 : 1;
                     na++;
                 }
             }
             wave_lds_sync();
-            // the item list: group 0's anchors in order, then group 1's, ...
-            const int n0 = __builtin_amdgcn_readlane(na, 0), n1 = __builtin_amdgcn_readlane(na, 16),
-                      n2 = __builtin_amdgcn_readlane(na, 32), n3 = __builtin_amdgcn_readlane(na, 48);
+            // this round's anchors for the next rounds (a group that found none, i.e. a stable one,
+            // keeps the anchors it last searched)
+            if (M.anch && x0 < aw && na > 0 && c < M.nanchors)
+                M.anch[gbase() + c] = c < na ? anch[gq * PT_MFMA_MAX_ANCHORS + c] : PT_MFMA_NO_ANCHOR;
+            // the item list: group 0's anchors to search in order, then group 1's, ...
+            const int n0 = __builtin_amdgcn_readlane(ns, 0), n1 = __builtin_amdgcn_readlane(ns, 16),
+                      n2 = __builtin_amdgcn_readlane(ns, 32), n3 = __builtin_amdgcn_readlane(ns, 48);
             const int pre = gq == 0 ? 0 : (gq == 1 ? n0 : (gq == 2 ? n0 + n1 : n0 + n1 + n2));
-            if (c < na) {
-                items[wave][pre + c][0] = gq;
-                items[wave][pre + c][1] = itmp[(gq * PT_MFMA_MAX_ANCHORS + c) * 2];
-                items[wave][pre + c][2] = itmp[(gq * PT_MFMA_MAX_ANCHORS + c) * 2 + 1];
-            }
+            if (c < ns) items[wave][pre + c] = gq | (itmp[gq * PT_MFMA_MAX_ANCHORS + c] << 2);
             nitems = n0 + n1 + n2 + n3;
         }
     }
@@ -350,7 +366,13 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
     static_assert((uint64_t)PT_MFMA_NK_PAD + 127 + 256ull * 192 * 128 * 128 < (1ull << 32), "pad keys do not wrap");
     static_assert(((uint64_t)4 * PT_MFMA_OFF << 7) + 127 < INV_LO, "keys inside b stay below INV_LO");
     // item fields are wave-uniform: read them into scalar registers
-    auto item = [&](int i, int f) { return __builtin_amdgcn_readfirstlane(items[wave][i][f]); };
+    auto item = [&](int i) { return __builtin_amdgcn_readfirstlane(items[wave][i]); };
+    auto item_ox = [&](int e) { return (int)(int16_t)(__builtin_amdgcn_readfirstlane(anch[(e & 3) * PT_MFMA_MAX_ANCHORS + (e >> 2)]) & 0xffff); };
+    auto item_oy = [&](int e) { return __builtin_amdgcn_readfirstlane(anch[(e & 3) * PT_MFMA_MAX_ANCHORS + (e >> 2)]) >> 16; };
+    auto fetch = [&](int i, const int (&off)[PER], uint32_t (&reg)[PER]) {
+        const int e = item(i);
+        band_fetch(M, xw0 + 16 * (e & 3), y, item_ox(e), item_oy(e), off, reg);
+    };
     // -256 in a scalar register the compiler does not see as a constant: a constant factor would
     // be strength-reduced to a shift and a subtract (two instructions instead of one v_mad_i32_i24)
     int m256;
@@ -363,19 +385,19 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
     uint32_t reg[PER];
     uint32_t *bufs = wsh[wave];
     if (nitems > 0) {
-        band_fetch(M, xw0 + 16 * item(0, 0), y, item(0, 1), item(0, 2), off, reg);
+        fetch(0, off, reg);
         wave_lds_sync();   // phase 1's reads of the hash table are done
 #pragma unroll
         for (int k = 0; k < PER; k++)
             if (lane + 64 * k < BANDW) bufs[lane + 64 * k] = reg[k];
-        if (nitems > 1) band_fetch(M, xw0 + 16 * item(1, 0), y, item(1, 1), item(1, 2), off, reg);
+        if (nitems > 1) fetch(1, off, reg);
     }
     v4i A[4];
     uint32_t best[4] = {~0u, ~0u, ~0u, ~0u};
-    int cur_g = -1, ga = 0, gfirst = 0;   // group of the current items, anchor index in it, its first item
+    int cur_g = -1;   // group of the current items
     const int bew = M.bw - 7, beh = M.bh - 7;
     for (int it = 0; it < nitems; it++) {
-        const int gi = item(it, 0);
+        const int ei = item(it), gi = ei & 3, ga = ei >> 2;   // group, anchor index
         const int x0 = xw0 + 16 * gi;
         wave_lds_sync();   // item it - 1's reads are done, item it's band is written
         if (it + 1 < nitems) {
@@ -383,13 +405,11 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
 #pragma unroll
             for (int k = 0; k < PER; k++)
                 if (lane + 64 * k < BANDW) nb1[lane + 64 * k] = reg[k];
-            if (it + 2 < nitems) band_fetch(M, xw0 + 16 * item(it + 2, 0), y, item(it + 2, 1), item(it + 2, 2), off, reg);
+            if (it + 2 < nitems) fetch(it + 2, off, reg);
         }
         const uint32_t *bnd = bufs + (it & 1) * BANDW;
         if (gi != cur_g) {
             cur_g = gi;
-            ga = 0;
-            gfirst = it;
             // A fragments of member jj: k-step s covers patch row s + 4 (g >> 1), pixels 4(g & 1)..+3,
             // read from the int8 query (a valid member's patch lies inside it)
             const bool mv = x0 + jj < aw;
@@ -428,8 +448,7 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
                 for (int t = 0; t < 4; t++) best[t] = min(best[t], base + (uint32_t)__mul24(C[t], m256));
             }
         }
-        ga++;
-        if (it + 1 == nitems || item(it + 1, 0) != gi) {
+        if (it + 1 == nitems || (item(it + 1) & 3) != gi) {
             // best (key, jj) of member 4g + t over the group's 16 candidate lanes: the row minimum
             // of the keys, then the smallest jj holding it; row g's results go through LDS to
             // member i = 4g + t, lane 16 gi + i
@@ -448,8 +467,9 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
             if (g == gi && key < INV_LO) {
                 const int ord = (int)(key & 127u), j = 16 * (ord & 1) + (int)kj, dyi = (ord >> 1) % (2 * R + 1),
                           an = (ord >> 1) / (2 * R + 1);
-                if (gfirst + an < nitems) {
-                    const int oxa = items[wave][gfirst + an][1], oya = items[wave][gfirst + an][2];   // per lane
+                if (an < PT_MFMA_MAX_ANCHORS) {
+                    const int pa = anch[gi * PT_MFMA_MAX_ANCHORS + an];   // per lane
+                    const int oxa = (int)(int16_t)(pa & 0xffff), oya = pa >> 16;
                     const int xs = x0 + oxa - 8 + j, ys = y + oya - R + dyi;
                     // a key below INV_LO is a candidate inside b; the check keeps the stored
                     // match inside b whatever the key holds
@@ -493,15 +513,16 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
     nxt.fl[vi] = fl | (v != v_in ? F_CHG : 0);
 }
 
-__global__ __launch_bounds__(MW, 5) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J, int stable) {
-    mfma_round_body(M, cur, nxt, J, stable);
+__global__ __launch_bounds__(MW, 5) void k_mfma_round(MfmaArgs M, NnfField cur, NnfField nxt, int J, int stable,
+                                                      int seen) {
+    mfma_round_body(M, cur, nxt, J, stable, seen);
 }
 
 // One round of every pair of a batch (blockIdx.z = pair), from field src into field src ^ 1.
 __global__ __launch_bounds__(MW, 5) void k_mfma_round_batch(const PairLookup *__restrict__ P, int src, int J,
-                                                            int stable) {
+                                                            int stable, int seen) {
     const PairLookup &p = P[blockIdx.z];
-    mfma_round_body(p.M, p.f[src], p.f[src ^ 1], J, stable);
+    mfma_round_body(p.M, p.f[src], p.f[src ^ 1], J, stable, seen);
 }
 
 }  // namespace
@@ -523,17 +544,18 @@ int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, uint32_t 
 
 long pt_mfma_round_workgroups(int ann_w, int ann_h) { return (long)((ann_w + TILE_X - 1) / TILE_X) * ann_h; }
 
-int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, const int *jumps, NnfField f0, NnfField f1, hipStream_t s,
-                          int *which) {
+int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, const int *jumps, bool seen_ok, NnfField f0, NnfField f1,
+                          hipStream_t s, int *which) {
     if (M.nanchors < 1 || M.nanchors > PT_MFMA_MAX_ANCHORS)
         return pt_failf(PT_E_INVALID, "mfma_anchors must be in 1..%d", PT_MFMA_MAX_ANCHORS);
+    if (seen_ok && !M.anch) return pt_fail(PT_E_INVALID, "MFMA rounds: no anchor buffer");
     dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, M.ann_h);
     const long nwg = pt_mfma_round_workgroups(M.ann_w, M.ann_h);
     for (int r = 0; r < rounds; r++) {
         MfmaArgs Mr = M;
         if (M.items_out) Mr.items_out = M.items_out + r * nwg;
         const int J = pt_mfma_jump_at(jumps, rounds, r), stable = r > 0 && pt_mfma_jump_at(jumps, rounds, r - 1) == J;
-        hipLaunchKernelGGL(k_mfma_round, grid, dim3(MW), 0, s, Mr, f0, f1, J, stable);
+        hipLaunchKernelGGL(k_mfma_round, grid, dim3(MW), 0, s, Mr, f0, f1, J, stable, (int)(seen_ok && r > 0));
         std::swap(f0, f1);
     }
     PT_LAUNCH_CHECK();
@@ -548,11 +570,13 @@ int pt_mfma_rounds_batch_launch(const PairLookup *P_dev, const PairLookup &P0, i
     if (M.nanchors < 1 || M.nanchors > PT_MFMA_MAX_ANCHORS)
         return pt_failf(PT_E_INVALID, "mfma_anchors must be in 1..%d", PT_MFMA_MAX_ANCHORS);
     if (M.items_out) return pt_fail(PT_E_INVALID, "batched MFMA rounds keep no item counts");
+    if (!M.anch) return pt_fail(PT_E_INVALID, "batched MFMA rounds: no anchor buffer");
     dim3 grid((M.ann_w + TILE_X - 1) / TILE_X, M.ann_h, npairs);
     int src = 0;
     for (int r = 0; r < rounds; r++) {
         const int J = pt_mfma_jump_at(jumps, rounds, r), stable = r > 0 && pt_mfma_jump_at(jumps, rounds, r - 1) == J;
-        hipLaunchKernelGGL(k_mfma_round_batch, grid, dim3(MW), 0, s, P_dev, src, J, stable);
+        // a batch has no previous fields: every score is the int8 SSD of its match
+        hipLaunchKernelGGL(k_mfma_round_batch, grid, dim3(MW), 0, s, P_dev, src, J, stable, (int)(r > 0));
         src ^= 1;
     }
     PT_LAUNCH_CHECK();

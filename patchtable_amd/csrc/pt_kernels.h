@@ -263,7 +263,15 @@ struct MfmaArgs {
     // optional instrumentation: per round, the items (group, anchor) of every workgroup, one entry
     // per workgroup (round r at items_out + r * workgroups); null: not written
     int32_t *items_out = nullptr;
+    // every 16-patch group's anchors in the last round that searched it: pt_mfma_anchor_words
+    // packed offsets (x in the low 16 bits), PT_MFMA_NO_ANCHOR past the group's count. Each round
+    // writes it; a round with `seen` set skips an anchor found there (see mfma_round_body).
+    int32_t *anch = nullptr;
 };
+constexpr int32_t PT_MFMA_NO_ANCHOR = (int32_t)0x80008000u;   // no offset packs to this (|x| < 32768)
+static inline size_t pt_mfma_anchor_words(int ann_w, int ann_h, int nanchors) {
+    return (size_t)((ann_w + 15) / 16) * (size_t)ann_h * (size_t)nanchors;
+}
 // Workgroups of one MFMA round launch.
 long pt_mfma_round_workgroups(int ann_w, int ann_h);
 // Jump of MFMA round r of `rounds`: jumps[r] when the list (pt_params.mfma_jumps, 0-terminated)
@@ -284,9 +292,12 @@ int pt_quant_rgb0_launch(const float *img, long npx, uint32_t *out, hipStream_t 
 // The padded int8 copy of b and its patch norm keys (margins written by the caller: zeros in the
 // copy, PT_MFMA_NK_PAD in the norm keys).
 int pt_mfma_prepare_launch(const float *b, int h, int w, uint32_t *bq, uint32_t *bnorm, hipStream_t s);
-// Runs `rounds` rounds from f0; *which = 1 if the result is in f1.
-int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, const int *jumps, NnfField f0, NnfField f1, hipStream_t s,
-                          int *which);
+// Runs `rounds` rounds from f0; *which = 1 if the result is in f1. seen_ok: every match's score
+// is at most the int8 SSD of the match (no previous field, or one weighted by a temporal scale of
+// at most 1 and no per-patch scales), so rounds after the first may skip the anchors a group
+// searched in its last round (M.anch must then be set).
+int pt_mfma_rounds_launch(const MfmaArgs &M, int rounds, const int *jumps, bool seen_ok, NnfField f0, NnfField f1,
+                          hipStream_t s, int *which);
 // One pair of a batch (one query against many tables), read by the batched kernels from a device
 // array indexed by the pair: the lookup's arguments (L; G: the same for the query_step > 1 gather,
 // whose lanes are single patches), its two fields, its gathered field and its NNF output.
@@ -338,6 +349,7 @@ struct LookupBufs {
     int *bad = nullptr;
     uint32_t *aq = nullptr;   // int8 RGB0 query (MFMA lookup)
     int32_t *qnorm = nullptr; // ... and |q|^2 of its patches
+    int32_t *anch = nullptr;  // ... and each group's anchors of its last searched round
 };
 struct pt_table;
 // Enqueues one lookup on the table's stream (table.cpp). ws: the buffers to work in, or NULL for

@@ -95,6 +95,7 @@ struct pt_table {
     DevBuf<float> a, adesc, adescI, yuv, temp;
     DevBuf<uint32_t> aq;
     DevBuf<int32_t> qnorm;   // MFMA lookup: |q|^2 of every query patch (int8 RGB0)
+    DevBuf<int32_t> anch;    // ... and each 16-patch group's anchors of its last searched round
     DevBuf<int32_t> nxy, nxy2, gxy, ixy;
     DevBuf<float> nd, nd2, gd, id;
     DevBuf<uint8_t> nfl, nfl2;
@@ -912,9 +913,11 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
         if (mfma) {
             TRY(t->aq.ensure((size_t)ah * aw));
             TRY(t->qnorm.ensure(n));
+            TRY(t->anch.ensure(pt_mfma_anchor_words(ann_w, ann_h, p.mfma_anchors)));
         }
         B.aq = t->aq.p;
         B.qnorm = t->qnorm.p;
+        B.anch = t->anch.p;
         B.adesc = t->adesc.p;
         B.adescI = t->adescI.p;
         B.yuv = t->yuv.p;
@@ -993,13 +996,17 @@ int pt_lookup_enqueue(pt_table *t, const float *a_dev, int h, int w, int c, doub
         MfmaArgs M;
         fill_mfma_args(t, p, a_dev, B.aq, ah, aw, M);
         M.qnorm = B.qnorm;
+        M.anch = B.anch;
+        // a previous field's scores are its int8 SSDs times the temporal weight: at most the SSD
+        // when that weight is 1 / (1 + coherence_temporal) with coherence_temporal >= 0
+        const bool seen_ok = !prev_dev || (!sv_dev && p.coherence_temporal >= 0.0);
         if (t->mfma_stats && !ws) {
             const size_t n = (size_t)p.mfma_rounds * pt_mfma_round_workgroups(ann_w, ann_h);
             TRY(t->mfma_items.ensure(n));
             M.items_out = t->mfma_items.p;
         }
         int which = 0;
-        TRY(pt_mfma_rounds_launch(M, p.mfma_rounds, p.mfma_jumps, f0, f1, s, &which));
+        TRY(pt_mfma_rounds_launch(M, p.mfma_rounds, p.mfma_jumps, seen_ok, f0, f1, s, &which));
         launches = p.mfma_rounds;
         if (timing) HIPTRY(hipEventRecord(t->ev[10], s));
         // the exact distances and the double output in one pass (k_init_prev's initial distances,
@@ -1107,6 +1114,7 @@ int pt_lookup_workspace_layout(const pt_table *t, int h, int w, int c, bool prev
     if (p.mfma_rounds > 0) {
         B.aq = (uint32_t *)take((size_t)ah * aw * 4);
         B.qnorm = (int32_t *)take(n * 4);
+        B.anch = (int32_t *)take(pt_mfma_anchor_words(ann_w, ann_h, p.mfma_anchors) * 4);
     }
     if (bytes) *bytes = off;
     if (bufs) *bufs = B;
@@ -1856,6 +1864,8 @@ int batch_prepare(pt_batch *bt, const float *a_dev, int h, int w, double *const 
         if (!mfma) {
             TRY(t->hist.ensure((size_t)nv * PT_LOOKUP_HIST));
             TRY(t->hn.ensure(nv));
+        } else {
+            TRY(t->anch.ensure(pt_mfma_anchor_words(ann_w, ann_h, p.mfma_anchors)));
         }
         if (qs > 1) {
             TRY(t->gxy.ensure(np));
@@ -1876,6 +1886,7 @@ int batch_prepare(pt_batch *bt, const float *a_dev, int h, int w, double *const 
             set_int8_args(t, bt->aq.p, bt->qnorm.p, nullptr, 1.0, P.L);
             fill_mfma_args(t, p, a_dev, bt->aq.p, ah, aw, P.M);
             P.M.qnorm = bt->qnorm.p;
+            P.M.anch = t->anch.p;
         }
         P.f[0].xy = t->nxy.p;
         P.f[0].d = t->nd.p;

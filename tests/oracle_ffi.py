@@ -78,6 +78,7 @@ def lib():
             "orc_brute_f64": (_int, [_f32p, _int, _i32p, _int, _f32p, _int, _int, _i32p, _f64p]),
             "orc_key_round_weights": (_int, [_int]),
             "orc_mfma_items": (_int, [_P(ctypes.c_long), _int]),
+            "orc_set_mfma_seen_skip": (None, [_int]),
             "orc_key_loss": (ctypes.c_double, [_f64p, _f64p, _f64p, _f64p, _f32p, _f32p, _int, _i32p, _int,
                                                ctypes.c_double]),
             "orc_key_forward": (_int, [_f32p, _f32p, _f32p, _f32p, _f32p, _int, _int, _f32p]),
@@ -305,6 +306,12 @@ def dt(nslices, table, algo, dt_iters=-1):
     rc = lib().orc_dt(len(nslices), ns, i32(t), algo, dt_iters)
     assert rc == 0
     return t
+
+
+def set_mfma_seen_skip(on):
+    """1 (default): the MFMA model skips anchors a group searched in its last round with anchors;
+    0: it searches every anchor (the full search the skip is checked against)."""
+    lib().orc_set_mfma_seen_skip(int(on))
 
 
 def mfma_items():

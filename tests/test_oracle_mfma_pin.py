@@ -102,3 +102,44 @@ def test_mfma_restatement_two_offsets(anchors):
         return [(0, 0)] if x0 == 0 else [(12, 0)]
     got = run(a, b, prev, anchors)
     assert np.array_equal(got, expected(a, b, prev, anchors_of_group))
+
+
+@pytest.mark.parametrize("switches,prev", [
+    ("-mfma_rounds 4 -mfma_anchors 7 -mfma_jumps 24,6,1,1", None),
+    ("-mfma_rounds 6 -mfma_anchors 3", None),
+    ("-mfma_rounds 5 -mfma_anchors 6 -mfma_jumps 8,2,1,1", "temporal"),
+    ("-mfma_rounds 4 -mfma_anchors 5 -coherence_temporal -0.5", "temporal"),   # weights above 1: no skip
+    ("-mfma_rounds 4 -mfma_anchors 5", "sv"),                                  # per-patch weights: no skip
+])
+def test_mfma_skip_of_searched_anchors_changes_nothing(switches, prev):
+    """A round after the first does not search the anchors a group searched in its last round
+    with anchors, when every score is at most the int8 SSD of its match: those bands hold no
+    candidate strictly below any member's score. The fields with and without the skip are equal
+    bit for bit; with the skip the rounds after the first score fewer items (and exactly as many
+    where it is off: weights above 1 or per-patch weights)."""
+    from patchtable_amd import synth_pair
+    a, _ = synth_pair(61, 70, 110)
+    _, b = synth_pair(62, 80, 120)
+    ah, aw = a.shape[0] - 7, a.shape[1] - 7
+    p = speed1("-limit 0.1 " + switches + ("" if prev is None else " -coherence_temporal 1.5"
+                                            if "coherence_temporal" not in switches else ""))
+    o = O.OracleTable(p.raw, b)
+    prev_f = sv = None
+    if prev is not None:
+        prev_f = o.lookup_parallel(a[:, ::-1].copy())   # a poor previous field
+        if prev == "sv":
+            sv = np.random.default_rng(5).uniform(0.5, 2.0, (ah, aw)).astype(np.float32)
+    out = {}
+    for on in (0, 1):
+        O.set_mfma_seen_skip(on)
+        try:
+            out[on] = (o.lookup_parallel(a, prev=prev_f, sv=sv), O.mfma_items())
+        finally:
+            O.set_mfma_seen_skip(1)
+    assert np.array_equal(out[0][0], out[1][0])
+    full, skip = out[0][1], out[1][1]
+    assert skip[0] == full[0]
+    if prev is None or ("coherence_temporal" not in switches and prev != "sv"):
+        assert all(s <= f for s, f in zip(skip, full)) and sum(skip) < sum(full)
+    else:
+        assert skip == full
