@@ -396,6 +396,9 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
     uint32_t best[4] = {~0u, ~0u, ~0u, ~0u};
     int cur_g = -1;   // group of the current items
     const int bew = M.bw - 7, beh = M.bh - 7;
+    // this lane's first band word: row 4 (g >> 1), column jj + 4 (g & 1); the fragments and norm
+    // keys it reads lie at constant distances from it
+    const int lofs = 4 * (g >> 1) * BC + jj + 4 * (g & 1);
     for (int it = 0; it < nitems; it++) {
         const int ei = item(it), gi = ei & 3, ga = ei >> 2;   // group, anchor index
         const int x0 = xw0 + 16 * gi;
@@ -408,6 +411,7 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
             if (it + 2 < nitems) fetch(it + 2, off, reg);
         }
         const uint32_t *bnd = bufs + (it & 1) * BANDW;
+        const uint32_t *bl = bnd + lofs;
         if (gi != cur_g) {
             cur_g = gi;
             // A fragments of member jj: k-step s covers patch row s + 4 (g >> 1), pixels 4(g & 1)..+3,
@@ -421,14 +425,13 @@ __device__ __forceinline__ void mfma_round_body(const MfmaArgs &M, NnfField cur,
         }
 #pragma unroll
         for (int b = 0; b < 2; b++) {
-            const int col = 16 * b + jj + 4 * (g & 1);
-            // B fragment k: band row k + 4 (g >> 1), words col..col+3; candidate row dyi at k-step s
-            // uses fragment dyi + s, so the three rows share six fragments
+            // B fragment k: band row k + 4 (g >> 1), words 16 b + jj + 4 (g & 1) .. +3; candidate row
+            // dyi at k-step s uses fragment dyi + s, so the three rows share six fragments
             // (fragment k is loaded just before row k - 3 first needs it and dropped after row k,
             // so at most five are held)
             v4i F[6];
             auto frag = [&](int k) {
-                const uint32_t *p = bnd + (k + 4 * (g >> 1)) * BC + col;
+                const uint32_t *p = bl + k * BC + 16 * b;
                 F[k] = v4i{(int)p[0], (int)p[1], (int)p[2], (int)p[3]};
             };
             static_assert(R == 1, "six fragments cover candidate rows -1, 0, +1");
