@@ -32,8 +32,11 @@
 //
 // Work layout: a workgroup of one wave owns 64 query patches of a row, its 4 groups in turn. A
 // wave stages each anchor's band of b (39 x 10 pixels, int8 RGB0) in its own LDS region; each
-// lane loads the B fragments of one candidate column for the 9 row pairs the band's 3 rows of
-// candidates use, and the 3 x 4 MFMAs of a 16-candidate block read them from registers.
+// lane loads the six B fragments of one candidate column that the band's 3 rows of candidates
+// share, and the 3 x 4 MFMAs of a 16-candidate block read them from registers. After the first
+// round a group does not search again an anchor it searched in its last round with anchors,
+// when no score exceeds the int8 SSD of its match (see mfma_round_body): the decisions are
+// those of the full search.
 #include "pt_device.h"
 #include "pt_kernels.h"
 #include "pt_tile.h"
