@@ -433,7 +433,9 @@ MFMA_GRID = ["-mfma_rounds 6 -mfma_anchors 5", "-mfma_rounds 5", "-mfma_rounds 5
              "-mfma_rounds 6", "-mfma_rounds 4 -mfma_anchors 8", "-mfma_rounds 6 -mfma_anchors 4",
              "-mfma_rounds 4 -mfma_anchors 7 -mfma_jumps 24,6,1,1",
              "-mfma_rounds 4 -mfma_anchors 8 -mfma_jumps 16,4,1,1",
-             "-mfma_rounds 5 -mfma_anchors 6 -mfma_jumps 20,8,3,1,1"]
+             "-mfma_rounds 5 -mfma_anchors 6 -mfma_jumps 20,8,3,1,1",
+             "-mfma_rounds 4 -mfma_anchors 8 -mfma_jumps 24,6,1,1",
+             "-mfma_rounds 5 -mfma_anchors 8 -mfma_jumps 24,6,2,1,1"]
 
 
 def params(extra=""):
